@@ -1,0 +1,301 @@
+"""Python mirror of the C ABI in include/agrav.h (ctypes over libagrav.so).
+
+The methods keep the names of the AREPO functions they replace
+(construct_forcetree, gravity_tree, long_range_force,
+gravity_force_finalize, compute_grav_accelerations, gravity_forcetest, ...).
+There is no CPU fallback: if libagrav.so is missing or cannot be loaded this
+module raises, and every call goes to the HIP kernels.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libagrav.so")
+
+OK, ERR_BAD_PARAM, ERR_CAPACITY, ERR_OUT_OF_BOX, ERR_HIP, ERR_OOM, ERR_STATE, ERR_FFT, ERR_NO_DEVICE = \
+    0, -1, -2, -3, -4, -5, -6, -7, -8
+FP64, FP32 = 0, 1
+WALK_PLAIN, WALK_GROUP = 0, 1
+MAX_SOFT = 8
+
+
+class AgravError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("agrav error %d: %s" % (code, msg))
+        self.code = code
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("periodic", C.c_int),
+        ("box_size", C.c_double),
+        ("pmgrid", C.c_int),
+        ("asmth", C.c_double),
+        ("rcut", C.c_double),
+        ("G", C.c_double),
+        ("err_tol_theta", C.c_double),
+        ("err_tol_force_acc", C.c_double),
+        ("type_of_opening_criterion", C.c_int),
+        ("n_soft_types", C.c_int),
+        ("softening", C.c_double * MAX_SOFT),
+        ("precision", C.c_int),
+        ("walk", C.c_int),
+        ("group_size", C.c_int),
+        ("tree_alloc_factor", C.c_double),
+    ]
+
+
+class Timings(C.Structure):
+    _fields_ = [
+        ("treebuild", C.c_double),
+        ("treewalk", C.c_double),
+        ("pm_grav", C.c_double),
+        ("finalize", C.c_double),
+        ("walk_kernel", C.c_double),
+        ("interactions", C.c_longlong),
+        ("numnodes", C.c_int),
+        ("walk_retries", C.c_int),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    """Load libagrav.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libagrav.so not built: run `python -m arepo_amd.build` (%s missing)" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        _declare(L)
+        _lib = L
+    return _lib
+
+
+EXPORTED = [
+    "agrav_status_string", "agrav_last_error", "agrav_default_params", "agrav_create", "agrav_destroy",
+    "agrav_set_softenings", "agrav_set_particles", "agrav_set_particles_device", "agrav_set_old_acc",
+    "agrav_construct_forcetree", "agrav_gravity_tree", "agrav_long_range_force", "agrav_gravity_force_finalize",
+    "agrav_compute_grav_accelerations", "agrav_get_accelerations", "agrav_get_accelerations_device",
+    "agrav_get_components", "agrav_gravity_forcetest", "agrav_gravity_direct", "agrav_get_timings", "agrav_get_tree",
+    "agrav_get_order", "agrav_get_tables", "agrav_hilbert_key21",
+]
+
+
+def _declare(L):
+    vp = C.c_void_p
+    L.agrav_status_string.restype = C.c_char_p
+    L.agrav_status_string.argtypes = [C.c_int]
+    L.agrav_last_error.restype = C.c_char_p
+    L.agrav_last_error.argtypes = [vp]
+    L.agrav_default_params.argtypes = [C.POINTER(Params)]
+    L.agrav_create.argtypes = [C.POINTER(Params), C.POINTER(vp)]
+    L.agrav_destroy.argtypes = [vp]
+    L.agrav_set_softenings.argtypes = [vp, C.c_int, vp]
+    L.agrav_set_particles.argtypes = [vp, C.c_int64, vp, vp, vp]
+    L.agrav_set_particles_device.argtypes = [vp, C.c_int64, vp, vp, vp]
+    L.agrav_set_old_acc.argtypes = [vp, vp]
+    L.agrav_construct_forcetree.argtypes = [vp, C.POINTER(C.c_int)]
+    L.agrav_gravity_tree.argtypes = [vp, C.c_int]
+    L.agrav_long_range_force.argtypes = [vp]
+    L.agrav_gravity_force_finalize.argtypes = [vp]
+    L.agrav_compute_grav_accelerations.argtypes = [vp, C.c_int]
+    L.agrav_get_accelerations.argtypes = [vp, vp, vp, vp, vp]
+    L.agrav_get_accelerations_device.argtypes = [vp, C.POINTER(vp), C.POINTER(vp)]
+    L.agrav_get_components.argtypes = [vp, vp, vp, vp, vp]
+    L.agrav_gravity_forcetest.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp]
+    L.agrav_gravity_direct.argtypes = [vp, vp, vp]
+    L.agrav_get_timings.argtypes = [vp, C.POINTER(Timings)]
+    L.agrav_get_tree.argtypes = [vp, C.POINTER(C.c_int), vp, vp, vp, vp, vp, vp]
+    L.agrav_get_order.argtypes = [vp, vp, vp]
+    L.agrav_get_tables.argtypes = [vp, vp, vp, vp, vp]
+    L.agrav_hilbert_key21.restype = C.c_uint64
+    L.agrav_hilbert_key21.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
+    for name in EXPORTED:
+        fn = getattr(L, name)
+        if fn.restype is C.c_int or name in ("agrav_destroy", "agrav_default_params"):
+            pass
+    for name in EXPORTED:
+        if name not in ("agrav_status_string", "agrav_last_error", "agrav_hilbert_key21", "agrav_destroy",
+                        "agrav_default_params"):
+            getattr(L, name).restype = C.c_int
+    L.agrav_destroy.restype = None
+    L.agrav_default_params.restype = None
+
+
+def make_params(periodic=True, box_size=1.0, pmgrid=0, asmth=1.25, rcut=4.5, G=1.0, err_tol_theta=0.7,
+                err_tol_force_acc=0.0025, opening=1, soft=(0.01,), precision=FP64, walk=WALK_GROUP, group_size=64):
+    p = Params()
+    lib().agrav_default_params(C.byref(p))
+    p.periodic = int(periodic)
+    p.box_size = float(box_size)
+    p.pmgrid = int(pmgrid)
+    p.asmth = asmth
+    p.rcut = rcut
+    p.G = G
+    p.err_tol_theta = err_tol_theta
+    p.err_tol_force_acc = err_tol_force_acc
+    p.type_of_opening_criterion = opening
+    p.n_soft_types = len(soft)
+    for i, s in enumerate(soft):
+        p.softening[i] = s
+    p.precision = precision
+    p.walk = walk
+    p.group_size = group_size
+    return p
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+class Gravity:
+    """One solver context (agrav_ctx): device buffers persist between calls."""
+
+    def __init__(self, params: Params):
+        self.L = lib()
+        h = C.c_void_p()
+        self._check(self.L.agrav_create(C.byref(params), C.byref(h)), None)
+        self.h = h
+        self.params = params
+        self.n = 0
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.L.agrav_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc, h=None):
+        if rc != OK:
+            msg = self.L.agrav_last_error(h if h is not None else (self.h if hasattr(self, "h") else None))
+            raise AgravError(rc, msg.decode() if msg else "")
+
+    # -- inputs -----------------------------------------------------------
+    def set_particles(self, pos, mass, softtype=None):
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        mass = np.ascontiguousarray(mass, dtype=np.float64)
+        st = None if softtype is None else np.ascontiguousarray(softtype, dtype=np.uint8)
+        self._check(self.L.agrav_set_particles(self.h, len(pos), _ptr(pos), _ptr(mass), _ptr(st)))
+        self.n = len(pos)
+
+    def set_particles_device(self, n, d_pos, d_mass, d_softtype=None):
+        self._check(self.L.agrav_set_particles_device(self.h, n, d_pos, d_mass, d_softtype))
+        self.n = n
+
+    def set_old_acc(self, old_acc):
+        oa = np.ascontiguousarray(old_acc, dtype=np.float32)
+        self._check(self.L.agrav_set_old_acc(self.h, _ptr(oa)))
+
+    def set_softenings(self, eps):
+        e = np.ascontiguousarray(eps, dtype=np.float64)
+        self._check(self.L.agrav_set_softenings(self.h, len(e), _ptr(e)))
+
+    # -- the gravity step ---------------------------------------------------
+    def construct_forcetree(self):
+        m = C.c_int()
+        self._check(self.L.agrav_construct_forcetree(self.h, C.byref(m)))
+        return m.value
+
+    def gravity_tree(self, use_bh=False):
+        self._check(self.L.agrav_gravity_tree(self.h, int(use_bh)))
+
+    def long_range_force(self):
+        self._check(self.L.agrav_long_range_force(self.h))
+
+    def gravity_force_finalize(self):
+        self._check(self.L.agrav_gravity_force_finalize(self.h))
+
+    def compute_grav_accelerations(self, first_step=True):
+        self._check(self.L.agrav_compute_grav_accelerations(self.h, int(first_step)))
+
+    # -- outputs ------------------------------------------------------------
+    def get_accelerations(self):
+        n = self.n
+        acc = np.zeros((n, 3))
+        pot = np.zeros(n)
+        nint = np.zeros(n, np.int32)
+        oa = np.zeros(n, np.float32)
+        self._check(self.L.agrav_get_accelerations(self.h, _ptr(acc), _ptr(pot), _ptr(nint), _ptr(oa)))
+        return dict(acc=acc, pot=pot, nint=nint, old_acc=oa)
+
+    def get_components(self, tree=True, pm=False):
+        n = self.n
+        at = np.zeros((n, 3)) if tree else None
+        pt = np.zeros(n) if tree else None
+        ap = np.zeros((n, 3)) if pm else None
+        pp = np.zeros(n) if pm else None
+        self._check(self.L.agrav_get_components(self.h, _ptr(at), _ptr(pt), _ptr(ap), _ptr(pp)))
+        return dict(acc_tree=at, pot_tree=pt, acc_pm=ap, pot_pm=pp)
+
+    def gravity_forcetest(self, sel):
+        sel = np.ascontiguousarray(sel, dtype=np.int32)
+        ns = len(sel)
+        acc = np.zeros((ns, 3))
+        accs = np.zeros((ns, 3))
+        pot = np.zeros(ns)
+        pots = np.zeros(ns)
+        self._check(self.L.agrav_gravity_forcetest(self.h, ns, _ptr(sel), _ptr(acc), _ptr(accs), _ptr(pot), _ptr(pots)))
+        return dict(acc=acc, acc_short=accs, pot=pot, pot_short=pots)
+
+    def gravity_direct(self):
+        acc = np.zeros((self.n, 3))
+        pot = np.zeros(self.n)
+        self._check(self.L.agrav_gravity_direct(self.h, _ptr(acc), _ptr(pot)))
+        return acc, pot
+
+    def timings(self):
+        t = Timings()
+        self._check(self.L.agrav_get_timings(self.h, C.byref(t)))
+        return t.as_dict()
+
+    def get_tree(self):
+        m = C.c_int()
+        self._check(self.L.agrav_get_tree(self.h, C.byref(m), None, None, None, None, None, None))
+        M = m.value
+        center = np.zeros((M, 3))
+        ln = np.zeros(M)
+        com = np.zeros((M, 3))
+        mass = np.zeros(M)
+        level = np.zeros(M, np.int32)
+        npart = np.zeros(M, np.int32)
+        self._check(self.L.agrav_get_tree(self.h, C.byref(m), _ptr(center), _ptr(ln), _ptr(com), _ptr(mass),
+                                          _ptr(level), _ptr(npart)))
+        return dict(center=center, len=ln, com=com, mass=mass, level=level, npart=npart)
+
+    def get_order(self):
+        perm = np.zeros(self.n, np.int32)
+        keys = np.zeros(self.n, np.uint64)
+        self._check(self.L.agrav_get_order(self.h, _ptr(perm), _ptr(keys)))
+        return perm, keys
+
+    def get_tables(self, ewald_tree=False, ewald_forcetest=False):
+        tf = np.zeros(128, np.float32)
+        tp = np.zeros(128, np.float32)
+        et = np.zeros(4 * 65 ** 3) if ewald_tree else None
+        ef = np.zeros(4 * 129 ** 3) if ewald_forcetest else None
+        self._check(self.L.agrav_get_tables(self.h, _ptr(tf), _ptr(tp), _ptr(et), _ptr(ef)))
+        return dict(tab_force=tf, tab_pot=tp, ewald_tree=et, ewald_forcetest=ef)
+
+
+def hilbert_key21(x, y, z):
+    return int(lib().agrav_hilbert_key21(int(x), int(y), int(z)))

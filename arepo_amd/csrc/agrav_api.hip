@@ -1,0 +1,728 @@
+// agrav_api.hip -- the C ABI (include/agrav.h): context life cycle, particle
+// upload, the compute_grav_accelerations sequence (accel.c:58-347) and
+// result download.  Device work of a context runs on its own stream.
+#include "agrav_ctx.h"
+
+#include <math.h>
+#include <string.h>
+
+#include <vector>
+
+namespace agrav
+{
+
+static thread_local std::string g_last_msg;
+
+void set_error_msg(const std::string &m) { g_last_msg = m; }
+
+agrav_status hip_fail(hipError_t e, const char *what, const char *file, int line)
+{
+  char buf[512];
+  snprintf(buf, sizeof(buf), "HIP error %d (%s) in %s at %s:%d", (int)e, hipGetErrorString(e), what, file, line);
+  g_last_msg = buf;
+  return AGRAV_ERR_HIP;
+}
+
+agrav_status fail(agrav_ctx *c, agrav_status s, const std::string &m)
+{
+  if(c)
+    c->err = m;
+  g_last_msg = m;
+  return s;
+}
+
+static constexpr int B = 256;
+
+// gravity_force_finalize, accel.c:204-347 (no comoving terms, OmegaLambda = 0)
+__global__ void k_finalize(int64_t n, const int *__restrict__ perm, const double *__restrict__ at, const double *__restrict__ pt,
+                           const int *__restrict__ nint, const double *__restrict__ acc_pm, const double *__restrict__ pot_pm,
+                           int have_pm, double G, double pot_const, double *acc, double *pot, int *nint_out, float *oldacc,
+                           float *oldacc_s)
+{
+  int64_t k = (int64_t)blockIdx.x * B + threadIdx.x;
+  if(k >= n)
+    return;
+  int i      = perm[k];
+  double a[3] = {at[3 * k], at[3 * k + 1], at[3 * k + 2]};
+  double p[3] = {0, 0, 0}, pp = 0;
+  if(have_pm)
+    {
+      p[0] = acc_pm[3 * (int64_t)i];
+      p[1] = acc_pm[3 * (int64_t)i + 1];
+      p[2] = acc_pm[3 * (int64_t)i + 2];
+      pp   = pot_pm[i];
+    }
+  double ox = a[0] + p[0] / G, oy = a[1] + p[1] / G, oz = a[2] + p[2] / G;
+  float oa  = (float)sqrt(ox * ox + oy * oy + oz * oz);  // accel.c:237-255
+  oldacc[i] = oa;
+  oldacc_s[k] = oa;
+  for(int d = 0; d < 3; d++)
+    acc[3 * (int64_t)i + d] = a[d] * G + p[d];  // accel.c:258-265 (+ GravPM in the kick)
+  pot[i]      = (pt[k] + pot_const) * G + pp;   // accel.c:269-286
+  nint_out[i] = nint[k];
+}
+
+agrav_status finalize(agrav_ctx *c)
+{
+  if(!c->have_tree_force)
+    return fail(c, AGRAV_ERR_STATE, "finalize before gravity_tree");
+  const int64_t n = c->n;
+  agrav_status st;
+  if((st = c->d_acc.ensure(3 * (n ? n : 1))) || (st = c->d_pot.ensure(n ? n : 1)) || (st = c->d_nint_out.ensure(n ? n : 1)) ||
+     (st = c->d_oldacc.ensure(n ? n : 1)))
+    return st;
+  const bool pm = c->par.pmgrid > 0;
+  if(pm && !c->have_pm)
+    return fail(c, AGRAV_ERR_STATE, "finalize with PMGRID before long_range_force");
+  double pot_const = 0;
+  if(pm && c->par.periodic)
+    {
+      double asmth = c->par.asmth * c->par.box_size / c->par.pmgrid;
+      double L     = c->par.box_size;
+      pot_const    = c->mass_total * M_PI / (asmth * asmth * L * L * L);  // accel.c:269-274
+    }
+  if(!pm)
+    {
+      if((st = c->d_acc_pm.ensure(1)) || (st = c->d_pot_pm.ensure(1)))
+        return st;
+    }
+  AG_HIP(hipEventRecord(c->ev[0], c->stream));
+  if(n > 0)
+    k_finalize<<<(unsigned)((n + B - 1) / B), B, 0, c->stream>>>(n, c->d_idx2.p, c->d_at.p, c->d_pt.p, c->d_nint.p,
+                                                                c->d_acc_pm.p, c->d_pot_pm.p, pm, c->par.G, pot_const,
+                                                                c->d_acc.p, c->d_pot.p, c->d_nint_out.p, c->d_oldacc.p,
+                                                                c->d_oldacc_s.p);
+  AG_HIP(hipGetLastError());
+  AG_HIP(hipEventRecord(c->ev[1], c->stream));
+  AG_HIP(hipEventSynchronize(c->ev[1]));
+  float ms = 0;
+  AG_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+  c->tim.finalize = ms;
+  c->have_oldacc  = true;
+  c->have_final   = true;
+  return AGRAV_OK;
+}
+
+static void soft_from(agrav_ctx *c, int nsoft, const double *eps)
+{
+  // set_softenings, grav_softening.c:90-95
+  c->soft.nsoft = nsoft;
+  for(int i = 0; i < nsoft; i++)
+    c->soft.h[i] = 2.8 * eps[i];
+  c->soft.h[nsoft] = 0;
+}
+
+}  // namespace agrav
+
+using namespace agrav;
+
+extern "C" {
+
+const char *agrav_status_string(agrav_status s)
+{
+  switch(s)
+    {
+      case AGRAV_OK:
+        return "ok";
+      case AGRAV_ERR_BAD_PARAM:
+        return "bad parameter";
+      case AGRAV_ERR_CAPACITY:
+        return "capacity exceeded";
+      case AGRAV_ERR_OUT_OF_BOX:
+        return "particle out of box";
+      case AGRAV_ERR_HIP:
+        return "HIP error";
+      case AGRAV_ERR_OOM:
+        return "out of device memory";
+      case AGRAV_ERR_STATE:
+        return "call order violated";
+      case AGRAV_ERR_FFT:
+        return "rocFFT error";
+      case AGRAV_ERR_NO_DEVICE:
+        return "no GPU device";
+    }
+  return "unknown";
+}
+
+const char *agrav_last_error(const agrav_ctx *ctx)
+{
+  if(ctx && !ctx->err.empty())
+    return ctx->err.c_str();
+  return g_last_msg.c_str();
+}
+
+void agrav_default_params(agrav_params *p)
+{
+  memset(p, 0, sizeof(*p));
+  p->periodic                  = 1;
+  p->box_size                  = 1.0;
+  p->pmgrid                    = 0;
+  p->asmth                     = 1.25;  // allvars.h:461
+  p->rcut                      = 4.5;   // allvars.h:469
+  p->G                         = 1.0;
+  p->err_tol_theta             = 0.7;
+  p->err_tol_force_acc         = 0.0025;
+  p->type_of_opening_criterion = 1;
+  p->n_soft_types              = 1;
+  p->softening[0]              = 0.01;
+  p->precision                 = AGRAV_FP64;
+  p->walk                      = AGRAV_WALK_GROUP;
+  p->group_size                = 64;
+  p->tree_alloc_factor         = 0;
+}
+
+static agrav_status check_params(const agrav_params *p)
+{
+  if(!p)
+    return AGRAV_ERR_BAD_PARAM;
+  if(p->n_soft_types < 1 || p->n_soft_types > AGRAV_MAX_SOFT_TYPES)
+    return AGRAV_ERR_BAD_PARAM;
+  for(int i = 0; i < p->n_soft_types; i++)
+    if(!(p->softening[i] > 0))
+      return AGRAV_ERR_BAD_PARAM;
+  if(p->periodic && !(p->box_size > 0))
+    return AGRAV_ERR_BAD_PARAM;
+  if(p->pmgrid < 0 || (p->pmgrid > 0 && (!p->periodic || (p->pmgrid & (p->pmgrid - 1)) || p->pmgrid < 8 || p->pmgrid > 1024)))
+    return AGRAV_ERR_BAD_PARAM;
+  if(p->pmgrid > 0 && (!(p->asmth > 0) || !(p->rcut > 0)))
+    return AGRAV_ERR_BAD_PARAM;
+  if(!(p->G > 0) || p->err_tol_theta < 0 || p->err_tol_force_acc < 0)
+    return AGRAV_ERR_BAD_PARAM;
+  if(p->type_of_opening_criterion != 0 && p->type_of_opening_criterion != 1)
+    return AGRAV_ERR_BAD_PARAM;
+  if(p->type_of_opening_criterion == 0 && !(p->err_tol_theta > 0))
+    return AGRAV_ERR_BAD_PARAM;
+  if(p->precision != AGRAV_FP64 && p->precision != AGRAV_FP32)
+    return AGRAV_ERR_BAD_PARAM;
+  if(p->walk != AGRAV_WALK_PLAIN && p->walk != AGRAV_WALK_GROUP)
+    return AGRAV_ERR_BAD_PARAM;
+  if(p->precision == AGRAV_FP32 && p->walk == AGRAV_WALK_PLAIN)
+    return AGRAV_ERR_BAD_PARAM;  // the per-target walk is the FP64 restatement only
+  if(p->group_size < 0 || p->group_size > 64)
+    return AGRAV_ERR_BAD_PARAM;
+  return AGRAV_OK;
+}
+
+agrav_status agrav_create(const agrav_params *p, agrav_ctx **out)
+{
+  if(!out)
+    return AGRAV_ERR_BAD_PARAM;
+  *out            = nullptr;
+  agrav_status st = check_params(p);
+  if(st != AGRAV_OK)
+    {
+      set_error_msg("invalid agrav_params");
+      return st;
+    }
+  int ndev = 0;
+  if(hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+    {
+      (void)hipGetLastError();
+      set_error_msg("no HIP device available");
+      return AGRAV_ERR_NO_DEVICE;
+    }
+  agrav_ctx *c = new agrav_ctx();
+  c->par       = *p;
+  if(c->par.group_size == 0)
+    c->par.group_size = 64;
+  if(hipGetDevice(&c->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+    {
+      delete c;
+      set_error_msg("cannot create a HIP stream");
+      return AGRAV_ERR_HIP;
+    }
+  for(auto &e : c->ev)
+    if(hipEventCreate(&e) != hipSuccess)
+      {
+        agrav_destroy(c);
+        return AGRAV_ERR_HIP;
+      }
+  soft_from(c, p->n_soft_types, p->softening);
+  // force_short_range_init, forcetree_walk.c:72-85 (FP64 compute, FP32 store)
+  for(int i = 0; i <= AG_NTAB; i++)
+    {
+      double u        = ((p->rcut / 2.0) / AG_NTAB) * i;
+      c->tab_pot[i]   = (float)(-erfc(u));
+      c->tab_force[i] = u > 0 ? (float)((erfc(u) + 2.0 * u / sqrt(M_PI) * exp(-u * u) - 1.0) / (u * u)) : 0.f;
+    }
+  float tabs[256];
+  memcpy(tabs, c->tab_force, 128 * sizeof(float));
+  memcpy(tabs + 128, c->tab_pot, 128 * sizeof(float));
+  if((st = c->d_tab.ensure(256)) || (st = c->d_soft.ensure(AG_MAXSOFT + 1)))
+    {
+      agrav_destroy(c);
+      return st;
+    }
+  if(hipMemcpy(c->d_tab.p, tabs, sizeof(tabs), hipMemcpyHostToDevice) != hipSuccess ||
+     hipMemcpy(c->d_soft.p, c->soft.h, sizeof(c->soft.h), hipMemcpyHostToDevice) != hipSuccess)
+    {
+      agrav_destroy(c);
+      return AGRAV_ERR_HIP;
+    }
+  if(c->par.pmgrid > 0 && (st = pm_init(c)) != AGRAV_OK)
+    {
+      set_error_msg(c->err);
+      agrav_destroy(c);
+      return st;
+    }
+  *out = c;
+  return AGRAV_OK;
+}
+
+void agrav_destroy(agrav_ctx *c)
+{
+  if(!c)
+    return;
+  if(c->stream)
+    (void)hipStreamSynchronize(c->stream);
+  pm_free(c);
+  for(auto &e : c->ev)
+    if(e)
+      (void)hipEventDestroy(e);
+  // DBuf members release their memory explicitly
+  c->d_soft.release();
+  c->d_tab.release();
+  c->d_ewald.release();
+  c->d_ewald_f.release();
+  c->d_ewald_ft.release();
+  c->d_pos_in.release();
+  c->d_mass_in.release();
+  c->d_stype_in.release();
+  c->d_oldacc.release();
+  c->d_key.release();
+  c->d_key2.release();
+  c->d_idx.release();
+  c->d_idx2.release();
+  c->d_px.release();
+  c->d_py.release();
+  c->d_pz.release();
+  c->d_pm.release();
+  c->d_ps.release();
+  c->d_pnext.release();
+  c->d_prec.release();
+  c->d_oldacc_s.release();
+  c->d_bfirst.release();
+  c->d_blast.release();
+  c->d_bprefix.release();
+  c->d_bparent.release();
+  c->d_bcount.release();
+  c->d_bbase.release();
+  c->d_lvlmask.release();
+  c->d_pcnt.release();
+  c->d_pbase.release();
+  c->d_tfirst.release();
+  c->d_tlast.release();
+  c->d_tlevel.release();
+  c->d_nfirst.release();
+  c->d_nlast.release();
+  c->d_nlevel.release();
+  c->d_sx.release();
+  c->d_sy.release();
+  c->d_sz.release();
+  c->d_nm.release();
+  c->d_cx.release();
+  c->d_cy.release();
+  c->d_cz.release();
+  c->d_len.release();
+  c->d_nextnode.release();
+  c->d_sibling.release();
+  c->d_maxsoft.release();
+  c->d_recA.release();
+  c->d_recB.release();
+  c->d_cbeg.release();
+  c->d_ccnt.release();
+  c->d_child.release();
+  c->d_lvlcnt.release();
+  c->d_lvloff.release();
+  c->d_lvlnodes.release();
+  c->d_at.release();
+  c->d_pt.release();
+  c->d_nint.release();
+  c->d_stack.release();
+  c->d_failed.release();
+  c->d_counters.release();
+  c->d_isum.release();
+  c->d_acc_pm.release();
+  c->d_pot_pm.release();
+  c->d_acc.release();
+  c->d_pot.release();
+  c->d_nint_out.release();
+  c->d_temp.release();
+  c->d_dom.release();
+  if(c->stream)
+    (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+agrav_status agrav_set_softenings(agrav_ctx *c, int nsoft, const double *eps)
+{
+  if(!c || !eps || nsoft < 1 || nsoft > AGRAV_MAX_SOFT_TYPES)
+    return AGRAV_ERR_BAD_PARAM;
+  for(int i = 0; i < nsoft; i++)
+    if(!(eps[i] > 0))
+      return fail(c, AGRAV_ERR_BAD_PARAM, "softening must be positive");
+  c->par.n_soft_types = nsoft;
+  for(int i = 0; i < nsoft; i++)
+    c->par.softening[i] = eps[i];
+  soft_from(c, nsoft, eps);
+  AG_HIP(hipMemcpyAsync(c->d_soft.p, c->soft.h, sizeof(c->soft.h), hipMemcpyHostToDevice, c->stream));
+  AG_HIP(hipStreamSynchronize(c->stream));
+  c->have_tree = c->have_tree_force = c->have_final = false;
+  return AGRAV_OK;
+}
+
+static agrav_status set_particles_common(agrav_ctx *c, int64_t n, const double *pos, const double *mass, const uint8_t *stype,
+                                         hipMemcpyKind kind)
+{
+  if(!c || n < 0 || (n > 0 && (!pos || !mass)))
+    return AGRAV_ERR_BAD_PARAM;
+  if(n >= (1ll << 30))
+    return fail(c, AGRAV_ERR_CAPACITY, "particle count exceeds 2^30 per context");
+  agrav_status st;
+  const int64_t nn = n ? n : 1;
+  if((st = c->d_pos_in.ensure(3 * nn)) || (st = c->d_mass_in.ensure(nn)) || (st = c->d_oldacc.ensure(nn)))
+    return st;
+  hipStream_t s = c->stream;
+  if(n > 0)
+    {
+      AG_HIP(hipMemcpyAsync(c->d_pos_in.p, pos, 3 * n * sizeof(double), kind, s));
+      AG_HIP(hipMemcpyAsync(c->d_mass_in.p, mass, n * sizeof(double), kind, s));
+    }
+  c->have_stype = stype != nullptr;
+  if(stype)
+    {
+      if((st = c->d_stype_in.ensure(nn)))
+        return st;
+      if(kind == hipMemcpyHostToDevice)
+        for(int64_t i = 0; i < n; i++)
+          if(stype[i] >= c->soft.nsoft)
+            return fail(c, AGRAV_ERR_BAD_PARAM, "softening type out of range");
+      AG_HIP(hipMemcpyAsync(c->d_stype_in.p, stype, n, kind, s));
+    }
+  // total mass (host sum for host input, device copy-back otherwise), fixed order
+  double mt = 0;
+  if(kind == hipMemcpyHostToDevice)
+    {
+      for(int64_t i = 0; i < n; i++)
+        mt += mass[i];
+    }
+  else if(n > 0)
+    {
+      std::vector<double> h(n);
+      AG_HIP(hipMemcpyAsync(h.data(), mass, n * sizeof(double), hipMemcpyDeviceToHost, s));
+      AG_HIP(hipStreamSynchronize(s));
+      for(int64_t i = 0; i < n; i++)
+        mt += h[i];
+    }
+  AG_HIP(hipMemsetAsync(c->d_oldacc.p, 0, nn * sizeof(float), s));
+  AG_HIP(hipStreamSynchronize(s));
+  c->mass_total     = mt;
+  c->n              = n;
+  c->have_particles = true;
+  c->have_oldacc    = false;
+  c->have_tree = c->have_tree_force = c->have_pm = c->have_final = false;
+  return AGRAV_OK;
+}
+
+agrav_status agrav_set_particles(agrav_ctx *c, int64_t n, const double *pos, const double *mass, const uint8_t *stype)
+{
+  return set_particles_common(c, n, pos, mass, stype, hipMemcpyHostToDevice);
+}
+
+agrav_status agrav_set_particles_device(agrav_ctx *c, int64_t n, const double *pos, const double *mass, const uint8_t *stype)
+{
+  return set_particles_common(c, n, pos, mass, stype, hipMemcpyDeviceToDevice);
+}
+
+agrav_status agrav_set_old_acc(agrav_ctx *c, const float *old_acc)
+{
+  if(!c || !old_acc)
+    return AGRAV_ERR_BAD_PARAM;
+  if(!c->have_particles)
+    return fail(c, AGRAV_ERR_STATE, "set_old_acc before set_particles");
+  if(c->n == 0)
+    return AGRAV_OK;
+  AG_HIP(hipMemcpyAsync(c->d_oldacc.p, old_acc, c->n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  AG_HIP(hipStreamSynchronize(c->stream));
+  c->have_oldacc = true;
+  c->have_tree   = false;  // the sorted copy is refreshed by the next build
+  return AGRAV_OK;
+}
+
+agrav_status agrav_construct_forcetree(agrav_ctx *c, int *numnodes)
+{
+  if(!c)
+    return AGRAV_ERR_BAD_PARAM;
+  if(!c->have_particles)
+    return fail(c, AGRAV_ERR_STATE, "construct_forcetree before set_particles");
+  AG_HIP(hipEventRecord(c->ev[7], c->stream));
+  agrav_status st = build_tree(c);
+  if(st != AGRAV_OK)
+    {
+      c->have_tree = false;
+      return st;
+    }
+  AG_HIP(hipEventRecord(c->ev[1], c->stream));
+  AG_HIP(hipEventSynchronize(c->ev[1]));
+  float ms = 0;
+  AG_HIP(hipEventElapsedTime(&ms, c->ev[7], c->ev[1]));
+  c->tim.treebuild = ms;
+  c->tim.numnodes  = c->m;
+  if(numnodes)
+    *numnodes = c->m;
+  c->have_tree_force = false;
+  return AGRAV_OK;
+}
+
+agrav_status agrav_gravity_tree(agrav_ctx *c, int use_bh)
+{
+  if(!c)
+    return AGRAV_ERR_BAD_PARAM;
+  return walk_tree(c, use_bh);
+}
+
+agrav_status agrav_long_range_force(agrav_ctx *c)
+{
+  if(!c)
+    return AGRAV_ERR_BAD_PARAM;
+  if(!c->have_particles)
+    return fail(c, AGRAV_ERR_STATE, "long_range_force before set_particles");
+  if(c->par.pmgrid <= 0)
+    return fail(c, AGRAV_ERR_STATE, "long_range_force without PMGRID");
+  return pm_force(c);
+}
+
+agrav_status agrav_gravity_force_finalize(agrav_ctx *c)
+{
+  if(!c)
+    return AGRAV_ERR_BAD_PARAM;
+  return finalize(c);
+}
+
+agrav_status agrav_compute_grav_accelerations(agrav_ctx *c, int first_step)
+{
+  // accel.c:58-80 and gravity() accel.c:94-192
+  if(!c)
+    return AGRAV_ERR_BAD_PARAM;
+  if(!c->have_particles)
+    return fail(c, AGRAV_ERR_STATE, "compute_grav_accelerations before set_particles");
+  agrav_status st;
+  if(c->par.pmgrid > 0 && (st = pm_force(c)) != AGRAV_OK)
+    return st;
+  if((st = agrav_construct_forcetree(c, nullptr)) != AGRAV_OK)
+    return st;
+  const bool seed = c->par.type_of_opening_criterion == 1 && first_step && c->par.err_tol_theta > 0;
+  if(seed)
+    {
+      // Barnes-Hut pass that seeds OldAcc (accel.c:62-72); the tree of the
+      // second gravity() call would be identical, so it is reused.
+      if((st = walk_tree(c, 1)) != AGRAV_OK || (st = finalize(c)) != AGRAV_OK)
+        return st;
+    }
+  if((st = walk_tree(c, c->par.type_of_opening_criterion == 0)) != AGRAV_OK)
+    return st;
+  return finalize(c);
+}
+
+agrav_status agrav_get_accelerations(agrav_ctx *c, double *acc, double *pot, int32_t *nint, float *old_acc)
+{
+  if(!c)
+    return AGRAV_ERR_BAD_PARAM;
+  if(!c->have_final)
+    return fail(c, AGRAV_ERR_STATE, "no results: run gravity_force_finalize first");
+  const int64_t n = c->n;
+  hipStream_t s   = c->stream;
+  if(n > 0)
+    {
+      if(acc)
+        AG_HIP(hipMemcpyAsync(acc, c->d_acc.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost, s));
+      if(pot)
+        AG_HIP(hipMemcpyAsync(pot, c->d_pot.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+      if(nint)
+        AG_HIP(hipMemcpyAsync(nint, c->d_nint_out.p, n * sizeof(int), hipMemcpyDeviceToHost, s));
+      if(old_acc)
+        AG_HIP(hipMemcpyAsync(old_acc, c->d_oldacc.p, n * sizeof(float), hipMemcpyDeviceToHost, s));
+    }
+  AG_HIP(hipStreamSynchronize(s));
+  return AGRAV_OK;
+}
+
+agrav_status agrav_get_accelerations_device(agrav_ctx *c, const double **acc, const double **pot)
+{
+  if(!c)
+    return AGRAV_ERR_BAD_PARAM;
+  if(!c->have_final)
+    return fail(c, AGRAV_ERR_STATE, "no results");
+  if(acc)
+    *acc = c->d_acc.p;
+  if(pot)
+    *pot = c->d_pot.p;
+  return AGRAV_OK;
+}
+
+agrav_status agrav_get_components(agrav_ctx *c, double *acc_tree, double *pot_tree, double *acc_pm, double *pot_pm)
+{
+  if(!c)
+    return AGRAV_ERR_BAD_PARAM;
+  const int64_t n = c->n;
+  hipStream_t s   = c->stream;
+  if(n == 0)
+    return AGRAV_OK;
+  if((acc_tree || pot_tree) && !c->have_tree_force)
+    return fail(c, AGRAV_ERR_STATE, "no tree force");
+  if((acc_pm || pot_pm) && !c->have_pm)
+    return fail(c, AGRAV_ERR_STATE, "no PM force");
+  if(acc_tree || pot_tree)
+    {
+      std::vector<double> at(3 * n), pt(n);
+      std::vector<int> perm(n);
+      AG_HIP(hipMemcpyAsync(at.data(), c->d_at.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost, s));
+      AG_HIP(hipMemcpyAsync(pt.data(), c->d_pt.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+      AG_HIP(hipMemcpyAsync(perm.data(), c->d_idx2.p, n * sizeof(int), hipMemcpyDeviceToHost, s));
+      AG_HIP(hipStreamSynchronize(s));
+      for(int64_t k = 0; k < n; k++)
+        {
+          int i = perm[k];
+          if(acc_tree)
+            for(int d = 0; d < 3; d++)
+              acc_tree[3 * (int64_t)i + d] = at[3 * k + d];
+          if(pot_tree)
+            pot_tree[i] = pt[k];
+        }
+    }
+  if(acc_pm)
+    AG_HIP(hipMemcpyAsync(acc_pm, c->d_acc_pm.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost, s));
+  if(pot_pm)
+    AG_HIP(hipMemcpyAsync(pot_pm, c->d_pot_pm.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  AG_HIP(hipStreamSynchronize(s));
+  return AGRAV_OK;
+}
+
+agrav_status agrav_gravity_forcetest(agrav_ctx *c, int ns, const int32_t *sel, double *acc, double *acc_short, double *pot,
+                                     double *pot_short)
+{
+  if(!c || ns < 0 || (ns > 0 && !sel))
+    return AGRAV_ERR_BAD_PARAM;
+  return forcetest(c, ns, sel, acc, acc_short, pot, pot_short);
+}
+
+agrav_status agrav_gravity_direct(agrav_ctx *c, double *acc, double *pot)
+{
+  if(!c)
+    return AGRAV_ERR_BAD_PARAM;
+  return direct_all(c, acc, pot);
+}
+
+agrav_status agrav_get_timings(agrav_ctx *c, agrav_timings *t)
+{
+  if(!c || !t)
+    return AGRAV_ERR_BAD_PARAM;
+  *t = c->tim;
+  return AGRAV_OK;
+}
+
+agrav_status agrav_get_tree(agrav_ctx *c, int *numnodes, double *center, double *len, double *com, double *mass, int32_t *level,
+                            int32_t *npart)
+{
+  if(!c)
+    return AGRAV_ERR_BAD_PARAM;
+  if(!c->have_tree)
+    return fail(c, AGRAV_ERR_STATE, "no tree");
+  const int M   = c->m;
+  hipStream_t s = c->stream;
+  if(numnodes)
+    *numnodes = M;
+  if(M == 0)
+    return AGRAV_OK;
+  std::vector<double> a(M), b(M), d(M);
+  if(center)
+    {
+      AG_HIP(hipMemcpyAsync(a.data(), c->d_cx.p, M * sizeof(double), hipMemcpyDeviceToHost, s));
+      AG_HIP(hipMemcpyAsync(b.data(), c->d_cy.p, M * sizeof(double), hipMemcpyDeviceToHost, s));
+      AG_HIP(hipMemcpyAsync(d.data(), c->d_cz.p, M * sizeof(double), hipMemcpyDeviceToHost, s));
+      AG_HIP(hipStreamSynchronize(s));
+      for(int j = 0; j < M; j++)
+        {
+          center[3 * j]     = a[j];
+          center[3 * j + 1] = b[j];
+          center[3 * j + 2] = d[j];
+        }
+    }
+  if(com)
+    {
+      AG_HIP(hipMemcpyAsync(a.data(), c->d_sx.p, M * sizeof(double), hipMemcpyDeviceToHost, s));
+      AG_HIP(hipMemcpyAsync(b.data(), c->d_sy.p, M * sizeof(double), hipMemcpyDeviceToHost, s));
+      AG_HIP(hipMemcpyAsync(d.data(), c->d_sz.p, M * sizeof(double), hipMemcpyDeviceToHost, s));
+      AG_HIP(hipStreamSynchronize(s));
+      for(int j = 0; j < M; j++)
+        {
+          com[3 * j]     = a[j];
+          com[3 * j + 1] = b[j];
+          com[3 * j + 2] = d[j];
+        }
+    }
+  if(len)
+    AG_HIP(hipMemcpyAsync(len, c->d_len.p, M * sizeof(double), hipMemcpyDeviceToHost, s));
+  if(mass)
+    AG_HIP(hipMemcpyAsync(mass, c->d_nm.p, M * sizeof(double), hipMemcpyDeviceToHost, s));
+  if(level)
+    AG_HIP(hipMemcpyAsync(level, c->d_nlevel.p, M * sizeof(int), hipMemcpyDeviceToHost, s));
+  if(npart)
+    {
+      std::vector<int> f(M), l(M);
+      AG_HIP(hipMemcpyAsync(f.data(), c->d_nfirst.p, M * sizeof(int), hipMemcpyDeviceToHost, s));
+      AG_HIP(hipMemcpyAsync(l.data(), c->d_nlast.p, M * sizeof(int), hipMemcpyDeviceToHost, s));
+      AG_HIP(hipStreamSynchronize(s));
+      for(int j = 0; j < M; j++)
+        npart[j] = l[j] - f[j] + 1;
+    }
+  AG_HIP(hipStreamSynchronize(s));
+  return AGRAV_OK;
+}
+
+agrav_status agrav_get_order(agrav_ctx *c, int32_t *perm, uint64_t *keys)
+{
+  if(!c)
+    return AGRAV_ERR_BAD_PARAM;
+  if(!c->have_tree)
+    return fail(c, AGRAV_ERR_STATE, "no tree");
+  const int64_t n = c->n;
+  if(n == 0)
+    return AGRAV_OK;
+  if(perm)
+    AG_HIP(hipMemcpyAsync(perm, c->d_idx2.p, n * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  if(keys)
+    AG_HIP(hipMemcpyAsync(keys, c->d_key2.p, n * sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
+  AG_HIP(hipStreamSynchronize(c->stream));
+  return AGRAV_OK;
+}
+
+agrav_status agrav_get_tables(agrav_ctx *c, float *tab_force, float *tab_pot, double *ewald_tree, double *ewald_forcetest)
+{
+  if(!c)
+    return AGRAV_ERR_BAD_PARAM;
+  if(tab_force)
+    memcpy(tab_force, c->tab_force, sizeof(c->tab_force));
+  if(tab_pot)
+    memcpy(tab_pot, c->tab_pot, sizeof(c->tab_pot));
+  agrav_status st;
+  if(ewald_tree)
+    {
+      if(!c->ewald_ready && (st = ewald_tree_table(c)) != AGRAV_OK)
+        return st;
+      AG_HIP(hipMemcpyAsync(ewald_tree, c->d_ewald.p, 4 * 65 * 65 * 65 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    }
+  if(ewald_forcetest)
+    {
+      if(!c->ewald_ft_ready && (st = ewald_forcetest_table(c)) != AGRAV_OK)
+        return st;
+      AG_HIP(hipMemcpyAsync(ewald_forcetest, c->d_ewald_ft.p, (size_t)4 * 129 * 129 * 129 * sizeof(double), hipMemcpyDeviceToHost,
+                            c->stream));
+    }
+  AG_HIP(hipStreamSynchronize(c->stream));
+  return AGRAV_OK;
+}
+
+uint64_t agrav_hilbert_key21(uint32_t x, uint32_t y, uint32_t z) { return hilbert_key21(x, y, z); }
+
+}  // extern "C"

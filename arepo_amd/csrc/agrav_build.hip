@@ -1,0 +1,659 @@
+// agrav_build.hip -- octree construction (construct_forcetree, forcetree.c:110-1429).
+//
+// AREPO inserts particles one by one into a pointer octree (forcetree.c:835-938)
+// and then computes moments with a recursive depth-first pass (:1217-1429).
+// Here the same octree -- every cube that holds two or more particles, single
+// particles hanging in the child slots -- is derived in parallel from sorted
+// 63-bit Peano-Hilbert keys:
+//   1. domain cube (domain_box.c:107-197) and keys from AREPO's 52-bit integer
+//      coordinates (forcetree.c:493-505), top 21 bits per axis;
+//   2. LSD radix sort of the keys (Hilbert order, like domain.c:140);
+//   3. binary radix tree over the sorted keys (Karras, HPG 2012);
+//   4. each binary node whose prefix crosses 3-bit digit boundaries yields the
+//      octree cubes in between; cubes are placed in pre-order (first particle,
+//      then level) so AREPO's nextnode/sibling threading (forcetree.c:1288-1330)
+//      follows from array positions;
+//   5. moments level by level, deepest first, in FP64.
+// Cube centres and lengths use AREPO's integer formula (forcetree.c:903-906),
+// so node geometry is bit-identical to the reference's.  Cells of the
+// 21st level that still hold several particles become buckets whose
+// particles are direct children (AREPO would split them further, up to level 30).
+#include "agrav_ctx.h"
+
+namespace agrav
+{
+
+static constexpr int B = 256;
+
+static inline unsigned nblk(int64_t n, int b = B) { return (unsigned)((n + b - 1) / b); }
+
+__device__ inline unsigned long long ordered_bits(double v)
+{
+  unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  return (u & 0x8000000000000000ull) ? ~u : (u | 0x8000000000000000ull);
+}
+__device__ inline double from_ordered(unsigned long long u)
+{
+  u = (u & 0x8000000000000000ull) ? (u & 0x7FFFFFFFFFFFFFFFull) : ~u;
+  return __longlong_as_double((long long)u);
+}
+
+__device__ inline double wrap_box(double x, double L)
+{
+  if(x < 0)
+    x += L;
+  if(x >= L)
+    x -= L;
+  return x;
+}
+
+// ---------------------------------------------------------------------------
+// 1. domain extent (domain_box.c:107-197, DO_NOT_RANDOMIZE_DOMAINCENTER)
+// ---------------------------------------------------------------------------
+
+__global__ void k_extent(const double *__restrict__ pos, int64_t n, unsigned long long *mm)
+{
+  // mm[0..2] = ordered min, mm[3..5] = ordered max (initialised by the host)
+  __shared__ unsigned long long smin[3][B / 64], smax[3][B / 64];
+  unsigned long long lo[3] = {~0ull, ~0ull, ~0ull}, hi[3] = {0, 0, 0};
+  for(int64_t i = (int64_t)blockIdx.x * B + threadIdx.x; i < n; i += (int64_t)gridDim.x * B)
+    for(int d = 0; d < 3; d++)
+      {
+        unsigned long long o = ordered_bits(pos[3 * i + d]);
+        lo[d]                = o < lo[d] ? o : lo[d];
+        hi[d]                = o > hi[d] ? o : hi[d];
+      }
+  for(int d = 0; d < 3; d++)
+    {
+      for(int off = 32; off > 0; off >>= 1)
+        {
+          unsigned long long a = __shfl_xor(lo[d], off, 64), b = __shfl_xor(hi[d], off, 64);
+          lo[d]                = a < lo[d] ? a : lo[d];
+          hi[d]                = b > hi[d] ? b : hi[d];
+        }
+      if((threadIdx.x & 63) == 0)
+        {
+          smin[d][threadIdx.x >> 6] = lo[d];
+          smax[d][threadIdx.x >> 6] = hi[d];
+        }
+    }
+  __syncthreads();
+  if(threadIdx.x < 3)
+    {
+      int d                = threadIdx.x;
+      unsigned long long a = smin[d][0], b = smax[d][0];
+      for(int w = 1; w < B / 64; w++)
+        {
+          a = smin[d][w] < a ? smin[d][w] : a;
+          b = smax[d][w] > b ? smax[d][w] : b;
+        }
+      atomicMin(&mm[d], a);
+      atomicMax(&mm[3 + d], b);
+    }
+}
+
+__global__ void k_domain_finish(const unsigned long long *mm, Domain *dom, int periodic, double box, int64_t n)
+{
+  double xmin[3], xmax[3];
+  for(int d = 0; d < 3; d++)
+    {
+      xmin[d] = 0;  // preset to the simulation box (domain_box.c:112-116)
+      xmax[d] = box;
+      if(!periodic && n > 0)
+        {
+          double a = from_ordered(mm[d]), b = from_ordered(mm[3 + d]);
+          xmin[d]  = a < xmin[d] ? a : xmin[d];
+          xmax[d]  = b > xmax[d] ? b : xmax[d];
+        }
+      dom->xmin[d] = xmin[d];
+      dom->xmax[d] = xmax[d];
+    }
+  double len = 0;
+  for(int d = 0; d < 3; d++)
+    if(xmax[d] - xmin[d] > len)
+      len = xmax[d] - xmin[d];
+  if(len <= 0)
+    len = 1.0;
+  len *= periodic ? 1.00001 : 1.2;  // domain_box.c:161-168
+  for(int d = 0; d < 3; d++)
+    {
+      dom->center[d] = 0.5 * (xmin[d] + xmax[d]);
+      dom->corner[d] = 0.5 * (xmin[d] + xmax[d]) - 0.5 * len;
+    }
+  dom->len     = len;
+  dom->inv_len = 1.0 / len;
+  dom->bigfac  = len / (double)(1ll << 52);
+  if(periodic)
+    {
+      dom->fix_unit = box / 4294967296.0;  // 2^32 steps per box: wrap-around is the nearest image
+      for(int d = 0; d < 3; d++)
+        dom->fix_org[d] = 0;
+    }
+  else
+    {
+      dom->fix_unit = len / 2147483648.0;  // 2^31 steps per domain: differences fit in int32
+      for(int d = 0; d < 3; d++)
+        dom->fix_org[d] = dom->corner[d];
+    }
+  dom->fix_inv    = 1.0 / dom->fix_unit;
+  dom->out_of_box = 0;
+}
+
+__device__ inline uint32_t to_fix(double x, const Domain &dom, int d)
+{
+  return (uint32_t)(long long)llrint((x - dom.fix_org[d]) * dom.fix_inv);
+}
+
+// ---------------------------------------------------------------------------
+// 2. keys (forcetree.c:394-417 wrap, :493-505 integer coordinates)
+// ---------------------------------------------------------------------------
+
+__device__ inline uint64_t int52(double x, const Domain &dom, int d)
+{
+  double v = (x - dom.corner[d]) * dom.inv_len + 1.0;
+  return (uint64_t)__double_as_longlong(v) & 0xFFFFFFFFFFFFFull;
+}
+
+__global__ void k_keys(const double *__restrict__ pos, int64_t n, const Domain *__restrict__ domp, int periodic, double box,
+                       uint64_t *__restrict__ keys, int *__restrict__ idx, Domain *dom_w)
+{
+  int64_t i = (int64_t)blockIdx.x * B + threadIdx.x;
+  if(i >= n)
+    return;
+  const Domain dom = *domp;
+  uint32_t c[3];
+  bool bad = false;
+  for(int d = 0; d < 3; d++)
+    {
+      double x = pos[3 * i + d];
+      if(periodic)
+        x = wrap_box(x, box);
+      if(!(x >= dom.corner[d] && x < dom.corner[d] + dom.len))
+        bad = true;
+      c[d] = (uint32_t)(int52(x, dom, d) >> (52 - AG_KEY_LEVELS));
+    }
+  if(bad)
+    {
+      atomicExch(&dom_w->out_of_box, 1);
+      c[0] = c[1] = c[2] = 0;
+    }
+  keys[i] = hilbert_key21(c[0], c[1], c[2]);
+  idx[i]  = (int)i;
+}
+
+__global__ void k_gather(const int *__restrict__ idx, const double *__restrict__ pos, const double *__restrict__ mass,
+                         const uint8_t *__restrict__ stype, const float *__restrict__ oldacc, int64_t n, int periodic, double box,
+                         const Domain *__restrict__ domp, double *px, double *py, double *pz, double *pm, uint8_t *ps, uint4 *prec,
+                         float *oldacc_s)
+{
+  int64_t k = (int64_t)blockIdx.x * B + threadIdx.x;
+  if(k >= n)
+    return;
+  const Domain dom = *domp;
+  int i            = idx[k];
+  double x = pos[3 * (int64_t)i], y = pos[3 * (int64_t)i + 1], z = pos[3 * (int64_t)i + 2];
+  if(periodic)
+    {
+      x = wrap_box(x, box);
+      y = wrap_box(y, box);
+      z = wrap_box(z, box);
+    }
+  px[k]     = x;
+  py[k]     = y;
+  pz[k]     = z;
+  double m  = mass[i];
+  pm[k]     = m;
+  if(ps)
+    ps[k] = stype ? stype[i] : 0;
+  prec[k] = make_uint4(to_fix(x, dom, 0), to_fix(y, dom, 1), to_fix(z, dom, 2), __float_as_uint((float)m));
+  if(oldacc_s)
+    oldacc_s[k] = oldacc ? oldacc[i] : 0.f;
+}
+
+// ---------------------------------------------------------------------------
+// 3. binary radix tree (Karras 2012)
+// ---------------------------------------------------------------------------
+
+__device__ inline int kdelta(const uint64_t *__restrict__ k, int n, int i, int j)
+{
+  if(j < 0 || j >= n)
+    return -1;
+  uint64_t a = k[i], b = k[j];
+  if(a == b)
+    return 64 + __clz((unsigned)(i ^ j));
+  return __clzll((long long)(a ^ b)) - 1;  // keys use bits 62..0
+}
+
+__device__ inline int oct_level(int prefix) { return min(prefix / 3, AG_KEY_LEVELS); }
+
+__global__ void k_binary(const uint64_t *__restrict__ keys, int n, int *bfirst, int *blast, int *bprefix, int *bparent)
+{
+  int i = blockIdx.x * B + threadIdx.x;
+  if(i >= n - 1)
+    return;
+  int dl = kdelta(keys, n, i, i - 1), dr = kdelta(keys, n, i, i + 1);
+  int d    = dr > dl ? 1 : -1;
+  int dmin = dr > dl ? dl : dr;
+  int lmax = 2;
+  while(kdelta(keys, n, i, i + lmax * d) > dmin)
+    lmax <<= 1;
+  int l = 0;
+  for(int t = lmax >> 1; t >= 1; t >>= 1)
+    if(kdelta(keys, n, i, i + (l + t) * d) > dmin)
+      l += t;
+  int j     = i + l * d;
+  int dnode = kdelta(keys, n, i, j);
+  int s = 0, t = l;
+  do
+    {
+      t = (t + 1) >> 1;
+      if(kdelta(keys, n, i, i + (s + t) * d) > dnode)
+        s += t;
+    }
+  while(t > 1);
+  int gamma = i + s * d + (d < 0 ? -1 : 0);
+  int first = i < j ? i : j, last = i < j ? j : i;
+  bfirst[i]  = first;
+  blast[i]   = last;
+  bprefix[i] = dnode;
+  if(first != gamma)
+    bparent[gamma] = i;
+  if(last != gamma + 1)
+    bparent[gamma + 1] = i;
+  if(i == 0)
+    bparent[0] = -1;
+}
+
+// number of octree cubes each binary node stands for
+__global__ void k_oct_count(int n, const int *__restrict__ bprefix, const int *__restrict__ bparent, int *bcount)
+{
+  int i = blockIdx.x * B + threadIdx.x;
+  if(i >= n - 1)
+    return;
+  int lp    = (i == 0) ? -1 : oct_level(bprefix[bparent[i]]);
+  bcount[i] = oct_level(bprefix[i]) - lp;
+}
+
+__global__ void k_oct_emit(int n, const int *__restrict__ bfirst, const int *__restrict__ blast, const int *__restrict__ bprefix,
+                           const int *__restrict__ bparent, const int *__restrict__ bcount, const int *__restrict__ bbase,
+                           int *tfirst, int *tlast, int *tlevel, unsigned *lvlmask, int *lvlcnt)
+{
+  int i = blockIdx.x * B + threadIdx.x;
+  if(i >= n - 1)
+    return;
+  int c = bcount[i];
+  if(c <= 0)
+    return;
+  int lp = (i == 0) ? -1 : oct_level(bprefix[bparent[i]]);
+  int f = bfirst[i], la = blast[i], b = bbase[i];
+  for(int k = 0; k < c; k++)
+    {
+      int L         = lp + 1 + k;
+      tfirst[b + k] = f;
+      tlast[b + k]  = la;
+      tlevel[b + k] = L;
+      atomicOr(&lvlmask[f], 1u << L);
+      atomicAdd(&lvlcnt[L], 1);
+    }
+}
+
+__global__ void k_single_root(int *tfirst, int *tlast, int *tlevel, unsigned *lvlmask, int *lvlcnt)
+{
+  tfirst[0]  = 0;
+  tlast[0]   = 0;
+  tlevel[0]  = 0;
+  lvlmask[0] = 1u;
+  lvlcnt[0]  = 1;
+}
+
+__global__ void k_first_count(const unsigned *__restrict__ lvlmask, int n, int *pcnt)
+{
+  int f = blockIdx.x * B + threadIdx.x;
+  if(f < n)
+    pcnt[f] = __popc(lvlmask[f]);
+}
+
+__device__ inline int pnext_of(int p, int n, const unsigned *__restrict__ lvlmask, const int *__restrict__ pbase)
+{
+  int q = p + 1;
+  if(q >= n)
+    return -1;
+  return lvlmask[q] ? n + pbase[q] : q;
+}
+
+// place cubes in pre-order and derive geometry and threading
+__global__ void k_place(int mt, int n, const int *__restrict__ tfirst, const int *__restrict__ tlast, const int *__restrict__ tlevel,
+                        const unsigned *__restrict__ lvlmask, const int *__restrict__ pbase, const double *__restrict__ px,
+                        const double *__restrict__ py, const double *__restrict__ pz, const Domain *__restrict__ domp, int *nfirst,
+                        int *nlast, int *nlevel, double *cx, double *cy, double *cz, double *len, int *nextnode, int *sibling)
+{
+  int t = blockIdx.x * B + threadIdx.x;
+  if(t >= mt)
+    return;
+  const Domain dom = *domp;
+  int f = tfirst[t], la = tlast[t], L = tlevel[t];
+  unsigned mask = lvlmask[f];
+  int j         = pbase[f] + __popc(mask & ((1u << L) - 1u));
+  nfirst[j]     = f;
+  nlast[j]      = la;
+  nlevel[j]     = L;
+  len[j]        = ldexp(dom.len, -L);
+  if(L == 0)
+    {
+      cx[j] = dom.center[0];
+      cy[j] = dom.center[1];
+      cz[j] = dom.center[2];
+    }
+  else
+    {
+      // forcetree.c:903-906: ((ipos & centermask) | mask) * BigFac + corner
+      const int sh       = 52 - L;
+      const uint64_t half = 1ull << (51 - L);
+      double p[3]        = {px[f], py[f], pz[f]};
+      double c[3];
+      for(int d = 0; d < 3; d++)
+        {
+          uint64_t ip = int52(p[d], dom, d);
+          c[d]        = (double)(((ip >> sh) << sh) | half) * dom.bigfac + dom.corner[d];
+        }
+      cx[j] = c[0];
+      cy[j] = c[1];
+      cz[j] = c[2];
+    }
+  bool deeper = (mask >> (L + 1)) != 0u;
+  nextnode[j] = deeper ? n + j + 1 : f;
+  sibling[j]  = pnext_of(la, n, lvlmask, pbase);
+}
+
+__global__ void k_pnext(int n, const unsigned *__restrict__ lvlmask, const int *__restrict__ pbase, int *pnext)
+{
+  int p = blockIdx.x * B + threadIdx.x;
+  if(p < n)
+    pnext[p] = pnext_of(p, n, lvlmask, pbase);
+}
+
+__global__ void k_level_scatter(int m, const int *__restrict__ nlevel, const int *__restrict__ lvloff, int *lvlfill, int *lvlnodes)
+{
+  int j = blockIdx.x * B + threadIdx.x;
+  if(j >= m)
+    return;
+  int L                      = nlevel[j];
+  int slot                   = atomicAdd(&lvlfill[L], 1);
+  lvlnodes[lvloff[L] + slot] = j;
+}
+
+// ---------------------------------------------------------------------------
+// 5. moments (forcetree.c:1217-1429): mass, FP64 centre of mass, maxsofttype
+// ---------------------------------------------------------------------------
+
+__global__ void k_moments(int cnt, const int *__restrict__ list, int n, const int *__restrict__ nextnode,
+                          const int *__restrict__ sibling, const int *__restrict__ pnext, const double *__restrict__ px,
+                          const double *__restrict__ py, const double *__restrict__ pz, const double *__restrict__ pm,
+                          const uint8_t *__restrict__ ps, const double *__restrict__ hsoft, int nsoft, const double *__restrict__ cx,
+                          const double *__restrict__ cy, const double *__restrict__ cz, const double *__restrict__ len,
+                          double *sx, double *sy, double *sz, double *nm, uint8_t *maxsoft, int *ccnt, uint4 *recA, uint4 *recB,
+                          const Domain *__restrict__ domp)
+{
+  int t = blockIdx.x * B + threadIdx.x;
+  if(t >= cnt)
+    return;
+  int j   = list[t];
+  int c   = nextnode[j];
+  int end = sibling[j];
+  double m = 0, s0 = 0, s1 = 0, s2 = 0;
+  int ms   = nsoft;  // sentinel type with zero softening (forcetree.c:1254)
+  int k    = 0;
+  while(c != end)
+    {
+      if(c < n)
+        {
+          double mm = pm[c];
+          m += mm;
+          s0 += mm * px[c];
+          s1 += mm * py[c];
+          s2 += mm * pz[c];
+          int st = ps ? ps[c] : 0;
+          if(hsoft[ms] < hsoft[st])
+            ms = st;
+          c = pnext[c];
+        }
+      else
+        {
+          int q     = c - n;
+          double mm = nm[q];
+          m += mm;
+          s0 += mm * sx[q];
+          s1 += mm * sy[q];
+          s2 += mm * sz[q];
+          int st = maxsoft[q];
+          if(hsoft[ms] < hsoft[st])
+            ms = st;
+          c = sibling[q];
+        }
+      k++;
+    }
+  if(m != 0)
+    {
+      s0 /= m;
+      s1 /= m;
+      s2 /= m;
+    }
+  else
+    {
+      s0 = cx[j];
+      s1 = cy[j];
+      s2 = cz[j];
+    }
+  sx[j]      = s0;
+  sy[j]      = s1;
+  sz[j]      = s2;
+  nm[j]      = m;
+  maxsoft[j] = (uint8_t)ms;
+  ccnt[j]    = k;
+  const Domain dom = *domp;
+  recA[j] = make_uint4(to_fix(s0, dom, 0), to_fix(s1, dom, 1), to_fix(s2, dom, 2), __float_as_uint((float)m));
+  recB[j] = make_uint4(to_fix(cx[j], dom, 0), to_fix(cy[j], dom, 1), to_fix(cz[j], dom, 2), __float_as_uint((float)len[j]));
+}
+
+__global__ void k_fill_children(int m, int n, const int *__restrict__ nextnode, const int *__restrict__ sibling,
+                                const int *__restrict__ pnext, const int *__restrict__ cbeg, int *child)
+{
+  int j = blockIdx.x * B + threadIdx.x;
+  if(j >= m)
+    return;
+  int c = nextnode[j], end = sibling[j], o = cbeg[j];
+  while(c != end)
+    {
+      child[o++] = c;
+      c          = (c < n) ? pnext[c] : sibling[c - n];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host driver
+// ---------------------------------------------------------------------------
+
+agrav_status build_tree(agrav_ctx *c)
+{
+  hipStream_t s   = c->stream;
+  const int64_t n = c->n;
+  const int N     = (int)n;
+  agrav_status st;
+  if(n <= 0)
+    {
+      c->m         = 0;
+      c->have_tree = true;
+      return AGRAV_OK;
+    }
+  if(n >= (1ll << 30))
+    return fail(c, AGRAV_ERR_CAPACITY, "particle count exceeds 2^30 per context");
+
+  // scratch
+  size_t temp = sort_temp_bytes(n);
+  size_t t2   = scan_temp_bytes(n) + 4096;
+  if(t2 > temp)
+    temp = t2;
+  if((st = c->d_temp.ensure(temp + 4096)) != AGRAV_OK)
+    return st;
+  if((st = c->d_dom.ensure(1)) != AGRAV_OK)
+    return st;
+  if((st = c->d_counters.ensure(64)) != AGRAV_OK)
+    return st;
+
+  // 1. domain
+  unsigned long long mm_init[6] = {~0ull, ~0ull, ~0ull, 0, 0, 0};
+  unsigned long long *d_mm      = (unsigned long long *)c->d_counters.p;  // 6 x u64 = 12 ints
+  AG_HIP(hipMemcpyAsync(d_mm, mm_init, sizeof(mm_init), hipMemcpyHostToDevice, s));
+  if(!c->par.periodic)
+    {
+      unsigned g = nblk(n);
+      if(g > 1024)
+        g = 1024;
+      k_extent<<<g, B, 0, s>>>(c->d_pos_in.p, n, d_mm);
+    }
+  k_domain_finish<<<1, 1, 0, s>>>(d_mm, c->d_dom.p, c->par.periodic, c->par.box_size, n);
+  AG_HIP(hipGetLastError());
+
+  // 2. keys and sort
+  if((st = c->d_key.ensure(n)) || (st = c->d_key2.ensure(n)) || (st = c->d_idx.ensure(n)) || (st = c->d_idx2.ensure(n)))
+    return st;
+  k_keys<<<nblk(n), B, 0, s>>>(c->d_pos_in.p, n, c->d_dom.p, c->par.periodic, c->par.box_size, c->d_key.p, c->d_idx.p,
+                               c->d_dom.p);
+  AG_HIP(hipGetLastError());
+  if((st = radix_sort_u64(c->d_key.p, c->d_idx.p, c->d_key2.p, c->d_idx2.p, n, 3 * AG_KEY_LEVELS, c->d_temp.p, s)) != AGRAV_OK)
+    return st;
+  // sorted keys/perm are now in d_key2/d_idx2
+
+  if((st = c->d_px.ensure(n)) || (st = c->d_py.ensure(n)) || (st = c->d_pz.ensure(n)) || (st = c->d_pm.ensure(n)) ||
+     (st = c->d_prec.ensure(n)) || (st = c->d_pnext.ensure(n)) || (st = c->d_oldacc_s.ensure(n)))
+    return st;
+  const bool multisoft = c->soft.nsoft > 1 || c->have_stype;
+  if(multisoft && (st = c->d_ps.ensure(n)))
+    return st;
+  k_gather<<<nblk(n), B, 0, s>>>(c->d_idx2.p, c->d_pos_in.p, c->d_mass_in.p, c->have_stype ? c->d_stype_in.p : nullptr,
+                                 c->have_oldacc ? c->d_oldacc.p : nullptr, n, c->par.periodic, c->par.box_size, c->d_dom.p,
+                                 c->d_px.p, c->d_py.p, c->d_pz.p, c->d_pm.p, multisoft ? c->d_ps.p : nullptr, c->d_prec.p,
+                                 c->d_oldacc_s.p);
+  AG_HIP(hipGetLastError());
+
+  // 3./4. binary tree -> octree cubes
+  const int nb = N > 1 ? N - 1 : 1;
+  if((st = c->d_bfirst.ensure(nb)) || (st = c->d_blast.ensure(nb)) || (st = c->d_bprefix.ensure(nb)) ||
+     (st = c->d_bparent.ensure(nb)) || (st = c->d_bcount.ensure(nb)) || (st = c->d_bbase.ensure(nb)) ||
+     (st = c->d_lvlmask.ensure(n)) || (st = c->d_pcnt.ensure(n)) || (st = c->d_pbase.ensure(n)) ||
+     (st = c->d_lvlcnt.ensure(AG_KEY_LEVELS + 2)) || (st = c->d_lvloff.ensure(AG_KEY_LEVELS + 2)))
+    return st;
+  AG_HIP(hipMemsetAsync(c->d_lvlmask.p, 0, n * sizeof(unsigned), s));
+  AG_HIP(hipMemsetAsync(c->d_lvlcnt.p, 0, (AG_KEY_LEVELS + 2) * sizeof(int), s));
+  int *d_total = c->d_counters.p + 16;
+  int mt       = 1;
+  if(N > 1)
+    {
+      k_binary<<<nblk(N - 1), B, 0, s>>>(c->d_key2.p, N, c->d_bfirst.p, c->d_blast.p, c->d_bprefix.p, c->d_bparent.p);
+      k_oct_count<<<nblk(N - 1), B, 0, s>>>(N, c->d_bprefix.p, c->d_bparent.p, c->d_bcount.p);
+      AG_HIP(hipGetLastError());
+      if((st = exclusive_scan_i32(c->d_bcount.p, c->d_bbase.p, N - 1, d_total, c->d_temp.p, s)) != AGRAV_OK)
+        return st;
+      AG_HIP(hipMemcpyAsync(&mt, d_total, sizeof(int), hipMemcpyDeviceToHost, s));
+      AG_HIP(hipStreamSynchronize(s));
+    }
+  // capacity check in the spirit of TreeAllocFactor (forcetree.c:186-228): the
+  // exact count is known here, so the node arrays are sized to it.
+  if(mt <= 0 || mt > 24 * N + 1)
+    return fail(c, AGRAV_ERR_CAPACITY, "octree node count out of range");
+  const int M = mt;
+  if((st = c->d_tfirst.ensure(M)) || (st = c->d_tlast.ensure(M)) || (st = c->d_tlevel.ensure(M)) ||
+     (st = c->d_nfirst.ensure(M)) || (st = c->d_nlast.ensure(M)) || (st = c->d_nlevel.ensure(M)) || (st = c->d_sx.ensure(M)) ||
+     (st = c->d_sy.ensure(M)) || (st = c->d_sz.ensure(M)) || (st = c->d_nm.ensure(M)) || (st = c->d_cx.ensure(M)) ||
+     (st = c->d_cy.ensure(M)) || (st = c->d_cz.ensure(M)) || (st = c->d_len.ensure(M)) || (st = c->d_nextnode.ensure(M)) ||
+     (st = c->d_sibling.ensure(M)) || (st = c->d_maxsoft.ensure(M)) || (st = c->d_recA.ensure(M)) ||
+     (st = c->d_recB.ensure(M)) || (st = c->d_cbeg.ensure(M)) || (st = c->d_ccnt.ensure(M)) ||
+     (st = c->d_child.ensure((size_t)M + n)) || (st = c->d_lvlnodes.ensure(M)))
+    return st;
+  if(N > 1)
+    k_oct_emit<<<nblk(N - 1), B, 0, s>>>(N, c->d_bfirst.p, c->d_blast.p, c->d_bprefix.p, c->d_bparent.p, c->d_bcount.p,
+                                         c->d_bbase.p, c->d_tfirst.p, c->d_tlast.p, c->d_tlevel.p, c->d_lvlmask.p, c->d_lvlcnt.p);
+  else
+    k_single_root<<<1, 1, 0, s>>>(c->d_tfirst.p, c->d_tlast.p, c->d_tlevel.p, c->d_lvlmask.p, c->d_lvlcnt.p);
+  k_first_count<<<nblk(N), B, 0, s>>>(c->d_lvlmask.p, N, c->d_pcnt.p);
+  AG_HIP(hipGetLastError());
+  if((st = exclusive_scan_i32(c->d_pcnt.p, c->d_pbase.p, N, nullptr, c->d_temp.p, s)) != AGRAV_OK)
+    return st;
+  k_place<<<nblk(M), B, 0, s>>>(M, N, c->d_tfirst.p, c->d_tlast.p, c->d_tlevel.p, c->d_lvlmask.p, c->d_pbase.p, c->d_px.p,
+                                c->d_py.p, c->d_pz.p, c->d_dom.p, c->d_nfirst.p, c->d_nlast.p, c->d_nlevel.p, c->d_cx.p, c->d_cy.p,
+                                c->d_cz.p, c->d_len.p, c->d_nextnode.p, c->d_sibling.p);
+  k_pnext<<<nblk(N), B, 0, s>>>(N, c->d_lvlmask.p, c->d_pbase.p, c->d_pnext.p);
+  AG_HIP(hipGetLastError());
+
+  // 5. moments, deepest level first
+  AG_HIP(hipMemcpyAsync(c->lvlcnt_h, c->d_lvlcnt.p, (AG_KEY_LEVELS + 1) * sizeof(int), hipMemcpyDeviceToHost, s));
+  AG_HIP(hipStreamSynchronize(s));
+  int off[AG_KEY_LEVELS + 2];
+  off[0] = 0;
+  for(int L = 0; L <= AG_KEY_LEVELS; L++)
+    off[L + 1] = off[L] + c->lvlcnt_h[L];
+  if(off[AG_KEY_LEVELS + 1] != M)
+    return fail(c, AGRAV_ERR_CAPACITY, "inconsistent octree level counts");
+  AG_HIP(hipMemcpyAsync(c->d_lvloff.p, off, (AG_KEY_LEVELS + 2) * sizeof(int), hipMemcpyHostToDevice, s));
+  int *d_fill = c->d_counters.p + 24;  // 22 ints
+  AG_HIP(hipMemsetAsync(d_fill, 0, (AG_KEY_LEVELS + 1) * sizeof(int), s));
+  k_level_scatter<<<nblk(M), B, 0, s>>>(M, c->d_nlevel.p, c->d_lvloff.p, d_fill, c->d_lvlnodes.p);
+  AG_HIP(hipGetLastError());
+  for(int L = AG_KEY_LEVELS; L >= 0; L--)
+    {
+      int cnt = c->lvlcnt_h[L];
+      if(cnt == 0)
+        continue;
+      k_moments<<<nblk(cnt), B, 0, s>>>(cnt, c->d_lvlnodes.p + off[L], N, c->d_nextnode.p, c->d_sibling.p, c->d_pnext.p, c->d_px.p,
+                                        c->d_py.p, c->d_pz.p, c->d_pm.p, multisoft ? c->d_ps.p : nullptr, c->d_soft.p,
+                                        c->soft.nsoft, c->d_cx.p, c->d_cy.p, c->d_cz.p, c->d_len.p, c->d_sx.p, c->d_sy.p,
+                                        c->d_sz.p, c->d_nm.p, c->d_maxsoft.p, c->d_ccnt.p, c->d_recA.p, c->d_recB.p, c->d_dom.p);
+    }
+  AG_HIP(hipGetLastError());
+  if((st = exclusive_scan_i32(c->d_ccnt.p, c->d_cbeg.p, M, nullptr, c->d_temp.p, s)) != AGRAV_OK)
+    return st;
+  k_fill_children<<<nblk(M), B, 0, s>>>(M, N, c->d_nextnode.p, c->d_sibling.p, c->d_pnext.p, c->d_cbeg.p, c->d_child.p);
+  AG_HIP(hipGetLastError());
+
+  // out-of-box check (forcetree.c:432-438 aborts; here it is an error code)
+  AG_HIP(hipMemcpyAsync(&c->h_dom, c->d_dom.p, sizeof(Domain), hipMemcpyDeviceToHost, s));
+  AG_HIP(hipStreamSynchronize(s));
+  if(c->h_dom.out_of_box)
+    return fail(c, AGRAV_ERR_OUT_OF_BOX, "particle outside the periodic box after one wrap");
+  c->m         = M;
+  c->have_tree = true;
+  return AGRAV_OK;
+}
+
+TreeView tree_view(const agrav_ctx *c)
+{
+  TreeView v;
+  v.n        = (int)c->n;
+  v.m        = c->m;
+  v.px       = c->d_px.p;
+  v.py       = c->d_py.p;
+  v.pz       = c->d_pz.p;
+  v.pm       = c->d_pm.p;
+  v.ps       = (c->soft.nsoft > 1 || c->have_stype) ? c->d_ps.p : nullptr;
+  v.pnext    = c->d_pnext.p;
+  v.prec     = c->d_prec.p;
+  v.sx       = c->d_sx.p;
+  v.sy       = c->d_sy.p;
+  v.sz       = c->d_sz.p;
+  v.nm       = c->d_nm.p;
+  v.cx       = c->d_cx.p;
+  v.cy       = c->d_cy.p;
+  v.cz       = c->d_cz.p;
+  v.len      = c->d_len.p;
+  v.nextnode = c->d_nextnode.p;
+  v.sibling  = c->d_sibling.p;
+  v.maxsoft  = c->d_maxsoft.p;
+  v.recA     = c->d_recA.p;
+  v.recB     = c->d_recB.p;
+  v.cbeg     = c->d_cbeg.p;
+  v.ccnt     = c->d_ccnt.p;
+  v.child    = c->d_child.p;
+  return v;
+}
+
+}  // namespace agrav

@@ -1,0 +1,160 @@
+// agrav_ctx.h -- the context behind the C ABI: parameters, the HIP stream,
+// and every device buffer of one gravity solver instance.  Buffers grow on
+// demand and are kept between calls, so a context that is reused (the serving
+// loop) keeps its tree and meshes resident in HBM.
+#pragma once
+
+#include <rocfft/rocfft.h>
+
+#include <vector>
+
+#include "agrav_internal.h"
+
+namespace agrav
+{
+
+template <class T> struct DBuf
+{
+  T *p       = nullptr;
+  size_t cap = 0;  // elements
+  agrav_status ensure(size_t count)
+  {
+    if(count <= cap && p)
+      return AGRAV_OK;
+    if(p)
+      (void)hipFree(p);
+    p         = nullptr;
+    cap       = 0;
+    size_t nb = (count ? count : 1) * sizeof(T);
+    if(hipMalloc((void **)&p, nb) != hipSuccess)
+      {
+        (void)hipGetLastError();
+        p = nullptr;
+        set_error_msg("hipMalloc failed");
+        return AGRAV_ERR_OOM;
+      }
+    cap = count ? count : 1;
+    return AGRAV_OK;
+  }
+  void release()
+  {
+    if(p)
+      (void)hipFree(p);
+    p   = nullptr;
+    cap = 0;
+  }
+};
+
+struct PmState
+{
+  int grid = 0;
+  int prec = -1;  // agrav_precision of the plans
+  rocfft_plan fwd = nullptr, bwd = nullptr;
+  rocfft_execution_info info_f = nullptr, info_b = nullptr;
+  DBuf<char> work;
+  DBuf<unsigned long long> rho_fix;  // fixed-point CIC mass grid (deterministic deposit)
+  DBuf<char> real;                   // G^3 real grid (float or double)
+  DBuf<char> cplx;                   // G*G*(G/2+1) complex grid
+  DBuf<char> force;                  // 3 x G^3 real force grids
+  DBuf<double> green;                // G*G*(G/2+1) Green's function x deconvolution (FP64)
+  double mass_scale = 0;             // fixed-point units per mass unit
+};
+
+}  // namespace agrav
+
+struct agrav_ctx
+{
+  agrav_params par{};
+  hipStream_t stream = nullptr;
+  int device         = 0;
+  std::string err;
+
+  agrav::SoftTable soft{};
+  agrav::DBuf<double> d_soft;   // ForceSoftening on device (nsoft+1)
+  float tab_force[AG_NTAB + 1];
+  float tab_pot[AG_NTAB + 1];
+  agrav::DBuf<float> d_tab;     // [force 128 | pot 128]
+  agrav::DBuf<double> d_ewald;  // tree-walk Ewald table 4 x 65^3 (FP64)
+  agrav::DBuf<float> d_ewald_f; // same in FP32 for the FP32 walk
+  agrav::DBuf<double> d_ewald_ft; // FORCETEST table 4 x 129^3
+  bool ewald_ready = false, ewald_ft_ready = false;
+
+  // particles in caller order
+  int64_t n = 0;
+  agrav::DBuf<double> d_pos_in, d_mass_in;
+  agrav::DBuf<uint8_t> d_stype_in;
+  agrav::DBuf<float> d_oldacc;
+  bool have_stype = false, have_oldacc = false;
+  double mass_total = 0;
+
+  // keys and sorted particles
+  agrav::DBuf<uint64_t> d_key, d_key2;
+  agrav::DBuf<int> d_idx, d_idx2;
+  agrav::DBuf<double> d_px, d_py, d_pz, d_pm;
+  agrav::DBuf<uint8_t> d_ps;
+  agrav::DBuf<int> d_pnext;
+  agrav::DBuf<uint4> d_prec;
+  agrav::DBuf<float> d_oldacc_s;
+
+  // binary radix tree and octree construction temporaries
+  agrav::DBuf<int> d_bfirst, d_blast, d_bprefix, d_bparent, d_bcount, d_bbase;
+  agrav::DBuf<unsigned> d_lvlmask;
+  agrav::DBuf<int> d_pcnt, d_pbase;
+  agrav::DBuf<int> d_tfirst, d_tlast, d_tlevel;
+
+  // octree (pre-order)
+  int m = 0;
+  agrav::DBuf<int> d_nfirst, d_nlast, d_nlevel;
+  agrav::DBuf<double> d_sx, d_sy, d_sz, d_nm, d_cx, d_cy, d_cz, d_len;
+  agrav::DBuf<int> d_nextnode, d_sibling;
+  agrav::DBuf<uint8_t> d_maxsoft;
+  agrav::DBuf<uint4> d_recA, d_recB;
+  agrav::DBuf<int> d_cbeg, d_ccnt, d_child;
+  agrav::DBuf<int> d_lvlcnt, d_lvloff, d_lvlnodes;
+  int lvlcnt_h[AG_KEY_LEVELS + 2];
+
+  // walk outputs in sorted order (G=1 units) and per-target counts
+  agrav::DBuf<double> d_at, d_pt;
+  agrav::DBuf<int> d_nint;
+  agrav::DBuf<int> d_stack;       // group-walk spill/overflow bookkeeping
+  agrav::DBuf<int> d_failed;      // groups that overflowed their LDS stack
+  agrav::DBuf<int> d_counters;    // [0] group work counter, [1] failed count, [2..] flags
+  agrav::DBuf<unsigned long long> d_isum; // interaction total
+
+  // PM
+  agrav::PmState pm;
+  agrav::DBuf<double> d_acc_pm, d_pot_pm;  // caller order
+
+  // final results in caller order
+  agrav::DBuf<double> d_acc, d_pot;
+  agrav::DBuf<int> d_nint_out;
+
+  // scratch
+  agrav::DBuf<char> d_temp;
+  agrav::DBuf<agrav::Domain> d_dom;
+  agrav::Domain h_dom{};
+
+  // state
+  bool have_particles = false, have_tree = false, have_tree_force = false, have_pm = false, have_final = false;
+
+  // timing
+  hipEvent_t ev[8] = {};
+  agrav_timings tim{};
+};
+
+namespace agrav
+{
+agrav_status fail(agrav_ctx *c, agrav_status s, const std::string &m);
+agrav_status build_tree(agrav_ctx *c);
+agrav_status walk_tree(agrav_ctx *c, int use_bh);
+agrav_status pm_init(agrav_ctx *c);
+void pm_free(agrav_ctx *c);
+agrav_status pm_force(agrav_ctx *c);
+agrav_status ewald_tree_table(agrav_ctx *c);
+agrav_status ewald_forcetest_table(agrav_ctx *c);
+agrav_status forcetest(agrav_ctx *c, int ns, const int *sel, double *acc, double *acc_short, double *pot, double *pot_short);
+agrav_status direct_all(agrav_ctx *c, double *acc, double *pot);
+agrav_status finalize(agrav_ctx *c);
+WalkConst walk_const(const agrav_ctx *c, int use_bh);
+TreeView tree_view(const agrav_ctx *c);
+}  // namespace agrav

@@ -1,0 +1,154 @@
+// agrav_internal.h -- shared declarations of the HIP gravity library.
+//
+// Layout of the device state (all struct-of-arrays, particles in Hilbert order
+// after the build):
+//   particles  : sorted positions (double x/y/z), mass, softening type,
+//                caller index, and a 16-byte record {x,y,z fixed-point, mass f32}
+//                for the FP32 walk;
+//   nodes      : pre-order octree, each node covering the contiguous particle
+//                range [first, last]; double COM/center/len/mass, AREPO's
+//                nextnode/sibling threading (entries < N are particles, >= N
+//                are nodes), and two 16-byte records for the FP32 walk;
+//   children   : CSR lists of each node's child entries (group walk).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+
+#include "../../include/agrav.h"
+
+#define AG_NTAB 127
+#define AG_KEY_LEVELS 21
+#define AG_MAXSOFT AGRAV_MAX_SOFT_TYPES
+
+namespace agrav
+{
+
+struct Status
+{
+  agrav_status code = AGRAV_OK;
+  std::string msg;
+};
+
+// Device-side domain description, written by the extent kernel.
+struct Domain
+{
+  double corner[3];
+  double center[3];
+  double len;
+  double inv_len;
+  double bigfac;    // len / 2^52 (domain_box.c:196)
+  double fix_unit;  // length of one fixed-point step
+  double fix_inv;   // 1 / fix_unit
+  double fix_org[3]; // fixed-point origin (0 for periodic boxes, the corner otherwise)
+  double xmin[3], xmax[3];
+  int out_of_box;
+  int pad;
+};
+
+struct SoftTable
+{
+  double h[AG_MAXSOFT + 1]; // ForceSoftening, sentinel 0 at [nsoft]
+  int nsoft;
+};
+
+// Kernel-facing tree view.
+struct TreeView
+{
+  int n;        // particles
+  int m;        // nodes
+  // particles (sorted)
+  const double *px, *py, *pz, *pm;
+  const uint8_t *ps;      // softening types (nullptr when single type)
+  const int *pnext;       // AREPO Nextnode[] of particles
+  const uint4 *prec;      // FP32 record {fx, fy, fz, mass bits}
+  // nodes
+  const double *sx, *sy, *sz, *nm; // COM and mass
+  const double *cx, *cy, *cz, *len;
+  const int *nextnode, *sibling;
+  const uint8_t *maxsoft;
+  const uint4 *recA;      // {comfx, comfy, comfz, mass bits}
+  const uint4 *recB;      // {cenfx, cenfy, cenfz, len bits}
+  const int *cbeg;        // children CSR
+  const int *ccnt;
+  const int *child;
+};
+
+struct WalkConst
+{
+  int periodic;
+  double box, half_box;
+  int pm;             // TreePM short-range walk
+  double rcut, rcut2, asmthinv2, asmthfac;
+  double theta2;      // BH
+  double errtol;      // ErrTolForceAcc
+  int use_bh;
+  int ewald;          // periodic tree-only: Ewald correction per interaction
+  double ewald_fac;   // 2 EN / box
+  int multisoft;
+  float fix_unit;     // fixed-point step as float
+};
+
+// hip error helper
+#define AG_HIP(call)                                                                                \
+  do                                                                                                \
+    {                                                                                               \
+      hipError_t e_ = (call);                                                                       \
+      if(e_ != hipSuccess)                                                                          \
+        return ::agrav::hip_fail(e_, #call, __FILE__, __LINE__);                                    \
+    }                                                                                               \
+  while(0)
+
+agrav_status hip_fail(hipError_t e, const char *what, const char *file, int line);
+void set_error_msg(const std::string &m);
+
+// scan / sort primitives (agrav_prims.hip)
+size_t scan_temp_bytes(int64_t n);
+agrav_status exclusive_scan_i32(const int *d_in, int *d_out, int64_t n, int *d_total, void *d_temp, hipStream_t s);
+size_t sort_temp_bytes(int64_t n);
+// LSD radix sort of 64-bit keys with 32-bit values; bits [0, key_bits) are sorted.
+// Results end up in keys_out/vals_out.
+agrav_status radix_sort_u64(uint64_t *keys_in, int *vals_in, uint64_t *keys_out, int *vals_out, int64_t n, int key_bits,
+                            void *d_temp, hipStream_t s);
+
+__host__ __device__ inline uint64_t hilbert_key21(uint32_t x, uint32_t y, uint32_t z)
+{
+  // 3-D Hilbert index by Skilling's transpose method ("Programming the
+  // Hilbert curve", AIP Conf. Proc. 707, 381 (2004)): undo excess work,
+  // Gray-encode, then interleave the transposed coordinates MSB first.
+  uint32_t X[3] = {x, y, z};
+  const uint32_t M = 1u << (AG_KEY_LEVELS - 1);
+  for(uint32_t Q = M; Q > 1; Q >>= 1)
+    {
+      uint32_t P = Q - 1;
+      for(int i = 0; i < 3; i++)
+        {
+          if(X[i] & Q)
+            X[0] ^= P;
+          else
+            {
+              uint32_t t = (X[0] ^ X[i]) & P;
+              X[0] ^= t;
+              X[i] ^= t;
+            }
+        }
+    }
+  X[1] ^= X[0];
+  X[2] ^= X[1];
+  uint32_t t = 0;
+  for(uint32_t Q = M; Q > 1; Q >>= 1)
+    if(X[2] & Q)
+      t ^= Q - 1;
+  X[0] ^= t;
+  X[1] ^= t;
+  X[2] ^= t;
+  uint64_t key = 0;
+  for(int b = AG_KEY_LEVELS - 1; b >= 0; b--)
+    {
+      key = (key << 3) | ((uint64_t)((X[0] >> b) & 1) << 2) | ((uint64_t)((X[1] >> b) & 1) << 1) | (uint64_t)((X[2] >> b) & 1);
+    }
+  return key;
+}
+
+}  // namespace agrav

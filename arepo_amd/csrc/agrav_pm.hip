@@ -1,0 +1,366 @@
+// agrav_pm.hip -- periodic particle-mesh long-range force
+// (long_range_force, longrange.c:98-198; pmforce_periodic, pm_periodic.c:1572-1936).
+//
+// Steps and their reference counterparts:
+//   CIC deposit        pm_periodic.c:1049-1170  -> k_cic_deposit: 64-bit fixed-point
+//                      atomics (order-independent, so the mesh is bit-reproducible
+//                      whatever the atomic order), converted to FP32/FP64 afterwards;
+//   r2c / c2r 3-D FFT  pm_mpi_fft.c:400-496     -> rocFFT out-of-place plans;
+//   Green's function   pm_periodic.c:1626-1707  -> FP64 table built once, applied in FP64;
+//   4-point FD         pm_periodic.c:1731-1800  -> k_fd4 (all three components at once);
+//   CIC readout        pm_periodic.c:1322-1550  -> k_cic_readout (potential + 3 forces).
+// FFTs are unnormalised like FFTW; 4 pi G / V and 1/(2d) are applied as in the reference.
+#include "agrav_ctx.h"
+
+#include <math.h>
+
+namespace agrav
+{
+
+static constexpr int B = 256;
+static inline unsigned nb(int64_t n) { return (unsigned)((n + B - 1) / B); }
+
+__device__ inline double wrap_pm(double x, double L)
+{
+  if(x < 0)
+    x += L;
+  if(x >= L)
+    x -= L;
+  return x;
+}
+
+struct Cic
+{
+  int s[3], ss[3];
+  double d[3];
+};
+
+__device__ inline Cic cic_of(const double *p, int G, double L)
+{
+  // slab = (int)(x*PMGRID/L), dx = x*PMGRID/L - slab, periodic neighbour (pm_periodic.c:1051-1107)
+  Cic c;
+  const double to_slab = G / L;
+  for(int k = 0; k < 3; k++)
+    {
+      double x = wrap_pm(p[k], L);
+      c.s[k]   = (int)(to_slab * x);
+      c.d[k]   = to_slab * x - c.s[k];
+      if(c.s[k] >= G)
+        c.s[k] -= G;
+      c.ss[k] = c.s[k] + 1;
+      if(c.ss[k] >= G)
+        c.ss[k] -= G;
+    }
+  return c;
+}
+
+__global__ void k_mass_partial(const double *__restrict__ m, int64_t n, double *part)
+{
+  __shared__ double s[B];
+  double v = 0;
+  for(int64_t i = (int64_t)blockIdx.x * B + threadIdx.x; i < n; i += (int64_t)gridDim.x * B)
+    v += m[i];
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for(int o = B / 2; o > 0; o >>= 1)
+    {
+      if(threadIdx.x < o)
+        s[threadIdx.x] += s[threadIdx.x + o];
+      __syncthreads();
+    }
+  if(threadIdx.x == 0)
+    part[blockIdx.x] = s[0];
+}
+
+__global__ void k_mass_final(const double *__restrict__ part, int np, double *out)
+{
+  if(threadIdx.x == 0 && blockIdx.x == 0)
+    {
+      double v = 0;
+      for(int i = 0; i < np; i++)
+        v += part[i];
+      out[0] = v;
+      // fixed-point scale: a power of two leaving two bits of headroom below 2^63
+      int e;
+      frexp(v > 0 ? v : 1.0, &e);
+      out[1] = ldexp(1.0, 61 - e);
+    }
+}
+
+__global__ void k_cic_deposit(const double *__restrict__ pos, const double *__restrict__ mass, int64_t n, int G, double L,
+                              const double *__restrict__ scale_p, unsigned long long *rho)
+{
+  int64_t i = (int64_t)blockIdx.x * B + threadIdx.x;
+  if(i >= n)
+    return;
+  const double scale = scale_p[1];
+  Cic c              = cic_of(&pos[3 * i], G, L);
+  double m           = mass[i] * scale;
+  const double *d    = c.d;
+#define FI(x, y, z) (((size_t)(x)*G + (y)) * G + (z))
+  atomicAdd(&rho[FI(c.s[0], c.s[1], c.s[2])], (unsigned long long)llrint(m * (1.0 - d[0]) * (1.0 - d[1]) * (1.0 - d[2])));
+  atomicAdd(&rho[FI(c.s[0], c.s[1], c.ss[2])], (unsigned long long)llrint(m * (1.0 - d[0]) * (1.0 - d[1]) * (d[2])));
+  atomicAdd(&rho[FI(c.s[0], c.ss[1], c.s[2])], (unsigned long long)llrint(m * (1.0 - d[0]) * (d[1]) * (1.0 - d[2])));
+  atomicAdd(&rho[FI(c.s[0], c.ss[1], c.ss[2])], (unsigned long long)llrint(m * (1.0 - d[0]) * (d[1]) * (d[2])));
+  atomicAdd(&rho[FI(c.ss[0], c.s[1], c.s[2])], (unsigned long long)llrint(m * (d[0]) * (1.0 - d[1]) * (1.0 - d[2])));
+  atomicAdd(&rho[FI(c.ss[0], c.s[1], c.ss[2])], (unsigned long long)llrint(m * (d[0]) * (1.0 - d[1]) * (d[2])));
+  atomicAdd(&rho[FI(c.ss[0], c.ss[1], c.s[2])], (unsigned long long)llrint(m * (d[0]) * (d[1]) * (1.0 - d[2])));
+  atomicAdd(&rho[FI(c.ss[0], c.ss[1], c.ss[2])], (unsigned long long)llrint(m * (d[0]) * (d[1]) * (d[2])));
+#undef FI
+}
+
+template <typename R>
+__global__ void k_fix_to_real(const unsigned long long *__restrict__ rho, int64_t g3, const double *__restrict__ scale_p, R *out)
+{
+  int64_t i = (int64_t)blockIdx.x * B + threadIdx.x;
+  if(i < g3)
+    out[i] = (R)((double)rho[i] / scale_p[1]);
+}
+
+__global__ void k_green_table(int G, double L, double asmth, double *green)
+{
+  // pm_periodic.c:1626-1707; index = kz + (G/2+1)*(y + G*x)
+  int64_t i   = (int64_t)blockIdx.x * B + threadIdx.x;
+  const int gz = G / 2 + 1;
+  if(i >= (int64_t)G * G * gz)
+    return;
+  int z = (int)(i % gz), y = (int)((i / gz) % G), x = (int)(i / ((int64_t)gz * G));
+  double d     = L / G;
+  double dhalf = 0.5 * d;
+  double kfac  = 2.0 * M_PI / (G * d);
+  int xx = x >= G / 2 ? x - G : x, yy = y >= G / 2 ? y - G : y, zz = z >= G / 2 ? z - G : z;
+  double kx = kfac * xx, ky = kfac * yy, kz = kfac * zz;
+  double k2 = kx * kx + ky * ky + kz * kz;
+  double v  = 0;
+  if(k2 > 0)
+    {
+      double smth = -exp(-k2 * asmth * asmth) / k2;
+      double fx = 1, fy = 1, fz = 1;
+      if(xx != 0)
+        {
+          fx = kx * dhalf;
+          fx = sin(fx) / fx;
+        }
+      if(yy != 0)
+        {
+          fy = ky * dhalf;
+          fy = sin(fy) / fy;
+        }
+      if(zz != 0)
+        {
+          fz = kz * dhalf;
+          fz = sin(fz) / fz;
+        }
+      double ff = 1 / (fx * fy * fz);
+      v         = smth * ff * ff * ff * ff;
+    }
+  green[i] = v;  // k = 0 -> 0 (pm_periodic.c:1706-1707)
+}
+
+template <typename R> __global__ void k_green_apply(R *cplx, const double *__restrict__ green, int64_t nc)
+{
+  int64_t i = (int64_t)blockIdx.x * B + threadIdx.x;
+  if(i >= nc)
+    return;
+  double g        = green[i];
+  cplx[2 * i]     = (R)((double)cplx[2 * i] * g);
+  cplx[2 * i + 1] = (R)((double)cplx[2 * i + 1] * g);
+}
+
+template <typename R> __global__ void k_fd4(const R *__restrict__ phi, int G, double fac, R *force)
+{
+  // F = fac/(2d) [ 4/3 (phi_{-1} - phi_{+1}) - 1/6 (phi_{-2} - phi_{+2}) ]   (fac includes 1/(2d))
+  int64_t i  = (int64_t)blockIdx.x * B + threadIdx.x;
+  int64_t g3 = (int64_t)G * G * G;
+  if(i >= g3)
+    return;
+  int z = (int)(i % G), y = (int)((i / G) % G), x = (int)(i / ((int64_t)G * G));
+  auto at = [&](int a, int b, int c) -> double { return (double)phi[((int64_t)a * G + b) * G + c]; };
+  int xl = (x - 1 + G) % G, xr = (x + 1) % G, xll = (x - 2 + G) % G, xrr = (x + 2) % G;
+  int yl = (y - 1 + G) % G, yr = (y + 1) % G, yll = (y - 2 + G) % G, yrr = (y + 2) % G;
+  int zl = (z - 1 + G) % G, zr = (z + 1) % G, zll = (z - 2 + G) % G, zrr = (z + 2) % G;
+  force[i]          = (R)(fac * ((4.0 / 3) * (at(xl, y, z) - at(xr, y, z)) - (1.0 / 6) * (at(xll, y, z) - at(xrr, y, z))));
+  force[g3 + i]     = (R)(fac * ((4.0 / 3) * (at(x, yl, z) - at(x, yr, z)) - (1.0 / 6) * (at(x, yll, z) - at(x, yrr, z))));
+  force[2 * g3 + i] = (R)(fac * ((4.0 / 3) * (at(x, y, zl) - at(x, y, zr)) - (1.0 / 6) * (at(x, y, zll) - at(x, y, zrr))));
+}
+
+template <typename R>
+__global__ void k_cic_readout(const double *__restrict__ pos, int64_t n, int G, double L, const R *__restrict__ phi,
+                              const R *__restrict__ force, double fac_pot, double *acc_pm, double *pot_pm)
+{
+  int64_t i = (int64_t)blockIdx.x * B + threadIdx.x;
+  if(i >= n)
+    return;
+  Cic c           = cic_of(&pos[3 * i], G, L);
+  const double *d = c.d;
+  int64_t g3      = (int64_t)G * G * G;
+  int64_t idx[8]  = {((int64_t)c.s[0] * G + c.s[1]) * G + c.s[2],   ((int64_t)c.s[0] * G + c.s[1]) * G + c.ss[2],
+                     ((int64_t)c.s[0] * G + c.ss[1]) * G + c.s[2],  ((int64_t)c.s[0] * G + c.ss[1]) * G + c.ss[2],
+                     ((int64_t)c.ss[0] * G + c.s[1]) * G + c.s[2],  ((int64_t)c.ss[0] * G + c.s[1]) * G + c.ss[2],
+                     ((int64_t)c.ss[0] * G + c.ss[1]) * G + c.s[2], ((int64_t)c.ss[0] * G + c.ss[1]) * G + c.ss[2]};
+  double w[8] = {(1.0 - d[0]) * (1.0 - d[1]) * (1.0 - d[2]), (1.0 - d[0]) * (1.0 - d[1]) * (d[2]),
+                 (1.0 - d[0]) * (d[1]) * (1.0 - d[2]),       (1.0 - d[0]) * (d[1]) * (d[2]),
+                 (d[0]) * (1.0 - d[1]) * (1.0 - d[2]),       (d[0]) * (1.0 - d[1]) * (d[2]),
+                 (d[0]) * (d[1]) * (1.0 - d[2]),             (d[0]) * (d[1]) * (d[2])};
+  double p = 0, f0 = 0, f1 = 0, f2 = 0;
+#pragma unroll
+  for(int q = 0; q < 8; q++)
+    {
+      p += (double)phi[idx[q]] * w[q];
+      f0 += (double)force[idx[q]] * w[q];
+      f1 += (double)force[g3 + idx[q]] * w[q];
+      f2 += (double)force[2 * g3 + idx[q]] * w[q];
+    }
+  pot_pm[i]         = p * fac_pot;
+  acc_pm[3 * i + 0] = f0;
+  acc_pm[3 * i + 1] = f1;
+  acc_pm[3 * i + 2] = f2;
+}
+
+static agrav_status fft_check(agrav_ctx *c, rocfft_status s, const char *what)
+{
+  if(s != rocfft_status_success)
+    return fail(c, AGRAV_ERR_FFT, std::string("rocFFT: ") + what);
+  return AGRAV_OK;
+}
+
+void pm_free(agrav_ctx *c)
+{
+  PmState &p = c->pm;
+  if(p.fwd)
+    rocfft_plan_destroy(p.fwd);
+  if(p.bwd)
+    rocfft_plan_destroy(p.bwd);
+  if(p.info_f)
+    rocfft_execution_info_destroy(p.info_f);
+  if(p.info_b)
+    rocfft_execution_info_destroy(p.info_b);
+  p.fwd = p.bwd = nullptr;
+  p.info_f = p.info_b = nullptr;
+  p.work.release();
+  p.rho_fix.release();
+  p.real.release();
+  p.cplx.release();
+  p.force.release();
+  p.green.release();
+  p.grid = 0;
+}
+
+agrav_status pm_init(agrav_ctx *c)
+{
+  // pm_init_periodic, pm_periodic.c:173-255
+  const int G = c->par.pmgrid;
+  if(G <= 0)
+    return AGRAV_OK;
+  PmState &p = c->pm;
+  if(p.grid == G && p.prec == c->par.precision)
+    return AGRAV_OK;
+  pm_free(c);
+  static bool setup_done = false;
+  if(!setup_done)
+    {
+      if(rocfft_setup() != rocfft_status_success)
+        return fail(c, AGRAV_ERR_FFT, "rocfft_setup failed");
+      setup_done = true;
+    }
+  const bool sp       = c->par.precision == AGRAV_FP32;
+  const size_t rsz    = sp ? sizeof(float) : sizeof(double);
+  const int64_t g3    = (int64_t)G * G * G;
+  const int64_t nc    = (int64_t)G * G * (G / 2 + 1);
+  const size_t len[3] = {(size_t)G, (size_t)G, (size_t)G};
+  rocfft_precision pr = sp ? rocfft_precision_single : rocfft_precision_double;
+  agrav_status st;
+  if((st = fft_check(c, rocfft_plan_create(&p.fwd, rocfft_placement_notinplace, rocfft_transform_type_real_forward, pr, 3, len, 1,
+                                           nullptr),
+                     "plan forward")))
+    return st;
+  if((st = fft_check(c, rocfft_plan_create(&p.bwd, rocfft_placement_notinplace, rocfft_transform_type_real_inverse, pr, 3, len, 1,
+                                           nullptr),
+                     "plan inverse")))
+    return st;
+  size_t w1 = 0, w2 = 0;
+  rocfft_plan_get_work_buffer_size(p.fwd, &w1);
+  rocfft_plan_get_work_buffer_size(p.bwd, &w2);
+  size_t w = w1 > w2 ? w1 : w2;
+  if((st = p.work.ensure(w ? w : 1)) || (st = p.rho_fix.ensure(g3)) || (st = p.real.ensure(g3 * rsz)) ||
+     (st = p.cplx.ensure(nc * 2 * rsz)) || (st = p.force.ensure(3 * g3 * rsz)) || (st = p.green.ensure(nc)))
+    return st;
+  if((st = fft_check(c, rocfft_execution_info_create(&p.info_f), "info")) ||
+     (st = fft_check(c, rocfft_execution_info_create(&p.info_b), "info")))
+    return st;
+  if(w)
+    {
+      rocfft_execution_info_set_work_buffer(p.info_f, p.work.p, w);
+      rocfft_execution_info_set_work_buffer(p.info_b, p.work.p, w);
+    }
+  rocfft_execution_info_set_stream(p.info_f, c->stream);
+  rocfft_execution_info_set_stream(p.info_b, c->stream);
+  double asmth = c->par.asmth * c->par.box_size / G;  // pm_periodic.c:190
+  k_green_table<<<nb(nc), B, 0, c->stream>>>(G, c->par.box_size, asmth, p.green.p);
+  AG_HIP(hipGetLastError());
+  p.grid = G;
+  p.prec = c->par.precision;
+  return AGRAV_OK;
+}
+
+template <typename R> static agrav_status pm_run(agrav_ctx *c)
+{
+  PmState &p      = c->pm;
+  const int G     = p.grid;
+  const double L  = c->par.box_size;
+  const int64_t n = c->n;
+  const int64_t g3 = (int64_t)G * G * G;
+  const int64_t nc = (int64_t)G * G * (G / 2 + 1);
+  hipStream_t s   = c->stream;
+  agrav_status st;
+  if((st = c->d_acc_pm.ensure(3 * (n ? n : 1))) || (st = c->d_pot_pm.ensure(n ? n : 1)) || (st = c->d_temp.ensure(4096 * 8)))
+    return st;
+  double *part = (double *)c->d_temp.p;  // 256 partials, then [total, scale] at part+512
+  k_mass_partial<<<256, B, 0, s>>>(c->d_mass_in.p, n, part);
+  k_mass_final<<<1, 1, 0, s>>>(part, 256, part + 512);
+  AG_HIP(hipMemsetAsync(p.rho_fix.p, 0, g3 * sizeof(unsigned long long), s));
+  if(n > 0)
+    k_cic_deposit<<<nb(n), B, 0, s>>>(c->d_pos_in.p, c->d_mass_in.p, n, G, L, part + 512, p.rho_fix.p);
+  k_fix_to_real<R><<<nb(g3), B, 0, s>>>(p.rho_fix.p, g3, part + 512, (R *)p.real.p);
+  AG_HIP(hipGetLastError());
+  void *in[1]  = {p.real.p};
+  void *out[1] = {p.cplx.p};
+  if((st = fft_check(c, rocfft_execute(p.fwd, in, out, p.info_f), "execute forward")))
+    return st;
+  k_green_apply<R><<<nb(nc), B, 0, s>>>((R *)p.cplx.p, p.green.p, nc);
+  AG_HIP(hipGetLastError());
+  void *in2[1]  = {p.cplx.p};
+  void *out2[1] = {p.real.p};
+  if((st = fft_check(c, rocfft_execute(p.bwd, in2, out2, p.info_b), "execute inverse")))
+    return st;
+  const double fac_pot = 4 * M_PI * c->par.G / (L * L * L);  // pm_periodic.c:1590
+  const double fac_fd  = fac_pot / (2 * (L / G));            // :1592
+  k_fd4<R><<<nb(g3), B, 0, s>>>((const R *)p.real.p, G, fac_fd, (R *)p.force.p);
+  if(n > 0)
+    k_cic_readout<R><<<nb(n), B, 0, s>>>(c->d_pos_in.p, n, G, L, (const R *)p.real.p, (const R *)p.force.p, fac_pot,
+                                         c->d_acc_pm.p, c->d_pot_pm.p);
+  AG_HIP(hipGetLastError());
+  return AGRAV_OK;
+}
+
+agrav_status pm_force(agrav_ctx *c)
+{
+  if(c->par.pmgrid <= 0)
+    return AGRAV_OK;
+  agrav_status st = pm_init(c);
+  if(st != AGRAV_OK)
+    return st;
+  AG_HIP(hipEventRecord(c->ev[5], c->stream));
+  st = (c->par.precision == AGRAV_FP32) ? pm_run<float>(c) : pm_run<double>(c);
+  if(st != AGRAV_OK)
+    return st;
+  AG_HIP(hipEventRecord(c->ev[6], c->stream));
+  AG_HIP(hipEventSynchronize(c->ev[6]));
+  float ms = 0;
+  AG_HIP(hipEventElapsedTime(&ms, c->ev[5], c->ev[6]));
+  c->tim.pm_grav = ms;
+  c->have_pm     = true;
+  return AGRAV_OK;
+}
+
+}  // namespace agrav

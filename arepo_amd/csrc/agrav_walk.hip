@@ -1,0 +1,790 @@
+// agrav_walk.hip -- the tree walk: gravity_tree (gravtree.c:357-707) with
+// force_treeevaluate (forcetree_walk.c:115-514).
+//
+// Two kernels:
+//  * k_walk_plain: one lane per target, AREPO's stackless nextnode/sibling
+//    walk with the per-particle opening test, FP64 throughout.  This is the
+//    plain GPU restatement of the reference loop (same interaction set).
+//  * k_walk_group: one 64-lane wavefront per group of <= 64 consecutive
+//    targets in Peano-Hilbert order.  The wavefront evaluates up to 64 nodes
+//    at a time against the group's bounding box with conservative versions of
+//    AREPO's tests (a node is accepted only if every target would accept it),
+//    keeps the pending nodes in an LDS stack, and stages accepted nodes and
+//    particles as float4 tiles in LDS that every lane (= target) consumes by
+//    broadcast reads.  FP32 variant: offsets are formed in fixed point
+//    relative to the group centre and converted to float, forces are summed
+//    per tile and the tile sums are added with compensated (Kahan) sums.
+#include "agrav_ctx.h"
+
+namespace agrav
+{
+
+// spline-softening coefficients (allvars.h:557-573)
+#define SF1 10.666666666667
+#define SF2 32.0
+#define SF3 (-38.4)
+#define SF4 (-2.8)
+#define SF5 5.333333333333
+#define SF6 6.4
+#define SF7 (-9.6)
+#define SF8 21.333333333333
+#define SF9 (-48.0)
+#define SF10 38.4
+#define SF11 (-10.666666666667)
+#define SF12 (-0.066666666667)
+#define SF13 (-3.2)
+#define SF14 0.066666666667
+#define SF15 (-16.0)
+#define SF16 9.6
+#define SF17 (-2.133333333333)
+
+__device__ inline double nearest_d(double x, const WalkConst &wc)
+{
+  if(!wc.periodic)
+    return x;
+  return x > wc.half_box ? x - wc.box : (x < -wc.half_box ? x + wc.box : x);
+}
+
+// pair kernel of forcetree_walk.c:364-487 in type T; returns false beyond the table
+template <typename T, bool PM>
+__device__ inline bool pair_kernel(T r, T hmax, const float *__restrict__ s_tab, T asmthfac, T asmthinv2, T &fac, T &wp)
+{
+  T ff = 0, fp = 0;
+  if(PM)
+    {
+      T tabentry = asmthfac * r;
+      int ti     = (int)tabentry;
+      if(ti >= AG_NTAB)
+        return false;
+      T w = tabentry - (T)ti;
+      ff  = ((T)1 - w) * (T)s_tab[ti] + w * (T)s_tab[ti + 1];
+      fp  = ((T)1 - w) * (T)s_tab[128 + ti] + w * (T)s_tab[128 + ti + 1];
+    }
+  if(r >= hmax)
+    {
+      T rinv  = (T)1 / r;
+      T rinv3 = rinv * rinv * rinv;
+      if(PM)
+        {
+          fac = rinv3 + rinv * ff * asmthinv2;
+          wp  = rinv * fp;
+        }
+      else
+        {
+          fac = rinv3;
+          wp  = -rinv;
+        }
+    }
+  else
+    {
+      T h_inv  = (T)1 / hmax;
+      T h3_inv = h_inv * h_inv * h_inv;
+      T u      = r * h_inv;
+      if(u < (T)0.5)
+        {
+          T u2 = u * u;
+          fac  = h3_inv * ((T)SF1 + u2 * ((T)SF2 * u + (T)SF3));
+          wp   = h_inv * ((T)SF4 + u2 * ((T)SF5 + u2 * ((T)SF6 * u + (T)SF7)));
+        }
+      else
+        {
+          T u2 = u * u, u3 = u2 * u;
+          fac  = h3_inv * ((T)SF8 + (T)SF9 * u + (T)SF10 * u2 + (T)SF11 * u3 + (T)SF12 / u3);
+          wp   = h_inv * ((T)SF13 + (T)SF14 / u + u2 * ((T)SF1 + u * ((T)SF15 + u * ((T)SF16 + (T)SF17 * u))));
+        }
+      if(PM && r > (T)0)
+        {
+          T rinv = (T)1 / r;
+          fac += rinv * ff * asmthinv2;
+          wp += rinv * (fp + (T)1);
+        }
+    }
+  return true;
+}
+
+// Ewald correction (forcetree_ewald.c:268-382), table [4][65][65][65]
+template <typename T, typename TT>
+__device__ inline void ewald_add(const TT *__restrict__ tab, T fac_intp, T dx, T dy, T dz, T m, T &ax, T &ay, T &az, T &pot)
+{
+  const int EN = 64, n1 = 65, size = n1 * n1 * n1;
+  T sx = dx < 0 ? (T)1 : (T)-1, sy = dy < 0 ? (T)1 : (T)-1, sz = dz < 0 ? (T)1 : (T)-1;
+  T u = fabs(dx) * fac_intp, v = fabs(dy) * fac_intp, w = fabs(dz) * fac_intp;
+  int i = (int)u, j = (int)v, k = (int)w;
+  i = i >= EN ? EN - 1 : i;
+  j = j >= EN ? EN - 1 : j;
+  k = k >= EN ? EN - 1 : k;
+  u -= i;
+  v -= j;
+  w -= k;
+  T f[8] = {(1 - u) * (1 - v) * (1 - w), (1 - u) * (1 - v) * w, (1 - u) * v * (1 - w), (1 - u) * v * w,
+            u * (1 - v) * (1 - w),       u * (1 - v) * w,       u * v * (1 - w),       u * v * w};
+  int o[8] = {(i * n1 + j) * n1 + k,           (i * n1 + j) * n1 + k + 1,           (i * n1 + j + 1) * n1 + k,
+              (i * n1 + j + 1) * n1 + k + 1,   ((i + 1) * n1 + j) * n1 + k,         ((i + 1) * n1 + j) * n1 + k + 1,
+              ((i + 1) * n1 + j + 1) * n1 + k, ((i + 1) * n1 + j + 1) * n1 + k + 1};
+  T c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+#pragma unroll
+  for(int q = 0; q < 8; q++)
+    {
+      c0 += (T)tab[o[q]] * f[q];
+      c1 += (T)tab[size + o[q]] * f[q];
+      c2 += (T)tab[2 * size + o[q]] * f[q];
+      c3 += (T)tab[3 * size + o[q]] * f[q];
+    }
+  ax += m * sx * c0;
+  ay += m * sy * c1;
+  az += m * sz * c2;
+  pot += m * c3;
+}
+
+// ---------------------------------------------------------------------------
+// plain walk: one lane per target (forcetree_walk.c:115-514)
+// ---------------------------------------------------------------------------
+
+template <bool PM, bool EWALD>
+__global__ __launch_bounds__(256) void k_walk_plain(TreeView tv, WalkConst wc, const double *__restrict__ hsoft,
+                                                    const float *__restrict__ oldacc_s, const float *__restrict__ tabs,
+                                                    const double *__restrict__ ewald, int ntarget, const int *__restrict__ tlist,
+                                                    int tl_group, double *at, double *pt, int *nint, unsigned long long *isum)
+{
+  __shared__ float s_tab[256];
+  s_tab[threadIdx.x] = tabs[threadIdx.x];
+  __syncthreads();
+  int t = blockIdx.x * 256 + threadIdx.x;
+  int i = -1;
+  if(t < ntarget)
+    {
+      if(tlist)
+        {
+          // targets of failed groups: tl_group particles per group id
+          int g = tlist[t / tl_group];
+          i     = g * tl_group + (t % tl_group);
+          if(i >= tv.n)
+            i = -1;
+        }
+      else
+        i = t;
+    }
+  int nin = 0;
+  if(i >= 0)
+    {
+      const int N       = tv.n;
+      const double posx = tv.px[i], posy = tv.py[i], posz = tv.pz[i];
+      const double h_i  = hsoft[tv.ps ? tv.ps[i] : 0];
+      const double aold = wc.errtol * (double)oldacc_s[i];
+      double ax = 0, ay = 0, az = 0, pot = 0;
+      int no = N;
+      while(no >= 0)
+        {
+          double dx, dy, dz, r2, mass, hmax;
+          if(no < N)
+            {
+              dx         = nearest_d(tv.px[no] - posx, wc);
+              dy         = nearest_d(tv.py[no] - posy, wc);
+              dz         = nearest_d(tv.pz[no] - posz, wc);
+              r2         = dx * dx + dy * dy + dz * dz;
+              mass       = tv.pm[no];
+              double h_j = hsoft[tv.ps ? tv.ps[no] : 0];
+              hmax       = h_j > h_i ? h_j : h_i;
+              no         = tv.pnext[no];
+            }
+          else
+            {
+              const int k = no - N;
+              mass        = tv.nm[k];
+              dx          = nearest_d(tv.sx[k] - posx, wc);
+              dy          = nearest_d(tv.sy[k] - posy, wc);
+              dz          = nearest_d(tv.sz[k] - posz, wc);
+              r2          = dx * dx + dy * dy + dz * dz;
+              double ln   = tv.len[k];
+              if(PM && r2 > wc.rcut2)
+                {
+                  double eff = wc.rcut + 0.5 * ln;
+                  double d0  = nearest_d(tv.cx[k] - posx, wc);
+                  double d1  = nearest_d(tv.cy[k] - posy, wc);
+                  double d2  = nearest_d(tv.cz[k] - posz, wc);
+                  if(d0 < -eff || d0 > eff || d1 < -eff || d1 > eff || d2 < -eff || d2 > eff)
+                    {
+                      no = tv.sibling[k];
+                      continue;
+                    }
+                }
+              if(wc.use_bh)
+                {
+                  if(ln * ln > r2 * wc.theta2)
+                    {
+                      no = tv.nextnode[k];
+                      continue;
+                    }
+                }
+              else
+                {
+                  double len2 = ln * ln;
+                  if(len2 > r2 * (1.2 * 1.2))
+                    {
+                      no = tv.nextnode[k];
+                      continue;
+                    }
+                  if(mass * len2 > r2 * r2 * aold)
+                    {
+                      no = tv.nextnode[k];
+                      continue;
+                    }
+                  if(fabs(nearest_d(tv.cx[k] - posx, wc)) < 0.60 * ln && fabs(nearest_d(tv.cy[k] - posy, wc)) < 0.60 * ln &&
+                     fabs(nearest_d(tv.cz[k] - posz, wc)) < 0.60 * ln)
+                    {
+                      no = tv.nextnode[k];
+                      continue;
+                    }
+                }
+              double h_j = hsoft[tv.maxsoft[k]];
+              if(h_j > h_i)
+                {
+                  if(r2 < h_j * h_j)
+                    {
+                      no = tv.nextnode[k];
+                      continue;
+                    }
+                  hmax = h_j;
+                }
+              else
+                hmax = h_i;
+              no = tv.sibling[k];
+            }
+          if(mass != 0)
+            {
+              double r = sqrt(r2), fac, wp;
+              if(!pair_kernel<double, PM>(r, hmax, s_tab, wc.asmthfac, wc.asmthinv2, fac, wp))
+                continue;
+              pot += mass * wp;
+              fac *= mass;
+              ax += dx * fac;
+              ay += dy * fac;
+              az += dz * fac;
+              if(EWALD)
+                ewald_add<double, double>(ewald, wc.ewald_fac, dx, dy, dz, mass, ax, ay, az, pot);
+              nin++;
+            }
+        }
+      at[3 * (int64_t)i + 0] = ax;
+      at[3 * (int64_t)i + 1] = ay;
+      at[3 * (int64_t)i + 2] = az;
+      pt[i]                  = pot;
+      nint[i]                = nin;
+    }
+  // interaction total
+  unsigned long long v = (unsigned long long)nin;
+  for(int off = 32; off > 0; off >>= 1)
+    v += __shfl_xor(v, off, 64);
+  if((threadIdx.x & 63) == 0 && v)
+    atomicAdd(isum, v);
+}
+
+// ---------------------------------------------------------------------------
+// group walk
+// ---------------------------------------------------------------------------
+
+static constexpr int kStackCap = 2048;
+static constexpr int kListCap  = 256;
+
+__device__ inline int wave_incl_scan_i(int v)
+{
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for(int d = 1; d < 64; d <<= 1)
+    {
+      int t = __shfl_up(v, d, 64);
+      if(lane >= d)
+        v += t;
+    }
+  return v;
+}
+template <typename T> __device__ inline T wave_min(T v)
+{
+#pragma unroll
+  for(int off = 32; off > 0; off >>= 1)
+    {
+      T o = __shfl_xor(v, off, 64);
+      v   = o < v ? o : v;
+    }
+  return v;
+}
+template <typename T> __device__ inline T wave_max(T v)
+{
+#pragma unroll
+  for(int off = 32; off > 0; off >>= 1)
+    {
+      T o = __shfl_xor(v, off, 64);
+      v   = o > v ? o : v;
+    }
+  return v;
+}
+
+// Kahan-compensated accumulation of a tile sum into a running total
+template <typename T> __device__ inline void kahan_add(T &sum, T &comp, T x)
+{
+  T y = x - comp;
+  T t = sum + y;
+  comp = (t - sum) - y;
+  sum  = t;
+}
+
+template <typename T> struct Vec4
+{
+  T x, y, z, w;
+};
+
+template <typename T, bool PM, bool EWALD, bool MSOFT>
+__device__ inline void tile_compute(const Vec4<T> *__restrict__ s_list, const T *__restrict__ s_h, int nl, T tx, T ty, T tz, T hi,
+                                    const float *__restrict__ s_tab, const WalkConst &wc, T asmthfac, T asmthinv2,
+                                    const float *__restrict__ ewald_f, const double *__restrict__ ewald_d, T &ax, T &ay, T &az, T &pp,
+                                    T &cax, T &cay, T &caz, T &cpp, int &nin)
+{
+  T fx = 0, fy = 0, fz = 0, fp = 0;
+  for(int k = 0; k < nl; k++)
+    {
+      Vec4<T> e = s_list[k];
+      T dx = e.x - tx, dy = e.y - ty, dz = e.z - tz;
+      T r2   = dx * dx + dy * dy + dz * dz;
+      T hmax = hi;
+      if(MSOFT)
+        hmax = s_h[k] > hi ? s_h[k] : hi;
+      T r = sqrt(r2);
+      T fac, wp;
+      if(e.w == (T)0 || !pair_kernel<T, PM>(r, hmax, s_tab, asmthfac, asmthinv2, fac, wp))
+        continue;
+      fp += e.w * wp;
+      fac *= e.w;
+      fx += dx * fac;
+      fy += dy * fac;
+      fz += dz * fac;
+      if(EWALD)
+        {
+          if constexpr(sizeof(T) == 4)
+            ewald_add<T, float>(ewald_f, (T)wc.ewald_fac, dx, dy, dz, e.w, fx, fy, fz, fp);
+          else
+            ewald_add<T, double>(ewald_d, (T)wc.ewald_fac, dx, dy, dz, e.w, fx, fy, fz, fp);
+        }
+      nin++;
+    }
+  // tile sums join the running totals through compensated additions
+  kahan_add(ax, cax, fx);
+  kahan_add(ay, cay, fy);
+  kahan_add(az, caz, fz);
+  kahan_add(pp, cpp, fp);
+}
+
+template <typename T, bool PM, bool EWALD, bool MSOFT>
+__global__ __launch_bounds__(64) void k_walk_group(TreeView tv, WalkConst wc, const double *__restrict__ hsoft,
+                                                   const float *__restrict__ oldacc_s, const float *__restrict__ tabs,
+                                                   const float *__restrict__ ewald_f, const double *__restrict__ ewald_d, int gs,
+                                                   double *at, double *pt, int *nint, int *failed, int *nfailed,
+                                                   unsigned long long *isum)
+{
+  __shared__ int s_stack[kStackCap];
+  __shared__ Vec4<T> s_list[kListCap];
+  __shared__ T s_h[MSOFT ? kListCap : 1];
+  __shared__ float s_tab[256];
+
+  const int lane   = threadIdx.x;
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for(int q = lane; q < 256; q += 64)
+    s_tab[q] = tabs[q];
+
+  const int N  = tv.n;
+  const int g  = blockIdx.x;
+  const int p0 = g * gs;
+  const int np = min(gs, N - p0);
+  const bool has = lane < np;
+  const int pi   = p0 + (has ? lane : 0);
+
+  // ---- targets relative to the group centre
+  T tx, ty, tz, hx, hy, hz;
+  uint32_t cfx = 0, cfy = 0, cfz = 0;  // fixed-point group centre (FP32 path)
+  double c0 = 0, c1 = 0, c2 = 0;       // FP64 group centre
+  const T unit = (T)wc.fix_unit;
+  if constexpr(sizeof(T) == 4)
+    {
+      uint4 tr  = tv.prec[pi];
+      uint32_t rx = __shfl(tr.x, 0, 64), ry = __shfl(tr.y, 0, 64), rz = __shfl(tr.z, 0, 64);
+      int ox = (int)(tr.x - rx), oy = (int)(tr.y - ry), oz = (int)(tr.z - rz);
+      int64_t mnx = wave_min((int64_t)ox), mxx = wave_max((int64_t)ox);
+      int64_t mny = wave_min((int64_t)oy), mxy = wave_max((int64_t)oy);
+      int64_t mnz = wave_min((int64_t)oz), mxz = wave_max((int64_t)oz);
+      cfx = rx + (uint32_t)(int32_t)((mnx + mxx) >> 1);
+      cfy = ry + (uint32_t)(int32_t)((mny + mxy) >> 1);
+      cfz = rz + (uint32_t)(int32_t)((mnz + mxz) >> 1);
+      tx  = (T)(int)(tr.x - cfx) * unit;
+      ty  = (T)(int)(tr.y - cfy) * unit;
+      tz  = (T)(int)(tr.z - cfz) * unit;
+    }
+  else
+    {
+      double rx = __shfl(tv.px[pi], 0, 64), ry = __shfl(tv.py[pi], 0, 64), rz = __shfl(tv.pz[pi], 0, 64);
+      double ox = nearest_d(tv.px[pi] - rx, wc), oy = nearest_d(tv.py[pi] - ry, wc), oz = nearest_d(tv.pz[pi] - rz, wc);
+      c0 = rx + 0.5 * (wave_min(ox) + wave_max(ox));
+      c1 = ry + 0.5 * (wave_min(oy) + wave_max(oy));
+      c2 = rz + 0.5 * (wave_min(oz) + wave_max(oz));
+      tx = (T)nearest_d(tv.px[pi] - c0, wc);
+      ty = (T)nearest_d(tv.py[pi] - c1, wc);
+      tz = (T)nearest_d(tv.pz[pi] - c2, wc);
+    }
+  // half-extent of the target box, padded by one fixed-point step and a relative epsilon
+  const T pad = sizeof(T) == 4 ? (T)1e-6 : (T)1e-13;
+  hx = wave_max((T)fabs(tx)) * ((T)1 + pad) + unit;
+  hy = wave_max((T)fabs(ty)) * ((T)1 + pad) + unit;
+  hz = wave_max((T)fabs(tz)) * ((T)1 + pad) + unit;
+  const T hi     = (T)hsoft[tv.ps ? tv.ps[pi] : 0];
+  const T hi_min = wave_min(hi);
+  T aold_min     = 0;
+  if(!wc.use_bh)
+    aold_min = (T)wc.errtol * wave_min((T)oldacc_s[pi]);
+
+  const T rcut = (T)wc.rcut, rcut2 = (T)wc.rcut2, theta2 = (T)wc.theta2;
+  const T asmthfac = (T)wc.asmthfac, asmthinv2 = (T)wc.asmthinv2;
+
+  // ---- walk
+  int sp = 1;
+  if(lane == 0)
+    s_stack[0] = N;  // root entry
+  int nl = 0;
+  T ax = 0, ay = 0, az = 0, pp = 0, cax = 0, cay = 0, caz = 0, cpp = 0;
+  int nin        = 0;
+  bool overflow  = false;
+  __syncthreads();
+
+  while(sp > 0)
+    {
+      const int take = sp < 64 ? sp : 64;
+      const int base = sp - take;
+      const int e    = lane < take ? s_stack[base + lane] : -1;
+      sp             = base;
+      __builtin_amdgcn_wave_barrier();
+
+      int act = 0;  // 1 accept node, 2 open node, 3 particle
+      int j   = 0;
+      Vec4<T> item{0, 0, 0, 0};
+      T hj = 0;
+      if(e >= N)
+        {
+          j = e - N;
+          T m, ln, dcx, dcy, dcz, sxr, syr, szr;
+          if constexpr(sizeof(T) == 4)
+            {
+              uint4 A = tv.recA[j], Bq = tv.recB[j];
+              m       = __uint_as_float(A.w);
+              ln      = __uint_as_float(Bq.w);
+              dcx     = (T)(int)(Bq.x - cfx) * unit;
+              dcy     = (T)(int)(Bq.y - cfy) * unit;
+              dcz     = (T)(int)(Bq.z - cfz) * unit;
+              sxr     = (T)(int)(A.x - cfx) * unit;
+              syr     = (T)(int)(A.y - cfy) * unit;
+              szr     = (T)(int)(A.z - cfz) * unit;
+            }
+          else
+            {
+              m   = tv.nm[j];
+              ln  = tv.len[j];
+              dcx = nearest_d(tv.cx[j] - c0, wc);
+              dcy = nearest_d(tv.cy[j] - c1, wc);
+              dcz = nearest_d(tv.cz[j] - c2, wc);
+              sxr = nearest_d(tv.sx[j] - c0, wc);
+              syr = nearest_d(tv.sy[j] - c1, wc);
+              szr = nearest_d(tv.sz[j] - c2, wc);
+            }
+          const T gx = (T)fabs(dcx) - hx, gy = (T)fabs(dcy) - hy, gz = (T)fabs(dcz) - hz;
+          bool discard = false;
+          if(PM)
+            {
+              // forcetree_walk.c:217-244 for every target of the group
+              T eff   = rcut + (T)0.5 * ln;
+              discard = gx > eff || gy > eff || gz > eff;
+            }
+          if(!discard)
+            {
+              T ddx = (T)fabs(sxr) - hx, ddy = (T)fabs(syr) - hy, ddz = (T)fabs(szr) - hz;
+              ddx   = ddx > 0 ? ddx : 0;
+              ddy   = ddy > 0 ? ddy : 0;
+              ddz   = ddz > 0 ? ddz : 0;
+              T r2  = ddx * ddx + ddy * ddy + ddz * ddz;  // nearest target distance bound
+              T len2 = ln * ln;
+              bool open;
+              if(wc.use_bh)
+                open = len2 > r2 * theta2;  // forcetree_walk.c:246-254
+              else
+                open = (len2 > r2 * (T)1.44) || (m * len2 > r2 * r2 * aold_min) ||
+                       (gx < (T)0.6 * ln && gy < (T)0.6 * ln && gz < (T)0.6 * ln);  // :259-289
+              if(MSOFT)
+                {
+                  hj = (T)hsoft[tv.maxsoft[j]];
+                  if(!open && hj > hi_min && r2 < hj * hj)  // :294-320
+                    open = true;
+                }
+              act  = open ? 2 : 1;
+              item = Vec4<T>{sxr, syr, szr, m};
+            }
+        }
+      else if(e >= 0)
+        {
+          T x, y, z, m;
+          if constexpr(sizeof(T) == 4)
+            {
+              uint4 P = tv.prec[e];
+              x       = (T)(int)(P.x - cfx) * unit;
+              y       = (T)(int)(P.y - cfy) * unit;
+              z       = (T)(int)(P.z - cfz) * unit;
+              m       = __uint_as_float(P.w);
+            }
+          else
+            {
+              x = nearest_d(tv.px[e] - c0, wc);
+              y = nearest_d(tv.py[e] - c1, wc);
+              z = nearest_d(tv.pz[e] - c2, wc);
+              m = tv.pm[e];
+            }
+          act  = 3;
+          item = Vec4<T>{x, y, z, m};
+          if(MSOFT)
+            hj = (T)hsoft[tv.ps[e]];
+          if(PM)
+            {
+              // a particle farther than Rcut from the whole group fails the table test for every target
+              T ddx = (T)fabs(x) - hx, ddy = (T)fabs(y) - hy, ddz = (T)fabs(z) - hz;
+              ddx   = ddx > 0 ? ddx : 0;
+              ddy   = ddy > 0 ? ddy : 0;
+              ddz   = ddz > 0 ? ddz : 0;
+              if(ddx * ddx + ddy * ddy + ddz * ddz > rcut2 * (T)1.0001)
+                act = 0;
+            }
+        }
+
+      // stage accepted entries in the LDS tile
+      const bool accp   = (act == 1 || act == 3);
+      const uint64_t bm = __ballot(accp);
+      const int cnt     = __popcll(bm);
+      if(nl + cnt > kListCap)
+        {
+          __builtin_amdgcn_wave_barrier();
+          tile_compute<T, PM, EWALD, MSOFT>(s_list, s_h, nl, tx, ty, tz, hi, s_tab, wc, asmthfac, asmthinv2, ewald_f, ewald_d,
+                                            ax, ay, az, pp, cax, cay, caz, cpp, nin);
+          nl = 0;
+          __builtin_amdgcn_wave_barrier();
+        }
+      if(accp)
+        {
+          int slot     = nl + __popcll(bm & lt);
+          s_list[slot] = item;
+          if(MSOFT)
+            s_h[slot] = hj;
+        }
+      nl += cnt;
+
+      // push the children of opened nodes
+      const int nch  = (act == 2) ? tv.ccnt[j] : 0;
+      const int incl = wave_incl_scan_i(nch);
+      const int tot  = __shfl(incl, 63, 64);
+      if(sp + tot > kStackCap)
+        {
+          overflow = true;
+          break;
+        }
+      if(nch)
+        {
+          const int o  = sp + incl - nch;
+          const int cb = tv.cbeg[j];
+          for(int k = 0; k < nch; k++)
+            s_stack[o + k] = tv.child[cb + k];
+        }
+      sp += tot;
+      __builtin_amdgcn_wave_barrier();
+    }
+
+  if(overflow)
+    {
+      if(lane == 0)
+        {
+          int slot     = atomicAdd(nfailed, 1);
+          failed[slot] = g;
+        }
+      return;
+    }
+  __builtin_amdgcn_wave_barrier();
+  tile_compute<T, PM, EWALD, MSOFT>(s_list, s_h, nl, tx, ty, tz, hi, s_tab, wc, asmthfac, asmthinv2, ewald_f, ewald_d, ax, ay,
+                                    az, pp, cax, cay, caz, cpp, nin);
+
+  if(has)
+    {
+      at[3 * (int64_t)pi + 0] = (double)ax - (double)cax;
+      at[3 * (int64_t)pi + 1] = (double)ay - (double)cay;
+      at[3 * (int64_t)pi + 2] = (double)az - (double)caz;
+      pt[pi]                  = (double)pp - (double)cpp;
+      nint[pi]                = nin;
+    }
+  unsigned long long v = has ? (unsigned long long)nin : 0ull;
+  for(int off = 32; off > 0; off >>= 1)
+    v += __shfl_xor(v, off, 64);
+  if(lane == 0 && v)
+    atomicAdd(isum, v);
+}
+
+// ---------------------------------------------------------------------------
+// host driver
+// ---------------------------------------------------------------------------
+
+WalkConst walk_const(const agrav_ctx *c, int use_bh)
+{
+  WalkConst wc{};
+  wc.periodic = c->par.periodic;
+  wc.box      = c->par.box_size;
+  wc.half_box = 0.5 * c->par.box_size;
+  wc.pm       = c->par.pmgrid > 0;
+  if(wc.pm)
+    {
+      double asmth  = c->par.asmth * c->par.box_size / c->par.pmgrid;  // pm_periodic.c:190-191
+      double rcut   = c->par.rcut * asmth;
+      wc.rcut       = rcut;
+      wc.rcut2      = rcut * rcut;
+      double ai     = 0.5 / asmth;
+      wc.asmthinv2  = ai * ai;
+      wc.asmthfac   = ai * (AG_NTAB / (c->par.rcut / 2.0));
+    }
+  wc.theta2    = c->par.err_tol_theta * c->par.err_tol_theta;
+  wc.errtol    = c->par.err_tol_force_acc;
+  wc.use_bh    = use_bh;
+  wc.ewald     = c->par.periodic && c->par.pmgrid == 0;
+  wc.ewald_fac = wc.ewald ? 2.0 * 64 / c->par.box_size : 0;
+  wc.multisoft = c->soft.nsoft > 1 || c->have_stype;
+  wc.fix_unit  = (float)c->h_dom.fix_unit;
+  return wc;
+}
+
+template <typename T, bool PM, bool EW, bool MS>
+static void launch_group(int ngroups, hipStream_t s, const TreeView &tv, const WalkConst &wc, agrav_ctx *c, int gs)
+{
+  k_walk_group<T, PM, EW, MS><<<ngroups, 64, 0, s>>>(tv, wc, c->d_soft.p, c->d_oldacc_s.p, c->d_tab.p, c->d_ewald_f.p,
+                                                      c->d_ewald.p, gs, c->d_at.p, c->d_pt.p, c->d_nint.p, c->d_failed.p,
+                                                      c->d_counters.p + 48, c->d_isum.p);
+}
+
+template <typename T>
+static void dispatch_group(bool pm, bool ew, bool ms, int ngroups, hipStream_t s, const TreeView &tv, const WalkConst &wc,
+                           agrav_ctx *c, int gs)
+{
+  if(pm)
+    {
+      if(ms)
+        launch_group<T, true, false, true>(ngroups, s, tv, wc, c, gs);
+      else
+        launch_group<T, true, false, false>(ngroups, s, tv, wc, c, gs);
+    }
+  else if(ew)
+    {
+      if(ms)
+        launch_group<T, false, true, true>(ngroups, s, tv, wc, c, gs);
+      else
+        launch_group<T, false, true, false>(ngroups, s, tv, wc, c, gs);
+    }
+  else
+    {
+      if(ms)
+        launch_group<T, false, false, true>(ngroups, s, tv, wc, c, gs);
+      else
+        launch_group<T, false, false, false>(ngroups, s, tv, wc, c, gs);
+    }
+}
+
+static void launch_plain(bool pm, bool ew, int nt, const int *tlist, int tl_group, hipStream_t s, const TreeView &tv,
+                         const WalkConst &wc, agrav_ctx *c)
+{
+  unsigned g = (unsigned)((nt + 255) / 256);
+  if(g == 0)
+    return;
+  if(pm)
+    k_walk_plain<true, false><<<g, 256, 0, s>>>(tv, wc, c->d_soft.p, c->d_oldacc_s.p, c->d_tab.p, c->d_ewald.p, nt, tlist,
+                                                 tl_group, c->d_at.p, c->d_pt.p, c->d_nint.p, c->d_isum.p);
+  else if(ew)
+    k_walk_plain<false, true><<<g, 256, 0, s>>>(tv, wc, c->d_soft.p, c->d_oldacc_s.p, c->d_tab.p, c->d_ewald.p, nt, tlist,
+                                                 tl_group, c->d_at.p, c->d_pt.p, c->d_nint.p, c->d_isum.p);
+  else
+    k_walk_plain<false, false><<<g, 256, 0, s>>>(tv, wc, c->d_soft.p, c->d_oldacc_s.p, c->d_tab.p, c->d_ewald.p, nt, tlist,
+                                                  tl_group, c->d_at.p, c->d_pt.p, c->d_nint.p, c->d_isum.p);
+}
+
+agrav_status walk_tree(agrav_ctx *c, int use_bh)
+{
+  if(!c->have_tree)
+    return fail(c, AGRAV_ERR_STATE, "gravity_tree called before construct_forcetree");
+  const int64_t n = c->n;
+  hipStream_t s   = c->stream;
+  agrav_status st;
+  if((st = c->d_at.ensure(3 * (n ? n : 1))) || (st = c->d_pt.ensure(n ? n : 1)) || (st = c->d_nint.ensure(n ? n : 1)) ||
+     (st = c->d_isum.ensure(1)))
+    return st;
+  AG_HIP(hipMemsetAsync(c->d_isum.p, 0, sizeof(unsigned long long), s));
+  if(n == 0)
+    {
+      c->have_tree_force = true;
+      return AGRAV_OK;
+    }
+  if(!use_bh && c->par.type_of_opening_criterion == 0)
+    use_bh = 1;
+  const WalkConst wc = walk_const(c, use_bh);
+  const TreeView tv  = tree_view(c);
+  const bool pm = wc.pm, ew = wc.ewald, ms = wc.multisoft;
+  if(ew && !c->ewald_ready)
+    {
+      if((st = ewald_tree_table(c)) != AGRAV_OK)
+        return st;
+    }
+  if(!ew)
+    {
+      // kernels take the pointers unconditionally
+      if((st = c->d_ewald.ensure(1)) || (st = c->d_ewald_f.ensure(1)))
+        return st;
+    }
+  c->tim.walk_retries = 0;
+  AG_HIP(hipEventRecord(c->ev[2], s));
+  if(c->par.walk == AGRAV_WALK_PLAIN)
+    {
+      launch_plain(pm, ew, (int)n, nullptr, 1, s, tv, wc, c);
+      AG_HIP(hipGetLastError());
+      AG_HIP(hipEventRecord(c->ev[3], s));
+    }
+  else
+    {
+      int gs      = c->par.group_size > 0 ? c->par.group_size : 64;
+      int ngroups = (int)((n + gs - 1) / gs);
+      if((st = c->d_failed.ensure(ngroups)) != AGRAV_OK)
+        return st;
+      AG_HIP(hipMemsetAsync(c->d_counters.p + 48, 0, sizeof(int), s));
+      if(c->par.precision == AGRAV_FP32)
+        dispatch_group<float>(pm, ew, ms, ngroups, s, tv, wc, c, gs);
+      else
+        dispatch_group<double>(pm, ew, ms, ngroups, s, tv, wc, c, gs);
+      AG_HIP(hipGetLastError());
+      AG_HIP(hipEventRecord(c->ev[3], s));
+      // groups whose LDS stack overflowed are re-walked one lane per target
+      int nf = 0;
+      AG_HIP(hipMemcpyAsync(&nf, c->d_counters.p + 48, sizeof(int), hipMemcpyDeviceToHost, s));
+      AG_HIP(hipStreamSynchronize(s));
+      if(nf > 0)
+        {
+          c->tim.walk_retries = nf;
+          launch_plain(pm, ew, nf * gs, c->d_failed.p, gs, s, tv, wc, c);
+          AG_HIP(hipGetLastError());
+        }
+    }
+  AG_HIP(hipEventRecord(c->ev[4], s));
+  unsigned long long tot = 0;
+  AG_HIP(hipMemcpyAsync(&tot, c->d_isum.p, sizeof(tot), hipMemcpyDeviceToHost, s));
+  AG_HIP(hipStreamSynchronize(s));
+  float ms_walk = 0, ms_kernel = 0;
+  AG_HIP(hipEventElapsedTime(&ms_kernel, c->ev[2], c->ev[3]));
+  AG_HIP(hipEventElapsedTime(&ms_walk, c->ev[2], c->ev[4]));
+  c->tim.treewalk     = ms_walk;
+  c->tim.walk_kernel  = ms_kernel;
+  c->tim.interactions = (long long)tot;
+  c->have_tree_force  = true;
+  return AGRAV_OK;
+}
+
+}  // namespace agrav

@@ -1,0 +1,105 @@
+"""Seeded synthetic particle sets for the gravity workloads.
+
+All generators use numpy's Philox bit generator so that a (config, seed)
+pair always yields the same particles.  Units: G = 1; periodic sets live in
+the unit box.  Each function returns (pos[N,3] float64, mass[N] float64, eps)
+with eps the Plummer-equivalent softening.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def rng(seed: int) -> np.random.Generator:
+    return np.random.Generator(np.random.Philox(seed))
+
+
+def _isotropic(r: np.ndarray, g: np.random.Generator) -> np.ndarray:
+    mu = g.uniform(-1.0, 1.0, len(r))
+    phi = g.uniform(0.0, 2 * np.pi, len(r))
+    s = np.sqrt(1 - mu * mu)
+    return np.stack([r * s * np.cos(phi), r * s * np.sin(phi), r * mu], axis=1)
+
+
+def hernquist_radii(n: int, a: float, g: np.random.Generator, rmax_over_a: float = 30.0) -> np.ndarray:
+    """Radii from the Hernquist mass profile M(<r) = M r^2/(r+a)^2 truncated at rmax."""
+    smax = rmax_over_a / (1.0 + rmax_over_a)
+    s = np.sqrt(g.random(n)) * smax
+    return a * s / (1.0 - s)
+
+
+def uniform_box(n: int = 32768, seed: int = 1):
+    """Uniform random particles in the periodic unit box, equal masses, eps = (1/N^(1/3))/40."""
+    g = rng(seed)
+    pos = g.random((n, 3))
+    mass = np.full(n, 1.0 / n)
+    eps = (1.0 / round(n ** (1.0 / 3.0))) / 40.0
+    return pos, mass, eps
+
+
+def clustered_box(n: int = 128 ** 3, nhalo: int = 256, seed: int = 2, mass_range: float = 1000.0, slope: float = 1.9):
+    """Half uniform, half in `nhalo` Hernquist spheres (a ~ U(0.005, 0.02), dN/dM ~ M^-slope on
+    [Mmin, mass_range*Mmin], normalised to half the mass), wrapped into the periodic unit box."""
+    g = rng(seed)
+    n_uni = n // 2
+    n_cl = n - n_uni
+    # power-law halo masses by inverse CDF of dN/dM ~ M^-slope on [1, mass_range]
+    u = g.random(nhalo)
+    k = 1.0 - slope
+    m = (1.0 + u * (mass_range ** k - 1.0)) ** (1.0 / k)
+    counts = np.floor(m / m.sum() * n_cl).astype(np.int64)
+    short = n_cl - counts.sum()
+    order = np.argsort(-(m / m.sum() * n_cl - counts))
+    counts[order[:short]] += 1
+    centres = g.random((nhalo, 3))
+    scales = g.uniform(0.005, 0.02, nhalo)
+    parts = [g.random((n_uni, 3))]
+    for c, a, cnt in zip(centres, scales, counts):
+        if cnt == 0:
+            continue
+        r = hernquist_radii(int(cnt), a, g)
+        parts.append(c + _isotropic(r, g))
+    pos = np.concatenate(parts) % 1.0
+    pos[pos >= 1.0] = 0.0
+    mass = np.full(n, 1.0 / n)
+    eps = (1.0 / round(n ** (1.0 / 3.0))) / 40.0
+    return pos, mass, eps
+
+
+def halo_disk(n: int = 1 << 20, seed: int = 3, eps: float = 0.01):
+    """Isolated Hernquist halo (M=1, a=1, r<30a) plus exponential disk (M=0.1, Rd=0.2, sech^2 z0=0.02),
+    equal particle masses (total 1.1)."""
+    g = rng(seed)
+    n_halo = int(round(n / 1.1))
+    n_disk = n - n_halo
+    r = hernquist_radii(n_halo, 1.0, g)
+    halo = _isotropic(r, g)
+    R = 0.2 * g.gamma(2.0, 1.0, n_disk)
+    ph = g.uniform(0, 2 * np.pi, n_disk)
+    z = 0.02 * np.arctanh(np.clip(2 * g.random(n_disk) - 1, -1 + 1e-15, 1 - 1e-15))
+    disk = np.stack([R * np.cos(ph), R * np.sin(ph), z], axis=1)
+    pos = np.concatenate([halo, disk])
+    mass = np.full(n, 1.1 / n)
+    return pos, mass, eps
+
+
+def nfw_like_group(n: int, conc: float, g: np.random.Generator, rvir: float = 1.0):
+    """NFW profile truncated at the virial radius: M(<x) ~ ln(1+x) - x/(1+x), x = r/rs, rs = rvir/conc."""
+    rs = rvir / conc
+    xmax = conc
+    mx = np.log1p(xmax) - xmax / (1 + xmax)
+    u = g.random(n) * mx
+    # invert by bisection on x in [0, xmax]
+    lo = np.zeros(n)
+    hi = np.full(n, xmax)
+    for _ in range(60):
+        mid = 0.5 * (lo + hi)
+        f = np.log1p(mid) - mid / (1 + mid)
+        lo = np.where(f < u, mid, lo)
+        hi = np.where(f < u, hi, mid)
+    return _isotropic(0.5 * (lo + hi) * rs, g)
+
+
+def half_mass_radius(pos: np.ndarray) -> float:
+    r = np.sqrt((pos * pos).sum(1))
+    return float(np.median(r))

@@ -1,0 +1,180 @@
+/*
+ * agrav.h -- C ABI of the MI355X-native TreePM gravity step.
+ *
+ * A drop-in for AREPO's gravity hot path: tree build, tree walk, periodic PM
+ * and the FORCETEST direct-summation check.  Every entry point names the
+ * reference function it replaces (paths relative to AREPO's src/).
+ *
+ * Conventions follow the reference: tree accelerations and potentials are in
+ * G=1 units until agrav_gravity_force_finalize (accel.c:258-265); PM results
+ * include G (pm_periodic.c:1590); OldAcc is |a_tree + a_PM/G| in G=1 units
+ * (accel.c:237-255).  Particle arrays are in the caller's order; positions
+ * are interleaved x,y,z per particle like P[i].Pos (allvars.h:1216-1274).
+ *
+ * A context is not thread-safe; distinct contexts may be used from distinct
+ * host threads.  All device work of a context runs on its own HIP stream.
+ */
+#ifndef AGRAV_H
+#define AGRAV_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Errors replace terminate_program()/MPI_Abort (allvars.h:173) and the
+ * negative returns of force_treebuild (forcetree.c:186-228). */
+typedef enum
+{
+  AGRAV_OK            = 0,
+  AGRAV_ERR_BAD_PARAM = -1, /* invalid parameter or argument */
+  AGRAV_ERR_CAPACITY  = -2, /* a capacity limit was exceeded (forcetree.c:930-933) */
+  AGRAV_ERR_OUT_OF_BOX = -3, /* particle outside the periodic box (forcetree.c:432-438) */
+  AGRAV_ERR_HIP       = -4, /* HIP runtime error */
+  AGRAV_ERR_OOM       = -5, /* device allocation failed */
+  AGRAV_ERR_STATE     = -6, /* call order violated (e.g. walk before build) */
+  AGRAV_ERR_FFT       = -7, /* rocFFT error */
+  AGRAV_ERR_NO_DEVICE = -8  /* no usable GPU */
+} agrav_status;
+
+typedef enum
+{
+  AGRAV_FP64 = 0, /* DOUBLEPRECISION=1 semantics throughout */
+  AGRAV_FP32 = 1  /* FP32 interactions on node-relative offsets, compensated sums;
+                     FP64 kept for positions, COM accumulation, Green's function */
+} agrav_precision;
+
+typedef enum
+{
+  AGRAV_WALK_PLAIN = 0, /* one lane per target, AREPO's per-particle opening test */
+  AGRAV_WALK_GROUP = 1  /* one wavefront per Hilbert-ordered target group, node tiles in LDS */
+} agrav_walk_kind;
+
+#define AGRAV_MAX_SOFT_TYPES 8
+
+/* Runtime replacement of the gravity Config.sh flags and param.txt entries
+ * (Template-Config.sh:66-136, parameters.c:214-363). */
+typedef struct
+{
+  int periodic;                   /* 1 unless GRAVITY_NOT_PERIODIC */
+  double box_size;                /* BoxSize; isolated runs normally pass 0 */
+  int pmgrid;                     /* PMGRID, 0 = tree only (power of two, periodic only) */
+  double asmth;                   /* ASMTH in mesh cells (allvars.h:461), default 1.25 */
+  double rcut;                    /* RCUT in units of Asmth (allvars.h:469), default 4.5 */
+  double G;                       /* GravityConstantInternal */
+  double err_tol_theta;           /* ErrTolTheta (BH, and the first-step seeding pass) */
+  double err_tol_force_acc;       /* ErrTolForceAcc */
+  int type_of_opening_criterion;  /* 0 = Barnes-Hut, 1 = relative */
+  int n_soft_types;               /* number of softening types in use (>=1) */
+  double softening[AGRAV_MAX_SOFT_TYPES]; /* Plummer-equivalent softening per type (SofteningComovingType*) */
+  int precision;                  /* agrav_precision */
+  int walk;                       /* agrav_walk_kind */
+  int group_size;                 /* targets per group for AGRAV_WALK_GROUP (8..64, 0 = 64) */
+  double tree_alloc_factor;       /* TreeAllocFactor (init.c:110); 0 = 0.7 (10 below 1000 particles) */
+} agrav_params;
+
+/* Per-stage device times (ms) of the last call, named like AREPO's cpu.txt
+ * timers (utils/timer.h:174-187). */
+typedef struct
+{
+  double treebuild;   /* keys, sort, octree, moments */
+  double treewalk;    /* the walk kernel(s) of the last agrav_gravity_tree */
+  double pm_grav;     /* long-range PM force */
+  double finalize;    /* gravity_force_finalize */
+  double walk_kernel; /* average duration of one walk kernel launch */
+  long long interactions; /* sum of per-target interaction counts of the last walk */
+  int numnodes;       /* octree nodes of the last build */
+  int walk_retries;   /* groups re-run by the fallback walk (stack overflow) */
+} agrav_timings;
+
+typedef struct agrav_ctx agrav_ctx;
+
+const char *agrav_status_string(agrav_status s);
+/* Human-readable detail of the last error on this context. */
+const char *agrav_last_error(const agrav_ctx *ctx);
+
+/* Fill p with the defaults of the reference (allvars.h:461-469, init.c:110). */
+void agrav_default_params(agrav_params *p);
+
+/* begrun1(): set_softenings (grav_softening.c:55-96), force_short_range_init
+ * (forcetree_walk.c:72-85), ewald_init (forcetree_ewald.c:89-252) and
+ * pm_init_periodic (pm_periodic.c:173-255).  Selects the current HIP device. */
+agrav_status agrav_create(const agrav_params *p, agrav_ctx **out);
+void agrav_destroy(agrav_ctx *ctx);
+
+/* Change the softening table between calls: set_softenings (grav_softening.c:55-96). */
+agrav_status agrav_set_softenings(agrav_ctx *ctx, int n_soft_types, const double *softening);
+
+/* Particle load: replaces the reads of P[i].Pos/Mass/SofteningType
+ * (forcetree.c:394-417).  Host pointers; data are copied to HBM.
+ * softtype may be NULL (all type 0). */
+agrav_status agrav_set_particles(agrav_ctx *ctx, int64_t n, const double *pos, const double *mass, const uint8_t *softtype);
+/* Same with device pointers (already resident in HBM); copied device-to-device on the ctx stream. */
+agrav_status agrav_set_particles_device(agrav_ctx *ctx, int64_t n, const double *d_pos, const double *d_mass,
+                                        const uint8_t *d_softtype);
+/* P[i].OldAcc (accel.c:237-255), host pointer, caller order. */
+agrav_status agrav_set_old_acc(agrav_ctx *ctx, const float *old_acc);
+
+/* construct_forcetree (forcetree.c:110-155): domain cube, Peano-Hilbert keys,
+ * sort, octree, bottom-up moments.  numnodes may be NULL. */
+agrav_status agrav_construct_forcetree(agrav_ctx *ctx, int *numnodes);
+
+/* gravity_tree (gravtree.c:357-707) over all particles with force_treeevaluate
+ * (forcetree_walk.c:115-514).  use_bh=1: Barnes-Hut with ErrTolTheta;
+ * use_bh=0: the configured criterion (relative needs OldAcc). */
+agrav_status agrav_gravity_tree(agrav_ctx *ctx, int use_bh);
+
+/* long_range_force / pmforce_periodic (longrange.c:98-198, pm_periodic.c:1572-1936). */
+agrav_status agrav_long_range_force(agrav_ctx *ctx);
+
+/* gravity_force_finalize (accel.c:204-347): OldAcc, x G, PM potential constant, + PM. */
+agrav_status agrav_gravity_force_finalize(agrav_ctx *ctx);
+
+/* compute_grav_accelerations (accel.c:58-80) + gravity (accel.c:94-192):
+ * first_step=1 with the relative criterion runs PM and a Barnes-Hut pass to
+ * seed OldAcc first (accel.c:62-72). */
+agrav_status agrav_compute_grav_accelerations(agrav_ctx *ctx, int first_step);
+
+/* Results in the caller's order (host pointers, any may be NULL):
+ * acc[3n] = GravAccel (+GravPM), pot[n] = Potential, nint[n], old_acc[n]. */
+agrav_status agrav_get_accelerations(agrav_ctx *ctx, double *acc, double *pot, int32_t *nint, float *old_acc);
+/* Device pointers to the same results (valid until the next call on ctx). */
+agrav_status agrav_get_accelerations_device(agrav_ctx *ctx, const double **acc, const double **pot);
+/* Raw parts: tree force (G=1 units) and PM force, caller order, host pointers (may be NULL). */
+agrav_status agrav_get_components(agrav_ctx *ctx, double *acc_tree, double *pot_tree, double *acc_pm, double *pot_pm);
+
+/* gravity_forcetest / gravity_forcetest_evaluate (gravtree_forcetest.c:257-648):
+ * FP64 direct summation for ns selected particles (caller indices) over all
+ * particles, with the 129^3 Ewald table (gravtree_forcetest.c:933-1085) when
+ * periodic.  Outputs in G=1 units (any may be NULL): true force and
+ * potential, and the exact-erfc short-range parts (PMGRID only). */
+agrav_status agrav_gravity_forcetest(agrav_ctx *ctx, int ns, const int32_t *sel, double *acc, double *acc_short, double *pot,
+                                     double *pot_short);
+
+/* gravity_direct (gravdirect.c:61-257) with force_evaluate_direct
+ * (forcetree_walk.c:557-708): all-pairs tree-equivalent force, G=1 units,
+ * caller order (host pointers). */
+agrav_status agrav_gravity_direct(agrav_ctx *ctx, double *acc, double *pot);
+
+agrav_status agrav_get_timings(agrav_ctx *ctx, agrav_timings *t);
+
+/* Tree inspection for tests: node count and per-node data (host pointers of
+ * numnodes entries; any may be NULL).  Node 0 is the root; order is pre-order. */
+agrav_status agrav_get_tree(agrav_ctx *ctx, int *numnodes, double *center, double *len, double *com, double *mass, int32_t *level,
+                            int32_t *npart);
+/* Particle permutation: sorted position k holds caller index perm[k]; keys[k] its 63-bit Hilbert key. */
+agrav_status agrav_get_order(agrav_ctx *ctx, int32_t *perm, uint64_t *keys);
+
+/* The tables the kernels use: short-range (forcetree_walk.c:72-85), 2 x 128 floats;
+ * Ewald tree table (forcetree_ewald.c:89-252), 4 x 65^3 doubles (periodic tree-only);
+ * FORCETEST table (gravtree_forcetest.c:933-1008), 4 x 129^3 doubles (periodic). */
+agrav_status agrav_get_tables(agrav_ctx *ctx, float *tab_force, float *tab_pot, double *ewald_tree, double *ewald_forcetest);
+
+/* Standalone 63-bit Peano-Hilbert key of 21-bit integer coordinates (host side, for checks). */
+uint64_t agrav_hilbert_key21(uint32_t x, uint32_t y, uint32_t z);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

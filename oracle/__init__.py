@@ -1,0 +1,316 @@
+"""ctypes binding of the CPU FP64 oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: the checker for the HIP product path.  Only the
+tests, ``__graft_entry__.smoke()`` and ``bench.py``'s cpu_baseline leg may
+import this module.  Functions restate AREPO's gravity step; each wrapper
+names the C function whose header cites the reference file:line.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+NTAB = 127
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("periodic", C.c_int),
+        ("box_size", C.c_double),
+        ("pmgrid", C.c_int),
+        ("asmth", C.c_double),
+        ("rcut", C.c_double),
+        ("G", C.c_double),
+        ("err_tol_theta", C.c_double),
+        ("err_tol_force_acc", C.c_double),
+        ("opening", C.c_int),
+        ("nsoft", C.c_int),
+        ("soft", C.c_double * 8),
+    ]
+
+
+def make_params(periodic=True, box_size=1.0, pmgrid=0, asmth=1.25, rcut=4.5, G=1.0, err_tol_theta=0.7,
+                err_tol_force_acc=0.0025, opening=1, soft=(0.01,)):
+    p = Params()
+    p.periodic = int(periodic)
+    p.box_size = box_size
+    p.pmgrid = pmgrid
+    p.asmth = asmth
+    p.rcut = rcut
+    p.G = G
+    p.err_tol_theta = err_tol_theta
+    p.err_tol_force_acc = err_tol_force_acc
+    p.opening = opening
+    p.nsoft = len(soft)
+    for i, s in enumerate(soft):
+        p.soft[i] = s
+    return p
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = C.CDLL(_LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_u8 = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+
+
+def _declare(L):
+    P = C.POINTER(Params)
+    L.orc_set_softenings.argtypes = [P, _dp]
+    L.orc_short_range_tables.argtypes = [C.c_double, _fp, _fp]
+    L.orc_domain_extent.argtypes = [P, C.c_int, _dp, _dp, _dp, C.POINTER(C.c_double)]
+    L.orc_tree_build.argtypes = [P, C.c_int, _dp, _dp, C.c_void_p, C.c_double, C.POINTER(C.c_void_p)]
+    L.orc_tree_build.restype = C.c_int
+    L.orc_tree_free.argtypes = [C.c_void_p]
+    L.orc_tree_numnodes.argtypes = [C.c_void_p]
+    L.orc_tree_export.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _ip, _ip]
+    L.orc_tree_walk.argtypes = [P, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, _dp, _dp, _ip,
+                                C.c_int]
+    L.orc_tree_walk.restype = C.c_longlong
+    L.orc_pm_periodic.argtypes = [P, C.c_int, _dp, _dp, _dp, _dp]
+    L.orc_ewald_psi.argtypes = [C.c_double] * 3
+    L.orc_ewald_psi.restype = C.c_double
+    L.orc_ewald_force.argtypes = [C.c_double] * 3 + [_dp]
+    L.orc_ewald_table_tree.argtypes = [C.c_int, C.c_double, _dp, C.c_int]
+    L.orc_ewald_corr.argtypes = [_dp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, _dp]
+    L.orc_ewald_pot_corr.argtypes = [_dp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]
+    L.orc_ewald_pot_corr.restype = C.c_double
+    L.orc_forcetest_ewald_exact.argtypes = [C.c_double] * 4 + [_dp]
+    L.orc_forcetest_ewald_table.argtypes = [C.c_int, C.c_double, _dp, C.c_int]
+    L.orc_forcetest_ewald_lookup.argtypes = [_dp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, _dp]
+    L.orc_forcetest.argtypes = [P, C.c_int, _dp, _dp, C.c_void_p, C.c_int, _ip, C.c_int, C.c_void_p, C.c_int,
+                                _dp, _dp, _dp, _dp, C.c_int]
+    L.orc_compute_grav_accelerations.argtypes = [P, C.c_int, _dp, _dp, C.c_void_p, C.c_int, _fp, _dp, _dp, _ip,
+                                                 C.c_int, _dp]
+    L.orc_compute_grav_accelerations.restype = C.c_longlong
+    L.orc_direct_summation.argtypes = [P, C.c_int, _dp, _dp, C.c_void_p, _dp, _dp, C.c_int]
+    L.orc_group_potential_bh.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_double, _dp]
+    L.orc_group_potential_bh.restype = C.c_longlong
+    L.orc_fft3d.argtypes = [C.c_int, _dp, _dp, C.c_int]
+
+
+def _pos(pos):
+    return np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+
+
+def _u8ptr(a):
+    if a is None:
+        return None
+    return np.ascontiguousarray(a, dtype=np.uint8).ctypes.data
+
+
+def force_softening(p):
+    out = np.zeros(p.nsoft + 1)
+    lib().orc_set_softenings(C.byref(p), out)
+    return out
+
+
+def short_range_tables(rcut=4.5):
+    tf = np.zeros(NTAB + 1, np.float32)
+    tp = np.zeros(NTAB + 1, np.float32)
+    lib().orc_short_range_tables(rcut, tf, tp)
+    return tf, tp
+
+
+def domain_extent(p, pos):
+    pos = _pos(pos)
+    corner = np.zeros(3)
+    center = np.zeros(3)
+    ln = C.c_double()
+    lib().orc_domain_extent(C.byref(p), len(pos), pos, corner, center, C.byref(ln))
+    return corner, center, ln.value
+
+
+class Tree:
+    """Oracle octree (forcetree.c restatement).  Keeps its inputs alive."""
+
+    def __init__(self, p, pos, mass, stype=None, alloc_factor=None):
+        self.p = p
+        self.pos = _pos(pos)
+        self.mass = np.ascontiguousarray(mass, dtype=np.float64)
+        self.stype = None if stype is None else np.ascontiguousarray(stype, dtype=np.uint8)
+        n = len(self.pos)
+        af = alloc_factor if alloc_factor is not None else (10.0 if n < 1000 else 0.7)
+        h = C.c_void_p()
+        while True:
+            rc = lib().orc_tree_build(C.byref(p), n, self.pos, self.mass, _u8ptr(self.stype), af, C.byref(h))
+            if rc >= 0 or alloc_factor is not None:
+                break
+            af *= 1.15
+        self.rc = rc
+        self.h = h
+        self.n = n
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().orc_tree_free(self.h)
+            self.h = C.c_void_p()
+
+    @property
+    def numnodes(self):
+        return lib().orc_tree_numnodes(self.h)
+
+    def export(self):
+        M = self.numnodes
+        center = np.zeros(3 * M)
+        ln = np.zeros(M)
+        com = np.zeros(3 * M)
+        mass = np.zeros(M)
+        level = np.zeros(M, np.int32)
+        npart = np.zeros(M, np.int32)
+        lib().orc_tree_export(self.h, center, ln, com, mass, level, npart)
+        return dict(center=center.reshape(-1, 3), len=ln, com=com.reshape(-1, 3), mass=mass, level=level,
+                    npart=npart)
+
+    def walk(self, use_bh, old_acc=None, ewald_tab=None, targets=None, nthreads=0):
+        nt = self.n if targets is None else len(targets)
+        acc = np.zeros(3 * nt)
+        pot = np.zeros(nt)
+        nint = np.zeros(nt, np.int32)
+        oa = None if old_acc is None else np.ascontiguousarray(old_acc, dtype=np.float32)
+        tg = None if targets is None else np.ascontiguousarray(targets, dtype=np.int32)
+        tab = None if ewald_tab is None else np.ascontiguousarray(ewald_tab, dtype=np.float64)
+        total = lib().orc_tree_walk(C.byref(self.p), self.h, int(use_bh), None if oa is None else oa.ctypes.data,
+                                    None if tab is None else tab.ctypes.data, nt,
+                                    None if tg is None else tg.ctypes.data, acc, pot, nint, nthreads)
+        return acc.reshape(-1, 3), pot, nint, total
+
+
+def pm_periodic(p, pos, mass):
+    pos = _pos(pos)
+    mass = np.ascontiguousarray(mass, dtype=np.float64)
+    acc = np.zeros(3 * len(pos))
+    pot = np.zeros(len(pos))
+    rc = lib().orc_pm_periodic(C.byref(p), len(pos), pos, mass, acc, pot)
+    if rc != 0:
+        raise ValueError("orc_pm_periodic failed %d" % rc)
+    return acc.reshape(-1, 3), pot
+
+
+def ewald_psi(x, y, z):
+    return lib().orc_ewald_psi(x, y, z)
+
+
+def ewald_force(x, y, z):
+    f = np.zeros(3)
+    lib().orc_ewald_force(x, y, z, f)
+    return f
+
+
+_EWALD_CACHE = {}
+
+
+def ewald_table_tree(box=1.0, en=64, nthreads=0):
+    """forcetree_ewald.c table (cached per box size: it takes seconds to compute)."""
+    key = (float(box), en)
+    if key not in _EWALD_CACHE:
+        tab = np.zeros(4 * (en + 1) ** 3)
+        lib().orc_ewald_table_tree(en, box, tab, nthreads)
+        _EWALD_CACHE[key] = tab
+    return _EWALD_CACHE[key]
+
+
+def ewald_corr(tab, box, d, en=64):
+    f = np.zeros(3)
+    lib().orc_ewald_corr(tab, en, box, d[0], d[1], d[2], f)
+    return f, lib().orc_ewald_pot_corr(tab, en, box, d[0], d[1], d[2])
+
+
+def forcetest_ewald_exact(box, x, y, z):
+    f = np.zeros(4)
+    lib().orc_forcetest_ewald_exact(box, x, y, z, f)
+    return f
+
+
+def forcetest_ewald_table(tew_n, box=1.0, nthreads=0):
+    tab = np.zeros(4 * (tew_n + 1) ** 3)
+    lib().orc_forcetest_ewald_table(tew_n, box, tab, nthreads)
+    return tab
+
+
+def forcetest_ewald_lookup(tab, tew_n, box, d):
+    f = np.zeros(4)
+    lib().orc_forcetest_ewald_lookup(tab, tew_n, box, d[0], d[1], d[2], f)
+    return f
+
+
+def forcetest(p, pos, mass, sel, stype=None, ewald=0, tab=None, tew_n=0, nthreads=0):
+    """FORCETEST direct sum (G=1 units): returns acc, acc_short, pot, pot_short for the selected targets."""
+    pos = _pos(pos)
+    mass = np.ascontiguousarray(mass, dtype=np.float64)
+    sel = np.ascontiguousarray(sel, dtype=np.int32)
+    ns = len(sel)
+    acc = np.zeros(3 * ns)
+    accs = np.zeros(3 * ns)
+    pot = np.zeros(ns)
+    pots = np.zeros(ns)
+    st = None if stype is None else np.ascontiguousarray(stype, dtype=np.uint8)
+    tb = None if tab is None else np.ascontiguousarray(tab, dtype=np.float64)
+    lib().orc_forcetest(C.byref(p), len(pos), pos, mass, None if st is None else st.ctypes.data, ns, sel, ewald,
+                        None if tb is None else tb.ctypes.data, tew_n, acc, accs, pot, pots, nthreads)
+    return acc.reshape(-1, 3), accs.reshape(-1, 3), pot, pots
+
+
+def compute_grav_accelerations(p, pos, mass, stype=None, first_step=True, old_acc=None, nthreads=0):
+    pos = _pos(pos)
+    n = len(pos)
+    mass = np.ascontiguousarray(mass, dtype=np.float64)
+    oa = np.zeros(n, np.float32) if old_acc is None else np.ascontiguousarray(old_acc, dtype=np.float32).copy()
+    acc = np.zeros(3 * n)
+    pot = np.zeros(n)
+    nint = np.zeros(n, np.int32)
+    t = np.zeros(3)
+    st = None if stype is None else np.ascontiguousarray(stype, dtype=np.uint8)
+    total = lib().orc_compute_grav_accelerations(C.byref(p), n, pos, mass, None if st is None else st.ctypes.data,
+                                                 int(first_step), oa, acc, pot, nint, nthreads, t)
+    if total < 0:
+        raise RuntimeError("oracle compute failed %d" % total)
+    return dict(acc=acc.reshape(-1, 3), pot=pot, nint=nint, old_acc=oa, total=total,
+                t_treebuild=t[0], t_treewalk=t[1], t_pm=t[2])
+
+
+def direct_summation(p, pos, mass, stype=None, nthreads=0):
+    pos = _pos(pos)
+    mass = np.ascontiguousarray(mass, dtype=np.float64)
+    acc = np.zeros(3 * len(pos))
+    pot = np.zeros(len(pos))
+    st = None if stype is None else np.ascontiguousarray(stype, dtype=np.uint8)
+    lib().orc_direct_summation(C.byref(p), len(pos), pos, mass, None if st is None else st.ctypes.data, acc, pot,
+                               nthreads)
+    return acc.reshape(-1, 3), pot
+
+
+def group_potential_bh(pos, mass, h, theta):
+    pos = _pos(pos)
+    mass = np.ascontiguousarray(mass, dtype=np.float64)
+    pot = np.zeros(len(pos))
+    total = lib().orc_group_potential_bh(len(pos), pos, mass, h, theta, pot)
+    return pot, total
+
+
+def fft3d(re, im, sign):
+    re = np.ascontiguousarray(re, dtype=np.float64).copy()
+    im = np.ascontiguousarray(im, dtype=np.float64).copy()
+    n = re.shape[0]
+    lib().orc_fft3d(n, re, im, sign)
+    return re, im

@@ -1,0 +1,1501 @@
+/*
+ * agrav_oracle.c -- CPU FP64 restatement of AREPO's gravity step.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP product path.  Only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
+ *
+ * The code restates the algorithm of the AREPO sources cited per function
+ * (paths relative to AREPO's src/).  It is written from those semantics, not
+ * copied: data live in plain arrays owned by an orc_tree, there is no MPI,
+ * no top-level domain tree (one rank: its empty top nodes carry no mass and
+ * do not change forces), and domain randomisation is off.
+ */
+#include "agrav_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+/* spline-softening coefficients, allvars.h:557-573 */
+#define SF1 10.666666666667
+#define SF2 32.0
+#define SF3 (-38.4)
+#define SF4 (-2.8)
+#define SF5 5.333333333333
+#define SF6 6.4
+#define SF7 (-9.6)
+#define SF8 21.333333333333
+#define SF9 (-48.0)
+#define SF10 38.4
+#define SF11 (-10.666666666667)
+#define SF12 (-0.066666666667)
+#define SF13 (-3.2)
+#define SF14 0.066666666667
+#define SF15 (-16.0)
+#define SF16 9.6
+#define SF17 (-2.133333333333)
+
+static int omp_threads(int nthreads)
+{
+#ifdef _OPENMP
+  return nthreads > 0 ? nthreads : omp_get_max_threads();
+#else
+  (void)nthreads;
+  return 1;
+#endif
+}
+
+/* ------------------------------------------------------------------ */
+/* parameters and tables                                               */
+/* ------------------------------------------------------------------ */
+
+void orc_set_softenings(const orc_params *p, double *fs)
+{
+  /* grav_softening.c:90-95 */
+  for(int i = 0; i < p->nsoft; i++)
+    fs[i] = 2.8 * p->soft[i];
+  fs[p->nsoft] = 0;
+}
+
+void orc_short_range_tables(double rcut, float *tf, float *tp)
+{
+  /* forcetree_walk.c:72-85 */
+  for(int i = 0; i <= ORC_NTAB; i++)
+    {
+      double u = ((rcut / 2.0) / ORC_NTAB) * i;
+      tp[i]    = (float)(-erfc(u));
+      if(u > 0)
+        tf[i] = (float)((erfc(u) + 2.0 * u / sqrt(M_PI) * exp(-u * u) - 1.0) / (u * u));
+      else
+        tf[i] = 0;
+    }
+}
+
+void orc_asmth_rcut(const orc_params *p, double *asmth_len, double *rcut_len)
+{
+  /* pm_periodic.c:190-191 */
+  double a   = p->pmgrid > 0 ? p->asmth * p->box_size / p->pmgrid : 0;
+  *asmth_len = a;
+  *rcut_len  = p->rcut * a;
+}
+
+static inline double nearest(double x, double box, int periodic)
+{
+  /* GRAVITY_NEAREST_X, allvars.h:603-612 */
+  if(!periodic)
+    return x;
+  double half = 0.5 * box;
+  return x > half ? x - box : (x < -half ? x + box : x);
+}
+
+void orc_domain_extent(const orc_params *p, int n, const double *pos, double corner[3], double center[3], double *len_out)
+{
+  /* domain_box.c:107-197 with DO_NOT_RANDOMIZE_DOMAINCENTER */
+  /* preset to the simulation box [0, BoxSize] (domain_box.c:112-116); isolated
+   * runs pass BoxSize = 0 so that the cube is the particles' extent plus the origin */
+  double xmin[3], xmax[3];
+  for(int j = 0; j < 3; j++)
+    {
+      xmin[j] = 0;
+      xmax[j] = p->box_size;
+    }
+  for(int i = 0; i < n; i++)
+    for(int j = 0; j < 3; j++)
+      {
+        double x = pos[3 * i + j];
+        if(p->periodic)
+          {
+            if(x < 0)
+              x += p->box_size;
+            if(x >= p->box_size)
+              x -= p->box_size;
+          }
+        if(xmin[j] > x)
+          xmin[j] = x;
+        if(xmax[j] < x)
+          xmax[j] = x;
+      }
+  double len = 0;
+  for(int j = 0; j < 3; j++)
+    if(xmax[j] - xmin[j] > len)
+      len = xmax[j] - xmin[j];
+  if(len <= 0)
+    len = 1.0;
+  len *= p->periodic ? 1.00001 : 1.2;
+  for(int j = 0; j < 3; j++)
+    {
+      center[j] = 0.5 * (xmin[j] + xmax[j]);
+      corner[j] = 0.5 * (xmin[j] + xmax[j]) - 0.5 * len;
+    }
+  *len_out = len;
+}
+
+/* ------------------------------------------------------------------ */
+/* tree                                                                */
+/* ------------------------------------------------------------------ */
+
+struct orc_tree
+{
+  int n, maxnodes, numnodes;
+  int *suns;          /* 8 per node */
+  double *center;     /* 3 per node */
+  double *len;
+  double *s;          /* centre of mass, 3 per node */
+  double *mass;
+  int *sibling, *nextnode, *father;
+  int *level;
+  int *npart;
+  uint8_t *maxsoft;
+  int *pnext;         /* Nextnode[] of particles */
+  uint64_t *intpos;   /* 3 per particle */
+  double *pos;        /* wrapped positions, 3 per particle */
+  const double *pmass;
+  const uint8_t *stype;
+  double force_softening[ORC_MAXSOFT + 1];
+  int nsoft;
+  double corner[3], dcenter[3], dlen, bigfac;
+  int periodic;
+  double box;
+};
+
+static uint64_t dbl_mantissa(double d)
+{
+  /* force_double_to_int, forcetree.h:122-131 */
+  uint64_t u;
+  memcpy(&u, &d, 8);
+  return u & 0xFFFFFFFFFFFFFull;
+}
+
+static int tree_new_node(orc_tree *t)
+{
+  if(t->numnodes >= t->maxnodes)
+    return -1;
+  int k = t->numnodes++;
+  for(int j = 0; j < 8; j++)
+    t->suns[8 * k + j] = -1;
+  return k;
+}
+
+/* forcetree.c:835-938; node k is referenced as n + k like Nodes[Tree_MaxPart + k] */
+static int tree_insert(orc_tree *t, int i)
+{
+  const int N      = t->n;
+  const uint64_t *ip = &t->intpos[3 * i];
+  int th = N + 0, parent = -1, levels = 0;
+  unsigned subnode = 0;
+  uint64_t mask     = 1ull << 51;
+  int shx = 51, shy = 50, shz = 49;
+  int64_t centermask = (int64_t)0xFFF0000000000000ull;
+
+  for(;;)
+    {
+      if(th >= N)
+        {
+          int k   = th - N;
+          subnode = (unsigned)(((ip[0] & mask) >> shx) | ((ip[1] & mask) >> shy) | ((ip[2] & mask) >> shz));
+          shx--;
+          shy--;
+          shz--;
+          centermask >>= 1;
+          mask >>= 1;
+          levels++;
+          if(levels > ORC_MAX_TREE_LEVEL)
+            {
+              /* coincident points: take the next free slot (forcetree.c:861-877) */
+              for(int j = 0; j < 8; j++)
+                {
+                  if(t->suns[8 * k + subnode] < 0)
+                    break;
+                  subnode++;
+                  if(subnode >= 8)
+                    subnode = 7;
+                }
+            }
+          int nn = t->suns[8 * k + subnode];
+          if(nn >= 0)
+            {
+              parent = th;
+              th     = nn;
+            }
+          else
+            {
+              t->suns[8 * k + subnode] = i;
+              return 0;
+            }
+        }
+      else
+        {
+          /* the slot holds one particle: replace it by a new node (forcetree.c:893-929) */
+          int k = tree_new_node(t);
+          if(k < 0)
+            return -1;
+          t->suns[8 * (parent - N) + subnode] = N + k;
+          t->len[k]                            = (double)(mask << 1) * t->bigfac;
+          for(int d = 0; d < 3; d++)
+            t->center[3 * k + d] = (double)((ip[d] & (uint64_t)centermask) | mask) * t->bigfac + t->corner[d];
+          t->level[k]       = levels;
+          const uint64_t *jp = &t->intpos[3 * th];
+          unsigned sub2 = (unsigned)(((jp[0] & mask) >> shx) | ((jp[1] & mask) >> shy) | ((jp[2] & mask) >> shz));
+          t->suns[8 * k + sub2] = th;
+          th                     = N + k;
+        }
+    }
+}
+
+/* forcetree.c:1217-1429 */
+static void tree_update_recursive(orc_tree *t, int no, int sib, int father, int *last)
+{
+  const int N = t->n;
+  if(no >= N)
+    {
+      int k = no - N;
+      int suns[8];
+      memcpy(suns, &t->suns[8 * k], sizeof(suns));
+      if(*last >= 0)
+        {
+          if(*last >= N)
+            t->nextnode[*last - N] = no;
+          else
+            t->pnext[*last] = no;
+        }
+      *last = no;
+
+      double mass = 0, s[3] = {0, 0, 0};
+      int np = 0;
+      int maxsoft = t->nsoft; /* sentinel with zero softening (forcetree.c:1254) */
+      for(int j = 0; j < 8; j++)
+        {
+          int p = suns[j];
+          if(p < 0)
+            continue;
+          int jj, nextsib = sib;
+          for(jj = j + 1; jj < 8; jj++)
+            if(suns[jj] >= 0)
+              {
+                nextsib = suns[jj];
+                break;
+              }
+          tree_update_recursive(t, p, nextsib, no, last);
+          if(p < N)
+            {
+              double m = t->pmass[p];
+              mass += m;
+              for(int d = 0; d < 3; d++)
+                s[d] += m * t->pos[3 * p + d];
+              int st = t->stype ? t->stype[p] : 0;
+              if(t->force_softening[maxsoft] < t->force_softening[st])
+                maxsoft = st;
+              np++;
+            }
+          else
+            {
+              int c = p - N;
+              mass += t->mass[c];
+              for(int d = 0; d < 3; d++)
+                s[d] += t->mass[c] * t->s[3 * c + d];
+              if(t->force_softening[maxsoft] < t->force_softening[t->maxsoft[c]])
+                maxsoft = t->maxsoft[c];
+              np += t->npart[c];
+            }
+        }
+      if(mass)
+        for(int d = 0; d < 3; d++)
+          s[d] /= mass;
+      else
+        for(int d = 0; d < 3; d++)
+          s[d] = t->center[3 * k + d];
+      t->mass[k] = mass;
+      for(int d = 0; d < 3; d++)
+        t->s[3 * k + d] = s[d];
+      t->maxsoft[k] = (uint8_t)maxsoft;
+      t->sibling[k] = sib;
+      t->father[k]  = father;
+      t->npart[k]   = np;
+    }
+  else
+    {
+      if(*last >= 0)
+        {
+          if(*last >= N)
+            t->nextnode[*last - N] = no;
+          else
+            t->pnext[*last] = no;
+        }
+      *last = no;
+    }
+}
+
+void orc_tree_free(orc_tree *t)
+{
+  if(!t)
+    return;
+  free(t->suns);
+  free(t->center);
+  free(t->len);
+  free(t->s);
+  free(t->mass);
+  free(t->sibling);
+  free(t->nextnode);
+  free(t->father);
+  free(t->level);
+  free(t->npart);
+  free(t->maxsoft);
+  free(t->pnext);
+  free(t->intpos);
+  free(t->pos);
+  free(t);
+}
+
+int orc_tree_build(const orc_params *p, int n, const double *pos, const double *mass, const uint8_t *stype, double alloc_factor,
+                   orc_tree **out)
+{
+  *out = NULL;
+  if(n <= 0)
+    return 0;
+  orc_tree *t = (orc_tree *)calloc(1, sizeof(orc_tree));
+  t->n        = n;
+  t->maxnodes = (int)(alloc_factor * n) + 1;
+  if(t->maxnodes < 2)
+    t->maxnodes = 2;
+  int M       = t->maxnodes;
+  t->suns     = (int *)malloc(sizeof(int) * 8 * M);
+  t->center   = (double *)malloc(sizeof(double) * 3 * M);
+  t->len      = (double *)malloc(sizeof(double) * M);
+  t->s        = (double *)malloc(sizeof(double) * 3 * M);
+  t->mass     = (double *)malloc(sizeof(double) * M);
+  t->sibling  = (int *)malloc(sizeof(int) * M);
+  t->nextnode = (int *)malloc(sizeof(int) * M);
+  t->father   = (int *)malloc(sizeof(int) * M);
+  t->level    = (int *)malloc(sizeof(int) * M);
+  t->npart    = (int *)malloc(sizeof(int) * M);
+  t->maxsoft  = (uint8_t *)malloc(M);
+  t->pnext    = (int *)malloc(sizeof(int) * n);
+  t->intpos   = (uint64_t *)malloc(sizeof(uint64_t) * 3 * n);
+  t->pos      = (double *)malloc(sizeof(double) * 3 * n);
+  t->pmass    = mass;
+  t->stype    = stype;
+  t->nsoft    = p->nsoft;
+  t->periodic = p->periodic;
+  t->box      = p->box_size;
+  orc_set_softenings(p, t->force_softening);
+
+  orc_domain_extent(p, n, pos, t->corner, t->dcenter, &t->dlen);
+  t->bigfac   = t->dlen / (double)(1ll << 52);
+  double ilen = 1.0 / t->dlen;
+
+  /* wrap into the box and form integer coordinates (forcetree.c:394-417, :493-505) */
+  for(int i = 0; i < n; i++)
+    for(int d = 0; d < 3; d++)
+      {
+        double x = pos[3 * i + d];
+        if(p->periodic)
+          {
+            if(x < 0)
+              x += p->box_size;
+            if(x >= p->box_size)
+              x -= p->box_size;
+          }
+        t->pos[3 * i + d]    = x;
+        t->intpos[3 * i + d] = dbl_mantissa((x - t->corner[d]) * ilen + 1.0);
+      }
+
+  /* root node (forcetree.c:339-351) */
+  int root = tree_new_node(t);
+  t->len[root] = t->dlen;
+  for(int d = 0; d < 3; d++)
+    t->center[3 * root + d] = t->dcenter[d];
+  t->level[root] = 0;
+
+  for(int i = 0; i < n; i++)
+    if(tree_insert(t, i) < 0)
+      {
+        orc_tree_free(t);
+        return -1;
+      }
+
+  int last = -1;
+  tree_update_recursive(t, n + 0, -1, -1, &last);
+  if(last >= n)
+    t->nextnode[last - n] = -1;
+  else
+    t->pnext[last] = -1;
+
+  *out = t;
+  return t->numnodes;
+}
+
+int orc_tree_numnodes(const orc_tree *t) { return t ? t->numnodes : 0; }
+
+void orc_tree_export(const orc_tree *t, double *center, double *len, double *com, double *mass, int *level, int *npart)
+{
+  int M = t->numnodes;
+  if(center)
+    memcpy(center, t->center, sizeof(double) * 3 * M);
+  if(len)
+    memcpy(len, t->len, sizeof(double) * M);
+  if(com)
+    memcpy(com, t->s, sizeof(double) * 3 * M);
+  if(mass)
+    memcpy(mass, t->mass, sizeof(double) * M);
+  if(level)
+    memcpy(level, t->level, sizeof(int) * M);
+  if(npart)
+    memcpy(npart, t->npart, sizeof(int) * M);
+}
+
+/* ------------------------------------------------------------------ */
+/* Ewald tables                                                        */
+/* ------------------------------------------------------------------ */
+
+double orc_ewald_psi(double x, double y, double z)
+{
+  /* forcetree_ewald.c:393-450 (cubic unit box) */
+  double r = sqrt(x * x + y * y + z * z);
+  if(r == 0)
+    return 0;
+  const double alpha = 3.0;
+  const int nmax     = 4;
+  double sum1        = 0;
+  for(int nx = -nmax; nx <= nmax; nx++)
+    for(int ny = -nmax; ny <= nmax; ny++)
+      for(int nz = -nmax; nz <= nmax; nz++)
+        {
+          double dx = x - nx, dy = y - ny, dz = z - nz;
+          double rr = sqrt(dx * dx + dy * dy + dz * dz);
+          sum1 += erfc(alpha * rr) / rr;
+        }
+  double alpha2 = alpha * alpha;
+  int kmax      = (int)(2 * alpha + 0.5);
+  double sum2   = 0;
+  for(int nx = -kmax; nx <= kmax; nx++)
+    for(int ny = -kmax; ny <= kmax; ny++)
+      for(int nz = -kmax; nz <= kmax; nz++)
+        {
+          double kx = 2.0 * M_PI * nx, ky = 2.0 * M_PI * ny, kz = 2.0 * M_PI * nz;
+          double k2 = kx * kx + ky * ky + kz * kz;
+          if(k2 > 0)
+            sum2 += 4.0 * M_PI / k2 * exp(-k2 / (4.0 * alpha2)) * cos(x * kx + y * ky + z * kz);
+        }
+  return M_PI / (alpha * alpha) - sum1 - sum2 + 1.0 / r;
+}
+
+void orc_ewald_force(double x, double y, double z, double force[3])
+{
+  /* forcetree_ewald.c:463-527 (cubic unit box) */
+  force[0] = force[1] = force[2] = 0;
+  double r2 = x * x + y * y + z * z;
+  if(r2 == 0)
+    return;
+  const double alpha  = 2.0;
+  const double alpha2 = alpha * alpha;
+  double r3inv        = 1.0 / (r2 * sqrt(r2));
+  force[0] += r3inv * x;
+  force[1] += r3inv * y;
+  force[2] += r3inv * z;
+  const int nmax = 4;
+  for(int nx = -nmax; nx <= nmax; nx++)
+    for(int ny = -nmax; ny <= nmax; ny++)
+      for(int nz = -nmax; nz <= nmax; nz++)
+        {
+          double dx = x - nx, dy = y - ny, dz = z - nz;
+          double rr2 = dx * dx + dy * dy + dz * dz;
+          double rr  = sqrt(rr2);
+          double val = erfc(alpha * rr) + 2.0 * alpha * rr / sqrt(M_PI) * exp(-alpha2 * rr2);
+          double v2  = val / (rr2 * rr);
+          force[0] -= dx * v2;
+          force[1] -= dy * v2;
+          force[2] -= dz * v2;
+        }
+  int hmax = (int)(2 * alpha + 0.5);
+  for(int hx = -hmax; hx <= hmax; hx++)
+    for(int hy = -hmax; hy <= hmax; hy++)
+      for(int hz = -hmax; hz <= hmax; hz++)
+        {
+          double h2 = hx * hx + hy * hy + hz * hz;
+          if(h2 > 0)
+            {
+              double hdotx = x * hx + y * hy + z * hz;
+              double val   = 2.0 / h2 * exp(-M_PI * M_PI * h2 / alpha2) * sin(2.0 * M_PI * hdotx);
+              force[0] -= hx * val;
+              force[1] -= hy * val;
+              force[2] -= hz * val;
+            }
+        }
+}
+
+void orc_ewald_table_tree(int en, double box, double *tab, int nthreads)
+{
+  /* forcetree_ewald.c:158-175 (table on one octant) and :239-249 (scaling) */
+  int n1   = en + 1;
+  int size = n1 * n1 * n1;
+  (void)omp_threads;
+#pragma omp parallel for schedule(dynamic, 64) num_threads(omp_threads(nthreads))
+  for(int n = 0; n < size; n++)
+    {
+      int i = n / (n1 * n1), j = (n / n1) % n1, k = n % n1;
+      double xx = 0.5 * i / en, yy = 0.5 * j / en, zz = 0.5 * k / en;
+      double f[3];
+      orc_ewald_force(xx, yy, zz, f);
+      tab[0 * size + n] = f[0] / (box * box);
+      tab[1 * size + n] = f[1] / (box * box);
+      tab[2 * size + n] = f[2] / (box * box);
+      tab[3 * size + n] = orc_ewald_psi(xx, yy, zz) / box;
+    }
+}
+
+static void trilinear_setup(double fac_intp, int en, double dx, double dy, double dz, int *i, int *j, int *k, double w[8])
+{
+  double u = dx * fac_intp;
+  *i       = (int)u;
+  if(*i >= en)
+    *i = en - 1;
+  u -= *i;
+  double v = dy * fac_intp;
+  *j       = (int)v;
+  if(*j >= en)
+    *j = en - 1;
+  v -= *j;
+  double ww = dz * fac_intp;
+  *k        = (int)ww;
+  if(*k >= en)
+    *k = en - 1;
+  ww -= *k;
+  w[0] = (1 - u) * (1 - v) * (1 - ww);
+  w[1] = (1 - u) * (1 - v) * (ww);
+  w[2] = (1 - u) * (v) * (1 - ww);
+  w[3] = (1 - u) * (v) * (ww);
+  w[4] = (u) * (1 - v) * (1 - ww);
+  w[5] = (u) * (1 - v) * (ww);
+  w[6] = (u) * (v) * (1 - ww);
+  w[7] = (u) * (v) * (ww);
+}
+
+static double trilinear(const double *t, int n1, int i, int j, int k, const double w[8])
+{
+#define T(a, b, c) t[((a)*n1 + (b)) * n1 + (c)]
+  return T(i, j, k) * w[0] + T(i, j, k + 1) * w[1] + T(i, j + 1, k) * w[2] + T(i, j + 1, k + 1) * w[3] + T(i + 1, j, k) * w[4] +
+         T(i + 1, j, k + 1) * w[5] + T(i + 1, j + 1, k) * w[6] + T(i + 1, j + 1, k + 1) * w[7];
+#undef T
+}
+
+void orc_ewald_corr(const double *tab, int en, double box, double dx, double dy, double dz, double fcorr[3])
+{
+  /* forcetree_ewald.c:268-328 */
+  double sx = -1, sy = -1, sz = -1;
+  if(dx < 0)
+    {
+      dx = -dx;
+      sx = +1;
+    }
+  if(dy < 0)
+    {
+      dy = -dy;
+      sy = +1;
+    }
+  if(dz < 0)
+    {
+      dz = -dz;
+      sz = +1;
+    }
+  int i, j, k, n1 = en + 1, size = n1 * n1 * n1;
+  double w[8];
+  trilinear_setup(2.0 * en / box, en, dx, dy, dz, &i, &j, &k, w);
+  fcorr[0] = sx * trilinear(tab + 0 * size, n1, i, j, k, w);
+  fcorr[1] = sy * trilinear(tab + 1 * size, n1, i, j, k, w);
+  fcorr[2] = sz * trilinear(tab + 2 * size, n1, i, j, k, w);
+}
+
+double orc_ewald_pot_corr(const double *tab, int en, double box, double dx, double dy, double dz)
+{
+  /* forcetree_ewald.c:344-382 */
+  int i, j, k, n1 = en + 1, size = n1 * n1 * n1;
+  double w[8];
+  trilinear_setup(2.0 * en / box, en, fabs(dx), fabs(dy), fabs(dz), &i, &j, &k, w);
+  return trilinear(tab + 3 * size, n1, i, j, k, w);
+}
+
+void orc_forcetest_ewald_exact(double box, double x, double y, double z, double force[4])
+{
+  /* gravtree_forcetest.c:820-916 */
+  for(int i = 0; i < 4; i++)
+    force[i] = 0;
+  double r2 = x * x + y * y + z * z;
+  if(r2 == 0)
+    return;
+  double sx = -1, sy = -1, sz = -1;
+  if(x < 0)
+    {
+      x  = -x;
+      sx = +1;
+    }
+  if(y < 0)
+    {
+      y  = -y;
+      sy = +1;
+    }
+  if(z < 0)
+    {
+      z  = -z;
+      sz = +1;
+    }
+  double alpha  = 2.0 / box;
+  double alpha2 = alpha * alpha;
+  double r      = sqrt(r2);
+  double r3inv  = 1.0 / (r2 * r);
+  force[0] += r3inv * x;
+  force[1] += r3inv * y;
+  force[2] += r3inv * z;
+  const int nmax = 6;
+  for(int nx = -nmax; nx <= nmax; nx++)
+    for(int ny = -nmax; ny <= nmax; ny++)
+      for(int nz = -nmax; nz <= nmax; nz++)
+        {
+          double dx = x - nx * box, dy = y - ny * box, dz = z - nz * box;
+          double rr2 = dx * dx + dy * dy + dz * dz;
+          double rr  = sqrt(rr2);
+          double ea  = erfc(alpha * rr);
+          double val = ea + 2.0 * alpha * rr / sqrt(M_PI) * exp(-alpha2 * rr2);
+          double v2  = val / (rr2 * rr);
+          force[0] -= dx * v2;
+          force[1] -= dy * v2;
+          force[2] -= dz * v2;
+          force[3] -= ea / rr;
+        }
+  int kmax = (int)(4 * alpha * box + 0.5);
+  double vol = box * box * box;
+  for(int nx = -kmax; nx <= kmax; nx++)
+    for(int ny = -kmax; ny <= kmax; ny++)
+      for(int nz = -kmax; nz <= kmax; nz++)
+        {
+          double kx = (2.0 * M_PI / box) * nx, ky = (2.0 * M_PI / box) * ny, kz = (2.0 * M_PI / box) * nz;
+          double k2 = kx * kx + ky * ky + kz * kz;
+          if(k2 > 0)
+            {
+              double kdotx = x * kx + y * ky + z * kz;
+              double vv    = 4.0 * M_PI / (k2 * vol) * exp(-k2 / (4.0 * alpha2));
+              double val   = vv * sin(kdotx);
+              force[0] -= kx * val;
+              force[1] -= ky * val;
+              force[2] -= kz * val;
+              force[3] -= vv * cos(kdotx);
+            }
+        }
+  force[3] += M_PI / (alpha2 * vol) + 1.0 / r;
+  force[0] *= sx;
+  force[1] *= sy;
+  force[2] *= sz;
+}
+
+void orc_forcetest_ewald_table(int tew_n, double box, double *tab, int nthreads)
+{
+  /* gravtree_forcetest.c:933-1008 */
+  int n1   = tew_n + 1;
+  int size = n1 * n1 * n1;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(omp_threads(nthreads))
+  for(int n = 0; n < size; n++)
+    {
+      int i = n / (n1 * n1), j = (n / n1) % n1, k = n % n1;
+      double f[4];
+      orc_forcetest_ewald_exact(box, 0.5 * i / tew_n * box, 0.5 * j / tew_n * box, 0.5 * k / tew_n * box, f);
+      for(int c = 0; c < 4; c++)
+        tab[c * size + n] = f[c];
+    }
+}
+
+void orc_forcetest_ewald_lookup(const double *tab, int tew_n, double box, double dx, double dy, double dz, double force[4])
+{
+  /* gravtree_forcetest.c:1019-1085 */
+  double sx = 1, sy = 1, sz = 1;
+  if(dx < 0)
+    {
+      dx = -dx;
+      sx = -1;
+    }
+  if(dy < 0)
+    {
+      dy = -dy;
+      sy = -1;
+    }
+  if(dz < 0)
+    {
+      dz = -dz;
+      sz = -1;
+    }
+  int i, j, k, n1 = tew_n + 1, size = n1 * n1 * n1;
+  double w[8];
+  trilinear_setup(2.0 * tew_n / box, tew_n, dx, dy, dz, &i, &j, &k, w);
+  for(int c = 0; c < 4; c++)
+    force[c] = trilinear(tab + c * size, n1, i, j, k, w);
+  force[0] *= sx;
+  force[1] *= sy;
+  force[2] *= sz;
+}
+
+/* ------------------------------------------------------------------ */
+/* tree walk                                                           */
+/* ------------------------------------------------------------------ */
+
+typedef struct
+{
+  int pm;
+  double rcut2, rcut, asmthinv2, asmthfac;
+  float tf[ORC_NTAB + 1], tp[ORC_NTAB + 1];
+} shortrange_ctx;
+
+static void shortrange_setup(const orc_params *p, shortrange_ctx *c)
+{
+  c->pm = p->pmgrid > 0;
+  if(!c->pm)
+    return;
+  double asmth, rcut;
+  orc_asmth_rcut(p, &asmth, &rcut);
+  c->rcut      = rcut;
+  c->rcut2     = rcut * rcut;
+  double ai    = 0.5 / asmth;
+  c->asmthinv2 = ai * ai;
+  c->asmthfac  = ai * (ORC_NTAB / (p->rcut / 2.0));
+  orc_short_range_tables(p->rcut, c->tf, c->tp);
+}
+
+/* pair kernel of forcetree_walk.c:364-487; returns 0 when beyond the table (no interaction) */
+static inline int pair_kernel(const shortrange_ctx *c, double r, double hmax, double *fac_out, double *wp_out)
+{
+  double ff = 0, fp = 0;
+  if(c->pm)
+    {
+      double tabentry = c->asmthfac * r;
+      int ti          = (int)tabentry;
+      if(ti >= ORC_NTAB)
+        return 0;
+      double w = tabentry - ti;
+      ff       = (1.0 - w) * c->tf[ti] + w * c->tf[ti + 1];
+      fp       = (1.0 - w) * c->tp[ti] + w * c->tp[ti + 1];
+    }
+  double fac, wp;
+  if(r >= hmax)
+    {
+      double rinv  = 1.0 / r;
+      double rinv3 = rinv * rinv * rinv;
+      if(c->pm)
+        {
+          fac = rinv3 + rinv * ff * c->asmthinv2;
+          wp  = rinv * fp;
+        }
+      else
+        {
+          fac = rinv3;
+          wp  = -rinv;
+        }
+    }
+  else
+    {
+      double h_inv  = 1.0 / hmax;
+      double h3_inv = h_inv * h_inv * h_inv;
+      double u      = r * h_inv;
+      if(u < 0.5)
+        {
+          double u2 = u * u;
+          fac       = h3_inv * (SF1 + u2 * (SF2 * u + SF3));
+          wp        = h_inv * (SF4 + u2 * (SF5 + u2 * (SF6 * u + SF7)));
+        }
+      else
+        {
+          double u2 = u * u, u3 = u2 * u;
+          fac = h3_inv * (SF8 + SF9 * u + SF10 * u2 + SF11 * u3 + SF12 / u3);
+          wp  = h_inv * (SF13 + SF14 / u + u2 * (SF1 + u * (SF15 + u * (SF16 + SF17 * u))));
+        }
+      if(c->pm && r > 0)
+        {
+          double rinv = 1.0 / r;
+          fac += rinv * ff * c->asmthinv2;
+          wp += rinv * (fp + 1.0);
+        }
+    }
+  *fac_out = fac;
+  *wp_out  = wp;
+  return 1;
+}
+
+/* force_treeevaluate, forcetree_walk.c:115-514, mode 0, one rank */
+static int walk_one(const orc_params *p, const orc_tree *t, const shortrange_ctx *c, int use_bh, double aold_abs,
+                    const double *ewald_tab, const double pos[3], int stype, double out_acc[3], double *out_pot)
+{
+  const int N    = t->n;
+  const int per  = p->periodic;
+  const double L = p->box_size;
+  double ax = 0, ay = 0, az = 0, pot = 0;
+  int nint      = 0;
+  double aold   = p->err_tol_force_acc * aold_abs;
+  double h_i    = t->force_softening[stype];
+  double theta2 = p->err_tol_theta * p->err_tol_theta;
+  int no        = N + 0;
+
+  while(no >= 0)
+    {
+      double dx, dy, dz, r2, mass, hmax;
+      if(no < N)
+        {
+          dx   = nearest(t->pos[3 * no + 0] - pos[0], L, per);
+          dy   = nearest(t->pos[3 * no + 1] - pos[1], L, per);
+          dz   = nearest(t->pos[3 * no + 2] - pos[2], L, per);
+          r2   = dx * dx + dy * dy + dz * dz;
+          mass = t->pmass[no];
+          double h_j = t->force_softening[t->stype ? t->stype[no] : 0];
+          hmax       = h_j > h_i ? h_j : h_i;
+          no         = t->pnext[no];
+        }
+      else
+        {
+          int k     = no - N;
+          mass      = t->mass[k];
+          dx        = nearest(t->s[3 * k + 0] - pos[0], L, per);
+          dy        = nearest(t->s[3 * k + 1] - pos[1], L, per);
+          dz        = nearest(t->s[3 * k + 2] - pos[2], L, per);
+          r2        = dx * dx + dy * dy + dz * dz;
+          double ln = t->len[k];
+          if(c->pm && r2 > c->rcut2)
+            {
+              double eff = c->rcut + 0.5 * ln;
+              int skip   = 0;
+              for(int d = 0; d < 3 && !skip; d++)
+                {
+                  double dist = nearest(t->center[3 * k + d] - pos[d], L, per);
+                  if(dist < -eff || dist > eff)
+                    skip = 1;
+                }
+              if(skip)
+                {
+                  no = t->sibling[k];
+                  continue;
+                }
+            }
+          if(use_bh)
+            {
+              if(ln * ln > r2 * theta2)
+                {
+                  no = t->nextnode[k];
+                  continue;
+                }
+            }
+          else
+            {
+              double len2 = ln * ln;
+              if(len2 > r2 * (1.2 * 1.2))
+                {
+                  no = t->nextnode[k];
+                  continue;
+                }
+              if(mass * len2 > r2 * r2 * aold)
+                {
+                  no = t->nextnode[k];
+                  continue;
+                }
+              if(fabs(nearest(t->center[3 * k + 0] - pos[0], L, per)) < 0.60 * ln &&
+                 fabs(nearest(t->center[3 * k + 1] - pos[1], L, per)) < 0.60 * ln &&
+                 fabs(nearest(t->center[3 * k + 2] - pos[2], L, per)) < 0.60 * ln)
+                {
+                  no = t->nextnode[k];
+                  continue;
+                }
+            }
+          double h_j = t->force_softening[t->maxsoft[k]];
+          if(h_j > h_i)
+            {
+              if(r2 < h_j * h_j)
+                {
+                  no = t->nextnode[k];
+                  continue;
+                }
+              hmax = h_j;
+            }
+          else
+            hmax = h_i;
+          no = t->sibling[k];
+        }
+
+      if(mass)
+        {
+          double r = sqrt(r2), fac, wp;
+          if(!pair_kernel(c, r, hmax, &fac, &wp))
+            continue;
+          pot += mass * wp;
+          fac *= mass;
+          ax += dx * fac;
+          ay += dy * fac;
+          az += dz * fac;
+          if(ewald_tab)
+            {
+              double fc[3];
+              orc_ewald_corr(ewald_tab, 64, L, dx, dy, dz, fc);
+              ax += mass * fc[0];
+              ay += mass * fc[1];
+              az += mass * fc[2];
+              pot += mass * orc_ewald_pot_corr(ewald_tab, 64, L, dx, dy, dz);
+            }
+          nint++;
+        }
+    }
+  out_acc[0] = ax;
+  out_acc[1] = ay;
+  out_acc[2] = az;
+  *out_pot   = pot;
+  return nint;
+}
+
+long long orc_tree_walk(const orc_params *p, const orc_tree *t, int use_bh, const float *old_acc, const double *ewald_tab,
+                        int ntarget, const int *targets, double *acc, double *pot, int *nint, int nthreads)
+{
+  shortrange_ctx c;
+  shortrange_setup(p, &c);
+  long long total = 0;
+  int nt          = targets ? ntarget : t->n;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : total) num_threads(omp_threads(nthreads))
+  for(int q = 0; q < nt; q++)
+    {
+      int i = targets ? targets[q] : q;
+      double a[3], pt;
+      double aold = old_acc ? old_acc[i] : 0;
+      int ni = walk_one(p, t, &c, use_bh, aold, ewald_tab, &t->pos[3 * i], t->stype ? t->stype[i] : 0, a, &pt);
+      acc[3 * q + 0] = a[0];
+      acc[3 * q + 1] = a[1];
+      acc[3 * q + 2] = a[2];
+      pot[q]         = pt;
+      if(nint)
+        nint[q] = ni;
+      total += ni;
+    }
+  return total;
+}
+
+/* ------------------------------------------------------------------ */
+/* FFT and periodic PM                                                 */
+/* ------------------------------------------------------------------ */
+
+static void fft1d(int n, double *re, double *im, int stride, int sign)
+{
+  /* iterative radix-2 Cooley-Tukey, unnormalised, exp(sign*2*pi*i*j*k/n) */
+  for(int i = 1, j = 0; i < n; i++)
+    {
+      int bit = n >> 1;
+      for(; j & bit; bit >>= 1)
+        j ^= bit;
+      j ^= bit;
+      if(i < j)
+        {
+          double tr = re[i * stride], ti = im[i * stride];
+          re[i * stride] = re[j * stride];
+          im[i * stride] = im[j * stride];
+          re[j * stride] = tr;
+          im[j * stride] = ti;
+        }
+    }
+  for(int len = 2; len <= n; len <<= 1)
+    {
+      double ang = sign * 2.0 * M_PI / len;
+      for(int i = 0; i < n; i += len)
+        for(int k = 0; k < len / 2; k++)
+          {
+            double wr = cos(ang * k), wi = sin(ang * k);
+            int a = (i + k) * stride, b = (i + k + len / 2) * stride;
+            double xr = re[b] * wr - im[b] * wi;
+            double xi = re[b] * wi + im[b] * wr;
+            re[b]     = re[a] - xr;
+            im[b]     = im[a] - xi;
+            re[a] += xr;
+            im[a] += xi;
+          }
+    }
+}
+
+void orc_fft3d(int n, double *re, double *im, int sign)
+{
+  size_t n2 = (size_t)n * n;
+  for(size_t a = 0; a < n2; a++) /* along z (contiguous) */
+    fft1d(n, re + a * n, im + a * n, 1, sign);
+  for(int x = 0; x < n; x++) /* along y */
+    for(int z = 0; z < n; z++)
+      fft1d(n, re + (size_t)x * n2 + z, im + (size_t)x * n2 + z, n, sign);
+  for(size_t yz = 0; yz < n2; yz++) /* along x */
+    fft1d(n, re + yz, im + yz, (int)n2, sign);
+}
+
+int orc_pm_periodic(const orc_params *p, int n, const double *pos, const double *mass, double *acc_pm, double *pot_pm)
+{
+  const int G    = p->pmgrid;
+  const double L = p->box_size;
+  if(G <= 0 || (G & (G - 1)))
+    return -1;
+  size_t G3  = (size_t)G * G * G;
+  double *re = (double *)calloc(G3, sizeof(double));
+  double *im = (double *)calloc(G3, sizeof(double));
+  double *fg = (double *)malloc(G3 * sizeof(double));
+  if(!re || !im || !fg)
+    return -2;
+#define IDX(x, y, z) (((size_t)(x)*G + (y)) * G + (z))
+  double to_slab = G / L;
+
+  /* CIC deposit, pm_periodic.c:1049-1170 */
+  for(int i = 0; i < n; i++)
+    {
+      double x[3];
+      int s[3], ss[3];
+      double d[3];
+      for(int k = 0; k < 3; k++)
+        {
+          x[k] = pos[3 * i + k];
+          if(x[k] < 0)
+            x[k] += L;
+          if(x[k] >= L)
+            x[k] -= L;
+          s[k] = (int)(to_slab * x[k]);
+          d[k] = to_slab * x[k] - s[k];
+          if(s[k] >= G)
+            s[k] -= G;
+          ss[k] = s[k] + 1;
+          if(ss[k] >= G)
+            ss[k] -= G;
+        }
+      double m = mass[i];
+      re[IDX(s[0], s[1], s[2])] += m * (1.0 - d[0]) * (1.0 - d[1]) * (1.0 - d[2]);
+      re[IDX(s[0], s[1], ss[2])] += m * (1.0 - d[0]) * (1.0 - d[1]) * (d[2]);
+      re[IDX(s[0], ss[1], s[2])] += m * (1.0 - d[0]) * (d[1]) * (1.0 - d[2]);
+      re[IDX(s[0], ss[1], ss[2])] += m * (1.0 - d[0]) * (d[1]) * (d[2]);
+      re[IDX(ss[0], s[1], s[2])] += m * (d[0]) * (1.0 - d[1]) * (1.0 - d[2]);
+      re[IDX(ss[0], s[1], ss[2])] += m * (d[0]) * (1.0 - d[1]) * (d[2]);
+      re[IDX(ss[0], ss[1], s[2])] += m * (d[0]) * (d[1]) * (1.0 - d[2]);
+      re[IDX(ss[0], ss[1], ss[2])] += m * (d[0]) * (d[1]) * (d[2]);
+    }
+
+  orc_fft3d(G, re, im, -1);
+
+  /* Green's function and CIC deconvolution, pm_periodic.c:1626-1707 */
+  double asmth, rcut;
+  orc_asmth_rcut(p, &asmth, &rcut);
+  double asmth2 = asmth * asmth;
+  double dcell  = L / G;
+  double dhalf  = 0.5 * dcell;
+  double kfac   = 2.0 * M_PI / (G * dcell);
+  for(int x = 0; x < G; x++)
+    for(int y = 0; y < G; y++)
+      for(int z = 0; z < G; z++)
+        {
+          int xx = x >= G / 2 ? x - G : x, yy = y >= G / 2 ? y - G : y, zz = z >= G / 2 ? z - G : z;
+          double kx = kfac * xx, ky = kfac * yy, kz = kfac * zz;
+          double k2 = kx * kx + ky * ky + kz * kz;
+          size_t ip = IDX(x, y, z);
+          if(k2 > 0)
+            {
+              double smth = -exp(-k2 * asmth2) / k2;
+              double fx = 1, fy = 1, fz = 1;
+              if(xx != 0)
+                {
+                  fx = kx * dhalf;
+                  fx = sin(fx) / fx;
+                }
+              if(yy != 0)
+                {
+                  fy = ky * dhalf;
+                  fy = sin(fy) / fy;
+                }
+              if(zz != 0)
+                {
+                  fz = kz * dhalf;
+                  fz = sin(fz) / fz;
+                }
+              double ff = 1 / (fx * fy * fz);
+              smth *= ff * ff * ff * ff;
+              re[ip] *= smth;
+              im[ip] *= smth;
+            }
+          else
+            re[ip] = im[ip] = 0;
+        }
+
+  orc_fft3d(G, re, im, +1); /* re now holds the (unnormalised) potential */
+
+  double fac_pot = 4 * M_PI * p->G / (L * L * L);
+  double fac_fd  = fac_pot / (2 * dcell);
+
+  /* readout helper, pm_periodic.c:1322-1550 */
+  for(int i = 0; i < n; i++)
+    {
+      acc_pm[3 * i + 0] = acc_pm[3 * i + 1] = acc_pm[3 * i + 2] = 0;
+      pot_pm[i]                                                = 0;
+    }
+  for(int dim = -1; dim < 3; dim++)
+    {
+      const double *grid = re;
+      if(dim >= 0)
+        {
+          /* 4-point finite difference, pm_periodic.c:1731-1800 */
+          for(int x = 0; x < G; x++)
+            for(int y = 0; y < G; y++)
+              for(int z = 0; z < G; z++)
+                {
+                  int c[3] = {x, y, z}, l[3], r[3], ll[3], rr[3];
+                  for(int k = 0; k < 3; k++)
+                    l[k] = r[k] = ll[k] = rr[k] = c[k];
+                  l[dim]  = (c[dim] - 1 + G) % G;
+                  r[dim]  = (c[dim] + 1) % G;
+                  ll[dim] = (c[dim] - 2 + G) % G;
+                  rr[dim] = (c[dim] + 2) % G;
+                  fg[IDX(x, y, z)] = fac_fd * ((4.0 / 3) * (re[IDX(l[0], l[1], l[2])] - re[IDX(r[0], r[1], r[2])]) -
+                                               (1.0 / 6) * (re[IDX(ll[0], ll[1], ll[2])] - re[IDX(rr[0], rr[1], rr[2])]));
+                }
+          grid = fg;
+        }
+      for(int i = 0; i < n; i++)
+        {
+          double x[3];
+          int s[3], ss[3];
+          double d[3];
+          for(int k = 0; k < 3; k++)
+            {
+              x[k] = pos[3 * i + k];
+              if(x[k] < 0)
+                x[k] += L;
+              if(x[k] >= L)
+                x[k] -= L;
+              s[k] = (int)(to_slab * x[k]);
+              d[k] = to_slab * x[k] - s[k];
+              if(s[k] >= G)
+                s[k] -= G;
+              ss[k] = s[k] + 1;
+              if(ss[k] >= G)
+                ss[k] -= G;
+            }
+          double v = grid[IDX(s[0], s[1], s[2])] * (1.0 - d[0]) * (1.0 - d[1]) * (1.0 - d[2]) +
+                     grid[IDX(s[0], s[1], ss[2])] * (1.0 - d[0]) * (1.0 - d[1]) * (d[2]) +
+                     grid[IDX(s[0], ss[1], s[2])] * (1.0 - d[0]) * (d[1]) * (1.0 - d[2]) +
+                     grid[IDX(s[0], ss[1], ss[2])] * (1.0 - d[0]) * (d[1]) * (d[2]) +
+                     grid[IDX(ss[0], s[1], s[2])] * (d[0]) * (1.0 - d[1]) * (1.0 - d[2]) +
+                     grid[IDX(ss[0], s[1], ss[2])] * (d[0]) * (1.0 - d[1]) * (d[2]) +
+                     grid[IDX(ss[0], ss[1], s[2])] * (d[0]) * (d[1]) * (1.0 - d[2]) +
+                     grid[IDX(ss[0], ss[1], ss[2])] * (d[0]) * (d[1]) * (d[2]);
+          if(dim < 0)
+            pot_pm[i] += v * fac_pot;
+          else
+            acc_pm[3 * i + dim] += v;
+        }
+    }
+#undef IDX
+  free(re);
+  free(im);
+  free(fg);
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* FORCETEST direct summation                                          */
+/* ------------------------------------------------------------------ */
+
+void orc_forcetest(const orc_params *p, int n, const double *pos, const double *mass, const uint8_t *stype, int ns,
+                   const int *sel, int ewald, const double *tab, int tew_n, double *acc, double *acc_short, double *pot,
+                   double *pot_short, int nthreads)
+{
+  double fs[ORC_MAXSOFT + 1];
+  orc_set_softenings(p, fs);
+  double asmth = 0, rcut = 0;
+  if(p->pmgrid > 0)
+    orc_asmth_rcut(p, &asmth, &rcut);
+  const double L   = p->box_size;
+  const int per    = p->periodic;
+  const double vol = L * L * L;
+#pragma omp parallel for schedule(dynamic, 4) num_threads(omp_threads(nthreads))
+  for(int q = 0; q < ns; q++)
+    {
+      int i      = sel[q];
+      double h_i = fs[stype ? stype[i] : 0];
+      double a[3] = {0, 0, 0}, as[3] = {0, 0, 0}, pt = 0, pts = 0;
+      for(int j = 0; j < n; j++)
+        {
+          double h_j  = fs[stype ? stype[j] : 0];
+          double hmax = h_j > h_i ? h_j : h_i;
+          double dx   = nearest(pos[3 * j + 0] - pos[3 * i + 0], L, per);
+          double dy   = nearest(pos[3 * j + 1] - pos[3 * i + 1], L, per);
+          double dz   = nearest(pos[3 * j + 2] - pos[3 * i + 2], L, per);
+          double r2   = dx * dx + dy * dy + dz * dz;
+          double m    = mass[j];
+          double r    = sqrt(r2);
+          double fac_newton = 0, wp_newton = 0, fac, wp;
+          if(r > 0)
+            {
+              fac_newton = m / (r2 * r);
+              wp_newton  = -m / r;
+            }
+          if(r >= hmax)
+            {
+              fac = fac_newton;
+              wp  = wp_newton;
+            }
+          else
+            {
+              double h_inv = 1.0 / hmax, h3_inv = h_inv * h_inv * h_inv, u = r * h_inv;
+              if(u < 0.5)
+                {
+                  double u2 = u * u;
+                  fac       = m * h3_inv * (SF1 + u2 * (SF2 * u + SF3));
+                  wp        = m * h_inv * (SF4 + u2 * (SF5 + u2 * (SF6 * u + SF7)));
+                }
+              else
+                {
+                  double u2 = u * u, u3 = u2 * u;
+                  fac = m * h3_inv * (SF8 + SF9 * u + SF10 * u2 + SF11 * u3 + SF12 / u3);
+                  wp  = m * h_inv * (SF13 + SF14 / u + u2 * (SF1 + u * (SF15 + u * (SF16 + SF17 * u))));
+                }
+            }
+          double anx = dx * fac, any = dy * fac, anz = dz * fac, pn = wp;
+          if(p->pmgrid > 0)
+            {
+              double u  = 0.5 / asmth * r;
+              double ff = erfc(u) + 2.0 * u / sqrt(M_PI) * exp(-u * u) - 1.0;
+              double fp = erfc(u);
+              double fs_ = fac + fac_newton * ff;
+              double ws  = wp + wp_newton * (fp - 1.0);
+              as[0] += dx * fs_;
+              as[1] += dy * fs_;
+              as[2] += dz * fs_;
+              pts += ws + m * M_PI / (asmth * asmth * vol);
+            }
+          double fc[4] = {0, 0, 0, 0};
+          if(per && ewald == 1)
+            orc_forcetest_ewald_lookup(tab, tew_n, L, dx, dy, dz, fc);
+          else if(per && ewald == 2)
+            {
+              /* the exact series in the table's sign convention (gravtree_forcetest.c:1040-1084 flips
+               * the sign relative to the raw series) */
+              orc_forcetest_ewald_exact(L, fabs(dx), fabs(dy), fabs(dz), fc);
+              fc[0] *= dx < 0 ? -1 : 1;
+              fc[1] *= dy < 0 ? -1 : 1;
+              fc[2] *= dz < 0 ? -1 : 1;
+            }
+          a[0] += anx + m * fc[0];
+          a[1] += any + m * fc[1];
+          a[2] += anz + m * fc[2];
+          pt += pn + m * fc[3];
+        }
+      if(acc)
+        for(int d = 0; d < 3; d++)
+          acc[3 * q + d] = a[d];
+      if(acc_short)
+        for(int d = 0; d < 3; d++)
+          acc_short[3 * q + d] = as[d];
+      if(pot)
+        pot[q] = pt;
+      if(pot_short)
+        pot_short[q] = pts;
+    }
+}
+
+void orc_direct_summation(const orc_params *p, int n, const double *pos, const double *mass, const uint8_t *stype, double *acc,
+                          double *pot, int nthreads)
+{
+  /* force_evaluate_direct, forcetree_walk.c:557-708 */
+  shortrange_ctx c;
+  shortrange_setup(p, &c);
+  double fs[ORC_MAXSOFT + 1];
+  orc_set_softenings(p, fs);
+  const double L = p->box_size;
+  const int per  = p->periodic;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(omp_threads(nthreads))
+  for(int i = 0; i < n; i++)
+    {
+      double h_i = fs[stype ? stype[i] : 0];
+      double ax = 0, ay = 0, az = 0, pt = 0;
+      for(int j = 0; j < n; j++)
+        {
+          double h_j  = fs[stype ? stype[j] : 0];
+          double hmax = h_j > h_i ? h_j : h_i;
+          double dx   = nearest(pos[3 * j + 0] - pos[3 * i + 0], L, per);
+          double dy   = nearest(pos[3 * j + 1] - pos[3 * i + 1], L, per);
+          double dz   = nearest(pos[3 * j + 2] - pos[3 * i + 2], L, per);
+          double r    = sqrt(dx * dx + dy * dy + dz * dz), fac, wp;
+          if(!pair_kernel(&c, r, hmax, &fac, &wp))
+            continue;
+          pt += mass[j] * wp;
+          fac *= mass[j];
+          ax += dx * fac;
+          ay += dy * fac;
+          az += dz * fac;
+        }
+      acc[3 * i + 0] = ax;
+      acc[3 * i + 1] = ay;
+      acc[3 * i + 2] = az;
+      pot[i]         = pt;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* driver: accel.c + longrange.c                                       */
+/* ------------------------------------------------------------------ */
+
+static double wall(void)
+{
+#ifdef _OPENMP
+  return omp_get_wtime();
+#else
+  return 0;
+#endif
+}
+
+long long orc_compute_grav_accelerations(const orc_params *p, int n, const double *pos, const double *mass, const uint8_t *stype,
+                                         int first_step, float *old_acc, double *acc, double *pot, int *nint, int nthreads,
+                                         double *t_out)
+{
+  if(n <= 0)
+    return 0;
+  double *acc_pm = (double *)calloc(3 * (size_t)n, sizeof(double));
+  double *pot_pm = (double *)calloc((size_t)n, sizeof(double));
+  double *at     = (double *)malloc(3 * sizeof(double) * n);
+  double *pt     = (double *)malloc(sizeof(double) * n);
+  float *oa      = old_acc ? old_acc : (float *)calloc((size_t)n, sizeof(float));
+  double *ewald  = NULL;
+  if(p->periodic && p->pmgrid == 0)
+    {
+      ewald = (double *)malloc(sizeof(double) * 4 * 65 * 65 * 65);
+      orc_ewald_table_tree(64, p->box_size, ewald, nthreads);
+    }
+  double m_total = 0;
+  for(int i = 0; i < n; i++)
+    m_total += mass[i];
+  double asmth = 0, rcut = 0;
+  if(p->pmgrid > 0)
+    orc_asmth_rcut(p, &asmth, &rcut);
+
+  int passes      = (p->opening == 1 && first_step && p->err_tol_theta > 0) ? 2 : 1;
+  long long total = 0;
+  double tb = 0, tw = 0, tpm = 0;
+  for(int pass = 0; pass < passes; pass++)
+    {
+      int use_bh = (p->opening == 0) || (pass == 0 && passes == 2);
+      double t0  = wall();
+      if(p->pmgrid > 0 && pass == 0)
+        orc_pm_periodic(p, n, pos, mass, acc_pm, pot_pm);
+      double t1 = wall();
+      if(pass == 0)
+        tpm = t1 - t0;
+      orc_tree *t = NULL;
+      double af   = n < 1000 ? 10.0 : 0.7; /* init.c:110-114 */
+      int rc;
+      while((rc = orc_tree_build(p, n, pos, mass, stype, af, &t)) < 0)
+        {
+          af *= 1.15; /* forcetree.c:210 */
+          if(af > 30.0)
+            return -1;
+        }
+      double t2 = wall();
+      total     = orc_tree_walk(p, t, use_bh, oa, ewald, n, NULL, at, pt, nint, nthreads);
+      double t3 = wall();
+      orc_tree_free(t);
+      /* gravity_force_finalize, accel.c:204-347 */
+      for(int i = 0; i < n; i++)
+        {
+          double ax = at[3 * i + 0] + acc_pm[3 * i + 0] / p->G;
+          double ay = at[3 * i + 1] + acc_pm[3 * i + 1] / p->G;
+          double az = at[3 * i + 2] + acc_pm[3 * i + 2] / p->G;
+          oa[i]     = (float)sqrt(ax * ax + ay * ay + az * az);
+          for(int d = 0; d < 3; d++)
+            acc[3 * i + d] = at[3 * i + d] * p->G + acc_pm[3 * i + d];
+          double pp = pt[i];
+          if(p->pmgrid > 0 && p->periodic)
+            pp += m_total * M_PI / (asmth * asmth * p->box_size * p->box_size * p->box_size);
+          pot[i] = pp * p->G + pot_pm[i];
+        }
+      tb = t2 - t1;
+      tw = t3 - t2;
+    }
+  if(t_out)
+    {
+      t_out[0] = tb;
+      t_out[1] = tw;
+      t_out[2] = tpm;
+    }
+  free(acc_pm);
+  free(pot_pm);
+  free(at);
+  free(pt);
+  free(ewald);
+  if(!old_acc)
+    free(oa);
+  return total;
+}
+
+/* ------------------------------------------------------------------ */
+/* unbinding potential: subfind_loctree.c:564-730                      */
+/* ------------------------------------------------------------------ */
+
+long long orc_group_potential_bh(int n, const double *pos, const double *mass, double h, double theta, double *pot)
+{
+  orc_params p;
+  memset(&p, 0, sizeof(p));
+  p.periodic = 0;
+  p.box_size = 1;
+  p.nsoft    = 1;
+  p.soft[0]  = h / 2.8;
+  orc_tree *t = NULL;
+  double af   = n < 1000 ? 10.0 : 0.7;
+  while(orc_tree_build(&p, n, pos, mass, NULL, af, &t) < 0)
+    af *= 1.15;
+  if(!t)
+    return 0;
+  long long total = 0;
+  const int N     = n;
+  double theta2   = theta * theta;
+  for(int i = 0; i < n; i++)
+    {
+      const double *x = &t->pos[3 * i];
+      double pt       = 0;
+      int no          = N;
+      while(no >= 0)
+        {
+          double dx, dy, dz, r2, m;
+          if(no < N)
+            {
+              dx = t->pos[3 * no] - x[0];
+              dy = t->pos[3 * no + 1] - x[1];
+              dz = t->pos[3 * no + 2] - x[2];
+              m  = mass[no];
+              no = t->pnext[no];
+            }
+          else
+            {
+              int k = no - N;
+              m     = t->mass[k];
+              dx    = t->s[3 * k] - x[0];
+              dy    = t->s[3 * k + 1] - x[1];
+              dz    = t->s[3 * k + 2] - x[2];
+              r2    = dx * dx + dy * dy + dz * dz;
+              if(t->len[k] * t->len[k] > r2 * theta2 && m)
+                {
+                  no = t->nextnode[k];
+                  continue;
+                }
+              no = t->sibling[k];
+            }
+          r2       = dx * dx + dy * dy + dz * dz;
+          double r = sqrt(r2);
+          if(r >= h)
+            pt -= m / r;
+          else
+            {
+              double hi = 1.0 / h, u = r * hi, wp;
+              if(u < 0.5)
+                wp = -2.8 + u * u * (5.333333333333 + u * u * (6.4 * u - 9.6));
+              else
+                wp = -3.2 + 0.066666666667 / u + u * u * (10.666666666667 + u * (-16.0 + u * (9.6 - 2.133333333333 * u)));
+              pt += m * hi * wp;
+            }
+          total++;
+        }
+      pot[i] = pt;
+    }
+  orc_tree_free(t);
+  return total;
+}
