@@ -20,6 +20,11 @@
 // particles are direct children (AREPO would split them further, up to level 30).
 #include "agrav_ctx.h"
 
+// The domain cube, integer coordinates and cube centres must round exactly like
+// the reference's C code (no fused multiply-add), so that node geometry is
+// bit-identical.
+#pragma clang fp contract(off)
+
 namespace agrav
 {
 
@@ -392,7 +397,7 @@ __global__ void k_moments(int cnt, const int *__restrict__ list, int n, const in
                           const uint8_t *__restrict__ ps, const double *__restrict__ hsoft, int nsoft, const double *__restrict__ cx,
                           const double *__restrict__ cy, const double *__restrict__ cz, const double *__restrict__ len,
                           double *sx, double *sy, double *sz, double *nm, uint8_t *maxsoft, int *ccnt, uint4 *recA, uint4 *recB,
-                          const Domain *__restrict__ domp)
+                          const Domain *__restrict__ domp, int limit, int *err)
 {
   int t = blockIdx.x * B + threadIdx.x;
   if(t >= cnt)
@@ -405,6 +410,11 @@ __global__ void k_moments(int cnt, const int *__restrict__ list, int n, const in
   int k    = 0;
   while(c != end)
     {
+      if(k >= limit || c < 0)
+        {
+          atomicExch(err, 1);  // broken threading: stop instead of looping
+          break;
+        }
       if(c < n)
         {
           double mm = pm[c];
@@ -456,13 +466,14 @@ __global__ void k_moments(int cnt, const int *__restrict__ list, int n, const in
 }
 
 __global__ void k_fill_children(int m, int n, const int *__restrict__ nextnode, const int *__restrict__ sibling,
-                                const int *__restrict__ pnext, const int *__restrict__ cbeg, int *child)
+                                const int *__restrict__ pnext, const int *__restrict__ cbeg, const int *__restrict__ ccnt,
+                                int *child)
 {
   int j = blockIdx.x * B + threadIdx.x;
   if(j >= m)
     return;
-  int c = nextnode[j], end = sibling[j], o = cbeg[j];
-  while(c != end)
+  int c = nextnode[j], end = sibling[j], o = cbeg[j], stop = cbeg[j] + ccnt[j];
+  while(c != end && c >= 0 && o < stop)
     {
       child[o++] = c;
       c          = (c < n) ? pnext[c] : sibling[c - n];
@@ -596,7 +607,9 @@ agrav_status build_tree(agrav_ctx *c)
     return fail(c, AGRAV_ERR_CAPACITY, "inconsistent octree level counts");
   AG_HIP(hipMemcpyAsync(c->d_lvloff.p, off, (AG_KEY_LEVELS + 2) * sizeof(int), hipMemcpyHostToDevice, s));
   int *d_fill = c->d_counters.p + 24;  // 22 ints
+  int *d_berr = c->d_counters.p + 50;
   AG_HIP(hipMemsetAsync(d_fill, 0, (AG_KEY_LEVELS + 1) * sizeof(int), s));
+  AG_HIP(hipMemsetAsync(d_berr, 0, sizeof(int), s));
   k_level_scatter<<<nblk(M), B, 0, s>>>(M, c->d_nlevel.p, c->d_lvloff.p, d_fill, c->d_lvlnodes.p);
   AG_HIP(hipGetLastError());
   for(int L = AG_KEY_LEVELS; L >= 0; L--)
@@ -607,17 +620,23 @@ agrav_status build_tree(agrav_ctx *c)
       k_moments<<<nblk(cnt), B, 0, s>>>(cnt, c->d_lvlnodes.p + off[L], N, c->d_nextnode.p, c->d_sibling.p, c->d_pnext.p, c->d_px.p,
                                         c->d_py.p, c->d_pz.p, c->d_pm.p, multisoft ? c->d_ps.p : nullptr, c->d_soft.p,
                                         c->soft.nsoft, c->d_cx.p, c->d_cy.p, c->d_cz.p, c->d_len.p, c->d_sx.p, c->d_sy.p,
-                                        c->d_sz.p, c->d_nm.p, c->d_maxsoft.p, c->d_ccnt.p, c->d_recA.p, c->d_recB.p, c->d_dom.p);
+                                        c->d_sz.p, c->d_nm.p, c->d_maxsoft.p, c->d_ccnt.p, c->d_recA.p, c->d_recB.p, c->d_dom.p,
+                                        N + M, d_berr);
     }
   AG_HIP(hipGetLastError());
   if((st = exclusive_scan_i32(c->d_ccnt.p, c->d_cbeg.p, M, nullptr, c->d_temp.p, s)) != AGRAV_OK)
     return st;
-  k_fill_children<<<nblk(M), B, 0, s>>>(M, N, c->d_nextnode.p, c->d_sibling.p, c->d_pnext.p, c->d_cbeg.p, c->d_child.p);
+  k_fill_children<<<nblk(M), B, 0, s>>>(M, N, c->d_nextnode.p, c->d_sibling.p, c->d_pnext.p, c->d_cbeg.p, c->d_ccnt.p,
+                                         c->d_child.p);
   AG_HIP(hipGetLastError());
 
   // out-of-box check (forcetree.c:432-438 aborts; here it is an error code)
+  int berr = 0;
   AG_HIP(hipMemcpyAsync(&c->h_dom, c->d_dom.p, sizeof(Domain), hipMemcpyDeviceToHost, s));
+  AG_HIP(hipMemcpyAsync(&berr, d_berr, sizeof(int), hipMemcpyDeviceToHost, s));
   AG_HIP(hipStreamSynchronize(s));
+  if(berr)
+    return fail(c, AGRAV_ERR_CAPACITY, "octree threading inconsistent (moment pass stopped)");
   if(c->h_dom.out_of_box)
     return fail(c, AGRAV_ERR_OUT_OF_BOX, "particle outside the periodic box after one wrap");
   c->m         = M;

@@ -173,8 +173,15 @@ __global__ __launch_bounds__(256) void k_walk_plain(TreeView tv, WalkConst wc, c
       const double aold = wc.errtol * (double)oldacc_s[i];
       double ax = 0, ay = 0, az = 0, pot = 0;
       int no = N;
+      int steps = 0;
+      const int max_steps = N + tv.m + 1;  // every entry is visited at most once per target
       while(no >= 0)
         {
+          if(++steps > max_steps)
+            {
+              nin = -1;  // broken threading: give up on this target
+              break;
+            }
           double dx, dy, dz, r2, mass, hmax;
           if(no < N)
             {
@@ -272,7 +279,7 @@ __global__ __launch_bounds__(256) void k_walk_plain(TreeView tv, WalkConst wc, c
       nint[i]                = nin;
     }
   // interaction total
-  unsigned long long v = (unsigned long long)nin;
+  unsigned long long v = (unsigned long long)(nin > 0 ? nin : 0);
   for(int off = 32; off > 0; off >>= 1)
     v += __shfl_xor(v, off, 64);
   if((threadIdx.x & 63) == 0 && v)
@@ -450,6 +457,8 @@ __global__ __launch_bounds__(64) void k_walk_group(TreeView tv, WalkConst wc, co
   T ax = 0, ay = 0, az = 0, pp = 0, cax = 0, cay = 0, caz = 0, cpp = 0;
   int nin        = 0;
   bool overflow  = false;
+  int pushed     = 1;
+  const int max_push = N + tv.m;  // each entry is pushed at most once per group
   __syncthreads();
 
   while(sp > 0)
@@ -582,7 +591,8 @@ __global__ __launch_bounds__(64) void k_walk_group(TreeView tv, WalkConst wc, co
       const int nch  = (act == 2) ? tv.ccnt[j] : 0;
       const int incl = wave_incl_scan_i(nch);
       const int tot  = __shfl(incl, 63, 64);
-      if(sp + tot > kStackCap)
+      pushed += tot;
+      if(sp + tot > kStackCap || pushed > max_push)
         {
           overflow = true;
           break;

@@ -1,0 +1,215 @@
+#!/usr/bin/env python3
+"""Benchmark of the TreePM gravity step on one MI355X.
+
+Workload (the metric's configuration that fits one GPU): N = 128^3 = 2,097,152
+particles in the periodic unit box, half uniform and half in 256 Hernquist
+spheres, PMGRID = 128, ASMTH = 1.25, RCUT = 4.5, ErrTolForceAcc = 0.0025
+(relative criterion), softening 1/40 of the mean spacing.
+
+A step is one full gravity solve as AREPO does it every step
+(compute_grav_accelerations with OldAcc from the previous step): periodic PM,
+octree build, tree walk, finalize.  Inputs are resident in HBM before the
+timed region.  `value` is particles per second of the FP32 group-walk path.
+
+Also reported: interactions/s, the plain FP64 GPU path (one lane per target,
+AREPO's per-particle criterion) and the FP64 group walk on the same inputs,
+the relative force error of each against an FP64 direct (Ewald) summation on
+a seeded random subset, the roofline of the dominant kernel (the walk), and
+the CPU oracle timed on a bounded sample on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particles/s & interactions/s, 128^3 TreePM, FP32 vs FP64; p50/p99 latency"
+HBM_PEAK = 8.0e12
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def rel_err(a, b):
+    return np.linalg.norm(a - b, axis=1) / np.maximum(np.linalg.norm(b, axis=1), 1e-300)
+
+
+def make_params(A, precision, walk, eps, pmgrid=128):
+    return A.make_params(periodic=True, box_size=1.0, pmgrid=pmgrid, asmth=1.25, rcut=4.5, G=1.0,
+                         err_tol_theta=0.7, err_tol_force_acc=0.0025, opening=1, soft=(eps,),
+                         precision=precision, walk=walk, group_size=64)
+
+
+def run_path(A, pos, mass, eps, precision, walk, steps, warmup, old_acc=None):
+    """Warm up, then time `steps` full solves; returns timing dict and the last result."""
+    g = A.Gravity(make_params(A, precision, walk, eps))
+    g.set_particles(pos, mass)
+    if old_acc is not None:
+        g.set_old_acc(old_acc)
+    first = old_acc is None
+    for w in range(max(warmup, 1 if first else 0)):
+        g.compute_grav_accelerations(first_step=first)
+        first = False
+    step_ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ts = time.perf_counter()
+        g.compute_grav_accelerations(first_step=False)  # synchronises at the end (event wait)
+        step_ms.append(1e3 * (time.perf_counter() - ts))
+    total = time.perf_counter() - t0
+    tim = g.timings()
+    res = g.get_accelerations()
+    g.close()
+    return dict(total_s=total, step_ms=step_ms, timings=tim), res
+
+
+def cpu_baseline(pos, mass, eps, old_acc, sel, threads):
+    """Oracle (FP64 AREPO restatement) on the host: full tree build and PM, walk timed on a random
+    sample of targets and scaled to all particles."""
+    import oracle as O
+    p = O.make_params(periodic=True, box_size=1.0, pmgrid=128, asmth=1.25, rcut=4.5, err_tol_theta=0.7,
+                      err_tol_force_acc=0.0025, opening=1, soft=(eps,))
+    n = len(pos)
+    t0 = time.perf_counter()
+    acc_pm, _ = O.pm_periodic(p, pos, mass)
+    t1 = time.perf_counter()
+    tree = O.Tree(p, pos, mass)
+    t2 = time.perf_counter()
+    sample = len(sel)
+    acc, pot, nint, tot = tree.walk(False, old_acc=old_acc, targets=sel, nthreads=threads)
+    t3 = time.perf_counter()
+    t_walk_full = (t3 - t2) * n / sample
+    t_total = (t1 - t0) + (t2 - t1) + t_walk_full
+    return dict(value=n / t_total, t_pm=t1 - t0, t_build=t2 - t1, t_walk_sample=t3 - t2, sample=sample,
+                ia_per_part=tot / sample, sel=sel, acc_tree=acc, pot_tree=pot, acc_pm=acc_pm, params=p)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=128 ** 3)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--err-sample", type=int, default=16384)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-errors", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU only: the gravity step of one particle set runs on a single MI355X "
+              "(multi-GPU sharding is described in DESIGN.md, not run); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from arepo_amd import agrav as A
+    from arepo_amd import ics
+
+    log("generating clustered 128^3 set (seed %d)" % args.seed)
+    pos, mass, eps = ics.clustered_box(args.n, seed=args.seed)
+    n = len(pos)
+    # one seeded target subset serves the error measurement and the CPU sample
+    sel = np.sort(np.random.Generator(np.random.Philox(99)).choice(n, min(args.err_sample, n), replace=False))
+    sel = sel.astype(np.int32)
+
+    # FP32 group walk: the measured path
+    log("FP32 group walk: warmup %d, steps %d" % (args.warmup, args.steps))
+    r32, res32 = run_path(A, pos, mass, eps, A.FP32, A.WALK_GROUP, args.steps, args.warmup)
+    tim = r32["timings"]
+    ms_per_step = 1e3 * r32["total_s"] / args.steps
+    value = n * args.steps / r32["total_s"]
+    inter = tim["interactions"]
+    log("fp32 group: %.2f ms/step, walk %.2f ms, build %.2f ms, pm %.2f ms, ia/part %.1f"
+        % (ms_per_step, tim["treewalk"], tim["treebuild"], tim["pm_grav"], inter / n))
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particles/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded clustered particle set generated in-process)",
+        "config": {"workload": "treepm_128cubed_clustered", "n_particles": n, "pmgrid": 128, "asmth": 1.25,
+                   "rcut": 4.5, "err_tol_force_acc": 0.0025, "opening": "relative", "softening": eps,
+                   "precision": "fp32", "walk": "group64", "seed": args.seed},
+        "interactions_per_s": inter * args.steps / r32["total_s"],
+        "interactions_per_particle": inter / n,
+        "latency_ms": {"p50": float(np.percentile(r32["step_ms"], 50)),
+                       "p99": float(np.percentile(r32["step_ms"], 99))},
+        "stage_ms": {"treebuild": tim["treebuild"], "treewalk": tim["treewalk"], "pm_grav": tim["pm_grav"],
+                     "finalize": tim["finalize"]},
+    }
+
+    # roofline of the dominant kernel (the walk): algorithmic bytes = one 16-byte source record per
+    # interaction + 52 bytes per target (position, mass, softening/OldAcc in, acc + pot out)
+    walk_s = tim["walk_kernel"] * 1e-3
+    alg_bytes = inter * 16 + n * 52
+    achieved = alg_bytes / walk_s if walk_s > 0 else 0.0
+    out["roofline"] = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                       "frac": achieved / HBM_PEAK, "traffic": None, "kernel": "k_walk_group<float,PM>",
+                       "kernel_ms": tim["walk_kernel"], "alg_bytes_per_launch": alg_bytes}
+
+    res64p = None
+    if not args.no_fp64:
+        log("FP64 plain walk (one lane per target)")
+        r64p, res64p = run_path(A, pos, mass, eps, A.FP64, A.WALK_PLAIN, 1, 0, old_acc=res32["old_acc"])
+        log("FP64 group walk")
+        r64g, res64g = run_path(A, pos, mass, eps, A.FP64, A.WALK_GROUP, 2, 0, old_acc=res32["old_acc"])
+        out["fp64_plain"] = {"ms_per_step": 1e3 * r64p["total_s"], "walk_ms": r64p["timings"]["treewalk"],
+                             "interactions": r64p["timings"]["interactions"]}
+        out["fp64_group"] = {"ms_per_step": 1e3 * r64g["total_s"] / 2, "walk_ms": r64g["timings"]["treewalk"],
+                             "interactions": r64g["timings"]["interactions"]}
+        out["speedup_fp32_vs_fp64_plain"] = out["fp64_plain"]["ms_per_step"] / ms_per_step
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(os.cpu_count() or 1, 16)
+        log("CPU oracle baseline: %d threads, walk sample %d" % (threads, len(sel)))
+        cpu = cpu_baseline(pos, mass, eps, res32["old_acc"], sel, threads)
+        out["cpu_baseline"] = {"value": cpu["value"], "unit": "particles/s", "cores": threads, "kind": "port",
+                               "sample": "full tree build + 128^3 PM on the host, relative walk of %d random "
+                                         "targets scaled to N (build %.2f s, pm %.2f s, walk sample %.2f s)"
+                                         % (cpu["sample"], cpu["t_build"], cpu["t_pm"], cpu["t_walk_sample"])}
+        out["speedup_vs_cpu"] = value / cpu["value"]
+
+    if not args.no_errors:
+        log("FP64 direct summation (Ewald) on %d targets" % len(sel))
+        g = A.Gravity(make_params(A, A.FP64, A.WALK_GROUP, eps))
+        g.set_particles(pos, mass)
+        t0 = time.perf_counter()
+        ft = g.gravity_forcetest(sel)
+        log("direct sum took %.1f s" % (time.perf_counter() - t0))
+        g.close()
+        errs = {}
+        for name, res in (("fp32_group", res32), ("fp64_plain", res64p)):
+            if res is None:
+                continue
+            e = rel_err(res["acc"][sel], ft["acc"])
+            errs[name] = {"median": float(np.median(e)), "p99": float(np.percentile(e, 99))}
+        if cpu is not None:
+            # the oracle's own TreePM error on its sample targets (tree part from the oracle, PM from the oracle)
+            e = rel_err(cpu["acc_tree"] + cpu["acc_pm"][sel], ft["acc"])
+            errs["cpu_oracle"] = {"median": float(np.median(e)), "p99": float(np.percentile(e, 99))}
+        out["force_error_vs_direct"] = errs
+        out["force_error_sample"] = int(args.err_sample)
+
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,90 @@
+"""CPU tests of the C ABI boundary: the library loads, exports exactly what
+include/agrav.h declares, host-side helpers behave, and calls without a GPU
+fail with an error code instead of falling back to the CPU."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, gpu_available
+
+
+def _declared():
+    src = open(os.path.join(ROOT, "include", "agrav.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(agrav_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_declared_symbols(agrav):
+    names = _declared()
+    assert len(names) >= 20
+    lib = C.CDLL(agrav.LIB_PATH)
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(agrav.EXPORTED) == set(names)
+
+
+def test_status_strings(agrav):
+    L = agrav.lib()
+    for code in range(0, -9, -1):
+        s = L.agrav_status_string(code)
+        assert s and s != b"unknown"
+
+
+def test_default_params_match_reference_defaults(agrav):
+    p = agrav.Params()
+    agrav.lib().agrav_default_params(C.byref(p))
+    assert p.asmth == 1.25 and p.rcut == 4.5  # allvars.h:461, :469
+    assert p.type_of_opening_criterion == 1 and p.G == 1.0
+
+
+def test_create_without_gpu_fails_loudly(agrav):
+    if gpu_available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(agrav.AgravError) as ei:
+        agrav.Gravity(agrav.make_params())
+    assert ei.value.code in (agrav.ERR_NO_DEVICE, agrav.ERR_HIP)
+
+
+def _cells(level):
+    """All cells of a 2^level grid, keyed by the top `level` bits of the 21-bit key coordinates."""
+    n = 1 << level
+    idx = np.arange(n)
+    X, Y, Z = np.meshgrid(idx, idx, idx, indexing="ij")
+    return np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+
+
+@pytest.mark.parametrize("level", [1, 2, 3])
+def test_hilbert_key_is_a_hilbert_curve(agrav, level):
+    shift = 21 - level
+    cells = _cells(level)
+    keys = np.array([agrav.hilbert_key21(*(c << shift)) >> (3 * shift) for c in cells])
+    # bijection onto 0 .. 8^level - 1
+    assert np.array_equal(np.sort(keys), np.arange(8 ** level))
+    order = cells[np.argsort(keys)]
+    # consecutive cells along the curve are face neighbours (the defining Hilbert property)
+    steps = np.abs(np.diff(order, axis=0)).sum(1)
+    assert np.all(steps == 1)
+
+
+def test_hilbert_key_prefix_property(agrav):
+    # the key's top 3L bits depend only on the top L bits of each coordinate: every octree cube is a
+    # contiguous key range (what the octree construction relies on)
+    g = np.random.Generator(np.random.Philox(1))
+    for _ in range(200):
+        L = int(g.integers(1, 21))
+        base = g.integers(0, 1 << 21, 3)
+        mask = ~np.int64((1 << (21 - L)) - 1)
+        a = base & mask
+        b = a + g.integers(0, 1 << (21 - L), 3)
+        ka = agrav.hilbert_key21(*a) >> (3 * (21 - L))
+        kb = agrav.hilbert_key21(*b) >> (3 * (21 - L))
+        assert ka == kb
+
+
+def test_hilbert_key_fits_63_bits(agrav):
+    m = (1 << 21) - 1
+    for c in [(0, 0, 0), (m, m, m), (m, 0, 0), (0, m, 0), (0, 0, m)]:
+        assert agrav.hilbert_key21(*c) < (1 << 63)

@@ -1,0 +1,238 @@
+"""CPU tests that pin the oracle (the checker) against mathematics and independent libraries.
+
+The reference ships no force-level golden vectors, so each oracle function is
+pinned by something a plausible mistake would break: closed forms and
+symmetries, agreement of two independent formulations (Ewald sums with
+different splitting parameters, tree walk vs direct sum, short- plus
+long-range split vs the full periodic force), numpy's FFT, and fixtures
+computed with scipy.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def test_shortrange_tables_match_scipy_fixture(oracle):
+    g = np.load(os.path.join(HERE, "golden", "shortrange_tables.npz"))
+    tf, tp = oracle.short_range_tables(float(g["rcut"]))
+    assert np.array_equal(tf, g["tab_force"])
+    assert np.array_equal(tp, g["tab_pot"])
+
+
+def _pair(oracle, r, h, pm=0, box=0.0):
+    """Acceleration and potential on particle 0 from a unit mass at distance r along x (two-body direct sum)."""
+    p = oracle.make_params(periodic=False, box_size=box, pmgrid=pm, soft=(h / 2.8,))
+    pos = np.array([[0.0, 0.0, 0.0], [r, 0.0, 0.0]])
+    acc, pot = oracle.direct_summation(p, pos, np.array([1e-30, 1.0]))
+    return acc[0, 0], pot[0]
+
+
+def test_spline_softening_closed_form(oracle):
+    h = 0.1
+    # Newtonian outside the kernel support
+    for r in (0.1, 0.15, 0.4):
+        a, ph = _pair(oracle, r, h)
+        assert a == pytest.approx(1 / r ** 2, rel=1e-12)
+        assert ph == pytest.approx(-1 / r, rel=1e-12)
+    # Plummer-equivalent depth: phi(0) = -m/eps with eps = h/2.8
+    p = oracle.make_params(periodic=False, box_size=0.0, soft=(h / 2.8,))
+    _, pot = oracle.direct_summation(p, np.zeros((1, 3)), np.ones(1))
+    assert pot[0] == pytest.approx(-2.8 / h, rel=1e-12)
+    # force is minus the derivative of the potential everywhere inside the kernel
+    for r in np.linspace(0.01, 0.099, 23):
+        d = 1e-6
+        _, pp = _pair(oracle, r + d, h)
+        _, pm_ = _pair(oracle, r - d, h)
+        a, _ = _pair(oracle, r, h)
+        assert a == pytest.approx((pp - pm_) / (2 * d), rel=2e-7)
+    # continuity at u = 0.5 and u = 1
+    for u in (0.5, 1.0):
+        a1, p1 = _pair(oracle, u * h * (1 - 1e-9), h)
+        a2, p2 = _pair(oracle, u * h * (1 + 1e-9), h)
+        assert a1 == pytest.approx(a2, rel=1e-6) and p1 == pytest.approx(p2, rel=1e-6)
+
+
+def test_ewald_two_formulations_agree(oracle):
+    """ewald_force/psi (alpha=2 and 3, 9^3 images) and the FORCETEST series (alpha=2/L, 13^3 images, larger
+    k-space) are independent truncations of the same lattice sum: they must agree."""
+    g = np.random.Generator(np.random.Philox(3))
+    for x in g.uniform(0.02, 0.5, (20, 3)):
+        f = oracle.ewald_force(*x)
+        psi = oracle.ewald_psi(*x)
+        ft = oracle.forcetest_ewald_exact(1.0, *x)
+        # the FORCETEST series carries the table's sign convention (negated for positive offsets)
+        np.testing.assert_allclose(f, -ft[:3], rtol=1e-9, atol=1e-9)
+        assert psi == pytest.approx(ft[3], rel=1e-9, abs=1e-9)
+
+
+def test_ewald_symmetry_points(oracle):
+    # at the body centre of the cell the periodic force vanishes: correction = +x/r^3
+    x = np.array([0.5, 0.5, 0.5])
+    r3 = np.linalg.norm(x) ** 3
+    np.testing.assert_allclose(oracle.ewald_force(*x), x / r3, rtol=1e-10)
+    # at a face centre the normal component of the periodic force vanishes
+    assert oracle.ewald_force(0.5, 0.0, 0.0)[0] == pytest.approx(1 / 0.25, rel=1e-10)
+    # antisymmetry of the force correction, symmetry of the potential
+    f1 = oracle.ewald_force(0.1, 0.2, 0.3)
+    f2 = oracle.ewald_force(-0.1, -0.2, -0.3)
+    np.testing.assert_allclose(f1, -f2, rtol=1e-12)
+    assert oracle.ewald_psi(0.1, 0.2, 0.3) == pytest.approx(oracle.ewald_psi(-0.1, -0.2, -0.3), rel=1e-12)
+
+
+def test_ewald_table_lookup(oracle):
+    tab = oracle.ewald_table_tree(2.0)
+    # grid points reproduce the series exactly (scaled to box 2)
+    for (i, j, k) in [(0, 3, 7), (64, 64, 64), (10, 20, 30)]:
+        x = np.array([i, j, k]) * 0.5 / 64
+        f, p = oracle.ewald_corr(tab, 2.0, 2.0 * x)
+        np.testing.assert_allclose(-f, oracle.ewald_force(*x) / 4.0, rtol=1e-12, atol=1e-14)
+        assert p == pytest.approx(oracle.ewald_psi(*x) / 2.0, rel=1e-12, abs=1e-14)
+    # trilinear interpolation between grid points stays within 1e-3 of the series
+    g = np.random.Generator(np.random.Philox(4))
+    for x in g.uniform(0.05, 0.5, (10, 3)):
+        f, p = oracle.ewald_corr(tab, 2.0, 2.0 * x)
+        ref = oracle.ewald_force(*x) / 4.0
+        assert np.linalg.norm(-f - ref) < 2e-3 * np.linalg.norm(ref) + 1e-3
+
+
+def test_forcetest_table_matches_series(oracle):
+    tew = 16
+    tab = oracle.forcetest_ewald_table(tew, 1.0)
+    g = np.random.Generator(np.random.Philox(5))
+    for x in g.uniform(-0.5, 0.5, (20, 3)):
+        t = oracle.forcetest_ewald_lookup(tab, tew, 1.0, x)
+        e = oracle.forcetest_ewald_exact(1.0, *np.abs(x)) * np.r_[np.sign(x), 1]
+        assert np.linalg.norm(t[:3] - e[:3]) < 0.02 * np.linalg.norm(e[:3]) + 0.05
+
+
+def test_fft_matches_numpy(oracle):
+    g = np.random.Generator(np.random.Philox(6))
+    for n in (4, 8, 16):
+        re = g.standard_normal((n, n, n))
+        im = g.standard_normal((n, n, n))
+        fr, fi = oracle.fft3d(re, im, -1)
+        ref = np.fft.fftn(re + 1j * im)
+        np.testing.assert_allclose(fr + 1j * fi, ref, rtol=1e-10, atol=1e-10 * np.abs(ref).max())
+        br, bi = oracle.fft3d(re, im, +1)
+        ref = np.fft.ifftn(re + 1j * im) * n ** 3
+        np.testing.assert_allclose(br + 1j * bi, ref, rtol=1e-10, atol=1e-10 * np.abs(ref).max())
+
+
+def test_pm_uniform_lattice_has_no_force(oracle):
+    G = 16
+    ax = (np.arange(G) + 0.5) / G
+    pos = np.stack(np.meshgrid(ax, ax, ax, indexing="ij"), -1).reshape(-1, 3)
+    p = oracle.make_params(periodic=True, box_size=1.0, pmgrid=G, soft=(0.001,))
+    acc, pot = oracle.pm_periodic(p, pos, np.full(len(pos), 1.0 / len(pos)))
+    assert np.abs(acc).max() < 1e-12
+    assert np.ptp(pot) < 1e-12
+
+
+def test_treepm_split_reproduces_periodic_force(oracle):
+    """Short-range tree force + PM long-range force = full periodic (Ewald) force for a pair, at separations
+    across the split scale; the residual is the mesh's own error (a few per cent at most)."""
+    G = 64
+    asmth = 1.25 / G
+    p = oracle.make_params(periodic=True, box_size=1.0, pmgrid=G, soft=(1e-5,), opening=0, err_tol_theta=1e-3)
+    for r in (1.0 * asmth, 2.0 * asmth, 3.5 * asmth, 6.0 * asmth):
+        pos = np.array([[0.3, 0.41, 0.52], [0.3 + r, 0.41, 0.52]])
+        mass = np.array([1e-12, 1.0])
+        res = oracle.compute_grav_accelerations(p, pos, mass, first_step=False)
+        ft, _, _, _ = oracle.forcetest(p, pos, mass, np.array([0], np.int32), ewald=2)
+        err = np.linalg.norm(res["acc"][0] - ft[0]) / np.linalg.norm(ft[0])
+        assert err < 0.03, (r / asmth, err)
+
+
+def _two_level_pair(level):
+    # two particles that share their first `level` octants and split below it
+    d = 0.5 ** (level + 2)
+    return np.array([[0.3, 0.3, 0.3], [0.3 + d, 0.3, 0.3]])
+
+
+def test_tree_invariants(oracle):
+    g = np.random.Generator(np.random.Philox(7))
+    pos = g.random((3000, 3))
+    mass = g.uniform(0.5, 1.5, 3000)
+    p = oracle.make_params(periodic=True, box_size=1.0, soft=(0.001,))
+    t = oracle.Tree(p, pos, mass)
+    ex = t.export()
+    assert ex["mass"][0] == pytest.approx(mass.sum(), rel=1e-13)
+    assert ex["npart"][0] == 3000
+    assert np.all(ex["npart"] >= 2)
+    corner, center, dlen = oracle.domain_extent(p, pos)
+    assert dlen == pytest.approx(1.00001) and np.allclose(center, 0.5)
+    np.testing.assert_array_equal(ex["len"], dlen * 0.5 ** ex["level"])
+    inside = np.abs(ex["com"] - ex["center"]) <= 0.5 * ex["len"][:, None] * (1 + 1e-12)
+    assert inside.all()
+    # a pair separating at level L creates exactly L+1 nested cubes (root chain)
+    pn = oracle.make_params(periodic=False, box_size=1.0, soft=(0.001,))
+    for L in (1, 4, 9):
+        pos2 = _two_level_pair(L)
+        t2 = oracle.Tree(pn, pos2, np.ones(2))
+        assert t2.numnodes >= 1 and np.all(t2.export()["npart"] == 2)
+
+
+def test_walk_tends_to_direct_sum(oracle):
+    g = np.random.Generator(np.random.Philox(8))
+    pos = g.random((1500, 3))
+    mass = np.full(1500, 1 / 1500)
+    for periodic, pm in ((False, 0), (True, 16)):
+        p = oracle.make_params(periodic=periodic, box_size=1.0 if periodic else 0.0, pmgrid=pm, soft=(0.002,),
+                               opening=0, err_tol_theta=1e-6)
+        t = oracle.Tree(p, pos, mass)
+        acc, pot, nint, tot = t.walk(True)
+        ad, pd = oracle.direct_summation(p, pos, mass)
+        assert np.max(rel_err(acc, ad)) < 1e-11
+        np.testing.assert_allclose(pot, pd, rtol=1e-11)
+        if pm == 0:
+            assert np.all(nint == 1500)
+
+
+def test_relative_criterion_accuracy(oracle):
+    from arepo_amd import ics
+    g = ics.rng(9)
+    pos = np.concatenate([g.random((2000, 3)), 0.5 + ics._isotropic(ics.hernquist_radii(2000, 0.05, g), g)])
+    mass = np.full(4000, 1 / 4000)
+    ad, _ = oracle.direct_summation(oracle.make_params(periodic=False, box_size=0.0, soft=(0.002,)), pos, mass)
+    last = None
+    for tol in (0.01, 0.0025, 0.0005):
+        p = oracle.make_params(periodic=False, box_size=0.0, soft=(0.002,), err_tol_force_acc=tol)
+        r = oracle.compute_grav_accelerations(p, pos, mass, first_step=True)
+        e = rel_err(r["acc"], ad)
+        assert np.median(e) < 2 * tol  # the criterion controls the error to order ErrTolForceAcc
+        if last is not None:
+            assert np.median(e) < last  # error falls with the tolerance
+        last = np.median(e)
+
+
+def test_group_potential_matches_hernquist(oracle):
+    from arepo_amd import ics
+    g = ics.rng(10)
+    n = 20000
+    r = ics.hernquist_radii(n, 1.0, g, rmax_over_a=1000.0)
+    pos = ics._isotropic(r, g)
+    m = np.full(n, 1.0 / n)
+    h = 0.01
+    pot, tot = oracle.group_potential_bh(pos, m, h, 0.5)
+    pot = pot + 2.8 * m / h  # the walk includes the particle's own softened self-potential -2.8 m/h
+    rr = np.linalg.norm(pos, axis=1)
+    # analytic Hernquist potential -M/(r+a) of the (nearly untruncated) sphere, radius by radius
+    for lo, hi in ((0.3, 1.0), (1.0, 3.0), (3.0, 20.0)):
+        sel = (rr > lo) & (rr < hi)
+        ratio = pot[sel] / (-1.0 / (rr[sel] + 1.0))
+        assert abs(np.median(ratio) - 1) < 0.02, (lo, hi, np.median(ratio))
+
+
+def test_orc_compute_first_step_seeds_old_acc(oracle):
+    g = np.random.Generator(np.random.Philox(11))
+    pos = g.random((2000, 3))
+    mass = np.full(2000, 1 / 2000)
+    p = oracle.make_params(periodic=True, box_size=1.0, pmgrid=16, soft=(0.002,))
+    r = oracle.compute_grav_accelerations(p, pos, mass, first_step=True)
+    # OldAcc is |a_tree + a_PM/G| (G = 1 here, so |acc|)
+    np.testing.assert_allclose(r["old_acc"], np.linalg.norm(r["acc"], axis=1), rtol=1e-6)
