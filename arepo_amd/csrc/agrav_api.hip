@@ -36,8 +36,8 @@ static constexpr int B = 256;
 // gravity_force_finalize, accel.c:204-347 (no comoving terms, OmegaLambda = 0)
 __global__ void k_finalize(int64_t n, const int *__restrict__ perm, const double *__restrict__ at, const double *__restrict__ pt,
                            const int *__restrict__ nint, const double *__restrict__ acc_pm, const double *__restrict__ pot_pm,
-                           int have_pm, double G, double pot_const, double *acc, double *pot, int *nint_out, float *oldacc,
-                           float *oldacc_s)
+                           int have_pm, int pm_sorted, double G, double pot_const, double *acc, double *pot, int *nint_out,
+                           float *oldacc, float *oldacc_s)
 {
   int64_t k = (int64_t)blockIdx.x * B + threadIdx.x;
   if(k >= n)
@@ -47,10 +47,11 @@ __global__ void k_finalize(int64_t n, const int *__restrict__ perm, const double
   double p[3] = {0, 0, 0}, pp = 0;
   if(have_pm)
     {
-      p[0] = acc_pm[3 * (int64_t)i];
-      p[1] = acc_pm[3 * (int64_t)i + 1];
-      p[2] = acc_pm[3 * (int64_t)i + 2];
-      pp   = pot_pm[i];
+      const int64_t q = pm_sorted ? k : (int64_t)i;
+      p[0]            = acc_pm[3 * q];
+      p[1]            = acc_pm[3 * q + 1];
+      p[2]            = acc_pm[3 * q + 2];
+      pp              = pot_pm[q];
     }
   double ox = a[0] + p[0] / G, oy = a[1] + p[1] / G, oz = a[2] + p[2] / G;
   float oa  = (float)sqrt(ox * ox + oy * oy + oz * oz);  // accel.c:237-255
@@ -89,7 +90,7 @@ agrav_status finalize(agrav_ctx *c)
   AG_HIP(hipEventRecord(c->ev[0], c->stream));
   if(n > 0)
     k_finalize<<<(unsigned)((n + B - 1) / B), B, 0, c->stream>>>(n, c->d_idx2.p, c->d_at.p, c->d_pt.p, c->d_nint.p,
-                                                                c->d_acc_pm.p, c->d_pot_pm.p, pm, c->par.G, pot_const,
+                                                                c->d_acc_pm.p, c->d_pot_pm.p, pm, c->pm_sorted ? 1 : 0, c->par.G, pot_const,
                                                                 c->d_acc.p, c->d_pot.p, c->d_nint_out.p, c->d_oldacc.p,
                                                                 c->d_oldacc_s.p);
   AG_HIP(hipGetLastError());
@@ -329,6 +330,7 @@ void agrav_destroy(agrav_ctx *c)
   c->d_maxsoft.release();
   c->d_recA.release();
   c->d_recB.release();
+  c->d_nrange.release();
   c->d_cbeg.release();
   c->d_ccnt.release();
   c->d_child.release();
@@ -507,9 +509,11 @@ agrav_status agrav_compute_grav_accelerations(agrav_ctx *c, int first_step)
   if(!c->have_particles)
     return fail(c, AGRAV_ERR_STATE, "compute_grav_accelerations before set_particles");
   agrav_status st;
-  if(c->par.pmgrid > 0 && (st = pm_force(c)) != AGRAV_OK)
-    return st;
+  // the PM does not depend on the tree; running it after the build lets it use the
+  // Hilbert-ordered particles (the reference calls long_range_force first)
   if((st = agrav_construct_forcetree(c, nullptr)) != AGRAV_OK)
+    return st;
+  if(c->par.pmgrid > 0 && (st = pm_force(c)) != AGRAV_OK)
     return st;
   const bool seed = c->par.type_of_opening_criterion == 1 && first_step && c->par.err_tol_theta > 0;
   if(seed)
@@ -590,10 +594,28 @@ agrav_status agrav_get_components(agrav_ctx *c, double *acc_tree, double *pot_tr
             pot_tree[i] = pt[k];
         }
     }
-  if(acc_pm)
-    AG_HIP(hipMemcpyAsync(acc_pm, c->d_acc_pm.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost, s));
-  if(pot_pm)
-    AG_HIP(hipMemcpyAsync(pot_pm, c->d_pot_pm.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  if(acc_pm || pot_pm)
+    {
+      std::vector<double> ap(3 * n), pp(n);
+      std::vector<int> perm;
+      AG_HIP(hipMemcpyAsync(ap.data(), c->d_acc_pm.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost, s));
+      AG_HIP(hipMemcpyAsync(pp.data(), c->d_pot_pm.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+      if(c->pm_sorted)
+        {
+          perm.resize(n);
+          AG_HIP(hipMemcpyAsync(perm.data(), c->d_idx2.p, n * sizeof(int), hipMemcpyDeviceToHost, s));
+        }
+      AG_HIP(hipStreamSynchronize(s));
+      for(int64_t k = 0; k < n; k++)
+        {
+          int64_t i = c->pm_sorted ? perm[k] : k;
+          if(acc_pm)
+            for(int d = 0; d < 3; d++)
+              acc_pm[3 * i + d] = ap[3 * k + d];
+          if(pot_pm)
+            pot_pm[i] = pp[k];
+        }
+    }
   AG_HIP(hipStreamSynchronize(s));
   return AGRAV_OK;
 }

@@ -279,27 +279,35 @@ __global__ void k_oct_count(int n, const int *__restrict__ bprefix, const int *_
   bcount[i] = oct_level(bprefix[i]) - lp;
 }
 
-__global__ void k_oct_emit(int n, const int *__restrict__ bfirst, const int *__restrict__ blast, const int *__restrict__ bprefix,
-                           const int *__restrict__ bparent, const int *__restrict__ bcount, const int *__restrict__ bbase,
-                           int *tfirst, int *tlast, int *tlevel, unsigned *lvlmask, int *lvlcnt)
+__global__ __launch_bounds__(B) void k_oct_emit(int n, const int *__restrict__ bfirst, const int *__restrict__ blast,
+                                                const int *__restrict__ bprefix, const int *__restrict__ bparent,
+                                                const int *__restrict__ bcount, const int *__restrict__ bbase, int *tfirst,
+                                                int *tlast, int *tlevel, unsigned *lvlmask, int *lvlcnt)
 {
+  // per-level counts are aggregated in LDS first: one global atomic per block and level
+  __shared__ int s_cnt[AG_KEY_LEVELS + 1];
+  if(threadIdx.x <= AG_KEY_LEVELS)
+    s_cnt[threadIdx.x] = 0;
+  __syncthreads();
   int i = blockIdx.x * B + threadIdx.x;
-  if(i >= n - 1)
-    return;
-  int c = bcount[i];
-  if(c <= 0)
-    return;
-  int lp = (i == 0) ? -1 : oct_level(bprefix[bparent[i]]);
-  int f = bfirst[i], la = blast[i], b = bbase[i];
-  for(int k = 0; k < c; k++)
+  int c = (i < n - 1) ? bcount[i] : 0;
+  if(c > 0)
     {
-      int L         = lp + 1 + k;
-      tfirst[b + k] = f;
-      tlast[b + k]  = la;
-      tlevel[b + k] = L;
-      atomicOr(&lvlmask[f], 1u << L);
-      atomicAdd(&lvlcnt[L], 1);
+      int lp = (i == 0) ? -1 : oct_level(bprefix[bparent[i]]);
+      int f = bfirst[i], la = blast[i], b = bbase[i];
+      for(int k = 0; k < c; k++)
+        {
+          int L         = lp + 1 + k;
+          tfirst[b + k] = f;
+          tlast[b + k]  = la;
+          tlevel[b + k] = L;
+          atomicOr(&lvlmask[f], 1u << L);
+          atomicAdd(&s_cnt[L], 1);
+        }
     }
+  __syncthreads();
+  if(threadIdx.x <= AG_KEY_LEVELS && s_cnt[threadIdx.x])
+    atomicAdd(&lvlcnt[threadIdx.x], s_cnt[threadIdx.x]);
 }
 
 __global__ void k_single_root(int *tfirst, int *tlast, int *tlevel, unsigned *lvlmask, int *lvlcnt)
@@ -330,7 +338,8 @@ __device__ inline int pnext_of(int p, int n, const unsigned *__restrict__ lvlmas
 __global__ void k_place(int mt, int n, const int *__restrict__ tfirst, const int *__restrict__ tlast, const int *__restrict__ tlevel,
                         const unsigned *__restrict__ lvlmask, const int *__restrict__ pbase, const double *__restrict__ px,
                         const double *__restrict__ py, const double *__restrict__ pz, const Domain *__restrict__ domp, int *nfirst,
-                        int *nlast, int *nlevel, double *cx, double *cy, double *cz, double *len, int *nextnode, int *sibling)
+                        int *nlast, int *nlevel, double *cx, double *cy, double *cz, double *len, int *nextnode, int *sibling,
+                        int2 *nrange)
 {
   int t = blockIdx.x * B + threadIdx.x;
   if(t >= mt)
@@ -341,6 +350,7 @@ __global__ void k_place(int mt, int n, const int *__restrict__ tfirst, const int
   int j         = pbase[f] + __popc(mask & ((1u << L) - 1u));
   nfirst[j]     = f;
   nlast[j]      = la;
+  nrange[j]     = make_int2(f, la - f + 1);
   nlevel[j]     = L;
   len[j]        = ldexp(dom.len, -L);
   if(L == 0)
@@ -377,14 +387,23 @@ __global__ void k_pnext(int n, const unsigned *__restrict__ lvlmask, const int *
     pnext[p] = pnext_of(p, n, lvlmask, pbase);
 }
 
-__global__ void k_level_scatter(int m, const int *__restrict__ nlevel, const int *__restrict__ lvloff, int *lvlfill, int *lvlnodes)
+__global__ __launch_bounds__(B) void k_level_scatter(int m, const int *__restrict__ nlevel, const int *__restrict__ lvloff,
+                                                     int *lvlfill, int *lvlnodes)
 {
-  int j = blockIdx.x * B + threadIdx.x;
-  if(j >= m)
-    return;
-  int L                      = nlevel[j];
-  int slot                   = atomicAdd(&lvlfill[L], 1);
-  lvlnodes[lvloff[L] + slot] = j;
+  // block-local ranks per level in LDS, one global reservation per block and level
+  __shared__ int s_cnt[AG_KEY_LEVELS + 1], s_base[AG_KEY_LEVELS + 1];
+  if(threadIdx.x <= AG_KEY_LEVELS)
+    s_cnt[threadIdx.x] = 0;
+  __syncthreads();
+  int j    = blockIdx.x * B + threadIdx.x;
+  int L    = (j < m) ? nlevel[j] : -1;
+  int rank = (L >= 0) ? atomicAdd(&s_cnt[L], 1) : 0;
+  __syncthreads();
+  if(threadIdx.x <= AG_KEY_LEVELS)
+    s_base[threadIdx.x] = s_cnt[threadIdx.x] ? atomicAdd(&lvlfill[threadIdx.x], s_cnt[threadIdx.x]) : 0;
+  __syncthreads();
+  if(L >= 0)
+    lvlnodes[lvloff[L] + s_base[L] + rank] = j;
 }
 
 // ---------------------------------------------------------------------------
@@ -578,7 +597,7 @@ agrav_status build_tree(agrav_ctx *c)
      (st = c->d_sy.ensure(M)) || (st = c->d_sz.ensure(M)) || (st = c->d_nm.ensure(M)) || (st = c->d_cx.ensure(M)) ||
      (st = c->d_cy.ensure(M)) || (st = c->d_cz.ensure(M)) || (st = c->d_len.ensure(M)) || (st = c->d_nextnode.ensure(M)) ||
      (st = c->d_sibling.ensure(M)) || (st = c->d_maxsoft.ensure(M)) || (st = c->d_recA.ensure(M)) ||
-     (st = c->d_recB.ensure(M)) || (st = c->d_cbeg.ensure(M)) || (st = c->d_ccnt.ensure(M)) ||
+     (st = c->d_recB.ensure(M)) || (st = c->d_nrange.ensure(M)) || (st = c->d_cbeg.ensure(M)) || (st = c->d_ccnt.ensure(M)) ||
      (st = c->d_child.ensure((size_t)M + n)) || (st = c->d_lvlnodes.ensure(M)))
     return st;
   if(N > 1)
@@ -592,7 +611,7 @@ agrav_status build_tree(agrav_ctx *c)
     return st;
   k_place<<<nblk(M), B, 0, s>>>(M, N, c->d_tfirst.p, c->d_tlast.p, c->d_tlevel.p, c->d_lvlmask.p, c->d_pbase.p, c->d_px.p,
                                 c->d_py.p, c->d_pz.p, c->d_dom.p, c->d_nfirst.p, c->d_nlast.p, c->d_nlevel.p, c->d_cx.p, c->d_cy.p,
-                                c->d_cz.p, c->d_len.p, c->d_nextnode.p, c->d_sibling.p);
+                                c->d_cz.p, c->d_len.p, c->d_nextnode.p, c->d_sibling.p, c->d_nrange.p);
   k_pnext<<<nblk(N), B, 0, s>>>(N, c->d_lvlmask.p, c->d_pbase.p, c->d_pnext.p);
   AG_HIP(hipGetLastError());
 
@@ -669,6 +688,7 @@ TreeView tree_view(const agrav_ctx *c)
   v.maxsoft  = c->d_maxsoft.p;
   v.recA     = c->d_recA.p;
   v.recB     = c->d_recB.p;
+  v.nrange   = c->d_nrange.p;
   v.cbeg     = c->d_cbeg.p;
   v.ccnt     = c->d_ccnt.p;
   v.child    = c->d_child.p;

@@ -109,6 +109,7 @@ struct agrav_ctx
   agrav::DBuf<int> d_nextnode, d_sibling;
   agrav::DBuf<uint8_t> d_maxsoft;
   agrav::DBuf<uint4> d_recA, d_recB;
+  agrav::DBuf<int2> d_nrange;
   agrav::DBuf<int> d_cbeg, d_ccnt, d_child;
   agrav::DBuf<int> d_lvlcnt, d_lvloff, d_lvlnodes;
   int lvlcnt_h[AG_KEY_LEVELS + 2];
@@ -123,7 +124,8 @@ struct agrav_ctx
 
   // PM
   agrav::PmState pm;
-  agrav::DBuf<double> d_acc_pm, d_pot_pm;  // caller order
+  agrav::DBuf<double> d_acc_pm, d_pot_pm;  // caller order, or sorted order when pm_sorted
+  bool pm_sorted = false;
 
   // final results in caller order
   agrav::DBuf<double> d_acc, d_pot;

@@ -70,6 +70,7 @@ struct TreeView
   const uint8_t *maxsoft;
   const uint4 *recA;      // {comfx, comfy, comfz, mass bits}
   const uint4 *recB;      // {cenfx, cenfy, cenfz, len bits}
+  const int2 *nrange;     // {first particle, particle count} per node
   const int *cbeg;        // children CSR
   const int *ccnt;
   const int *child;

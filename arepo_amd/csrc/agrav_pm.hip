@@ -109,6 +109,143 @@ __global__ void k_cic_deposit(const double *__restrict__ pos, const double *__re
 #undef FI
 }
 
+// Deposit of Hilbert-ordered particles: the 256 particles of a block occupy a
+// compact region, so their CIC footprint is first accumulated in an LDS tile with
+// 64-bit LDS atomics and then added to the mesh with one global atomic per
+// touched cell.  Blocks whose footprint does not fit the tile use global atomics.
+static constexpr int kTile = 4096;
+
+__device__ inline int wrap_cell(int v, int G) { return v < 0 ? v + G : (v >= G ? v - G : v); }
+
+__global__ __launch_bounds__(B) void k_cic_deposit_sorted(const double *__restrict__ px, const double *__restrict__ py,
+                                                          const double *__restrict__ pz, const double *__restrict__ pm,
+                                                          int64_t n, int G, double L, const double *__restrict__ scale_p,
+                                                          unsigned long long *rho)
+{
+  __shared__ unsigned long long s_tile[kTile];
+  __shared__ int s_lo[3], s_hi[3];
+  const int64_t i = (int64_t)blockIdx.x * B + threadIdx.x;
+  const bool has  = i < n;
+  const int64_t i0 = (int64_t)blockIdx.x * B;
+  double p[3]     = {0, 0, 0};
+  if(has)
+    {
+      p[0] = px[i];
+      p[1] = py[i];
+      p[2] = pz[i];
+    }
+  Cic c = cic_of(p, G, L);
+  // cell offsets relative to the block's first particle, unwrapped to (-G/2, G/2]
+  double q[3] = {px[i0], py[i0], pz[i0]};
+  Cic c0      = cic_of(q, G, L);
+  int off[3];
+  for(int k = 0; k < 3; k++)
+    {
+      int d = c.s[k] - c0.s[k];
+      if(d > G / 2)
+        d -= G;
+      if(d <= -G / 2)
+        d += G;
+      off[k] = d;
+    }
+  if(threadIdx.x < 3)
+    {
+      s_lo[threadIdx.x] = 1 << 30;
+      s_hi[threadIdx.x] = -(1 << 30);
+    }
+  __syncthreads();
+  if(has)
+    for(int k = 0; k < 3; k++)
+      {
+        atomicMin(&s_lo[k], off[k]);
+        atomicMax(&s_hi[k], off[k]);
+      }
+  __syncthreads();
+  const int ex = s_hi[0] - s_lo[0] + 2, ey = s_hi[1] - s_lo[1] + 2, ez = s_hi[2] - s_lo[2] + 2;
+  const bool tiled = (int64_t)ex * ey * ez <= kTile && ex <= G && ey <= G && ez <= G;
+  const double scale = scale_p[1];
+  const double m     = has ? pm[i] * scale : 0.0;
+  const double *d    = c.d;
+  const double w[8]  = {(1.0 - d[0]) * (1.0 - d[1]) * (1.0 - d[2]), (1.0 - d[0]) * (1.0 - d[1]) * (d[2]),
+                        (1.0 - d[0]) * (d[1]) * (1.0 - d[2]),       (1.0 - d[0]) * (d[1]) * (d[2]),
+                        (d[0]) * (1.0 - d[1]) * (1.0 - d[2]),       (d[0]) * (1.0 - d[1]) * (d[2]),
+                        (d[0]) * (d[1]) * (1.0 - d[2]),             (d[0]) * (d[1]) * (d[2])};
+  if(tiled)
+    {
+      for(int t = threadIdx.x; t < ex * ey * ez; t += B)
+        s_tile[t] = 0;
+      __syncthreads();
+      if(has)
+        {
+          const int bx = off[0] - s_lo[0], by = off[1] - s_lo[1], bz = off[2] - s_lo[2];
+#pragma unroll
+          for(int q8 = 0; q8 < 8; q8++)
+            {
+              int ox = (q8 >> 2) & 1, oy = (q8 >> 1) & 1, oz = q8 & 1;
+              int t  = ((bx + ox) * ey + (by + oy)) * ez + (bz + oz);
+              atomicAdd(&s_tile[t], (unsigned long long)llrint(m * w[q8]));
+            }
+        }
+      __syncthreads();
+      for(int t = threadIdx.x; t < ex * ey * ez; t += B)
+        {
+          unsigned long long v = s_tile[t];
+          if(v)
+            {
+              int tz = t % ez, ty = (t / ez) % ey, tx = t / (ez * ey);
+              int gx = wrap_cell(c0.s[0] + s_lo[0] + tx, G), gy = wrap_cell(c0.s[1] + s_lo[1] + ty, G),
+                  gz = wrap_cell(c0.s[2] + s_lo[2] + tz, G);
+              atomicAdd(&rho[((size_t)gx * G + gy) * G + gz], v);
+            }
+        }
+    }
+  else if(has)
+    {
+#pragma unroll
+      for(int q8 = 0; q8 < 8; q8++)
+        {
+          int gx = ((q8 >> 2) & 1) ? c.ss[0] : c.s[0], gy = ((q8 >> 1) & 1) ? c.ss[1] : c.s[1],
+              gz = (q8 & 1) ? c.ss[2] : c.s[2];
+          atomicAdd(&rho[((size_t)gx * G + gy) * G + gz], (unsigned long long)llrint(m * w[q8]));
+        }
+    }
+}
+
+template <typename R>
+__global__ void k_cic_readout_sorted(const double *__restrict__ px, const double *__restrict__ py, const double *__restrict__ pz,
+                                     int64_t n, int G, double L, const R *__restrict__ phi, const R *__restrict__ force,
+                                     double fac_pot, double *acc_pm, double *pot_pm)
+{
+  int64_t i = (int64_t)blockIdx.x * B + threadIdx.x;
+  if(i >= n)
+    return;
+  double p[3]     = {px[i], py[i], pz[i]};
+  Cic c           = cic_of(p, G, L);
+  const double *d = c.d;
+  int64_t g3      = (int64_t)G * G * G;
+  int64_t idx[8]  = {((int64_t)c.s[0] * G + c.s[1]) * G + c.s[2],   ((int64_t)c.s[0] * G + c.s[1]) * G + c.ss[2],
+                     ((int64_t)c.s[0] * G + c.ss[1]) * G + c.s[2],  ((int64_t)c.s[0] * G + c.ss[1]) * G + c.ss[2],
+                     ((int64_t)c.ss[0] * G + c.s[1]) * G + c.s[2],  ((int64_t)c.ss[0] * G + c.s[1]) * G + c.ss[2],
+                     ((int64_t)c.ss[0] * G + c.ss[1]) * G + c.s[2], ((int64_t)c.ss[0] * G + c.ss[1]) * G + c.ss[2]};
+  double w[8] = {(1.0 - d[0]) * (1.0 - d[1]) * (1.0 - d[2]), (1.0 - d[0]) * (1.0 - d[1]) * (d[2]),
+                 (1.0 - d[0]) * (d[1]) * (1.0 - d[2]),       (1.0 - d[0]) * (d[1]) * (d[2]),
+                 (d[0]) * (1.0 - d[1]) * (1.0 - d[2]),       (d[0]) * (1.0 - d[1]) * (d[2]),
+                 (d[0]) * (d[1]) * (1.0 - d[2]),             (d[0]) * (d[1]) * (d[2])};
+  double pp = 0, f0 = 0, f1 = 0, f2 = 0;
+#pragma unroll
+  for(int q = 0; q < 8; q++)
+    {
+      pp += (double)phi[idx[q]] * w[q];
+      f0 += (double)force[idx[q]] * w[q];
+      f1 += (double)force[g3 + idx[q]] * w[q];
+      f2 += (double)force[2 * g3 + idx[q]] * w[q];
+    }
+  pot_pm[i]         = pp * fac_pot;
+  acc_pm[3 * i + 0] = f0;
+  acc_pm[3 * i + 1] = f1;
+  acc_pm[3 * i + 2] = f2;
+}
+
 template <typename R>
 __global__ void k_fix_to_real(const unsigned long long *__restrict__ rho, int64_t g3, const double *__restrict__ scale_p, R *out)
 {
@@ -319,7 +456,12 @@ template <typename R> static agrav_status pm_run(agrav_ctx *c)
   k_mass_partial<<<256, B, 0, s>>>(c->d_mass_in.p, n, part);
   k_mass_final<<<1, 1, 0, s>>>(part, 256, part + 512);
   AG_HIP(hipMemsetAsync(p.rho_fix.p, 0, g3 * sizeof(unsigned long long), s));
-  if(n > 0)
+  // with a current tree the particles are available in Hilbert order: tiled deposit and
+  // coherent readout, results kept in sorted order (the finalize kernel maps them back)
+  const bool sorted = c->have_tree;
+  if(n > 0 && sorted)
+    k_cic_deposit_sorted<<<nb(n), B, 0, s>>>(c->d_px.p, c->d_py.p, c->d_pz.p, c->d_pm.p, n, G, L, part + 512, p.rho_fix.p);
+  else if(n > 0)
     k_cic_deposit<<<nb(n), B, 0, s>>>(c->d_pos_in.p, c->d_mass_in.p, n, G, L, part + 512, p.rho_fix.p);
   k_fix_to_real<R><<<nb(g3), B, 0, s>>>(p.rho_fix.p, g3, part + 512, (R *)p.real.p);
   AG_HIP(hipGetLastError());
@@ -336,10 +478,14 @@ template <typename R> static agrav_status pm_run(agrav_ctx *c)
   const double fac_pot = 4 * M_PI * c->par.G / (L * L * L);  // pm_periodic.c:1590
   const double fac_fd  = fac_pot / (2 * (L / G));            // :1592
   k_fd4<R><<<nb(g3), B, 0, s>>>((const R *)p.real.p, G, fac_fd, (R *)p.force.p);
-  if(n > 0)
+  if(n > 0 && sorted)
+    k_cic_readout_sorted<R><<<nb(n), B, 0, s>>>(c->d_px.p, c->d_py.p, c->d_pz.p, n, G, L, (const R *)p.real.p,
+                                                (const R *)p.force.p, fac_pot, c->d_acc_pm.p, c->d_pot_pm.p);
+  else if(n > 0)
     k_cic_readout<R><<<nb(n), B, 0, s>>>(c->d_pos_in.p, n, G, L, (const R *)p.real.p, (const R *)p.force.p, fac_pot,
                                          c->d_acc_pm.p, c->d_pot_pm.p);
   AG_HIP(hipGetLastError());
+  c->pm_sorted = sorted;
   return AGRAV_OK;
 }
 

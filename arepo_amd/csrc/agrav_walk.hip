@@ -290,8 +290,11 @@ __global__ __launch_bounds__(256) void k_walk_plain(TreeView tv, WalkConst wc, c
 // group walk
 // ---------------------------------------------------------------------------
 
-static constexpr int kStackCap = 2048;
-static constexpr int kListCap  = 256;
+static constexpr int kStackCap = 1024;
+static constexpr int kListCap  = 128;
+// an opened node with at most this many particles is expanded straight into the
+// interaction tile (its particles are contiguous in Hilbert order)
+static constexpr int kLeafAppend = 8;
 
 __device__ inline int wave_incl_scan_i(int v)
 {
@@ -469,8 +472,9 @@ __global__ __launch_bounds__(64) void k_walk_group(TreeView tv, WalkConst wc, co
       sp             = base;
       __builtin_amdgcn_wave_barrier();
 
-      int act = 0;  // 1 accept node, 2 open node, 3 particle
+      int act = 0;  // 1 accept node, 2 open node, 3 particle, 4 expand small node into particles
       int j   = 0;
+      int pbase = 0, pcnt = 0;
       Vec4<T> item{0, 0, 0, 0};
       T hj = 0;
       if(e >= N)
@@ -530,6 +534,16 @@ __global__ __launch_bounds__(64) void k_walk_group(TreeView tv, WalkConst wc, co
                 }
               act  = open ? 2 : 1;
               item = Vec4<T>{sxr, syr, szr, m};
+              if(open)
+                {
+                  const int2 rg = tv.nrange[j];
+                  if(rg.y <= kLeafAppend)
+                    {
+                      act   = 4;
+                      pbase = rg.x;
+                      pcnt  = rg.y;
+                    }
+                }
             }
         }
       else if(e >= 0)
@@ -586,6 +600,52 @@ __global__ __launch_bounds__(64) void k_walk_group(TreeView tv, WalkConst wc, co
             s_h[slot] = hj;
         }
       nl += cnt;
+
+      // small opened nodes: their particle ranges go straight into the tile, one
+      // particle per lane and round so that a round never adds more than 64 entries
+      {
+        const int pc   = (act == 4) ? pcnt : 0;
+        const int maxc = wave_max(pc);
+        for(int q = 0; q < maxc; q++)
+          {
+            const bool mine    = q < pc;
+            const uint64_t rb  = __ballot(mine);
+            const int rcnt     = __popcll(rb);
+            if(nl + rcnt > kListCap)
+              {
+                __builtin_amdgcn_wave_barrier();
+                tile_compute<T, PM, EWALD, MSOFT>(s_list, s_h, nl, tx, ty, tz, hi, s_tab, wc, asmthfac, asmthinv2, ewald_f,
+                                                  ewald_d, ax, ay, az, pp, cax, cay, caz, cpp, nin);
+                nl = 0;
+                __builtin_amdgcn_wave_barrier();
+              }
+            if(mine)
+              {
+                const int e2 = pbase + q;
+                T x, y, z, m;
+                if constexpr(sizeof(T) == 4)
+                  {
+                    uint4 P = tv.prec[e2];
+                    x       = (T)(int)(P.x - cfx) * unit;
+                    y       = (T)(int)(P.y - cfy) * unit;
+                    z       = (T)(int)(P.z - cfz) * unit;
+                    m       = __uint_as_float(P.w);
+                  }
+                else
+                  {
+                    x = nearest_d(tv.px[e2] - c0, wc);
+                    y = nearest_d(tv.py[e2] - c1, wc);
+                    z = nearest_d(tv.pz[e2] - c2, wc);
+                    m = tv.pm[e2];
+                  }
+                const int slot = nl + __popcll(rb & lt);
+                s_list[slot]   = Vec4<T>{x, y, z, m};
+                if(MSOFT)
+                  s_h[slot] = (T)hsoft[tv.ps[e2]];
+              }
+            nl += rcnt;
+          }
+      }
 
       // push the children of opened nodes
       const int nch  = (act == 2) ? tv.ccnt[j] : 0;

@@ -100,6 +100,7 @@ def main():
     ap.add_argument("--n", type=int, default=128 ** 3)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--err-sample", type=int, default=16384)
+    ap.add_argument("--cpu-sample", type=int, default=262144)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-errors", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
@@ -153,6 +154,8 @@ def main():
                        "p99": float(np.percentile(r32["step_ms"], 99))},
         "stage_ms": {"treebuild": tim["treebuild"], "treewalk": tim["treewalk"], "pm_grav": tim["pm_grav"],
                      "finalize": tim["finalize"]},
+        "numnodes": tim["numnodes"],
+        "walk_fallback_groups": tim["walk_retries"],
     }
 
     # roofline of the dominant kernel (the walk): algorithmic bytes = one 16-byte source record per
@@ -179,8 +182,10 @@ def main():
     cpu = None
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
-        log("CPU oracle baseline: %d threads, walk sample %d" % (threads, len(sel)))
-        cpu = cpu_baseline(pos, mass, eps, res32["old_acc"], sel, threads)
+        extra = np.random.Generator(np.random.Philox(98)).choice(n, min(args.cpu_sample, n), replace=False)
+        cpu_sel = np.union1d(sel, extra).astype(np.int32)
+        log("CPU oracle baseline: %d threads, walk sample %d" % (threads, len(cpu_sel)))
+        cpu = cpu_baseline(pos, mass, eps, res32["old_acc"], cpu_sel, threads)
         out["cpu_baseline"] = {"value": cpu["value"], "unit": "particles/s", "cores": threads, "kind": "port",
                                "sample": "full tree build + 128^3 PM on the host, relative walk of %d random "
                                          "targets scaled to N (build %.2f s, pm %.2f s, walk sample %.2f s)"
@@ -203,7 +208,8 @@ def main():
             errs[name] = {"median": float(np.median(e)), "p99": float(np.percentile(e, 99))}
         if cpu is not None:
             # the oracle's own TreePM error on its sample targets (tree part from the oracle, PM from the oracle)
-            e = rel_err(cpu["acc_tree"] + cpu["acc_pm"][sel], ft["acc"])
+            pos_in = np.searchsorted(cpu["sel"], sel)
+            e = rel_err(cpu["acc_tree"][pos_in] + cpu["acc_pm"][sel], ft["acc"])
             errs["cpu_oracle"] = {"median": float(np.median(e)), "p99": float(np.percentile(e, 99))}
         out["force_error_vs_direct"] = errs
         out["force_error_sample"] = int(args.err_sample)
