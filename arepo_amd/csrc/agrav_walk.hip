@@ -344,7 +344,7 @@ template <typename T> struct Vec4
 };
 
 template <typename T, bool PM, bool EWALD, bool MSOFT>
-__device__ inline void tile_compute(const Vec4<T> *__restrict__ s_list, const T *__restrict__ s_h, int nl, T tx, T ty, T tz, T hi,
+__device__ inline void tile_compute_generic(const Vec4<T> *__restrict__ s_list, const T *__restrict__ s_h, int nl, T tx, T ty, T tz, T hi,
                                     const float *__restrict__ s_tab, const WalkConst &wc, T asmthfac, T asmthinv2,
                                     const float *__restrict__ ewald_f, const double *__restrict__ ewald_d, T &ax, T &ay, T &az, T &pp,
                                     T &cax, T &cay, T &caz, T &cpp, int &nin)
@@ -381,6 +381,111 @@ __device__ inline void tile_compute(const Vec4<T> *__restrict__ s_list, const T 
   kahan_add(ay, cay, fy);
   kahan_add(az, caz, fz);
   kahan_add(pp, cpp, fp);
+}
+
+// FP32 tile loop: one reciprocal square root per interaction (v_rsq_f32, about
+// 1 ulp), the short-range factors from the LDS tables without branches, and the
+// softened kernel only for the rare entries inside the softening length.  The
+// arithmetic is the reference's pair kernel (forcetree_walk.c:364-487).
+template <bool PM, bool EWALD, bool MSOFT>
+__device__ inline void tile_compute_f32(const Vec4<float> *__restrict__ s_list, const float *__restrict__ s_h, int nl, float tx,
+                                        float ty, float tz, float hi, const float *__restrict__ s_tab, const WalkConst &wc,
+                                        float asmthfac, float asmthinv2, const float *__restrict__ ewald_f, float &ax, float &ay,
+                                        float &az, float &pp, float &cax, float &cay, float &caz, float &cpp, int &nin)
+{
+  float fx = 0, fy = 0, fz = 0, fp = 0;
+  const float hi_inv = __builtin_amdgcn_rcpf(hi);
+  for(int k = 0; k < nl; k++)
+    {
+      const Vec4<float> e = s_list[k];
+      const float dx = e.x - tx, dy = e.y - ty, dz = e.z - tz;
+      const float r2   = dx * dx + dy * dy + dz * dz;
+      const float rinv = __builtin_amdgcn_rsqf(fmaxf(r2, 1e-30f));
+      const float r    = r2 * rinv;
+      float hmax = hi, hinv = hi_inv;
+      if(MSOFT)
+        {
+          hmax = s_h[k] > hi ? s_h[k] : hi;
+          hinv = __builtin_amdgcn_rcpf(hmax);
+        }
+      float ff = 0, fpt = 0;
+      bool valid = e.w != 0.f;
+      if(PM)
+        {
+          const float te = asmthfac * r;
+          const int ti   = (int)te;
+          valid          = valid && ti < AG_NTAB;
+          const int tc   = ti < AG_NTAB ? ti : AG_NTAB - 1;
+          const float w  = te - (float)tc;
+          const float f0 = s_tab[tc], f1 = s_tab[tc + 1];
+          const float p0 = s_tab[128 + tc], p1 = s_tab[128 + tc + 1];
+          ff             = f0 + w * (f1 - f0);
+          fpt            = p0 + w * (p1 - p0);
+        }
+      const float rinv2 = rinv * rinv;
+      float fac, wp;
+      if(PM)
+        {
+          fac = rinv2 * rinv + rinv * ff * asmthinv2;
+          wp  = rinv * fpt;
+        }
+      else
+        {
+          fac = rinv2 * rinv;
+          wp  = -rinv;
+        }
+      if(r < hmax)
+        {
+          // spline softening (allvars.h:557-573); u < 1 here
+          const float h3 = hinv * hinv * hinv;
+          const float u  = r * hinv;
+          const float u2 = u * u;
+          if(u < 0.5f)
+            {
+              fac = h3 * ((float)SF1 + u2 * ((float)SF2 * u + (float)SF3));
+              wp  = hinv * ((float)SF4 + u2 * ((float)SF5 + u2 * ((float)SF6 * u + (float)SF7)));
+            }
+          else
+            {
+              const float ui = __builtin_amdgcn_rcpf(u);
+              const float u3 = u2 * u;
+              fac = h3 * ((float)SF8 + (float)SF9 * u + (float)SF10 * u2 + (float)SF11 * u3 + (float)SF12 * ui * ui * ui);
+              wp  = hinv * ((float)SF13 + (float)SF14 * ui + u2 * ((float)SF1 + u * ((float)SF15 + u * ((float)SF16 + (float)SF17 * u))));
+            }
+          if(PM && r2 > 0.f)
+            {
+              fac += rinv * ff * asmthinv2;
+              wp += rinv * (fpt + 1.0f);
+            }
+        }
+      const float m = valid ? e.w : 0.f;
+      fp += m * wp;
+      const float mf = m * fac;
+      fx += dx * mf;
+      fy += dy * mf;
+      fz += dz * mf;
+      if(EWALD && valid)
+        ewald_add<float, float>(ewald_f, (float)wc.ewald_fac, dx, dy, dz, e.w, fx, fy, fz, fp);
+      nin += valid ? 1 : 0;
+    }
+  kahan_add(ax, cax, fx);
+  kahan_add(ay, cay, fy);
+  kahan_add(az, caz, fz);
+  kahan_add(pp, cpp, fp);
+}
+
+template <typename T, bool PM, bool EWALD, bool MSOFT>
+__device__ inline void tile_compute(const Vec4<T> *__restrict__ s_list, const T *__restrict__ s_h, int nl, T tx, T ty, T tz, T hi,
+                                    const float *__restrict__ s_tab, const WalkConst &wc, T asmthfac, T asmthinv2,
+                                    const float *__restrict__ ewald_f, const double *__restrict__ ewald_d, T &ax, T &ay, T &az, T &pp,
+                                    T &cax, T &cay, T &caz, T &cpp, int &nin)
+{
+  if constexpr(sizeof(T) == 4)
+    tile_compute_f32<PM, EWALD, MSOFT>(s_list, s_h, nl, tx, ty, tz, hi, s_tab, wc, asmthfac, asmthinv2, ewald_f, ax, ay, az, pp,
+                                       cax, cay, caz, cpp, nin);
+  else
+    tile_compute_generic<T, PM, EWALD, MSOFT>(s_list, s_h, nl, tx, ty, tz, hi, s_tab, wc, asmthfac, asmthinv2, ewald_f, ewald_d,
+                                              ax, ay, az, pp, cax, cay, caz, cpp, nin);
 }
 
 template <typename T, bool PM, bool EWALD, bool MSOFT>

@@ -71,6 +71,22 @@ def run_path(A, pos, mass, eps, precision, walk, steps, warmup, old_acc=None):
     return dict(total_s=total, step_ms=step_ms, timings=tim), res
 
 
+def copy_probe(nbytes=256 << 20):
+    """One device-to-device copy of known size (the runtime's copy kernel), used to scale the
+    FETCH_SIZE / WRITE_SIZE counters of a profiled run."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    hip.hipMemset(a, 1, C.c_size_t(nbytes))
+    hip.hipDeviceSynchronize()
+    assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log("copy probe: %d bytes device-to-device" % nbytes)
+
+
 def cpu_baseline(pos, mass, eps, old_acc, sel, threads):
     """Oracle (FP64 AREPO restatement) on the host: full tree build and PM, walk timed on a random
     sample of targets and scaled to all particles."""
@@ -104,6 +120,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-errors", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -121,6 +138,9 @@ def main():
     # one seeded target subset serves the error measurement and the CPU sample
     sel = np.sort(np.random.Generator(np.random.Philox(99)).choice(n, min(args.err_sample, n), replace=False))
     sel = sel.astype(np.int32)
+
+    if args.copy_probe:
+        copy_probe()
 
     # FP32 group walk: the measured path
     log("FP32 group walk: warmup %d, steps %d" % (args.warmup, args.steps))
