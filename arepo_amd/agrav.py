@@ -46,6 +46,7 @@ class Params(C.Structure):
         ("walk", C.c_int),
         ("group_size", C.c_int),
         ("tree_alloc_factor", C.c_double),
+        ("leaf_bucket", C.c_int),
     ]
 
 
@@ -120,10 +121,6 @@ def _declare(L):
     L.agrav_hilbert_key21.restype = C.c_uint64
     L.agrav_hilbert_key21.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
     for name in EXPORTED:
-        fn = getattr(L, name)
-        if fn.restype is C.c_int or name in ("agrav_destroy", "agrav_default_params"):
-            pass
-    for name in EXPORTED:
         if name not in ("agrav_status_string", "agrav_last_error", "agrav_hilbert_key21", "agrav_destroy",
                         "agrav_default_params"):
             getattr(L, name).restype = C.c_int
@@ -132,7 +129,8 @@ def _declare(L):
 
 
 def make_params(periodic=True, box_size=1.0, pmgrid=0, asmth=1.25, rcut=4.5, G=1.0, err_tol_theta=0.7,
-                err_tol_force_acc=0.0025, opening=1, soft=(0.01,), precision=FP64, walk=WALK_GROUP, group_size=64):
+                err_tol_force_acc=0.0025, opening=1, soft=(0.01,), precision=FP64, walk=WALK_GROUP, group_size=64,
+                leaf_bucket=8):
     p = Params()
     lib().agrav_default_params(C.byref(p))
     p.periodic = int(periodic)
@@ -150,6 +148,7 @@ def make_params(periodic=True, box_size=1.0, pmgrid=0, asmth=1.25, rcut=4.5, G=1
     p.precision = precision
     p.walk = walk
     p.group_size = group_size
+    p.leaf_bucket = leaf_bucket
     return p
 
 

@@ -170,6 +170,7 @@ void agrav_default_params(agrav_params *p)
   p->walk                      = AGRAV_WALK_GROUP;
   p->group_size                = 64;
   p->tree_alloc_factor         = 0;
+  p->leaf_bucket               = 8;
 }
 
 static agrav_status check_params(const agrav_params *p)
@@ -200,6 +201,8 @@ static agrav_status check_params(const agrav_params *p)
   if(p->precision == AGRAV_FP32 && p->walk == AGRAV_WALK_PLAIN)
     return AGRAV_ERR_BAD_PARAM;  // the per-target walk is the FP64 restatement only
   if(p->group_size < 0 || p->group_size > 64)
+    return AGRAV_ERR_BAD_PARAM;
+  if(p->leaf_bucket < 0 || p->leaf_bucket > 64)
     return AGRAV_ERR_BAD_PARAM;
   return AGRAV_OK;
 }
@@ -333,6 +336,7 @@ void agrav_destroy(agrav_ctx *c)
   c->d_nrange.release();
   c->d_cbeg.release();
   c->d_ccnt.release();
+  c->d_cnn.release();
   c->d_child.release();
   c->d_lvlcnt.release();
   c->d_lvloff.release();

@@ -484,18 +484,30 @@ __global__ void k_moments(int cnt, const int *__restrict__ list, int n, const in
   recB[j] = make_uint4(to_fix(cx[j], dom, 0), to_fix(cy[j], dom, 1), to_fix(cz[j], dom, 2), __float_as_uint((float)len[j]));
 }
 
+// CSR child lists, node children first (the group walk pushes them) and then
+// particle children (the group walk streams them straight into its tile)
 __global__ void k_fill_children(int m, int n, const int *__restrict__ nextnode, const int *__restrict__ sibling,
                                 const int *__restrict__ pnext, const int *__restrict__ cbeg, const int *__restrict__ ccnt,
-                                int *child)
+                                int *child, int *cnn)
 {
   int j = blockIdx.x * B + threadIdx.x;
   if(j >= m)
     return;
-  int c = nextnode[j], end = sibling[j], o = cbeg[j], stop = cbeg[j] + ccnt[j];
-  while(c != end && c >= 0 && o < stop)
+  const int end = sibling[j], stop = cbeg[j] + ccnt[j];
+  int o = cbeg[j], steps = 0;
+  for(int c = nextnode[j]; c != end && c >= 0 && steps < ccnt[j]; steps++)
     {
-      child[o++] = c;
-      c          = (c < n) ? pnext[c] : sibling[c - n];
+      if(c >= n && o < stop)
+        child[o++] = c;
+      c = (c < n) ? pnext[c] : sibling[c - n];
+    }
+  cnn[j] = o - cbeg[j];
+  steps  = 0;
+  for(int c = nextnode[j]; c != end && c >= 0 && steps < ccnt[j]; steps++)
+    {
+      if(c < n && o < stop)
+        child[o++] = c;
+      c = (c < n) ? pnext[c] : sibling[c - n];
     }
 }
 
@@ -597,7 +609,7 @@ agrav_status build_tree(agrav_ctx *c)
      (st = c->d_sy.ensure(M)) || (st = c->d_sz.ensure(M)) || (st = c->d_nm.ensure(M)) || (st = c->d_cx.ensure(M)) ||
      (st = c->d_cy.ensure(M)) || (st = c->d_cz.ensure(M)) || (st = c->d_len.ensure(M)) || (st = c->d_nextnode.ensure(M)) ||
      (st = c->d_sibling.ensure(M)) || (st = c->d_maxsoft.ensure(M)) || (st = c->d_recA.ensure(M)) ||
-     (st = c->d_recB.ensure(M)) || (st = c->d_nrange.ensure(M)) || (st = c->d_cbeg.ensure(M)) || (st = c->d_ccnt.ensure(M)) ||
+     (st = c->d_recB.ensure(M)) || (st = c->d_nrange.ensure(M)) || (st = c->d_cbeg.ensure(M)) || (st = c->d_ccnt.ensure(M)) || (st = c->d_cnn.ensure(M)) ||
      (st = c->d_child.ensure((size_t)M + n)) || (st = c->d_lvlnodes.ensure(M)))
     return st;
   if(N > 1)
@@ -646,7 +658,7 @@ agrav_status build_tree(agrav_ctx *c)
   if((st = exclusive_scan_i32(c->d_ccnt.p, c->d_cbeg.p, M, nullptr, c->d_temp.p, s)) != AGRAV_OK)
     return st;
   k_fill_children<<<nblk(M), B, 0, s>>>(M, N, c->d_nextnode.p, c->d_sibling.p, c->d_pnext.p, c->d_cbeg.p, c->d_ccnt.p,
-                                         c->d_child.p);
+                                         c->d_child.p, c->d_cnn.p);
   AG_HIP(hipGetLastError());
 
   // out-of-box check (forcetree.c:432-438 aborts; here it is an error code)
@@ -691,6 +703,7 @@ TreeView tree_view(const agrav_ctx *c)
   v.nrange   = c->d_nrange.p;
   v.cbeg     = c->d_cbeg.p;
   v.ccnt     = c->d_ccnt.p;
+  v.cnn      = c->d_cnn.p;
   v.child    = c->d_child.p;
   return v;
 }

@@ -110,7 +110,7 @@ struct agrav_ctx
   agrav::DBuf<uint8_t> d_maxsoft;
   agrav::DBuf<uint4> d_recA, d_recB;
   agrav::DBuf<int2> d_nrange;
-  agrav::DBuf<int> d_cbeg, d_ccnt, d_child;
+  agrav::DBuf<int> d_cbeg, d_ccnt, d_cnn, d_child;
   agrav::DBuf<int> d_lvlcnt, d_lvloff, d_lvlnodes;
   int lvlcnt_h[AG_KEY_LEVELS + 2];
 

@@ -72,7 +72,8 @@ struct TreeView
   const uint4 *recB;      // {cenfx, cenfy, cenfz, len bits}
   const int2 *nrange;     // {first particle, particle count} per node
   const int *cbeg;        // children CSR
-  const int *ccnt;
+  const int *ccnt;         // children per node
+  const int *cnn;          // of which nodes (listed first)
   const int *child;
 };
 
@@ -89,6 +90,7 @@ struct WalkConst
   double ewald_fac;   // 2 EN / box
   int multisoft;
   float fix_unit;     // fixed-point step as float
+  int leaf_bucket;    // opened nodes with at most this many particles are expanded into particles
 };
 
 // hip error helper

@@ -292,9 +292,6 @@ __global__ __launch_bounds__(256) void k_walk_plain(TreeView tv, WalkConst wc, c
 
 static constexpr int kStackCap = 1024;
 static constexpr int kListCap  = 128;
-// an opened node with at most this many particles is expanded straight into the
-// interaction tile (its particles are contiguous in Hilbert order)
-static constexpr int kLeafAppend = 8;
 
 __device__ inline int wave_incl_scan_i(int v)
 {
@@ -641,8 +638,10 @@ __global__ __launch_bounds__(64) void k_walk_group(TreeView tv, WalkConst wc, co
               item = Vec4<T>{sxr, syr, szr, m};
               if(open)
                 {
+                  // a small opened node is expanded straight into its particles, which are
+                  // contiguous in Hilbert order
                   const int2 rg = tv.nrange[j];
-                  if(rg.y <= kLeafAppend)
+                  if(rg.y <= wc.leaf_bucket)
                     {
                       act   = 4;
                       pbase = rg.x;
@@ -706,28 +705,30 @@ __global__ __launch_bounds__(64) void k_walk_group(TreeView tv, WalkConst wc, co
         }
       nl += cnt;
 
-      // small opened nodes: their particle ranges go straight into the tile, one
-      // particle per lane and round so that a round never adds more than 64 entries
+      // particles of small opened nodes (contiguous ranges) and the particle children of
+      // opened nodes go straight into the tile, one particle per lane and round so that a
+      // round never adds more than 64 entries; particles beyond Rcut of the whole group
+      // are dropped here
       {
-        const int pc   = (act == 4) ? pcnt : 0;
+        int pc = 0;
+        bool from_list = false;
+        if(act == 4)
+          pc = pcnt;
+        else if(act == 2)
+          {
+            const int nn = tv.cnn[j];
+            pc           = tv.ccnt[j] - nn;
+            pbase        = tv.cbeg[j] + nn;
+            from_list    = true;
+          }
         const int maxc = wave_max(pc);
         for(int q = 0; q < maxc; q++)
           {
-            const bool mine    = q < pc;
-            const uint64_t rb  = __ballot(mine);
-            const int rcnt     = __popcll(rb);
-            if(nl + rcnt > kListCap)
+            bool keep = q < pc;
+            T x = 0, y = 0, z = 0, m = 0, hp = 0;
+            if(keep)
               {
-                __builtin_amdgcn_wave_barrier();
-                tile_compute<T, PM, EWALD, MSOFT>(s_list, s_h, nl, tx, ty, tz, hi, s_tab, wc, asmthfac, asmthinv2, ewald_f,
-                                                  ewald_d, ax, ay, az, pp, cax, cay, caz, cpp, nin);
-                nl = 0;
-                __builtin_amdgcn_wave_barrier();
-              }
-            if(mine)
-              {
-                const int e2 = pbase + q;
-                T x, y, z, m;
+                const int e2 = from_list ? tv.child[pbase + q] : pbase + q;
                 if constexpr(sizeof(T) == 4)
                   {
                     uint4 P = tv.prec[e2];
@@ -743,17 +744,40 @@ __global__ __launch_bounds__(64) void k_walk_group(TreeView tv, WalkConst wc, co
                     z = nearest_d(tv.pz[e2] - c2, wc);
                     m = tv.pm[e2];
                   }
+                if(MSOFT)
+                  hp = (T)hsoft[tv.ps[e2]];
+                if(PM)
+                  {
+                    T ddx = (T)fabs(x) - hx, ddy = (T)fabs(y) - hy, ddz = (T)fabs(z) - hz;
+                    ddx   = ddx > 0 ? ddx : 0;
+                    ddy   = ddy > 0 ? ddy : 0;
+                    ddz   = ddz > 0 ? ddz : 0;
+                    keep  = ddx * ddx + ddy * ddy + ddz * ddz <= rcut2 * (T)1.0001;
+                  }
+              }
+            const uint64_t rb = __ballot(keep);
+            const int rcnt    = __popcll(rb);
+            if(nl + rcnt > kListCap)
+              {
+                __builtin_amdgcn_wave_barrier();
+                tile_compute<T, PM, EWALD, MSOFT>(s_list, s_h, nl, tx, ty, tz, hi, s_tab, wc, asmthfac, asmthinv2, ewald_f,
+                                                  ewald_d, ax, ay, az, pp, cax, cay, caz, cpp, nin);
+                nl = 0;
+                __builtin_amdgcn_wave_barrier();
+              }
+            if(keep)
+              {
                 const int slot = nl + __popcll(rb & lt);
                 s_list[slot]   = Vec4<T>{x, y, z, m};
                 if(MSOFT)
-                  s_h[slot] = (T)hsoft[tv.ps[e2]];
+                  s_h[slot] = hp;
               }
             nl += rcnt;
           }
       }
 
       // push the children of opened nodes
-      const int nch  = (act == 2) ? tv.ccnt[j] : 0;
+      const int nch  = (act == 2) ? tv.cnn[j] : 0;  // node children only
       const int incl = wave_incl_scan_i(nch);
       const int tot  = __shfl(incl, 63, 64);
       pushed += tot;
@@ -829,6 +853,7 @@ WalkConst walk_const(const agrav_ctx *c, int use_bh)
   wc.ewald_fac = wc.ewald ? 2.0 * 64 / c->par.box_size : 0;
   wc.multisoft = c->soft.nsoft > 1 || c->have_stype;
   wc.fix_unit  = (float)c->h_dom.fix_unit;
+  wc.leaf_bucket = c->par.leaf_bucket > 0 ? c->par.leaf_bucket : 0;
   return wc;
 }
 

@@ -71,7 +71,9 @@ typedef struct
   int precision;                  /* agrav_precision */
   int walk;                       /* agrav_walk_kind */
   int group_size;                 /* targets per group for AGRAV_WALK_GROUP (8..64, 0 = 64) */
-  double tree_alloc_factor;       /* TreeAllocFactor (init.c:110); 0 = 0.7 (10 below 1000 particles) */
+  double tree_alloc_factor;       /* TreeAllocFactor (init.c:110); unused: node arrays are sized exactly */
+  int leaf_bucket;                /* group walk: an opened node with at most this many particles is expanded
+                                     straight into particles (0..64, default 8) */
 } agrav_params;
 
 /* Per-stage device times (ms) of the last call, named like AREPO's cpu.txt
