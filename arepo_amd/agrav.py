@@ -47,6 +47,7 @@ class Params(C.Structure):
         ("group_size", C.c_int),
         ("tree_alloc_factor", C.c_double),
         ("leaf_bucket", C.c_int),
+        ("eval_potential", C.c_int),
     ]
 
 
@@ -60,6 +61,8 @@ class Timings(C.Structure):
         ("interactions", C.c_longlong),
         ("numnodes", C.c_int),
         ("walk_retries", C.c_int),
+        ("walk_popped", C.c_longlong),
+        ("tile_entries", C.c_longlong),
     ]
 
     def as_dict(self):
@@ -130,7 +133,7 @@ def _declare(L):
 
 def make_params(periodic=True, box_size=1.0, pmgrid=0, asmth=1.25, rcut=4.5, G=1.0, err_tol_theta=0.7,
                 err_tol_force_acc=0.0025, opening=1, soft=(0.01,), precision=FP64, walk=WALK_GROUP, group_size=64,
-                leaf_bucket=8):
+                leaf_bucket=8, eval_potential=True):
     p = Params()
     lib().agrav_default_params(C.byref(p))
     p.periodic = int(periodic)
@@ -149,6 +152,7 @@ def make_params(periodic=True, box_size=1.0, pmgrid=0, asmth=1.25, rcut=4.5, G=1
     p.walk = walk
     p.group_size = group_size
     p.leaf_bucket = leaf_bucket
+    p.eval_potential = int(eval_potential)
     return p
 
 

@@ -171,6 +171,7 @@ void agrav_default_params(agrav_params *p)
   p->group_size                = 64;
   p->tree_alloc_factor         = 0;
   p->leaf_bucket               = 8;
+  p->eval_potential            = 1;
 }
 
 static agrav_status check_params(const agrav_params *p)
@@ -341,6 +342,9 @@ void agrav_destroy(agrav_ctx *c)
   c->d_lvlcnt.release();
   c->d_lvloff.release();
   c->d_lvlnodes.release();
+  c->d_pparent.release();
+  c->d_nparent.release();
+  c->d_arrived.release();
   c->d_at.release();
   c->d_pt.release();
   c->d_nint.release();

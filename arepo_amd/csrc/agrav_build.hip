@@ -387,102 +387,9 @@ __global__ void k_pnext(int n, const unsigned *__restrict__ lvlmask, const int *
     pnext[p] = pnext_of(p, n, lvlmask, pbase);
 }
 
-__global__ __launch_bounds__(B) void k_level_scatter(int m, const int *__restrict__ nlevel, const int *__restrict__ lvloff,
-                                                     int *lvlfill, int *lvlnodes)
-{
-  // block-local ranks per level in LDS, one global reservation per block and level
-  __shared__ int s_cnt[AG_KEY_LEVELS + 1], s_base[AG_KEY_LEVELS + 1];
-  if(threadIdx.x <= AG_KEY_LEVELS)
-    s_cnt[threadIdx.x] = 0;
-  __syncthreads();
-  int j    = blockIdx.x * B + threadIdx.x;
-  int L    = (j < m) ? nlevel[j] : -1;
-  int rank = (L >= 0) ? atomicAdd(&s_cnt[L], 1) : 0;
-  __syncthreads();
-  if(threadIdx.x <= AG_KEY_LEVELS)
-    s_base[threadIdx.x] = s_cnt[threadIdx.x] ? atomicAdd(&lvlfill[threadIdx.x], s_cnt[threadIdx.x]) : 0;
-  __syncthreads();
-  if(L >= 0)
-    lvlnodes[lvloff[L] + s_base[L] + rank] = j;
-}
-
 // ---------------------------------------------------------------------------
-// 5. moments (forcetree.c:1217-1429): mass, FP64 centre of mass, maxsofttype
+// 5. child lists and moments (forcetree.c:1217-1429)
 // ---------------------------------------------------------------------------
-
-__global__ void k_moments(int cnt, const int *__restrict__ list, int n, const int *__restrict__ nextnode,
-                          const int *__restrict__ sibling, const int *__restrict__ pnext, const double *__restrict__ px,
-                          const double *__restrict__ py, const double *__restrict__ pz, const double *__restrict__ pm,
-                          const uint8_t *__restrict__ ps, const double *__restrict__ hsoft, int nsoft, const double *__restrict__ cx,
-                          const double *__restrict__ cy, const double *__restrict__ cz, const double *__restrict__ len,
-                          double *sx, double *sy, double *sz, double *nm, uint8_t *maxsoft, int *ccnt, uint4 *recA, uint4 *recB,
-                          const Domain *__restrict__ domp, int limit, int *err)
-{
-  int t = blockIdx.x * B + threadIdx.x;
-  if(t >= cnt)
-    return;
-  int j   = list[t];
-  int c   = nextnode[j];
-  int end = sibling[j];
-  double m = 0, s0 = 0, s1 = 0, s2 = 0;
-  int ms   = nsoft;  // sentinel type with zero softening (forcetree.c:1254)
-  int k    = 0;
-  while(c != end)
-    {
-      if(k >= limit || c < 0)
-        {
-          atomicExch(err, 1);  // broken threading: stop instead of looping
-          break;
-        }
-      if(c < n)
-        {
-          double mm = pm[c];
-          m += mm;
-          s0 += mm * px[c];
-          s1 += mm * py[c];
-          s2 += mm * pz[c];
-          int st = ps ? ps[c] : 0;
-          if(hsoft[ms] < hsoft[st])
-            ms = st;
-          c = pnext[c];
-        }
-      else
-        {
-          int q     = c - n;
-          double mm = nm[q];
-          m += mm;
-          s0 += mm * sx[q];
-          s1 += mm * sy[q];
-          s2 += mm * sz[q];
-          int st = maxsoft[q];
-          if(hsoft[ms] < hsoft[st])
-            ms = st;
-          c = sibling[q];
-        }
-      k++;
-    }
-  if(m != 0)
-    {
-      s0 /= m;
-      s1 /= m;
-      s2 /= m;
-    }
-  else
-    {
-      s0 = cx[j];
-      s1 = cy[j];
-      s2 = cz[j];
-    }
-  sx[j]      = s0;
-  sy[j]      = s1;
-  sz[j]      = s2;
-  nm[j]      = m;
-  maxsoft[j] = (uint8_t)ms;
-  ccnt[j]    = k;
-  const Domain dom = *domp;
-  recA[j] = make_uint4(to_fix(s0, dom, 0), to_fix(s1, dom, 1), to_fix(s2, dom, 2), __float_as_uint((float)m));
-  recB[j] = make_uint4(to_fix(cx[j], dom, 0), to_fix(cy[j], dom, 1), to_fix(cz[j], dom, 2), __float_as_uint((float)len[j]));
-}
 
 // CSR child lists, node children first (the group walk pushes them) and then
 // particle children (the group walk streams them straight into its tile)
@@ -508,6 +415,123 @@ __global__ void k_fill_children(int m, int n, const int *__restrict__ nextnode, 
       if(c < n && o < stop)
         child[o++] = c;
       c = (c < n) ? pnext[c] : sibling[c - n];
+    }
+}
+
+// children of each node, by walking its threading from nextnode to sibling
+__global__ void k_count_children(int m, int n, const int *__restrict__ nextnode, const int *__restrict__ sibling,
+                                 const int *__restrict__ pnext, int *ccnt, int limit, int *err)
+{
+  int j = blockIdx.x * B + threadIdx.x;
+  if(j >= m)
+    return;
+  int k = 0;
+  for(int c = nextnode[j], end = sibling[j]; c != end; k++)
+    {
+      if(k >= limit || c < 0)
+        {
+          atomicExch(err, 1);  // broken threading: stop instead of looping
+          break;
+        }
+      c = (c < n) ? pnext[c] : sibling[c - n];
+    }
+  ccnt[j] = k;
+}
+
+// parent of every particle and node (-1 for the root)
+__global__ void k_parents(int m, int n, const int *__restrict__ cbeg, const int *__restrict__ ccnt, const int *__restrict__ child,
+                          int *pparent, int *nparent)
+{
+  int j = blockIdx.x * B + threadIdx.x;
+  if(j >= m)
+    return;
+  if(j == 0)
+    nparent[0] = -1;
+  for(int q = cbeg[j], e = cbeg[j] + ccnt[j]; q < e; q++)
+    {
+      int c = child[q];
+      if(c < n)
+        pparent[c] = j;
+      else
+        nparent[c - n] = j;
+    }
+}
+
+// Moments bottom-up in one launch (forcetree.c:1217-1429): a lane per particle
+// climbs towards the root; at every node the last child to arrive (counted with an
+// atomic) sums all children of that node in their fixed CSR order, so the result
+// does not depend on arrival order, and then continues upwards.
+__global__ void k_moments_up(int n, const int *__restrict__ pparent, const int *__restrict__ nparent, const int *__restrict__ cbeg,
+                             const int *__restrict__ ccnt, const int *__restrict__ child, const double *__restrict__ px,
+                             const double *__restrict__ py, const double *__restrict__ pz, const double *__restrict__ pm,
+                             const uint8_t *__restrict__ ps, const double *__restrict__ hsoft, int nsoft,
+                             const double *__restrict__ cx, const double *__restrict__ cy, const double *__restrict__ cz,
+                             const double *__restrict__ len, double *sx, double *sy, double *sz, double *nm, uint8_t *maxsoft,
+                             uint4 *recA, uint4 *recB, int *arrived, const Domain *__restrict__ domp)
+{
+  int p = blockIdx.x * B + threadIdx.x;
+  if(p >= n)
+    return;
+  const Domain dom = *domp;
+  int j            = pparent[p];
+  while(j >= 0)
+    {
+      __threadfence();
+      const int old = atomicAdd(&arrived[j], 1);
+      if(old != ccnt[j] - 1)
+        return;  // another child of j finishes later and carries on
+      __threadfence();
+      double m = 0, s0 = 0, s1 = 0, s2 = 0;
+      int ms = nsoft;  // sentinel type with zero softening (forcetree.c:1254)
+      for(int q = cbeg[j], e = cbeg[j] + ccnt[j]; q < e; q++)
+        {
+          const int c = child[q];
+          double mm, x, y, z;
+          int st;
+          if(c < n)
+            {
+              mm = pm[c];
+              x  = px[c];
+              y  = py[c];
+              z  = pz[c];
+              st = ps ? ps[c] : 0;
+            }
+          else
+            {
+              const int k = c - n;
+              mm          = __hip_atomic_load(&nm[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              x           = __hip_atomic_load(&sx[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              y           = __hip_atomic_load(&sy[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              z           = __hip_atomic_load(&sz[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              st          = __hip_atomic_load(&maxsoft[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+          m += mm;
+          s0 += mm * x;
+          s1 += mm * y;
+          s2 += mm * z;
+          if(hsoft[ms] < hsoft[st])
+            ms = st;
+        }
+      if(m != 0)
+        {
+          s0 /= m;
+          s1 /= m;
+          s2 /= m;
+        }
+      else
+        {
+          s0 = cx[j];
+          s1 = cy[j];
+          s2 = cz[j];
+        }
+      sx[j]      = s0;
+      sy[j]      = s1;
+      sz[j]      = s2;
+      nm[j]      = m;
+      maxsoft[j] = (uint8_t)ms;
+      recA[j]    = make_uint4(to_fix(s0, dom, 0), to_fix(s1, dom, 1), to_fix(s2, dom, 2), __float_as_uint((float)m));
+      recB[j]    = make_uint4(to_fix(cx[j], dom, 0), to_fix(cy[j], dom, 1), to_fix(cz[j], dom, 2), __float_as_uint((float)len[j]));
+      j          = nparent[j];
     }
 }
 
@@ -610,7 +634,8 @@ agrav_status build_tree(agrav_ctx *c)
      (st = c->d_cy.ensure(M)) || (st = c->d_cz.ensure(M)) || (st = c->d_len.ensure(M)) || (st = c->d_nextnode.ensure(M)) ||
      (st = c->d_sibling.ensure(M)) || (st = c->d_maxsoft.ensure(M)) || (st = c->d_recA.ensure(M)) ||
      (st = c->d_recB.ensure(M)) || (st = c->d_nrange.ensure(M)) || (st = c->d_cbeg.ensure(M)) || (st = c->d_ccnt.ensure(M)) || (st = c->d_cnn.ensure(M)) ||
-     (st = c->d_child.ensure((size_t)M + n)) || (st = c->d_lvlnodes.ensure(M)))
+     (st = c->d_child.ensure((size_t)M + n)) || (st = c->d_nparent.ensure(M)) || (st = c->d_arrived.ensure(M)) ||
+     (st = c->d_pparent.ensure(n)))
     return st;
   if(N > 1)
     k_oct_emit<<<nblk(N - 1), B, 0, s>>>(N, c->d_bfirst.p, c->d_blast.p, c->d_bprefix.p, c->d_bparent.p, c->d_bcount.p,
@@ -627,38 +652,22 @@ agrav_status build_tree(agrav_ctx *c)
   k_pnext<<<nblk(N), B, 0, s>>>(N, c->d_lvlmask.p, c->d_pbase.p, c->d_pnext.p);
   AG_HIP(hipGetLastError());
 
-  // 5. moments, deepest level first
-  AG_HIP(hipMemcpyAsync(c->lvlcnt_h, c->d_lvlcnt.p, (AG_KEY_LEVELS + 1) * sizeof(int), hipMemcpyDeviceToHost, s));
-  AG_HIP(hipStreamSynchronize(s));
-  int off[AG_KEY_LEVELS + 2];
-  off[0] = 0;
-  for(int L = 0; L <= AG_KEY_LEVELS; L++)
-    off[L + 1] = off[L] + c->lvlcnt_h[L];
-  if(off[AG_KEY_LEVELS + 1] != M)
-    return fail(c, AGRAV_ERR_CAPACITY, "inconsistent octree level counts");
-  AG_HIP(hipMemcpyAsync(c->d_lvloff.p, off, (AG_KEY_LEVELS + 2) * sizeof(int), hipMemcpyHostToDevice, s));
-  int *d_fill = c->d_counters.p + 24;  // 22 ints
+  // 5. child lists, parents, then moments bottom-up in one launch
   int *d_berr = c->d_counters.p + 50;
-  AG_HIP(hipMemsetAsync(d_fill, 0, (AG_KEY_LEVELS + 1) * sizeof(int), s));
   AG_HIP(hipMemsetAsync(d_berr, 0, sizeof(int), s));
-  k_level_scatter<<<nblk(M), B, 0, s>>>(M, c->d_nlevel.p, c->d_lvloff.p, d_fill, c->d_lvlnodes.p);
-  AG_HIP(hipGetLastError());
-  for(int L = AG_KEY_LEVELS; L >= 0; L--)
-    {
-      int cnt = c->lvlcnt_h[L];
-      if(cnt == 0)
-        continue;
-      k_moments<<<nblk(cnt), B, 0, s>>>(cnt, c->d_lvlnodes.p + off[L], N, c->d_nextnode.p, c->d_sibling.p, c->d_pnext.p, c->d_px.p,
-                                        c->d_py.p, c->d_pz.p, c->d_pm.p, multisoft ? c->d_ps.p : nullptr, c->d_soft.p,
-                                        c->soft.nsoft, c->d_cx.p, c->d_cy.p, c->d_cz.p, c->d_len.p, c->d_sx.p, c->d_sy.p,
-                                        c->d_sz.p, c->d_nm.p, c->d_maxsoft.p, c->d_ccnt.p, c->d_recA.p, c->d_recB.p, c->d_dom.p,
-                                        N + M, d_berr);
-    }
+  k_count_children<<<nblk(M), B, 0, s>>>(M, N, c->d_nextnode.p, c->d_sibling.p, c->d_pnext.p, c->d_ccnt.p, N + M, d_berr);
   AG_HIP(hipGetLastError());
   if((st = exclusive_scan_i32(c->d_ccnt.p, c->d_cbeg.p, M, nullptr, c->d_temp.p, s)) != AGRAV_OK)
     return st;
   k_fill_children<<<nblk(M), B, 0, s>>>(M, N, c->d_nextnode.p, c->d_sibling.p, c->d_pnext.p, c->d_cbeg.p, c->d_ccnt.p,
                                          c->d_child.p, c->d_cnn.p);
+  k_parents<<<nblk(M), B, 0, s>>>(M, N, c->d_cbeg.p, c->d_ccnt.p, c->d_child.p, c->d_pparent.p, c->d_nparent.p);
+  AG_HIP(hipMemsetAsync(c->d_arrived.p, 0, M * sizeof(int), s));
+  k_moments_up<<<nblk(N), B, 0, s>>>(N, c->d_pparent.p, c->d_nparent.p, c->d_cbeg.p, c->d_ccnt.p, c->d_child.p, c->d_px.p,
+                                      c->d_py.p, c->d_pz.p, c->d_pm.p, multisoft ? c->d_ps.p : nullptr, c->d_soft.p,
+                                      c->soft.nsoft, c->d_cx.p, c->d_cy.p, c->d_cz.p, c->d_len.p, c->d_sx.p, c->d_sy.p,
+                                      c->d_sz.p, c->d_nm.p, c->d_maxsoft.p, c->d_recA.p, c->d_recB.p, c->d_arrived.p,
+                                      c->d_dom.p);
   AG_HIP(hipGetLastError());
 
   // out-of-box check (forcetree.c:432-438 aborts; here it is an error code)

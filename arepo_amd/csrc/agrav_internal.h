@@ -16,7 +16,7 @@
 #include <stdint.h>
 #include <string>
 
-#include "../../include/agrav.h"
+#include "agrav.h"
 
 #define AG_NTAB 127
 #define AG_KEY_LEVELS 21
@@ -91,6 +91,7 @@ struct WalkConst
   int multisoft;
   float fix_unit;     // fixed-point step as float
   int leaf_bucket;    // opened nodes with at most this many particles are expanded into particles
+  int eval_pot;       // EVALPOTENTIAL: also sum the potential
 };
 
 // hip error helper

@@ -384,7 +384,7 @@ __device__ inline void tile_compute_generic(const Vec4<T> *__restrict__ s_list, 
 // 1 ulp), the short-range factors from the LDS tables without branches, and the
 // softened kernel only for the rare entries inside the softening length.  The
 // arithmetic is the reference's pair kernel (forcetree_walk.c:364-487).
-template <bool PM, bool EWALD, bool MSOFT>
+template <bool PM, bool EWALD, bool MSOFT, bool POT>
 __device__ inline void tile_compute_f32(const Vec4<float> *__restrict__ s_list, const float *__restrict__ s_h, int nl, float tx,
                                         float ty, float tz, float hi, const float *__restrict__ s_tab, const WalkConst &wc,
                                         float asmthfac, float asmthinv2, const float *__restrict__ ewald_f, float &ax, float &ay,
@@ -415,15 +415,16 @@ __device__ inline void tile_compute_f32(const Vec4<float> *__restrict__ s_list, 
           const int tc   = ti < AG_NTAB ? ti : AG_NTAB - 1;
           const float w  = te - (float)tc;
           const float f0 = s_tab[tc], f1 = s_tab[tc + 1];
-          const float p0 = s_tab[128 + tc], p1 = s_tab[128 + tc + 1];
-          ff             = f0 + w * (f1 - f0);
-          fpt            = p0 + w * (p1 - p0);
+          const float p0 = POT ? s_tab[128 + tc] : 0.f, p1 = POT ? s_tab[128 + tc + 1] : 0.f;
+          ff             = f0 + w * (f1 - f0);  // force table is pre-scaled by asmthinv2
+          if(POT)
+            fpt = p0 + w * (p1 - p0);
         }
       const float rinv2 = rinv * rinv;
       float fac, wp;
       if(PM)
         {
-          fac = rinv2 * rinv + rinv * ff * asmthinv2;
+          fac = rinv * (rinv2 + ff);
           wp  = rinv * fpt;
         }
       else
@@ -451,12 +452,13 @@ __device__ inline void tile_compute_f32(const Vec4<float> *__restrict__ s_list, 
             }
           if(PM && r2 > 0.f)
             {
-              fac += rinv * ff * asmthinv2;
+              fac += rinv * ff;
               wp += rinv * (fpt + 1.0f);
             }
         }
       const float m = valid ? e.w : 0.f;
-      fp += m * wp;
+      if(POT)
+        fp += m * wp;
       const float mf = m * fac;
       fx += dx * mf;
       fy += dy * mf;
@@ -478,8 +480,14 @@ __device__ inline void tile_compute(const Vec4<T> *__restrict__ s_list, const T 
                                     T &cax, T &cay, T &caz, T &cpp, int &nin)
 {
   if constexpr(sizeof(T) == 4)
-    tile_compute_f32<PM, EWALD, MSOFT>(s_list, s_h, nl, tx, ty, tz, hi, s_tab, wc, asmthfac, asmthinv2, ewald_f, ax, ay, az, pp,
-                                       cax, cay, caz, cpp, nin);
+    {
+      if(wc.eval_pot)
+        tile_compute_f32<PM, EWALD, MSOFT, true>(s_list, s_h, nl, tx, ty, tz, hi, s_tab, wc, asmthfac, asmthinv2, ewald_f, ax, ay,
+                                                 az, pp, cax, cay, caz, cpp, nin);
+      else
+        tile_compute_f32<PM, EWALD, MSOFT, false>(s_list, s_h, nl, tx, ty, tz, hi, s_tab, wc, asmthfac, asmthinv2, ewald_f, ax,
+                                                  ay, az, pp, cax, cay, caz, cpp, nin);
+    }
   else
     tile_compute_generic<T, PM, EWALD, MSOFT>(s_list, s_h, nl, tx, ty, tz, hi, s_tab, wc, asmthfac, asmthinv2, ewald_f, ewald_d,
                                               ax, ay, az, pp, cax, cay, caz, cpp, nin);
@@ -499,8 +507,10 @@ __global__ __launch_bounds__(64) void k_walk_group(TreeView tv, WalkConst wc, co
 
   const int lane   = threadIdx.x;
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  // the FP32 tile loop uses the force table pre-multiplied by (0.5/Asmth)^2
+  const float tscale = (sizeof(T) == 4 && PM) ? (float)wc.asmthinv2 : 1.0f;
   for(int q = lane; q < 256; q += 64)
-    s_tab[q] = tabs[q];
+    s_tab[q] = q < 128 ? tabs[q] * tscale : tabs[q];
 
   const int N  = tv.n;
   const int g  = blockIdx.x;
@@ -561,6 +571,7 @@ __global__ __launch_bounds__(64) void k_walk_group(TreeView tv, WalkConst wc, co
   int nl = 0;
   T ax = 0, ay = 0, az = 0, pp = 0, cax = 0, cay = 0, caz = 0, cpp = 0;
   int nin        = 0;
+  unsigned long long n_popped = 0, n_tiled = 0;  // diagnostics: stack entries evaluated, tile entries computed
   bool overflow  = false;
   int pushed     = 1;
   const int max_push = N + tv.m;  // each entry is pushed at most once per group
@@ -570,6 +581,7 @@ __global__ __launch_bounds__(64) void k_walk_group(TreeView tv, WalkConst wc, co
     {
       const int take = sp < 64 ? sp : 64;
       const int base = sp - take;
+      n_popped += take;
       const int e    = lane < take ? s_stack[base + lane] : -1;
       sp             = base;
       __builtin_amdgcn_wave_barrier();
@@ -691,6 +703,7 @@ __global__ __launch_bounds__(64) void k_walk_group(TreeView tv, WalkConst wc, co
       if(nl + cnt > kListCap)
         {
           __builtin_amdgcn_wave_barrier();
+          n_tiled += nl;
           tile_compute<T, PM, EWALD, MSOFT>(s_list, s_h, nl, tx, ty, tz, hi, s_tab, wc, asmthfac, asmthinv2, ewald_f, ewald_d,
                                             ax, ay, az, pp, cax, cay, caz, cpp, nin);
           nl = 0;
@@ -760,7 +773,8 @@ __global__ __launch_bounds__(64) void k_walk_group(TreeView tv, WalkConst wc, co
             if(nl + rcnt > kListCap)
               {
                 __builtin_amdgcn_wave_barrier();
-                tile_compute<T, PM, EWALD, MSOFT>(s_list, s_h, nl, tx, ty, tz, hi, s_tab, wc, asmthfac, asmthinv2, ewald_f,
+                n_tiled += nl;
+          tile_compute<T, PM, EWALD, MSOFT>(s_list, s_h, nl, tx, ty, tz, hi, s_tab, wc, asmthfac, asmthinv2, ewald_f,
                                                   ewald_d, ax, ay, az, pp, cax, cay, caz, cpp, nin);
                 nl = 0;
                 __builtin_amdgcn_wave_barrier();
@@ -807,7 +821,8 @@ __global__ __launch_bounds__(64) void k_walk_group(TreeView tv, WalkConst wc, co
       return;
     }
   __builtin_amdgcn_wave_barrier();
-  tile_compute<T, PM, EWALD, MSOFT>(s_list, s_h, nl, tx, ty, tz, hi, s_tab, wc, asmthfac, asmthinv2, ewald_f, ewald_d, ax, ay,
+  n_tiled += nl;
+          tile_compute<T, PM, EWALD, MSOFT>(s_list, s_h, nl, tx, ty, tz, hi, s_tab, wc, asmthfac, asmthinv2, ewald_f, ewald_d, ax, ay,
                                     az, pp, cax, cay, caz, cpp, nin);
 
   if(has)
@@ -821,8 +836,13 @@ __global__ __launch_bounds__(64) void k_walk_group(TreeView tv, WalkConst wc, co
   unsigned long long v = has ? (unsigned long long)nin : 0ull;
   for(int off = 32; off > 0; off >>= 1)
     v += __shfl_xor(v, off, 64);
-  if(lane == 0 && v)
-    atomicAdd(isum, v);
+  if(lane == 0)
+    {
+      if(v)
+        atomicAdd(isum, v);
+      atomicAdd(isum + 1, n_popped);
+      atomicAdd(isum + 2, n_tiled);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -854,6 +874,7 @@ WalkConst walk_const(const agrav_ctx *c, int use_bh)
   wc.multisoft = c->soft.nsoft > 1 || c->have_stype;
   wc.fix_unit  = (float)c->h_dom.fix_unit;
   wc.leaf_bucket = c->par.leaf_bucket > 0 ? c->par.leaf_bucket : 0;
+  wc.eval_pot    = c->par.eval_potential ? 1 : 0;
   return wc;
 }
 
@@ -917,9 +938,9 @@ agrav_status walk_tree(agrav_ctx *c, int use_bh)
   hipStream_t s   = c->stream;
   agrav_status st;
   if((st = c->d_at.ensure(3 * (n ? n : 1))) || (st = c->d_pt.ensure(n ? n : 1)) || (st = c->d_nint.ensure(n ? n : 1)) ||
-     (st = c->d_isum.ensure(1)))
+     (st = c->d_isum.ensure(4)))
     return st;
-  AG_HIP(hipMemsetAsync(c->d_isum.p, 0, sizeof(unsigned long long), s));
+  AG_HIP(hipMemsetAsync(c->d_isum.p, 0, 4 * sizeof(unsigned long long), s));
   if(n == 0)
     {
       c->have_tree_force = true;
@@ -974,9 +995,12 @@ agrav_status walk_tree(agrav_ctx *c, int use_bh)
         }
     }
   AG_HIP(hipEventRecord(c->ev[4], s));
-  unsigned long long tot = 0;
-  AG_HIP(hipMemcpyAsync(&tot, c->d_isum.p, sizeof(tot), hipMemcpyDeviceToHost, s));
+  unsigned long long cnt[3] = {0, 0, 0};
+  AG_HIP(hipMemcpyAsync(cnt, c->d_isum.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
   AG_HIP(hipStreamSynchronize(s));
+  const unsigned long long tot = cnt[0];
+  c->tim.walk_popped           = (long long)cnt[1];
+  c->tim.tile_entries          = (long long)cnt[2];
   float ms_walk = 0, ms_kernel = 0;
   AG_HIP(hipEventElapsedTime(&ms_kernel, c->ev[2], c->ev[3]));
   AG_HIP(hipEventElapsedTime(&ms_walk, c->ev[2], c->ev[4]));

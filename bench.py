@@ -42,15 +42,15 @@ def rel_err(a, b):
     return np.linalg.norm(a - b, axis=1) / np.maximum(np.linalg.norm(b, axis=1), 1e-300)
 
 
-def make_params(A, precision, walk, eps, pmgrid=128):
+def make_params(A, precision, walk, eps, pmgrid=128, pot=False):
     return A.make_params(periodic=True, box_size=1.0, pmgrid=pmgrid, asmth=1.25, rcut=4.5, G=1.0,
                          err_tol_theta=0.7, err_tol_force_acc=0.0025, opening=1, soft=(eps,),
-                         precision=precision, walk=walk, group_size=64)
+                         precision=precision, walk=walk, group_size=64, leaf_bucket=4, eval_potential=pot)
 
 
-def run_path(A, pos, mass, eps, precision, walk, steps, warmup, old_acc=None):
+def run_path(A, pos, mass, eps, precision, walk, steps, warmup, old_acc=None, pot=False):
     """Warm up, then time `steps` full solves; returns timing dict and the last result."""
-    g = A.Gravity(make_params(A, precision, walk, eps))
+    g = A.Gravity(make_params(A, precision, walk, eps, pot=pot))
     g.set_particles(pos, mass)
     if old_acc is not None:
         g.set_old_acc(old_acc)
@@ -167,7 +167,8 @@ def main():
         "data": "synthetic (seeded clustered particle set generated in-process)",
         "config": {"workload": "treepm_128cubed_clustered", "n_particles": n, "pmgrid": 128, "asmth": 1.25,
                    "rcut": 4.5, "err_tol_force_acc": 0.0025, "opening": "relative", "softening": eps,
-                   "precision": "fp32", "walk": "group64", "seed": args.seed},
+                   "precision": "fp32", "walk": "group64", "leaf_bucket": 4,
+                   "evalpotential": False, "seed": args.seed},
         "interactions_per_s": inter * args.steps / r32["total_s"],
         "interactions_per_particle": inter / n,
         "latency_ms": {"p50": float(np.percentile(r32["step_ms"], 50)),
@@ -186,6 +187,11 @@ def main():
     out["roofline"] = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                        "frac": achieved / HBM_PEAK, "traffic": None, "kernel": "k_walk_group<float,PM>",
                        "kernel_ms": tim["walk_kernel"], "alg_bytes_per_launch": alg_bytes}
+
+    # the same path with EVALPOTENTIAL (potentials summed too)
+    rpot, _ = run_path(A, pos, mass, eps, A.FP32, A.WALK_GROUP, 3, 0, old_acc=res32["old_acc"], pot=True)
+    out["fp32_group_with_potential"] = {"ms_per_step": 1e3 * rpot["total_s"] / 3,
+                                        "walk_ms": rpot["timings"]["treewalk"]}
 
     res64p = None
     if not args.no_fp64:

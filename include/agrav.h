@@ -74,6 +74,8 @@ typedef struct
   double tree_alloc_factor;       /* TreeAllocFactor (init.c:110); unused: node arrays are sized exactly */
   int leaf_bucket;                /* group walk: an opened node with at most this many particles is expanded
                                      straight into particles (0..64, default 8) */
+  int eval_potential;             /* EVALPOTENTIAL (Template-Config.sh): 1 = potentials too (default), 0 = forces
+                                     only; the FP32 group walk then skips the potential arithmetic */
 } agrav_params;
 
 /* Per-stage device times (ms) of the last call, named like AREPO's cpu.txt
@@ -88,6 +90,8 @@ typedef struct
   long long interactions; /* sum of per-target interaction counts of the last walk */
   int numnodes;       /* octree nodes of the last build */
   int walk_retries;   /* groups re-run by the fallback walk (stack overflow) */
+  long long walk_popped;  /* group walk: stack entries evaluated (traversal work) */
+  long long tile_entries; /* group walk: interaction-tile entries streamed by all 64 lanes */
 } agrav_timings;
 
 typedef struct agrav_ctx agrav_ctx;
