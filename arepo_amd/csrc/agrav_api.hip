@@ -342,9 +342,6 @@ void agrav_destroy(agrav_ctx *c)
   c->d_lvlcnt.release();
   c->d_lvloff.release();
   c->d_lvlnodes.release();
-  c->d_pparent.release();
-  c->d_nparent.release();
-  c->d_arrived.release();
   c->d_at.release();
   c->d_pt.release();
   c->d_nint.release();

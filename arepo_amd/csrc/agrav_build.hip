@@ -387,6 +387,25 @@ __global__ void k_pnext(int n, const unsigned *__restrict__ lvlmask, const int *
     pnext[p] = pnext_of(p, n, lvlmask, pbase);
 }
 
+__global__ __launch_bounds__(B) void k_level_scatter(int m, const int *__restrict__ nlevel, const int *__restrict__ lvloff,
+                                                     int *lvlfill, int *lvlnodes)
+{
+  // block-local ranks per level in LDS, one global reservation per block and level
+  __shared__ int s_cnt[AG_KEY_LEVELS + 1], s_base[AG_KEY_LEVELS + 1];
+  if(threadIdx.x <= AG_KEY_LEVELS)
+    s_cnt[threadIdx.x] = 0;
+  __syncthreads();
+  int j    = blockIdx.x * B + threadIdx.x;
+  int L    = (j < m) ? nlevel[j] : -1;
+  int rank = (L >= 0) ? atomicAdd(&s_cnt[L], 1) : 0;
+  __syncthreads();
+  if(threadIdx.x <= AG_KEY_LEVELS)
+    s_base[threadIdx.x] = s_cnt[threadIdx.x] ? atomicAdd(&lvlfill[threadIdx.x], s_cnt[threadIdx.x]) : 0;
+  __syncthreads();
+  if(L >= 0)
+    lvlnodes[lvloff[L] + s_base[L] + rank] = j;
+}
+
 // ---------------------------------------------------------------------------
 // 5. child lists and moments (forcetree.c:1217-1429)
 // ---------------------------------------------------------------------------
@@ -438,100 +457,112 @@ __global__ void k_count_children(int m, int n, const int *__restrict__ nextnode,
   ccnt[j] = k;
 }
 
-// parent of every particle and node (-1 for the root)
-__global__ void k_parents(int m, int n, const int *__restrict__ cbeg, const int *__restrict__ ccnt, const int *__restrict__ child,
-                          int *pparent, int *nparent)
+// One node's moments from its CSR children (forcetree.c:1217-1429): mass, FP64
+// centre of mass (geometric centre when empty) and maxsofttype, plus the two FP32
+// walk records.  Children are summed in their fixed list order.
+template <bool VOLATILE>
+__device__ inline void node_moments(int j, int n, const int *__restrict__ cbeg, const int *__restrict__ ccnt, const int *__restrict__ child,
+                                    const double *__restrict__ px, const double *__restrict__ py, const double *__restrict__ pz,
+                                    const double *__restrict__ pm, const uint8_t *__restrict__ ps, const double *__restrict__ hsoft,
+                                    int nsoft, const double *__restrict__ cx, const double *__restrict__ cy,
+                                    const double *__restrict__ cz, const double *__restrict__ len, double *sx, double *sy,
+                                    double *sz, double *nm, uint8_t *maxsoft, uint4 *recA, uint4 *recB, const Domain &dom)
 {
-  int j = blockIdx.x * B + threadIdx.x;
-  if(j >= m)
-    return;
-  if(j == 0)
-    nparent[0] = -1;
+  double m = 0, s0 = 0, s1 = 0, s2 = 0;
+  int ms = nsoft;  // sentinel type with zero softening (forcetree.c:1254)
   for(int q = cbeg[j], e = cbeg[j] + ccnt[j]; q < e; q++)
     {
-      int c = child[q];
+      const int c = child[q];
+      double mm, x, y, z;
+      int st;
       if(c < n)
-        pparent[c] = j;
-      else
-        nparent[c - n] = j;
-    }
-}
-
-// Moments bottom-up in one launch (forcetree.c:1217-1429): a lane per particle
-// climbs towards the root; at every node the last child to arrive (counted with an
-// atomic) sums all children of that node in their fixed CSR order, so the result
-// does not depend on arrival order, and then continues upwards.
-__global__ void k_moments_up(int n, const int *__restrict__ pparent, const int *__restrict__ nparent, const int *__restrict__ cbeg,
-                             const int *__restrict__ ccnt, const int *__restrict__ child, const double *__restrict__ px,
-                             const double *__restrict__ py, const double *__restrict__ pz, const double *__restrict__ pm,
-                             const uint8_t *__restrict__ ps, const double *__restrict__ hsoft, int nsoft,
-                             const double *__restrict__ cx, const double *__restrict__ cy, const double *__restrict__ cz,
-                             const double *__restrict__ len, double *sx, double *sy, double *sz, double *nm, uint8_t *maxsoft,
-                             uint4 *recA, uint4 *recB, int *arrived, const Domain *__restrict__ domp)
-{
-  int p = blockIdx.x * B + threadIdx.x;
-  if(p >= n)
-    return;
-  const Domain dom = *domp;
-  int j            = pparent[p];
-  while(j >= 0)
-    {
-      __threadfence();
-      const int old = atomicAdd(&arrived[j], 1);
-      if(old != ccnt[j] - 1)
-        return;  // another child of j finishes later and carries on
-      __threadfence();
-      double m = 0, s0 = 0, s1 = 0, s2 = 0;
-      int ms = nsoft;  // sentinel type with zero softening (forcetree.c:1254)
-      for(int q = cbeg[j], e = cbeg[j] + ccnt[j]; q < e; q++)
         {
-          const int c = child[q];
-          double mm, x, y, z;
-          int st;
-          if(c < n)
+          mm = pm[c];
+          x  = px[c];
+          y  = py[c];
+          z  = pz[c];
+          st = ps ? ps[c] : 0;
+        }
+      else
+        {
+          const int k = c - n;
+          if(VOLATILE)
             {
-              mm = pm[c];
-              x  = px[c];
-              y  = py[c];
-              z  = pz[c];
-              st = ps ? ps[c] : 0;
+              // written earlier in this launch by another lane: agent-coherent loads
+              mm = __hip_atomic_load(&nm[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              x  = __hip_atomic_load(&sx[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              y  = __hip_atomic_load(&sy[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              z  = __hip_atomic_load(&sz[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              st = __hip_atomic_load(&maxsoft[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
           else
             {
-              const int k = c - n;
-              mm          = __hip_atomic_load(&nm[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-              x           = __hip_atomic_load(&sx[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-              y           = __hip_atomic_load(&sy[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-              z           = __hip_atomic_load(&sz[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-              st          = __hip_atomic_load(&maxsoft[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              mm = nm[k];
+              x  = sx[k];
+              y  = sy[k];
+              z  = sz[k];
+              st = maxsoft[k];
             }
-          m += mm;
-          s0 += mm * x;
-          s1 += mm * y;
-          s2 += mm * z;
-          if(hsoft[ms] < hsoft[st])
-            ms = st;
         }
-      if(m != 0)
-        {
-          s0 /= m;
-          s1 /= m;
-          s2 /= m;
-        }
-      else
-        {
-          s0 = cx[j];
-          s1 = cy[j];
-          s2 = cz[j];
-        }
-      sx[j]      = s0;
-      sy[j]      = s1;
-      sz[j]      = s2;
-      nm[j]      = m;
-      maxsoft[j] = (uint8_t)ms;
-      recA[j]    = make_uint4(to_fix(s0, dom, 0), to_fix(s1, dom, 1), to_fix(s2, dom, 2), __float_as_uint((float)m));
-      recB[j]    = make_uint4(to_fix(cx[j], dom, 0), to_fix(cy[j], dom, 1), to_fix(cz[j], dom, 2), __float_as_uint((float)len[j]));
-      j          = nparent[j];
+      m += mm;
+      s0 += mm * x;
+      s1 += mm * y;
+      s2 += mm * z;
+      if(hsoft[ms] < hsoft[st])
+        ms = st;
+    }
+  if(m != 0)
+    {
+      s0 /= m;
+      s1 /= m;
+      s2 /= m;
+    }
+  else
+    {
+      s0 = cx[j];
+      s1 = cy[j];
+      s2 = cz[j];
+    }
+  sx[j]      = s0;
+  sy[j]      = s1;
+  sz[j]      = s2;
+  nm[j]      = m;
+  maxsoft[j] = (uint8_t)ms;
+  recA[j]    = make_uint4(to_fix(s0, dom, 0), to_fix(s1, dom, 1), to_fix(s2, dom, 2), __float_as_uint((float)m));
+  recB[j]    = make_uint4(to_fix(cx[j], dom, 0), to_fix(cy[j], dom, 1), to_fix(cz[j], dom, 2), __float_as_uint((float)len[j]));
+}
+
+#define MOMENT_ARGS                                                                                                            \
+  n, cbeg, ccnt, child, px, py, pz, pm, ps, hsoft, nsoft, cx, cy, cz, len, sx, sy, sz, nm, maxsoft, recA, recB, dom
+#define MOMENT_PARAMS                                                                                                          \
+  int n, const int *__restrict__ cbeg, const int *__restrict__ ccnt, const int *__restrict__ child, const double *__restrict__ px, \
+      const double *__restrict__ py, const double *__restrict__ pz, const double *__restrict__ pm, const uint8_t *__restrict__ ps,  \
+      const double *__restrict__ hsoft, int nsoft, const double *__restrict__ cx, const double *__restrict__ cy,                   \
+      const double *__restrict__ cz, const double *__restrict__ len, double *sx, double *sy, double *sz, double *nm,               \
+      uint8_t *maxsoft, uint4 *recA, uint4 *recB, const Domain *__restrict__ domp
+
+// one level with many nodes: a lane per node
+__global__ __launch_bounds__(B) void k_moments_level(const int *__restrict__ list, int cnt, MOMENT_PARAMS)
+{
+  int t = blockIdx.x * B + threadIdx.x;
+  if(t >= cnt)
+    return;
+  const Domain dom = *domp;
+  node_moments<false>(list[t], MOMENT_ARGS);
+}
+
+// a run of consecutive small levels (deepest first) in one workgroup: levels are
+// separated by workgroup barriers, child data are read past L1
+__global__ __launch_bounds__(1024) void k_moments_levels_small(const int *__restrict__ list, const int *__restrict__ lvloff, int lhi,
+                                                                int llo, MOMENT_PARAMS)
+{
+  const Domain dom = *domp;
+  for(int L = lhi; L >= llo; L--)
+    {
+      for(int t = lvloff[L] + threadIdx.x; t < lvloff[L + 1]; t += blockDim.x)
+        node_moments<true>(list[t], MOMENT_ARGS);
+      __threadfence_block();
+      __syncthreads();
     }
 }
 
@@ -634,8 +665,7 @@ agrav_status build_tree(agrav_ctx *c)
      (st = c->d_cy.ensure(M)) || (st = c->d_cz.ensure(M)) || (st = c->d_len.ensure(M)) || (st = c->d_nextnode.ensure(M)) ||
      (st = c->d_sibling.ensure(M)) || (st = c->d_maxsoft.ensure(M)) || (st = c->d_recA.ensure(M)) ||
      (st = c->d_recB.ensure(M)) || (st = c->d_nrange.ensure(M)) || (st = c->d_cbeg.ensure(M)) || (st = c->d_ccnt.ensure(M)) || (st = c->d_cnn.ensure(M)) ||
-     (st = c->d_child.ensure((size_t)M + n)) || (st = c->d_nparent.ensure(M)) || (st = c->d_arrived.ensure(M)) ||
-     (st = c->d_pparent.ensure(n)))
+     (st = c->d_child.ensure((size_t)M + n)) || (st = c->d_lvlnodes.ensure(M)))
     return st;
   if(N > 1)
     k_oct_emit<<<nblk(N - 1), B, 0, s>>>(N, c->d_bfirst.p, c->d_blast.p, c->d_bprefix.p, c->d_bparent.p, c->d_bcount.p,
@@ -652,7 +682,8 @@ agrav_status build_tree(agrav_ctx *c)
   k_pnext<<<nblk(N), B, 0, s>>>(N, c->d_lvlmask.p, c->d_pbase.p, c->d_pnext.p);
   AG_HIP(hipGetLastError());
 
-  // 5. child lists, parents, then moments bottom-up in one launch
+  // 5. child lists, then moments level by level (deepest first); runs of small
+  //    levels share one single-workgroup launch
   int *d_berr = c->d_counters.p + 50;
   AG_HIP(hipMemsetAsync(d_berr, 0, sizeof(int), s));
   k_count_children<<<nblk(M), B, 0, s>>>(M, N, c->d_nextnode.p, c->d_sibling.p, c->d_pnext.p, c->d_ccnt.p, N + M, d_berr);
@@ -661,13 +692,45 @@ agrav_status build_tree(agrav_ctx *c)
     return st;
   k_fill_children<<<nblk(M), B, 0, s>>>(M, N, c->d_nextnode.p, c->d_sibling.p, c->d_pnext.p, c->d_cbeg.p, c->d_ccnt.p,
                                          c->d_child.p, c->d_cnn.p);
-  k_parents<<<nblk(M), B, 0, s>>>(M, N, c->d_cbeg.p, c->d_ccnt.p, c->d_child.p, c->d_pparent.p, c->d_nparent.p);
-  AG_HIP(hipMemsetAsync(c->d_arrived.p, 0, M * sizeof(int), s));
-  k_moments_up<<<nblk(N), B, 0, s>>>(N, c->d_pparent.p, c->d_nparent.p, c->d_cbeg.p, c->d_ccnt.p, c->d_child.p, c->d_px.p,
-                                      c->d_py.p, c->d_pz.p, c->d_pm.p, multisoft ? c->d_ps.p : nullptr, c->d_soft.p,
-                                      c->soft.nsoft, c->d_cx.p, c->d_cy.p, c->d_cz.p, c->d_len.p, c->d_sx.p, c->d_sy.p,
-                                      c->d_sz.p, c->d_nm.p, c->d_maxsoft.p, c->d_recA.p, c->d_recB.p, c->d_arrived.p,
-                                      c->d_dom.p);
+  AG_HIP(hipMemcpyAsync(c->lvlcnt_h, c->d_lvlcnt.p, (AG_KEY_LEVELS + 1) * sizeof(int), hipMemcpyDeviceToHost, s));
+  AG_HIP(hipStreamSynchronize(s));
+  int off[AG_KEY_LEVELS + 2];
+  off[0] = 0;
+  for(int L = 0; L <= AG_KEY_LEVELS; L++)
+    off[L + 1] = off[L] + c->lvlcnt_h[L];
+  if(off[AG_KEY_LEVELS + 1] != M)
+    return fail(c, AGRAV_ERR_CAPACITY, "inconsistent octree level counts");
+  AG_HIP(hipMemcpyAsync(c->d_lvloff.p, off, (AG_KEY_LEVELS + 2) * sizeof(int), hipMemcpyHostToDevice, s));
+  int *d_fill = c->d_counters.p + 24;  // 22 ints
+  AG_HIP(hipMemsetAsync(d_fill, 0, (AG_KEY_LEVELS + 1) * sizeof(int), s));
+  k_level_scatter<<<nblk(M), B, 0, s>>>(M, c->d_nlevel.p, c->d_lvloff.p, d_fill, c->d_lvlnodes.p);
+  AG_HIP(hipGetLastError());
+  const uint8_t *d_ps = multisoft ? c->d_ps.p : nullptr;
+  const int small = 4096;
+  for(int L = AG_KEY_LEVELS; L >= 0;)
+    {
+      if(c->lvlcnt_h[L] >= small)
+        {
+          k_moments_level<<<nblk(c->lvlcnt_h[L]), B, 0, s>>>(c->d_lvlnodes.p + off[L], c->lvlcnt_h[L], N, c->d_cbeg.p, c->d_ccnt.p,
+                                                              c->d_child.p, c->d_px.p, c->d_py.p, c->d_pz.p, c->d_pm.p, d_ps,
+                                                              c->d_soft.p, c->soft.nsoft, c->d_cx.p, c->d_cy.p, c->d_cz.p,
+                                                              c->d_len.p, c->d_sx.p, c->d_sy.p, c->d_sz.p, c->d_nm.p,
+                                                              c->d_maxsoft.p, c->d_recA.p, c->d_recB.p, c->d_dom.p);
+          L--;
+        }
+      else
+        {
+          int lo = L;
+          while(lo - 1 >= 0 && c->lvlcnt_h[lo - 1] < small)
+            lo--;
+          k_moments_levels_small<<<1, 1024, 0, s>>>(c->d_lvlnodes.p, c->d_lvloff.p, L, lo, N, c->d_cbeg.p, c->d_ccnt.p,
+                                                    c->d_child.p, c->d_px.p, c->d_py.p, c->d_pz.p, c->d_pm.p, d_ps, c->d_soft.p,
+                                                    c->soft.nsoft, c->d_cx.p, c->d_cy.p, c->d_cz.p, c->d_len.p, c->d_sx.p,
+                                                    c->d_sy.p, c->d_sz.p, c->d_nm.p, c->d_maxsoft.p, c->d_recA.p, c->d_recB.p,
+                                                    c->d_dom.p);
+          L = lo - 1;
+        }
+    }
   AG_HIP(hipGetLastError());
 
   // out-of-box check (forcetree.c:432-438 aborts; here it is an error code)

@@ -112,7 +112,6 @@ struct agrav_ctx
   agrav::DBuf<int2> d_nrange;
   agrav::DBuf<int> d_cbeg, d_ccnt, d_cnn, d_child;
   agrav::DBuf<int> d_lvlcnt, d_lvloff, d_lvlnodes;
-  agrav::DBuf<int> d_pparent, d_nparent, d_arrived;  // parents and arrival counters of the bottom-up moment pass
   int lvlcnt_h[AG_KEY_LEVELS + 2];
 
   // walk outputs in sorted order (G=1 units) and per-target counts
