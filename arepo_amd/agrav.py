@@ -91,7 +91,32 @@ EXPORTED = [
     "agrav_compute_grav_accelerations", "agrav_get_accelerations", "agrav_get_accelerations_device",
     "agrav_get_components", "agrav_gravity_forcetest", "agrav_gravity_direct", "agrav_get_timings", "agrav_get_tree",
     "agrav_get_order", "agrav_get_tables", "agrav_hilbert_key21",
+    "agrav_server_create", "agrav_server_submit", "agrav_server_wait", "agrav_server_poll", "agrav_server_stats",
+    "agrav_server_destroy",
 ]
+
+
+class Request(C.Structure):
+    _fields_ = [
+        ("n", C.c_int64),
+        ("pos", C.c_void_p),
+        ("mass", C.c_void_p),
+        ("softtype", C.c_void_p),
+        ("softening", C.c_double),
+        ("acc", C.c_void_p),
+        ("pot", C.c_void_p),
+    ]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [("status", C.c_int), ("queue_ms", C.c_double), ("run_ms", C.c_double), ("total_ms", C.c_double),
+                ("slot", C.c_int)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("completed", C.c_longlong), ("failed", C.c_longlong), ("queued", C.c_int), ("running", C.c_int),
+                ("peak_running", C.c_int), ("slots", C.c_int), ("hbm_in_use", C.c_size_t), ("hbm_peak", C.c_size_t),
+                ("hbm_budget", C.c_size_t)]
 
 
 def _declare(L):
@@ -121,13 +146,20 @@ def _declare(L):
     L.agrav_get_tree.argtypes = [vp, C.POINTER(C.c_int), vp, vp, vp, vp, vp, vp]
     L.agrav_get_order.argtypes = [vp, vp, vp]
     L.agrav_get_tables.argtypes = [vp, vp, vp, vp, vp]
+    L.agrav_server_create.argtypes = [C.POINTER(Params), C.c_int, C.c_size_t, C.POINTER(vp)]
+    L.agrav_server_submit.argtypes = [vp, C.POINTER(Request), C.POINTER(C.c_uint64)]
+    L.agrav_server_wait.argtypes = [vp, C.c_uint64, C.POINTER(RequestResult)]
+    L.agrav_server_poll.argtypes = [vp, C.c_uint64, C.POINTER(C.c_int)]
+    L.agrav_server_stats.argtypes = [vp, C.POINTER(ServerStats)]
+    L.agrav_server_destroy.argtypes = [vp]
     L.agrav_hilbert_key21.restype = C.c_uint64
     L.agrav_hilbert_key21.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
     for name in EXPORTED:
         if name not in ("agrav_status_string", "agrav_last_error", "agrav_hilbert_key21", "agrav_destroy",
-                        "agrav_default_params"):
+                        "agrav_default_params", "agrav_server_destroy"):
             getattr(L, name).restype = C.c_int
     L.agrav_destroy.restype = None
+    L.agrav_server_destroy.restype = None
     L.agrav_default_params.restype = None
 
 
@@ -302,3 +334,69 @@ class Gravity:
 
 def hilbert_key21(x, y, z):
     return int(lib().agrav_hilbert_key21(int(x), int(y), int(z)))
+
+
+class Server:
+    """Continuous-batching server (agrav_server): `slots` resident solver contexts with their own streams."""
+
+    def __init__(self, params: Params, slots=4, hbm_budget_bytes=0):
+        self.L = lib()
+        h = C.c_void_p()
+        rc = self.L.agrav_server_create(C.byref(params), slots, hbm_budget_bytes, C.byref(h))
+        if rc != OK:
+            raise AgravError(rc, (self.L.agrav_last_error(None) or b"").decode())
+        self.h = h
+        self._keep = {}
+
+    def submit(self, pos, mass, softening=0.0, softtype=None, potential=True):
+        """Queue one solve; returns a ticket.  Arrays are kept alive until wait()."""
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        mass = np.ascontiguousarray(mass, dtype=np.float64)
+        st = None if softtype is None else np.ascontiguousarray(softtype, dtype=np.uint8)
+        n = len(pos)
+        acc = np.zeros((n, 3))
+        pot = np.zeros(n) if potential else None
+        r = Request(n, _ptr(pos), _ptr(mass), _ptr(st), float(softening), _ptr(acc), _ptr(pot))
+        t = C.c_uint64()
+        rc = self.L.agrav_server_submit(self.h, C.byref(r), C.byref(t))
+        if rc != OK:
+            raise AgravError(rc, "submit failed")
+        self._keep[t.value] = (pos, mass, st, acc, pot)
+        return t.value
+
+    def wait(self, ticket):
+        if ticket not in self._keep:
+            raise AgravError(ERR_BAD_PARAM, "unknown ticket %r" % (ticket,))
+        res = RequestResult()
+        rc = self.L.agrav_server_wait(self.h, ticket, C.byref(res))
+        pos, mass, st, acc, pot = self._keep.pop(ticket)
+        if rc != OK:
+            raise AgravError(rc, "request failed")
+        return dict(acc=acc, pot=pot, queue_ms=res.queue_ms, run_ms=res.run_ms, total_ms=res.total_ms, slot=res.slot)
+
+    def poll(self, ticket):
+        d = C.c_int()
+        self.L.agrav_server_poll(self.h, ticket, C.byref(d))
+        return bool(d.value)
+
+    def stats(self):
+        s = ServerStats()
+        self.L.agrav_server_stats(self.h, C.byref(s))
+        return {k: getattr(s, k) for k, _ in s._fields_}
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.L.agrav_server_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -71,6 +71,68 @@ def run_path(A, pos, mass, eps, precision, walk, steps, warmup, old_acc=None, po
     return dict(total_s=total, step_ms=step_ms, timings=tim), res
 
 
+def serving_run(A, precision, walk, slots, rate, seconds, seed, max_requests=None):
+    """Poisson stream of isolated Hernquist solves (N log-normal, median 5e4, sigma 1.2 in ln, clipped to
+    [256, 4M]) submitted in real time to a continuous-batching server; returns latency percentiles and
+    throughput.  Particle sets are random subsets of one pre-drawn 4M-particle Hernquist sample (a subset
+    of independent draws is itself a Hernquist sample)."""
+    import threading
+    from arepo_amd import ics
+    g = ics.rng(seed)
+    pool_n = 1 << 22
+    pool = ics._isotropic(ics.hernquist_radii(pool_n, 1.0, g), g)
+    t_arr = np.cumsum(g.exponential(1.0 / rate, int(rate * seconds * 1.5) + 10))
+    t_arr = t_arr[t_arr < seconds]
+    if max_requests is not None:
+        t_arr = t_arr[:max_requests]
+    sizes = np.clip(np.exp(np.log(5e4) + 1.2 * g.standard_normal(len(t_arr))), 256, 4e6).astype(np.int64)
+    starts = g.integers(0, pool_n - sizes + 1)
+    eps = (1.0 + np.sqrt(2.0)) / 50.0  # 1/50 of the Hernquist half-mass radius (a = 1)
+    p = A.make_params(periodic=False, box_size=0.0, pmgrid=0, soft=(eps,), precision=precision, walk=walk,
+                      group_size=64, leaf_bucket=4, eval_potential=True)
+    lat, runs = [], []
+    with A.Server(p, slots=slots, hbm_budget_bytes=64 << 30) as srv:
+        # warm the slots up (device buffers, first-launch costs)
+        for t in [srv.submit(pool[:20000], np.full(20000, 1 / 20000)) for _ in range(slots)]:
+            srv.wait(t)
+        tickets = []
+        done = threading.Event()
+        results = []
+
+        def collector():
+            k = 0
+            while not done.is_set() or k < len(tickets):
+                if k < len(tickets):
+                    r = srv.wait(tickets[k])
+                    results.append(r)
+                    k += 1
+                else:
+                    time.sleep(0.0005)
+
+        th = threading.Thread(target=collector)
+        th.start()
+        t0 = time.perf_counter()
+        for ta, n, s0 in zip(t_arr, sizes, starts):
+            now = time.perf_counter() - t0
+            if ta > now:
+                time.sleep(ta - now)
+            tickets.append(srv.submit(pool[s0:s0 + n], np.full(n, 1.0 / n)))
+        done.set()
+        th.join()
+        t_end = time.perf_counter() - t0
+        st = srv.stats()
+    lat = np.array([r["total_ms"] for r in results])
+    runs = np.array([r["run_ms"] for r in results])
+    return {"requests": int(len(results)), "arrival_rate_per_s": rate, "window_s": float(seconds),
+            "makespan_s": t_end, "solves_per_s": len(results) / t_end,
+            "particles_per_s": float(sizes[:len(results)].sum() / t_end),
+            "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                           "max": float(lat.max())},
+            "run_ms": {"p50": float(np.percentile(runs, 50)), "p99": float(np.percentile(runs, 99))},
+            "mean_n": float(sizes[:len(results)].mean()), "slots": slots, "peak_running": st["peak_running"],
+            "failed": st["failed"]}
+
+
 def copy_probe(nbytes=256 << 20):
     """One device-to-device copy of known size (the runtime's copy kernel), used to scale the
     FETCH_SIZE / WRITE_SIZE counters of a profiled run."""
@@ -121,6 +183,8 @@ def main():
     ap.add_argument("--no-errors", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--copy-probe", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--serve-rate", type=float, default=200.0)
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -239,6 +303,15 @@ def main():
             errs["cpu_oracle"] = {"median": float(np.median(e)), "p99": float(np.percentile(e, 99))}
         out["force_error_vs_direct"] = errs
         out["force_error_sample"] = int(args.err_sample)
+
+    if args.serve_seconds > 0:
+        log("serving: Poisson %.0f req/s for %.0f s, FP32 group walk, 4 slots" % (args.serve_rate, args.serve_seconds))
+        out["serving"] = serving_run(A, A.FP32, A.WALK_GROUP, 4, args.serve_rate, args.serve_seconds, seed=5)
+        log("serving: %s" % json.dumps(out["serving"]))
+        # baseline: the plain one-solve-per-launch FP64 path (one slot), on the first 2 s of the same stream
+        base_s = min(2.0, args.serve_seconds)
+        log("serving baseline: FP64 plain walk, 1 slot, first %.0f s of the stream" % base_s)
+        out["serving_fp64_plain_1slot"] = serving_run(A, A.FP64, A.WALK_PLAIN, 1, args.serve_rate, base_s, seed=5)
 
     print(json.dumps(out), flush=True)
 

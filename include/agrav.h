@@ -177,6 +177,50 @@ agrav_status agrav_get_order(agrav_ctx *ctx, int32_t *perm, uint64_t *keys);
  * FORCETEST table (gravtree_forcetest.c:933-1008), 4 x 129^3 doubles (periodic). */
 agrav_status agrav_get_tables(agrav_ctx *ctx, float *tab_force, float *tab_pot, double *ewald_tree, double *ewald_forcetest);
 
+/* ---- serving: continuous batching of independent solves ------------------
+ * No counterpart in the reference, which solves independent sets one after
+ * another (subfind_unbind, subfind_serial.c:429-560).  Requests are queued in
+ * arrival order; `slots` resident contexts (own stream, own device buffers kept
+ * between requests) admit the oldest request whose HBM footprint fits the
+ * budget.  Each request runs compute_grav_accelerations(first_step=1) on its set
+ * with the server's parameters (softening per request).  Results are identical
+ * to a solve of the same set alone. */
+typedef struct agrav_server agrav_server;
+
+typedef struct
+{
+  int64_t n;
+  const double *pos;        /* 3n, host; kept alive by the caller until the request completes */
+  const double *mass;       /* n */
+  const uint8_t *softtype;  /* n or NULL */
+  double softening;         /* softening of type 0 for this set; 0 = the server's parameter */
+  double *acc;              /* 3n output (host) */
+  double *pot;              /* n output (host) or NULL */
+} agrav_request;
+
+typedef struct
+{
+  agrav_status status;
+  double queue_ms, run_ms, total_ms; /* submit->start, start->end, submit->end */
+  int slot;
+} agrav_request_result;
+
+typedef struct
+{
+  long long completed, failed;
+  int queued, running, peak_running, slots;
+  size_t hbm_in_use, hbm_peak, hbm_budget;
+} agrav_server_info;
+
+agrav_status agrav_server_create(const agrav_params *p, int slots, size_t hbm_budget_bytes, agrav_server **out);
+agrav_status agrav_server_submit(agrav_server *s, const agrav_request *r, uint64_t *ticket);
+/* Blocks until the request is done; releases its ticket. */
+agrav_status agrav_server_wait(agrav_server *s, uint64_t ticket, agrav_request_result *res);
+agrav_status agrav_server_poll(agrav_server *s, uint64_t ticket, int *done);
+agrav_status agrav_server_stats(agrav_server *s, agrav_server_info *st);
+/* Finishes the queued requests, then stops the workers and frees the slots. */
+void agrav_server_destroy(agrav_server *s);
+
 /* Standalone 63-bit Peano-Hilbert key of 21-bit integer coordinates (host side, for checks). */
 uint64_t agrav_hilbert_key21(uint32_t x, uint32_t y, uint32_t z);
 
