@@ -91,9 +91,27 @@ EXPORTED = [
     "agrav_compute_grav_accelerations", "agrav_get_accelerations", "agrav_get_accelerations_device",
     "agrav_get_components", "agrav_gravity_forcetest", "agrav_gravity_direct", "agrav_get_timings", "agrav_get_tree",
     "agrav_get_order", "agrav_get_tables", "agrav_hilbert_key21",
+    "agrav_batch_set_particles", "agrav_batch_autotune", "agrav_batch_get_tuning", "agrav_batch_set_tuning",
     "agrav_server_create", "agrav_server_submit", "agrav_server_wait", "agrav_server_poll", "agrav_server_stats",
     "agrav_server_destroy",
 ]
+
+
+BATCH_CLASSES = 4
+
+
+class BatchTuning(C.Structure):
+    _fields_ = [
+        ("group_size", C.c_int * BATCH_CLASSES),
+        ("leaf_bucket", C.c_int * BATCH_CLASSES),
+        ("class_max_n", C.c_int64 * BATCH_CLASSES),
+        ("sample_particles", C.c_int64 * BATCH_CLASSES),
+        ("best_ms", C.c_double * BATCH_CLASSES),
+        ("tuned", C.c_int),
+    ]
+
+    def as_dict(self):
+        return {k: (list(getattr(self, k)) if k != "tuned" else self.tuned) for k, _ in self._fields_}
 
 
 class Request(C.Structure):
@@ -146,6 +164,10 @@ def _declare(L):
     L.agrav_get_tree.argtypes = [vp, C.POINTER(C.c_int), vp, vp, vp, vp, vp, vp]
     L.agrav_get_order.argtypes = [vp, vp, vp]
     L.agrav_get_tables.argtypes = [vp, vp, vp, vp, vp]
+    L.agrav_batch_set_particles.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp]
+    L.agrav_batch_autotune.argtypes = [vp, C.c_int64, C.POINTER(BatchTuning)]
+    L.agrav_batch_get_tuning.argtypes = [vp, C.POINTER(BatchTuning)]
+    L.agrav_batch_set_tuning.argtypes = [vp, C.POINTER(BatchTuning)]
     L.agrav_server_create.argtypes = [C.POINTER(Params), C.c_int, C.c_size_t, C.POINTER(vp)]
     L.agrav_server_submit.argtypes = [vp, C.POINTER(Request), C.POINTER(C.c_uint64)]
     L.agrav_server_wait.argtypes = [vp, C.c_uint64, C.POINTER(RequestResult)]
@@ -236,6 +258,39 @@ class Gravity:
     def set_particles_device(self, n, d_pos, d_mass, d_softtype=None):
         self._check(self.L.agrav_set_particles_device(self.h, n, d_pos, d_mass, d_softtype))
         self.n = n
+
+    def batch_set_particles(self, offsets, pos_rel, mass, softening, centres=None):
+        """Independent isolated sets: set s is particles offsets[s]:offsets[s+1], positions
+        as FP32 offsets from its centre (centres: (nsets, 3) or None), one softening per set."""
+        off = np.ascontiguousarray(offsets, dtype=np.int64)
+        rel = np.ascontiguousarray(pos_rel, dtype=np.float32).reshape(-1, 3)
+        m = np.ascontiguousarray(mass, dtype=np.float64)
+        eps = np.ascontiguousarray(softening, dtype=np.float64)
+        cen = None if centres is None else np.ascontiguousarray(centres, dtype=np.float64).reshape(-1, 3)
+        nsets = len(off) - 1
+        if len(eps) != nsets or off[-1] != len(rel) or len(m) != len(rel) or (cen is not None and len(cen) != nsets):
+            raise ValueError("batch arrays do not match the offsets")
+        self._check(self.L.agrav_batch_set_particles(self.h, nsets, _ptr(off), _ptr(rel), _ptr(cen), _ptr(m), _ptr(eps)))
+        self.n = len(rel)
+
+    def batch_autotune(self, sample_particles=0):
+        t = BatchTuning()
+        self._check(self.L.agrav_batch_autotune(self.h, int(sample_particles), C.byref(t)))
+        return t.as_dict()
+
+    def batch_get_tuning(self):
+        t = BatchTuning()
+        self._check(self.L.agrav_batch_get_tuning(self.h, C.byref(t)))
+        return t.as_dict()
+
+    def batch_set_tuning(self, group_size, leaf_bucket):
+        t = BatchTuning()
+        gs = [group_size] * BATCH_CLASSES if np.isscalar(group_size) else list(group_size)
+        lb = [leaf_bucket] * BATCH_CLASSES if np.isscalar(leaf_bucket) else list(leaf_bucket)
+        for k in range(BATCH_CLASSES):
+            t.group_size[k] = int(gs[k])
+            t.leaf_bucket[k] = int(lb[k])
+        self._check(self.L.agrav_batch_set_tuning(self.h, C.byref(t)))
 
     def set_old_acc(self, old_acc):
         oa = np.ascontiguousarray(old_acc, dtype=np.float32)

@@ -425,6 +425,7 @@ static agrav_status set_particles_common(agrav_ctx *c, int64_t n, const double *
   AG_HIP(hipStreamSynchronize(s));
   c->mass_total     = mt;
   c->n              = n;
+  c->batch_nsets    = 0;
   c->have_particles = true;
   c->have_oldacc    = false;
   c->have_tree = c->have_tree_force = c->have_pm = c->have_final = false;

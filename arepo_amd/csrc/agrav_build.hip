@@ -149,6 +149,82 @@ __device__ inline uint32_t to_fix(double x, const Domain &dom, int d)
   return (uint32_t)(long long)llrint((x - dom.fix_org[d]) * dom.fix_inv);
 }
 
+// Geometry of one build: a single particle set (one domain, 21 Hilbert levels) or
+// a batch of independent sets, each with its own domain cube; batch keys carry the
+// set index in their top `sbits` bits above `levels` Hilbert levels per axis.
+struct BuildGeo
+{
+  const Domain *dom;      // 1 domain, or one per set
+  const int *pset_in;     // set of each caller-order particle (batch only)
+  const int *pset;        // set of each sorted particle (batch only)
+  int sbits;              // key bits above the Hilbert digits
+  int levels;             // Hilbert levels per axis
+};
+
+// batch: each set's domain cube (domain_box.c:107-197, isolated rule with
+// BoxSize = 0: the cube holds the set's particles and the origin)
+__global__ __launch_bounds__(256) void k_set_domains(const double *__restrict__ pos, const int64_t *__restrict__ off, Domain *sdom)
+{
+  __shared__ double s_lo[3][4], s_hi[3][4];
+  const int s = blockIdx.x;
+  double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};  // preset [0, BoxSize] with BoxSize = 0
+  for(int64_t i = off[s] + threadIdx.x; i < off[s + 1]; i += blockDim.x)
+    for(int d = 0; d < 3; d++)
+      {
+        double x = pos[3 * i + d];
+        lo[d]    = x < lo[d] ? x : lo[d];
+        hi[d]    = x > hi[d] ? x : hi[d];
+      }
+  for(int d = 0; d < 3; d++)
+    {
+      for(int o = 32; o > 0; o >>= 1)
+        {
+          double a = __shfl_xor(lo[d], o, 64), bb = __shfl_xor(hi[d], o, 64);
+          lo[d]    = a < lo[d] ? a : lo[d];
+          hi[d]    = bb > hi[d] ? bb : hi[d];
+        }
+      if((threadIdx.x & 63) == 0)
+        {
+          s_lo[d][threadIdx.x >> 6] = lo[d];
+          s_hi[d][threadIdx.x >> 6] = hi[d];
+        }
+    }
+  __syncthreads();
+  if(threadIdx.x == 0)
+    {
+      Domain dm{};
+      double len = 0;
+      for(int d = 0; d < 3; d++)
+        {
+          double a = s_lo[d][0], bb = s_hi[d][0];
+          for(int w = 1; w < 4; w++)
+            {
+              a  = s_lo[d][w] < a ? s_lo[d][w] : a;
+              bb = s_hi[d][w] > bb ? s_hi[d][w] : bb;
+            }
+          dm.xmin[d] = a;
+          dm.xmax[d] = bb;
+          if(bb - a > len)
+            len = bb - a;
+        }
+      if(len <= 0)
+        len = 1.0;
+      len *= 1.2;  // domain_box.c:165
+      for(int d = 0; d < 3; d++)
+        {
+          dm.center[d]  = 0.5 * (dm.xmin[d] + dm.xmax[d]);
+          dm.corner[d]  = dm.center[d] - 0.5 * len;
+          dm.fix_org[d] = dm.corner[d];
+        }
+      dm.len      = len;
+      dm.inv_len  = 1.0 / len;
+      dm.bigfac   = len / (double)(1ll << 52);
+      dm.fix_unit = len / 2147483648.0;
+      dm.fix_inv  = 1.0 / dm.fix_unit;
+      sdom[s]     = dm;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // 2. keys (forcetree.c:394-417 wrap, :493-505 integer coordinates)
 // ---------------------------------------------------------------------------
@@ -159,13 +235,14 @@ __device__ inline uint64_t int52(double x, const Domain &dom, int d)
   return (uint64_t)__double_as_longlong(v) & 0xFFFFFFFFFFFFFull;
 }
 
-__global__ void k_keys(const double *__restrict__ pos, int64_t n, const Domain *__restrict__ domp, int periodic, double box,
+__global__ void k_keys(const double *__restrict__ pos, int64_t n, BuildGeo geo, int periodic, double box,
                        uint64_t *__restrict__ keys, int *__restrict__ idx, Domain *dom_w)
 {
   int64_t i = (int64_t)blockIdx.x * B + threadIdx.x;
   if(i >= n)
     return;
-  const Domain dom = *domp;
+  const int set    = geo.pset_in ? geo.pset_in[i] : 0;
+  const Domain dom = geo.dom[set];
   uint32_t c[3];
   bool bad = false;
   for(int d = 0; d < 3; d++)
@@ -182,20 +259,24 @@ __global__ void k_keys(const double *__restrict__ pos, int64_t n, const Domain *
       atomicExch(&dom_w->out_of_box, 1);
       c[0] = c[1] = c[2] = 0;
     }
-  keys[i] = hilbert_key21(c[0], c[1], c[2]);
-  idx[i]  = (int)i;
+  const int hb = 3 * geo.levels;  // Hilbert bits kept; their top digits are the coarser cells
+  keys[i]      = ((uint64_t)set << hb) | (hilbert_key21(c[0], c[1], c[2]) >> (3 * AG_KEY_LEVELS - hb));
+  idx[i]       = (int)i;
 }
 
 __global__ void k_gather(const int *__restrict__ idx, const double *__restrict__ pos, const double *__restrict__ mass,
                          const uint8_t *__restrict__ stype, const float *__restrict__ oldacc, int64_t n, int periodic, double box,
-                         const Domain *__restrict__ domp, double *px, double *py, double *pz, double *pm, uint8_t *ps, uint4 *prec,
+                         BuildGeo geo, int *pset_sorted, double *px, double *py, double *pz, double *pm, uint8_t *ps, uint4 *prec,
                          float *oldacc_s)
 {
   int64_t k = (int64_t)blockIdx.x * B + threadIdx.x;
   if(k >= n)
     return;
-  const Domain dom = *domp;
   int i            = idx[k];
+  const int set    = geo.pset_in ? geo.pset_in[i] : 0;
+  const Domain dom = geo.dom[set];
+  if(pset_sorted)
+    pset_sorted[k] = set;
   double x = pos[3 * (int64_t)i], y = pos[3 * (int64_t)i + 1], z = pos[3 * (int64_t)i + 2];
   if(periodic)
     {
@@ -229,7 +310,12 @@ __device__ inline int kdelta(const uint64_t *__restrict__ k, int n, int i, int j
   return __clzll((long long)(a ^ b)) - 1;  // keys use bits 62..0
 }
 
-__device__ inline int oct_level(int prefix) { return min(prefix / 3, AG_KEY_LEVELS); }
+// octree level of a binary node with the given common-prefix length; -1 while the
+// prefix has not left the set-index bits of a batch key
+__device__ inline int oct_level(int prefix, int sbits, int levels)
+{
+  return prefix < sbits ? -1 : min((prefix - sbits) / 3, levels);
+}
 
 __global__ void k_binary(const uint64_t *__restrict__ keys, int n, int *bfirst, int *blast, int *bprefix, int *bparent)
 {
@@ -270,19 +356,20 @@ __global__ void k_binary(const uint64_t *__restrict__ keys, int n, int *bfirst, 
 }
 
 // number of octree cubes each binary node stands for
-__global__ void k_oct_count(int n, const int *__restrict__ bprefix, const int *__restrict__ bparent, int *bcount)
+__global__ void k_oct_count(int n, const int *__restrict__ bprefix, const int *__restrict__ bparent, int sbits, int levels,
+                            int *bcount)
 {
   int i = blockIdx.x * B + threadIdx.x;
   if(i >= n - 1)
     return;
-  int lp    = (i == 0) ? -1 : oct_level(bprefix[bparent[i]]);
-  bcount[i] = oct_level(bprefix[i]) - lp;
+  int lp    = (i == 0) ? -1 : oct_level(bprefix[bparent[i]], sbits, levels);
+  bcount[i] = oct_level(bprefix[i], sbits, levels) - lp;
 }
 
 __global__ __launch_bounds__(B) void k_oct_emit(int n, const int *__restrict__ bfirst, const int *__restrict__ blast,
                                                 const int *__restrict__ bprefix, const int *__restrict__ bparent,
-                                                const int *__restrict__ bcount, const int *__restrict__ bbase, int *tfirst,
-                                                int *tlast, int *tlevel, unsigned *lvlmask, int *lvlcnt)
+                                                const int *__restrict__ bcount, const int *__restrict__ bbase, int sbits,
+                                                int levels, int *tfirst, int *tlast, int *tlevel, unsigned *lvlmask, int *lvlcnt)
 {
   // per-level counts are aggregated in LDS first: one global atomic per block and level
   __shared__ int s_cnt[AG_KEY_LEVELS + 1];
@@ -293,7 +380,7 @@ __global__ __launch_bounds__(B) void k_oct_emit(int n, const int *__restrict__ b
   int c = (i < n - 1) ? bcount[i] : 0;
   if(c > 0)
     {
-      int lp = (i == 0) ? -1 : oct_level(bprefix[bparent[i]]);
+      int lp = (i == 0) ? -1 : oct_level(bprefix[bparent[i]], sbits, levels);
       int f = bfirst[i], la = blast[i], b = bbase[i];
       for(int k = 0; k < c; k++)
         {
@@ -337,15 +424,15 @@ __device__ inline int pnext_of(int p, int n, const unsigned *__restrict__ lvlmas
 // place cubes in pre-order and derive geometry and threading
 __global__ void k_place(int mt, int n, const int *__restrict__ tfirst, const int *__restrict__ tlast, const int *__restrict__ tlevel,
                         const unsigned *__restrict__ lvlmask, const int *__restrict__ pbase, const double *__restrict__ px,
-                        const double *__restrict__ py, const double *__restrict__ pz, const Domain *__restrict__ domp, int *nfirst,
+                        const double *__restrict__ py, const double *__restrict__ pz, BuildGeo geo, int *nfirst,
                         int *nlast, int *nlevel, double *cx, double *cy, double *cz, double *len, int *nextnode, int *sibling,
                         int2 *nrange)
 {
   int t = blockIdx.x * B + threadIdx.x;
   if(t >= mt)
     return;
-  const Domain dom = *domp;
   int f = tfirst[t], la = tlast[t], L = tlevel[t];
+  const Domain dom = geo.dom[geo.pset ? geo.pset[f] : 0];
   unsigned mask = lvlmask[f];
   int j         = pbase[f] + __popc(mask & ((1u << L) - 1u));
   nfirst[j]     = f;
@@ -466,8 +553,10 @@ __device__ inline void node_moments(int j, int n, const int *__restrict__ cbeg, 
                                     const double *__restrict__ pm, const uint8_t *__restrict__ ps, const double *__restrict__ hsoft,
                                     int nsoft, const double *__restrict__ cx, const double *__restrict__ cy,
                                     const double *__restrict__ cz, const double *__restrict__ len, double *sx, double *sy,
-                                    double *sz, double *nm, uint8_t *maxsoft, uint4 *recA, uint4 *recB, const Domain &dom)
+                                    double *sz, double *nm, uint8_t *maxsoft, uint4 *recA, uint4 *recB, const Domain *__restrict__ doms,
+                                    const int *__restrict__ nfirst, const int *__restrict__ pset)
 {
+  const Domain &dom = doms[pset ? pset[nfirst[j]] : 0];
   double m = 0, s0 = 0, s1 = 0, s2 = 0;
   int ms = nsoft;  // sentinel type with zero softening (forcetree.c:1254)
   for(int q = cbeg[j], e = cbeg[j] + ccnt[j]; q < e; q++)
@@ -533,13 +622,14 @@ __device__ inline void node_moments(int j, int n, const int *__restrict__ cbeg, 
 }
 
 #define MOMENT_ARGS                                                                                                            \
-  n, cbeg, ccnt, child, px, py, pz, pm, ps, hsoft, nsoft, cx, cy, cz, len, sx, sy, sz, nm, maxsoft, recA, recB, dom
+  n, cbeg, ccnt, child, px, py, pz, pm, ps, hsoft, nsoft, cx, cy, cz, len, sx, sy, sz, nm, maxsoft, recA, recB, domp, nfirst, pset
 #define MOMENT_PARAMS                                                                                                          \
   int n, const int *__restrict__ cbeg, const int *__restrict__ ccnt, const int *__restrict__ child, const double *__restrict__ px, \
       const double *__restrict__ py, const double *__restrict__ pz, const double *__restrict__ pm, const uint8_t *__restrict__ ps,  \
       const double *__restrict__ hsoft, int nsoft, const double *__restrict__ cx, const double *__restrict__ cy,                   \
       const double *__restrict__ cz, const double *__restrict__ len, double *sx, double *sy, double *sz, double *nm,               \
-      uint8_t *maxsoft, uint4 *recA, uint4 *recB, const Domain *__restrict__ domp
+      uint8_t *maxsoft, uint4 *recA, uint4 *recB, const Domain *__restrict__ domp, const int *__restrict__ nfirst,       \
+      const int *__restrict__ pset
 
 // one level with many nodes: a lane per node
 __global__ __launch_bounds__(B) void k_moments_level(const int *__restrict__ list, int cnt, MOMENT_PARAMS)
@@ -547,7 +637,6 @@ __global__ __launch_bounds__(B) void k_moments_level(const int *__restrict__ lis
   int t = blockIdx.x * B + threadIdx.x;
   if(t >= cnt)
     return;
-  const Domain dom = *domp;
   node_moments<false>(list[t], MOMENT_ARGS);
 }
 
@@ -556,7 +645,6 @@ __global__ __launch_bounds__(B) void k_moments_level(const int *__restrict__ lis
 __global__ __launch_bounds__(1024) void k_moments_levels_small(const int *__restrict__ list, const int *__restrict__ lvloff, int lhi,
                                                                 int llo, MOMENT_PARAMS)
 {
-  const Domain dom = *domp;
   for(int L = lhi; L >= llo; L--)
     {
       for(int t = lvloff[L] + threadIdx.x; t < lvloff[L + 1]; t += blockDim.x)
@@ -564,6 +652,25 @@ __global__ __launch_bounds__(1024) void k_moments_levels_small(const int *__rest
       __threadfence_block();
       __syncthreads();
     }
+}
+
+// batch: the entry where each set's walk starts (its top cube, or its particle
+// when alone) and the entry that follows the set in the threaded order
+__global__ void k_set_roots(int nsets, int n, const int64_t *__restrict__ off, const unsigned *__restrict__ lvlmask,
+                            const int *__restrict__ pbase, int *root, int *end)
+{
+  int s = blockIdx.x * B + threadIdx.x;
+  if(s >= nsets)
+    return;
+  const int f = (int)off[s], l = (int)off[s + 1];
+  if(l <= f)
+    {
+      root[s] = -1;
+      end[s]  = -1;
+      return;
+    }
+  root[s] = (l - f > 1 && lvlmask[f]) ? n + pbase[f] : f;
+  end[s]  = pnext_of(l - 1, n, lvlmask, pbase);
 }
 
 // ---------------------------------------------------------------------------
@@ -597,11 +704,13 @@ agrav_status build_tree(agrav_ctx *c)
   if((st = c->d_counters.ensure(64)) != AGRAV_OK)
     return st;
 
-  // 1. domain
+  // 1. domain (one cube, or one per set of a batch)
+  const int nsets = c->batch_nsets;
+  BuildGeo geo{c->d_dom.p, nullptr, nullptr, 0, AG_KEY_LEVELS};
   unsigned long long mm_init[6] = {~0ull, ~0ull, ~0ull, 0, 0, 0};
   unsigned long long *d_mm      = (unsigned long long *)c->d_counters.p;  // 6 x u64 = 12 ints
   AG_HIP(hipMemcpyAsync(d_mm, mm_init, sizeof(mm_init), hipMemcpyHostToDevice, s));
-  if(!c->par.periodic)
+  if(!c->par.periodic && nsets == 0)
     {
       unsigned g = nblk(n);
       if(g > 1024)
@@ -610,14 +719,30 @@ agrav_status build_tree(agrav_ctx *c)
     }
   k_domain_finish<<<1, 1, 0, s>>>(d_mm, c->d_dom.p, c->par.periodic, c->par.box_size, n);
   AG_HIP(hipGetLastError());
+  if(nsets > 0)
+    {
+      if((st = c->d_sdom.ensure(nsets)) || (st = c->d_pset.ensure(n)))
+        return st;
+      int bits = 0;
+      while((1ll << bits) < nsets)
+        bits++;
+      geo.levels  = (63 - bits) / 3;
+      geo.sbits   = 63 - 3 * geo.levels;
+      geo.dom     = c->d_sdom.p;
+      geo.pset_in = c->d_pset_in.p;
+      geo.pset    = c->d_pset.p;
+      k_set_domains<<<nsets, 256, 0, s>>>(c->d_pos_in.p, c->d_soff.p, c->d_sdom.p);
+      AG_HIP(hipGetLastError());
+    }
+  c->geo_sbits  = geo.sbits;
+  c->geo_levels = geo.levels;
 
   // 2. keys and sort
   if((st = c->d_key.ensure(n)) || (st = c->d_key2.ensure(n)) || (st = c->d_idx.ensure(n)) || (st = c->d_idx2.ensure(n)))
     return st;
-  k_keys<<<nblk(n), B, 0, s>>>(c->d_pos_in.p, n, c->d_dom.p, c->par.periodic, c->par.box_size, c->d_key.p, c->d_idx.p,
-                               c->d_dom.p);
+  k_keys<<<nblk(n), B, 0, s>>>(c->d_pos_in.p, n, geo, c->par.periodic, c->par.box_size, c->d_key.p, c->d_idx.p, c->d_dom.p);
   AG_HIP(hipGetLastError());
-  if((st = radix_sort_u64(c->d_key.p, c->d_idx.p, c->d_key2.p, c->d_idx2.p, n, 3 * AG_KEY_LEVELS, c->d_temp.p, s)) != AGRAV_OK)
+  if((st = radix_sort_u64(c->d_key.p, c->d_idx.p, c->d_key2.p, c->d_idx2.p, n, 63, c->d_temp.p, s)) != AGRAV_OK)
     return st;
   // sorted keys/perm are now in d_key2/d_idx2
 
@@ -628,8 +753,8 @@ agrav_status build_tree(agrav_ctx *c)
   if(multisoft && (st = c->d_ps.ensure(n)))
     return st;
   k_gather<<<nblk(n), B, 0, s>>>(c->d_idx2.p, c->d_pos_in.p, c->d_mass_in.p, c->have_stype ? c->d_stype_in.p : nullptr,
-                                 c->have_oldacc ? c->d_oldacc.p : nullptr, n, c->par.periodic, c->par.box_size, c->d_dom.p,
-                                 c->d_px.p, c->d_py.p, c->d_pz.p, c->d_pm.p, multisoft ? c->d_ps.p : nullptr, c->d_prec.p,
+                                 c->have_oldacc ? c->d_oldacc.p : nullptr, n, c->par.periodic, c->par.box_size, geo,
+                                 nsets > 0 ? c->d_pset.p : nullptr, c->d_px.p, c->d_py.p, c->d_pz.p, c->d_pm.p, multisoft ? c->d_ps.p : nullptr, c->d_prec.p,
                                  c->d_oldacc_s.p);
   AG_HIP(hipGetLastError());
 
@@ -647,7 +772,7 @@ agrav_status build_tree(agrav_ctx *c)
   if(N > 1)
     {
       k_binary<<<nblk(N - 1), B, 0, s>>>(c->d_key2.p, N, c->d_bfirst.p, c->d_blast.p, c->d_bprefix.p, c->d_bparent.p);
-      k_oct_count<<<nblk(N - 1), B, 0, s>>>(N, c->d_bprefix.p, c->d_bparent.p, c->d_bcount.p);
+      k_oct_count<<<nblk(N - 1), B, 0, s>>>(N, c->d_bprefix.p, c->d_bparent.p, geo.sbits, geo.levels, c->d_bcount.p);
       AG_HIP(hipGetLastError());
       if((st = exclusive_scan_i32(c->d_bcount.p, c->d_bbase.p, N - 1, d_total, c->d_temp.p, s)) != AGRAV_OK)
         return st;
@@ -669,7 +794,8 @@ agrav_status build_tree(agrav_ctx *c)
     return st;
   if(N > 1)
     k_oct_emit<<<nblk(N - 1), B, 0, s>>>(N, c->d_bfirst.p, c->d_blast.p, c->d_bprefix.p, c->d_bparent.p, c->d_bcount.p,
-                                         c->d_bbase.p, c->d_tfirst.p, c->d_tlast.p, c->d_tlevel.p, c->d_lvlmask.p, c->d_lvlcnt.p);
+                                         c->d_bbase.p, geo.sbits, geo.levels, c->d_tfirst.p, c->d_tlast.p, c->d_tlevel.p,
+                                         c->d_lvlmask.p, c->d_lvlcnt.p);
   else
     k_single_root<<<1, 1, 0, s>>>(c->d_tfirst.p, c->d_tlast.p, c->d_tlevel.p, c->d_lvlmask.p, c->d_lvlcnt.p);
   k_first_count<<<nblk(N), B, 0, s>>>(c->d_lvlmask.p, N, c->d_pcnt.p);
@@ -677,9 +803,15 @@ agrav_status build_tree(agrav_ctx *c)
   if((st = exclusive_scan_i32(c->d_pcnt.p, c->d_pbase.p, N, nullptr, c->d_temp.p, s)) != AGRAV_OK)
     return st;
   k_place<<<nblk(M), B, 0, s>>>(M, N, c->d_tfirst.p, c->d_tlast.p, c->d_tlevel.p, c->d_lvlmask.p, c->d_pbase.p, c->d_px.p,
-                                c->d_py.p, c->d_pz.p, c->d_dom.p, c->d_nfirst.p, c->d_nlast.p, c->d_nlevel.p, c->d_cx.p, c->d_cy.p,
+                                c->d_py.p, c->d_pz.p, geo, c->d_nfirst.p, c->d_nlast.p, c->d_nlevel.p, c->d_cx.p, c->d_cy.p,
                                 c->d_cz.p, c->d_len.p, c->d_nextnode.p, c->d_sibling.p, c->d_nrange.p);
   k_pnext<<<nblk(N), B, 0, s>>>(N, c->d_lvlmask.p, c->d_pbase.p, c->d_pnext.p);
+  if(nsets > 0)
+    {
+      if((st = c->d_sroot.ensure(nsets)) || (st = c->d_send.ensure(nsets)))
+        return st;
+      k_set_roots<<<nblk(nsets), B, 0, s>>>(nsets, N, c->d_soff.p, c->d_lvlmask.p, c->d_pbase.p, c->d_sroot.p, c->d_send.p);
+    }
   AG_HIP(hipGetLastError());
 
   // 5. child lists, then moments level by level (deepest first); runs of small
@@ -715,7 +847,8 @@ agrav_status build_tree(agrav_ctx *c)
                                                               c->d_child.p, c->d_px.p, c->d_py.p, c->d_pz.p, c->d_pm.p, d_ps,
                                                               c->d_soft.p, c->soft.nsoft, c->d_cx.p, c->d_cy.p, c->d_cz.p,
                                                               c->d_len.p, c->d_sx.p, c->d_sy.p, c->d_sz.p, c->d_nm.p,
-                                                              c->d_maxsoft.p, c->d_recA.p, c->d_recB.p, c->d_dom.p);
+                                                              c->d_maxsoft.p, c->d_recA.p, c->d_recB.p, geo.dom, c->d_nfirst.p,
+                                                              geo.pset);
           L--;
         }
       else
@@ -727,7 +860,7 @@ agrav_status build_tree(agrav_ctx *c)
                                                     c->d_child.p, c->d_px.p, c->d_py.p, c->d_pz.p, c->d_pm.p, d_ps, c->d_soft.p,
                                                     c->soft.nsoft, c->d_cx.p, c->d_cy.p, c->d_cz.p, c->d_len.p, c->d_sx.p,
                                                     c->d_sy.p, c->d_sz.p, c->d_nm.p, c->d_maxsoft.p, c->d_recA.p, c->d_recB.p,
-                                                    c->d_dom.p);
+                                                    geo.dom, c->d_nfirst.p, geo.pset);
           L = lo - 1;
         }
     }

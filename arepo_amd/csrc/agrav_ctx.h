@@ -87,6 +87,22 @@ struct agrav_ctx
   bool have_stype = false, have_oldacc = false;
   double mass_total = 0;
 
+  // batch of independent isolated sets (batch_nsets = 0: one particle set)
+  int batch_nsets = 0;
+  int geo_sbits = 0, geo_levels = AG_KEY_LEVELS;
+  std::vector<int64_t> h_soff;       // set offsets (nsets + 1), caller and sorted order alike
+  agrav::DBuf<int64_t> d_soff;
+  agrav::DBuf<int> d_pset_in, d_pset; // set of each particle in caller / sorted order
+  agrav::DBuf<int> d_sroot, d_send;   // top entry of each set and the entry after it
+  agrav::DBuf<agrav::Domain> d_sdom;  // domain cube per set
+  agrav::DBuf<double> d_shd;          // softening length h = 2.8 eps per set
+  agrav::DBuf<int4> d_groups;         // walk groups {first target, count, set, leaf bucket}
+  int batch_ngroups = 0;
+  bool batch_groups_dirty = true;
+  int batch_gs[4] = {64, 64, 64, 64}; // group size per set-size class
+  int batch_lb[4] = {4, 4, 4, 4};     // leaf bucket per set-size class
+  int batch_tuned = 0;
+
   // keys and sorted particles
   agrav::DBuf<uint64_t> d_key, d_key2;
   agrav::DBuf<int> d_idx, d_idx2;
@@ -158,5 +174,8 @@ agrav_status forcetest(agrav_ctx *c, int ns, const int *sel, double *acc, double
 agrav_status direct_all(agrav_ctx *c, double *acc, double *pot);
 agrav_status finalize(agrav_ctx *c);
 WalkConst walk_const(const agrav_ctx *c, int use_bh);
+BatchView batch_view(const agrav_ctx *c);
+agrav_status batch_groups(agrav_ctx *c);
+agrav_status walk_batch_groups(agrav_ctx *c, int use_bh, const int4 *groups, int ngroups, float *kernel_ms);
 TreeView tree_view(const agrav_ctx *c);
 }  // namespace agrav

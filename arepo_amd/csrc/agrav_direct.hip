@@ -14,6 +14,9 @@
 
 #include <math.h>
 
+#include <algorithm>
+#include <vector>
+
 namespace agrav
 {
 
@@ -588,6 +591,98 @@ static int pick_chunks(int64_t ntarget_blocks, int64_t n)
   return (int)want;
 }
 
+// Batch of isolated sets: FP64 direct summation of each target over its own set
+// (Newton + spline softening with the set's h, gravtree_forcetest.c:542-581; no
+// periodic or short-range parts), one block per target, sources strided over the
+// threads and reduced in a fixed order.
+__global__ __launch_bounds__(B) void k_direct_batch(int ns, const int *__restrict__ sel, const double *__restrict__ pos,
+                                                    const double *__restrict__ mass, const int *__restrict__ pset,
+                                                    const int64_t *__restrict__ off, const double *__restrict__ sh, double *out)
+{
+  __shared__ double s_v[4][B];
+  const int q = blockIdx.x;
+  const int i = sel ? sel[q] : q;
+  const int set = pset[i];
+  const double px = pos[3 * (int64_t)i], py = pos[3 * (int64_t)i + 1], pz = pos[3 * (int64_t)i + 2];
+  const double h = sh[set], h_inv = 1.0 / h, h3_inv = h_inv * h_inv * h_inv;
+  double v[4] = {0, 0, 0, 0};
+  for(int64_t j = off[set] + threadIdx.x; j < off[set + 1]; j += B)
+    {
+      const double dx = pos[3 * j] - px, dy = pos[3 * j + 1] - py, dz = pos[3 * j + 2] - pz;
+      const double r2 = dx * dx + dy * dy + dz * dz, r = sqrt(r2), m = mass[j];
+      double fac, wp;
+      if(r >= h)
+        {
+          fac = m / (r2 * r);
+          wp  = -m / r;
+        }
+      else
+        {
+          const double u = r * h_inv, u2 = u * u;
+          if(u < 0.5)
+            {
+              fac = m * h3_inv * (SF1 + u2 * (SF2 * u + SF3));
+              wp  = m * h_inv * (SF4 + u2 * (SF5 + u2 * (SF6 * u + SF7)));
+            }
+          else
+            {
+              const double u3 = u2 * u;
+              fac = m * h3_inv * (SF8 + SF9 * u + SF10 * u2 + SF11 * u3 + SF12 / u3);
+              wp  = m * h_inv * (SF13 + SF14 / u + u2 * (SF1 + u * (SF15 + u * (SF16 + SF17 * u))));
+            }
+        }
+      v[0] += dx * fac;
+      v[1] += dy * fac;
+      v[2] += dz * fac;
+      v[3] += wp;
+    }
+  for(int k = 0; k < 4; k++)
+    s_v[k][threadIdx.x] = v[k];
+  __syncthreads();
+  for(int w = B / 2; w > 0; w >>= 1)
+    {
+      if(threadIdx.x < w)
+        for(int k = 0; k < 4; k++)
+          s_v[k][threadIdx.x] += s_v[k][threadIdx.x + w];
+      __syncthreads();
+    }
+  if(threadIdx.x < 4)
+    out[(size_t)q * 4 + threadIdx.x] = s_v[threadIdx.x][0];
+}
+
+static agrav_status direct_batch(agrav_ctx *c, int ns, const int *sel_h, double *acc, double *pot)
+{
+  hipStream_t s = c->stream;
+  agrav_status st;
+  DBuf<int> dsel;
+  DBuf<double> dout;
+  if((sel_h && (st = dsel.ensure(ns))) || (st = dout.ensure((size_t)ns * 4)))
+    {
+      dsel.release();
+      dout.release();
+      return st;
+    }
+  if(sel_h)
+    AG_HIP(hipMemcpyAsync(dsel.p, sel_h, ns * sizeof(int), hipMemcpyHostToDevice, s));
+  k_direct_batch<<<ns, B, 0, s>>>(ns, sel_h ? dsel.p : nullptr, c->d_pos_in.p, c->d_mass_in.p, c->d_pset_in.p, c->d_soff.p,
+                                  c->d_shd.p, dout.p);
+  AG_HIP(hipGetLastError());
+  std::vector<double> h((size_t)ns * 4);
+  AG_HIP(hipMemcpyAsync(h.data(), dout.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  AG_HIP(hipStreamSynchronize(s));
+  dsel.release();
+  dout.release();
+  for(int q = 0; q < ns; q++)
+    {
+      if(acc)
+        for(int d = 0; d < 3; d++)
+          acc[3 * (size_t)q + d] = h[(size_t)q * 4 + d];
+      if(pot)
+        pot[q] = h[(size_t)q * 4 + 3];
+    }
+  return AGRAV_OK;
+}
+
 agrav_status forcetest(agrav_ctx *c, int ns, const int *sel_h, double *acc, double *acc_short, double *pot, double *pot_short)
 {
   if(!c->have_particles)
@@ -597,6 +692,15 @@ agrav_status forcetest(agrav_ctx *c, int ns, const int *sel_h, double *acc, doub
   for(int q = 0; q < ns; q++)
     if(sel_h[q] < 0 || sel_h[q] >= c->n)
       return fail(c, AGRAV_ERR_BAD_PARAM, "forcetest index out of range");
+  if(c->batch_nsets > 0)
+    {
+      // isolated sets: no short-range split
+      if(acc_short)
+        std::fill(acc_short, acc_short + 3 * (size_t)ns, 0.0);
+      if(pot_short)
+        std::fill(pot_short, pot_short + ns, 0.0);
+      return direct_batch(c, ns, sel_h, acc, pot);
+    }
   agrav_status st;
   if(c->par.periodic && !c->ewald_ft_ready)
     if((st = ewald_forcetest_table(c)) != AGRAV_OK)
@@ -670,6 +774,8 @@ agrav_status direct_all(agrav_ctx *c, double *acc, double *pot)
   const int64_t n = c->n;
   if(n == 0)
     return AGRAV_OK;
+  if(c->batch_nsets > 0)
+    return direct_batch(c, (int)n, nullptr, acc, pot);
   agrav_status st;
   const bool ew = c->par.periodic && c->par.pmgrid == 0;
   if(ew && !c->ewald_ready)

@@ -94,6 +94,18 @@ struct WalkConst
   int eval_pot;       // EVALPOTENTIAL: also sum the potential
 };
 
+// Batch of independent isolated sets sharing one tree build (group walk queue
+// and per-set geometry).
+struct BatchView
+{
+  const int4 *groups;  // {first target, count, set, leaf bucket} per walk group, largest sets first
+  const int *pset;     // set of each sorted particle
+  const int *sroot;    // top entry of each set: its cube, or its particle when alone
+  const int *send;     // entry after the set in the threaded (nextnode) order, -1 for the last
+  const double *sh;    // softening length h = 2.8 eps per set
+  const Domain *dom;   // domain cube per set
+};
+
 // hip error helper
 #define AG_HIP(call)                                                                                \
   do                                                                                                \

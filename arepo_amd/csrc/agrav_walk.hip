@@ -144,7 +144,8 @@ template <bool PM, bool EWALD>
 __global__ __launch_bounds__(256) void k_walk_plain(TreeView tv, WalkConst wc, const double *__restrict__ hsoft,
                                                     const float *__restrict__ oldacc_s, const float *__restrict__ tabs,
                                                     const double *__restrict__ ewald, int ntarget, const int *__restrict__ tlist,
-                                                    int tl_group, double *at, double *pt, int *nint, unsigned long long *isum)
+                                                    int tl_group, BatchView bv, double *at, double *pt, int *nint,
+                                                    unsigned long long *isum)
 {
   __shared__ float s_tab[256];
   s_tab[threadIdx.x] = tabs[threadIdx.x];
@@ -153,7 +154,13 @@ __global__ __launch_bounds__(256) void k_walk_plain(TreeView tv, WalkConst wc, c
   int i = -1;
   if(t < ntarget)
     {
-      if(tlist)
+      if(tlist && bv.pset)
+        {
+          // targets of failed groups of a batch: up to tl_group targets per group
+          const int4 d = bv.groups[tlist[t / tl_group]];
+          i            = (t % tl_group) < d.y ? d.x + (t % tl_group) : -1;
+        }
+      else if(tlist)
         {
           // targets of failed groups: tl_group particles per group id
           int g = tlist[t / tl_group];
@@ -169,13 +176,17 @@ __global__ __launch_bounds__(256) void k_walk_plain(TreeView tv, WalkConst wc, c
     {
       const int N       = tv.n;
       const double posx = tv.px[i], posy = tv.py[i], posz = tv.pz[i];
-      const double h_i  = hsoft[tv.ps ? tv.ps[i] : 0];
+      // a set of a batch is walked from its own top entry to the entry after it, with its softening
+      const int set     = bv.pset ? bv.pset[i] : 0;
+      const double *hs  = bv.pset ? bv.sh + set : hsoft;
+      const double h_i  = hs[tv.ps ? tv.ps[i] : 0];
       const double aold = wc.errtol * (double)oldacc_s[i];
       double ax = 0, ay = 0, az = 0, pot = 0;
-      int no = N;
+      int no         = bv.pset ? bv.sroot[set] : N;
+      const int stop = bv.pset ? bv.send[set] : -1;
       int steps = 0;
       const int max_steps = N + tv.m + 1;  // every entry is visited at most once per target
-      while(no >= 0)
+      while(no >= 0 && no != stop)
         {
           if(++steps > max_steps)
             {
@@ -190,7 +201,7 @@ __global__ __launch_bounds__(256) void k_walk_plain(TreeView tv, WalkConst wc, c
               dz         = nearest_d(tv.pz[no] - posz, wc);
               r2         = dx * dx + dy * dy + dz * dz;
               mass       = tv.pm[no];
-              double h_j = hsoft[tv.ps ? tv.ps[no] : 0];
+              double h_j = hs[tv.ps ? tv.ps[no] : 0];
               hmax       = h_j > h_i ? h_j : h_i;
               no         = tv.pnext[no];
             }
@@ -243,7 +254,7 @@ __global__ __launch_bounds__(256) void k_walk_plain(TreeView tv, WalkConst wc, c
                       continue;
                     }
                 }
-              double h_j = hsoft[tv.maxsoft[k]];
+              double h_j = bv.pset ? h_i : hsoft[tv.maxsoft[k]];
               if(h_j > h_i)
                 {
                   if(r2 < h_j * h_j)
@@ -493,29 +504,44 @@ __device__ inline void tile_compute(const Vec4<T> *__restrict__ s_list, const T 
                                               ax, ay, az, pp, cax, cay, caz, cpp, nin);
 }
 
-template <typename T, bool PM, bool EWALD, bool MSOFT>
-__global__ __launch_bounds__(64) void k_walk_group(TreeView tv, WalkConst wc, const double *__restrict__ hsoft,
-                                                   const float *__restrict__ oldacc_s, const float *__restrict__ tabs,
-                                                   const float *__restrict__ ewald_f, const double *__restrict__ ewald_d, int gs,
-                                                   double *at, double *pt, int *nint, int *failed, int *nfailed,
-                                                   unsigned long long *isum)
+// LDS of one group walk (one wavefront per block)
+template <typename T, bool MSOFT> struct WalkLds
 {
-  __shared__ int s_stack[kStackCap];
-  __shared__ Vec4<T> s_list[kListCap];
-  __shared__ T s_h[MSOFT ? kListCap : 1];
-  __shared__ float s_tab[256];
+  int stack[kStackCap];
+  Vec4<T> list[kListCap];
+  T h[MSOFT ? kListCap : 1];
+  float tab[256];  // [force | potential] short-range tables
+};
 
+// load the short-range tables; the FP32 tile loop uses the force table
+// pre-multiplied by (0.5/Asmth)^2
+template <typename T, bool PM> __device__ inline void load_tables(float *s_tab, const float *__restrict__ tabs, const WalkConst &wc)
+{
+  const float tscale = (sizeof(T) == 4 && PM) ? (float)wc.asmthinv2 : 1.0f;
+  for(int q = threadIdx.x; q < 256; q += 64)
+    s_tab[q] = q < 128 ? tabs[q] * tscale : tabs[q];
+}
+
+// Walk of one group: the np (<= 64) Hilbert-consecutive targets from p0, one per
+// lane, against the tree below the entry `root` (the tree root N, or the top entry
+// of one set of a batch).  `hsoft` is indexed by softening type; `unit` is the
+// fixed-point step of the targets' domain.  Results go to at/pt/nint at the
+// targets' sorted indices.  Returns false when the LDS stack overflowed: nothing
+// is written then and the caller re-walks the group one lane per target.
+template <typename T, bool PM, bool EWALD, bool MSOFT>
+__device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, const WalkConst &wc, const double *__restrict__ hsoft,
+                                  const float *__restrict__ oldacc_s, const float *__restrict__ ewald_f,
+                                  const double *__restrict__ ewald_d, int p0, int np, int root, T unit, int leaf_bucket, double *at,
+                                  double *pt, int *nint, unsigned long long (&cnt)[3])
+{
+  int *s_stack     = lds.stack;
+  Vec4<T> *s_list  = lds.list;
+  T *s_h           = lds.h;
+  const float *s_tab = lds.tab;
   const int lane   = threadIdx.x;
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  // the FP32 tile loop uses the force table pre-multiplied by (0.5/Asmth)^2
-  const float tscale = (sizeof(T) == 4 && PM) ? (float)wc.asmthinv2 : 1.0f;
-  for(int q = lane; q < 256; q += 64)
-    s_tab[q] = q < 128 ? tabs[q] * tscale : tabs[q];
 
   const int N  = tv.n;
-  const int g  = blockIdx.x;
-  const int p0 = g * gs;
-  const int np = min(gs, N - p0);
   const bool has = lane < np;
   const int pi   = p0 + (has ? lane : 0);
 
@@ -523,7 +549,6 @@ __global__ __launch_bounds__(64) void k_walk_group(TreeView tv, WalkConst wc, co
   T tx, ty, tz, hx, hy, hz;
   uint32_t cfx = 0, cfy = 0, cfz = 0;  // fixed-point group centre (FP32 path)
   double c0 = 0, c1 = 0, c2 = 0;       // FP64 group centre
-  const T unit = (T)wc.fix_unit;
   if constexpr(sizeof(T) == 4)
     {
       uint4 tr  = tv.prec[pi];
@@ -566,8 +591,9 @@ __global__ __launch_bounds__(64) void k_walk_group(TreeView tv, WalkConst wc, co
 
   // ---- walk
   int sp = 1;
+  __builtin_amdgcn_wave_barrier();
   if(lane == 0)
-    s_stack[0] = N;  // root entry
+    s_stack[0] = root;
   int nl = 0;
   T ax = 0, ay = 0, az = 0, pp = 0, cax = 0, cay = 0, caz = 0, cpp = 0;
   int nin        = 0;
@@ -653,7 +679,7 @@ __global__ __launch_bounds__(64) void k_walk_group(TreeView tv, WalkConst wc, co
                   // a small opened node is expanded straight into its particles, which are
                   // contiguous in Hilbert order
                   const int2 rg = tv.nrange[j];
-                  if(rg.y <= wc.leaf_bucket)
+                  if(rg.y <= leaf_bucket)
                     {
                       act   = 4;
                       pbase = rg.x;
@@ -812,14 +838,7 @@ __global__ __launch_bounds__(64) void k_walk_group(TreeView tv, WalkConst wc, co
     }
 
   if(overflow)
-    {
-      if(lane == 0)
-        {
-          int slot     = atomicAdd(nfailed, 1);
-          failed[slot] = g;
-        }
-      return;
-    }
+    return false;
   __builtin_amdgcn_wave_barrier();
   n_tiled += nl;
           tile_compute<T, PM, EWALD, MSOFT>(s_list, s_h, nl, tx, ty, tz, hi, s_tab, wc, asmthfac, asmthinv2, ewald_f, ewald_d, ax, ay,
@@ -836,13 +855,74 @@ __global__ __launch_bounds__(64) void k_walk_group(TreeView tv, WalkConst wc, co
   unsigned long long v = has ? (unsigned long long)nin : 0ull;
   for(int off = 32; off > 0; off >>= 1)
     v += __shfl_xor(v, off, 64);
-  if(lane == 0)
+  cnt[0] += v;  // wave-uniform: interactions, stack entries evaluated, tile entries
+  cnt[1] += n_popped;
+  cnt[2] += n_tiled;
+  return true;
+}
+
+__device__ inline void flush_counts(const unsigned long long (&cnt)[3], unsigned long long *isum)
+{
+  if(threadIdx.x == 0)
+    for(int k = 0; k < 3; k++)
+      if(cnt[k])
+        atomicAdd(isum + k, cnt[k]);
+}
+
+// one group of gs consecutive targets per block
+template <typename T, bool PM, bool EWALD, bool MSOFT>
+__global__ __launch_bounds__(64) void k_walk_group(TreeView tv, WalkConst wc, const double *__restrict__ hsoft,
+                                                   const float *__restrict__ oldacc_s, const float *__restrict__ tabs,
+                                                   const float *__restrict__ ewald_f, const double *__restrict__ ewald_d, int gs,
+                                                   double *at, double *pt, int *nint, int *failed, int *nfailed,
+                                                   unsigned long long *isum)
+{
+  __shared__ WalkLds<T, MSOFT> lds;
+  load_tables<T, PM>(lds.tab, tabs, wc);
+  const int g  = blockIdx.x;
+  const int p0 = g * gs;
+  const int np = min(gs, tv.n - p0);
+  unsigned long long cnt[3] = {0, 0, 0};
+  if(!walk_group<T, PM, EWALD, MSOFT>(lds, tv, wc, hsoft, oldacc_s, ewald_f, ewald_d, p0, np, tv.n, (T)wc.fix_unit,
+                                      wc.leaf_bucket, at, pt, nint, cnt) &&
+     threadIdx.x == 0)
     {
-      if(v)
-        atomicAdd(isum, v);
-      atomicAdd(isum + 1, n_popped);
-      atomicAdd(isum + 2, n_tiled);
+      int slot     = atomicAdd(nfailed, 1);
+      failed[slot] = g;
     }
+  flush_counts(cnt, isum);
+}
+
+// Batch of independent isolated sets: a persistent grid whose waves take groups
+// {first target, count, set, leaf bucket} from a queue (largest sets first) until
+// it is empty; every wave leaves when the shared counter passes the group count.
+template <typename T>
+__global__ __launch_bounds__(64) void k_walk_batch(TreeView tv, WalkConst wc, BatchView bv, const float *__restrict__ oldacc_s,
+                                                   const float *__restrict__ tabs, int ngroups, int *queue, double *at, double *pt,
+                                                   int *nint, int *failed, int *nfailed, unsigned long long *isum)
+{
+  __shared__ WalkLds<T, false> lds;
+  load_tables<T, false>(lds.tab, tabs, wc);
+  unsigned long long cnt[3] = {0, 0, 0};
+  for(;;)
+    {
+      int g = 0;
+      if(threadIdx.x == 0)
+        g = atomicAdd(queue, 1);
+      g = __shfl(g, 0, 64);
+      if(g >= ngroups)
+        break;
+      const int4 d = bv.groups[g];
+      const T unit = (T)bv.dom[d.z].fix_unit;
+      if(!walk_group<T, false, false, false>(lds, tv, wc, bv.sh + d.z, oldacc_s, nullptr, nullptr, d.x, d.y, bv.sroot[d.z], unit,
+                                             d.w, at, pt, nint, cnt) &&
+         threadIdx.x == 0)
+        {
+          int slot     = atomicAdd(nfailed, 1);
+          failed[slot] = g;
+        }
+    }
+  flush_counts(cnt, isum);
 }
 
 // ---------------------------------------------------------------------------
@@ -914,20 +994,94 @@ static void dispatch_group(bool pm, bool ew, bool ms, int ngroups, hipStream_t s
 }
 
 static void launch_plain(bool pm, bool ew, int nt, const int *tlist, int tl_group, hipStream_t s, const TreeView &tv,
-                         const WalkConst &wc, agrav_ctx *c)
+                         const WalkConst &wc, agrav_ctx *c, const BatchView &bv)
 {
   unsigned g = (unsigned)((nt + 255) / 256);
   if(g == 0)
     return;
   if(pm)
     k_walk_plain<true, false><<<g, 256, 0, s>>>(tv, wc, c->d_soft.p, c->d_oldacc_s.p, c->d_tab.p, c->d_ewald.p, nt, tlist,
-                                                 tl_group, c->d_at.p, c->d_pt.p, c->d_nint.p, c->d_isum.p);
+                                                 tl_group, bv, c->d_at.p, c->d_pt.p, c->d_nint.p, c->d_isum.p);
   else if(ew)
     k_walk_plain<false, true><<<g, 256, 0, s>>>(tv, wc, c->d_soft.p, c->d_oldacc_s.p, c->d_tab.p, c->d_ewald.p, nt, tlist,
-                                                 tl_group, c->d_at.p, c->d_pt.p, c->d_nint.p, c->d_isum.p);
+                                                 tl_group, bv, c->d_at.p, c->d_pt.p, c->d_nint.p, c->d_isum.p);
   else
     k_walk_plain<false, false><<<g, 256, 0, s>>>(tv, wc, c->d_soft.p, c->d_oldacc_s.p, c->d_tab.p, c->d_ewald.p, nt, tlist,
-                                                  tl_group, c->d_at.p, c->d_pt.p, c->d_nint.p, c->d_isum.p);
+                                                  tl_group, bv, c->d_at.p, c->d_pt.p, c->d_nint.p, c->d_isum.p);
+}
+
+BatchView batch_view(const agrav_ctx *c)
+{
+  BatchView bv{};
+  if(c->batch_nsets > 0)
+    bv = BatchView{c->d_groups.p, c->d_pset.p, c->d_sroot.p, c->d_send.p, c->d_shd.p, c->d_sdom.p};
+  return bv;
+}
+
+// persistent grid of the batched walk: as many one-wave blocks as fit on the device at once
+template <typename T> static int batch_grid(agrav_ctx *c)
+{
+  int per_cu = 0, cus = 0;
+  if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(&k_walk_batch<T>), 64, 0) != hipSuccess ||
+     per_cu < 1)
+    {
+      (void)hipGetLastError();
+      per_cu = 8;
+    }
+  if(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus < 1)
+    {
+      (void)hipGetLastError();
+      cus = 256;
+    }
+  return per_cu * cus;
+}
+
+// One batched group walk over the groups [0, ngroups) of `groups` (all sets of the
+// batch, or a sample of them while tuning); failed groups are re-walked one lane
+// per target.  Interaction totals accumulate in d_isum.
+agrav_status walk_batch_groups(agrav_ctx *c, int use_bh, const int4 *groups, int ngroups, float *kernel_ms)
+{
+  hipStream_t s      = c->stream;
+  agrav_status st;
+  WalkConst wc       = walk_const(c, use_bh);
+  wc.multisoft       = 0;
+  const TreeView tv  = tree_view(c);
+  BatchView bv       = batch_view(c);
+  bv.groups          = groups;
+  if(!c->have_tree || c->batch_nsets < 1)
+    return fail(c, AGRAV_ERR_STATE, "batched walk without a batch tree");
+  // every buffer the kernels write (the tuner may walk before any solve has)
+  const int64_t nn = c->n ? c->n : 1;
+  if((st = c->d_at.ensure(3 * nn)) || (st = c->d_pt.ensure(nn)) || (st = c->d_nint.ensure(nn)) || (st = c->d_isum.ensure(4)) ||
+     (st = c->d_failed.ensure(ngroups > 0 ? ngroups : 1)))
+    return st;
+  int *queue = c->d_counters.p + 52, *nfailed = c->d_counters.p + 48;
+  AG_HIP(hipMemsetAsync(c->d_counters.p + 48, 0, 8 * sizeof(int), s));
+  AG_HIP(hipEventRecord(c->ev[2], s));
+  if(ngroups > 0)
+    {
+      if(c->par.precision == AGRAV_FP32)
+        k_walk_batch<float><<<batch_grid<float>(c), 64, 0, s>>>(tv, wc, bv, c->d_oldacc_s.p, c->d_tab.p, ngroups, queue, c->d_at.p,
+                                                                 c->d_pt.p, c->d_nint.p, c->d_failed.p, nfailed, c->d_isum.p);
+      else
+        k_walk_batch<double><<<batch_grid<double>(c), 64, 0, s>>>(tv, wc, bv, c->d_oldacc_s.p, c->d_tab.p, ngroups, queue,
+                                                                   c->d_at.p, c->d_pt.p, c->d_nint.p, c->d_failed.p, nfailed,
+                                                                   c->d_isum.p);
+      AG_HIP(hipGetLastError());
+    }
+  AG_HIP(hipEventRecord(c->ev[3], s));
+  int nf = 0;
+  AG_HIP(hipMemcpyAsync(&nf, nfailed, sizeof(int), hipMemcpyDeviceToHost, s));
+  AG_HIP(hipStreamSynchronize(s));
+  if(nf > 0)
+    {
+      c->tim.walk_retries += nf;
+      launch_plain(false, false, nf * 64, c->d_failed.p, 64, s, tv, wc, c, bv);
+      AG_HIP(hipGetLastError());
+    }
+  if(kernel_ms)
+    AG_HIP(hipEventElapsedTime(kernel_ms, c->ev[2], c->ev[3]));
+  return AGRAV_OK;
 }
 
 agrav_status walk_tree(agrav_ctx *c, int use_bh)
@@ -963,15 +1117,23 @@ agrav_status walk_tree(agrav_ctx *c, int use_bh)
         return st;
     }
   c->tim.walk_retries = 0;
-  AG_HIP(hipEventRecord(c->ev[2], s));
-  if(c->par.walk == AGRAV_WALK_PLAIN)
+  if(c->batch_nsets > 0 && c->par.walk != AGRAV_WALK_PLAIN)
     {
-      launch_plain(pm, ew, (int)n, nullptr, 1, s, tv, wc, c);
+      // batch of sets: one persistent launch over the group queue
+      if((st = batch_groups(c)) != AGRAV_OK ||
+         (st = walk_batch_groups(c, use_bh, c->d_groups.p, c->batch_ngroups, nullptr)) != AGRAV_OK)
+        return st;
+    }
+  else if(c->par.walk == AGRAV_WALK_PLAIN)
+    {
+      AG_HIP(hipEventRecord(c->ev[2], s));
+      launch_plain(pm, ew, (int)n, nullptr, 1, s, tv, wc, c, batch_view(c));
       AG_HIP(hipGetLastError());
       AG_HIP(hipEventRecord(c->ev[3], s));
     }
   else
     {
+      AG_HIP(hipEventRecord(c->ev[2], s));
       int gs      = c->par.group_size > 0 ? c->par.group_size : 64;
       int ngroups = (int)((n + gs - 1) / gs);
       if((st = c->d_failed.ensure(ngroups)) != AGRAV_OK)
@@ -990,7 +1152,7 @@ agrav_status walk_tree(agrav_ctx *c, int use_bh)
       if(nf > 0)
         {
           c->tim.walk_retries = nf;
-          launch_plain(pm, ew, nf * gs, c->d_failed.p, gs, s, tv, wc, c);
+          launch_plain(pm, ew, nf * gs, c->d_failed.p, gs, s, tv, wc, c, BatchView{});
           AG_HIP(hipGetLastError());
         }
     }

@@ -103,3 +103,42 @@ def nfw_like_group(n: int, conc: float, g: np.random.Generator, rvir: float = 1.
 def half_mass_radius(pos: np.ndarray) -> float:
     r = np.sqrt((pos * pos).sum(1))
     return float(np.median(r))
+
+
+def _nfw_radii(n: int, conc: float, rvir: float, g: np.random.Generator) -> np.ndarray:
+    """NFW radii truncated at rvir, by interpolation in the tabulated inverse of M(<r)."""
+    x = np.concatenate([[0.0], np.geomspace(1e-5, conc, 2048)])
+    mx = np.log1p(x) - x / (1 + x)
+    return np.interp(g.random(n) * mx[-1], mx, x) * (rvir / conc)
+
+
+def unbinding_batch(nsets: int = 4096, nmin: int = 32, nmax: int = 200000, seed: int = 4, mpart: float = 1e-4):
+    """Independent isolated groups for batched potential solves.
+
+    Particle counts are log-uniform on [nmin, nmax]; each group is NFW-like or
+    Hernquist-like (alternating), concentration U(4, 15), virial radius growing as
+    N^(1/3) at fixed particle mass; softening = half-mass radius / 50.  Returns
+    (offsets[nsets+1] int64, pos_rel[N,3] float32 offsets from each group's
+    centre, mass[N] float64, eps[nsets], centres[nsets,3]).
+    """
+    g = rng(seed)
+    sizes = np.floor(np.exp(g.uniform(np.log(nmin), np.log(nmax + 1), nsets))).astype(np.int64)
+    sizes = np.clip(sizes, nmin, nmax)
+    conc = g.uniform(4.0, 15.0, nsets)
+    centres = g.uniform(0.0, 100.0, (nsets, 3))
+    off = np.zeros(nsets + 1, np.int64)
+    off[1:] = np.cumsum(sizes)
+    pos = np.empty((int(off[-1]), 3), np.float32)
+    eps = np.empty(nsets)
+    for s in range(nsets):
+        n = int(sizes[s])
+        rvir = 0.05 * (n / 1000.0) ** (1.0 / 3.0)
+        if s % 2 == 0:
+            r = _nfw_radii(n, conc[s], rvir, g)
+        else:
+            r = hernquist_radii(n, rvir / conc[s], g, rmax_over_a=conc[s])
+        p = _isotropic(r, g)
+        pos[off[s]:off[s + 1]] = p
+        eps[s] = float(np.median(r)) / 50.0
+    mass = np.full(int(off[-1]), mpart)
+    return off, pos, mass, eps, centres

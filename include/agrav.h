@@ -177,6 +177,42 @@ agrav_status agrav_get_order(agrav_ctx *ctx, int32_t *perm, uint64_t *keys);
  * FORCETEST table (gravtree_forcetest.c:933-1008), 4 x 129^3 doubles (periodic). */
 agrav_status agrav_get_tables(agrav_ctx *ctx, float *tab_force, float *tab_pot, double *ewald_tree, double *ewald_forcetest);
 
+/* ---- batch: many independent isolated sets in one build and one walk -------
+ * Replaces the per-group loop of subfind_unbind (subfind_serial.c:429-560):
+ * subfind_loctree_treebuild (subfind_loctree.c:192) and
+ * subfind_loctree_treeevaluate_potential (subfind_loctree.c:564-730) for every
+ * member of every group.  Set s holds the particles [offsets[s], offsets[s+1])
+ * (nsets+1 offsets from 0); positions are FP32 offsets from the set's centre
+ * (centres: 3 doubles per set, NULL = origin); softening[s] is the set's
+ * Plummer-equivalent softening.  Sets do not interact.  The parameters must be
+ * isolated (periodic = 0, pmgrid = 0).  After this call
+ * agrav_compute_grav_accelerations solves the whole batch (Barnes-Hut with
+ * err_tol_theta when type_of_opening_criterion = 0, as the unbinding walk at
+ * subfind_loctree.c:639, else the relative criterion seeded by a Barnes-Hut
+ * pass); results come back through agrav_get_accelerations(_device) in the
+ * caller's order.  The potential includes each particle's own softened term, as
+ * in the reference walk. */
+#define AGRAV_BATCH_CLASSES 4
+typedef struct
+{
+  int group_size[AGRAV_BATCH_CLASSES];         /* targets per wavefront group, 1..64 */
+  int leaf_bucket[AGRAV_BATCH_CLASSES];        /* opened nodes of at most this many particles go to the tile as particles */
+  int64_t class_max_n[AGRAV_BATCH_CLASSES];    /* class k: sets with class_max_n[k-1] < n <= class_max_n[k]; -1 = no limit */
+  int64_t sample_particles[AGRAV_BATCH_CLASSES]; /* autotune: particles in the timed sample of the class */
+  double best_ms[AGRAV_BATCH_CLASSES];         /* autotune: walk time of the sample with the chosen setting */
+  int tuned;
+} agrav_batch_tuning;
+
+agrav_status agrav_batch_set_particles(agrav_ctx *ctx, int nsets, const int64_t *offsets, const float *pos_rel,
+                                       const double *centres, const double *mass, const double *softening);
+/* Times every candidate group size (16, 32, 64) and leaf bucket (0, 4, 8, 16) on a
+ * deterministic sample of about sample_particles particles per set-size class
+ * (<= 0: 4M) and keeps the fastest per class for the following solves.  Builds
+ * the tree first if needed.  out may be NULL. */
+agrav_status agrav_batch_autotune(agrav_ctx *ctx, int64_t sample_particles, agrav_batch_tuning *out);
+agrav_status agrav_batch_get_tuning(agrav_ctx *ctx, agrav_batch_tuning *out);
+agrav_status agrav_batch_set_tuning(agrav_ctx *ctx, const agrav_batch_tuning *t);
+
 /* ---- serving: continuous batching of independent solves ------------------
  * No counterpart in the reference, which solves independent sets one after
  * another (subfind_unbind, subfind_serial.c:429-560).  Requests are queued in
