@@ -133,6 +133,123 @@ def serving_run(A, precision, walk, slots, rate, seconds, seed, max_requests=Non
             "failed": st["failed"]}
 
 
+def batch_run(A, nsets, steps, seed, cpu_particles, threads, err_sample):
+    """Batched potential + acceleration solves of independent isolated groups (unbinding): particle counts
+    log-uniform on [32, 200000], NFW- or Hernquist-like, softening = half-mass radius / 50, FP32 offsets
+    from each group's centre; Barnes-Hut with theta = 0.7 as the reference's unbinding walk.  One tree
+    build and one persistent walk launch per solve, group size and leaf bucket autotuned per size class.
+    Baselines: the same sets solved one launch sequence per set (sample), and the oracle's restatement
+    of the reference unbinding walk on the host (sample)."""
+    from concurrent.futures import ThreadPoolExecutor
+    from arepo_amd import ics
+    log("batch: generating %d groups" % nsets)
+    off, rel, mass, eps, cen = ics.unbinding_batch(nsets, seed=seed)
+    n = int(off[-1])
+    sizes = np.diff(off)
+    p = A.make_params(periodic=False, box_size=0.0, pmgrid=0, soft=(0.01,), precision=A.FP32, walk=A.WALK_GROUP,
+                      err_tol_theta=0.7, opening=0, leaf_bucket=4, eval_potential=True)
+    res = {"n_sets": nsets, "n_particles": n, "n_min": int(sizes.min()), "n_max": int(sizes.max()),
+           "criterion": "barnes-hut theta 0.7 (subfind_loctree.c:639)", "precision": "fp32"}
+    g = A.Gravity(p)
+    t0 = time.perf_counter()
+    g.batch_set_particles(off, rel, mass, eps, centres=cen)
+    res["upload_s"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    tun = g.batch_autotune()
+    res["autotune_s"] = time.perf_counter() - t0
+    res["tuning"] = {k: tun[k] for k in ("group_size", "leaf_bucket", "class_max_n", "sample_particles", "best_ms")}
+    log("batch: autotune %.1f s -> %s" % (res["autotune_s"], json.dumps(res["tuning"])))
+    g.compute_grav_accelerations(first_step=True)  # warm-up
+    step_ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ts = time.perf_counter()
+        g.compute_grav_accelerations(first_step=True)
+        step_ms.append(1e3 * (time.perf_counter() - ts))
+    total = time.perf_counter() - t0
+    tim = g.timings()
+    res.update({"steps": steps, "ms_per_solve": 1e3 * total / steps, "particles_per_s": n * steps / total,
+                "interactions_per_s": tim["interactions"] * steps / total,
+                "interactions_per_particle": tim["interactions"] / n,
+                "stage_ms": {"treebuild": tim["treebuild"], "treewalk": tim["treewalk"], "finalize": tim["finalize"]},
+                "walk_kernel_ms": tim["walk_kernel"], "numnodes": tim["numnodes"],
+                "walk_fallback_groups": tim["walk_retries"]})
+    log("batch: %.1f ms/solve (%.3g particles/s), build %.1f ms, walk %.1f ms"
+        % (res["ms_per_solve"], res["particles_per_s"], tim["treebuild"], tim["treewalk"]))
+    out = g.get_accelerations()
+    # errors against FP64 direct summation over each target's own group
+    sel = np.sort(np.random.Generator(np.random.Philox(97)).choice(n, min(err_sample, n), replace=False))
+    sel = sel.astype(np.int32)
+    ft = g.gravity_forcetest(sel)
+    ea = rel_err(out["acc"][sel], ft["acc"])
+    ep = np.abs(out["pot"][sel] - ft["pot"]) / np.abs(ft["pot"])
+    res["error_vs_direct"] = {"acc": {"median": float(np.median(ea)), "p99": float(np.percentile(ea, 99))},
+                              "pot": {"median": float(np.median(ep)), "p99": float(np.percentile(ep, 99))},
+                              "targets": int(len(sel))}
+    g.close()
+    # FP64 group walk on the same batch
+    p64 = A.make_params(periodic=False, box_size=0.0, pmgrid=0, soft=(0.01,), precision=A.FP64,
+                        walk=A.WALK_GROUP, err_tol_theta=0.7, opening=0, leaf_bucket=4, eval_potential=True)
+    with A.Gravity(p64) as g64:
+        g64.batch_set_particles(off, rel, mass, eps, centres=cen)
+        g64.batch_set_tuning(tun["group_size"], tun["leaf_bucket"])
+        g64.compute_grav_accelerations(first_step=True)
+        t0 = time.perf_counter()
+        g64.compute_grav_accelerations(first_step=True)
+        res["fp64_group_ms_per_solve"] = 1e3 * (time.perf_counter() - t0)
+    # one launch sequence per set (the per-group loop of the reference, on the GPU), every 16th set
+    samp = np.arange(0, nsets, 16)
+    with A.Gravity(p) as g1:
+        g1.set_particles(rel[off[0]:off[1]].astype(np.float64) + cen[0], mass[off[0]:off[1]])
+        g1.compute_grav_accelerations()
+        t_sets = 0.0
+        for s in samp:
+            pos = rel[off[s]:off[s + 1]].astype(np.float64) + cen[s]
+            g1.set_softenings([eps[s]])
+            g1.set_particles(pos, mass[off[s]:off[s + 1]])
+            t0 = time.perf_counter()
+            g1.compute_grav_accelerations(first_step=True)
+            t_sets += time.perf_counter() - t0
+    n_samp = int(sizes[samp].sum())
+    res["per_set_launches"] = {"sets": int(len(samp)), "particles": n_samp, "seconds": t_sets,
+                               "particles_per_s": n_samp / t_sets,
+                               "batched_speedup": res["particles_per_s"] / (n_samp / t_sets)}
+    # the oracle's unbinding walk (FP64 restatement of subfind_loctree.c:564-730) on the host, by group,
+    # groups in parallel: every k-th group until about cpu_particles particles
+    if cpu_particles > 0:
+        import oracle as O
+        O.lib()
+        stride = max(1, int(np.ceil(n / cpu_particles)))
+        cs = np.arange(0, nsets, stride)
+
+        def one(s):
+            pos = rel[off[s]:off[s + 1]].astype(np.float64) + cen[s]
+            return O.group_potential_bh(pos, mass[off[s]:off[s + 1]], 2.8 * eps[s], 0.7)[0]
+
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(threads) as ex:
+            pots = list(ex.map(one, cs))
+        t_cpu = time.perf_counter() - t0
+        n_cpu = int(sizes[cs].sum())
+        res["cpu_baseline"] = {"value": n_cpu / t_cpu, "unit": "particles/s", "cores": threads, "kind": "port",
+                               "sample": "oracle unbinding walk (tree build + Barnes-Hut potential) of every "
+                                         "%d-th group: %d groups, %d particles, %.1f s" % (stride, len(cs), n_cpu, t_cpu)}
+        res["speedup_vs_cpu"] = res["particles_per_s"] / res["cpu_baseline"]["value"]
+        # the oracle walk's own potential error on the sampled targets that fall in its groups
+        in_cs = np.isin(np.searchsorted(off, sel, side="right") - 1, cs)
+        if in_cs.any():
+            lookup = {int(s): q for q, s in enumerate(cs)}
+            gsel = np.searchsorted(off, sel[in_cs], side="right") - 1
+            pc = np.array([pots[lookup[int(s)]][i - off[s]] for s, i in zip(gsel, sel[in_cs])])
+            e = np.abs(pc - ft["pot"][in_cs]) / np.abs(ft["pot"][in_cs])
+            res["error_vs_direct"]["cpu_oracle_pot"] = {"median": float(np.median(e)),
+                                                        "p99": float(np.percentile(e, 99)), "targets": int(len(e))}
+            eg = ep[in_cs]
+            res["error_vs_direct"]["gpu_pot_same_targets"] = {"median": float(np.median(eg)),
+                                                              "p99": float(np.percentile(eg, 99))}
+    return res
+
+
 def copy_probe(nbytes=256 << 20):
     """One device-to-device copy of known size (the runtime's copy kernel), used to scale the
     FETCH_SIZE / WRITE_SIZE counters of a profiled run."""
@@ -185,6 +302,9 @@ def main():
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--batch-sets", type=int, default=4096)
+    ap.add_argument("--batch-steps", type=int, default=2)
+    ap.add_argument("--batch-cpu-particles", type=int, default=8 << 20)
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -312,6 +432,12 @@ def main():
         base_s = min(2.0, args.serve_seconds)
         log("serving baseline: FP64 plain walk, 1 slot, first %.0f s of the stream" % base_s)
         out["serving_fp64_plain_1slot"] = serving_run(A, A.FP64, A.WALK_PLAIN, 1, args.serve_rate, base_s, seed=5)
+
+    if args.batch_sets > 0:
+        out["batched_unbinding"] = batch_run(A, args.batch_sets, args.batch_steps, 4,
+                                             0 if args.no_cpu else args.batch_cpu_particles,
+                                             min(os.cpu_count() or 1, 16), args.err_sample)
+        log("batch: %s" % json.dumps(out["batched_unbinding"]))
 
     print(json.dumps(out), flush=True)
 
