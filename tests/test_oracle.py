@@ -228,6 +228,25 @@ def test_group_potential_matches_hernquist(oracle):
         assert abs(np.median(ratio) - 1) < 0.02, (lo, hi, np.median(ratio))
 
 
+def test_group_potential_theta_zero_is_direct_sum(oracle):
+    # with theta -> 0 every node is opened: the unbinding walk reduces to the spline
+    # direct sum over all particles, self term included (subfind_loctree.c:564-730)
+    from arepo_amd import ics
+    g = ics.rng(12)
+    n = 1500
+    pos = ics._isotropic(ics.hernquist_radii(n, 0.1, g), g) + 3.0
+    m = 1e-3 * (1 + np.arange(n) % 4)
+    h = 0.02
+    pot, _ = oracle.group_potential_bh(pos, m, h, 1e-6)
+    p = oracle.make_params(periodic=False, box_size=0.0, soft=(h / 2.8,))
+    _, pd = oracle.direct_summation(p, pos, m)
+    np.testing.assert_allclose(pot, pd, rtol=1e-11)
+    # and with theta = 0.7 it is an approximation of that sum, not something else
+    pot7, _ = oracle.group_potential_bh(pos, m, h, 0.7)
+    e = np.abs(pot7 - pd) / np.abs(pd)
+    assert 0 < np.median(e) < 5e-3
+
+
 def test_orc_compute_first_step_seeds_old_acc(oracle):
     g = np.random.Generator(np.random.Philox(11))
     pos = g.random((2000, 3))
