@@ -351,6 +351,15 @@ template <typename T> struct Vec4
   T x, y, z, w;
 };
 
+// Nearest periodic image of a pair offset (GRAVITY_NEAREST, allvars.h:603-612).
+// The group walk holds offsets from the group centre, each within half a box,
+// so a target-source offset can reach a full box; tree-only periodic runs add
+// the Ewald correction of exactly this image, whose table covers half a box.
+template <typename T> __device__ inline T nearest_image(T d, T box, T half)
+{
+  return d > half ? d - box : (d < -half ? d + box : d);
+}
+
 template <typename T, bool PM, bool EWALD, bool MSOFT>
 __device__ inline void tile_compute_generic(const Vec4<T> *__restrict__ s_list, const T *__restrict__ s_h, int nl, T tx, T ty, T tz, T hi,
                                     const float *__restrict__ s_tab, const WalkConst &wc, T asmthfac, T asmthinv2,
@@ -358,10 +367,17 @@ __device__ inline void tile_compute_generic(const Vec4<T> *__restrict__ s_list, 
                                     T &cax, T &cay, T &caz, T &cpp, int &nin)
 {
   T fx = 0, fy = 0, fz = 0, fp = 0;
+  const T box = (T)wc.box, half = (T)wc.half_box;
   for(int k = 0; k < nl; k++)
     {
       Vec4<T> e = s_list[k];
       T dx = e.x - tx, dy = e.y - ty, dz = e.z - tz;
+      if(EWALD)
+        {
+          dx = nearest_image(dx, box, half);
+          dy = nearest_image(dy, box, half);
+          dz = nearest_image(dz, box, half);
+        }
       T r2   = dx * dx + dy * dy + dz * dz;
       T hmax = hi;
       if(MSOFT)
@@ -403,10 +419,17 @@ __device__ inline void tile_compute_f32(const Vec4<float> *__restrict__ s_list, 
 {
   float fx = 0, fy = 0, fz = 0, fp = 0;
   const float hi_inv = __builtin_amdgcn_rcpf(hi);
+  const float box = (float)wc.box, half = (float)wc.half_box;
   for(int k = 0; k < nl; k++)
     {
       const Vec4<float> e = s_list[k];
-      const float dx = e.x - tx, dy = e.y - ty, dz = e.z - tz;
+      float dx = e.x - tx, dy = e.y - ty, dz = e.z - tz;
+      if(EWALD)
+        {
+          dx = nearest_image(dx, box, half);
+          dy = nearest_image(dy, box, half);
+          dz = nearest_image(dz, box, half);
+        }
       const float r2   = dx * dx + dy * dy + dz * dz;
       const float rinv = __builtin_amdgcn_rsqf(fmaxf(r2, 1e-30f));
       const float r    = r2 * rinv;

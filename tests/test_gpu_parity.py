@@ -117,7 +117,7 @@ def _direct(oracle, pos, mass, periodic, pmgrid, eps):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
-@pytest.mark.parametrize("case", ["iso_rel", "treepm_rel"])
+@pytest.mark.parametrize("case", ["iso_rel", "treepm_rel", "ewald_rel"])
 def test_group_walk_accuracy(agrav, oracle, precision, case):
     """Group walk: its error against direct summation (same pair kernel) is at most the
     per-particle walk's (AREPO's) error, up to FP32 rounding, and it never does fewer interactions."""
@@ -125,10 +125,20 @@ def test_group_walk_accuracy(agrav, oracle, precision, case):
     if case == "iso_rel":
         pos, mass = _clustered(6000, 21)
         periodic, pmgrid = False, 0
-    else:
+    elif case == "treepm_rel":
         pos, mass = _clustered(8000, 22, periodic=True)
         periodic, pmgrid = True, 32
-    ad, _ = _direct(oracle, pos, mass, periodic, pmgrid, 0.002)
+    else:
+        # tree-only periodic: sources anywhere in the box, pair offsets up to half a box
+        pos, mass = _uniform(6000, 23)
+        periodic, pmgrid = True, 0
+    if case == "ewald_rel":
+        # the periodic (Ewald) direct sum of FORCETEST, pinned against the oracle elsewhere
+        with agrav.Gravity(agrav.make_params(periodic=True, box_size=1.0, pmgrid=0, soft=(0.002,))) as g:
+            g.set_particles(pos, mass)
+            ad = g.gravity_forcetest(np.arange(len(pos), dtype=np.int32))["acc"]
+    else:
+        ad, _ = _direct(oracle, pos, mass, periodic, pmgrid, 0.002)
     oa = np.linalg.norm(ad, axis=1).astype(np.float32)
     comp, acc, pot, nint, tot, tim = _walk_both(agrav, oracle, pos, mass, periodic, pmgrid, False, old_acc=oa,
                                                 walk=agrav.WALK_GROUP, precision=prec)
