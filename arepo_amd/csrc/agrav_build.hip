@@ -355,27 +355,35 @@ __global__ void k_binary(const uint64_t *__restrict__ keys, int n, int *bfirst, 
     bparent[0] = -1;
 }
 
-// number of octree cubes each binary node stands for
-__global__ void k_oct_count(int n, const int *__restrict__ bprefix, const int *__restrict__ bparent, int sbits, int levels,
-                            int *bcount)
+// number of octree cubes each binary node stands for, and the cubes per level
+// (aggregated in LDS: one global atomic per block and level)
+__global__ __launch_bounds__(B) void k_oct_count(int n, const int *__restrict__ bprefix, const int *__restrict__ bparent, int sbits,
+                                                 int levels, int *bcount, int *lvlcnt)
 {
-  int i = blockIdx.x * B + threadIdx.x;
-  if(i >= n - 1)
-    return;
-  int lp    = (i == 0) ? -1 : oct_level(bprefix[bparent[i]], sbits, levels);
-  bcount[i] = oct_level(bprefix[i], sbits, levels) - lp;
-}
-
-__global__ __launch_bounds__(B) void k_oct_emit(int n, const int *__restrict__ bfirst, const int *__restrict__ blast,
-                                                const int *__restrict__ bprefix, const int *__restrict__ bparent,
-                                                const int *__restrict__ bcount, const int *__restrict__ bbase, int sbits,
-                                                int levels, int *tfirst, int *tlast, int *tlevel, unsigned *lvlmask, int *lvlcnt)
-{
-  // per-level counts are aggregated in LDS first: one global atomic per block and level
   __shared__ int s_cnt[AG_KEY_LEVELS + 1];
   if(threadIdx.x <= AG_KEY_LEVELS)
     s_cnt[threadIdx.x] = 0;
   __syncthreads();
+  int i = blockIdx.x * B + threadIdx.x;
+  if(i < n - 1)
+    {
+      const int lp = (i == 0) ? -1 : oct_level(bprefix[bparent[i]], sbits, levels);
+      const int lv = oct_level(bprefix[i], sbits, levels);
+      bcount[i]    = lv - lp;
+      for(int L = lp + 1; L <= lv; L++)
+        atomicAdd(&s_cnt[L], 1);
+    }
+  __syncthreads();
+  if(threadIdx.x <= AG_KEY_LEVELS && s_cnt[threadIdx.x])
+    atomicAdd(&lvlcnt[threadIdx.x], s_cnt[threadIdx.x]);
+}
+
+// the cubes of each binary node: levels lp+1 .. its own, all over its particle range
+__global__ __launch_bounds__(B) void k_oct_emit(int n, const int *__restrict__ bfirst, const int *__restrict__ blast,
+                                                const int *__restrict__ bprefix, const int *__restrict__ bparent,
+                                                const int *__restrict__ bcount, const int *__restrict__ bbase, int sbits,
+                                                int levels, int *tfirst, int *tlast, int *tlevel)
+{
   int i = blockIdx.x * B + threadIdx.x;
   int c = (i < n - 1) ? bcount[i] : 0;
   if(c > 0)
@@ -384,17 +392,28 @@ __global__ __launch_bounds__(B) void k_oct_emit(int n, const int *__restrict__ b
       int f = bfirst[i], la = blast[i], b = bbase[i];
       for(int k = 0; k < c; k++)
         {
-          int L         = lp + 1 + k;
           tfirst[b + k] = f;
           tlast[b + k]  = la;
-          tlevel[b + k] = L;
-          atomicOr(&lvlmask[f], 1u << L);
-          atomicAdd(&s_cnt[L], 1);
+          tlevel[b + k] = lp + 1 + k;
         }
     }
-  __syncthreads();
-  if(threadIdx.x <= AG_KEY_LEVELS && s_cnt[threadIdx.x])
-    atomicAdd(&lvlcnt[threadIdx.x], s_cnt[threadIdx.x]);
+}
+
+// Levels at which a cube starts at particle f, from its neighbours' keys alone:
+// the level-L cell of f (prefix of sbits + 3L bits) starts at f when f-1 differs
+// within that prefix, and is a cube (>= 2 particles) when f+1 shares it.
+__global__ void k_lvlmask(const uint64_t *__restrict__ keys, int n, int sbits, int levels, unsigned *lvlmask)
+{
+  int f = blockIdx.x * B + threadIdx.x;
+  if(f >= n)
+    return;
+  const int a = kdelta(keys, n, f, f - 1), b = kdelta(keys, n, f, f + 1);  // -1 beyond the ends
+  const int lo = a < sbits ? 0 : (a - sbits) / 3 + 1;
+  const int hi = b < sbits ? -1 : min((b - sbits) / 3, levels);
+  unsigned m   = 0;
+  for(int L = lo; L <= hi; L++)
+    m |= 1u << L;
+  lvlmask[f] = m;
 }
 
 __global__ void k_single_root(int *tfirst, int *tlast, int *tlevel, unsigned *lvlmask, int *lvlcnt)
@@ -765,18 +784,22 @@ agrav_status build_tree(agrav_ctx *c)
      (st = c->d_lvlmask.ensure(n)) || (st = c->d_pcnt.ensure(n)) || (st = c->d_pbase.ensure(n)) ||
      (st = c->d_lvlcnt.ensure(AG_KEY_LEVELS + 2)) || (st = c->d_lvloff.ensure(AG_KEY_LEVELS + 2)))
     return st;
-  AG_HIP(hipMemsetAsync(c->d_lvlmask.p, 0, n * sizeof(unsigned), s));
   AG_HIP(hipMemsetAsync(c->d_lvlcnt.p, 0, (AG_KEY_LEVELS + 2) * sizeof(int), s));
   int *d_total = c->d_counters.p + 16;
   int mt       = 1;
+  for(int L = 0; L <= AG_KEY_LEVELS; L++)
+    c->lvlcnt_h[L] = L == 0 ? 1 : 0;  // a single particle: the root alone
   if(N > 1)
     {
       k_binary<<<nblk(N - 1), B, 0, s>>>(c->d_key2.p, N, c->d_bfirst.p, c->d_blast.p, c->d_bprefix.p, c->d_bparent.p);
-      k_oct_count<<<nblk(N - 1), B, 0, s>>>(N, c->d_bprefix.p, c->d_bparent.p, geo.sbits, geo.levels, c->d_bcount.p);
+      k_oct_count<<<nblk(N - 1), B, 0, s>>>(N, c->d_bprefix.p, c->d_bparent.p, geo.sbits, geo.levels, c->d_bcount.p,
+                                            c->d_lvlcnt.p);
       AG_HIP(hipGetLastError());
       if((st = exclusive_scan_i32(c->d_bcount.p, c->d_bbase.p, N - 1, d_total, c->d_temp.p, s)) != AGRAV_OK)
         return st;
+      // the one host synchronisation of the octree: node count and cubes per level
       AG_HIP(hipMemcpyAsync(&mt, d_total, sizeof(int), hipMemcpyDeviceToHost, s));
+      AG_HIP(hipMemcpyAsync(c->lvlcnt_h, c->d_lvlcnt.p, (AG_KEY_LEVELS + 1) * sizeof(int), hipMemcpyDeviceToHost, s));
       AG_HIP(hipStreamSynchronize(s));
     }
   // capacity check in the spirit of TreeAllocFactor (forcetree.c:186-228): the
@@ -793,9 +816,11 @@ agrav_status build_tree(agrav_ctx *c)
      (st = c->d_child.ensure((size_t)M + n)) || (st = c->d_lvlnodes.ensure(M)))
     return st;
   if(N > 1)
-    k_oct_emit<<<nblk(N - 1), B, 0, s>>>(N, c->d_bfirst.p, c->d_blast.p, c->d_bprefix.p, c->d_bparent.p, c->d_bcount.p,
-                                         c->d_bbase.p, geo.sbits, geo.levels, c->d_tfirst.p, c->d_tlast.p, c->d_tlevel.p,
-                                         c->d_lvlmask.p, c->d_lvlcnt.p);
+    {
+      k_oct_emit<<<nblk(N - 1), B, 0, s>>>(N, c->d_bfirst.p, c->d_blast.p, c->d_bprefix.p, c->d_bparent.p, c->d_bcount.p,
+                                           c->d_bbase.p, geo.sbits, geo.levels, c->d_tfirst.p, c->d_tlast.p, c->d_tlevel.p);
+      k_lvlmask<<<nblk(N), B, 0, s>>>(c->d_key2.p, N, geo.sbits, geo.levels, c->d_lvlmask.p);
+    }
   else
     k_single_root<<<1, 1, 0, s>>>(c->d_tfirst.p, c->d_tlast.p, c->d_tlevel.p, c->d_lvlmask.p, c->d_lvlcnt.p);
   k_first_count<<<nblk(N), B, 0, s>>>(c->d_lvlmask.p, N, c->d_pcnt.p);
@@ -824,8 +849,6 @@ agrav_status build_tree(agrav_ctx *c)
     return st;
   k_fill_children<<<nblk(M), B, 0, s>>>(M, N, c->d_nextnode.p, c->d_sibling.p, c->d_pnext.p, c->d_cbeg.p, c->d_ccnt.p,
                                          c->d_child.p, c->d_cnn.p);
-  AG_HIP(hipMemcpyAsync(c->lvlcnt_h, c->d_lvlcnt.p, (AG_KEY_LEVELS + 1) * sizeof(int), hipMemcpyDeviceToHost, s));
-  AG_HIP(hipStreamSynchronize(s));
   int off[AG_KEY_LEVELS + 2];
   off[0] = 0;
   for(int L = 0; L <= AG_KEY_LEVELS; L++)

@@ -144,6 +144,7 @@ static constexpr int kSortBlock = 256;
 static constexpr int kSortWaves = kSortBlock / 64;
 static constexpr int kSortRounds = 8;  // keys per lane
 static constexpr int kSortTile  = kSortBlock * kSortRounds;
+static_assert(kSortBlock == 256, "the scatter gives each thread one of the 256 digits");
 
 __global__ __launch_bounds__(kSortBlock) void k_radix_hist(const uint64_t *__restrict__ keys, int64_t n, int shift,
                                                            int ntiles, int *__restrict__ hist)
@@ -162,23 +163,32 @@ __global__ __launch_bounds__(kSortBlock) void k_radix_hist(const uint64_t *__res
   hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = s_h[threadIdx.x];
 }
 
+// One tile of 2048 keys: rank stably by the current digit, stage the tile in LDS
+// in digit order, then write each digit's run to its global position with
+// consecutive lanes on consecutive addresses (coalesced stores).
 __global__ __launch_bounds__(kSortBlock) void k_radix_scatter(const uint64_t *__restrict__ kin, const int *__restrict__ vin,
                                                               uint64_t *__restrict__ kout, int *__restrict__ vout, int64_t n,
                                                               int shift, int ntiles, const int *__restrict__ offs)
 {
+  __shared__ uint64_t s_key[kSortTile];
+  __shared__ int s_val[kSortTile];
   __shared__ int s_cnt[kSortWaves][256];
+  __shared__ int s_dbase[256], s_gbase[256], s_wsum[kSortWaves];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for(int i = lane; i < 256; i += 64)
     s_cnt[w][i] = 0;
   __builtin_amdgcn_wave_barrier();
 
-  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  int64_t wbase     = (int64_t)blockIdx.x * kSortTile + (int64_t)w * (64 * kSortRounds);
+  const uint64_t lt    = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const int64_t tbase  = (int64_t)blockIdx.x * kSortTile;
+  const int tcount     = (int)(n - tbase < kSortTile ? n - tbase : kSortTile);
+  const int64_t wbase  = tbase + (int64_t)w * (64 * kSortRounds);
   uint64_t key[kSortRounds];
   int val[kSortRounds];
   int pos[kSortRounds];
   int dig[kSortRounds];
 
+  // 1. rank inside the wave: 8 ballots give the lanes with the same digit
 #pragma unroll
   for(int r = 0; r < kSortRounds; r++)
     {
@@ -205,29 +215,51 @@ __global__ __launch_bounds__(kSortBlock) void k_radix_scatter(const uint64_t *__
       __builtin_amdgcn_wave_barrier();
     }
   __syncthreads();
-  // per-digit base of each wave inside the tile
-  for(int d = threadIdx.x; d < 256; d += kSortBlock)
-    {
-      int run = 0;
+
+  // 2. per digit: wave offsets, tile-local start (scan over digits), global start
+  {
+    const int d = threadIdx.x;  // kSortBlock == 256 digits
+    int run     = 0;
 #pragma unroll
-      for(int ww = 0; ww < kSortWaves; ww++)
-        {
-          int c        = s_cnt[ww][d];
-          s_cnt[ww][d] = run;
-          run += c;
-        }
-    }
+    for(int ww = 0; ww < kSortWaves; ww++)
+      {
+        int c        = s_cnt[ww][d];
+        s_cnt[ww][d] = run;
+        run += c;
+      }
+    const int incl = wave_incl_scan(run);
+    if(lane == 63)
+      s_wsum[w] = incl;
+    __syncthreads();
+    int wpre = 0;
+#pragma unroll
+    for(int ww = 0; ww < kSortWaves; ww++)
+      wpre += ww < w ? s_wsum[ww] : 0;
+    s_dbase[d] = wpre + incl - run;
+    s_gbase[d] = offs[(int64_t)d * ntiles + blockIdx.x];
+  }
   __syncthreads();
+
+  // 3. stage the tile in digit order
 #pragma unroll
   for(int r = 0; r < kSortRounds; r++)
+    if(dig[r] >= 0)
+      {
+        const int d  = dig[r];
+        const int lr = s_dbase[d] + s_cnt[w][d] + pos[r];
+        s_key[lr]    = key[r];
+        s_val[lr]    = val[r];
+      }
+  __syncthreads();
+
+  // 4. runs of equal digits go out to consecutive global addresses
+  for(int i = threadIdx.x; i < tcount; i += kSortBlock)
     {
-      if(dig[r] >= 0)
-        {
-          int d      = dig[r];
-          int64_t dst = (int64_t)offs[(int64_t)d * ntiles + blockIdx.x] + s_cnt[w][d] + pos[r];
-          kout[dst]  = key[r];
-          vout[dst]  = val[r];
-        }
+      const uint64_t k = s_key[i];
+      const int d      = (int)((k >> shift) & 255);
+      const int64_t dst = (int64_t)s_gbase[d] + (i - s_dbase[d]);
+      kout[dst]        = k;
+      vout[dst]        = s_val[i];
     }
 }
 

@@ -302,6 +302,14 @@ __global__ __launch_bounds__(256) void k_walk_plain(TreeView tv, WalkConst wc, c
 // ---------------------------------------------------------------------------
 
 static constexpr int kStackCap = 1024;
+
+// build-time switches of the group walk (kept for A/B timing)
+#ifndef AG_NODE_RCUT_PRUNE
+#define AG_NODE_RCUT_PRUNE 1
+#endif
+#ifndef AG_XCD_REMAP
+#define AG_XCD_REMAP 1
+#endif
 static constexpr int kListCap  = 128;
 
 __device__ inline int wave_incl_scan_i(int v)
@@ -411,15 +419,23 @@ __device__ inline void tile_compute_generic(const Vec4<T> *__restrict__ s_list, 
 // 1 ulp), the short-range factors from the LDS tables without branches, and the
 // softened kernel only for the rare entries inside the softening length.  The
 // arithmetic is the reference's pair kernel (forcetree_walk.c:364-487).
+// Table layout (FP32 walks): float2 {f_i, f_(i+1) - f_i} for i = 0..127 with
+// entry 127 = {f_127, 0}, so an index clamped to 127 reads in bounds, then the
+// potential table p_0..p_127 and a copy of p_127.  Zero-mass entries never
+// enter the tile, so validity is only the table test.
 template <bool PM, bool EWALD, bool MSOFT, bool POT>
-__device__ inline void tile_compute_f32(const Vec4<float> *__restrict__ s_list, const float *__restrict__ s_h, int nl, float tx,
+__device__ inline void tile_compute_f32(const Vec4<float> *__restrict__ s_list, const float *__restrict__ s_h, int nl_in, float tx,
                                         float ty, float tz, float hi, const float *__restrict__ s_tab, const WalkConst &wc,
                                         float asmthfac, float asmthinv2, const float *__restrict__ ewald_f, float &ax, float &ay,
                                         float &az, float &pp, float &cax, float &cay, float &caz, float &cpp, int &nin)
 {
   float fx = 0, fy = 0, fz = 0, fp = 0;
-  const float hi_inv = __builtin_amdgcn_rcpf(hi);
+  const int nl         = __builtin_amdgcn_readfirstlane(nl_in);  // wave-uniform: scalar loop control
+  const float2 *ftab   = reinterpret_cast<const float2 *>(s_tab);
+  const float *ptab    = s_tab + 256;
+  const float hi_inv   = __builtin_amdgcn_rcpf(hi);
   const float box = (float)wc.box, half = (float)wc.half_box;
+  int cnt = 0;
   for(int k = 0; k < nl; k++)
     {
       const Vec4<float> e = s_list[k];
@@ -440,19 +456,21 @@ __device__ inline void tile_compute_f32(const Vec4<float> *__restrict__ s_list, 
           hinv = __builtin_amdgcn_rcpf(hmax);
         }
       float ff = 0, fpt = 0;
-      bool valid = e.w != 0.f;
+      bool valid = true;
       if(PM)
         {
           const float te = asmthfac * r;
           const int ti   = (int)te;
-          valid          = valid && ti < AG_NTAB;
-          const int tc   = ti < AG_NTAB ? ti : AG_NTAB - 1;
+          valid          = ti < AG_NTAB;
+          const int tc   = min(ti, AG_NTAB);
           const float w  = te - (float)tc;
-          const float f0 = s_tab[tc], f1 = s_tab[tc + 1];
-          const float p0 = POT ? s_tab[128 + tc] : 0.f, p1 = POT ? s_tab[128 + tc + 1] : 0.f;
-          ff             = f0 + w * (f1 - f0);  // force table is pre-scaled by asmthinv2
+          const float2 fd = ftab[tc];
+          ff = fd.x + w * fd.y;  // force table is pre-scaled by asmthinv2
           if(POT)
-            fpt = p0 + w * (p1 - p0);
+            {
+              const float p0 = ptab[tc], p1 = ptab[tc + 1];
+              fpt = p0 + w * (p1 - p0);
+            }
         }
       const float rinv2 = rinv * rinv;
       float fac, wp;
@@ -499,8 +517,9 @@ __device__ inline void tile_compute_f32(const Vec4<float> *__restrict__ s_list, 
       fz += dz * mf;
       if(EWALD && valid)
         ewald_add<float, float>(ewald_f, (float)wc.ewald_fac, dx, dy, dz, e.w, fx, fy, fz, fp);
-      nin += valid ? 1 : 0;
+      cnt += valid ? 1 : 0;
     }
+  nin += cnt;
   kahan_add(ax, cax, fx);
   kahan_add(ay, cay, fy);
   kahan_add(az, caz, fz);
@@ -530,19 +549,33 @@ __device__ inline void tile_compute(const Vec4<T> *__restrict__ s_list, const T 
 // LDS of one group walk (one wavefront per block)
 template <typename T, bool MSOFT> struct WalkLds
 {
+  alignas(16) float tab[388];  // short-range tables: FP64 walks [force 128 | potential 128], FP32 walks see tile_compute_f32
   int stack[kStackCap];
   Vec4<T> list[kListCap];
   T h[MSOFT ? kListCap : 1];
-  float tab[256];  // [force | potential] short-range tables
 };
 
-// load the short-range tables; the FP32 tile loop uses the force table
-// pre-multiplied by (0.5/Asmth)^2
+// load the short-range tables (tabs = [force 128 | potential 128], entries 0..127)
 template <typename T, bool PM> __device__ inline void load_tables(float *s_tab, const float *__restrict__ tabs, const WalkConst &wc)
 {
-  const float tscale = (sizeof(T) == 4 && PM) ? (float)wc.asmthinv2 : 1.0f;
-  for(int q = threadIdx.x; q < 256; q += 64)
-    s_tab[q] = q < 128 ? tabs[q] * tscale : tabs[q];
+  if constexpr(sizeof(T) == 4)
+    {
+      // the FP32 tile loop uses the force table pre-multiplied by (0.5/Asmth)^2
+      const float tscale = PM ? (float)wc.asmthinv2 : 1.0f;
+      float2 *ftab       = reinterpret_cast<float2 *>(s_tab);
+      for(int q = threadIdx.x; q < 128; q += 64)
+        {
+          const float f0 = tabs[q] * tscale;
+          const float f1 = q < 127 ? tabs[q + 1] * tscale : f0;
+          ftab[q]        = make_float2(f0, f1 - f0);
+          s_tab[256 + q] = tabs[128 + q];
+        }
+      if(threadIdx.x == 0)
+        s_tab[384] = tabs[255];
+    }
+  else
+    for(int q = threadIdx.x; q < 256; q += 64)
+      s_tab[q] = tabs[q];
 }
 
 // Walk of one group: the np (<= 64) Hilbert-consecutive targets from p0, one per
@@ -696,6 +729,10 @@ __device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, co
                     open = true;
                 }
               act  = open ? 2 : 1;
+              // an accepted node whose COM lies beyond Rcut of the whole group fails the
+              // table test (tabindex < NTAB, forcetree_walk.c:372) for every target
+              if(PM && AG_NODE_RCUT_PRUNE && !open && r2 > rcut2 * (T)1.0001)
+                act = 0;
               item = Vec4<T>{sxr, syr, szr, m};
               if(open)
                 {
@@ -746,7 +783,7 @@ __device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, co
         }
 
       // stage accepted entries in the LDS tile
-      const bool accp   = (act == 1 || act == 3);
+      const bool accp   = (act == 1 || act == 3) && item.w != (T)0;
       const uint64_t bm = __ballot(accp);
       const int cnt     = __popcll(bm);
       if(nl + cnt > kListCap)
@@ -808,7 +845,8 @@ __device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, co
                   }
                 if(MSOFT)
                   hp = (T)hsoft[tv.ps[e2]];
-                if(PM)
+                keep = m != (T)0;
+                if(PM && keep)
                   {
                     T ddx = (T)fabs(x) - hx, ddy = (T)fabs(y) - hy, ddz = (T)fabs(z) - hz;
                     ddx   = ddx > 0 ? ddx : 0;
@@ -892,6 +930,18 @@ __device__ inline void flush_counts(const unsigned long long (&cnt)[3], unsigned
         atomicAdd(isum + k, cnt[k]);
 }
 
+
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
+// Renumbering gives every XCD one contiguous run of groups: Hilbert-neighbouring
+// groups, which read largely the same nodes, then share an L2.
+__device__ inline int xcd_block(int b, int nb)
+{
+  if(!AG_XCD_REMAP)
+    return b;
+  const int per = nb >> 3, rem = nb & 7, x = b & 7, k = b >> 3;
+  return x * per + (x < rem ? x : rem) + k;
+}
+
 // one group of gs consecutive targets per block
 template <typename T, bool PM, bool EWALD, bool MSOFT>
 __global__ __launch_bounds__(64) void k_walk_group(TreeView tv, WalkConst wc, const double *__restrict__ hsoft,
@@ -902,7 +952,7 @@ __global__ __launch_bounds__(64) void k_walk_group(TreeView tv, WalkConst wc, co
 {
   __shared__ WalkLds<T, MSOFT> lds;
   load_tables<T, PM>(lds.tab, tabs, wc);
-  const int g  = blockIdx.x;
+  const int g  = xcd_block(blockIdx.x, gridDim.x);
   const int p0 = g * gs;
   const int np = min(gs, tv.n - p0);
   unsigned long long cnt[3] = {0, 0, 0};
@@ -917,8 +967,10 @@ __global__ __launch_bounds__(64) void k_walk_group(TreeView tv, WalkConst wc, co
 }
 
 // Batch of independent isolated sets: a persistent grid whose waves take groups
-// {first target, count, set, leaf bucket} from a queue (largest sets first) until
-// it is empty; every wave leaves when the shared counter passes the group count.
+// {first target, count, set, leaf bucket} from one queue (largest sets first)
+// until it is empty; every wave leaves when the counter passes the group count.
+// (Per-XCD queues over contiguous parts of the list measured 3 % slower: the
+// sets are small enough to stay in L2 either way, and the split adds a tail.)
 template <typename T>
 __global__ __launch_bounds__(64) void k_walk_batch(TreeView tv, WalkConst wc, BatchView bv, const float *__restrict__ oldacc_s,
                                                    const float *__restrict__ tabs, int ngroups, int *queue, double *at, double *pt,
