@@ -306,18 +306,10 @@ void agrav_destroy(agrav_ctx *c)
   c->d_pnext.release();
   c->d_prec.release();
   c->d_oldacc_s.release();
-  c->d_bfirst.release();
-  c->d_blast.release();
-  c->d_bprefix.release();
-  c->d_bparent.release();
-  c->d_bcount.release();
-  c->d_bbase.release();
   c->d_lvlmask.release();
   c->d_pcnt.release();
   c->d_pbase.release();
-  c->d_tfirst.release();
-  c->d_tlast.release();
-  c->d_tlevel.release();
+  c->d_bcnt.release();
   c->d_nfirst.release();
   c->d_nlast.release();
   c->d_nlevel.release();
@@ -339,8 +331,15 @@ void agrav_destroy(agrav_ctx *c)
   c->d_ccnt.release();
   c->d_cnn.release();
   c->d_child.release();
-  c->d_lvlcnt.release();
   c->d_lvloff.release();
+  c->d_soff.release();
+  c->d_pset_in.release();
+  c->d_pset.release();
+  c->d_sroot.release();
+  c->d_send.release();
+  c->d_sdom.release();
+  c->d_shd.release();
+  c->d_groups.release();
   c->d_lvlnodes.release();
   c->d_at.release();
   c->d_pt.release();

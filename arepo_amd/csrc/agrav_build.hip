@@ -20,6 +20,8 @@
 // particles are direct children (AREPO would split them further, up to level 30).
 #include "agrav_ctx.h"
 
+#include <algorithm>
+
 // The domain cube, integer coordinates and cube centres must round exactly like
 // the reference's C code (no fused multiply-add), so that node geometry is
 // bit-identical.
@@ -317,119 +319,77 @@ __device__ inline int oct_level(int prefix, int sbits, int levels)
   return prefix < sbits ? -1 : min((prefix - sbits) / 3, levels);
 }
 
-__global__ void k_binary(const uint64_t *__restrict__ keys, int n, int *bfirst, int *blast, int *bprefix, int *bparent)
-{
-  int i = blockIdx.x * B + threadIdx.x;
-  if(i >= n - 1)
-    return;
-  int dl = kdelta(keys, n, i, i - 1), dr = kdelta(keys, n, i, i + 1);
-  int d    = dr > dl ? 1 : -1;
-  int dmin = dr > dl ? dl : dr;
-  int lmax = 2;
-  while(kdelta(keys, n, i, i + lmax * d) > dmin)
-    lmax <<= 1;
-  int l = 0;
-  for(int t = lmax >> 1; t >= 1; t >>= 1)
-    if(kdelta(keys, n, i, i + (l + t) * d) > dmin)
-      l += t;
-  int j     = i + l * d;
-  int dnode = kdelta(keys, n, i, j);
-  int s = 0, t = l;
-  do
-    {
-      t = (t + 1) >> 1;
-      if(kdelta(keys, n, i, i + (s + t) * d) > dnode)
-        s += t;
-    }
-  while(t > 1);
-  int gamma = i + s * d + (d < 0 ? -1 : 0);
-  int first = i < j ? i : j, last = i < j ? j : i;
-  bfirst[i]  = first;
-  blast[i]   = last;
-  bprefix[i] = dnode;
-  if(first != gamma)
-    bparent[gamma] = i;
-  if(last != gamma + 1)
-    bparent[gamma + 1] = i;
-  if(i == 0)
-    bparent[0] = -1;
-}
-
-// number of octree cubes each binary node stands for, and the cubes per level
-// (aggregated in LDS: one global atomic per block and level)
-__global__ __launch_bounds__(B) void k_oct_count(int n, const int *__restrict__ bprefix, const int *__restrict__ bparent, int sbits,
-                                                 int levels, int *bcount, int *lvlcnt)
+// Levels at which a cube starts at particle f, from its neighbours' keys alone:
+// the level-L cell of f (prefix of sbits + 3L bits) starts at f when f-1 differs
+// within that prefix, and is a cube (>= 2 particles, as AREPO splits every cell
+// holding two or more) when f+1 shares it.  Also the cubes per particle, and per
+// block the cubes per level (ballots, then one row of partial counts per block:
+// global atomics on the same few words would serialise).
+__global__ __launch_bounds__(B) void k_lvlmask(const uint64_t *__restrict__ keys, int n, int sbits, int levels, unsigned *lvlmask,
+                                               int *pcnt, int *lvlpart)
 {
   __shared__ int s_cnt[AG_KEY_LEVELS + 1];
   if(threadIdx.x <= AG_KEY_LEVELS)
     s_cnt[threadIdx.x] = 0;
   __syncthreads();
-  int i = blockIdx.x * B + threadIdx.x;
-  if(i < n - 1)
+  const int f = blockIdx.x * B + threadIdx.x;
+  unsigned m  = 0;
+  if(f < n)
     {
-      const int lp = (i == 0) ? -1 : oct_level(bprefix[bparent[i]], sbits, levels);
-      const int lv = oct_level(bprefix[i], sbits, levels);
-      bcount[i]    = lv - lp;
-      for(int L = lp + 1; L <= lv; L++)
-        atomicAdd(&s_cnt[L], 1);
+      const int a = kdelta(keys, n, f, f - 1), b = kdelta(keys, n, f, f + 1);  // -1 beyond the ends
+      const int lo = a < sbits ? 0 : (a - sbits) / 3 + 1;
+      const int hi = b < sbits ? -1 : min((b - sbits) / 3, levels);
+      for(int L = lo; L <= hi; L++)
+        m |= 1u << L;
+      lvlmask[f] = m;
+      pcnt[f]    = __popc(m);
+    }
+  for(int L = 0; L <= levels; L++)
+    {
+      const int c = __popcll(__ballot((m >> L) & 1u));
+      if(c && (threadIdx.x & 63) == 0)
+        atomicAdd(&s_cnt[L], c);
     }
   __syncthreads();
-  if(threadIdx.x <= AG_KEY_LEVELS && s_cnt[threadIdx.x])
-    atomicAdd(&lvlcnt[threadIdx.x], s_cnt[threadIdx.x]);
+  if(threadIdx.x <= AG_KEY_LEVELS)
+    lvlpart[(int64_t)blockIdx.x * (AG_KEY_LEVELS + 1) + threadIdx.x] = s_cnt[threadIdx.x];
 }
 
-// the cubes of each binary node: levels lp+1 .. its own, all over its particle range
-__global__ __launch_bounds__(B) void k_oct_emit(int n, const int *__restrict__ bfirst, const int *__restrict__ blast,
-                                                const int *__restrict__ bprefix, const int *__restrict__ bparent,
-                                                const int *__restrict__ bcount, const int *__restrict__ bbase, int sbits,
-                                                int levels, int *tfirst, int *tlast, int *tlevel)
+// cubes per level: block L sums column L of the partial counts
+__global__ __launch_bounds__(B) void k_level_totals(const int *__restrict__ lvlpart, int nrows, int *lvlcnt)
 {
-  int i = blockIdx.x * B + threadIdx.x;
-  int c = (i < n - 1) ? bcount[i] : 0;
-  if(c > 0)
+  __shared__ int s_w[B / 64];
+  const int L = blockIdx.x;
+  int c       = 0;
+  for(int r0 = threadIdx.x * 8; r0 < nrows; r0 += B * 8)
     {
-      int lp = (i == 0) ? -1 : oct_level(bprefix[bparent[i]], sbits, levels);
-      int f = bfirst[i], la = blast[i], b = bbase[i];
-      for(int k = 0; k < c; k++)
-        {
-          tfirst[b + k] = f;
-          tlast[b + k]  = la;
-          tlevel[b + k] = lp + 1 + k;
-        }
+      int v[8];
+#pragma unroll
+      for(int u = 0; u < 8; u++)
+        v[u] = r0 + u < nrows ? lvlpart[(int64_t)(r0 + u) * (AG_KEY_LEVELS + 1) + L] : 0;
+#pragma unroll
+      for(int u = 0; u < 8; u++)
+        c += v[u];
+    }
+  for(int o = 32; o > 0; o >>= 1)
+    c += __shfl_xor(c, o, 64);
+  if((threadIdx.x & 63) == 0)
+    s_w[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if(threadIdx.x == 0)
+    {
+      int t = 0;
+      for(int w = 0; w < B / 64; w++)
+        t += s_w[w];
+      lvlcnt[L] = t;
     }
 }
 
-// Levels at which a cube starts at particle f, from its neighbours' keys alone:
-// the level-L cell of f (prefix of sbits + 3L bits) starts at f when f-1 differs
-// within that prefix, and is a cube (>= 2 particles) when f+1 shares it.
-__global__ void k_lvlmask(const uint64_t *__restrict__ keys, int n, int sbits, int levels, unsigned *lvlmask)
+__global__ void k_single_root(unsigned *lvlmask, int *pcnt, int *lvlcnt)
 {
-  int f = blockIdx.x * B + threadIdx.x;
-  if(f >= n)
-    return;
-  const int a = kdelta(keys, n, f, f - 1), b = kdelta(keys, n, f, f + 1);  // -1 beyond the ends
-  const int lo = a < sbits ? 0 : (a - sbits) / 3 + 1;
-  const int hi = b < sbits ? -1 : min((b - sbits) / 3, levels);
-  unsigned m   = 0;
-  for(int L = lo; L <= hi; L++)
-    m |= 1u << L;
-  lvlmask[f] = m;
-}
-
-__global__ void k_single_root(int *tfirst, int *tlast, int *tlevel, unsigned *lvlmask, int *lvlcnt)
-{
-  tfirst[0]  = 0;
-  tlast[0]   = 0;
-  tlevel[0]  = 0;
   lvlmask[0] = 1u;
+  pcnt[0]    = 1;
   lvlcnt[0]  = 1;
-}
-
-__global__ void k_first_count(const unsigned *__restrict__ lvlmask, int n, int *pcnt)
-{
-  int f = blockIdx.x * B + threadIdx.x;
-  if(f < n)
-    pcnt[f] = __popc(lvlmask[f]);
 }
 
 __device__ inline int pnext_of(int p, int n, const unsigned *__restrict__ lvlmask, const int *__restrict__ pbase)
@@ -440,50 +400,88 @@ __device__ inline int pnext_of(int p, int n, const unsigned *__restrict__ lvlmas
   return lvlmask[q] ? n + pbase[q] : q;
 }
 
-// place cubes in pre-order and derive geometry and threading
-__global__ void k_place(int mt, int n, const int *__restrict__ tfirst, const int *__restrict__ tlast, const int *__restrict__ tlevel,
-                        const unsigned *__restrict__ lvlmask, const int *__restrict__ pbase, const double *__restrict__ px,
-                        const double *__restrict__ py, const double *__restrict__ pz, BuildGeo geo, int *nfirst,
-                        int *nlast, int *nlevel, double *cx, double *cy, double *cz, double *len, int *nextnode, int *sibling,
-                        int2 *nrange)
+__device__ inline int key_lcp(uint64_t a, uint64_t b) { return a == b ? 64 : __clzll((long long)(a ^ b)) - 1; }
+
+// Place the cubes starting at each particle in pre-order (first particle, then
+// level) and derive their geometry and first link.
+__global__ void k_place(int n, const unsigned *__restrict__ lvlmask, const int *__restrict__ pbase, const double *__restrict__ px,
+                        const double *__restrict__ py, const double *__restrict__ pz, BuildGeo geo, int *nfirst, int *nlevel,
+                        double *cx, double *cy, double *cz, double *len, int *nextnode)
 {
-  int t = blockIdx.x * B + threadIdx.x;
-  if(t >= mt)
+  const int f = blockIdx.x * B + threadIdx.x;
+  if(f >= n)
     return;
-  int f = tfirst[t], la = tlast[t], L = tlevel[t];
+  const unsigned mask = lvlmask[f];
+  if(!mask)
+    return;
   const Domain dom = geo.dom[geo.pset ? geo.pset[f] : 0];
-  unsigned mask = lvlmask[f];
-  int j         = pbase[f] + __popc(mask & ((1u << L) - 1u));
-  nfirst[j]     = f;
-  nlast[j]      = la;
-  nrange[j]     = make_int2(f, la - f + 1);
-  nlevel[j]     = L;
-  len[j]        = ldexp(dom.len, -L);
-  if(L == 0)
+  double ipf[3]    = {px[f], py[f], pz[f]};
+  uint64_t ip[3];
+  for(int d = 0; d < 3; d++)
+    ip[d] = int52(ipf[d], dom, d);
+  int j = pbase[f];
+  for(unsigned rest = mask; rest; j++)
     {
-      cx[j] = dom.center[0];
-      cy[j] = dom.center[1];
-      cz[j] = dom.center[2];
+      const int L = __ffs(rest) - 1;  // shallowest remaining cube first: pre-order
+      rest &= rest - 1;
+      nfirst[j] = f;
+      nlevel[j] = L;
+      len[j]    = ldexp(dom.len, -L);
+      if(L == 0)
+        {
+          cx[j] = dom.center[0];
+          cy[j] = dom.center[1];
+          cz[j] = dom.center[2];
+        }
+      else
+        {
+          // forcetree.c:903-906: ((ipos & centermask) | mask) * BigFac + corner
+          const int sh        = 52 - L;
+          const uint64_t half = 1ull << (51 - L);
+          cx[j] = (double)(((ip[0] >> sh) << sh) | half) * dom.bigfac + dom.corner[0];
+          cy[j] = (double)(((ip[1] >> sh) << sh) | half) * dom.bigfac + dom.corner[1];
+          cz[j] = (double)(((ip[2] >> sh) << sh) | half) * dom.bigfac + dom.corner[2];
+        }
+      nextnode[j] = rest ? n + j + 1 : f;  // a deeper cube starts at the same particle, or the particle itself
     }
+}
+
+// The last particle of every cube: the end of the run of sorted keys that share
+// the cube's prefix, by a galloping search from its first particle; then the
+// particle range and the sibling link (the entry after the cube).
+__global__ void k_cube_last(int m, int n, const uint64_t *__restrict__ keys, const int *__restrict__ nfirst,
+                            const int *__restrict__ nlevel, const unsigned *__restrict__ lvlmask, const int *__restrict__ pbase,
+                            int sbits, int *nlast, int2 *nrange, int *sibling)
+{
+  const int j = blockIdx.x * B + threadIdx.x;
+  if(j >= m)
+    return;
+  const int f = nfirst[j], q = sbits + 3 * nlevel[j];
+  int end     = f;
+  if(q == 0)
+    end = n - 1;  // the root of a single set holds every particle
   else
     {
-      // forcetree.c:903-906: ((ipos & centermask) | mask) * BigFac + corner
-      const int sh       = 52 - L;
-      const uint64_t half = 1ull << (51 - L);
-      double p[3]        = {px[f], py[f], pz[f]};
-      double c[3];
-      for(int d = 0; d < 3; d++)
+      const uint64_t kf = keys[f];
+      int step          = 1;
+      while(end + step < n && key_lcp(kf, keys[end + step]) >= q)
         {
-          uint64_t ip = int52(p[d], dom, d);
-          c[d]        = (double)(((ip >> sh) << sh) | half) * dom.bigfac + dom.corner[d];
+          end += step;
+          step <<= 1;
         }
-      cx[j] = c[0];
-      cy[j] = c[1];
-      cz[j] = c[2];
+      int hi = end + step < n ? end + step : n;  // first index known outside the cube
+      while(hi - end > 1)
+        {
+          const int mid = (end + hi) >> 1;
+          if(key_lcp(kf, keys[mid]) >= q)
+            end = mid;
+          else
+            hi = mid;
+        }
     }
-  bool deeper = (mask >> (L + 1)) != 0u;
-  nextnode[j] = deeper ? n + j + 1 : f;
-  sibling[j]  = pnext_of(la, n, lvlmask, pbase);
+  nlast[j]   = end;
+  nrange[j]  = make_int2(f, end - f + 1);
+  sibling[j] = pnext_of(end, n, lvlmask, pbase);
 }
 
 __global__ void k_pnext(int n, const unsigned *__restrict__ lvlmask, const int *__restrict__ pbase, int *pnext)
@@ -493,23 +491,55 @@ __global__ void k_pnext(int n, const unsigned *__restrict__ lvlmask, const int *
     pnext[p] = pnext_of(p, n, lvlmask, pbase);
 }
 
+// Nodes listed per level for the moment passes.  A block handles kScatterChunk
+// nodes: ranks inside the block by level come from wave-level grouping of equal
+// levels, then one global reservation per block and level (atomics on the same
+// 22 words serialise, so there are few of them).  The order inside a level
+// list does not matter: every node's moments are computed on their own.
+static constexpr int kScatterItems = 16;
+static constexpr int kScatterChunk = B * kScatterItems;
+
 __global__ __launch_bounds__(B) void k_level_scatter(int m, const int *__restrict__ nlevel, const int *__restrict__ lvloff,
                                                      int *lvlfill, int *lvlnodes)
 {
-  // block-local ranks per level in LDS, one global reservation per block and level
   __shared__ int s_cnt[AG_KEY_LEVELS + 1], s_base[AG_KEY_LEVELS + 1];
   if(threadIdx.x <= AG_KEY_LEVELS)
     s_cnt[threadIdx.x] = 0;
   __syncthreads();
-  int j    = blockIdx.x * B + threadIdx.x;
-  int L    = (j < m) ? nlevel[j] : -1;
-  int rank = (L >= 0) ? atomicAdd(&s_cnt[L], 1) : 0;
+  const int lane   = threadIdx.x & 63;
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const int j0     = blockIdx.x * kScatterChunk + (threadIdx.x >> 6) * 64 * kScatterItems;
+  int lv[kScatterItems], rk[kScatterItems];
+#pragma unroll
+  for(int r = 0; r < kScatterItems; r++)
+    {
+      const int j = j0 + r * 64 + lane;
+      const int L = j < m ? nlevel[j] : -1;
+      lv[r]       = L;
+      rk[r]       = 0;
+      uint64_t todo = __ballot(L >= 0);
+      while(todo)
+        {
+          const int leader = __ffsll((long long)todo) - 1;
+          const int Ll     = __shfl(L, leader, 64);
+          const uint64_t mk = __ballot(L == Ll);
+          int base = 0;
+          if(lane == leader)
+            base = atomicAdd(&s_cnt[Ll], __popcll(mk));
+          base = __shfl(base, leader, 64);
+          if(L == Ll)
+            rk[r] = base + __popcll(mk & lt);
+          todo &= ~mk;
+        }
+    }
   __syncthreads();
   if(threadIdx.x <= AG_KEY_LEVELS)
     s_base[threadIdx.x] = s_cnt[threadIdx.x] ? atomicAdd(&lvlfill[threadIdx.x], s_cnt[threadIdx.x]) : 0;
   __syncthreads();
-  if(L >= 0)
-    lvlnodes[lvloff[L] + s_base[L] + rank] = j;
+#pragma unroll
+  for(int r = 0; r < kScatterItems; r++)
+    if(lv[r] >= 0)
+      lvlnodes[lvloff[lv[r]] + s_base[lv[r]] + rk[r]] = j0 + r * 64 + lane;
 }
 
 // ---------------------------------------------------------------------------
@@ -578,46 +608,68 @@ __device__ inline void node_moments(int j, int n, const int *__restrict__ cbeg, 
   const Domain &dom = doms[pset ? pset[nfirst[j]] : 0];
   double m = 0, s0 = 0, s1 = 0, s2 = 0;
   int ms = nsoft;  // sentinel type with zero softening (forcetree.c:1254)
-  for(int q = cbeg[j], e = cbeg[j] + ccnt[j]; q < e; q++)
+  // children in batches of 8 (an octree cube has at most 8, bar buckets at the last
+  // level): all loads of a batch are issued before any is used, then the sums run
+  // in list order
+  const int b0 = cbeg[j], cn = ccnt[j];
+  for(int q0 = 0; q0 < cn; q0 += 8)
     {
-      const int c = child[q];
-      double mm, x, y, z;
-      int st;
-      if(c < n)
+      int cid[8];
+#pragma unroll
+      for(int u = 0; u < 8; u++)
+        cid[u] = q0 + u < cn ? child[b0 + q0 + u] : -1;
+      double mm[8], x[8], y[8], z[8];
+      int st[8];
+#pragma unroll
+      for(int u = 0; u < 8; u++)
         {
-          mm = pm[c];
-          x  = px[c];
-          y  = py[c];
-          z  = pz[c];
-          st = ps ? ps[c] : 0;
-        }
-      else
-        {
-          const int k = c - n;
-          if(VOLATILE)
+          const int c = cid[u];
+          if(c < 0)
             {
-              // written earlier in this launch by another lane: agent-coherent loads
-              mm = __hip_atomic_load(&nm[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-              x  = __hip_atomic_load(&sx[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-              y  = __hip_atomic_load(&sy[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-              z  = __hip_atomic_load(&sz[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-              st = __hip_atomic_load(&maxsoft[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              mm[u] = x[u] = y[u] = z[u] = 0;
+              st[u] = nsoft;
+            }
+          else if(c < n)
+            {
+              mm[u] = pm[c];
+              x[u]  = px[c];
+              y[u]  = py[c];
+              z[u]  = pz[c];
+              st[u] = ps ? ps[c] : 0;
             }
           else
             {
-              mm = nm[k];
-              x  = sx[k];
-              y  = sy[k];
-              z  = sz[k];
-              st = maxsoft[k];
+              const int k = c - n;
+              if(VOLATILE)
+                {
+                  // written earlier in this launch by another lane: agent-coherent loads
+                  mm[u] = __hip_atomic_load(&nm[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                  x[u]  = __hip_atomic_load(&sx[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                  y[u]  = __hip_atomic_load(&sy[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                  z[u]  = __hip_atomic_load(&sz[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                  st[u] = __hip_atomic_load(&maxsoft[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+              else
+                {
+                  mm[u] = nm[k];
+                  x[u]  = sx[k];
+                  y[u]  = sy[k];
+                  z[u]  = sz[k];
+                  st[u] = maxsoft[k];
+                }
             }
         }
-      m += mm;
-      s0 += mm * x;
-      s1 += mm * y;
-      s2 += mm * z;
-      if(hsoft[ms] < hsoft[st])
-        ms = st;
+#pragma unroll
+      for(int u = 0; u < 8; u++)
+        if(cid[u] >= 0)
+          {
+            m += mm[u];
+            s0 += mm[u] * x[u];
+            s1 += mm[u] * y[u];
+            s2 += mm[u] * z[u];
+            if(hsoft[ms] < hsoft[st[u]])
+              ms = st[u];
+          }
     }
   if(m != 0)
     {
@@ -640,6 +692,81 @@ __device__ inline void node_moments(int j, int n, const int *__restrict__ cbeg, 
   recB[j]    = make_uint4(to_fix(cx[j], dom, 0), to_fix(cy[j], dom, 1), to_fix(cz[j], dom, 2), __float_as_uint((float)len[j]));
 }
 
+// Cubes of at most kSmallCube particles take their moments straight from their
+// particle range (contiguous in Hilbert order), summed in particle order: all of
+// them at once, with no dependence between levels.  Larger cubes sum their
+// children, level by level.
+#ifndef AG_SMALL_CUBE
+#define AG_SMALL_CUBE 16
+#endif
+static constexpr int kSmallCube = AG_SMALL_CUBE;
+
+__global__ __launch_bounds__(B) void k_moments_direct(int m, const int2 *__restrict__ nrange, const double *__restrict__ px,
+                                                      const double *__restrict__ py, const double *__restrict__ pz,
+                                                      const double *__restrict__ pm, const uint8_t *__restrict__ ps,
+                                                      const double *__restrict__ hsoft, int nsoft, const double *__restrict__ cx,
+                                                      const double *__restrict__ cy, const double *__restrict__ cz,
+                                                      const double *__restrict__ len, double *sx, double *sy, double *sz, double *nm,
+                                                      uint8_t *maxsoft, uint4 *recA, uint4 *recB, const Domain *__restrict__ doms,
+                                                      const int *__restrict__ pset)
+{
+  const int j = blockIdx.x * B + threadIdx.x;
+  if(j >= m)
+    return;
+  const int2 rg = nrange[j];
+  if(rg.y > kSmallCube)
+    return;
+  const Domain &dom = doms[pset ? pset[rg.x] : 0];
+  double mass = 0, s0 = 0, s1 = 0, s2 = 0;
+  int ms = nsoft;  // sentinel type with zero softening (forcetree.c:1254)
+  for(int i0 = rg.x; i0 < rg.x + rg.y; i0 += 8)
+    {
+      double mm[8], x[8], y[8], z[8];
+      int st[8];
+#pragma unroll
+      for(int u = 0; u < 8; u++)
+        {
+          const int i = i0 + u;
+          const bool in = i < rg.x + rg.y;
+          mm[u] = in ? pm[i] : 0.0;
+          x[u]  = in ? px[i] : 0.0;
+          y[u]  = in ? py[i] : 0.0;
+          z[u]  = in ? pz[i] : 0.0;
+          st[u] = in ? (ps ? ps[i] : 0) : nsoft;
+        }
+#pragma unroll
+      for(int u = 0; u < 8; u++)
+        if(i0 + u < rg.x + rg.y)
+          {
+            mass += mm[u];
+            s0 += mm[u] * x[u];
+            s1 += mm[u] * y[u];
+            s2 += mm[u] * z[u];
+            if(hsoft[ms] < hsoft[st[u]])
+              ms = st[u];
+          }
+    }
+  if(mass != 0)
+    {
+      s0 /= mass;
+      s1 /= mass;
+      s2 /= mass;
+    }
+  else
+    {
+      s0 = cx[j];
+      s1 = cy[j];
+      s2 = cz[j];
+    }
+  sx[j]      = s0;
+  sy[j]      = s1;
+  sz[j]      = s2;
+  nm[j]      = mass;
+  maxsoft[j] = (uint8_t)ms;
+  recA[j]    = make_uint4(to_fix(s0, dom, 0), to_fix(s1, dom, 1), to_fix(s2, dom, 2), __float_as_uint((float)mass));
+  recB[j]    = make_uint4(to_fix(cx[j], dom, 0), to_fix(cy[j], dom, 1), to_fix(cz[j], dom, 2), __float_as_uint((float)len[j]));
+}
+
 #define MOMENT_ARGS                                                                                                            \
   n, cbeg, ccnt, child, px, py, pz, pm, ps, hsoft, nsoft, cx, cy, cz, len, sx, sy, sz, nm, maxsoft, recA, recB, domp, nfirst, pset
 #define MOMENT_PARAMS                                                                                                          \
@@ -650,24 +777,31 @@ __device__ inline void node_moments(int j, int n, const int *__restrict__ cbeg, 
       uint8_t *maxsoft, uint4 *recA, uint4 *recB, const Domain *__restrict__ domp, const int *__restrict__ nfirst,       \
       const int *__restrict__ pset
 
-// one level with many nodes: a lane per node
-__global__ __launch_bounds__(B) void k_moments_level(const int *__restrict__ list, int cnt, MOMENT_PARAMS)
+// one level with many nodes: a lane per node (small cubes are done already)
+__global__ __launch_bounds__(B) void k_moments_level(const int *__restrict__ list, int cnt, const int2 *__restrict__ nrange,
+                                                     MOMENT_PARAMS)
 {
   int t = blockIdx.x * B + threadIdx.x;
   if(t >= cnt)
     return;
-  node_moments<false>(list[t], MOMENT_ARGS);
+  const int j = list[t];
+  if(nrange[j].y > kSmallCube)
+    node_moments<false>(j, MOMENT_ARGS);
 }
 
 // a run of consecutive small levels (deepest first) in one workgroup: levels are
 // separated by workgroup barriers, child data are read past L1
 __global__ __launch_bounds__(1024) void k_moments_levels_small(const int *__restrict__ list, const int *__restrict__ lvloff, int lhi,
-                                                                int llo, MOMENT_PARAMS)
+                                                                int llo, const int2 *__restrict__ nrange, MOMENT_PARAMS)
 {
   for(int L = lhi; L >= llo; L--)
     {
       for(int t = lvloff[L] + threadIdx.x; t < lvloff[L + 1]; t += blockDim.x)
-        node_moments<true>(list[t], MOMENT_ARGS);
+        {
+          const int j = list[t];
+          if(nrange[j].y > kSmallCube)
+            node_moments<false>(j, MOMENT_ARGS);
+        }
       __threadfence_block();
       __syncthreads();
     }
@@ -720,8 +854,13 @@ agrav_status build_tree(agrav_ctx *c)
     return st;
   if((st = c->d_dom.ensure(1)) != AGRAV_OK)
     return st;
-  if((st = c->d_counters.ensure(64)) != AGRAV_OK)
+  // device counters: [0..11] domain extent (6 u64), [12] octree node count, [13] a scan or
+  // sort look-back that timed out, [14..35] cubes per level (read back together),
+  // [48..55] walk queue and failures, [50] threading error, [64..85] level fill
+  if((st = c->d_counters.ensure(128)) != AGRAV_OK)
     return st;
+  int *d_total = c->d_counters.p + 12, *d_perr = c->d_counters.p + 13, *d_lvlcnt = c->d_counters.p + 14;
+  AG_HIP(hipMemsetAsync(d_total, 0, 24 * sizeof(int), s));
 
   // 1. domain (one cube, or one per set of a batch)
   const int nsets = c->batch_nsets;
@@ -761,7 +900,7 @@ agrav_status build_tree(agrav_ctx *c)
     return st;
   k_keys<<<nblk(n), B, 0, s>>>(c->d_pos_in.p, n, geo, c->par.periodic, c->par.box_size, c->d_key.p, c->d_idx.p, c->d_dom.p);
   AG_HIP(hipGetLastError());
-  if((st = radix_sort_u64(c->d_key.p, c->d_idx.p, c->d_key2.p, c->d_idx2.p, n, 63, c->d_temp.p, s)) != AGRAV_OK)
+  if((st = radix_sort_u64(c->d_key.p, c->d_idx.p, c->d_key2.p, c->d_idx2.p, n, 63, c->d_temp.p, d_perr, s)) != AGRAV_OK)
     return st;
   // sorted keys/perm are now in d_key2/d_idx2
 
@@ -777,59 +916,49 @@ agrav_status build_tree(agrav_ctx *c)
                                  c->d_oldacc_s.p);
   AG_HIP(hipGetLastError());
 
-  // 3./4. binary tree -> octree cubes
-  const int nb = N > 1 ? N - 1 : 1;
-  if((st = c->d_bfirst.ensure(nb)) || (st = c->d_blast.ensure(nb)) || (st = c->d_bprefix.ensure(nb)) ||
-     (st = c->d_bparent.ensure(nb)) || (st = c->d_bcount.ensure(nb)) || (st = c->d_bbase.ensure(nb)) ||
-     (st = c->d_lvlmask.ensure(n)) || (st = c->d_pcnt.ensure(n)) || (st = c->d_pbase.ensure(n)) ||
-     (st = c->d_lvlcnt.ensure(AG_KEY_LEVELS + 2)) || (st = c->d_lvloff.ensure(AG_KEY_LEVELS + 2)))
+  // 3./4. octree cubes straight from the sorted keys
+  if((st = c->d_lvlmask.ensure(n)) || (st = c->d_pcnt.ensure(n)) || (st = c->d_pbase.ensure(n)) ||
+     (st = c->d_lvloff.ensure(AG_KEY_LEVELS + 2)))
     return st;
-  AG_HIP(hipMemsetAsync(c->d_lvlcnt.p, 0, (AG_KEY_LEVELS + 2) * sizeof(int), s));
-  int *d_total = c->d_counters.p + 16;
-  int mt       = 1;
-  for(int L = 0; L <= AG_KEY_LEVELS; L++)
-    c->lvlcnt_h[L] = L == 0 ? 1 : 0;  // a single particle: the root alone
   if(N > 1)
     {
-      k_binary<<<nblk(N - 1), B, 0, s>>>(c->d_key2.p, N, c->d_bfirst.p, c->d_blast.p, c->d_bprefix.p, c->d_bparent.p);
-      k_oct_count<<<nblk(N - 1), B, 0, s>>>(N, c->d_bprefix.p, c->d_bparent.p, geo.sbits, geo.levels, c->d_bcount.p,
-                                            c->d_lvlcnt.p);
-      AG_HIP(hipGetLastError());
-      if((st = exclusive_scan_i32(c->d_bcount.p, c->d_bbase.p, N - 1, d_total, c->d_temp.p, s)) != AGRAV_OK)
+      int *lvlpart = c->d_bcnt.p;  // nblk(N) rows of 22 partial counts
+      if((st = c->d_bcnt.ensure((size_t)nblk(N) * (AG_KEY_LEVELS + 1))) != AGRAV_OK)
         return st;
-      // the one host synchronisation of the octree: node count and cubes per level
-      AG_HIP(hipMemcpyAsync(&mt, d_total, sizeof(int), hipMemcpyDeviceToHost, s));
-      AG_HIP(hipMemcpyAsync(c->lvlcnt_h, c->d_lvlcnt.p, (AG_KEY_LEVELS + 1) * sizeof(int), hipMemcpyDeviceToHost, s));
-      AG_HIP(hipStreamSynchronize(s));
+      lvlpart = c->d_bcnt.p;
+      k_lvlmask<<<nblk(N), B, 0, s>>>(c->d_key2.p, N, geo.sbits, geo.levels, c->d_lvlmask.p, c->d_pcnt.p, lvlpart);
+      k_level_totals<<<AG_KEY_LEVELS + 1, B, 0, s>>>(lvlpart, (int)nblk(N), d_lvlcnt);
     }
+  else
+    k_single_root<<<1, 1, 0, s>>>(c->d_lvlmask.p, c->d_pcnt.p, d_lvlcnt);
+  AG_HIP(hipGetLastError());
+  if((st = exclusive_scan_i32(c->d_pcnt.p, c->d_pbase.p, N, d_total, c->d_temp.p, d_perr, s)) != AGRAV_OK)
+    return st;
+  // the one host synchronisation of the octree: node count and cubes per level
+  int hc[24];
+  AG_HIP(hipMemcpyAsync(hc, d_total, sizeof(hc), hipMemcpyDeviceToHost, s));
+  AG_HIP(hipStreamSynchronize(s));
+  for(int L = 0; L <= AG_KEY_LEVELS; L++)
+    c->lvlcnt_h[L] = hc[2 + L];
+  if(hc[1])
+    return fail(c, AGRAV_ERR_HIP, "sort/scan look-back did not complete");
+  const int mt = hc[0];
   // capacity check in the spirit of TreeAllocFactor (forcetree.c:186-228): the
   // exact count is known here, so the node arrays are sized to it.
   if(mt <= 0 || mt > 24 * N + 1)
     return fail(c, AGRAV_ERR_CAPACITY, "octree node count out of range");
   const int M = mt;
-  if((st = c->d_tfirst.ensure(M)) || (st = c->d_tlast.ensure(M)) || (st = c->d_tlevel.ensure(M)) ||
-     (st = c->d_nfirst.ensure(M)) || (st = c->d_nlast.ensure(M)) || (st = c->d_nlevel.ensure(M)) || (st = c->d_sx.ensure(M)) ||
+  if((st = c->d_nfirst.ensure(M)) || (st = c->d_nlast.ensure(M)) || (st = c->d_nlevel.ensure(M)) || (st = c->d_sx.ensure(M)) ||
      (st = c->d_sy.ensure(M)) || (st = c->d_sz.ensure(M)) || (st = c->d_nm.ensure(M)) || (st = c->d_cx.ensure(M)) ||
      (st = c->d_cy.ensure(M)) || (st = c->d_cz.ensure(M)) || (st = c->d_len.ensure(M)) || (st = c->d_nextnode.ensure(M)) ||
      (st = c->d_sibling.ensure(M)) || (st = c->d_maxsoft.ensure(M)) || (st = c->d_recA.ensure(M)) ||
-     (st = c->d_recB.ensure(M)) || (st = c->d_nrange.ensure(M)) || (st = c->d_cbeg.ensure(M)) || (st = c->d_ccnt.ensure(M)) || (st = c->d_cnn.ensure(M)) ||
-     (st = c->d_child.ensure((size_t)M + n)) || (st = c->d_lvlnodes.ensure(M)))
+     (st = c->d_recB.ensure(M)) || (st = c->d_nrange.ensure(M)) || (st = c->d_cbeg.ensure(M)) || (st = c->d_ccnt.ensure(M)) ||
+     (st = c->d_cnn.ensure(M)) || (st = c->d_child.ensure((size_t)M + n)) || (st = c->d_lvlnodes.ensure(M)))
     return st;
-  if(N > 1)
-    {
-      k_oct_emit<<<nblk(N - 1), B, 0, s>>>(N, c->d_bfirst.p, c->d_blast.p, c->d_bprefix.p, c->d_bparent.p, c->d_bcount.p,
-                                           c->d_bbase.p, geo.sbits, geo.levels, c->d_tfirst.p, c->d_tlast.p, c->d_tlevel.p);
-      k_lvlmask<<<nblk(N), B, 0, s>>>(c->d_key2.p, N, geo.sbits, geo.levels, c->d_lvlmask.p);
-    }
-  else
-    k_single_root<<<1, 1, 0, s>>>(c->d_tfirst.p, c->d_tlast.p, c->d_tlevel.p, c->d_lvlmask.p, c->d_lvlcnt.p);
-  k_first_count<<<nblk(N), B, 0, s>>>(c->d_lvlmask.p, N, c->d_pcnt.p);
-  AG_HIP(hipGetLastError());
-  if((st = exclusive_scan_i32(c->d_pcnt.p, c->d_pbase.p, N, nullptr, c->d_temp.p, s)) != AGRAV_OK)
-    return st;
-  k_place<<<nblk(M), B, 0, s>>>(M, N, c->d_tfirst.p, c->d_tlast.p, c->d_tlevel.p, c->d_lvlmask.p, c->d_pbase.p, c->d_px.p,
-                                c->d_py.p, c->d_pz.p, geo, c->d_nfirst.p, c->d_nlast.p, c->d_nlevel.p, c->d_cx.p, c->d_cy.p,
-                                c->d_cz.p, c->d_len.p, c->d_nextnode.p, c->d_sibling.p, c->d_nrange.p);
+  k_place<<<nblk(N), B, 0, s>>>(N, c->d_lvlmask.p, c->d_pbase.p, c->d_px.p, c->d_py.p, c->d_pz.p, geo, c->d_nfirst.p,
+                                c->d_nlevel.p, c->d_cx.p, c->d_cy.p, c->d_cz.p, c->d_len.p, c->d_nextnode.p);
+  k_cube_last<<<nblk(M), B, 0, s>>>(M, N, c->d_key2.p, c->d_nfirst.p, c->d_nlevel.p, c->d_lvlmask.p, c->d_pbase.p, geo.sbits,
+                                    c->d_nlast.p, c->d_nrange.p, c->d_sibling.p);
   k_pnext<<<nblk(N), B, 0, s>>>(N, c->d_lvlmask.p, c->d_pbase.p, c->d_pnext.p);
   if(nsets > 0)
     {
@@ -841,11 +970,13 @@ agrav_status build_tree(agrav_ctx *c)
 
   // 5. child lists, then moments level by level (deepest first); runs of small
   //    levels share one single-workgroup launch
+  if(scan_temp_bytes(M) + 4096 > c->d_temp.cap && (st = c->d_temp.ensure(scan_temp_bytes(M) + 4096)) != AGRAV_OK)
+    return st;
   int *d_berr = c->d_counters.p + 50;
   AG_HIP(hipMemsetAsync(d_berr, 0, sizeof(int), s));
   k_count_children<<<nblk(M), B, 0, s>>>(M, N, c->d_nextnode.p, c->d_sibling.p, c->d_pnext.p, c->d_ccnt.p, N + M, d_berr);
   AG_HIP(hipGetLastError());
-  if((st = exclusive_scan_i32(c->d_ccnt.p, c->d_cbeg.p, M, nullptr, c->d_temp.p, s)) != AGRAV_OK)
+  if((st = exclusive_scan_i32(c->d_ccnt.p, c->d_cbeg.p, M, nullptr, c->d_temp.p, d_perr, s)) != AGRAV_OK)
     return st;
   k_fill_children<<<nblk(M), B, 0, s>>>(M, N, c->d_nextnode.p, c->d_sibling.p, c->d_pnext.p, c->d_cbeg.p, c->d_ccnt.p,
                                          c->d_child.p, c->d_cnn.p);
@@ -856,17 +987,22 @@ agrav_status build_tree(agrav_ctx *c)
   if(off[AG_KEY_LEVELS + 1] != M)
     return fail(c, AGRAV_ERR_CAPACITY, "inconsistent octree level counts");
   AG_HIP(hipMemcpyAsync(c->d_lvloff.p, off, (AG_KEY_LEVELS + 2) * sizeof(int), hipMemcpyHostToDevice, s));
-  int *d_fill = c->d_counters.p + 24;  // 22 ints
+  int *d_fill = c->d_counters.p + 64;  // 22 ints
   AG_HIP(hipMemsetAsync(d_fill, 0, (AG_KEY_LEVELS + 1) * sizeof(int), s));
-  k_level_scatter<<<nblk(M), B, 0, s>>>(M, c->d_nlevel.p, c->d_lvloff.p, d_fill, c->d_lvlnodes.p);
+  k_level_scatter<<<(unsigned)((M + kScatterChunk - 1) / kScatterChunk), B, 0, s>>>(M, c->d_nlevel.p, c->d_lvloff.p, d_fill,
+                                                                                      c->d_lvlnodes.p);
   AG_HIP(hipGetLastError());
   const uint8_t *d_ps = multisoft ? c->d_ps.p : nullptr;
+  k_moments_direct<<<nblk(M), B, 0, s>>>(M, c->d_nrange.p, c->d_px.p, c->d_py.p, c->d_pz.p, c->d_pm.p, d_ps, c->d_soft.p,
+                                         c->soft.nsoft, c->d_cx.p, c->d_cy.p, c->d_cz.p, c->d_len.p, c->d_sx.p, c->d_sy.p,
+                                         c->d_sz.p, c->d_nm.p, c->d_maxsoft.p, c->d_recA.p, c->d_recB.p, geo.dom, geo.pset);
   const int small = 4096;
   for(int L = AG_KEY_LEVELS; L >= 0;)
     {
       if(c->lvlcnt_h[L] >= small)
         {
-          k_moments_level<<<nblk(c->lvlcnt_h[L]), B, 0, s>>>(c->d_lvlnodes.p + off[L], c->lvlcnt_h[L], N, c->d_cbeg.p, c->d_ccnt.p,
+          k_moments_level<<<nblk(c->lvlcnt_h[L]), B, 0, s>>>(c->d_lvlnodes.p + off[L], c->lvlcnt_h[L], c->d_nrange.p, N,
+                                                              c->d_cbeg.p, c->d_ccnt.p,
                                                               c->d_child.p, c->d_px.p, c->d_py.p, c->d_pz.p, c->d_pm.p, d_ps,
                                                               c->d_soft.p, c->soft.nsoft, c->d_cx.p, c->d_cy.p, c->d_cz.p,
                                                               c->d_len.p, c->d_sx.p, c->d_sy.p, c->d_sz.p, c->d_nm.p,
@@ -879,7 +1015,8 @@ agrav_status build_tree(agrav_ctx *c)
           int lo = L;
           while(lo - 1 >= 0 && c->lvlcnt_h[lo - 1] < small)
             lo--;
-          k_moments_levels_small<<<1, 1024, 0, s>>>(c->d_lvlnodes.p, c->d_lvloff.p, L, lo, N, c->d_cbeg.p, c->d_ccnt.p,
+          k_moments_levels_small<<<1, 1024, 0, s>>>(c->d_lvlnodes.p, c->d_lvloff.p, L, lo, c->d_nrange.p, N, c->d_cbeg.p,
+                                                    c->d_ccnt.p,
                                                     c->d_child.p, c->d_px.p, c->d_py.p, c->d_pz.p, c->d_pm.p, d_ps, c->d_soft.p,
                                                     c->soft.nsoft, c->d_cx.p, c->d_cy.p, c->d_cz.p, c->d_len.p, c->d_sx.p,
                                                     c->d_sy.p, c->d_sz.p, c->d_nm.p, c->d_maxsoft.p, c->d_recA.p, c->d_recB.p,
@@ -890,12 +1027,15 @@ agrav_status build_tree(agrav_ctx *c)
   AG_HIP(hipGetLastError());
 
   // out-of-box check (forcetree.c:432-438 aborts; here it is an error code)
-  int berr = 0;
+  int berr = 0, perr = 0;
+  AG_HIP(hipMemcpyAsync(&perr, d_perr, sizeof(int), hipMemcpyDeviceToHost, s));
   AG_HIP(hipMemcpyAsync(&c->h_dom, c->d_dom.p, sizeof(Domain), hipMemcpyDeviceToHost, s));
   AG_HIP(hipMemcpyAsync(&berr, d_berr, sizeof(int), hipMemcpyDeviceToHost, s));
   AG_HIP(hipStreamSynchronize(s));
   if(berr)
     return fail(c, AGRAV_ERR_CAPACITY, "octree threading inconsistent (moment pass stopped)");
+  if(perr)
+    return fail(c, AGRAV_ERR_HIP, "scan look-back did not complete");
   if(c->h_dom.out_of_box)
     return fail(c, AGRAV_ERR_OUT_OF_BOX, "particle outside the periodic box after one wrap");
   c->m         = M;

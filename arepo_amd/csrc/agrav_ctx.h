@@ -112,11 +112,10 @@ struct agrav_ctx
   agrav::DBuf<uint4> d_prec;
   agrav::DBuf<float> d_oldacc_s;
 
-  // binary radix tree and octree construction temporaries
-  agrav::DBuf<int> d_bfirst, d_blast, d_bprefix, d_bparent, d_bcount, d_bbase;
+  // octree construction: per particle, the levels of the cubes starting there
   agrav::DBuf<unsigned> d_lvlmask;
   agrav::DBuf<int> d_pcnt, d_pbase;
-  agrav::DBuf<int> d_tfirst, d_tlast, d_tlevel;
+  agrav::DBuf<int> d_bcnt;            // per-block partial counts
 
   // octree (pre-order)
   int m = 0;
@@ -127,7 +126,7 @@ struct agrav_ctx
   agrav::DBuf<uint4> d_recA, d_recB;
   agrav::DBuf<int2> d_nrange;
   agrav::DBuf<int> d_cbeg, d_ccnt, d_cnn, d_child;
-  agrav::DBuf<int> d_lvlcnt, d_lvloff, d_lvlnodes;
+  agrav::DBuf<int> d_lvloff, d_lvlnodes;
   int lvlcnt_h[AG_KEY_LEVELS + 2];
 
   // walk outputs in sorted order (G=1 units) and per-target counts

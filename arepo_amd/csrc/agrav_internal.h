@@ -119,14 +119,15 @@ struct BatchView
 agrav_status hip_fail(hipError_t e, const char *what, const char *file, int line);
 void set_error_msg(const std::string &m);
 
-// scan / sort primitives (agrav_prims.hip)
+// scan / sort primitives (agrav_prims.hip).  Both report a look-back that timed
+// out by setting *d_err (device int, cleared by the caller).
 size_t scan_temp_bytes(int64_t n);
-agrav_status exclusive_scan_i32(const int *d_in, int *d_out, int64_t n, int *d_total, void *d_temp, hipStream_t s);
+agrav_status exclusive_scan_i32(const int *d_in, int *d_out, int64_t n, int *d_total, void *d_temp, int *d_err, hipStream_t s);
 size_t sort_temp_bytes(int64_t n);
 // LSD radix sort of 64-bit keys with 32-bit values; bits [0, key_bits) are sorted.
 // Results end up in keys_out/vals_out.
 agrav_status radix_sort_u64(uint64_t *keys_in, int *vals_in, uint64_t *keys_out, int *vals_out, int64_t n, int key_bits,
-                            void *d_temp, hipStream_t s);
+                            void *d_temp, int *d_err, hipStream_t s);
 
 __host__ __device__ inline uint64_t hilbert_key21(uint32_t x, uint32_t y, uint32_t z)
 {

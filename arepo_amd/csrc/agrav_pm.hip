@@ -8,7 +8,7 @@
 //   r2c / c2r 3-D FFT  pm_mpi_fft.c:400-496     -> rocFFT out-of-place plans;
 //   Green's function   pm_periodic.c:1626-1707  -> FP64 table built once, applied in FP64;
 //   4-point FD         pm_periodic.c:1731-1800  -> k_fd4 (all three components at once);
-//   CIC readout        pm_periodic.c:1322-1550  -> k_cic_readout (potential + 3 forces).
+//   CIC readout        pm_periodic.c:1322-1550  -> k_cic_readout from one {F_x, F_y, F_z, phi} mesh.
 // FFTs are unnormalised like FFTW; 4 pi G / V and 1/(2d) are applied as in the reference.
 #include "agrav_ctx.h"
 
@@ -54,46 +54,12 @@ __device__ inline Cic cic_of(const double *p, int G, double L)
   return c;
 }
 
-__global__ void k_mass_partial(const double *__restrict__ m, int64_t n, double *part)
-{
-  __shared__ double s[B];
-  double v = 0;
-  for(int64_t i = (int64_t)blockIdx.x * B + threadIdx.x; i < n; i += (int64_t)gridDim.x * B)
-    v += m[i];
-  s[threadIdx.x] = v;
-  __syncthreads();
-  for(int o = B / 2; o > 0; o >>= 1)
-    {
-      if(threadIdx.x < o)
-        s[threadIdx.x] += s[threadIdx.x + o];
-      __syncthreads();
-    }
-  if(threadIdx.x == 0)
-    part[blockIdx.x] = s[0];
-}
-
-__global__ void k_mass_final(const double *__restrict__ part, int np, double *out)
-{
-  if(threadIdx.x == 0 && blockIdx.x == 0)
-    {
-      double v = 0;
-      for(int i = 0; i < np; i++)
-        v += part[i];
-      out[0] = v;
-      // fixed-point scale: a power of two leaving two bits of headroom below 2^63
-      int e;
-      frexp(v > 0 ? v : 1.0, &e);
-      out[1] = ldexp(1.0, 61 - e);
-    }
-}
-
 __global__ void k_cic_deposit(const double *__restrict__ pos, const double *__restrict__ mass, int64_t n, int G, double L,
-                              const double *__restrict__ scale_p, unsigned long long *rho)
+                              double scale, unsigned long long *rho)
 {
   int64_t i = (int64_t)blockIdx.x * B + threadIdx.x;
   if(i >= n)
     return;
-  const double scale = scale_p[1];
   Cic c              = cic_of(&pos[3 * i], G, L);
   double m           = mass[i] * scale;
   const double *d    = c.d;
@@ -119,8 +85,7 @@ __device__ inline int wrap_cell(int v, int G) { return v < 0 ? v + G : (v >= G ?
 
 __global__ __launch_bounds__(B) void k_cic_deposit_sorted(const double *__restrict__ px, const double *__restrict__ py,
                                                           const double *__restrict__ pz, const double *__restrict__ pm,
-                                                          int64_t n, int G, double L, const double *__restrict__ scale_p,
-                                                          unsigned long long *rho)
+                                                          int64_t n, int G, double L, double scale, unsigned long long *rho)
 {
   __shared__ unsigned long long s_tile[kTile];
   __shared__ int s_lo[3], s_hi[3];
@@ -163,7 +128,6 @@ __global__ __launch_bounds__(B) void k_cic_deposit_sorted(const double *__restri
   __syncthreads();
   const int ex = s_hi[0] - s_lo[0] + 2, ey = s_hi[1] - s_lo[1] + 2, ez = s_hi[2] - s_lo[2] + 2;
   const bool tiled = (int64_t)ex * ey * ez <= kTile && ex <= G && ey <= G && ez <= G;
-  const double scale = scale_p[1];
   const double m     = has ? pm[i] * scale : 0.0;
   const double *d    = c.d;
   const double w[8]  = {(1.0 - d[0]) * (1.0 - d[1]) * (1.0 - d[2]), (1.0 - d[0]) * (1.0 - d[1]) * (d[2]),
@@ -211,35 +175,48 @@ __global__ __launch_bounds__(B) void k_cic_deposit_sorted(const double *__restri
     }
 }
 
+// one 4-component mesh {F_x, F_y, F_z, phi} per cell: a particle's readout is 8 vector loads
+template <typename R> struct Mesh4
+{
+  R v[4];
+};
+
+template <typename R>
+__device__ inline void cic_gather4(const Cic &c, int G, const Mesh4<R> *__restrict__ mesh, double &f0, double &f1, double &f2,
+                                   double &pp)
+{
+  const double *d = c.d;
+  const int64_t idx[8] = {((int64_t)c.s[0] * G + c.s[1]) * G + c.s[2],   ((int64_t)c.s[0] * G + c.s[1]) * G + c.ss[2],
+                          ((int64_t)c.s[0] * G + c.ss[1]) * G + c.s[2],  ((int64_t)c.s[0] * G + c.ss[1]) * G + c.ss[2],
+                          ((int64_t)c.ss[0] * G + c.s[1]) * G + c.s[2],  ((int64_t)c.ss[0] * G + c.s[1]) * G + c.ss[2],
+                          ((int64_t)c.ss[0] * G + c.ss[1]) * G + c.s[2], ((int64_t)c.ss[0] * G + c.ss[1]) * G + c.ss[2]};
+  const double w[8] = {(1.0 - d[0]) * (1.0 - d[1]) * (1.0 - d[2]), (1.0 - d[0]) * (1.0 - d[1]) * (d[2]),
+                       (1.0 - d[0]) * (d[1]) * (1.0 - d[2]),       (1.0 - d[0]) * (d[1]) * (d[2]),
+                       (d[0]) * (1.0 - d[1]) * (1.0 - d[2]),       (d[0]) * (1.0 - d[1]) * (d[2]),
+                       (d[0]) * (d[1]) * (1.0 - d[2]),             (d[0]) * (d[1]) * (d[2])};
+  pp = f0 = f1 = f2 = 0;
+#pragma unroll
+  for(int q = 0; q < 8; q++)
+    {
+      const Mesh4<R> m = mesh[idx[q]];
+      f0 += (double)m.v[0] * w[q];
+      f1 += (double)m.v[1] * w[q];
+      f2 += (double)m.v[2] * w[q];
+      pp += (double)m.v[3] * w[q];
+    }
+}
+
 template <typename R>
 __global__ void k_cic_readout_sorted(const double *__restrict__ px, const double *__restrict__ py, const double *__restrict__ pz,
-                                     int64_t n, int G, double L, const R *__restrict__ phi, const R *__restrict__ force,
-                                     double fac_pot, double *acc_pm, double *pot_pm)
+                                     int64_t n, int G, double L, const Mesh4<R> *__restrict__ mesh, double fac_pot, double *acc_pm,
+                                     double *pot_pm)
 {
   int64_t i = (int64_t)blockIdx.x * B + threadIdx.x;
   if(i >= n)
     return;
-  double p[3]     = {px[i], py[i], pz[i]};
-  Cic c           = cic_of(p, G, L);
-  const double *d = c.d;
-  int64_t g3      = (int64_t)G * G * G;
-  int64_t idx[8]  = {((int64_t)c.s[0] * G + c.s[1]) * G + c.s[2],   ((int64_t)c.s[0] * G + c.s[1]) * G + c.ss[2],
-                     ((int64_t)c.s[0] * G + c.ss[1]) * G + c.s[2],  ((int64_t)c.s[0] * G + c.ss[1]) * G + c.ss[2],
-                     ((int64_t)c.ss[0] * G + c.s[1]) * G + c.s[2],  ((int64_t)c.ss[0] * G + c.s[1]) * G + c.ss[2],
-                     ((int64_t)c.ss[0] * G + c.ss[1]) * G + c.s[2], ((int64_t)c.ss[0] * G + c.ss[1]) * G + c.ss[2]};
-  double w[8] = {(1.0 - d[0]) * (1.0 - d[1]) * (1.0 - d[2]), (1.0 - d[0]) * (1.0 - d[1]) * (d[2]),
-                 (1.0 - d[0]) * (d[1]) * (1.0 - d[2]),       (1.0 - d[0]) * (d[1]) * (d[2]),
-                 (d[0]) * (1.0 - d[1]) * (1.0 - d[2]),       (d[0]) * (1.0 - d[1]) * (d[2]),
-                 (d[0]) * (d[1]) * (1.0 - d[2]),             (d[0]) * (d[1]) * (d[2])};
-  double pp = 0, f0 = 0, f1 = 0, f2 = 0;
-#pragma unroll
-  for(int q = 0; q < 8; q++)
-    {
-      pp += (double)phi[idx[q]] * w[q];
-      f0 += (double)force[idx[q]] * w[q];
-      f1 += (double)force[g3 + idx[q]] * w[q];
-      f2 += (double)force[2 * g3 + idx[q]] * w[q];
-    }
+  double p[3] = {px[i], py[i], pz[i]};
+  double f0, f1, f2, pp;
+  cic_gather4<R>(cic_of(p, G, L), G, mesh, f0, f1, f2, pp);
   pot_pm[i]         = pp * fac_pot;
   acc_pm[3 * i + 0] = f0;
   acc_pm[3 * i + 1] = f1;
@@ -247,11 +224,11 @@ __global__ void k_cic_readout_sorted(const double *__restrict__ px, const double
 }
 
 template <typename R>
-__global__ void k_fix_to_real(const unsigned long long *__restrict__ rho, int64_t g3, const double *__restrict__ scale_p, R *out)
+__global__ void k_fix_to_real(const unsigned long long *__restrict__ rho, int64_t g3, double scale, R *out)
 {
   int64_t i = (int64_t)blockIdx.x * B + threadIdx.x;
   if(i < g3)
-    out[i] = (R)((double)rho[i] / scale_p[1]);
+    out[i] = (R)((double)rho[i] / scale);
 }
 
 __global__ void k_green_table(int G, double L, double asmth, double *green)
@@ -304,7 +281,7 @@ template <typename R> __global__ void k_green_apply(R *cplx, const double *__res
   cplx[2 * i + 1] = (R)((double)cplx[2 * i + 1] * g);
 }
 
-template <typename R> __global__ void k_fd4(const R *__restrict__ phi, int G, double fac, R *force)
+template <typename R> __global__ void k_fd4(const R *__restrict__ phi, int G, double fac, Mesh4<R> *mesh)
 {
   // F = fac/(2d) [ 4/3 (phi_{-1} - phi_{+1}) - 1/6 (phi_{-2} - phi_{+2}) ]   (fac includes 1/(2d))
   int64_t i  = (int64_t)blockIdx.x * B + threadIdx.x;
@@ -316,39 +293,24 @@ template <typename R> __global__ void k_fd4(const R *__restrict__ phi, int G, do
   int xl = (x - 1 + G) % G, xr = (x + 1) % G, xll = (x - 2 + G) % G, xrr = (x + 2) % G;
   int yl = (y - 1 + G) % G, yr = (y + 1) % G, yll = (y - 2 + G) % G, yrr = (y + 2) % G;
   int zl = (z - 1 + G) % G, zr = (z + 1) % G, zll = (z - 2 + G) % G, zrr = (z + 2) % G;
-  force[i]          = (R)(fac * ((4.0 / 3) * (at(xl, y, z) - at(xr, y, z)) - (1.0 / 6) * (at(xll, y, z) - at(xrr, y, z))));
-  force[g3 + i]     = (R)(fac * ((4.0 / 3) * (at(x, yl, z) - at(x, yr, z)) - (1.0 / 6) * (at(x, yll, z) - at(x, yrr, z))));
-  force[2 * g3 + i] = (R)(fac * ((4.0 / 3) * (at(x, y, zl) - at(x, y, zr)) - (1.0 / 6) * (at(x, y, zll) - at(x, y, zrr))));
+  Mesh4<R> m;
+  m.v[0]  = (R)(fac * ((4.0 / 3) * (at(xl, y, z) - at(xr, y, z)) - (1.0 / 6) * (at(xll, y, z) - at(xrr, y, z))));
+  m.v[1]  = (R)(fac * ((4.0 / 3) * (at(x, yl, z) - at(x, yr, z)) - (1.0 / 6) * (at(x, yll, z) - at(x, yrr, z))));
+  m.v[2]  = (R)(fac * ((4.0 / 3) * (at(x, y, zl) - at(x, y, zr)) - (1.0 / 6) * (at(x, y, zll) - at(x, y, zrr))));
+  m.v[3]  = phi[i];
+  mesh[i] = m;
 }
 
 template <typename R>
-__global__ void k_cic_readout(const double *__restrict__ pos, int64_t n, int G, double L, const R *__restrict__ phi,
-                              const R *__restrict__ force, double fac_pot, double *acc_pm, double *pot_pm)
+__global__ void k_cic_readout(const double *__restrict__ pos, int64_t n, int G, double L, const Mesh4<R> *__restrict__ mesh,
+                              double fac_pot, double *acc_pm, double *pot_pm)
 {
   int64_t i = (int64_t)blockIdx.x * B + threadIdx.x;
   if(i >= n)
     return;
-  Cic c           = cic_of(&pos[3 * i], G, L);
-  const double *d = c.d;
-  int64_t g3      = (int64_t)G * G * G;
-  int64_t idx[8]  = {((int64_t)c.s[0] * G + c.s[1]) * G + c.s[2],   ((int64_t)c.s[0] * G + c.s[1]) * G + c.ss[2],
-                     ((int64_t)c.s[0] * G + c.ss[1]) * G + c.s[2],  ((int64_t)c.s[0] * G + c.ss[1]) * G + c.ss[2],
-                     ((int64_t)c.ss[0] * G + c.s[1]) * G + c.s[2],  ((int64_t)c.ss[0] * G + c.s[1]) * G + c.ss[2],
-                     ((int64_t)c.ss[0] * G + c.ss[1]) * G + c.s[2], ((int64_t)c.ss[0] * G + c.ss[1]) * G + c.ss[2]};
-  double w[8] = {(1.0 - d[0]) * (1.0 - d[1]) * (1.0 - d[2]), (1.0 - d[0]) * (1.0 - d[1]) * (d[2]),
-                 (1.0 - d[0]) * (d[1]) * (1.0 - d[2]),       (1.0 - d[0]) * (d[1]) * (d[2]),
-                 (d[0]) * (1.0 - d[1]) * (1.0 - d[2]),       (d[0]) * (1.0 - d[1]) * (d[2]),
-                 (d[0]) * (d[1]) * (1.0 - d[2]),             (d[0]) * (d[1]) * (d[2])};
-  double p = 0, f0 = 0, f1 = 0, f2 = 0;
-#pragma unroll
-  for(int q = 0; q < 8; q++)
-    {
-      p += (double)phi[idx[q]] * w[q];
-      f0 += (double)force[idx[q]] * w[q];
-      f1 += (double)force[g3 + idx[q]] * w[q];
-      f2 += (double)force[2 * g3 + idx[q]] * w[q];
-    }
-  pot_pm[i]         = p * fac_pot;
+  double f0, f1, f2, pp;
+  cic_gather4<R>(cic_of(&pos[3 * i], G, L), G, mesh, f0, f1, f2, pp);
+  pot_pm[i]         = pp * fac_pot;
   acc_pm[3 * i + 0] = f0;
   acc_pm[3 * i + 1] = f1;
   acc_pm[3 * i + 2] = f2;
@@ -420,7 +382,7 @@ agrav_status pm_init(agrav_ctx *c)
   rocfft_plan_get_work_buffer_size(p.bwd, &w2);
   size_t w = w1 > w2 ? w1 : w2;
   if((st = p.work.ensure(w ? w : 1)) || (st = p.rho_fix.ensure(g3)) || (st = p.real.ensure(g3 * rsz)) ||
-     (st = p.cplx.ensure(nc * 2 * rsz)) || (st = p.force.ensure(3 * g3 * rsz)) || (st = p.green.ensure(nc)))
+     (st = p.cplx.ensure(nc * 2 * rsz)) || (st = p.force.ensure(4 * g3 * rsz)) || (st = p.green.ensure(nc)))
     return st;
   if((st = fft_check(c, rocfft_execution_info_create(&p.info_f), "info")) ||
      (st = fft_check(c, rocfft_execution_info_create(&p.info_b), "info")))
@@ -450,20 +412,22 @@ template <typename R> static agrav_status pm_run(agrav_ctx *c)
   const int64_t nc = (int64_t)G * G * (G / 2 + 1);
   hipStream_t s   = c->stream;
   agrav_status st;
-  if((st = c->d_acc_pm.ensure(3 * (n ? n : 1))) || (st = c->d_pot_pm.ensure(n ? n : 1)) || (st = c->d_temp.ensure(4096 * 8)))
+  if((st = c->d_acc_pm.ensure(3 * (n ? n : 1))) || (st = c->d_pot_pm.ensure(n ? n : 1)))
     return st;
-  double *part = (double *)c->d_temp.p;  // 256 partials, then [total, scale] at part+512
-  k_mass_partial<<<256, B, 0, s>>>(c->d_mass_in.p, n, part);
-  k_mass_final<<<1, 1, 0, s>>>(part, 256, part + 512);
+  // fixed-point scale of the mass mesh: a power of two leaving two bits of headroom
+  // below 2^63 for the total mass (kept on the host since the particle load)
+  int e = 0;
+  frexp(c->mass_total > 0 ? c->mass_total : 1.0, &e);
+  const double scale = ldexp(1.0, 61 - e);
   AG_HIP(hipMemsetAsync(p.rho_fix.p, 0, g3 * sizeof(unsigned long long), s));
   // with a current tree the particles are available in Hilbert order: tiled deposit and
   // coherent readout, results kept in sorted order (the finalize kernel maps them back)
   const bool sorted = c->have_tree;
   if(n > 0 && sorted)
-    k_cic_deposit_sorted<<<nb(n), B, 0, s>>>(c->d_px.p, c->d_py.p, c->d_pz.p, c->d_pm.p, n, G, L, part + 512, p.rho_fix.p);
+    k_cic_deposit_sorted<<<nb(n), B, 0, s>>>(c->d_px.p, c->d_py.p, c->d_pz.p, c->d_pm.p, n, G, L, scale, p.rho_fix.p);
   else if(n > 0)
-    k_cic_deposit<<<nb(n), B, 0, s>>>(c->d_pos_in.p, c->d_mass_in.p, n, G, L, part + 512, p.rho_fix.p);
-  k_fix_to_real<R><<<nb(g3), B, 0, s>>>(p.rho_fix.p, g3, part + 512, (R *)p.real.p);
+    k_cic_deposit<<<nb(n), B, 0, s>>>(c->d_pos_in.p, c->d_mass_in.p, n, G, L, scale, p.rho_fix.p);
+  k_fix_to_real<R><<<nb(g3), B, 0, s>>>(p.rho_fix.p, g3, scale, (R *)p.real.p);
   AG_HIP(hipGetLastError());
   void *in[1]  = {p.real.p};
   void *out[1] = {p.cplx.p};
@@ -477,13 +441,13 @@ template <typename R> static agrav_status pm_run(agrav_ctx *c)
     return st;
   const double fac_pot = 4 * M_PI * c->par.G / (L * L * L);  // pm_periodic.c:1590
   const double fac_fd  = fac_pot / (2 * (L / G));            // :1592
-  k_fd4<R><<<nb(g3), B, 0, s>>>((const R *)p.real.p, G, fac_fd, (R *)p.force.p);
+  Mesh4<R> *mesh = reinterpret_cast<Mesh4<R> *>(p.force.p);
+  k_fd4<R><<<nb(g3), B, 0, s>>>((const R *)p.real.p, G, fac_fd, mesh);
   if(n > 0 && sorted)
-    k_cic_readout_sorted<R><<<nb(n), B, 0, s>>>(c->d_px.p, c->d_py.p, c->d_pz.p, n, G, L, (const R *)p.real.p,
-                                                (const R *)p.force.p, fac_pot, c->d_acc_pm.p, c->d_pot_pm.p);
+    k_cic_readout_sorted<R><<<nb(n), B, 0, s>>>(c->d_px.p, c->d_py.p, c->d_pz.p, n, G, L, mesh, fac_pot, c->d_acc_pm.p,
+                                                c->d_pot_pm.p);
   else if(n > 0)
-    k_cic_readout<R><<<nb(n), B, 0, s>>>(c->d_pos_in.p, n, G, L, (const R *)p.real.p, (const R *)p.force.p, fac_pot,
-                                         c->d_acc_pm.p, c->d_pot_pm.p);
+    k_cic_readout<R><<<nb(n), B, 0, s>>>(c->d_pos_in.p, n, G, L, mesh, fac_pot, c->d_acc_pm.p, c->d_pot_pm.p);
   AG_HIP(hipGetLastError());
   c->pm_sorted = sorted;
   return AGRAV_OK;

@@ -4,13 +4,23 @@
 // serial scans with cross-lane shuffles; the radix sort ranks keys stably
 // inside a wavefront with 8 ballots (one per digit bit) and per-wave digit
 // counters in LDS, so no key is ever compared against another lane's key.
+//
+// Both are single-pass over their data: tiles take tickets in order and find
+// the sum of all earlier tiles by decoupled look-back (Merrill and Garland,
+// "Single-pass parallel prefix scan with decoupled look-back", NVR-2016-002):
+// each tile publishes its own total, then walks back over its predecessors'
+// published states until it meets an inclusive prefix.  A state is one 64-bit
+// word {2-bit flag, 62-bit value} read and written with agent-scope atomics, so
+// it is coherent across the XCDs.  A tile only waits on tiles with earlier
+// tickets, which are already running; every wait is bounded and reports an
+// error instead of spinning on.
 #include "agrav_internal.h"
 
 namespace agrav
 {
 
 static constexpr int kScanBlock = 256;
-static constexpr int kScanItems = 8;
+static constexpr int kScanItems = 16;
 static constexpr int kScanTile  = kScanBlock * kScanItems;
 
 __device__ inline int wave_incl_scan(int v)
@@ -59,55 +69,155 @@ __device__ inline int block_excl_scan_tile(int vals[kScanItems], int *s_wave)
   return total;
 }
 
-__global__ __launch_bounds__(kScanBlock) void k_tile_sums(const int *__restrict__ in, int64_t n, int *__restrict__ sums)
+static constexpr uint64_t kFlagAgg  = 1ull << 62;
+static constexpr uint64_t kFlagIncl = 2ull << 62;
+static constexpr uint64_t kValMask  = (1ull << 62) - 1;
+static constexpr int kSpinLimit     = 1 << 22;
+
+// 32-bit states {2-bit flag, 30-bit value} for the sort, whose counts stay below 2^30
+static constexpr uint32_t kFlagAgg32  = 1u << 30;
+static constexpr uint32_t kFlagIncl32 = 2u << 30;
+static constexpr uint32_t kValMask32  = (1u << 30) - 1;
+
+template <typename W> __device__ inline W lb_load(const W *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+template <typename W> __device__ inline void lb_store(W *p, W v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+
+// Sum of the values of tiles 0 .. t-1 from their states (state of tile q at
+// state[q * stride]); the caller has published its own aggregate.  WIN states
+// are read at once (independent loads, one memory round trip), then summed in
+// order up to the nearest inclusive prefix; a window with a state not yet
+// published is read again.
+template <typename W, int WIN>
+__device__ inline int64_t look_back(const W *state, int64_t stride, int t, int *err)
 {
-  __shared__ int s_wave[kScanBlock / 64];
-  int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
-  int vals[kScanItems];
+  constexpr int sh  = 8 * sizeof(W) - 2;
+  constexpr W vmask = (W(1) << sh) - 1;
+  int64_t excl      = 0;
+  int spins         = 0;
+  for(int q = t - 1; q >= 0;)
+    {
+      W v[WIN];
 #pragma unroll
-  for(int i = 0; i < kScanItems; i++)
-    vals[i] = (base + i < n) ? in[base + i] : 0;
-  int total = block_excl_scan_tile(vals, s_wave);
-  if(threadIdx.x == 0)
-    sums[blockIdx.x] = total;
+      for(int k = 0; k < WIN; k++)
+        v[k] = q - k >= 0 ? lb_load(&state[(int64_t)(q - k) * stride]) : (W(2) << sh);  // before tile 0: prefix 0
+      bool ready = true, incl = false;
+      int64_t sum = 0;
+#pragma unroll
+      for(int k = 0; k < WIN; k++)
+        if(ready && !incl)
+          {
+            const W f = v[k] >> sh;
+            if(f == 0)
+              ready = false;
+            else
+              {
+                sum += (int64_t)(v[k] & vmask);
+                incl = f == 2;
+              }
+          }
+      if(!ready)
+        {
+          if(++spins > kSpinLimit)
+            {
+              atomicExch(err, 1);  // a predecessor never published: give up (error, not a hang)
+              break;
+            }
+          __builtin_amdgcn_s_sleep(1);
+          continue;
+        }
+      excl += sum;
+      if(incl)
+        break;
+      q -= WIN;
+    }
+  return excl;
 }
 
-__global__ __launch_bounds__(kScanBlock) void k_tile_scan(const int *__restrict__ in, int *__restrict__ out, int64_t n,
-                                                          const int *__restrict__ offsets, int *__restrict__ total_out)
+// The same look-back by a whole wavefront: lane k reads the state of tile q-k, so
+// one round trip covers 64 predecessors.  All lanes return the sum.
+__device__ inline int64_t look_back_wave(const uint64_t *state, int t, int *err)
+{
+  const int lane = threadIdx.x & 63;
+  int64_t excl   = 0;
+  int spins      = 0;
+  for(int q = t - 1; q >= 0;)
+    {
+      const int qq     = q - lane;
+      const uint64_t v = qq >= 0 ? lb_load(&state[qq]) : kFlagIncl;  // before tile 0: prefix 0
+      const uint64_t f = v >> 62;
+      const uint64_t incl_m  = __ballot(f == 2);
+      const uint64_t unready = __ballot(f == 0);
+      const int k_incl       = incl_m ? __ffsll((long long)incl_m) - 1 : 63;  // nearest inclusive prefix
+      const uint64_t need    = k_incl == 63 ? ~0ull : ((2ull << k_incl) - 1);
+      if(unready & need)
+        {
+          if(++spins > kSpinLimit)
+            {
+              if(lane == 0)
+                atomicExch(err, 1);  // a predecessor never published: give up (error, not a hang)
+              break;
+            }
+          __builtin_amdgcn_s_sleep(1);
+          continue;
+        }
+      int64_t part = lane <= k_incl ? (int64_t)(v & kValMask) : 0;
+      for(int o = 32; o > 0; o >>= 1)
+        part += __shfl_xor(part, o, 64);
+      excl += part;
+      if(incl_m)
+        break;
+      q -= 64;
+    }
+  return excl;
+}
+
+// single-pass exclusive scan of int32, 2048 items per tile
+__global__ __launch_bounds__(kScanBlock) void k_scan_lb(const int *__restrict__ in, int *__restrict__ out, int64_t n, int ntiles,
+                                                        int *__restrict__ total_out, uint64_t *state, int *ticket, int *err)
 {
   __shared__ int s_wave[kScanBlock / 64];
-  int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+  __shared__ int s_t;
+  __shared__ int64_t s_excl;
+  if(threadIdx.x == 0)
+    s_t = atomicAdd(ticket, 1);
+  __syncthreads();
+  const int t        = s_t;
+  const int64_t base = (int64_t)t * kScanTile + (int64_t)threadIdx.x * kScanItems;
   int vals[kScanItems];
 #pragma unroll
   for(int i = 0; i < kScanItems; i++)
     vals[i] = (base + i < n) ? in[base + i] : 0;
-  int last_in = (n > 0) ? 0 : 0;
-  (void)last_in;
-  int total = block_excl_scan_tile(vals, s_wave);
-  int off   = offsets ? offsets[blockIdx.x] : 0;
+  const int total = block_excl_scan_tile(vals, s_wave);
+  if(threadIdx.x < 64)
+    {
+      int64_t excl = 0;
+      if(t > 0)
+        {
+          if(threadIdx.x == 0)
+            lb_store(&state[t], kFlagAgg | (uint64_t)(uint32_t)total);
+          excl = look_back_wave(state, t, err);
+        }
+      if(threadIdx.x == 0)
+        {
+          lb_store(&state[t], kFlagIncl | ((uint64_t)(excl + total) & kValMask));
+          s_excl = excl;
+        }
+    }
+  __syncthreads();
+  const int off = (int)s_excl;
 #pragma unroll
   for(int i = 0; i < kScanItems; i++)
     if(base + i < n)
       out[base + i] = vals[i] + off;
-  if(total_out && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0)
+  if(total_out && t == ntiles - 1 && threadIdx.x == 0)
     *total_out = off + total;
 }
 
 static int64_t ntiles_of(int64_t n) { return (n + kScanTile - 1) / kScanTile; }
 
-size_t scan_temp_bytes(int64_t n)
-{
-  size_t bytes = 0;
-  int64_t t    = ntiles_of(n);
-  while(t > 1)
-    {
-      bytes += 2 * (size_t)t * sizeof(int) + 256;
-      t = ntiles_of(t);
-    }
-  return bytes + 256;
-}
+size_t scan_temp_bytes(int64_t n) { return (size_t)(ntiles_of(n) + 1) * sizeof(uint64_t) + 256; }
 
-agrav_status exclusive_scan_i32(const int *d_in, int *d_out, int64_t n, int *d_total, void *d_temp, hipStream_t s)
+agrav_status exclusive_scan_i32(const int *d_in, int *d_out, int64_t n, int *d_total, void *d_temp, int *d_err, hipStream_t s)
 {
   if(n <= 0)
     {
@@ -115,23 +225,11 @@ agrav_status exclusive_scan_i32(const int *d_in, int *d_out, int64_t n, int *d_t
         AG_HIP(hipMemsetAsync(d_total, 0, sizeof(int), s));
       return AGRAV_OK;
     }
-  int64_t nt = ntiles_of(n);
-  if(nt == 1)
-    {
-      k_tile_scan<<<1, kScanBlock, 0, s>>>(d_in, d_out, n, nullptr, d_total);
-      AG_HIP(hipGetLastError());
-      return AGRAV_OK;
-    }
-  int *sums    = (int *)d_temp;
-  int *offsets = sums + nt;
-  char *rest   = (char *)(offsets + nt);
-  rest         = (char *)(((uintptr_t)rest + 255) & ~(uintptr_t)255);
-  k_tile_sums<<<(unsigned)nt, kScanBlock, 0, s>>>(d_in, n, sums);
-  AG_HIP(hipGetLastError());
-  agrav_status st = exclusive_scan_i32(sums, offsets, nt, nullptr, rest, s);
-  if(st != AGRAV_OK)
-    return st;
-  k_tile_scan<<<(unsigned)nt, kScanBlock, 0, s>>>(d_in, d_out, n, offsets, d_total);
+  const int64_t nt = ntiles_of(n);
+  uint64_t *state  = (uint64_t *)d_temp;
+  int *ticket      = (int *)(state + nt);
+  AG_HIP(hipMemsetAsync(d_temp, 0, (size_t)nt * sizeof(uint64_t) + sizeof(uint64_t), s));
+  k_scan_lb<<<(unsigned)nt, kScanBlock, 0, s>>>(d_in, d_out, n, (int)nt, d_total, state, ticket, d_err);
   AG_HIP(hipGetLastError());
   return AGRAV_OK;
 }
@@ -142,47 +240,99 @@ agrav_status exclusive_scan_i32(const int *d_in, int *d_out, int64_t n, int *d_t
 
 static constexpr int kSortBlock = 256;
 static constexpr int kSortWaves = kSortBlock / 64;
-static constexpr int kSortRounds = 8;  // keys per lane
+#ifndef AG_SORT_ROUNDS
+#define AG_SORT_ROUNDS 16
+#endif
+static constexpr int kSortRounds = AG_SORT_ROUNDS;  // keys per lane
 static constexpr int kSortTile  = kSortBlock * kSortRounds;
 static_assert(kSortBlock == 256, "the scatter gives each thread one of the 256 digits");
 
-__global__ __launch_bounds__(kSortBlock) void k_radix_hist(const uint64_t *__restrict__ keys, int64_t n, int shift,
-                                                           int ntiles, int *__restrict__ hist)
+// digit histograms of all passes at once: per-block partials, one row per block
+static constexpr int kHistBlocks = 256;
+
+__global__ __launch_bounds__(kSortBlock) void k_radix_hist_all(const uint64_t *__restrict__ keys, int64_t n, int passes,
+                                                               int *__restrict__ part)
 {
-  __shared__ int s_h[256];
-  s_h[threadIdx.x] = 0;
+  __shared__ int s_h[8][256];
+  for(int q = threadIdx.x; q < 8 * 256; q += kSortBlock)
+    s_h[q >> 8][q & 255] = 0;
   __syncthreads();
-  int64_t base = (int64_t)blockIdx.x * kSortTile;
-  for(int i = threadIdx.x; i < kSortTile; i += kSortBlock)
+  for(int64_t k = (int64_t)blockIdx.x * kSortBlock + threadIdx.x; k < n; k += (int64_t)gridDim.x * kSortBlock)
     {
-      int64_t k = base + i;
-      if(k < n)
-        atomicAdd(&s_h[(keys[k] >> shift) & 255], 1);
+      const uint64_t key = keys[k];
+      for(int p = 0; p < passes; p++)
+        atomicAdd(&s_h[p][(key >> (8 * p)) & 255], 1);
     }
   __syncthreads();
-  hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = s_h[threadIdx.x];
+  for(int q = threadIdx.x; q < passes * 256; q += kSortBlock)
+    part[(int64_t)blockIdx.x * 2048 + q] = s_h[q >> 8][q & 255];
 }
 
-// One tile of 2048 keys: rank stably by the current digit, stage the tile in LDS
-// in digit order, then write each digit's run to its global position with
-// consecutive lanes on consecutive addresses (coalesced stores).
-__global__ __launch_bounds__(kSortBlock) void k_radix_scatter(const uint64_t *__restrict__ kin, const int *__restrict__ vin,
-                                                              uint64_t *__restrict__ kout, int *__restrict__ vout, int64_t n,
-                                                              int shift, int ntiles, const int *__restrict__ offs)
+// start of every digit in the output of every pass: block p sums the partials of
+// pass p (each wave a quarter of the rows, 8 loads in flight per lane) and scans
+// its 256 digit counts
+__global__ __launch_bounds__(kSortBlock) void k_radix_digit_base(const int *__restrict__ part, int nblocks, int *__restrict__ base)
+{
+  __shared__ int s_part[kSortWaves][256];
+  __shared__ int s_wave[kSortWaves];
+  const int p = blockIdx.x, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for(int d = lane; d < 256; d += 64)
+    {
+      int c = 0;
+      for(int b0 = w * 8; b0 < nblocks; b0 += kSortWaves * 8)
+        {
+          int v[8];
+#pragma unroll
+          for(int u = 0; u < 8; u++)
+            v[u] = b0 + u < nblocks ? part[(int64_t)(b0 + u) * 2048 + p * 256 + d] : 0;
+#pragma unroll
+          for(int u = 0; u < 8; u++)
+            c += v[u];
+        }
+      s_part[w][d] = c;
+    }
+  __syncthreads();
+  const int d = threadIdx.x;
+  int c       = 0;
+#pragma unroll
+  for(int ww = 0; ww < kSortWaves; ww++)
+    c += s_part[ww][d];
+  const int incl = wave_incl_scan(c);
+  if(lane == 63)
+    s_wave[w] = incl;
+  __syncthreads();
+  int wpre = 0;
+  for(int ww = 0; ww < w; ww++)
+    wpre += s_wave[ww];
+  base[p * 256 + d] = wpre + incl - c;
+}
+
+// One pass of the sort over one tile of 2048 keys: rank stably by the digit,
+// publish the tile's digit counts and look back for the counts of all earlier
+// tiles, stage the tile in LDS in digit order, then write each digit's run to
+// its global position with consecutive lanes on consecutive addresses.
+__global__ __launch_bounds__(kSortBlock) void k_radix_onesweep(const uint64_t *__restrict__ kin, const int *__restrict__ vin,
+                                                               uint64_t *__restrict__ kout, int *__restrict__ vout, int64_t n,
+                                                               int shift, const int *__restrict__ digit_base, uint32_t *state,
+                                                               int *ticket, int *err)
 {
   __shared__ uint64_t s_key[kSortTile];
   __shared__ int s_val[kSortTile];
   __shared__ int s_cnt[kSortWaves][256];
   __shared__ int s_dbase[256], s_gbase[256], s_wsum[kSortWaves];
+  __shared__ int s_t;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if(threadIdx.x == 0)
+    s_t = atomicAdd(ticket, 1);
   for(int i = lane; i < 256; i += 64)
     s_cnt[w][i] = 0;
-  __builtin_amdgcn_wave_barrier();
+  __syncthreads();
+  const int t = s_t;
 
-  const uint64_t lt    = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  const int64_t tbase  = (int64_t)blockIdx.x * kSortTile;
-  const int tcount     = (int)(n - tbase < kSortTile ? n - tbase : kSortTile);
-  const int64_t wbase  = tbase + (int64_t)w * (64 * kSortRounds);
+  const uint64_t lt   = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const int64_t tbase = (int64_t)t * kSortTile;
+  const int tcount    = (int)(n - tbase < kSortTile ? n - tbase : kSortTile);
+  const int64_t wbase = tbase + (int64_t)w * (64 * kSortRounds);
   uint64_t key[kSortRounds];
   int val[kSortRounds];
   int pos[kSortRounds];
@@ -216,9 +366,10 @@ __global__ __launch_bounds__(kSortBlock) void k_radix_scatter(const uint64_t *__
     }
   __syncthreads();
 
-  // 2. per digit: wave offsets, tile-local start (scan over digits), global start
+  // 2. per digit (one per thread): wave offsets, the tile's count published, the
+  //    earlier tiles' counts by look-back, the tile-local start by a scan over digits
   {
-    const int d = threadIdx.x;  // kSortBlock == 256 digits
+    const int d = threadIdx.x;
     int run     = 0;
 #pragma unroll
     for(int ww = 0; ww < kSortWaves; ww++)
@@ -227,6 +378,14 @@ __global__ __launch_bounds__(kSortBlock) void k_radix_scatter(const uint64_t *__
         s_cnt[ww][d] = run;
         run += c;
       }
+    int64_t excl = 0;
+    if(t > 0)
+      {
+        lb_store(&state[(int64_t)t * 256 + d], kFlagAgg32 | (uint32_t)run);
+        excl = look_back<uint32_t, 8>(state + d, 256, t, err);
+      }
+    lb_store(&state[(int64_t)t * 256 + d], kFlagIncl32 | ((uint32_t)(excl + run) & kValMask32));
+    s_gbase[d] = digit_base[d] + (int)excl;
     const int incl = wave_incl_scan(run);
     if(lane == 63)
       s_wsum[w] = incl;
@@ -236,7 +395,6 @@ __global__ __launch_bounds__(kSortBlock) void k_radix_scatter(const uint64_t *__
     for(int ww = 0; ww < kSortWaves; ww++)
       wpre += ww < w ? s_wsum[ww] : 0;
     s_dbase[d] = wpre + incl - run;
-    s_gbase[d] = offs[(int64_t)d * ntiles + blockIdx.x];
   }
   __syncthreads();
 
@@ -255,43 +413,44 @@ __global__ __launch_bounds__(kSortBlock) void k_radix_scatter(const uint64_t *__
   // 4. runs of equal digits go out to consecutive global addresses
   for(int i = threadIdx.x; i < tcount; i += kSortBlock)
     {
-      const uint64_t k = s_key[i];
-      const int d      = (int)((k >> shift) & 255);
+      const uint64_t k  = s_key[i];
+      const int d       = (int)((k >> shift) & 255);
       const int64_t dst = (int64_t)s_gbase[d] + (i - s_dbase[d]);
-      kout[dst]        = k;
-      vout[dst]        = s_val[i];
+      kout[dst]         = k;
+      vout[dst]         = s_val[i];
     }
 }
 
 size_t sort_temp_bytes(int64_t n)
 {
-  int64_t nt = (n + kSortTile - 1) / kSortTile;
-  size_t h   = (size_t)256 * nt * sizeof(int);
-  return 2 * h + scan_temp_bytes(256 * nt) + 1024;
+  const int64_t nt = (n + kSortTile - 1) / kSortTile;
+  // per pass: look-back states (nt x 256 words) + ticket; digit partials and bases
+  return (size_t)8 * (nt * 256 + 1) * sizeof(uint32_t) + (size_t)kHistBlocks * 2048 * sizeof(int) + 8 * 256 * sizeof(int) + 1024;
 }
 
 agrav_status radix_sort_u64(uint64_t *keys_in, int *vals_in, uint64_t *keys_out, int *vals_out, int64_t n, int key_bits,
-                            void *d_temp, hipStream_t s)
+                            void *d_temp, int *d_err, hipStream_t s)
 {
   if(n <= 0)
     return AGRAV_OK;
-  int ntiles   = (int)((n + kSortTile - 1) / kSortTile);
-  int *hist    = (int *)d_temp;
-  int *offs    = hist + (size_t)256 * ntiles;
-  char *stemp  = (char *)(offs + (size_t)256 * ntiles);
-  stemp        = (char *)(((uintptr_t)stemp + 255) & ~(uintptr_t)255);
-  int passes   = (key_bits + 7) / 8;
+  const int64_t nt  = (n + kSortTile - 1) / kSortTile;
+  const int passes  = (key_bits + 7) / 8;
+  uint32_t *states  = (uint32_t *)d_temp;                     // passes x (nt x 256 + 1)
+  const int64_t per = nt * 256 + 1;
+  int *part         = (int *)(states + (size_t)passes * per);  // kHistBlocks x 2048
+  int *dbase        = part + (size_t)kHistBlocks * 2048;       // passes x 256
+  AG_HIP(hipMemsetAsync(states, 0, (size_t)passes * per * sizeof(uint32_t), s));
+  const int hb = (int)(nt < kHistBlocks ? nt : kHistBlocks);
+  k_radix_hist_all<<<hb, kSortBlock, 0, s>>>(keys_in, n, passes, part);
+  k_radix_digit_base<<<passes, kSortBlock, 0, s>>>(part, hb, dbase);
+  AG_HIP(hipGetLastError());
   uint64_t *ka = keys_in, *kb = keys_out;
   int *va = vals_in, *vb = vals_out;
   for(int p = 0; p < passes; p++)
     {
-      int shift = 8 * p;
-      k_radix_hist<<<ntiles, kSortBlock, 0, s>>>(ka, n, shift, ntiles, hist);
-      AG_HIP(hipGetLastError());
-      agrav_status st = exclusive_scan_i32(hist, offs, (int64_t)256 * ntiles, nullptr, stemp, s);
-      if(st != AGRAV_OK)
-        return st;
-      k_radix_scatter<<<ntiles, kSortBlock, 0, s>>>(ka, va, kb, vb, n, shift, ntiles, offs);
+      uint32_t *st = states + (size_t)p * per;
+      int *ticket  = (int *)(st + nt * 256);
+      k_radix_onesweep<<<(unsigned)nt, kSortBlock, 0, s>>>(ka, va, kb, vb, n, 8 * p, dbase + 256 * p, st, ticket, d_err);
       AG_HIP(hipGetLastError());
       uint64_t *tk = ka;
       ka           = kb;
