@@ -331,6 +331,8 @@ void agrav_destroy(agrav_ctx *c)
   c->d_ccnt.release();
   c->d_cnn.release();
   c->d_child.release();
+  c->d_meta.release();
+  c->d_cprec.release();
   c->d_lvloff.release();
   c->d_soff.release();
   c->d_pset_in.release();

@@ -548,9 +548,13 @@ __global__ __launch_bounds__(B) void k_level_scatter(int m, const int *__restric
 
 // CSR child lists, node children first (the group walk pushes them) and then
 // particle children (the group walk streams them straight into its tile)
+// Also the walk's per-node meta record and a copy of the FP32 records of the
+// particle children next to their list entries, so that the group walk reads a
+// node and its particle children in one memory round trip each.
 __global__ void k_fill_children(int m, int n, const int *__restrict__ nextnode, const int *__restrict__ sibling,
                                 const int *__restrict__ pnext, const int *__restrict__ cbeg, const int *__restrict__ ccnt,
-                                int *child, int *cnn)
+                                const int2 *__restrict__ nrange, const uint4 *__restrict__ prec, int *child, int *cnn,
+                                int4 *meta, uint4 *cprec)
 {
   int j = blockIdx.x * B + threadIdx.x;
   if(j >= m)
@@ -563,14 +567,20 @@ __global__ void k_fill_children(int m, int n, const int *__restrict__ nextnode, 
         child[o++] = c;
       c = (c < n) ? pnext[c] : sibling[c - n];
     }
-  cnn[j] = o - cbeg[j];
-  steps  = 0;
+  const int nn = o - cbeg[j];
+  cnn[j]       = nn;
+  steps        = 0;
   for(int c = nextnode[j]; c != end && c >= 0 && steps < ccnt[j]; steps++)
     {
       if(c < n && o < stop)
-        child[o++] = c;
+        {
+          cprec[o]   = prec[c];
+          child[o++] = c;
+        }
       c = (c < n) ? pnext[c] : sibling[c - n];
     }
+  const int2 rg = nrange[j];
+  meta[j]       = make_int4(cbeg[j] + nn, o - cbeg[j] - nn, rg.x, min(nn, 15) | (min(rg.y, 255) << 4));
 }
 
 // children of each node, by walking its threading from nextnode to sibling
@@ -978,8 +988,10 @@ agrav_status build_tree(agrav_ctx *c)
   AG_HIP(hipGetLastError());
   if((st = exclusive_scan_i32(c->d_ccnt.p, c->d_cbeg.p, M, nullptr, c->d_temp.p, d_perr, s)) != AGRAV_OK)
     return st;
+  if((st = c->d_meta.ensure(M)) || (st = c->d_cprec.ensure((size_t)M + n)))
+    return st;
   k_fill_children<<<nblk(M), B, 0, s>>>(M, N, c->d_nextnode.p, c->d_sibling.p, c->d_pnext.p, c->d_cbeg.p, c->d_ccnt.p,
-                                         c->d_child.p, c->d_cnn.p);
+                                         c->d_nrange.p, c->d_prec.p, c->d_child.p, c->d_cnn.p, c->d_meta.p, c->d_cprec.p);
   int off[AG_KEY_LEVELS + 2];
   off[0] = 0;
   for(int L = 0; L <= AG_KEY_LEVELS; L++)
@@ -1073,6 +1085,8 @@ TreeView tree_view(const agrav_ctx *c)
   v.ccnt     = c->d_ccnt.p;
   v.cnn      = c->d_cnn.p;
   v.child    = c->d_child.p;
+  v.meta     = c->d_meta.p;
+  v.cprec    = c->d_cprec.p;
   return v;
 }
 

@@ -126,6 +126,8 @@ struct agrav_ctx
   agrav::DBuf<uint4> d_recA, d_recB;
   agrav::DBuf<int2> d_nrange;
   agrav::DBuf<int> d_cbeg, d_ccnt, d_cnn, d_child;
+  agrav::DBuf<int4> d_meta;
+  agrav::DBuf<uint4> d_cprec;
   agrav::DBuf<int> d_lvloff, d_lvlnodes;
   int lvlcnt_h[AG_KEY_LEVELS + 2];
 

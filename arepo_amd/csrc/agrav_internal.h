@@ -75,6 +75,9 @@ struct TreeView
   const int *ccnt;         // children per node
   const int *cnn;          // of which nodes (listed first)
   const int *child;
+  const int4 *meta;        // per node {start and count of its particle children in child[], first particle,
+                           //           node children | min(particles, 255) << 4}
+  const uint4 *cprec;      // FP32 records of particle children, at their child[] positions
 };
 
 struct WalkConst

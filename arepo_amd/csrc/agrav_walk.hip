@@ -310,6 +310,9 @@ static constexpr int kStackCap = 1024;
 #ifndef AG_XCD_REMAP
 #define AG_XCD_REMAP 1
 #endif
+#ifndef AG_WALK_POP
+#define AG_WALK_POP 1
+#endif
 static constexpr int kListCap  = 128;
 
 __device__ inline int wave_incl_scan_i(int v)
@@ -659,227 +662,267 @@ __device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, co
   const int max_push = N + tv.m;  // each entry is pushed at most once per group
   __syncthreads();
 
+  // entries popped per lane and round (FP32: AG_WALK_POP); the records of all of them
+  // are read before any is used.  More than one per lane deepens the stack enough to
+  // overflow it and measured slower, so one is the default.
+  constexpr int kPop = sizeof(T) == 4 ? AG_WALK_POP : 1;
   while(sp > 0)
     {
-      const int take = sp < 64 ? sp : 64;
+      const int take = sp < 64 * kPop ? sp : 64 * kPop;
       const int base = sp - take;
       n_popped += take;
-      const int e    = lane < take ? s_stack[base + lane] : -1;
-      sp             = base;
+      int ent[kPop];
+#pragma unroll
+      for(int h = 0; h < kPop; h++)
+        ent[h] = lane + 64 * h < take ? s_stack[base + 64 * h + lane] : -1;
+      sp = base;
       __builtin_amdgcn_wave_barrier();
 
-      int act = 0;  // 1 accept node, 2 open node, 3 particle, 4 expand small node into particles
-      int j   = 0;
-      int pbase = 0, pcnt = 0;
-      Vec4<T> item{0, 0, 0, 0};
-      T hj = 0;
-      if(e >= N)
+      uint4 rA[kPop], rB[kPop];
+      int4 rM[kPop];
+      if constexpr(sizeof(T) == 4)
         {
-          j = e - N;
-          T m, ln, dcx, dcy, dcz, sxr, syr, szr;
-          if constexpr(sizeof(T) == 4)
+#pragma unroll
+          for(int h = 0; h < kPop; h++)
             {
-              uint4 A = tv.recA[j], Bq = tv.recB[j];
-              m       = __uint_as_float(A.w);
-              ln      = __uint_as_float(Bq.w);
-              dcx     = (T)(int)(Bq.x - cfx) * unit;
-              dcy     = (T)(int)(Bq.y - cfy) * unit;
-              dcz     = (T)(int)(Bq.z - cfz) * unit;
-              sxr     = (T)(int)(A.x - cfx) * unit;
-              syr     = (T)(int)(A.y - cfy) * unit;
-              szr     = (T)(int)(A.z - cfz) * unit;
-            }
-          else
-            {
-              m   = tv.nm[j];
-              ln  = tv.len[j];
-              dcx = nearest_d(tv.cx[j] - c0, wc);
-              dcy = nearest_d(tv.cy[j] - c1, wc);
-              dcz = nearest_d(tv.cz[j] - c2, wc);
-              sxr = nearest_d(tv.sx[j] - c0, wc);
-              syr = nearest_d(tv.sy[j] - c1, wc);
-              szr = nearest_d(tv.sz[j] - c2, wc);
-            }
-          const T gx = (T)fabs(dcx) - hx, gy = (T)fabs(dcy) - hy, gz = (T)fabs(dcz) - hz;
-          bool discard = false;
-          if(PM)
-            {
-              // forcetree_walk.c:217-244 for every target of the group
-              T eff   = rcut + (T)0.5 * ln;
-              discard = gx > eff || gy > eff || gz > eff;
-            }
-          if(!discard)
-            {
-              T ddx = (T)fabs(sxr) - hx, ddy = (T)fabs(syr) - hy, ddz = (T)fabs(szr) - hz;
-              ddx   = ddx > 0 ? ddx : 0;
-              ddy   = ddy > 0 ? ddy : 0;
-              ddz   = ddz > 0 ? ddz : 0;
-              T r2  = ddx * ddx + ddy * ddy + ddz * ddz;  // nearest target distance bound
-              T len2 = ln * ln;
-              bool open;
-              if(wc.use_bh)
-                open = len2 > r2 * theta2;  // forcetree_walk.c:246-254
-              else
-                open = (len2 > r2 * (T)1.44) || (m * len2 > r2 * r2 * aold_min) ||
-                       (gx < (T)0.6 * ln && gy < (T)0.6 * ln && gz < (T)0.6 * ln);  // :259-289
-              if(MSOFT)
+              const int e = ent[h];
+              rA[h]       = make_uint4(0, 0, 0, 0);
+              rB[h]       = make_uint4(0, 0, 0, 0);
+              rM[h]       = make_int4(0, 0, 0, 0);
+              if(e >= N)
                 {
-                  hj = (T)hsoft[tv.maxsoft[j]];
-                  if(!open && hj > hi_min && r2 < hj * hj)  // :294-320
-                    open = true;
+                  rM[h] = tv.meta[e - N];
+                  rA[h] = tv.recA[e - N];
+                  rB[h] = tv.recB[e - N];
                 }
-              act  = open ? 2 : 1;
-              // an accepted node whose COM lies beyond Rcut of the whole group fails the
-              // table test (tabindex < NTAB, forcetree_walk.c:372) for every target
-              if(PM && AG_NODE_RCUT_PRUNE && !open && r2 > rcut2 * (T)1.0001)
-                act = 0;
-              item = Vec4<T>{sxr, syr, szr, m};
-              if(open)
+              else if(e >= 0)
+                rA[h] = tv.prec[e];
+            }
+        }
+
+      int nchs[kPop], cbs[kPop], nch_all = 0;
+#pragma unroll
+      for(int h = 0; h < kPop; h++)
+        {
+          const int e = ent[h];
+          int act = 0;  // 1 accept node, 2 open node, 3 particle, 4 expand small node into particles
+          int j   = 0;
+          int pbase = 0, pcnt = 0;
+          int4 mt   = make_int4(0, 0, 0, 0);  // node meta {particle children start, count, first particle, cnn | count8 << 4}
+          Vec4<T> item{0, 0, 0, 0};
+          T hj = 0;
+          if(e >= N)
+            {
+              j = e - N;
+              T m, ln, dcx, dcy, dcz, sxr, syr, szr;
+              if constexpr(sizeof(T) == 4)
                 {
-                  // a small opened node is expanded straight into its particles, which are
-                  // contiguous in Hilbert order
-                  const int2 rg = tv.nrange[j];
-                  if(rg.y <= leaf_bucket)
+                  const uint4 A = rA[h], Bq = rB[h];
+                  mt      = rM[h];
+                  m       = __uint_as_float(A.w);
+                  ln      = __uint_as_float(Bq.w);
+                  dcx     = (T)(int)(Bq.x - cfx) * unit;
+                  dcy     = (T)(int)(Bq.y - cfy) * unit;
+                  dcz     = (T)(int)(Bq.z - cfz) * unit;
+                  sxr     = (T)(int)(A.x - cfx) * unit;
+                  syr     = (T)(int)(A.y - cfy) * unit;
+                  szr     = (T)(int)(A.z - cfz) * unit;
+                }
+              else
+                {
+                  mt  = tv.meta[j];
+                  m   = tv.nm[j];
+                  ln  = tv.len[j];
+                  dcx = nearest_d(tv.cx[j] - c0, wc);
+                  dcy = nearest_d(tv.cy[j] - c1, wc);
+                  dcz = nearest_d(tv.cz[j] - c2, wc);
+                  sxr = nearest_d(tv.sx[j] - c0, wc);
+                  syr = nearest_d(tv.sy[j] - c1, wc);
+                  szr = nearest_d(tv.sz[j] - c2, wc);
+                }
+              const T gx = (T)fabs(dcx) - hx, gy = (T)fabs(dcy) - hy, gz = (T)fabs(dcz) - hz;
+              bool discard = false;
+              if(PM)
+                {
+                  // forcetree_walk.c:217-244 for every target of the group
+                  T eff   = rcut + (T)0.5 * ln;
+                  discard = gx > eff || gy > eff || gz > eff;
+                }
+              if(!discard)
+                {
+                  T ddx = (T)fabs(sxr) - hx, ddy = (T)fabs(syr) - hy, ddz = (T)fabs(szr) - hz;
+                  ddx   = ddx > 0 ? ddx : 0;
+                  ddy   = ddy > 0 ? ddy : 0;
+                  ddz   = ddz > 0 ? ddz : 0;
+                  T r2  = ddx * ddx + ddy * ddy + ddz * ddz;  // nearest target distance bound
+                  T len2 = ln * ln;
+                  bool open;
+                  if(wc.use_bh)
+                    open = len2 > r2 * theta2;  // forcetree_walk.c:246-254
+                  else
+                    open = (len2 > r2 * (T)1.44) || (m * len2 > r2 * r2 * aold_min) ||
+                           (gx < (T)0.6 * ln && gy < (T)0.6 * ln && gz < (T)0.6 * ln);  // :259-289
+                  if(MSOFT)
                     {
-                      act   = 4;
-                      pbase = rg.x;
-                      pcnt  = rg.y;
+                      hj = (T)hsoft[tv.maxsoft[j]];
+                      if(!open && hj > hi_min && r2 < hj * hj)  // :294-320
+                        open = true;
+                    }
+                  act  = open ? 2 : 1;
+                  // an accepted node whose COM lies beyond Rcut of the whole group fails the
+                  // table test (tabindex < NTAB, forcetree_walk.c:372) for every target
+                  if(PM && AG_NODE_RCUT_PRUNE && !open && r2 > rcut2 * (T)1.0001)
+                    act = 0;
+                  item = Vec4<T>{sxr, syr, szr, m};
+                  if(open)
+                    {
+                      // a small opened node is expanded straight into its particles, which are
+                      // contiguous in Hilbert order
+                      const int cnt8 = mt.w >> 4;  // particle count, saturated at 255
+                      if(cnt8 <= leaf_bucket)
+                        {
+                          act   = 4;
+                          pbase = mt.z;
+                          pcnt  = cnt8;
+                        }
                     }
                 }
             }
-        }
-      else if(e >= 0)
-        {
-          T x, y, z, m;
-          if constexpr(sizeof(T) == 4)
+          else if(e >= 0)
             {
-              uint4 P = tv.prec[e];
-              x       = (T)(int)(P.x - cfx) * unit;
-              y       = (T)(int)(P.y - cfy) * unit;
-              z       = (T)(int)(P.z - cfz) * unit;
-              m       = __uint_as_float(P.w);
+              T x, y, z, m;
+              if constexpr(sizeof(T) == 4)
+                {
+                  const uint4 P = rA[h];
+                  x             = (T)(int)(P.x - cfx) * unit;
+                  y             = (T)(int)(P.y - cfy) * unit;
+                  z             = (T)(int)(P.z - cfz) * unit;
+                  m             = __uint_as_float(P.w);
+                }
+              else
+                {
+                  x = nearest_d(tv.px[e] - c0, wc);
+                  y = nearest_d(tv.py[e] - c1, wc);
+                  z = nearest_d(tv.pz[e] - c2, wc);
+                  m = tv.pm[e];
+                }
+              act  = 3;
+              item = Vec4<T>{x, y, z, m};
+              if(MSOFT)
+                hj = (T)hsoft[tv.ps[e]];
+              if(PM)
+                {
+                  // a particle farther than Rcut from the whole group fails the table test for every target
+                  T ddx = (T)fabs(x) - hx, ddy = (T)fabs(y) - hy, ddz = (T)fabs(z) - hz;
+                  ddx   = ddx > 0 ? ddx : 0;
+                  ddy   = ddy > 0 ? ddy : 0;
+                  ddz   = ddz > 0 ? ddz : 0;
+                  if(ddx * ddx + ddy * ddy + ddz * ddz > rcut2 * (T)1.0001)
+                    act = 0;
+                }
             }
-          else
-            {
-              x = nearest_d(tv.px[e] - c0, wc);
-              y = nearest_d(tv.py[e] - c1, wc);
-              z = nearest_d(tv.pz[e] - c2, wc);
-              m = tv.pm[e];
-            }
-          act  = 3;
-          item = Vec4<T>{x, y, z, m};
-          if(MSOFT)
-            hj = (T)hsoft[tv.ps[e]];
-          if(PM)
-            {
-              // a particle farther than Rcut from the whole group fails the table test for every target
-              T ddx = (T)fabs(x) - hx, ddy = (T)fabs(y) - hy, ddz = (T)fabs(z) - hz;
-              ddx   = ddx > 0 ? ddx : 0;
-              ddy   = ddy > 0 ? ddy : 0;
-              ddz   = ddz > 0 ? ddz : 0;
-              if(ddx * ddx + ddy * ddy + ddz * ddz > rcut2 * (T)1.0001)
-                act = 0;
-            }
-        }
 
-      // stage accepted entries in the LDS tile
-      const bool accp   = (act == 1 || act == 3) && item.w != (T)0;
-      const uint64_t bm = __ballot(accp);
-      const int cnt     = __popcll(bm);
-      if(nl + cnt > kListCap)
-        {
-          __builtin_amdgcn_wave_barrier();
-          n_tiled += nl;
-          tile_compute<T, PM, EWALD, MSOFT>(s_list, s_h, nl, tx, ty, tz, hi, s_tab, wc, asmthfac, asmthinv2, ewald_f, ewald_d,
-                                            ax, ay, az, pp, cax, cay, caz, cpp, nin);
-          nl = 0;
-          __builtin_amdgcn_wave_barrier();
-        }
-      if(accp)
-        {
-          int slot     = nl + __popcll(bm & lt);
-          s_list[slot] = item;
-          if(MSOFT)
-            s_h[slot] = hj;
-        }
-      nl += cnt;
+          // stage accepted entries in the LDS tile
+          const bool accp   = (act == 1 || act == 3) && item.w != (T)0;
+          const uint64_t bm = __ballot(accp);
+          const int cnt     = __popcll(bm);
+          if(nl + cnt > kListCap)
+            {
+              __builtin_amdgcn_wave_barrier();
+              n_tiled += nl;
+              tile_compute<T, PM, EWALD, MSOFT>(s_list, s_h, nl, tx, ty, tz, hi, s_tab, wc, asmthfac, asmthinv2, ewald_f, ewald_d,
+                                                ax, ay, az, pp, cax, cay, caz, cpp, nin);
+              nl = 0;
+              __builtin_amdgcn_wave_barrier();
+            }
+          if(accp)
+            {
+              int slot     = nl + __popcll(bm & lt);
+              s_list[slot] = item;
+              if(MSOFT)
+                s_h[slot] = hj;
+            }
+          nl += cnt;
 
-      // particles of small opened nodes (contiguous ranges) and the particle children of
-      // opened nodes go straight into the tile, one particle per lane and round so that a
-      // round never adds more than 64 entries; particles beyond Rcut of the whole group
-      // are dropped here
-      {
-        int pc = 0;
-        bool from_list = false;
-        if(act == 4)
-          pc = pcnt;
-        else if(act == 2)
+          // particles of small opened nodes (contiguous ranges) and the particle children of
+          // opened nodes go straight into the tile, one particle per lane and round so that a
+          // round never adds more than 64 entries; particles beyond Rcut of the whole group
+          // are dropped here
           {
-            const int nn = tv.cnn[j];
-            pc           = tv.ccnt[j] - nn;
-            pbase        = tv.cbeg[j] + nn;
-            from_list    = true;
+            int pc = 0;
+            bool from_list = false;
+            if(act == 4)
+              pc = pcnt;
+            else if(act == 2)
+              {
+                pc        = mt.y;
+                pbase     = mt.x;
+                from_list = true;
+              }
+            const int maxc = wave_max(pc);
+            for(int q = 0; q < maxc; q++)
+              {
+                bool keep = q < pc;
+                T x = 0, y = 0, z = 0, m = 0, hp = 0;
+                if(keep)
+                  {
+                    if constexpr(sizeof(T) == 4)
+                      {
+                        // particle children come as records copied next to the child list
+                        const uint4 P = from_list ? tv.cprec[pbase + q] : tv.prec[pbase + q];
+                        x             = (T)(int)(P.x - cfx) * unit;
+                        y             = (T)(int)(P.y - cfy) * unit;
+                        z             = (T)(int)(P.z - cfz) * unit;
+                        m             = __uint_as_float(P.w);
+                      }
+                    else
+                      {
+                        const int e2 = from_list ? tv.child[pbase + q] : pbase + q;
+                        x            = nearest_d(tv.px[e2] - c0, wc);
+                        y            = nearest_d(tv.py[e2] - c1, wc);
+                        z            = nearest_d(tv.pz[e2] - c2, wc);
+                        m            = tv.pm[e2];
+                      }
+                    if(MSOFT)
+                      hp = (T)hsoft[tv.ps[from_list ? tv.child[pbase + q] : pbase + q]];
+                    keep = m != (T)0;
+                    if(PM && keep)
+                      {
+                        T ddx = (T)fabs(x) - hx, ddy = (T)fabs(y) - hy, ddz = (T)fabs(z) - hz;
+                        ddx   = ddx > 0 ? ddx : 0;
+                        ddy   = ddy > 0 ? ddy : 0;
+                        ddz   = ddz > 0 ? ddz : 0;
+                        keep  = ddx * ddx + ddy * ddy + ddz * ddz <= rcut2 * (T)1.0001;
+                      }
+                  }
+                const uint64_t rb = __ballot(keep);
+                const int rcnt    = __popcll(rb);
+                if(nl + rcnt > kListCap)
+                  {
+                    __builtin_amdgcn_wave_barrier();
+                    n_tiled += nl;
+                    tile_compute<T, PM, EWALD, MSOFT>(s_list, s_h, nl, tx, ty, tz, hi, s_tab, wc, asmthfac, asmthinv2, ewald_f,
+                                                      ewald_d, ax, ay, az, pp, cax, cay, caz, cpp, nin);
+                    nl = 0;
+                    __builtin_amdgcn_wave_barrier();
+                  }
+                if(keep)
+                  {
+                    const int slot = nl + __popcll(rb & lt);
+                    s_list[slot]   = Vec4<T>{x, y, z, m};
+                    if(MSOFT)
+                      s_h[slot] = hp;
+                  }
+                nl += rcnt;
+              }
           }
-        const int maxc = wave_max(pc);
-        for(int q = 0; q < maxc; q++)
-          {
-            bool keep = q < pc;
-            T x = 0, y = 0, z = 0, m = 0, hp = 0;
-            if(keep)
-              {
-                const int e2 = from_list ? tv.child[pbase + q] : pbase + q;
-                if constexpr(sizeof(T) == 4)
-                  {
-                    uint4 P = tv.prec[e2];
-                    x       = (T)(int)(P.x - cfx) * unit;
-                    y       = (T)(int)(P.y - cfy) * unit;
-                    z       = (T)(int)(P.z - cfz) * unit;
-                    m       = __uint_as_float(P.w);
-                  }
-                else
-                  {
-                    x = nearest_d(tv.px[e2] - c0, wc);
-                    y = nearest_d(tv.py[e2] - c1, wc);
-                    z = nearest_d(tv.pz[e2] - c2, wc);
-                    m = tv.pm[e2];
-                  }
-                if(MSOFT)
-                  hp = (T)hsoft[tv.ps[e2]];
-                keep = m != (T)0;
-                if(PM && keep)
-                  {
-                    T ddx = (T)fabs(x) - hx, ddy = (T)fabs(y) - hy, ddz = (T)fabs(z) - hz;
-                    ddx   = ddx > 0 ? ddx : 0;
-                    ddy   = ddy > 0 ? ddy : 0;
-                    ddz   = ddz > 0 ? ddz : 0;
-                    keep  = ddx * ddx + ddy * ddy + ddz * ddz <= rcut2 * (T)1.0001;
-                  }
-              }
-            const uint64_t rb = __ballot(keep);
-            const int rcnt    = __popcll(rb);
-            if(nl + rcnt > kListCap)
-              {
-                __builtin_amdgcn_wave_barrier();
-                n_tiled += nl;
-          tile_compute<T, PM, EWALD, MSOFT>(s_list, s_h, nl, tx, ty, tz, hi, s_tab, wc, asmthfac, asmthinv2, ewald_f,
-                                                  ewald_d, ax, ay, az, pp, cax, cay, caz, cpp, nin);
-                nl = 0;
-                __builtin_amdgcn_wave_barrier();
-              }
-            if(keep)
-              {
-                const int slot = nl + __popcll(rb & lt);
-                s_list[slot]   = Vec4<T>{x, y, z, m};
-                if(MSOFT)
-                  s_h[slot] = hp;
-              }
-            nl += rcnt;
-          }
-      }
+          nchs[h] = (act == 2) ? (mt.w & 15) : 0;  // node children only (at most 8)
+          cbs[h]  = mt.x - nchs[h];                  // node children precede the particle children
+          nch_all += nchs[h];
+        }
 
       // push the children of opened nodes
-      const int nch  = (act == 2) ? tv.cnn[j] : 0;  // node children only
-      const int incl = wave_incl_scan_i(nch);
+      const int incl = wave_incl_scan_i(nch_all);
       const int tot  = __shfl(incl, 63, 64);
       pushed += tot;
       if(sp + tot > kStackCap || pushed > max_push)
@@ -887,13 +930,11 @@ __device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, co
           overflow = true;
           break;
         }
-      if(nch)
-        {
-          const int o  = sp + incl - nch;
-          const int cb = tv.cbeg[j];
-          for(int k = 0; k < nch; k++)
-            s_stack[o + k] = tv.child[cb + k];
-        }
+      int o = sp + incl - nch_all;
+#pragma unroll
+      for(int h = 0; h < kPop; h++)
+        for(int k = 0; k < nchs[h]; k++)
+          s_stack[o++] = tv.child[cbs[h] + k];
       sp += tot;
       __builtin_amdgcn_wave_barrier();
     }
