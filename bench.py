@@ -250,6 +250,21 @@ def batch_run(A, nsets, steps, seed, cpu_particles, threads, err_sample):
     return res
 
 
+PMC_PROFILE = os.path.join(ROOT, "profiles", "k_walk_group_treepm128_pmc.md")
+
+
+def profiled_traffic():
+    """HBM bytes per launch of the walk kernel from the committed rocprofv3 counter summary
+    (FETCH_SIZE + WRITE_SIZE, each scaled by a copy of known size); None if absent."""
+    try:
+        for line in open(PMC_PROFILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1]), os.path.relpath(PMC_PROFILE, ROOT)
+    except OSError:
+        pass
+    return None, None
+
+
 def copy_probe(nbytes=256 << 20):
     """One device-to-device copy of known size (the runtime's copy kernel), used to scale the
     FETCH_SIZE / WRITE_SIZE counters of a profiled run."""
@@ -305,6 +320,7 @@ def main():
     ap.add_argument("--batch-sets", type=int, default=4096)
     ap.add_argument("--batch-steps", type=int, default=2)
     ap.add_argument("--batch-cpu-particles", type=int, default=8 << 20)
+    ap.add_argument("--batch-only", action="store_true", help="run only the batched-unbinding leg (profiling aid)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -315,6 +331,12 @@ def main():
 
     from arepo_amd import agrav as A
     from arepo_amd import ics
+
+    if args.batch_only:
+        res = batch_run(A, args.batch_sets, args.batch_steps, 4, 0 if args.no_cpu else args.batch_cpu_particles,
+                        min(os.cpu_count() or 1, 16), args.err_sample)
+        print(json.dumps({"batched_unbinding": res}), flush=True)
+        return
 
     log("generating clustered 128^3 set (seed %d)" % args.seed)
     pos, mass, eps = ics.clustered_box(args.n, seed=args.seed)
@@ -368,9 +390,10 @@ def main():
     walk_s = tim["walk_kernel"] * 1e-3
     alg_bytes = inter * 16 + n * 52
     achieved = alg_bytes / walk_s if walk_s > 0 else 0.0
+    traffic, traffic_src = profiled_traffic()
     out["roofline"] = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                       "frac": achieved / HBM_PEAK, "traffic": None, "kernel": "k_walk_group<float,PM>",
-                       "kernel_ms": tim["walk_kernel"], "alg_bytes_per_launch": alg_bytes}
+                       "frac": achieved / HBM_PEAK, "traffic": traffic, "traffic_source": traffic_src,
+                       "kernel": "k_walk_group<float,PM>", "kernel_ms": tim["walk_kernel"], "alg_bytes_per_launch": alg_bytes}
 
     # the same path with EVALPOTENTIAL (potentials summed too)
     rpot, _ = run_path(A, pos, mass, eps, A.FP32, A.WALK_GROUP, 3, 0, old_acc=res32["old_acc"], pot=True)
