@@ -299,9 +299,11 @@ __global__ void k_gather(const int *__restrict__ idx, const double *__restrict__
 }
 
 // ---------------------------------------------------------------------------
-// 3. binary radix tree (Karras 2012)
+// 3./4. octree cubes from the sorted keys
 // ---------------------------------------------------------------------------
 
+// number of equal leading bits of keys i and j (keys use bits 62..0); identical
+// keys count as longer than any key prefix; -1 beyond the ends
 __device__ inline int kdelta(const uint64_t *__restrict__ k, int n, int i, int j)
 {
   if(j < 0 || j >= n)
@@ -312,12 +314,6 @@ __device__ inline int kdelta(const uint64_t *__restrict__ k, int n, int i, int j
   return __clzll((long long)(a ^ b)) - 1;  // keys use bits 62..0
 }
 
-// octree level of a binary node with the given common-prefix length; -1 while the
-// prefix has not left the set-index bits of a batch key
-__device__ inline int oct_level(int prefix, int sbits, int levels)
-{
-  return prefix < sbits ? -1 : min((prefix - sbits) / 3, levels);
-}
 
 // Levels at which a cube starts at particle f, from its neighbours' keys alone:
 // the level-L cell of f (prefix of sbits + 3L bits) starts at f when f-1 differs
@@ -606,7 +602,6 @@ __global__ void k_count_children(int m, int n, const int *__restrict__ nextnode,
 // One node's moments from its CSR children (forcetree.c:1217-1429): mass, FP64
 // centre of mass (geometric centre when empty) and maxsofttype, plus the two FP32
 // walk records.  Children are summed in their fixed list order.
-template <bool VOLATILE>
 __device__ inline void node_moments(int j, int n, const int *__restrict__ cbeg, const int *__restrict__ ccnt, const int *__restrict__ child,
                                     const double *__restrict__ px, const double *__restrict__ py, const double *__restrict__ pz,
                                     const double *__restrict__ pm, const uint8_t *__restrict__ ps, const double *__restrict__ hsoft,
@@ -650,23 +645,11 @@ __device__ inline void node_moments(int j, int n, const int *__restrict__ cbeg, 
           else
             {
               const int k = c - n;
-              if(VOLATILE)
-                {
-                  // written earlier in this launch by another lane: agent-coherent loads
-                  mm[u] = __hip_atomic_load(&nm[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                  x[u]  = __hip_atomic_load(&sx[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                  y[u]  = __hip_atomic_load(&sy[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                  z[u]  = __hip_atomic_load(&sz[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                  st[u] = __hip_atomic_load(&maxsoft[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                }
-              else
-                {
-                  mm[u] = nm[k];
-                  x[u]  = sx[k];
-                  y[u]  = sy[k];
-                  z[u]  = sz[k];
-                  st[u] = maxsoft[k];
-                }
+              mm[u]       = nm[k];
+              x[u]        = sx[k];
+              y[u]        = sy[k];
+              z[u]        = sz[k];
+              st[u]       = maxsoft[k];
             }
         }
 #pragma unroll
@@ -796,11 +779,12 @@ __global__ __launch_bounds__(B) void k_moments_level(const int *__restrict__ lis
     return;
   const int j = list[t];
   if(nrange[j].y > kSmallCube)
-    node_moments<false>(j, MOMENT_ARGS);
+    node_moments(j, MOMENT_ARGS);
 }
 
-// a run of consecutive small levels (deepest first) in one workgroup: levels are
-// separated by workgroup barriers, child data are read past L1
+// a run of consecutive levels with few nodes (deepest first) in one workgroup:
+// levels are separated by workgroup barriers, which also make one level's
+// results visible to the next (all waves share the CU's caches)
 __global__ __launch_bounds__(1024) void k_moments_levels_small(const int *__restrict__ list, const int *__restrict__ lvloff, int lhi,
                                                                 int llo, const int2 *__restrict__ nrange, MOMENT_PARAMS)
 {
@@ -810,7 +794,7 @@ __global__ __launch_bounds__(1024) void k_moments_levels_small(const int *__rest
         {
           const int j = list[t];
           if(nrange[j].y > kSmallCube)
-            node_moments<false>(j, MOMENT_ARGS);
+            node_moments(j, MOMENT_ARGS);
         }
       __threadfence_block();
       __syncthreads();

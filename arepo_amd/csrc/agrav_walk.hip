@@ -666,9 +666,17 @@ __device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, co
   // are read before any is used.  More than one per lane deepens the stack enough to
   // overflow it and measured slower, so one is the default.
   constexpr int kPop = sizeof(T) == 4 ? AG_WALK_POP : 1;
+  // An opened node pushes at most 8 node children.  Near the top of the stack fewer
+  // entries are popped, so that the pushes can never pass the last kReserve slots; a
+  // single entry (a depth-first step) adds at most 7 per tree level, which the
+  // reserve holds down to the deepest level.  Deep walks (large isolated sets) then
+  // narrow instead of overflowing.
+  constexpr int kReserve = 8 * (AG_KEY_LEVELS + 2);
   while(sp > 0)
     {
-      const int take = sp < 64 * kPop ? sp : 64 * kPop;
+      int take       = sp < 64 * kPop ? sp : 64 * kPop;
+      const int room = (kStackCap - kReserve - sp) >> 3;
+      take           = take < room ? take : (room > 1 ? room : 1);
       const int base = sp - take;
       n_popped += take;
       int ent[kPop];
