@@ -48,6 +48,7 @@ class Params(C.Structure):
         ("tree_alloc_factor", C.c_double),
         ("leaf_bucket", C.c_int),
         ("eval_potential", C.c_int),
+        ("group_waves", C.c_int),
     ]
 
 
@@ -186,8 +187,8 @@ def _declare(L):
 
 
 def make_params(periodic=True, box_size=1.0, pmgrid=0, asmth=1.25, rcut=4.5, G=1.0, err_tol_theta=0.7,
-                err_tol_force_acc=0.0025, opening=1, soft=(0.01,), precision=FP64, walk=WALK_GROUP, group_size=64,
-                leaf_bucket=8, eval_potential=True):
+                err_tol_force_acc=0.0025, opening=1, soft=(0.01,), precision=FP64, walk=WALK_GROUP, group_size=0,
+                leaf_bucket=8, eval_potential=True, group_waves=0):
     p = Params()
     lib().agrav_default_params(C.byref(p))
     p.periodic = int(periodic)
@@ -207,6 +208,7 @@ def make_params(periodic=True, box_size=1.0, pmgrid=0, asmth=1.25, rcut=4.5, G=1
     p.group_size = group_size
     p.leaf_bucket = leaf_bucket
     p.eval_potential = int(eval_potential)
+    p.group_waves = int(group_waves)
     return p
 
 

@@ -168,10 +168,11 @@ void agrav_default_params(agrav_params *p)
   p->softening[0]              = 0.01;
   p->precision                 = AGRAV_FP64;
   p->walk                      = AGRAV_WALK_GROUP;
-  p->group_size                = 64;
+  p->group_size                = 0;  // by workload (walk_tree)
   p->tree_alloc_factor         = 0;
   p->leaf_bucket               = 8;
   p->eval_potential            = 1;
+  p->group_waves               = 0;
 }
 
 static agrav_status check_params(const agrav_params *p)
@@ -205,6 +206,8 @@ static agrav_status check_params(const agrav_params *p)
     return AGRAV_ERR_BAD_PARAM;
   if(p->leaf_bucket < 0 || p->leaf_bucket > 64)
     return AGRAV_ERR_BAD_PARAM;
+  if(p->group_waves != 0 && p->group_waves != 1 && p->group_waves != 2 && p->group_waves != 4)
+    return AGRAV_ERR_BAD_PARAM;
   return AGRAV_OK;
 }
 
@@ -228,8 +231,6 @@ agrav_status agrav_create(const agrav_params *p, agrav_ctx **out)
     }
   agrav_ctx *c = new agrav_ctx();
   c->par       = *p;
-  if(c->par.group_size == 0)
-    c->par.group_size = 64;
   if(hipGetDevice(&c->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
     {
       delete c;

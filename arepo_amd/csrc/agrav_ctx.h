@@ -159,6 +159,7 @@ struct agrav_ctx
   // timing
   hipEvent_t ev[8] = {};
   agrav_timings tim{};
+  int tim_group_waves = 1;  // wavefronts per group of the last group walk
 };
 
 namespace agrav

@@ -313,6 +313,9 @@ static constexpr int kStackCap = 1024;
 #ifndef AG_WALK_POP
 #define AG_WALK_POP 1
 #endif
+#ifndef AG_SPLIT_MIN
+#define AG_SPLIT_MIN 64
+#endif
 static constexpr int kListCap  = 128;
 
 __device__ inline int wave_incl_scan_i(int v)
@@ -566,38 +569,50 @@ template <typename T, bool PM> __device__ inline void load_tables(float *s_tab, 
       // the FP32 tile loop uses the force table pre-multiplied by (0.5/Asmth)^2
       const float tscale = PM ? (float)wc.asmthinv2 : 1.0f;
       float2 *ftab       = reinterpret_cast<float2 *>(s_tab);
-      for(int q = threadIdx.x; q < 128; q += 64)
+      for(int q = threadIdx.x & 63; q < 128; q += 64)
         {
           const float f0 = tabs[q] * tscale;
           const float f1 = q < 127 ? tabs[q + 1] * tscale : f0;
           ftab[q]        = make_float2(f0, f1 - f0);
           s_tab[256 + q] = tabs[128 + q];
         }
-      if(threadIdx.x == 0)
+      if((threadIdx.x & 63) == 0)
         s_tab[384] = tabs[255];
     }
   else
-    for(int q = threadIdx.x; q < 256; q += 64)
+    for(int q = threadIdx.x & 63; q < 256; q += 64)
       s_tab[q] = tabs[q];
 }
 
 // Walk of one group: the np (<= 64) Hilbert-consecutive targets from p0, one per
 // lane, against the tree below the entry `root` (the tree root N, or the top entry
 // of one set of a batch).  `hsoft` is indexed by softening type; `unit` is the
-// fixed-point step of the targets' domain.  Results go to at/pt/nint at the
-// targets' sorted indices.  Returns false when the LDS stack overflowed: nothing
-// is written then and the caller re-walks the group one lane per target.
+// fixed-point step of the targets' domain.  With nsplit > 1 the group's traversal
+// is shared by nsplit wavefronts, this one being number wsplit: all of them expand
+// the first levels alike (only wave 0 keeps their interactions), then the stack is
+// dealt out round-robin and each wave walks its share; every entry is evaluated by
+// exactly one wave, and the caller adds the waves' partial sums in wave order.
+// Returns false
+// when the LDS stack overflowed (the caller re-walks the group one lane per
+// target).
+// per-lane partial sums of one walk (Kahan pairs) and its interaction count
+template <typename T> struct GroupAcc
+{
+  T ax, ay, az, pp, cax, cay, caz, cpp;
+  int nin;
+};
+
 template <typename T, bool PM, bool EWALD, bool MSOFT>
 __device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, const WalkConst &wc, const double *__restrict__ hsoft,
                                   const float *__restrict__ oldacc_s, const float *__restrict__ ewald_f,
-                                  const double *__restrict__ ewald_d, int p0, int np, int root, T unit, int leaf_bucket, double *at,
-                                  double *pt, int *nint, unsigned long long (&cnt)[3])
+                                  const double *__restrict__ ewald_d, int p0, int np, int root, T unit, int leaf_bucket,
+                                  int nsplit, int wsplit, GroupAcc<T> &out, unsigned long long (&cnt)[3])
 {
   int *s_stack     = lds.stack;
   Vec4<T> *s_list  = lds.list;
   T *s_h           = lds.h;
   const float *s_tab = lds.tab;
-  const int lane   = threadIdx.x;
+  const int lane   = threadIdx.x & 63;
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 
   const int N  = tv.n;
@@ -672,8 +687,33 @@ __device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, co
   // reserve holds down to the deepest level.  Deep walks (large isolated sets) then
   // narrow instead of overflowing.
   constexpr int kReserve = 8 * (AG_KEY_LEVELS + 2);
+  // split group: all waves expand the same entries (only wave 0 keeps their
+  // interactions) until the stack holds kSplitMin entries; then each wave keeps every
+  // nsplit-th of them and walks on alone
+  const int kSplitMin = AG_SPLIT_MIN * nsplit;
+  bool split_done     = nsplit <= 1;
   while(sp > 0)
     {
+      if(!split_done && sp >= kSplitMin)
+        {
+          int kept = 0;
+          for(int c0 = 0; c0 < sp; c0 += 64)
+            {
+              const int i       = c0 + lane;
+              const int e       = i < sp ? s_stack[i] : -1;
+              const bool keep   = i < sp && i % nsplit == wsplit;
+              const uint64_t km = __ballot(keep);
+              __builtin_amdgcn_wave_barrier();  // the chunk is read before it is overwritten
+              if(keep)
+                s_stack[kept + __popcll(km & lt)] = e;
+              kept += __popcll(km);
+              __builtin_amdgcn_wave_barrier();
+            }
+          sp         = kept;
+          split_done = true;
+          if(sp == 0)
+            break;
+        }
       int take       = sp < 64 * kPop ? sp : 64 * kPop;
       const int room = (kStackCap - kReserve - sp) >> 3;
       take           = take < room ? take : (room > 1 ? room : 1);
@@ -685,6 +725,7 @@ __device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, co
         ent[h] = lane + 64 * h < take ? s_stack[base + 64 * h + lane] : -1;
       sp = base;
       __builtin_amdgcn_wave_barrier();
+      const bool keep_ia = split_done || wsplit == 0;
 
       uint4 rA[kPop], rB[kPop];
       int4 rM[kPop];
@@ -831,7 +872,7 @@ __device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, co
             }
 
           // stage accepted entries in the LDS tile
-          const bool accp   = (act == 1 || act == 3) && item.w != (T)0;
+          const bool accp   = (act == 1 || act == 3) && item.w != (T)0 && keep_ia;
           const uint64_t bm = __ballot(accp);
           const int cnt     = __popcll(bm);
           if(nl + cnt > kListCap)
@@ -867,6 +908,8 @@ __device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, co
                 pbase     = mt.x;
                 from_list = true;
               }
+            if(!keep_ia)
+              pc = 0;
             const int maxc = wave_max(pc);
             for(int q = 0; q < maxc; q++)
               {
@@ -954,14 +997,7 @@ __device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, co
           tile_compute<T, PM, EWALD, MSOFT>(s_list, s_h, nl, tx, ty, tz, hi, s_tab, wc, asmthfac, asmthinv2, ewald_f, ewald_d, ax, ay,
                                     az, pp, cax, cay, caz, cpp, nin);
 
-  if(has)
-    {
-      at[3 * (int64_t)pi + 0] = (double)ax - (double)cax;
-      at[3 * (int64_t)pi + 1] = (double)ay - (double)cay;
-      at[3 * (int64_t)pi + 2] = (double)az - (double)caz;
-      pt[pi]                  = (double)pp - (double)cpp;
-      nint[pi]                = nin;
-    }
+  out = GroupAcc<T>{ax, ay, az, pp, cax, cay, caz, cpp, nin};
   unsigned long long v = has ? (unsigned long long)nin : 0ull;
   for(int off = 32; off > 0; off >>= 1)
     v += __shfl_xor(v, off, 64);
@@ -973,7 +1009,7 @@ __device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, co
 
 __device__ inline void flush_counts(const unsigned long long (&cnt)[3], unsigned long long *isum)
 {
-  if(threadIdx.x == 0)
+  if((threadIdx.x & 63) == 0)
     for(int k = 0; k < 3; k++)
       if(cnt[k])
         atomicAdd(isum + k, cnt[k]);
@@ -991,28 +1027,86 @@ __device__ inline int xcd_block(int b, int nb)
   return x * per + (x < rem ? x : rem) + k;
 }
 
-// one group of gs consecutive targets per block
-template <typename T, bool PM, bool EWALD, bool MSOFT>
-__global__ __launch_bounds__(64) void k_walk_group(TreeView tv, WalkConst wc, const double *__restrict__ hsoft,
-                                                   const float *__restrict__ oldacc_s, const float *__restrict__ tabs,
-                                                   const float *__restrict__ ewald_f, const double *__restrict__ ewald_d, int gs,
-                                                   double *at, double *pt, int *nint, int *failed, int *nfailed,
-                                                   unsigned long long *isum)
+// One group of gs consecutive targets per block of W wavefronts (W > 1: the
+// group's traversal is shared, see walk_group); wave 0 adds the waves' partial
+// sums in wave order, so the result is deterministic for a given W.
+template <typename T, bool PM, bool EWALD, bool MSOFT, int W>
+__global__ __launch_bounds__(64 * W) void k_walk_group(TreeView tv, WalkConst wc, const double *__restrict__ hsoft,
+                                                       const float *__restrict__ oldacc_s, const float *__restrict__ tabs,
+                                                       const float *__restrict__ ewald_f, const double *__restrict__ ewald_d, int gs,
+                                                       double *at, double *pt, int *nint, int *failed, int *nfailed,
+                                                       unsigned long long *isum)
 {
-  __shared__ WalkLds<T, MSOFT> lds;
-  load_tables<T, PM>(lds.tab, tabs, wc);
+  __shared__ WalkLds<T, MSOFT> lds[W];
+  __shared__ int s_fail;
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  load_tables<T, PM>(lds[wv].tab, tabs, wc);
+  if(threadIdx.x == 0)
+    s_fail = 0;
   const int g  = xcd_block(blockIdx.x, gridDim.x);
   const int p0 = g * gs;
   const int np = min(gs, tv.n - p0);
   unsigned long long cnt[3] = {0, 0, 0};
-  if(!walk_group<T, PM, EWALD, MSOFT>(lds, tv, wc, hsoft, oldacc_s, ewald_f, ewald_d, p0, np, tv.n, (T)wc.fix_unit,
-                                      wc.leaf_bucket, at, pt, nint, cnt) &&
-     threadIdx.x == 0)
-    {
-      int slot     = atomicAdd(nfailed, 1);
-      failed[slot] = g;
-    }
+  GroupAcc<T> acc;
+  const bool ok = walk_group<T, PM, EWALD, MSOFT>(lds[wv], tv, wc, hsoft, oldacc_s, ewald_f, ewald_d, p0, np, tv.n, (T)wc.fix_unit,
+                                                  wc.leaf_bucket, W, wv, acc, cnt);
   flush_counts(cnt, isum);
+  const int64_t pi = (int64_t)p0 + lane;
+  if(W == 1)
+    {
+      if(ok && lane < np)
+        {
+          at[3 * pi + 0] = (double)acc.ax - (double)acc.cax;
+          at[3 * pi + 1] = (double)acc.ay - (double)acc.cay;
+          at[3 * pi + 2] = (double)acc.az - (double)acc.caz;
+          pt[pi]         = (double)acc.pp - (double)acc.cpp;
+          nint[pi]       = acc.nin;
+        }
+      else if(!ok && lane == 0)
+        {
+          int slot     = atomicAdd(nfailed, 1);
+          failed[slot] = g;
+        }
+      return;
+    }
+  // partial sums through LDS (the waves' stacks are free now)
+  if(!ok && lane == 0)
+    atomicOr(&s_fail, 1);
+  double *red = reinterpret_cast<double *>(lds[wv].stack);  // 4 x 64 doubles + 64 ints < 4 KB
+  red[lane]        = (double)acc.ax - (double)acc.cax;
+  red[64 + lane]   = (double)acc.ay - (double)acc.cay;
+  red[128 + lane]  = (double)acc.az - (double)acc.caz;
+  red[192 + lane]  = (double)acc.pp - (double)acc.cpp;
+  reinterpret_cast<int *>(red + 256)[lane] = acc.nin;
+  __syncthreads();
+  if(wv == 0)
+    {
+      if(s_fail)
+        {
+          if(lane == 0)
+            {
+              int slot     = atomicAdd(nfailed, 1);
+              failed[slot] = g;
+            }
+        }
+      else if(lane < np)
+        {
+          double s[4] = {0, 0, 0, 0};
+          int ni      = 0;
+          for(int v = 0; v < W; v++)
+            {
+              const double *r = reinterpret_cast<const double *>(lds[v].stack);
+              for(int k = 0; k < 4; k++)
+                s[k] += r[64 * k + lane];
+              ni += reinterpret_cast<const int *>(r + 256)[lane];
+            }
+          at[3 * pi + 0] = s[0];
+          at[3 * pi + 1] = s[1];
+          at[3 * pi + 2] = s[2];
+          pt[pi]         = s[3];
+          nint[pi]       = ni;
+        }
+    }
 }
 
 // Batch of independent isolated sets: a persistent grid whose waves take groups
@@ -1038,9 +1132,21 @@ __global__ __launch_bounds__(64) void k_walk_batch(TreeView tv, WalkConst wc, Ba
         break;
       const int4 d = bv.groups[g];
       const T unit = (T)bv.dom[d.z].fix_unit;
-      if(!walk_group<T, false, false, false>(lds, tv, wc, bv.sh + d.z, oldacc_s, nullptr, nullptr, d.x, d.y, bv.sroot[d.z], unit,
-                                             d.w, at, pt, nint, cnt) &&
-         threadIdx.x == 0)
+      GroupAcc<T> acc;
+      if(walk_group<T, false, false, false>(lds, tv, wc, bv.sh + d.z, oldacc_s, nullptr, nullptr, d.x, d.y, bv.sroot[d.z], unit, d.w,
+                                            1, 0, acc, cnt))
+        {
+          if((int)threadIdx.x < d.y)
+            {
+              const int64_t pi = (int64_t)d.x + threadIdx.x;
+              at[3 * pi + 0]   = (double)acc.ax - (double)acc.cax;
+              at[3 * pi + 1]   = (double)acc.ay - (double)acc.cay;
+              at[3 * pi + 2]   = (double)acc.az - (double)acc.caz;
+              pt[pi]           = (double)acc.pp - (double)acc.cpp;
+              nint[pi]         = acc.nin;
+            }
+        }
+      else if(threadIdx.x == 0)
         {
           int slot     = atomicAdd(nfailed, 1);
           failed[slot] = g;
@@ -1082,38 +1188,51 @@ WalkConst walk_const(const agrav_ctx *c, int use_bh)
   return wc;
 }
 
-template <typename T, bool PM, bool EW, bool MS>
-static void launch_group(int ngroups, hipStream_t s, const TreeView &tv, const WalkConst &wc, agrav_ctx *c, int gs)
+template <typename T, bool PM, bool EW, bool MS, int W>
+static void launch_group_w(int ngroups, hipStream_t s, const TreeView &tv, const WalkConst &wc, agrav_ctx *c, int gs)
 {
-  k_walk_group<T, PM, EW, MS><<<ngroups, 64, 0, s>>>(tv, wc, c->d_soft.p, c->d_oldacc_s.p, c->d_tab.p, c->d_ewald_f.p,
-                                                      c->d_ewald.p, gs, c->d_at.p, c->d_pt.p, c->d_nint.p, c->d_failed.p,
-                                                      c->d_counters.p + 48, c->d_isum.p);
+  k_walk_group<T, PM, EW, MS, W><<<ngroups, 64 * W, 0, s>>>(tv, wc, c->d_soft.p, c->d_oldacc_s.p, c->d_tab.p, c->d_ewald_f.p,
+                                                            c->d_ewald.p, gs, c->d_at.p, c->d_pt.p, c->d_nint.p, c->d_failed.p,
+                                                            c->d_counters.p + 48, c->d_isum.p);
+}
+
+template <typename T, bool PM, bool EW, bool MS>
+static void launch_group(int ngroups, hipStream_t s, const TreeView &tv, const WalkConst &wc, agrav_ctx *c, int gs, int waves)
+{
+  if constexpr(sizeof(T) == 4)
+    {
+      if(waves == 4)
+        return launch_group_w<T, PM, EW, MS, 4>(ngroups, s, tv, wc, c, gs);
+      if(waves == 2)
+        return launch_group_w<T, PM, EW, MS, 2>(ngroups, s, tv, wc, c, gs);
+    }
+  launch_group_w<T, PM, EW, MS, 1>(ngroups, s, tv, wc, c, gs);
 }
 
 template <typename T>
 static void dispatch_group(bool pm, bool ew, bool ms, int ngroups, hipStream_t s, const TreeView &tv, const WalkConst &wc,
-                           agrav_ctx *c, int gs)
+                           agrav_ctx *c, int gs, int waves)
 {
   if(pm)
     {
       if(ms)
-        launch_group<T, true, false, true>(ngroups, s, tv, wc, c, gs);
+        launch_group<T, true, false, true>(ngroups, s, tv, wc, c, gs, waves);
       else
-        launch_group<T, true, false, false>(ngroups, s, tv, wc, c, gs);
+        launch_group<T, true, false, false>(ngroups, s, tv, wc, c, gs, waves);
     }
   else if(ew)
     {
       if(ms)
-        launch_group<T, false, true, true>(ngroups, s, tv, wc, c, gs);
+        launch_group<T, false, true, true>(ngroups, s, tv, wc, c, gs, waves);
       else
-        launch_group<T, false, true, false>(ngroups, s, tv, wc, c, gs);
+        launch_group<T, false, true, false>(ngroups, s, tv, wc, c, gs, waves);
     }
   else
     {
       if(ms)
-        launch_group<T, false, false, true>(ngroups, s, tv, wc, c, gs);
+        launch_group<T, false, false, true>(ngroups, s, tv, wc, c, gs, waves);
       else
-        launch_group<T, false, false, false>(ngroups, s, tv, wc, c, gs);
+        launch_group<T, false, false, false>(ngroups, s, tv, wc, c, gs, waves);
     }
 }
 
@@ -1258,15 +1377,27 @@ agrav_status walk_tree(agrav_ctx *c, int use_bh)
   else
     {
       AG_HIP(hipEventRecord(c->ev[2], s));
-      int gs      = c->par.group_size > 0 ? c->par.group_size : 64;
+      // by workload when not set (MI355X timings of FP32 walks): periodic sets and large
+      // isolated sets fill the device with groups of 64; small isolated sets are bound
+      // by the latency of each group's walk and gain from groups of 16 (200k Hernquist
+      // particles: 1.3 ms against 8.3 ms) and from sharing a group over 4 wavefronts
+      const bool isolated = !pm && !ew;
+      int gs = c->par.group_size;
+      if(gs <= 0)
+        gs = (isolated && n < (1 << 19)) ? 16 : 64;
       int ngroups = (int)((n + gs - 1) / gs);
       if((st = c->d_failed.ensure(ngroups)) != AGRAV_OK)
         return st;
       AG_HIP(hipMemsetAsync(c->d_counters.p + 48, 0, sizeof(int), s));
+      // wavefronts per group: small sets have too few groups to fill the device
+      int waves = c->par.group_waves;
+      if(waves <= 0)
+        waves = isolated ? (n < (1 << 17) ? 4 : 1) : (n < (1 << 18) ? 4 : (n < (1 << 20) ? 2 : 1));
+      c->tim_group_waves = c->par.precision == AGRAV_FP32 ? waves : 1;
       if(c->par.precision == AGRAV_FP32)
-        dispatch_group<float>(pm, ew, ms, ngroups, s, tv, wc, c, gs);
+        dispatch_group<float>(pm, ew, ms, ngroups, s, tv, wc, c, gs, waves);
       else
-        dispatch_group<double>(pm, ew, ms, ngroups, s, tv, wc, c, gs);
+        dispatch_group<double>(pm, ew, ms, ngroups, s, tv, wc, c, gs, 1);
       AG_HIP(hipGetLastError());
       AG_HIP(hipEventRecord(c->ev[3], s));
       // groups whose LDS stack overflowed are re-walked one lane per target

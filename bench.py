@@ -89,7 +89,7 @@ def serving_run(A, precision, walk, slots, rate, seconds, seed, max_requests=Non
     starts = g.integers(0, pool_n - sizes + 1)
     eps = (1.0 + np.sqrt(2.0)) / 50.0  # 1/50 of the Hernquist half-mass radius (a = 1)
     p = A.make_params(periodic=False, box_size=0.0, pmgrid=0, soft=(eps,), precision=precision, walk=walk,
-                      group_size=64, leaf_bucket=4, eval_potential=True)
+                      group_size=0, leaf_bucket=4, eval_potential=True)
     lat, runs = [], []
     with A.Server(p, slots=slots, hbm_budget_bytes=64 << 30) as srv:
         # warm the slots up (device buffers, first-launch costs)

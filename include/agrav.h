@@ -70,12 +70,18 @@ typedef struct
   double softening[AGRAV_MAX_SOFT_TYPES]; /* Plummer-equivalent softening per type (SofteningComovingType*) */
   int precision;                  /* agrav_precision */
   int walk;                       /* agrav_walk_kind */
-  int group_size;                 /* targets per group for AGRAV_WALK_GROUP (8..64, 0 = 64) */
+  int group_size;                 /* targets per group for AGRAV_WALK_GROUP (1..64; 0 = by workload, the default:
+                                     64, or 16 for isolated sets below 2^19 particles) */
   double tree_alloc_factor;       /* TreeAllocFactor (init.c:110); unused: node arrays are sized exactly */
   int leaf_bucket;                /* group walk: an opened node with at most this many particles is expanded
                                      straight into particles (0..64, default 8) */
   int eval_potential;             /* EVALPOTENTIAL (Template-Config.sh): 1 = potentials too (default), 0 = forces
                                      only; the FP32 group walk then skips the potential arithmetic */
+  int group_waves;                /* FP32 group walk of one set: wavefronts sharing each group's traversal (1, 2
+                                     or 4; 0 = by workload: isolated sets 4 below 2^17 particles, periodic
+                                     4 below 2^18 and 2 below 2^20, else 1).  More waves cut the latency of
+                                     small sets; results are deterministic for a given value but differ in
+                                     the last bits between values */
 } agrav_params;
 
 /* Per-stage device times (ms) of the last call, named like AREPO's cpu.txt

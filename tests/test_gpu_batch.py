@@ -40,8 +40,9 @@ def _sets(sizes, seed):
 
 
 def _params(agrav, precision, walk, opening=0, theta=0.7, leaf_bucket=8):
+    # one wavefront per group, as the batched walk uses, so single solves compare bitwise
     return agrav.make_params(periodic=False, box_size=0.0, pmgrid=0, soft=(0.01,), precision=precision, walk=walk,
-                             opening=opening, err_tol_theta=theta, leaf_bucket=leaf_bucket)
+                             opening=opening, err_tol_theta=theta, leaf_bucket=leaf_bucket, group_size=64, group_waves=1)
 
 
 def _solve_batch(agrav, p, off, rel, mass, eps, cen, tuning=None):

@@ -307,3 +307,27 @@ def test_tiny_sets(agrav, oracle, n):
     ad, pd = oracle.direct_summation(op, pos, mass)
     np.testing.assert_allclose(r["acc"], ad, rtol=1e-12, atol=1e-14)
     np.testing.assert_allclose(r["pot"], pd, rtol=1e-12)
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_group_walk_wavefront_split(agrav, periodic):
+    """A group's traversal shared by 2 or 4 wavefronts evaluates the same entries as one
+    wavefront (same interaction count); only the summation order differs, and every
+    setting is deterministic."""
+    pos, mass = _clustered(20000, 61, periodic=periodic)
+    kw = dict(periodic=periodic, box_size=1.0 if periodic else 0.0, pmgrid=32 if periodic else 0, soft=(0.002,),
+              precision=agrav.FP32, walk=agrav.WALK_GROUP)
+    res = {}
+    for waves in (1, 2, 4):
+        outs = []
+        for rep in range(2):
+            with agrav.Gravity(agrav.make_params(**kw, group_waves=waves)) as g:
+                g.set_particles(pos, mass)
+                g.compute_grav_accelerations(first_step=True)
+                outs.append((g.get_accelerations(), g.timings()["interactions"]))
+        assert np.array_equal(outs[0][0]["acc"], outs[1][0]["acc"]), waves
+        res[waves] = outs[0]
+    for waves in (2, 4):
+        assert res[waves][1] == res[1][1]
+        np.testing.assert_array_equal(res[waves][0]["nint"], res[1][0]["nint"])
+        assert rel_err(res[waves][0]["acc"], res[1][0]["acc"]).max() < 1e-4
