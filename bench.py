@@ -31,7 +31,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "particles/s & interactions/s, 128^3 TreePM, FP32 vs FP64; p50/p99 latency"
-HBM_PEAK = 8.0e12
+HBM_PEAK = 8.0e12    # MI355X HBM3E, bytes/s
+FP32_PEAK = 157.3e12  # MI355X FP32, flop/s
 
 
 def log(*a):
@@ -392,16 +393,28 @@ def main():
     # interaction + 52 bytes per target (position, mass, softening/OldAcc in, acc + pot out)
     walk_s = tim["walk_kernel"] * 1e-3
     alg_bytes = inter * 16 + n * 52
-    achieved = alg_bytes / walk_s if walk_s > 0 else 0.0
-    # the algorithmic rate exceeds the HBM peak: records are reused from L2 and the LDS tile; the
-    # profiled counters (traffic, VALU issue time) say what actually bounds the kernel
+    alg_rate = alg_bytes / walk_s if walk_s > 0 else 0.0
     prof = profiled_counters()
+    traffic = prof.get("traffic_bytes_per_launch")
+    # Node and particle records are reused from L2 and from the LDS tile by all lanes of a group, so
+    # the algorithmic rate can pass the HBM peak.  Then the rate of the measured HBM traffic (rocprofv3
+    # FETCH_SIZE + WRITE_SIZE of this kernel at this workload) is the roofline figure, flagged
+    # cache-amplified, and the algorithmic rate is reported beside it; a fraction above 1 is never
+    # claimed.  The kernel's real limit is VALU issue, reported from the same profile.
+    amplified = alg_rate > HBM_PEAK and traffic is not None and walk_s > 0
+    achieved = traffic / walk_s if amplified else alg_rate
+    flops = inter * 28  # TreePM short-range monopole, 28 flops per interaction (sqrt and divide as 1)
     out["roofline"] = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                       "frac": achieved / HBM_PEAK, "traffic": prof.get("traffic_bytes_per_launch"),
-                       "kernel": "k_walk_group<float,PM>", "kernel_ms": tim["walk_kernel"], "alg_bytes_per_launch": alg_bytes,
-                       "profiled": {"source": os.path.relpath(PMC_PROFILE, ROOT) if prof else None,
-                                    "l2_hit_rate": prof.get("l2_hit_rate"),
-                                    "valu_issue_fraction_of_kernel_time": prof.get("valu_issue_fraction")}}
+                       "frac": achieved / HBM_PEAK, "traffic": traffic, "cache_amplified": bool(amplified),
+                       "kernel": "k_walk_group<float,PM>", "kernel_ms": tim["walk_kernel"],
+                       "algorithmic": {"bytes_per_launch": alg_bytes, "rate_GBps": alg_rate / 1e9,
+                                       "frac_of_peak": alg_rate / HBM_PEAK},
+                       "secondary": {"fp32_tflops": flops / walk_s / 1e12 if walk_s > 0 else 0.0,
+                                     "fp32_peak_tflops": FP32_PEAK / 1e12,
+                                     "fp32_frac": flops / walk_s / FP32_PEAK if walk_s > 0 else 0.0,
+                                     "valu_issue_fraction_of_kernel_time": prof.get("valu_issue_fraction"),
+                                     "l2_hit_rate": prof.get("l2_hit_rate")},
+                       "profiled": os.path.relpath(PMC_PROFILE, ROOT) if prof else None}
 
     # the same path with EVALPOTENTIAL (potentials summed too)
     rpot, _ = run_path(A, pos, mass, eps, A.FP32, A.WALK_GROUP, 3, 0, old_acc=res32["old_acc"], pot=True)
