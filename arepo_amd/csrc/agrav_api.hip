@@ -34,33 +34,34 @@ agrav_status fail(agrav_ctx *c, agrav_status s, const std::string &m)
 static constexpr int B = 256;
 
 // gravity_force_finalize, accel.c:204-347 (no comoving terms, OmegaLambda = 0)
-__global__ void k_finalize(int64_t n, const int *__restrict__ perm, const double *__restrict__ at, const double *__restrict__ pt,
-                           const int *__restrict__ nint, const double *__restrict__ acc_pm, const double *__restrict__ pot_pm,
-                           int have_pm, int pm_sorted, double G, double pot_const, double *acc, double *pot, int *nint_out,
-                           float *oldacc, float *oldacc_s)
+__global__ void k_finalize(int64_t n, FinalOut f, const double *__restrict__ at, const double *__restrict__ pt,
+                           const int *__restrict__ nint)
 {
   int64_t k = (int64_t)blockIdx.x * B + threadIdx.x;
   if(k >= n)
     return;
-  int i      = perm[k];
-  double a[3] = {at[3 * k], at[3 * k + 1], at[3 * k + 2]};
-  double p[3] = {0, 0, 0}, pp = 0;
-  if(have_pm)
+  finalize_one(f, k, at[3 * k], at[3 * k + 1], at[3 * k + 2], pt[k], nint[k]);
+}
+
+// what finalize writes, with its output buffers allocated
+agrav_status final_out(agrav_ctx *c, FinalOut &f)
+{
+  const int64_t n = c->n;
+  agrav_status st;
+  if((st = c->d_acc.ensure(3 * (n ? n : 1))) || (st = c->d_pot.ensure(n ? n : 1)) || (st = c->d_nint_out.ensure(n ? n : 1)) ||
+     (st = c->d_oldacc.ensure(n ? n : 1)))
+    return st;
+  const bool pm = c->par.pmgrid > 0;
+  double pot_const = 0;
+  if(pm && c->par.periodic)
     {
-      const int64_t q = pm_sorted ? k : (int64_t)i;
-      p[0]            = acc_pm[3 * q];
-      p[1]            = acc_pm[3 * q + 1];
-      p[2]            = acc_pm[3 * q + 2];
-      pp              = pot_pm[q];
+      double asmth = c->par.asmth * c->par.box_size / c->par.pmgrid;
+      double L     = c->par.box_size;
+      pot_const    = c->mass_total * M_PI / (asmth * asmth * L * L * L);  // accel.c:269-274
     }
-  double ox = a[0] + p[0] / G, oy = a[1] + p[1] / G, oz = a[2] + p[2] / G;
-  float oa  = (float)sqrt(ox * ox + oy * oy + oz * oz);  // accel.c:237-255
-  oldacc[i] = oa;
-  oldacc_s[k] = oa;
-  for(int d = 0; d < 3; d++)
-    acc[3 * (int64_t)i + d] = a[d] * G + p[d];  // accel.c:258-265 (+ GravPM in the kick)
-  pot[i]      = (pt[k] + pot_const) * G + pp;   // accel.c:269-286
-  nint_out[i] = nint[k];
+  f = FinalOut{c->d_idx2.p, pm ? c->d_acc_pm.p : nullptr, pm ? c->d_pot_pm.p : nullptr, c->pm_sorted ? 1 : 0, c->par.G,
+               pot_const, c->d_acc.p, c->d_pot.p, c->d_nint_out.p, c->d_oldacc.p, c->d_oldacc_s.p};
+  return AGRAV_OK;
 }
 
 agrav_status finalize(agrav_ctx *c)
@@ -69,30 +70,24 @@ agrav_status finalize(agrav_ctx *c)
     return fail(c, AGRAV_ERR_STATE, "finalize before gravity_tree");
   const int64_t n = c->n;
   agrav_status st;
-  if((st = c->d_acc.ensure(3 * (n ? n : 1))) || (st = c->d_pot.ensure(n ? n : 1)) || (st = c->d_nint_out.ensure(n ? n : 1)) ||
-     (st = c->d_oldacc.ensure(n ? n : 1)))
-    return st;
   const bool pm = c->par.pmgrid > 0;
   if(pm && !c->have_pm)
     return fail(c, AGRAV_ERR_STATE, "finalize with PMGRID before long_range_force");
-  double pot_const = 0;
-  if(pm && c->par.periodic)
+  if(c->final_fused)
     {
-      double asmth = c->par.asmth * c->par.box_size / c->par.pmgrid;
-      double L     = c->par.box_size;
-      pot_const    = c->mass_total * M_PI / (asmth * asmth * L * L * L);  // accel.c:269-274
+      // the walk's epilogue has written the outputs of every target
+      c->final_fused  = false;
+      c->tim.finalize = 0;
+      c->have_oldacc  = true;
+      c->have_final   = true;
+      return AGRAV_OK;
     }
-  if(!pm)
-    {
-      if((st = c->d_acc_pm.ensure(1)) || (st = c->d_pot_pm.ensure(1)))
-        return st;
-    }
+  FinalOut f;
+  if((st = final_out(c, f)) != AGRAV_OK)
+    return st;
   AG_HIP(hipEventRecord(c->ev[0], c->stream));
   if(n > 0)
-    k_finalize<<<(unsigned)((n + B - 1) / B), B, 0, c->stream>>>(n, c->d_idx2.p, c->d_at.p, c->d_pt.p, c->d_nint.p,
-                                                                c->d_acc_pm.p, c->d_pot_pm.p, pm, c->pm_sorted ? 1 : 0, c->par.G, pot_const,
-                                                                c->d_acc.p, c->d_pot.p, c->d_nint_out.p, c->d_oldacc.p,
-                                                                c->d_oldacc_s.p);
+    k_finalize<<<(unsigned)((n + B - 1) / B), B, 0, c->stream>>>(n, f, c->d_at.p, c->d_pt.p, c->d_nint.p);
   AG_HIP(hipGetLastError());
   AG_HIP(hipEventRecord(c->ev[1], c->stream));
   AG_HIP(hipEventSynchronize(c->ev[1]));
@@ -102,6 +97,14 @@ agrav_status finalize(agrav_ctx *c)
   c->have_oldacc  = true;
   c->have_final   = true;
   return AGRAV_OK;
+}
+
+// softening types of a device-resident input must index the softening table
+__global__ void k_check_stype(const uint8_t *__restrict__ st, int64_t n, int nsoft, int *bad)
+{
+  const int64_t i = (int64_t)blockIdx.x * B + threadIdx.x;
+  if(i < n && st[i] >= nsoft)
+    atomicOr(bad, 1);
 }
 
 static void soft_from(agrav_ctx *c, int nsoft, const double *eps)
@@ -407,6 +410,20 @@ static agrav_status set_particles_common(agrav_ctx *c, int64_t n, const double *
           if(stype[i] >= c->soft.nsoft)
             return fail(c, AGRAV_ERR_BAD_PARAM, "softening type out of range");
       AG_HIP(hipMemcpyAsync(c->d_stype_in.p, stype, n, kind, s));
+      if(kind != hipMemcpyHostToDevice && n > 0)
+        {
+          // the same range check on the device copy
+          if((st = c->d_counters.ensure(128)))
+            return st;
+          int bad = 0, *d_bad = c->d_counters.p + 100;
+          AG_HIP(hipMemsetAsync(d_bad, 0, sizeof(int), s));
+          k_check_stype<<<(unsigned)((n + B - 1) / B), B, 0, s>>>(c->d_stype_in.p, n, c->soft.nsoft, d_bad);
+          AG_HIP(hipGetLastError());
+          AG_HIP(hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, s));
+          AG_HIP(hipStreamSynchronize(s));
+          if(bad)
+            return fail(c, AGRAV_ERR_BAD_PARAM, "softening type out of range");
+        }
     }
   // total mass (host sum for host input, device copy-back otherwise), fixed order
   double mt = 0;
@@ -516,24 +533,26 @@ agrav_status agrav_compute_grav_accelerations(agrav_ctx *c, int first_step)
     return AGRAV_ERR_BAD_PARAM;
   if(!c->have_particles)
     return fail(c, AGRAV_ERR_STATE, "compute_grav_accelerations before set_particles");
-  agrav_status st;
   // the PM does not depend on the tree; running it after the build lets it use the
   // Hilbert-ordered particles (the reference calls long_range_force first)
-  if((st = agrav_construct_forcetree(c, nullptr)) != AGRAV_OK)
-    return st;
-  if(c->par.pmgrid > 0 && (st = pm_force(c)) != AGRAV_OK)
-    return st;
+  agrav_status st = agrav_construct_forcetree(c, nullptr);
+  if(st == AGRAV_OK && c->par.pmgrid > 0)
+    st = pm_force(c);
+  // the walks of a full step write the final outputs from their epilogue
+  c->fuse_final = true;
   const bool seed = c->par.type_of_opening_criterion == 1 && first_step && c->par.err_tol_theta > 0;
-  if(seed)
+  if(st == AGRAV_OK && seed)
     {
       // Barnes-Hut pass that seeds OldAcc (accel.c:62-72); the tree of the
       // second gravity() call would be identical, so it is reused.
-      if((st = walk_tree(c, 1)) != AGRAV_OK || (st = finalize(c)) != AGRAV_OK)
-        return st;
+      if((st = walk_tree(c, 1)) == AGRAV_OK)
+        st = finalize(c);
     }
-  if((st = walk_tree(c, c->par.type_of_opening_criterion == 0)) != AGRAV_OK)
-    return st;
-  return finalize(c);
+  if(st == AGRAV_OK && (st = walk_tree(c, c->par.type_of_opening_criterion == 0)) == AGRAV_OK)
+    st = finalize(c);
+  c->fuse_final  = false;
+  c->final_fused = false;
+  return st;
 }
 
 agrav_status agrav_get_accelerations(agrav_ctx *c, double *acc, double *pot, int32_t *nint, float *old_acc)

@@ -143,6 +143,10 @@ struct agrav_ctx
   agrav::PmState pm;
   agrav::DBuf<double> d_acc_pm, d_pot_pm;  // caller order, or sorted order when pm_sorted
   bool pm_sorted = false;
+  // finalize fused into the group walk's epilogue (set by compute_grav_accelerations)
+  bool fuse_final   = false;  // the next group walk may write the final outputs
+  bool final_fused  = false;  // the last walk wrote them for every target
+  agrav::FinalOut fin{};      // what the walk kernel writes (perm null: nothing)
 
   // final results in caller order
   agrav::DBuf<double> d_acc, d_pot;
@@ -175,6 +179,7 @@ agrav_status ewald_forcetest_table(agrav_ctx *c);
 agrav_status forcetest(agrav_ctx *c, int ns, const int *sel, double *acc, double *acc_short, double *pot, double *pot_short);
 agrav_status direct_all(agrav_ctx *c, double *acc, double *pot);
 agrav_status finalize(agrav_ctx *c);
+agrav_status final_out(agrav_ctx *c, FinalOut &f);
 WalkConst walk_const(const agrav_ctx *c, int use_bh);
 BatchView batch_view(const agrav_ctx *c);
 agrav_status batch_groups(agrav_ctx *c);

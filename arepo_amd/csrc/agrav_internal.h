@@ -109,6 +109,45 @@ struct BatchView
   const Domain *dom;   // domain cube per set
 };
 
+// gravity_force_finalize for one target (accel.c:204-347, no comoving terms,
+// OmegaLambda = 0): OldAcc = |a_tree + a_PM / G| (accel.c:237-255), a = a_tree G
+// + a_PM (accel.c:258-265), the periodic TreePM potential constant and G
+// (accel.c:269-286); outputs in caller order.  Used by the finalize kernel and by
+// the group walk's epilogue when the step fuses the two.
+struct FinalOut
+{
+  const int *perm;                // sorted position -> caller index; null: nothing to write
+  const double *acc_pm, *pot_pm;  // PM results, or null without PM
+  int pm_sorted;                  // PM results in sorted (1) or caller (0) order
+  double G, pot_const;
+  double *acc, *pot;
+  int *nint;
+  float *oldacc, *oldacc_s;  // caller order, sorted order
+};
+
+__device__ inline void finalize_one(const FinalOut &f, int64_t k, double ax, double ay, double az, double pt, int ni)
+{
+  const int i = f.perm[k];
+  double p0 = 0, p1 = 0, p2 = 0, pp = 0;
+  if(f.acc_pm)
+    {
+      const int64_t q = f.pm_sorted ? k : (int64_t)i;
+      p0              = f.acc_pm[3 * q];
+      p1              = f.acc_pm[3 * q + 1];
+      p2              = f.acc_pm[3 * q + 2];
+      pp              = f.pot_pm[q];
+    }
+  const double ox = ax + p0 / f.G, oy = ay + p1 / f.G, oz = az + p2 / f.G;
+  const float oa  = (float)sqrt(ox * ox + oy * oy + oz * oz);
+  f.oldacc[i]     = oa;
+  f.oldacc_s[k]   = oa;
+  f.acc[3 * (int64_t)i + 0] = ax * f.G + p0;
+  f.acc[3 * (int64_t)i + 1] = ay * f.G + p1;
+  f.acc[3 * (int64_t)i + 2] = az * f.G + p2;
+  f.pot[i]                  = (pt + f.pot_const) * f.G + pp;
+  f.nint[i]                 = ni;
+}
+
 // hip error helper
 #define AG_HIP(call)                                                                                \
   do                                                                                                \

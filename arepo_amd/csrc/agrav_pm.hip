@@ -35,16 +35,24 @@ struct Cic
   double d[3];
 };
 
-__device__ inline Cic cic_of(const double *p, int G, double L)
+// A position still outside [0, L) after one wrap (or not finite) is reported
+// through `ok` and mapped to cell 0, so that no mesh access leaves the mesh.
+__device__ inline Cic cic_of(const double *p, int G, double L, bool &ok)
 {
   // slab = (int)(x*PMGRID/L), dx = x*PMGRID/L - slab, periodic neighbour (pm_periodic.c:1051-1107)
   Cic c;
   const double to_slab = G / L;
+  ok                   = true;
   for(int k = 0; k < 3; k++)
     {
       double x = wrap_pm(p[k], L);
-      c.s[k]   = (int)(to_slab * x);
-      c.d[k]   = to_slab * x - c.s[k];
+      if(!(x >= 0 && x < L))
+        {
+          ok = false;
+          x  = 0;
+        }
+      c.s[k] = (int)(to_slab * x);
+      c.d[k] = to_slab * x - c.s[k];
       if(c.s[k] >= G)
         c.s[k] -= G;
       c.ss[k] = c.s[k] + 1;
@@ -53,14 +61,25 @@ __device__ inline Cic cic_of(const double *p, int G, double L)
     }
   return c;
 }
+__device__ inline Cic cic_of(const double *p, int G, double L)
+{
+  bool ok;
+  return cic_of(p, G, L, ok);
+}
 
 __global__ void k_cic_deposit(const double *__restrict__ pos, const double *__restrict__ mass, int64_t n, int G, double L,
-                              double scale, unsigned long long *rho)
+                              double scale, unsigned long long *rho, int *bad)
 {
   int64_t i = (int64_t)blockIdx.x * B + threadIdx.x;
   if(i >= n)
     return;
-  Cic c              = cic_of(&pos[3 * i], G, L);
+  bool ok;
+  Cic c              = cic_of(&pos[3 * i], G, L, ok);
+  if(!ok)
+    {
+      atomicOr(bad, 1);
+      return;
+    }
   double m           = mass[i] * scale;
   const double *d    = c.d;
 #define FI(x, y, z) (((size_t)(x)*G + (y)) * G + (z))
@@ -309,7 +328,8 @@ __global__ void k_cic_readout(const double *__restrict__ pos, int64_t n, int G, 
   if(i >= n)
     return;
   double f0, f1, f2, pp;
-  cic_gather4<R>(cic_of(&pos[3 * i], G, L), G, mesh, f0, f1, f2, pp);
+  bool ok;  // positions outside the box were reported by the deposit; read cell 0 instead
+  cic_gather4<R>(cic_of(&pos[3 * i], G, L, ok), G, mesh, f0, f1, f2, pp);
   pot_pm[i]         = pp * fac_pot;
   acc_pm[3 * i + 0] = f0;
   acc_pm[3 * i + 1] = f1;
@@ -425,8 +445,16 @@ template <typename R> static agrav_status pm_run(agrav_ctx *c)
   const bool sorted = c->have_tree;
   if(n > 0 && sorted)
     k_cic_deposit_sorted<<<nb(n), B, 0, s>>>(c->d_px.p, c->d_py.p, c->d_pz.p, c->d_pm.p, n, G, L, scale, p.rho_fix.p);
-  else if(n > 0)
-    k_cic_deposit<<<nb(n), B, 0, s>>>(c->d_pos_in.p, c->d_mass_in.p, n, G, L, scale, p.rho_fix.p);
+  int *d_bad = nullptr;
+  if(n > 0 && !sorted)
+    {
+      // unsorted input has not passed the tree build's box check: the deposit checks it
+      if((st = c->d_counters.ensure(128)))
+        return st;
+      d_bad = c->d_counters.p + 101;
+      AG_HIP(hipMemsetAsync(d_bad, 0, sizeof(int), s));
+      k_cic_deposit<<<nb(n), B, 0, s>>>(c->d_pos_in.p, c->d_mass_in.p, n, G, L, scale, p.rho_fix.p, d_bad);
+    }
   k_fix_to_real<R><<<nb(g3), B, 0, s>>>(p.rho_fix.p, g3, scale, (R *)p.real.p);
   AG_HIP(hipGetLastError());
   void *in[1]  = {p.real.p};
@@ -449,6 +477,14 @@ template <typename R> static agrav_status pm_run(agrav_ctx *c)
   else if(n > 0)
     k_cic_readout<R><<<nb(n), B, 0, s>>>(c->d_pos_in.p, n, G, L, mesh, fac_pot, c->d_acc_pm.p, c->d_pot_pm.p);
   AG_HIP(hipGetLastError());
+  if(d_bad)
+    {
+      int bad = 0;
+      AG_HIP(hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, s));
+      AG_HIP(hipStreamSynchronize(s));
+      if(bad)
+        return fail(c, AGRAV_ERR_OUT_OF_BOX, "particle outside the periodic box after one wrap");
+    }
   c->pm_sorted = sorted;
   return AGRAV_OK;
 }

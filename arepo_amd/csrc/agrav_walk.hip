@@ -429,6 +429,8 @@ __device__ inline void tile_compute_generic(const Vec4<T> *__restrict__ s_list, 
 // entry 127 = {f_127, 0}, so an index clamped to 127 reads in bounds, then the
 // potential table p_0..p_127 and a copy of p_127.  Zero-mass entries never
 // enter the tile, so validity is only the table test.
+static constexpr float kTinyR2 = 1e-30f;
+
 template <bool PM, bool EWALD, bool MSOFT, bool POT>
 __device__ inline void tile_compute_f32(const Vec4<float> *__restrict__ s_list, const float *__restrict__ s_h, int nl_in, float tx,
                                         float ty, float tz, float hi, const float *__restrict__ s_tab, const WalkConst &wc,
@@ -452,8 +454,10 @@ __device__ inline void tile_compute_f32(const Vec4<float> *__restrict__ s_list, 
           dy = nearest_image(dy, box, half);
           dz = nearest_image(dz, box, half);
         }
-      const float r2   = dx * dx + dy * dy + dz * dz;
-      const float rinv = __builtin_amdgcn_rsqf(fmaxf(r2, 1e-30f));
+      // the 1e-30 term keeps the reciprocal square root finite for a pair at distance
+      // zero (the target itself); any other pair has r2 >= (2^-32 box)^2, far above it
+      const float r2   = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, kTinyR2)));
+      const float rinv = __builtin_amdgcn_rsqf(r2);
       const float r    = r2 * rinv;
       float hmax = hi, hinv = hi_inv;
       if(MSOFT)
@@ -508,7 +512,7 @@ __device__ inline void tile_compute_f32(const Vec4<float> *__restrict__ s_list, 
               fac = h3 * ((float)SF8 + (float)SF9 * u + (float)SF10 * u2 + (float)SF11 * u3 + (float)SF12 * ui * ui * ui);
               wp  = hinv * ((float)SF13 + (float)SF14 * ui + u2 * ((float)SF1 + u * ((float)SF15 + u * ((float)SF16 + (float)SF17 * u))));
             }
-          if(PM && r2 > 0.f)
+          if(PM && r2 > kTinyR2)
             {
               fac += rinv * ff;
               wp += rinv * (fpt + 1.0f);
@@ -994,8 +998,8 @@ __device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, co
     return false;
   __builtin_amdgcn_wave_barrier();
   n_tiled += nl;
-          tile_compute<T, PM, EWALD, MSOFT>(s_list, s_h, nl, tx, ty, tz, hi, s_tab, wc, asmthfac, asmthinv2, ewald_f, ewald_d, ax, ay,
-                                    az, pp, cax, cay, caz, cpp, nin);
+  tile_compute<T, PM, EWALD, MSOFT>(s_list, s_h, nl, tx, ty, tz, hi, s_tab, wc, asmthfac, asmthinv2, ewald_f, ewald_d, ax, ay, az,
+                                    pp, cax, cay, caz, cpp, nin);
 
   out = GroupAcc<T>{ax, ay, az, pp, cax, cay, caz, cpp, nin};
   unsigned long long v = has ? (unsigned long long)nin : 0ull;
@@ -1035,7 +1039,7 @@ __global__ __launch_bounds__(64 * W) void k_walk_group(TreeView tv, WalkConst wc
                                                        const float *__restrict__ oldacc_s, const float *__restrict__ tabs,
                                                        const float *__restrict__ ewald_f, const double *__restrict__ ewald_d, int gs,
                                                        double *at, double *pt, int *nint, int *failed, int *nfailed,
-                                                       unsigned long long *isum)
+                                                       unsigned long long *isum, FinalOut fin)
 {
   __shared__ WalkLds<T, MSOFT> lds[W];
   __shared__ int s_fail;
@@ -1056,11 +1060,15 @@ __global__ __launch_bounds__(64 * W) void k_walk_group(TreeView tv, WalkConst wc
     {
       if(ok && lane < np)
         {
-          at[3 * pi + 0] = (double)acc.ax - (double)acc.cax;
-          at[3 * pi + 1] = (double)acc.ay - (double)acc.cay;
-          at[3 * pi + 2] = (double)acc.az - (double)acc.caz;
-          pt[pi]         = (double)acc.pp - (double)acc.cpp;
+          const double a0 = (double)acc.ax - (double)acc.cax, a1 = (double)acc.ay - (double)acc.cay,
+                       a2 = (double)acc.az - (double)acc.caz, p = (double)acc.pp - (double)acc.cpp;
+          at[3 * pi + 0] = a0;
+          at[3 * pi + 1] = a1;
+          at[3 * pi + 2] = a2;
+          pt[pi]         = p;
           nint[pi]       = acc.nin;
+          if(fin.perm)
+            finalize_one(fin, pi, a0, a1, a2, p, acc.nin);
         }
       else if(!ok && lane == 0)
         {
@@ -1105,6 +1113,8 @@ __global__ __launch_bounds__(64 * W) void k_walk_group(TreeView tv, WalkConst wc
           at[3 * pi + 2] = s[2];
           pt[pi]         = s[3];
           nint[pi]       = ni;
+          if(fin.perm)
+            finalize_one(fin, pi, s[0], s[1], s[2], s[3], ni);
         }
     }
 }
@@ -1193,7 +1203,7 @@ static void launch_group_w(int ngroups, hipStream_t s, const TreeView &tv, const
 {
   k_walk_group<T, PM, EW, MS, W><<<ngroups, 64 * W, 0, s>>>(tv, wc, c->d_soft.p, c->d_oldacc_s.p, c->d_tab.p, c->d_ewald_f.p,
                                                             c->d_ewald.p, gs, c->d_at.p, c->d_pt.p, c->d_nint.p, c->d_failed.p,
-                                                            c->d_counters.p + 48, c->d_isum.p);
+                                                            c->d_counters.p + 48, c->d_isum.p, c->fin);
 }
 
 template <typename T, bool PM, bool EW, bool MS>
@@ -1360,6 +1370,8 @@ agrav_status walk_tree(agrav_ctx *c, int use_bh)
         return st;
     }
   c->tim.walk_retries = 0;
+  c->final_fused      = false;
+  c->fin.perm         = nullptr;
   if(c->batch_nsets > 0 && c->par.walk != AGRAV_WALK_PLAIN)
     {
       // batch of sets: one persistent launch over the group queue
@@ -1394,6 +1406,10 @@ agrav_status walk_tree(agrav_ctx *c, int use_bh)
       if(waves <= 0)
         waves = isolated ? (n < (1 << 17) ? 4 : 1) : (n < (1 << 18) ? 4 : (n < (1 << 20) ? 2 : 1));
       c->tim_group_waves = c->par.precision == AGRAV_FP32 ? waves : 1;
+      // within a full step the walk's epilogue also writes the final outputs (the PM
+      // has run, in sorted order); the finalize kernel then has nothing left to do
+      if(c->fuse_final && (!pm || (c->have_pm && c->pm_sorted)) && (st = final_out(c, c->fin)) != AGRAV_OK)
+        return st;
       if(c->par.precision == AGRAV_FP32)
         dispatch_group<float>(pm, ew, ms, ngroups, s, tv, wc, c, gs, waves);
       else
@@ -1410,6 +1426,9 @@ agrav_status walk_tree(agrav_ctx *c, int use_bh)
           launch_plain(pm, ew, nf * gs, c->d_failed.p, gs, s, tv, wc, c, BatchView{});
           AG_HIP(hipGetLastError());
         }
+      // re-walked targets still need the finalize kernel
+      c->final_fused = c->fin.perm != nullptr && nf == 0;
+      c->fin.perm    = nullptr;
     }
   AG_HIP(hipEventRecord(c->ev[4], s));
   unsigned long long cnt[3] = {0, 0, 0};

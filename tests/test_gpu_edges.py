@@ -1,0 +1,222 @@
+"""GPU edge cases of the gravity step, through the C ABI.
+
+* empty particle sets (a rank with no particles is legal in the reference);
+* coincident particles: AREPO splits cells down to level 30 and then puts
+  coincident points into free slots (forcetree.c:861-877); this library keeps
+  a cell still shared at level 21 as one multi-particle leaf.  Accelerations
+  and potentials equal the oracle's anyway (all members of such a cell sit at
+  one point, so a deep node and its particles contribute the same pair terms);
+  only interaction counts may differ;
+* several softening types (the node softening test, forcetree_walk.c:294-320);
+* positions one box outside a periodic box, wrapped once as forcetree.c:404-409;
+* device-resident input (agrav_set_particles_device) equal to host input.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _clumped(n, seed, periodic):
+    """Uniform particles plus exact duplicates: one clump of 40 copies of a point and
+    20 pairs of coincident particles."""
+    g = np.random.Generator(np.random.Philox(seed))
+    pos = g.random((n, 3)) * (1.0 if periodic else 0.8) + (0.0 if periodic else 0.1)
+    pos[1:41] = pos[0]
+    for k in range(20):
+        pos[100 + 2 * k + 1] = pos[100 + 2 * k]
+    mass = g.uniform(0.5, 1.5, n) / n
+    return pos, mass
+
+
+@pytest.mark.parametrize("case", ["isolated", "treepm"])
+def test_empty_set(agrav, case):
+    kw = dict(periodic=False, box_size=0.0, pmgrid=0) if case == "isolated" else dict(periodic=True, box_size=1.0,
+                                                                                    pmgrid=32)
+    for prec, walk in ((agrav.FP64, agrav.WALK_PLAIN), (agrav.FP32, agrav.WALK_GROUP)):
+        with agrav.Gravity(agrav.make_params(**kw, soft=(0.01,), precision=prec, walk=walk)) as g:
+            g.set_particles(np.zeros((0, 3)), np.zeros(0))
+            g.compute_grav_accelerations(first_step=True)
+            g.compute_grav_accelerations(first_step=False)
+            r = g.get_accelerations()
+            assert r["acc"].shape == (0, 3) and r["pot"].shape == (0,)
+            assert g.timings()["interactions"] == 0
+            # the context stays usable for a non-empty set afterwards
+            pos = np.array([[0.25, 0.5, 0.5], [0.75, 0.5, 0.5]])
+            g.set_particles(pos, np.ones(2))
+            g.compute_grav_accelerations(first_step=True)
+            assert np.all(np.isfinite(g.get_accelerations()["acc"]))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_coincident_particles_direct_limit(agrav, oracle, precision):
+    """With a vanishing opening angle every node is opened and the walk is the direct sum,
+    whatever the leaf layout of the duplicates."""
+    pos, mass = _clumped(3000, 5, periodic=False)
+    prec = agrav.FP64 if precision == "fp64" else agrav.FP32
+    ad, pd = oracle.direct_summation(oracle.make_params(periodic=False, box_size=0.0, soft=(0.004,)), pos, mass)
+    for walk in (agrav.WALK_PLAIN, agrav.WALK_GROUP) if precision == "fp64" else (agrav.WALK_GROUP,):
+        gp = agrav.make_params(periodic=False, box_size=0.0, soft=(0.004,), opening=0, err_tol_theta=1e-4,
+                               precision=prec, walk=walk)
+        with agrav.Gravity(gp) as g:
+            g.set_particles(pos, mass)
+            g.compute_grav_accelerations()
+            r = g.get_accelerations()
+        tol = 1e-10 if precision == "fp64" else 2e-5
+        assert np.percentile(rel_err(r["acc"], ad), 99.9) < tol, walk
+        np.testing.assert_allclose(r["pot"], pd, rtol=tol)
+        # each clump member feels the same force (they sit at one point)
+        assert rel_err(r["acc"][1:41], np.repeat(r["acc"][:1], 40, axis=0)).max() < (1e-12 if precision == "fp64" else 1e-6)
+
+
+@pytest.mark.parametrize("case", ["isolated_rel", "treepm_rel"])
+def test_coincident_particles_match_oracle(agrav, oracle, case):
+    periodic = case == "treepm_rel"
+    pos, mass = _clumped(6000, 7, periodic)
+    kw = dict(periodic=periodic, box_size=1.0 if periodic else 0.0, pmgrid=32 if periodic else 0, soft=(0.003,),
+              err_tol_force_acc=0.0025, opening=1)
+    gp = agrav.make_params(**kw, precision=agrav.FP64, walk=agrav.WALK_PLAIN)
+    with agrav.Gravity(gp) as g:
+        g.set_particles(pos, mass)
+        g.compute_grav_accelerations(first_step=True)
+        r = g.get_accelerations()
+    o = oracle.compute_grav_accelerations(oracle.make_params(**kw), pos, mass, first_step=True)
+    e = rel_err(r["acc"], o["acc"])
+    assert np.percentile(e, 99.9) < 1e-9, e.max()
+    np.testing.assert_allclose(r["pot"], o["pot"], rtol=1e-9, atol=1e-12 * np.abs(o["pot"]).max())
+
+
+@pytest.mark.parametrize("case", ["isolated_rel", "treepm_rel"])
+def test_several_softening_types(agrav, oracle, case):
+    periodic = case == "treepm_rel"
+    g0 = np.random.Generator(np.random.Philox(41))
+    n = 8000
+    pos = g0.random((n, 3))
+    # a dense clump so that softened pairs and node softening tests occur
+    pos[:2000] = 0.5 + 0.02 * (g0.random((2000, 3)) - 0.5)
+    mass = np.full(n, 1.0 / n)
+    stype = g0.integers(0, 3, n).astype(np.uint8)
+    soft = (0.0005, 0.002, 0.006)
+    kw = dict(periodic=periodic, box_size=1.0 if periodic else 0.0, pmgrid=32 if periodic else 0, soft=soft,
+              err_tol_force_acc=0.0025, opening=1)
+    out = {}
+    for name, prec, walk in (("fp64_plain", agrav.FP64, agrav.WALK_PLAIN), ("fp32_group", agrav.FP32, agrav.WALK_GROUP),
+                             ("fp64_group", agrav.FP64, agrav.WALK_GROUP)):
+        with agrav.Gravity(agrav.make_params(**kw, precision=prec, walk=walk)) as g:
+            g.set_particles(pos, mass, softtype=stype)
+            g.compute_grav_accelerations(first_step=True)
+            out[name] = g.get_accelerations()
+    op = oracle.make_params(**kw)
+    o = oracle.compute_grav_accelerations(op, pos, mass, stype=stype, first_step=True)
+    e = rel_err(out["fp64_plain"]["acc"], o["acc"])
+    assert np.percentile(e, 99.9) < 1e-9, e.max()
+    assert np.mean(out["fp64_plain"]["nint"] == o["nint"]) > 0.999
+    # group walks: error against the direct sum no worse than the reference walk's
+    if periodic:
+        with agrav.Gravity(agrav.make_params(**kw)) as g:
+            g.set_particles(pos, mass, softtype=stype)
+            sel = np.arange(0, n, 4, dtype=np.int32)
+            ref = g.gravity_forcetest(sel)["acc"]
+    else:
+        sel = np.arange(n)
+        ref = oracle.direct_summation(op, pos, mass, stype=stype)[0]
+    eo = rel_err(o["acc"][sel], ref)
+    for name in ("fp32_group", "fp64_group"):
+        eg = rel_err(out[name]["acc"][sel], ref)
+        assert np.median(eg) <= 1.05 * np.median(eo) + 2e-6, (name, np.median(eg), np.median(eo))
+        assert np.percentile(eg, 99) <= 1.05 * np.percentile(eo, 99) + 2e-6, (name, np.percentile(eg, 99))
+
+
+def test_periodic_positions_wrapped_once(agrav, oracle):
+    g0 = np.random.Generator(np.random.Philox(43))
+    n = 5000
+    pos = g0.random((n, 3))
+    mass = np.full(n, 1.0 / n)
+    shifted = pos.copy()
+    shifted[::3, 0] -= 1.0
+    shifted[1::3, 1] += 1.0
+    kw = dict(periodic=True, box_size=1.0, pmgrid=32, soft=(0.002,), err_tol_force_acc=0.0025, opening=1)
+    res = []
+    for p in (pos, shifted):
+        with agrav.Gravity(agrav.make_params(**kw, precision=agrav.FP64, walk=agrav.WALK_PLAIN)) as g:
+            g.set_particles(p, mass)
+            g.compute_grav_accelerations(first_step=True)
+            res.append(g.get_accelerations())
+    o = oracle.compute_grav_accelerations(oracle.make_params(**kw), shifted, mass, first_step=True)
+    assert np.percentile(rel_err(res[1]["acc"], o["acc"]), 99.9) < 1e-9
+    # the same particles, up to the rounding of one box shift
+    assert np.percentile(rel_err(res[1]["acc"], res[0]["acc"]), 99) < 1e-6
+
+
+class _DeviceBuffers:
+    """Device copies of host arrays through the HIP runtime (hipMalloc / hipMemcpy)."""
+
+    def __init__(self):
+        import ctypes as C
+        self.C = C
+        self.hip = C.CDLL("libamdhip64.so")
+        self.ptrs = []
+
+    def put(self, a):
+        C = self.C
+        a = np.ascontiguousarray(a)
+        p = C.c_void_p()
+        assert self.hip.hipMalloc(C.byref(p), C.c_size_t(max(a.nbytes, 1))) == 0
+        self.ptrs.append(p)
+        assert self.hip.hipMemcpy(p, C.c_void_p(a.ctypes.data), C.c_size_t(a.nbytes), 1) == 0  # host to device
+        return p.value
+
+    def free(self):
+        for p in self.ptrs:
+            self.hip.hipFree(p)
+        self.ptrs = []
+
+
+def test_device_input_equals_host_input(agrav):
+    pos, mass = _clumped(4000, 9, periodic=True)
+    stype = np.random.Generator(np.random.Philox(3)).integers(0, 2, len(pos)).astype(np.uint8)
+    p = agrav.make_params(periodic=True, box_size=1.0, pmgrid=32, soft=(0.002, 0.004), precision=agrav.FP32,
+                          walk=agrav.WALK_GROUP)
+    with agrav.Gravity(p) as g:
+        g.set_particles(pos, mass, softtype=stype)
+        g.compute_grav_accelerations(first_step=True)
+        host = g.get_accelerations()
+    dev_buf = _DeviceBuffers()
+    try:
+        dpos, dmass, dst = dev_buf.put(pos.astype(np.float64)), dev_buf.put(mass.astype(np.float64)), dev_buf.put(stype)
+        bad = stype.copy()
+        bad[17] = 2
+        dbad = dev_buf.put(bad)
+        with agrav.Gravity(p) as g:
+            g.set_particles_device(len(pos), dpos, dmass, dst)
+            g.compute_grav_accelerations(first_step=True)
+            dev = g.get_accelerations()
+            assert np.array_equal(dev["acc"], host["acc"]) and np.array_equal(dev["pot"], host["pot"])
+            # a softening type beyond the table is rejected on the device copy too
+            with pytest.raises(agrav.AgravError) as ei:
+                g.set_particles_device(len(pos), dpos, dmass, dbad)
+            assert ei.value.code == agrav.ERR_BAD_PARAM
+    finally:
+        dev_buf.free()
+
+
+@pytest.mark.parametrize("bad", [2.5, float("nan")])
+def test_pm_without_tree_rejects_positions_outside_box(agrav, bad):
+    """The PM may run before any tree build (long_range_force right after the particle
+    load); positions still outside the box after one wrap are reported, not deposited."""
+    p = agrav.make_params(periodic=True, box_size=1.0, pmgrid=16, soft=(0.01,))
+    with agrav.Gravity(p) as g:
+        g.set_particles(np.array([[0.5, 0.5, 0.5], [bad, 0.1, 0.1]]), np.ones(2))
+        with pytest.raises(agrav.AgravError) as ei:
+            g.long_range_force()
+        assert ei.value.code == agrav.ERR_OUT_OF_BOX
+        with pytest.raises(agrav.AgravError) as ei:
+            g.construct_forcetree()
+        assert ei.value.code == agrav.ERR_OUT_OF_BOX
+        # one box outside is wrapped and accepted
+        g.set_particles(np.array([[0.5, 0.5, 0.5], [-0.75, 1.1, 0.1]]), np.ones(2))
+        g.long_range_force()
+        pm = g.get_components(tree=False, pm=True)["acc_pm"]
+        assert np.all(np.isfinite(pm))
