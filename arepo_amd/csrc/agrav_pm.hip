@@ -138,12 +138,24 @@ __global__ __launch_bounds__(B) void k_cic_deposit_sorted(const double *__restri
       s_hi[threadIdx.x] = -(1 << 30);
     }
   __syncthreads();
-  if(has)
-    for(int k = 0; k < 3; k++)
-      {
-        atomicMin(&s_lo[k], off[k]);
-        atomicMax(&s_hi[k], off[k]);
-      }
+  // the block's footprint: wave reductions, then one LDS atomic per wave (atomics of
+  // all lanes on the same word serialise)
+  const int lane = threadIdx.x & 63;
+  for(int k = 0; k < 3; k++)
+    {
+      int lo = has ? off[k] : (1 << 30), hi = has ? off[k] : -(1 << 30);
+#pragma unroll
+      for(int o = 32; o > 0; o >>= 1)
+        {
+          lo = min(lo, __shfl_xor(lo, o, 64));
+          hi = max(hi, __shfl_xor(hi, o, 64));
+        }
+      if(lane == 0)
+        {
+          atomicMin(&s_lo[k], lo);
+          atomicMax(&s_hi[k], hi);
+        }
+    }
   __syncthreads();
   const int ex = s_hi[0] - s_lo[0] + 2, ey = s_hi[1] - s_lo[1] + 2, ez = s_hi[2] - s_lo[2] + 2;
   const bool tiled = (int64_t)ex * ey * ez <= kTile && ex <= G && ey <= G && ez <= G;
