@@ -666,6 +666,9 @@ agrav_status agrav_get_timings(agrav_ctx *c, agrav_timings *t)
 {
   if(!c || !t)
     return AGRAV_ERR_BAD_PARAM;
+  agrav_status st = pm_time_settle(c);
+  if(st != AGRAV_OK)
+    return st;
   *t = c->tim;
   return AGRAV_OK;
 }

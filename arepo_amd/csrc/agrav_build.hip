@@ -854,14 +854,16 @@ agrav_status build_tree(agrav_ctx *c)
   if((st = c->d_counters.ensure(128)) != AGRAV_OK)
     return st;
   int *d_total = c->d_counters.p + 12, *d_perr = c->d_counters.p + 13, *d_lvlcnt = c->d_counters.p + 14;
-  AG_HIP(hipMemsetAsync(d_total, 0, 24 * sizeof(int), s));
+  // one upload sets the extent's start values and clears every other build counter
+  int cinit[86] = {};
+  unsigned long long *d_mm = (unsigned long long *)c->d_counters.p;  // 6 x u64 = 12 ints
+  for(int k = 0; k < 6; k++)
+    cinit[k] = -1;  // the three minima start at all ones, the maxima at zero
+  AG_HIP(hipMemcpyAsync(c->d_counters.p, cinit, sizeof(cinit), hipMemcpyHostToDevice, s));
 
   // 1. domain (one cube, or one per set of a batch)
   const int nsets = c->batch_nsets;
   BuildGeo geo{c->d_dom.p, nullptr, nullptr, 0, AG_KEY_LEVELS};
-  unsigned long long mm_init[6] = {~0ull, ~0ull, ~0ull, 0, 0, 0};
-  unsigned long long *d_mm      = (unsigned long long *)c->d_counters.p;  // 6 x u64 = 12 ints
-  AG_HIP(hipMemcpyAsync(d_mm, mm_init, sizeof(mm_init), hipMemcpyHostToDevice, s));
   if(!c->par.periodic && nsets == 0)
     {
       unsigned g = nblk(n);
@@ -966,8 +968,7 @@ agrav_status build_tree(agrav_ctx *c)
   //    levels share one single-workgroup launch
   if(scan_temp_bytes(M) + 4096 > c->d_temp.cap && (st = c->d_temp.ensure(scan_temp_bytes(M) + 4096)) != AGRAV_OK)
     return st;
-  int *d_berr = c->d_counters.p + 50;
-  AG_HIP(hipMemsetAsync(d_berr, 0, sizeof(int), s));
+  int *d_berr = c->d_counters.p + 50;  // cleared by the upload at the start
   k_count_children<<<nblk(M), B, 0, s>>>(M, N, c->d_nextnode.p, c->d_sibling.p, c->d_pnext.p, c->d_ccnt.p, N + M, d_berr);
   AG_HIP(hipGetLastError());
   if((st = exclusive_scan_i32(c->d_ccnt.p, c->d_cbeg.p, M, nullptr, c->d_temp.p, d_perr, s)) != AGRAV_OK)
@@ -983,8 +984,7 @@ agrav_status build_tree(agrav_ctx *c)
   if(off[AG_KEY_LEVELS + 1] != M)
     return fail(c, AGRAV_ERR_CAPACITY, "inconsistent octree level counts");
   AG_HIP(hipMemcpyAsync(c->d_lvloff.p, off, (AG_KEY_LEVELS + 2) * sizeof(int), hipMemcpyHostToDevice, s));
-  int *d_fill = c->d_counters.p + 64;  // 22 ints
-  AG_HIP(hipMemsetAsync(d_fill, 0, (AG_KEY_LEVELS + 1) * sizeof(int), s));
+  int *d_fill = c->d_counters.p + 64;  // 22 ints, cleared by the upload at the start
   k_level_scatter<<<(unsigned)((M + kScatterChunk - 1) / kScatterChunk), B, 0, s>>>(M, c->d_nlevel.p, c->d_lvloff.p, d_fill,
                                                                                       c->d_lvlnodes.p);
   AG_HIP(hipGetLastError());
@@ -1023,11 +1023,11 @@ agrav_status build_tree(agrav_ctx *c)
   AG_HIP(hipGetLastError());
 
   // out-of-box check (forcetree.c:432-438 aborts; here it is an error code)
-  int berr = 0, perr = 0;
-  AG_HIP(hipMemcpyAsync(&perr, d_perr, sizeof(int), hipMemcpyDeviceToHost, s));
+  int cnow[64];
+  AG_HIP(hipMemcpyAsync(cnow, c->d_counters.p, sizeof(cnow), hipMemcpyDeviceToHost, s));
   AG_HIP(hipMemcpyAsync(&c->h_dom, c->d_dom.p, sizeof(Domain), hipMemcpyDeviceToHost, s));
-  AG_HIP(hipMemcpyAsync(&berr, d_berr, sizeof(int), hipMemcpyDeviceToHost, s));
   AG_HIP(hipStreamSynchronize(s));
+  const int perr = cnow[13], berr = cnow[50];
   if(berr)
     return fail(c, AGRAV_ERR_CAPACITY, "octree threading inconsistent (moment pass stopped)");
   if(perr)

@@ -143,6 +143,7 @@ struct agrav_ctx
   agrav::PmState pm;
   agrav::DBuf<double> d_acc_pm, d_pot_pm;  // caller order, or sorted order when pm_sorted
   bool pm_sorted = false;
+  bool pm_time_pending = false;  // the last PM's events are recorded, its time not yet read
   // finalize fused into the group walk's epilogue (set by compute_grav_accelerations)
   bool fuse_final   = false;  // the next group walk may write the final outputs
   bool final_fused  = false;  // the last walk wrote them for every target
@@ -174,6 +175,7 @@ agrav_status walk_tree(agrav_ctx *c, int use_bh);
 agrav_status pm_init(agrav_ctx *c);
 void pm_free(agrav_ctx *c);
 agrav_status pm_force(agrav_ctx *c);
+agrav_status pm_time_settle(agrav_ctx *c);
 agrav_status ewald_tree_table(agrav_ctx *c);
 agrav_status ewald_forcetest_table(agrav_ctx *c);
 agrav_status forcetest(agrav_ctx *c, int ns, const int *sel, double *acc, double *acc_short, double *pot, double *pot_short);

@@ -501,23 +501,34 @@ template <typename R> static agrav_status pm_run(agrav_ctx *c)
   return AGRAV_OK;
 }
 
+// the PM's duration is read from its events once they have passed, so that the
+// host does not wait for the PM before it queues the walk
+agrav_status pm_time_settle(agrav_ctx *c)
+{
+  if(!c->pm_time_pending)
+    return AGRAV_OK;
+  AG_HIP(hipEventSynchronize(c->ev[6]));
+  float ms = 0;
+  AG_HIP(hipEventElapsedTime(&ms, c->ev[5], c->ev[6]));
+  c->tim.pm_grav      = ms;
+  c->pm_time_pending = false;
+  return AGRAV_OK;
+}
+
 agrav_status pm_force(agrav_ctx *c)
 {
   if(c->par.pmgrid <= 0)
     return AGRAV_OK;
   agrav_status st = pm_init(c);
-  if(st != AGRAV_OK)
+  if(st != AGRAV_OK || (st = pm_time_settle(c)) != AGRAV_OK)
     return st;
   AG_HIP(hipEventRecord(c->ev[5], c->stream));
   st = (c->par.precision == AGRAV_FP32) ? pm_run<float>(c) : pm_run<double>(c);
   if(st != AGRAV_OK)
     return st;
   AG_HIP(hipEventRecord(c->ev[6], c->stream));
-  AG_HIP(hipEventSynchronize(c->ev[6]));
-  float ms = 0;
-  AG_HIP(hipEventElapsedTime(&ms, c->ev[5], c->ev[6]));
-  c->tim.pm_grav = ms;
-  c->have_pm     = true;
+  c->pm_time_pending = true;
+  c->have_pm         = true;
   return AGRAV_OK;
 }
 

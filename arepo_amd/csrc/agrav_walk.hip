@@ -1203,7 +1203,7 @@ static void launch_group_w(int ngroups, hipStream_t s, const TreeView &tv, const
 {
   k_walk_group<T, PM, EW, MS, W><<<ngroups, 64 * W, 0, s>>>(tv, wc, c->d_soft.p, c->d_oldacc_s.p, c->d_tab.p, c->d_ewald_f.p,
                                                             c->d_ewald.p, gs, c->d_at.p, c->d_pt.p, c->d_nint.p, c->d_failed.p,
-                                                            c->d_counters.p + 48, c->d_isum.p, c->fin);
+                                                            reinterpret_cast<int *>(c->d_isum.p + 3), c->d_isum.p, c->fin);
 }
 
 template <typename T, bool PM, bool EW, bool MS>
@@ -1370,6 +1370,8 @@ agrav_status walk_tree(agrav_ctx *c, int use_bh)
         return st;
     }
   c->tim.walk_retries = 0;
+  unsigned long long cnt[4] = {0, 0, 0, 0};
+  bool have_cnt       = false;  // the counters are read already (group walk, no re-walk)
   c->final_fused      = false;
   c->fin.perm         = nullptr;
   if(c->batch_nsets > 0 && c->par.walk != AGRAV_WALK_PLAIN)
@@ -1400,7 +1402,6 @@ agrav_status walk_tree(agrav_ctx *c, int use_bh)
       int ngroups = (int)((n + gs - 1) / gs);
       if((st = c->d_failed.ensure(ngroups)) != AGRAV_OK)
         return st;
-      AG_HIP(hipMemsetAsync(c->d_counters.p + 48, 0, sizeof(int), s));
       // wavefronts per group: small sets have too few groups to fill the device
       int waves = c->par.group_waves;
       if(waves <= 0)
@@ -1416,10 +1417,12 @@ agrav_status walk_tree(agrav_ctx *c, int use_bh)
         dispatch_group<double>(pm, ew, ms, ngroups, s, tv, wc, c, gs, 1);
       AG_HIP(hipGetLastError());
       AG_HIP(hipEventRecord(c->ev[3], s));
-      // groups whose LDS stack overflowed are re-walked one lane per target
-      int nf = 0;
-      AG_HIP(hipMemcpyAsync(&nf, c->d_counters.p + 48, sizeof(int), hipMemcpyDeviceToHost, s));
+      // groups whose LDS stack overflowed (their count is isum[3]) are re-walked one
+      // lane per target
+      AG_HIP(hipMemcpyAsync(cnt, c->d_isum.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
       AG_HIP(hipStreamSynchronize(s));
+      const int nf = (int)(unsigned)cnt[3];
+      have_cnt     = nf == 0;
       if(nf > 0)
         {
           c->tim.walk_retries = nf;
@@ -1431,8 +1434,8 @@ agrav_status walk_tree(agrav_ctx *c, int use_bh)
       c->fin.perm    = nullptr;
     }
   AG_HIP(hipEventRecord(c->ev[4], s));
-  unsigned long long cnt[3] = {0, 0, 0};
-  AG_HIP(hipMemcpyAsync(cnt, c->d_isum.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
+  if(!have_cnt)
+    AG_HIP(hipMemcpyAsync(cnt, c->d_isum.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
   AG_HIP(hipStreamSynchronize(s));
   const unsigned long long tot = cnt[0];
   c->tim.walk_popped           = (long long)cnt[1];
