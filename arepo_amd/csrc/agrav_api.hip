@@ -337,7 +337,12 @@ void agrav_destroy(agrav_ctx *c)
   c->d_child.release();
   c->d_meta.release();
   c->d_cprec.release();
-  c->d_lvloff.release();
+  c->d_mom_tiles.release();
+  c->d_mom_tab.release();
+  c->d_mom_tbeg.release();
+  c->d_mom_hi.release();
+  c->d_mom_lo.release();
+  c->d_tcnt.release();
   c->d_soff.release();
   c->d_pset_in.release();
   c->d_pset.release();
@@ -346,7 +351,6 @@ void agrav_destroy(agrav_ctx *c)
   c->d_sdom.release();
   c->d_shd.release();
   c->d_groups.release();
-  c->d_lvlnodes.release();
   c->d_at.release();
   c->d_pt.release();
   c->d_nint.release();

@@ -487,57 +487,6 @@ __global__ void k_pnext(int n, const unsigned *__restrict__ lvlmask, const int *
     pnext[p] = pnext_of(p, n, lvlmask, pbase);
 }
 
-// Nodes listed per level for the moment passes.  A block handles kScatterChunk
-// nodes: ranks inside the block by level come from wave-level grouping of equal
-// levels, then one global reservation per block and level (atomics on the same
-// 22 words serialise, so there are few of them).  The order inside a level
-// list does not matter: every node's moments are computed on their own.
-static constexpr int kScatterItems = 16;
-static constexpr int kScatterChunk = B * kScatterItems;
-
-__global__ __launch_bounds__(B) void k_level_scatter(int m, const int *__restrict__ nlevel, const int *__restrict__ lvloff,
-                                                     int *lvlfill, int *lvlnodes)
-{
-  __shared__ int s_cnt[AG_KEY_LEVELS + 1], s_base[AG_KEY_LEVELS + 1];
-  if(threadIdx.x <= AG_KEY_LEVELS)
-    s_cnt[threadIdx.x] = 0;
-  __syncthreads();
-  const int lane   = threadIdx.x & 63;
-  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  const int j0     = blockIdx.x * kScatterChunk + (threadIdx.x >> 6) * 64 * kScatterItems;
-  int lv[kScatterItems], rk[kScatterItems];
-#pragma unroll
-  for(int r = 0; r < kScatterItems; r++)
-    {
-      const int j = j0 + r * 64 + lane;
-      const int L = j < m ? nlevel[j] : -1;
-      lv[r]       = L;
-      rk[r]       = 0;
-      uint64_t todo = __ballot(L >= 0);
-      while(todo)
-        {
-          const int leader = __ffsll((long long)todo) - 1;
-          const int Ll     = __shfl(L, leader, 64);
-          const uint64_t mk = __ballot(L == Ll);
-          int base = 0;
-          if(lane == leader)
-            base = atomicAdd(&s_cnt[Ll], __popcll(mk));
-          base = __shfl(base, leader, 64);
-          if(L == Ll)
-            rk[r] = base + __popcll(mk & lt);
-          todo &= ~mk;
-        }
-    }
-  __syncthreads();
-  if(threadIdx.x <= AG_KEY_LEVELS)
-    s_base[threadIdx.x] = s_cnt[threadIdx.x] ? atomicAdd(&lvlfill[threadIdx.x], s_cnt[threadIdx.x]) : 0;
-  __syncthreads();
-#pragma unroll
-  for(int r = 0; r < kScatterItems; r++)
-    if(lv[r] >= 0)
-      lvlnodes[lvloff[lv[r]] + s_base[lv[r]] + rk[r]] = j0 + r * 64 + lane;
-}
-
 // ---------------------------------------------------------------------------
 // 5. child lists and moments (forcetree.c:1217-1429)
 // ---------------------------------------------------------------------------
@@ -599,151 +548,276 @@ __global__ void k_count_children(int m, int n, const int *__restrict__ nextnode,
   ccnt[j] = k;
 }
 
-// One node's moments from its CSR children (forcetree.c:1217-1429): mass, FP64
-// centre of mass (geometric centre when empty) and maxsofttype, plus the two FP32
-// walk records.  Children are summed in their fixed list order.
-__device__ inline void node_moments(int j, int n, const int *__restrict__ cbeg, const int *__restrict__ ccnt, const int *__restrict__ child,
-                                    const double *__restrict__ px, const double *__restrict__ py, const double *__restrict__ pz,
-                                    const double *__restrict__ pm, const uint8_t *__restrict__ ps, const double *__restrict__ hsoft,
-                                    int nsoft, const double *__restrict__ cx, const double *__restrict__ cy,
-                                    const double *__restrict__ cz, const double *__restrict__ len, double *sx, double *sy,
-                                    double *sz, double *nm, uint8_t *maxsoft, uint4 *recA, uint4 *recB, const Domain *__restrict__ doms,
-                                    const int *__restrict__ nfirst, const int *__restrict__ pset)
+// Moments (forcetree.c:1217-1429) from prefix sums.  Every cube holds a
+// contiguous range of the Hilbert-ordered particles, so its mass and its
+// mass-weighted position are differences of two prefix sums over the particles.
+// The prefix sums of (m, m x, m y, m z) run in double-double arithmetic (an FP64
+// value and its rounding error; products split exactly with an FMA), so they are
+// exact to about 2^-104 of the running total; stored with the low part in FP32
+// they keep about 2^-77.  A cube's sums taken as differences then match a direct
+// FP64 sum over its particles (or its children, as the reference sums them) to
+// FP64 rounding, and every cube is done at once, with no pass per level.
+struct DD
 {
-  const Domain &dom = doms[pset ? pset[nfirst[j]] : 0];
-  double m = 0, s0 = 0, s1 = 0, s2 = 0;
-  int ms = nsoft;  // sentinel type with zero softening (forcetree.c:1254)
-  // children in batches of 8 (an octree cube has at most 8, bar buckets at the last
-  // level): all loads of a batch are issued before any is used, then the sums run
-  // in list order
-  const int b0 = cbeg[j], cn = ccnt[j];
-  for(int q0 = 0; q0 < cn; q0 += 8)
-    {
-      int cid[8];
-#pragma unroll
-      for(int u = 0; u < 8; u++)
-        cid[u] = q0 + u < cn ? child[b0 + q0 + u] : -1;
-      double mm[8], x[8], y[8], z[8];
-      int st[8];
-#pragma unroll
-      for(int u = 0; u < 8; u++)
-        {
-          const int c = cid[u];
-          if(c < 0)
-            {
-              mm[u] = x[u] = y[u] = z[u] = 0;
-              st[u] = nsoft;
-            }
-          else if(c < n)
-            {
-              mm[u] = pm[c];
-              x[u]  = px[c];
-              y[u]  = py[c];
-              z[u]  = pz[c];
-              st[u] = ps ? ps[c] : 0;
-            }
-          else
-            {
-              const int k = c - n;
-              mm[u]       = nm[k];
-              x[u]        = sx[k];
-              y[u]        = sy[k];
-              z[u]        = sz[k];
-              st[u]       = maxsoft[k];
-            }
-        }
-#pragma unroll
-      for(int u = 0; u < 8; u++)
-        if(cid[u] >= 0)
-          {
-            m += mm[u];
-            s0 += mm[u] * x[u];
-            s1 += mm[u] * y[u];
-            s2 += mm[u] * z[u];
-            if(hsoft[ms] < hsoft[st[u]])
-              ms = st[u];
-          }
-    }
-  if(m != 0)
-    {
-      s0 /= m;
-      s1 /= m;
-      s2 /= m;
-    }
-  else
-    {
-      s0 = cx[j];
-      s1 = cy[j];
-      s2 = cz[j];
-    }
-  sx[j]      = s0;
-  sy[j]      = s1;
-  sz[j]      = s2;
-  nm[j]      = m;
-  maxsoft[j] = (uint8_t)ms;
-  recA[j]    = make_uint4(to_fix(s0, dom, 0), to_fix(s1, dom, 1), to_fix(s2, dom, 2), __float_as_uint((float)m));
-  recB[j]    = make_uint4(to_fix(cx[j], dom, 0), to_fix(cy[j], dom, 1), to_fix(cz[j], dom, 2), __float_as_uint((float)len[j]));
+  double hi, lo;
+};
+
+__device__ inline DD dd_add(DD a, DD b)
+{
+  const double s  = a.hi + b.hi;
+  const double v  = s - a.hi;
+  const double e  = ((a.hi - (s - v)) + (b.hi - v)) + (a.lo + b.lo);  // two-sum error plus the low parts
+  const double hi = s + e;
+  return DD{hi, e - (hi - s)};
 }
 
-// Cubes of at most kSmallCube particles take their moments straight from their
-// particle range (contiguous in Hilbert order), summed in particle order: all of
-// them at once, with no dependence between levels.  Larger cubes sum their
-// children, level by level.
-#ifndef AG_SMALL_CUBE
-#define AG_SMALL_CUBE 16
-#endif
-static constexpr int kSmallCube = AG_SMALL_CUBE;
+struct DD4
+{
+  DD v[4];  // m, m x, m y, m z
+};
 
-__global__ __launch_bounds__(B) void k_moments_direct(int m, const int2 *__restrict__ nrange, const double *__restrict__ px,
-                                                      const double *__restrict__ py, const double *__restrict__ pz,
-                                                      const double *__restrict__ pm, const uint8_t *__restrict__ ps,
-                                                      const double *__restrict__ hsoft, int nsoft, const double *__restrict__ cx,
-                                                      const double *__restrict__ cy, const double *__restrict__ cz,
-                                                      const double *__restrict__ len, double *sx, double *sy, double *sz, double *nm,
-                                                      uint8_t *maxsoft, uint4 *recA, uint4 *recB, const Domain *__restrict__ doms,
-                                                      const int *__restrict__ pset)
+__device__ inline DD4 dd4_add(const DD4 &a, const DD4 &b)
+{
+  DD4 r;
+#pragma unroll
+  for(int k = 0; k < 4; k++)
+    r.v[k] = dd_add(a.v[k], b.v[k]);
+  return r;
+}
+
+__device__ inline DD4 dd4_zero()
+{
+  DD4 r;
+#pragma unroll
+  for(int k = 0; k < 4; k++)
+    r.v[k] = DD{0, 0};
+  return r;
+}
+
+__device__ inline DD4 particle_dd4(double m, double x, double y, double z)
+{
+  DD4 r;
+  r.v[0]         = DD{m, 0};
+  const double q[3] = {x, y, z};
+#pragma unroll
+  for(int k = 0; k < 3; k++)
+    {
+      const double p = m * q[k];
+      r.v[1 + k]     = DD{p, fma(m, q[k], -p)};  // exact product
+    }
+  return r;
+}
+
+static constexpr int kMomItems = 8;
+static constexpr int kMomTile  = B * kMomItems;
+
+// The prefix sums restart at every set of a batch, over tiles aligned to the set's
+// first particle, and every step below depends only on the set's own particles:
+// a set's moments come out bitwise the same in a batch as in a solve of that set
+// alone.  Entries: set s holds its n_s + 1 prefix values at [off_s + s, off_(s+1) + s].
+struct MomTiles
+{
+  const int4 *tab;   // {first particle, count, set, last tile of its set}; null: one set of n particles
+  const int *tbeg;   // first tile of every set (nsets + 1), or null
+  int n;
+};
+
+__device__ inline int4 mom_tile(const MomTiles &mt, int t, int ntiles)
+{
+  if(mt.tab)
+    return mt.tab[t];
+  const int st = t * kMomTile;
+  return make_int4(st, min(kMomTile, mt.n - st), 0, t == ntiles - 1);
+}
+
+// exclusive scan of one value per thread over the block (Hillis-Steele in LDS)
+__device__ inline DD4 block_excl_scan_dd4(DD4 v, DD4 *s_buf)
+{
+  const int t = threadIdx.x;
+  s_buf[t]    = v;
+  __syncthreads();
+  for(int d = 1; d < B; d <<= 1)
+    {
+      DD4 o = t >= d ? s_buf[t - d] : dd4_zero();
+      __syncthreads();
+      if(t >= d)
+        s_buf[t] = dd4_add(o, s_buf[t]);
+      __syncthreads();
+    }
+  const DD4 r = t > 0 ? s_buf[t - 1] : dd4_zero();
+  __syncthreads();
+  return r;
+}
+
+// 1. per tile: the sums of its (m, m x, m y, m z)
+__global__ __launch_bounds__(B) void k_mom_tile_sums(MomTiles mt, const double *__restrict__ px, const double *__restrict__ py,
+                                                     const double *__restrict__ pz, const double *__restrict__ pm, DD4 *tsum)
+{
+  __shared__ DD4 s_buf[B];
+  const int4 tl = mom_tile(mt, blockIdx.x, gridDim.x);
+  const int l0  = threadIdx.x * kMomItems;
+  DD4 acc       = dd4_zero();
+  for(int u = 0; u < kMomItems; u++)
+    if(l0 + u < tl.y)
+      {
+        const int i = tl.x + l0 + u;
+        acc         = dd4_add(acc, particle_dd4(pm[i], px[i], py[i], pz[i]));
+      }
+  const DD4 ex = block_excl_scan_dd4(acc, s_buf);
+  if(threadIdx.x == B - 1)
+    tsum[blockIdx.x] = dd4_add(ex, acc);
+}
+
+// 2. per set (one block each): exclusive prefix of its tile sums, in place, and the
+//    set's total; an empty set writes its single (zero) prefix entry
+__global__ __launch_bounds__(B) void k_mom_set_scan(MomTiles mt, int ntiles, const int64_t *__restrict__ soff, DD4 *tsum,
+                                                    DD4 *settot, double4 *phi, float4 *plo)
+{
+  __shared__ DD4 s_buf[B];
+  const int set = blockIdx.x;
+  const int t0 = mt.tbeg ? mt.tbeg[set] : 0, t1 = mt.tbeg ? mt.tbeg[set + 1] : ntiles;
+  const int nt = t1 - t0, per = (nt + B - 1) / B, b0 = t0 + threadIdx.x * per;
+  DD4 acc      = dd4_zero();
+  for(int k = b0; k < b0 + per && k < t1; k++)
+    acc = dd4_add(acc, tsum[k]);
+  DD4 run = block_excl_scan_dd4(acc, s_buf);
+  for(int k = b0; k < b0 + per && k < t1; k++)
+    {
+      const DD4 v = tsum[k];
+      tsum[k]     = run;
+      run         = dd4_add(run, v);
+    }
+  if(threadIdx.x == B - 1 && settot)
+    settot[set] = run;  // the last thread's running value has passed every tile
+  if(nt == 0 && threadIdx.x == 0 && soff)
+    {
+      const int64_t e = soff[set] + set;
+      phi[e]          = make_double4(0, 0, 0, 0);
+      plo[e]          = make_float4(0, 0, 0, 0);
+    }
+}
+
+// 2b. batch: exclusive prefix of the set totals (nsets + 1 entries), for the cubes
+//     above the set level, which span whole sets
+__global__ __launch_bounds__(B) void k_mom_settot_scan(int nsets, const DD4 *__restrict__ settot, DD4 *stp)
+{
+  __shared__ DD4 s_buf[B];
+  const int per = (nsets + B - 1) / B, b0 = threadIdx.x * per;
+  DD4 acc       = dd4_zero();
+  for(int k = b0; k < b0 + per && k < nsets; k++)
+    acc = dd4_add(acc, settot[k]);
+  DD4 run = block_excl_scan_dd4(acc, s_buf);
+  for(int k = b0; k < b0 + per && k < nsets; k++)
+    {
+      stp[k] = run;
+      run    = dd4_add(run, settot[k]);
+    }
+  if(threadIdx.x == B - 1)
+    stp[nsets] = run;
+}
+
+// 3. the prefix before every particle, and each set's total after its last particle:
+//    high parts in FP64, low parts in FP32
+__global__ __launch_bounds__(B) void k_mom_prefix(MomTiles mt, const double *__restrict__ px, const double *__restrict__ py,
+                                                  const double *__restrict__ pz, const double *__restrict__ pm,
+                                                  const DD4 *__restrict__ toff, double4 *phi, float4 *plo)
+{
+  __shared__ DD4 s_buf[B];
+  const int4 tl = mom_tile(mt, blockIdx.x, gridDim.x);
+  const int l0  = threadIdx.x * kMomItems;
+  DD4 v[kMomItems];
+  DD4 acc = dd4_zero();
+#pragma unroll
+  for(int u = 0; u < kMomItems; u++)
+    {
+      const int i = tl.x + l0 + u;
+      v[u]        = l0 + u < tl.y ? particle_dd4(pm[i], px[i], py[i], pz[i]) : dd4_zero();
+      acc         = dd4_add(acc, v[u]);
+    }
+  DD4 run = dd4_add(toff[blockIdx.x], block_excl_scan_dd4(acc, s_buf));
+  // one writer per entry: the prefix before a particle by the thread holding it, the
+  // set's total by the thread holding the set's last particle
+#pragma unroll
+  for(int u = 0; u <= kMomItems; u++)
+    {
+      const int l = l0 + u;
+      if((l < tl.y && u < kMomItems) || (l == tl.y && u > 0 && tl.w))
+        {
+          const int64_t e = (int64_t)tl.x + l + tl.z;
+          phi[e]          = make_double4(run.v[0].hi, run.v[1].hi, run.v[2].hi, run.v[3].hi);
+          plo[e]          = make_float4((float)run.v[0].lo, (float)run.v[1].lo, (float)run.v[2].lo, (float)run.v[3].lo);
+        }
+      if(u < kMomItems)
+        run = dd4_add(run, v[u]);
+    }
+}
+
+// count of particles of one softening type before every particle (input of a scan)
+__global__ void k_type_flags(int n, const uint8_t *__restrict__ ps, int t, int *flag)
+{
+  const int i = blockIdx.x * B + threadIdx.x;
+  if(i < n)
+    flag[i] = ps[i] == t ? 1 : 0;
+}
+
+// 4. every cube: sums as differences of the prefix at its last particle + 1 and at
+//    its first; FP64 centre of mass (geometric centre when massless), the softening
+//    type with the largest softening among its particles (sentinel nsoft when it has
+//    none, forcetree.c:1254), and the two FP32 walk records
+__global__ __launch_bounds__(B) void k_moments_prefix(int m, const int2 *__restrict__ nrange, const double4 *__restrict__ phi,
+                                                      const float4 *__restrict__ plo, const DD4 *__restrict__ stp,
+                                                      const int *__restrict__ tcnt, int np1, int nsoft, const double *__restrict__ hsoft,
+                                                      const double *__restrict__ cx, const double *__restrict__ cy,
+                                                      const double *__restrict__ cz, const double *__restrict__ len, double *sx,
+                                                      double *sy, double *sz, double *nm, uint8_t *maxsoft, uint4 *recA,
+                                                      uint4 *recB, const Domain *__restrict__ doms, const int *__restrict__ pset)
 {
   const int j = blockIdx.x * B + threadIdx.x;
   if(j >= m)
     return;
   const int2 rg = nrange[j];
-  if(rg.y > kSmallCube)
-    return;
-  const Domain &dom = doms[pset ? pset[rg.x] : 0];
-  double mass = 0, s0 = 0, s1 = 0, s2 = 0;
-  int ms = nsoft;  // sentinel type with zero softening (forcetree.c:1254)
-  for(int i0 = rg.x; i0 < rg.x + rg.y; i0 += 8)
+  const int a = rg.x, b = rg.x + rg.y;
+  const int sa = pset ? pset[a] : 0, sb = (pset && rg.y > 0) ? pset[b - 1] : sa;
+  DD lo4[4], hi4[4];
+  if(sa == sb)
     {
-      double mm[8], x[8], y[8], z[8];
-      int st[8];
-#pragma unroll
-      for(int u = 0; u < 8; u++)
-        {
-          const int i = i0 + u;
-          const bool in = i < rg.x + rg.y;
-          mm[u] = in ? pm[i] : 0.0;
-          x[u]  = in ? px[i] : 0.0;
-          y[u]  = in ? py[i] : 0.0;
-          z[u]  = in ? pz[i] : 0.0;
-          st[u] = in ? (ps ? ps[i] : 0) : nsoft;
-        }
-#pragma unroll
-      for(int u = 0; u < 8; u++)
-        if(i0 + u < rg.x + rg.y)
-          {
-            mass += mm[u];
-            s0 += mm[u] * x[u];
-            s1 += mm[u] * y[u];
-            s2 += mm[u] * z[u];
-            if(hsoft[ms] < hsoft[st[u]])
-              ms = st[u];
-          }
+      // a cube inside one set: its entries are shifted by the set index
+      const double4 ha = phi[(int64_t)a + sa], hb = phi[(int64_t)b + sa];
+      const float4 la = plo[(int64_t)a + sa], lb = plo[(int64_t)b + sa];
+      lo4[0] = DD{ha.x, (double)la.x};
+      lo4[1] = DD{ha.y, (double)la.y};
+      lo4[2] = DD{ha.z, (double)la.z};
+      lo4[3] = DD{ha.w, (double)la.w};
+      hi4[0] = DD{hb.x, (double)lb.x};
+      hi4[1] = DD{hb.y, (double)lb.y};
+      hi4[2] = DD{hb.z, (double)lb.z};
+      hi4[3] = DD{hb.w, (double)lb.w};
     }
+  else
+    {
+      // a cube above the set level holds whole sets
+      const DD4 p0 = stp[sa], p1 = stp[sb + 1];
+#pragma unroll
+      for(int k = 0; k < 4; k++)
+        {
+          lo4[k] = p0.v[k];
+          hi4[k] = p1.v[k];
+        }
+    }
+  double q[4];
+#pragma unroll
+  for(int k = 0; k < 4; k++)
+    {
+      const DD d = dd_add(hi4[k], DD{-lo4[k].hi, -lo4[k].lo});
+      q[k]       = d.hi + d.lo;
+    }
+  const double mass = q[0];
+  double s0, s1, s2;
   if(mass != 0)
     {
-      s0 /= mass;
-      s1 /= mass;
-      s2 /= mass;
+      s0 = q[1] / mass;
+      s1 = q[2] / mass;
+      s2 = q[3] / mass;
     }
   else
     {
@@ -751,6 +825,16 @@ __global__ __launch_bounds__(B) void k_moments_direct(int m, const int2 *__restr
       s1 = cy[j];
       s2 = cz[j];
     }
+  int ms = nsoft;
+  if(tcnt)
+    {
+      for(int t = 0; t < nsoft; t++)
+        if(tcnt[(int64_t)t * np1 + b] - tcnt[(int64_t)t * np1 + a] > 0 && hsoft[ms] < hsoft[t])
+          ms = t;
+    }
+  else if(rg.y > 0)
+    ms = 0;
+  const Domain &dom = doms[pset ? pset[a] : 0];
   sx[j]      = s0;
   sy[j]      = s1;
   sz[j]      = s2;
@@ -758,47 +842,6 @@ __global__ __launch_bounds__(B) void k_moments_direct(int m, const int2 *__restr
   maxsoft[j] = (uint8_t)ms;
   recA[j]    = make_uint4(to_fix(s0, dom, 0), to_fix(s1, dom, 1), to_fix(s2, dom, 2), __float_as_uint((float)mass));
   recB[j]    = make_uint4(to_fix(cx[j], dom, 0), to_fix(cy[j], dom, 1), to_fix(cz[j], dom, 2), __float_as_uint((float)len[j]));
-}
-
-#define MOMENT_ARGS                                                                                                            \
-  n, cbeg, ccnt, child, px, py, pz, pm, ps, hsoft, nsoft, cx, cy, cz, len, sx, sy, sz, nm, maxsoft, recA, recB, domp, nfirst, pset
-#define MOMENT_PARAMS                                                                                                          \
-  int n, const int *__restrict__ cbeg, const int *__restrict__ ccnt, const int *__restrict__ child, const double *__restrict__ px, \
-      const double *__restrict__ py, const double *__restrict__ pz, const double *__restrict__ pm, const uint8_t *__restrict__ ps,  \
-      const double *__restrict__ hsoft, int nsoft, const double *__restrict__ cx, const double *__restrict__ cy,                   \
-      const double *__restrict__ cz, const double *__restrict__ len, double *sx, double *sy, double *sz, double *nm,               \
-      uint8_t *maxsoft, uint4 *recA, uint4 *recB, const Domain *__restrict__ domp, const int *__restrict__ nfirst,       \
-      const int *__restrict__ pset
-
-// one level with many nodes: a lane per node (small cubes are done already)
-__global__ __launch_bounds__(B) void k_moments_level(const int *__restrict__ list, int cnt, const int2 *__restrict__ nrange,
-                                                     MOMENT_PARAMS)
-{
-  int t = blockIdx.x * B + threadIdx.x;
-  if(t >= cnt)
-    return;
-  const int j = list[t];
-  if(nrange[j].y > kSmallCube)
-    node_moments(j, MOMENT_ARGS);
-}
-
-// a run of consecutive levels with few nodes (deepest first) in one workgroup:
-// levels are separated by workgroup barriers, which also make one level's
-// results visible to the next (all waves share the CU's caches)
-__global__ __launch_bounds__(1024) void k_moments_levels_small(const int *__restrict__ list, const int *__restrict__ lvloff, int lhi,
-                                                                int llo, const int2 *__restrict__ nrange, MOMENT_PARAMS)
-{
-  for(int L = lhi; L >= llo; L--)
-    {
-      for(int t = lvloff[L] + threadIdx.x; t < lvloff[L + 1]; t += blockDim.x)
-        {
-          const int j = list[t];
-          if(nrange[j].y > kSmallCube)
-            node_moments(j, MOMENT_ARGS);
-        }
-      __threadfence_block();
-      __syncthreads();
-    }
 }
 
 // batch: the entry where each set's walk starts (its top cube, or its particle
@@ -850,12 +893,12 @@ agrav_status build_tree(agrav_ctx *c)
     return st;
   // device counters: [0..11] domain extent (6 u64), [12] octree node count, [13] a scan or
   // sort look-back that timed out, [14..35] cubes per level (read back together),
-  // [48..55] walk queue and failures, [50] threading error, [64..85] level fill
+  // [48..55] walk queue and failures, [50] threading error
   if((st = c->d_counters.ensure(128)) != AGRAV_OK)
     return st;
   int *d_total = c->d_counters.p + 12, *d_perr = c->d_counters.p + 13, *d_lvlcnt = c->d_counters.p + 14;
   // one upload sets the extent's start values and clears every other build counter
-  int cinit[86] = {};
+  int cinit[64] = {};
   unsigned long long *d_mm = (unsigned long long *)c->d_counters.p;  // 6 x u64 = 12 ints
   for(int k = 0; k < 6; k++)
     cinit[k] = -1;  // the three minima start at all ones, the maxima at zero
@@ -913,8 +956,7 @@ agrav_status build_tree(agrav_ctx *c)
   AG_HIP(hipGetLastError());
 
   // 3./4. octree cubes straight from the sorted keys
-  if((st = c->d_lvlmask.ensure(n)) || (st = c->d_pcnt.ensure(n)) || (st = c->d_pbase.ensure(n)) ||
-     (st = c->d_lvloff.ensure(AG_KEY_LEVELS + 2)))
+  if((st = c->d_lvlmask.ensure(n)) || (st = c->d_pcnt.ensure(n)) || (st = c->d_pbase.ensure(n)))
     return st;
   if(N > 1)
     {
@@ -949,7 +991,7 @@ agrav_status build_tree(agrav_ctx *c)
      (st = c->d_cy.ensure(M)) || (st = c->d_cz.ensure(M)) || (st = c->d_len.ensure(M)) || (st = c->d_nextnode.ensure(M)) ||
      (st = c->d_sibling.ensure(M)) || (st = c->d_maxsoft.ensure(M)) || (st = c->d_recA.ensure(M)) ||
      (st = c->d_recB.ensure(M)) || (st = c->d_nrange.ensure(M)) || (st = c->d_cbeg.ensure(M)) || (st = c->d_ccnt.ensure(M)) ||
-     (st = c->d_cnn.ensure(M)) || (st = c->d_child.ensure((size_t)M + n)) || (st = c->d_lvlnodes.ensure(M)))
+     (st = c->d_cnn.ensure(M)) || (st = c->d_child.ensure((size_t)M + n)))
     return st;
   k_place<<<nblk(N), B, 0, s>>>(N, c->d_lvlmask.p, c->d_pbase.p, c->d_px.p, c->d_py.p, c->d_pz.p, geo, c->d_nfirst.p,
                                 c->d_nlevel.p, c->d_cx.p, c->d_cy.p, c->d_cz.p, c->d_len.p, c->d_nextnode.p);
@@ -977,49 +1019,71 @@ agrav_status build_tree(agrav_ctx *c)
     return st;
   k_fill_children<<<nblk(M), B, 0, s>>>(M, N, c->d_nextnode.p, c->d_sibling.p, c->d_pnext.p, c->d_cbeg.p, c->d_ccnt.p,
                                          c->d_nrange.p, c->d_prec.p, c->d_child.p, c->d_cnn.p, c->d_meta.p, c->d_cprec.p);
-  int off[AG_KEY_LEVELS + 2];
-  off[0] = 0;
+  int mtot = 0;
   for(int L = 0; L <= AG_KEY_LEVELS; L++)
-    off[L + 1] = off[L] + c->lvlcnt_h[L];
-  if(off[AG_KEY_LEVELS + 1] != M)
+    mtot += c->lvlcnt_h[L];
+  if(mtot != M)
     return fail(c, AGRAV_ERR_CAPACITY, "inconsistent octree level counts");
-  AG_HIP(hipMemcpyAsync(c->d_lvloff.p, off, (AG_KEY_LEVELS + 2) * sizeof(int), hipMemcpyHostToDevice, s));
-  int *d_fill = c->d_counters.p + 64;  // 22 ints, cleared by the upload at the start
-  k_level_scatter<<<(unsigned)((M + kScatterChunk - 1) / kScatterChunk), B, 0, s>>>(M, c->d_nlevel.p, c->d_lvloff.p, d_fill,
-                                                                                      c->d_lvlnodes.p);
-  AG_HIP(hipGetLastError());
-  const uint8_t *d_ps = multisoft ? c->d_ps.p : nullptr;
-  k_moments_direct<<<nblk(M), B, 0, s>>>(M, c->d_nrange.p, c->d_px.p, c->d_py.p, c->d_pz.p, c->d_pm.p, d_ps, c->d_soft.p,
-                                         c->soft.nsoft, c->d_cx.p, c->d_cy.p, c->d_cz.p, c->d_len.p, c->d_sx.p, c->d_sy.p,
-                                         c->d_sz.p, c->d_nm.p, c->d_maxsoft.p, c->d_recA.p, c->d_recB.p, geo.dom, geo.pset);
-  const int small = 4096;
-  for(int L = AG_KEY_LEVELS; L >= 0;)
+  // moments of all cubes from prefix sums over the particles, restarted at every set
+  MomTiles mtl{nullptr, nullptr, N};
+  int ntiles = (N + kMomTile - 1) / kMomTile;
+  const int nseg = nsets > 0 ? nsets : 1;
+  if(nsets > 0)
     {
-      if(c->lvlcnt_h[L] >= small)
+      std::vector<int4> tab;
+      std::vector<int> tbeg(nsets + 1);
+      for(int q = 0; q < nsets; q++)
         {
-          k_moments_level<<<nblk(c->lvlcnt_h[L]), B, 0, s>>>(c->d_lvlnodes.p + off[L], c->lvlcnt_h[L], c->d_nrange.p, N,
-                                                              c->d_cbeg.p, c->d_ccnt.p,
-                                                              c->d_child.p, c->d_px.p, c->d_py.p, c->d_pz.p, c->d_pm.p, d_ps,
-                                                              c->d_soft.p, c->soft.nsoft, c->d_cx.p, c->d_cy.p, c->d_cz.p,
-                                                              c->d_len.p, c->d_sx.p, c->d_sy.p, c->d_sz.p, c->d_nm.p,
-                                                              c->d_maxsoft.p, c->d_recA.p, c->d_recB.p, geo.dom, c->d_nfirst.p,
-                                                              geo.pset);
-          L--;
+          tbeg[q]         = (int)tab.size();
+          const int64_t f = c->h_soff[q], e = c->h_soff[q + 1];
+          for(int64_t st0 = f; st0 < e; st0 += kMomTile)
+            tab.push_back(make_int4((int)st0, (int)std::min<int64_t>(kMomTile, e - st0), q, st0 + kMomTile >= e));
         }
-      else
+      tbeg[nsets] = (int)tab.size();
+      ntiles      = (int)tab.size();
+      if((st = c->d_mom_tab.ensure(tab.size() ? tab.size() : 1)) || (st = c->d_mom_tbeg.ensure(nsets + 1)))
+        return st;
+      if(!tab.empty())
+        AG_HIP(hipMemcpyAsync(c->d_mom_tab.p, tab.data(), tab.size() * sizeof(int4), hipMemcpyHostToDevice, s));
+      AG_HIP(hipMemcpyAsync(c->d_mom_tbeg.p, tbeg.data(), (nsets + 1) * sizeof(int), hipMemcpyHostToDevice, s));
+      AG_HIP(hipStreamSynchronize(s));  // the host vectors go out of scope
+      mtl.tab  = c->d_mom_tab.p;
+      mtl.tbeg = c->d_mom_tbeg.p;
+    }
+  const size_t ntile_bytes = (size_t)(ntiles ? ntiles : 1) * sizeof(DD4), nset_bytes = (size_t)(nseg + 1) * sizeof(DD4);
+  if((st = c->d_mom_tiles.ensure(ntile_bytes + 2 * nset_bytes)) || (st = c->d_mom_hi.ensure((size_t)N + nseg)) ||
+     (st = c->d_mom_lo.ensure((size_t)N + nseg)))
+    return st;
+  DD4 *tsum   = reinterpret_cast<DD4 *>(c->d_mom_tiles.p);
+  DD4 *settot = reinterpret_cast<DD4 *>(c->d_mom_tiles.p + ntile_bytes);
+  DD4 *stp    = reinterpret_cast<DD4 *>(c->d_mom_tiles.p + ntile_bytes + nset_bytes);
+  if(ntiles > 0)
+    k_mom_tile_sums<<<ntiles, B, 0, s>>>(mtl, c->d_px.p, c->d_py.p, c->d_pz.p, c->d_pm.p, tsum);
+  k_mom_set_scan<<<nseg, B, 0, s>>>(mtl, ntiles, nsets > 0 ? c->d_soff.p : nullptr, tsum, settot, c->d_mom_hi.p, c->d_mom_lo.p);
+  if(nsets > 0)
+    k_mom_settot_scan<<<1, B, 0, s>>>(nsets, settot, stp);
+  if(ntiles > 0)
+    k_mom_prefix<<<ntiles, B, 0, s>>>(mtl, c->d_px.p, c->d_py.p, c->d_pz.p, c->d_pm.p, tsum, c->d_mom_hi.p, c->d_mom_lo.p);
+  AG_HIP(hipGetLastError());
+  int *tcnt = nullptr;
+  if(multisoft)
+    {
+      // per softening type, the count of its particles before every particle
+      const int ns = c->soft.nsoft;
+      if((st = c->d_tcnt.ensure((size_t)ns * (N + 1))) || (st = c->d_pcnt.ensure(n)))
+        return st;
+      tcnt = c->d_tcnt.p;
+      for(int t = 0; t < ns; t++)
         {
-          int lo = L;
-          while(lo - 1 >= 0 && c->lvlcnt_h[lo - 1] < small)
-            lo--;
-          k_moments_levels_small<<<1, 1024, 0, s>>>(c->d_lvlnodes.p, c->d_lvloff.p, L, lo, c->d_nrange.p, N, c->d_cbeg.p,
-                                                    c->d_ccnt.p,
-                                                    c->d_child.p, c->d_px.p, c->d_py.p, c->d_pz.p, c->d_pm.p, d_ps, c->d_soft.p,
-                                                    c->soft.nsoft, c->d_cx.p, c->d_cy.p, c->d_cz.p, c->d_len.p, c->d_sx.p,
-                                                    c->d_sy.p, c->d_sz.p, c->d_nm.p, c->d_maxsoft.p, c->d_recA.p, c->d_recB.p,
-                                                    geo.dom, c->d_nfirst.p, geo.pset);
-          L = lo - 1;
+          k_type_flags<<<nblk(N), B, 0, s>>>(N, c->d_ps.p, t, c->d_pcnt.p);
+          if((st = exclusive_scan_i32(c->d_pcnt.p, tcnt + (size_t)t * (N + 1), N, tcnt + (size_t)t * (N + 1) + N, c->d_temp.p,
+                                      d_perr, s)) != AGRAV_OK)
+            return st;
         }
     }
+  k_moments_prefix<<<nblk(M), B, 0, s>>>(M, c->d_nrange.p, c->d_mom_hi.p, c->d_mom_lo.p, stp, tcnt, N + 1, c->soft.nsoft, c->d_soft.p,
+                                         c->d_cx.p, c->d_cy.p, c->d_cz.p, c->d_len.p, c->d_sx.p, c->d_sy.p, c->d_sz.p, c->d_nm.p,
+                                         c->d_maxsoft.p, c->d_recA.p, c->d_recB.p, geo.dom, geo.pset);
   AG_HIP(hipGetLastError());
 
   // out-of-box check (forcetree.c:432-438 aborts; here it is an error code)

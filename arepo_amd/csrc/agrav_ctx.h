@@ -128,7 +128,14 @@ struct agrav_ctx
   agrav::DBuf<int> d_cbeg, d_ccnt, d_cnn, d_child;
   agrav::DBuf<int4> d_meta;
   agrav::DBuf<uint4> d_cprec;
-  agrav::DBuf<int> d_lvloff, d_lvlnodes;
+  // moments from prefix sums: per-tile sums, prefix high (FP64) and low (FP32)
+  // parts of (m, m x, m y, m z), per-type particle counts (several softening types)
+  agrav::DBuf<char> d_mom_tiles;
+  agrav::DBuf<int4> d_mom_tab;  // batch: tiles aligned to the sets' first particles
+  agrav::DBuf<int> d_mom_tbeg;  // batch: first tile of every set
+  agrav::DBuf<double4> d_mom_hi;
+  agrav::DBuf<float4> d_mom_lo;
+  agrav::DBuf<int> d_tcnt;
   int lvlcnt_h[AG_KEY_LEVELS + 2];
 
   // walk outputs in sorted order (G=1 units) and per-target counts
