@@ -64,6 +64,10 @@ class Timings(C.Structure):
         ("walk_retries", C.c_int),
         ("walk_popped", C.c_longlong),
         ("tile_entries", C.c_longlong),
+        ("unbind_rounds", C.c_int),
+        ("unbind_solves", C.c_int),
+        ("unbind_solve_ms", C.c_double),
+        ("unbind_pass_ms", C.c_double),
     ]
 
     def as_dict(self):
@@ -93,7 +97,7 @@ EXPORTED = [
     "agrav_get_components", "agrav_gravity_forcetest", "agrav_gravity_direct", "agrav_get_timings", "agrav_get_tree",
     "agrav_get_order", "agrav_get_tables", "agrav_hilbert_key21",
     "agrav_batch_set_particles", "agrav_batch_autotune", "agrav_batch_get_tuning", "agrav_batch_set_tuning",
-    "agrav_server_create", "agrav_server_submit", "agrav_server_wait", "agrav_server_poll", "agrav_server_stats",
+    "agrav_subfind_unbind", "agrav_server_create", "agrav_server_submit", "agrav_server_wait", "agrav_server_poll", "agrav_server_stats",
     "agrav_server_destroy",
 ]
 
@@ -113,6 +117,16 @@ class BatchTuning(C.Structure):
 
     def as_dict(self):
         return {k: (list(getattr(self, k)) if k != "tuned" else self.tuned) for k, _ in self._fields_}
+
+
+class UnbindParams(C.Structure):
+    _fields_ = [
+        ("time_a", C.c_double),
+        ("hubble_a", C.c_double),
+        ("comoving", C.c_int),
+        ("des_link_ngb", C.c_int),
+        ("max_iter", C.c_int),
+    ]
 
 
 class Request(C.Structure):
@@ -169,6 +183,8 @@ def _declare(L):
     L.agrav_batch_autotune.argtypes = [vp, C.c_int64, C.POINTER(BatchTuning)]
     L.agrav_batch_get_tuning.argtypes = [vp, C.POINTER(BatchTuning)]
     L.agrav_batch_set_tuning.argtypes = [vp, C.POINTER(BatchTuning)]
+    L.agrav_subfind_unbind.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp, C.POINTER(UnbindParams), vp, vp,
+                                       vp, vp, vp, vp]
     L.agrav_server_create.argtypes = [C.POINTER(Params), C.c_int, C.c_size_t, C.POINTER(vp)]
     L.agrav_server_submit.argtypes = [vp, C.POINTER(Request), C.POINTER(C.c_uint64)]
     L.agrav_server_wait.argtypes = [vp, C.c_uint64, C.POINTER(RequestResult)]
@@ -293,6 +309,39 @@ class Gravity:
             t.group_size[k] = int(gs[k])
             t.leaf_bucket[k] = int(lb[k])
         self._check(self.L.agrav_batch_set_tuning(self.h, C.byref(t)))
+
+    def subfind_unbind(self, offsets, pos, vel, mass, softening, utherm=None, ptype=None, cubes=None, time_a=1.0,
+                       hubble_a=0.0, comoving=False, des_link_ngb=20, max_iter=0):
+        """SUBFIND unbinding (subfind_unbind, subfind_serial.c:429-637) of every group
+        offsets[g]:offsets[g+1] at once; theta is the context's err_tol_theta.
+
+        Returns dict(bound[n] bool, len[g], len_non_gas[g], iterations[g], potential[n], energy[n])."""
+        off = np.ascontiguousarray(offsets, dtype=np.int64)
+        ng = len(off) - 1
+        x = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        v = np.ascontiguousarray(vel, dtype=np.float64).reshape(-1, 3)
+        m = np.ascontiguousarray(mass, dtype=np.float64)
+        eps = np.ascontiguousarray(softening, dtype=np.float64)
+        ut = None if utherm is None else np.ascontiguousarray(utherm, dtype=np.float64)
+        ty = None if ptype is None else np.ascontiguousarray(ptype, dtype=np.uint8)
+        cb = None if cubes is None else np.ascontiguousarray(cubes, dtype=np.float64).reshape(-1, 4)
+        n = len(x)
+        if ng < 1 or off[-1] != n or len(v) != n or len(m) != n or len(eps) != ng or \
+                (cb is not None and len(cb) != ng) or (ut is not None and len(ut) != n) or \
+                (ty is not None and len(ty) != n):
+            raise ValueError("unbinding arrays do not match the offsets")
+        up = UnbindParams(time_a, hubble_a, int(comoving), int(des_link_ngb), int(max_iter))
+        bound = np.zeros(max(n, 1), dtype=np.uint8)
+        ln = np.zeros(ng, dtype=np.int32)
+        lng = np.zeros(ng, dtype=np.int32)
+        it = np.zeros(ng, dtype=np.int32)
+        pot = np.zeros(max(n, 1))
+        en = np.zeros(max(n, 1))
+        self._check(self.L.agrav_subfind_unbind(self.h, ng, _ptr(off), _ptr(x), _ptr(v), _ptr(m), _ptr(eps), _ptr(ut),
+                                                _ptr(ty), _ptr(cb), C.byref(up), _ptr(bound), _ptr(ln), _ptr(lng),
+                                                _ptr(it), _ptr(pot), _ptr(en)))
+        return dict(bound=bound[:n].astype(bool), len=ln, len_non_gas=lng, iterations=it, potential=pot[:n],
+                    energy=en[:n])
 
     def set_old_acc(self, old_acc):
         oa = np.ascontiguousarray(old_acc, dtype=np.float32)

@@ -26,6 +26,7 @@ SOURCES = [
     "agrav_pm.hip",
     "agrav_direct.hip",
     "agrav_batch.hip",
+    "agrav_unbind.hip",
     "agrav_serve.hip",
     "agrav_api.hip",
 ]

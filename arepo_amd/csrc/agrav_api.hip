@@ -365,6 +365,7 @@ void agrav_destroy(agrav_ctx *c)
   c->d_nint_out.release();
   c->d_temp.release();
   c->d_dom.release();
+  c->unb.release();
   if(c->stream)
     (void)hipStreamDestroy(c->stream);
   delete c;

@@ -227,6 +227,31 @@ __global__ __launch_bounds__(256) void k_set_domains(const double *__restrict__ 
     }
 }
 
+// batch: root cubes given per set {centre[3], len}, as SUBFIND keeps one local-tree
+// cube per group for every unbinding pass (subfind_loctree.c:101-206)
+__global__ void k_set_domains_fixed(const double *__restrict__ cube, int nsets, Domain *sdom)
+{
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if(s >= nsets)
+    return;
+  Domain dm{};
+  const double len = cube[4 * s + 3];
+  for(int d = 0; d < 3; d++)
+    {
+      dm.center[d]  = cube[4 * s + d];
+      dm.corner[d]  = cube[4 * s + d] - 0.5 * len;
+      dm.fix_org[d] = dm.corner[d];
+      dm.xmin[d]    = dm.corner[d];
+      dm.xmax[d]    = dm.corner[d] + len;
+    }
+  dm.len      = len;
+  dm.inv_len  = 1.0 / len;
+  dm.bigfac   = len / (double)(1ll << 52);
+  dm.fix_unit = len / 2147483648.0;
+  dm.fix_inv  = 1.0 / dm.fix_unit;
+  sdom[s]     = dm;
+}
+
 // ---------------------------------------------------------------------------
 // 2. keys (forcetree.c:394-417 wrap, :493-505 integer coordinates)
 // ---------------------------------------------------------------------------
@@ -928,7 +953,10 @@ agrav_status build_tree(agrav_ctx *c)
       geo.dom     = c->d_sdom.p;
       geo.pset_in = c->d_pset_in.p;
       geo.pset    = c->d_pset.p;
-      k_set_domains<<<nsets, 256, 0, s>>>(c->d_pos_in.p, c->d_soff.p, c->d_sdom.p);
+      if(c->batch_cubes)
+        k_set_domains_fixed<<<(nsets + 255) / 256, 256, 0, s>>>(c->batch_cubes, nsets, c->d_sdom.p);
+      else
+        k_set_domains<<<nsets, 256, 0, s>>>(c->d_pos_in.p, c->d_soff.p, c->d_sdom.p);
       AG_HIP(hipGetLastError());
     }
   c->geo_sbits  = geo.sbits;

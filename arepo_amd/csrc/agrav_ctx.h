@@ -60,6 +60,23 @@ struct PmState
   double mass_scale = 0;             // fixed-point units per mass unit
 };
 
+// device buffers of the unbinding loop (agrav_unbind.hip), kept between calls
+struct UnbindBufs
+{
+  DBuf<double> pos, vel, mass, uth, pot, E, cube, epsr, wbl, ppos;
+  DBuf<uint8_t> type, alive;
+  DBuf<int> gid, flag, dst, cidx, glist, gphase, res, cnt;
+  DBuf<int64_t> off, soff;
+  DBuf<char> scan_tmp;
+  void release()
+  {
+    pos.release(), vel.release(), mass.release(), uth.release(), pot.release(), E.release();
+    cube.release(), epsr.release(), wbl.release(), ppos.release(), type.release(), alive.release();
+    gid.release(), flag.release(), dst.release(), cidx.release(), glist.release(), gphase.release();
+    res.release(), cnt.release(), off.release(), soff.release(), scan_tmp.release();
+  }
+};
+
 }  // namespace agrav
 
 struct agrav_ctx
@@ -96,6 +113,7 @@ struct agrav_ctx
   agrav::DBuf<int> d_sroot, d_send;   // top entry of each set and the entry after it
   agrav::DBuf<agrav::Domain> d_sdom;  // domain cube per set
   agrav::DBuf<double> d_shd;          // softening length h = 2.8 eps per set
+  const double *batch_cubes = nullptr; // device {centre[3], len} per set: given root cubes, or null
   agrav::DBuf<int4> d_groups;         // walk groups {first target, count, set, leaf bucket}
   int batch_ngroups = 0;
   bool batch_groups_dirty = true;
@@ -167,6 +185,9 @@ struct agrav_ctx
 
   // state
   bool have_particles = false, have_tree = false, have_tree_force = false, have_pm = false, have_final = false;
+
+  // unbinding loop
+  agrav::UnbindBufs unb;
 
   // timing
   hipEvent_t ev[8] = {};

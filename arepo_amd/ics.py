@@ -142,3 +142,54 @@ def unbinding_batch(nsets: int = 4096, nmin: int = 32, nmax: int = 200000, seed:
         eps[s] = float(np.median(r)) / 50.0
     mass = np.full(int(off[-1]), mpart)
     return off, pos, mass, eps, centres
+
+
+def _shell_potential(r: np.ndarray, m: float) -> np.ndarray:
+    """Potential (G = 1) of equal masses m at radii r from spherical shells: -(M(<r)/r + sum_{r_j > r} m/r_j)."""
+    order = np.argsort(r)
+    rs = np.maximum(r[order], 1e-12)
+    inner = m * np.arange(len(rs)) / rs
+    outer = m * (np.cumsum((1.0 / rs)[::-1])[::-1] - 1.0 / rs)
+    phi = np.empty_like(r)
+    phi[order] = -(inner + outer)
+    return phi
+
+
+def unbinding_candidates(ngroups: int = 4096, nmin: int = 32, nmax: int = 200000, seed: int = 6, mpart: float = 1e-4,
+                         f_max: float = 1.3, interlopers: float = 0.05):
+    """Candidate groups for SUBFIND-style unbinding: the groups of `unbinding_batch` with velocities.
+
+    Each member moves isotropically at f times its escape speed from the group's
+    own spherical mass profile, f ~ U(0, f_max), so about (f_max - 1)/f_max of
+    the members start unbound; a fraction `interlopers` of each group is
+    replaced by particles spread uniformly over the group's sphere moving at
+    2-4 escape speeds; every group has a bulk velocity of about its half-mass
+    escape speed.  Returns (offsets[ngroups+1] int64, pos[N,3] float64 offsets
+    from each group's centre, vel[N,3] float64, mass[N], eps[ngroups]).
+    """
+    off, pos32, mass, eps, _ = unbinding_batch(ngroups, nmin, nmax, seed, mpart)
+    g = rng(seed + 1000)
+    pos = pos32.astype(np.float64)
+    vel = np.empty_like(pos)
+    for s in range(ngroups):
+        a, b = int(off[s]), int(off[s + 1])
+        n = b - a
+        if n == 0:
+            continue
+        p = pos[a:b]
+        r = np.sqrt((p * p).sum(1))
+        ni = int(interlopers * n)
+        if ni > 0:
+            rmax = float(r.max())
+            k = g.choice(n, ni, replace=False)
+            p[k] = _isotropic(rmax * g.random(ni) ** (1.0 / 3.0), g)
+            r = np.sqrt((p * p).sum(1))
+        vesc = np.sqrt(2.0 * -_shell_potential(r, mpart))
+        f = g.uniform(0.0, f_max, n)
+        if ni > 0:
+            f[k] = g.uniform(2.0, 4.0, ni)
+        d = g.normal(size=(n, 3))
+        d /= np.maximum(np.linalg.norm(d, axis=1), 1e-300)[:, None]
+        bulk = g.normal(size=3) * float(np.median(vesc)) / np.sqrt(3.0)
+        vel[a:b] = d * (f * vesc)[:, None] + bulk
+    return off, pos, vel, mass, eps

@@ -251,6 +251,76 @@ def batch_run(A, nsets, steps, seed, cpu_particles, threads, err_sample):
     return res
 
 
+def unbind_run(A, ngroups, seed, cpu_particles, threads):
+    """SUBFIND unbinding (subfind_unbind, subfind_serial.c:429-637) of candidate groups: the batch
+    workload's groups (log-uniform 32 .. 200000 members, NFW- or Hernquist-like) with isotropic velocities
+    at U(0, 1.3) x escape speed, 5 % interlopers at 2-4 x escape speed and a bulk velocity per group.
+    All groups advance one pass per round: one batched FP32 potential solve (autotuned group walk) for
+    the groups in a full pass, one workgroup per group for the rest.  Baseline: the oracle's restatement
+    of subfind_unbind on the host (groups in parallel, sample), and the bound sets of that sample compared
+    with the GPU's."""
+    from concurrent.futures import ThreadPoolExecutor
+    from arepo_amd import ics
+    log("unbind: generating %d candidate groups" % ngroups)
+    off, pos, vel, mass, eps = ics.unbinding_candidates(ngroups, seed=seed)
+    n = int(off[-1])
+    sizes = np.diff(off)
+    p = A.make_params(periodic=False, box_size=0.0, pmgrid=0, soft=(0.01,), precision=A.FP32, walk=A.WALK_GROUP,
+                      err_tol_theta=0.7, opening=0, leaf_bucket=4, eval_potential=True)
+    res = {"n_groups": ngroups, "n_particles": n, "n_min": int(sizes.min()), "n_max": int(sizes.max()),
+           "criterion": "barnes-hut theta 0.7 (ErrTolThetaSubfind, subfind_loctree.c:639)", "precision": "fp32",
+           "des_link_ngb": 20}
+    with A.Gravity(p) as g:
+        # the batch autotuner on the initial member sets sets the walk's group size and leaf bucket
+        g.batch_set_particles(off, pos.astype(np.float32), mass, eps)
+        tun = g.batch_autotune()
+        res["tuning"] = {k: tun[k] for k in ("group_size", "leaf_bucket")}
+        g.subfind_unbind(off, pos, vel, mass, eps, des_link_ngb=20)  # warm-up (allocations)
+        t0 = time.perf_counter()
+        r = g.subfind_unbind(off, pos, vel, mass, eps, des_link_ngb=20)
+        wall = time.perf_counter() - t0
+        tim = g.timings()
+    left = int(r["len"].sum())
+    res.update({"seconds": wall, "particles_per_s": n / wall, "members_left": left,
+                "bound_fraction": left / n, "rounds": tim["unbind_rounds"], "potential_solves": tim["unbind_solves"],
+                "solve_ms": tim["unbind_solve_ms"], "pass_ms": tim["unbind_pass_ms"],
+                "passes_per_group": {"median": float(np.median(r["iterations"])),
+                                     "max": int(r["iterations"].max())}})
+    log("unbind: %.3f s (%.3g particles/s), %d rounds, %d potential solves (%.0f ms), passes %.0f ms, %.3f bound"
+        % (wall, n / wall, tim["unbind_rounds"], tim["unbind_solves"], tim["unbind_solve_ms"],
+           tim["unbind_pass_ms"], left / n))
+    if cpu_particles > 0:
+        import oracle as O
+        O.lib()
+        stride = max(1, int(np.ceil(n / cpu_particles)))
+        cs = np.arange(0, ngroups, stride)
+        up = O.make_unbind_params(theta=0.7, des_link_ngb=20)
+
+        def one(s):
+            a, b = int(off[s]), int(off[s + 1])
+            return O.subfind_unbind(up, pos[a:b], vel[a:b], mass[a:b], eps[s])
+
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(threads) as ex:
+            outs = list(ex.map(one, cs))
+        t_cpu = time.perf_counter() - t0
+        n_cpu = int(sizes[cs].sum())
+        res["cpu_baseline"] = {"value": n_cpu / t_cpu, "unit": "particles/s", "cores": threads, "kind": "port",
+                               "sample": "oracle subfind_unbind of every %d-th group: %d groups, %d particles, %.1f s"
+                                         % (stride, len(cs), n_cpu, t_cpu)}
+        res["speedup_vs_cpu"] = res["particles_per_s"] / res["cpu_baseline"]["value"]
+        ob = np.concatenate([o["bound"] for o in outs])
+        gb = np.concatenate([r["bound"][off[s]:off[s + 1]] for s in cs])
+        res["bound_set_vs_oracle"] = {"members": int(len(ob)), "differ": int((ob != gb).sum()),
+                                      "differ_fraction": float((ob != gb).mean()),
+                                      "groups_same_len": int(sum(int(o["len"] == r["len"][s])
+                                                                 for o, s in zip(outs, cs))),
+                                      "groups": int(len(cs))}
+        log("unbind: cpu %.3g particles/s, bound sets differ for %d of %d members"
+            % (res["cpu_baseline"]["value"], res["bound_set_vs_oracle"]["differ"], len(ob)))
+    return res
+
+
 PMC_PROFILE = os.path.join(ROOT, "profiles", "k_walk_group_treepm128_pmc.md")
 
 
@@ -325,6 +395,9 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=2)
     ap.add_argument("--batch-cpu-particles", type=int, default=8 << 20)
     ap.add_argument("--batch-only", action="store_true", help="run only the batched-unbinding leg (profiling aid)")
+    ap.add_argument("--unbind-groups", type=int, default=4096)
+    ap.add_argument("--unbind-cpu-particles", type=int, default=2 << 20)
+    ap.add_argument("--unbind-only", action="store_true", help="run only the unbinding-loop leg (profiling aid)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -340,6 +413,12 @@ def main():
         res = batch_run(A, args.batch_sets, args.batch_steps, 4, 0 if args.no_cpu else args.batch_cpu_particles,
                         min(os.cpu_count() or 1, 16), args.err_sample)
         print(json.dumps({"batched_unbinding": res}), flush=True)
+        return
+
+    if args.unbind_only:
+        res = unbind_run(A, args.unbind_groups, 6, 0 if args.no_cpu else args.unbind_cpu_particles,
+                         min(os.cpu_count() or 1, 16))
+        print(json.dumps({"subfind_unbinding": res}), flush=True)
         return
 
     log("generating clustered 128^3 set (seed %d)" % args.seed)
@@ -482,6 +561,12 @@ def main():
                                              0 if args.no_cpu else args.batch_cpu_particles,
                                              min(os.cpu_count() or 1, 16), args.err_sample)
         log("batch: %s" % json.dumps(out["batched_unbinding"]))
+
+    if args.unbind_groups > 0:
+        out["subfind_unbinding"] = unbind_run(A, args.unbind_groups, 6,
+                                              0 if args.no_cpu else args.unbind_cpu_particles,
+                                              min(os.cpu_count() or 1, 16))
+        log("unbind: %s" % json.dumps(out["subfind_unbinding"]))
 
     print(json.dumps(out), flush=True)
 

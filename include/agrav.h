@@ -98,6 +98,10 @@ typedef struct
   int walk_retries;   /* groups re-run by the fallback walk (stack overflow) */
   long long walk_popped;  /* group walk: stack entries evaluated (traversal work) */
   long long tile_entries; /* group walk: interaction-tile entries streamed by all 64 lanes */
+  int unbind_rounds;      /* agrav_subfind_unbind: passes of the batched loop (the longest group's) */
+  int unbind_solves;      /* agrav_subfind_unbind: batched potential solves (tree build + walk) */
+  double unbind_solve_ms; /* agrav_subfind_unbind: device time of those solves */
+  double unbind_pass_ms;  /* agrav_subfind_unbind: device time of the per-group pass kernels */
 } agrav_timings;
 
 typedef struct agrav_ctx agrav_ctx;
@@ -218,6 +222,52 @@ agrav_status agrav_batch_set_particles(agrav_ctx *ctx, int nsets, const int64_t 
 agrav_status agrav_batch_autotune(agrav_ctx *ctx, int64_t sample_particles, agrav_batch_tuning *out);
 agrav_status agrav_batch_get_tuning(agrav_ctx *ctx, agrav_batch_tuning *out);
 agrav_status agrav_batch_set_tuning(agrav_ctx *ctx, const agrav_batch_tuning *t);
+
+/* ---- SUBFIND unbinding of many candidate groups ---------------------------
+ * Replaces subfind_unbind (subfind_serial.c:429-637) called once per candidate
+ * (subfind_serial.c:291, subfind_collective.c:1442).  Group g holds the
+ * particles [offsets[g], offsets[g+1]).  Every pass of the reference's loop is
+ * restated: the potential of all members (G/a x the Barnes-Hut walk of a local
+ * tree, err_tol_theta of the context = ErrTolThetaSubfind, subfind_loctree.c:639)
+ * in a full pass; the centre of mass about the member of minimum potential and
+ * the bulk velocity; binding energies with the Hubble flow, the self-energy
+ * G/a m/eps and, for gas (type 0), utherm; removal of the members with E > 0
+ * above the 0.25 len-th largest energy; the phase switch between full passes and
+ * energy-only passes; the stop when a full pass removes nobody or fewer than
+ * des_link_ngb members remain.  In an energy-only pass the reference evaluates
+ * the weakly bound members' potentials again but keeps the old values scaled by
+ * G/a once more (subfind_serial.c:491-497); that scaling is kept, the discarded
+ * walk is not run.  All groups advance together: the potentials of every group
+ * in a full pass come from one batched tree build and walk (the context's
+ * precision and walk kind), the rest of a pass runs one workgroup per group.
+ *
+ * Positions are FP64 in an isolated frame: a group in a periodic box is passed
+ * relative to a point inside it, wrapped to the nearest image (the reference's
+ * GRAVITY_NEAREST distances for groups smaller than half the box).
+ * softening[g] is the group's Plummer-equivalent softening (ForceSoftening/2.8
+ * of its members).  cubes (4 doubles per group {centre[3], len}, or NULL) is the
+ * local tree's root cube kept for every pass (subfind_loctree_findExtent,
+ * subfind_loctree.c:101-160); NULL computes it from each group's members.
+ * utherm and type may be NULL (no gas).  The context must be isolated
+ * (periodic = 0, pmgrid = 0).  Outputs: bound[n] (1 = member left), len[g]
+ * (members left; below des_link_ngb the group is dropped by the caller, as the
+ * reference does), and optionally len_non_gas[g], iterations[g] (passes counted
+ * as the reference's iter), potential[n] and bnd_energy[n] (PS.Potential and
+ * PS.BindingEnergy as last computed).  Members of equal minimum potential: the
+ * first in caller order is the reference point. */
+typedef struct
+{
+  double time_a;    /* All.Time: the scale factor when comoving */
+  double hubble_a;  /* hubble_function(All.Time), comoving only */
+  int comoving;     /* All.ComovingIntegrationOn */
+  int des_link_ngb; /* All.DesLinkNgb */
+  int max_iter;     /* MAXITER (allvars.h:482); <= 0: 300000 */
+} agrav_unbind_params;
+
+agrav_status agrav_subfind_unbind(agrav_ctx *ctx, int ngroups, const int64_t *offsets, const double *pos, const double *vel,
+                                  const double *mass, const double *softening, const double *utherm, const uint8_t *type,
+                                  const double *cubes, const agrav_unbind_params *up, uint8_t *bound, int32_t *len,
+                                  int32_t *len_non_gas, int32_t *iterations, double *potential, double *bnd_energy);
 
 /* ---- serving: continuous batching of independent solves ------------------
  * No counterpart in the reference, which solves independent sets one after

@@ -54,6 +54,23 @@ def make_params(periodic=True, box_size=1.0, pmgrid=0, asmth=1.25, rcut=4.5, G=1
     return p
 
 
+class UnbindParams(C.Structure):
+    _fields_ = [
+        ("G", C.c_double),
+        ("time_a", C.c_double),
+        ("hubble_a", C.c_double),
+        ("comoving", C.c_int),
+        ("theta", C.c_double),
+        ("des_link_ngb", C.c_int),
+        ("max_iter", C.c_int),
+    ]
+
+
+def make_unbind_params(G=1.0, time_a=1.0, hubble_a=0.0, comoving=False, theta=0.7, des_link_ngb=20,
+                       max_iter=300000):
+    return UnbindParams(G, time_a, hubble_a, int(comoving), theta, des_link_ngb, max_iter)
+
+
 def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
@@ -107,6 +124,13 @@ def _declare(L):
     L.orc_group_potential_bh.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_double, _dp]
     L.orc_group_potential_bh.restype = C.c_longlong
     L.orc_fft3d.argtypes = [C.c_int, _dp, _dp, C.c_int]
+    L.orc_group_potential_bh_cube.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_double, C.c_void_p, _dp]
+    L.orc_group_potential_bh_cube.restype = C.c_longlong
+    L.orc_loctree_extent.argtypes = [C.c_int, _dp, _dp]
+    L.orc_subfind_unbind.argtypes = [C.POINTER(UnbindParams), C.c_int, _dp, _dp, _dp, C.c_void_p, C.c_void_p,
+                                     C.c_double, C.c_void_p, _u8, _dp, _dp, C.POINTER(C.c_int),
+                                     C.POINTER(C.c_int)]
+    L.orc_subfind_unbind.restype = C.c_int
 
 
 def _pos(pos):
@@ -306,6 +330,47 @@ def group_potential_bh(pos, mass, h, theta):
     pot = np.zeros(len(pos))
     total = lib().orc_group_potential_bh(len(pos), pos, mass, h, theta, pot)
     return pot, total
+
+
+def group_potential_bh_cube(pos, mass, h, theta, cube=None):
+    pos = _pos(pos)
+    mass = np.ascontiguousarray(mass, dtype=np.float64)
+    pot = np.zeros(len(pos))
+    cb = None if cube is None else np.ascontiguousarray(cube, dtype=np.float64)
+    total = lib().orc_group_potential_bh_cube(len(pos), pos, mass, h, theta,
+                                              None if cb is None else cb.ctypes.data, pot)
+    return pot, total
+
+
+def loctree_extent(pos):
+    pos = _pos(pos)
+    cube = np.zeros(4)
+    lib().orc_loctree_extent(len(pos), pos, cube)
+    return cube
+
+
+def subfind_unbind(up, pos, vel, mass, eps, utherm=None, ptype=None, cube=None):
+    """subfind_unbind (subfind_serial.c:429-637) of one isolated group.
+
+    Returns dict(bound[n] bool, len, len_non_gas, iterations, potential[n], energy[n]); len is -1 when the
+    loop passed max_iter."""
+    pos = _pos(pos)
+    vel = _pos(vel)
+    n = len(pos)
+    mass = np.ascontiguousarray(mass, dtype=np.float64)
+    ut = None if utherm is None else np.ascontiguousarray(utherm, dtype=np.float64)
+    ty = None if ptype is None else np.ascontiguousarray(ptype, dtype=np.uint8)
+    cb = None if cube is None else np.ascontiguousarray(cube, dtype=np.float64)
+    bound = np.zeros(max(n, 1), dtype=np.uint8)
+    pot = np.zeros(max(n, 1))
+    en = np.zeros(max(n, 1))
+    lng = C.c_int(0)
+    it = C.c_int(0)
+    ln = lib().orc_subfind_unbind(C.byref(up), n, pos, vel, mass, None if ut is None else ut.ctypes.data,
+                                  None if ty is None else ty.ctypes.data, eps,
+                                  None if cb is None else cb.ctypes.data, bound, pot, en, C.byref(lng), C.byref(it))
+    return dict(bound=bound[:n].astype(bool), len=ln, len_non_gas=lng.value, iterations=it.value,
+                potential=pot[:n], energy=en[:n])
 
 
 def fft3d(re, im, sign):

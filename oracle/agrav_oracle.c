@@ -356,6 +356,12 @@ void orc_tree_free(orc_tree *t)
 int orc_tree_build(const orc_params *p, int n, const double *pos, const double *mass, const uint8_t *stype, double alloc_factor,
                    orc_tree **out)
 {
+  return orc_tree_build_cube(p, n, pos, mass, stype, alloc_factor, NULL, out);
+}
+
+int orc_tree_build_cube(const orc_params *p, int n, const double *pos, const double *mass, const uint8_t *stype,
+                        double alloc_factor, const double *cube, orc_tree **out)
+{
   *out = NULL;
   if(n <= 0)
     return 0;
@@ -386,7 +392,19 @@ int orc_tree_build(const orc_params *p, int n, const double *pos, const double *
   t->box      = p->box_size;
   orc_set_softenings(p, t->force_softening);
 
-  orc_domain_extent(p, n, pos, t->corner, t->dcenter, &t->dlen);
+  if(cube)
+    {
+      /* a given root cube {centre[3], len}, as the subfind local tree keeps one per
+       * group (subfind_loctree.c:169-179, :203-206) */
+      t->dlen = cube[3];
+      for(int j = 0; j < 3; j++)
+        {
+          t->dcenter[j] = cube[j];
+          t->corner[j]  = cube[j] - 0.5 * cube[3];
+        }
+    }
+  else
+    orc_domain_extent(p, n, pos, t->corner, t->dcenter, &t->dlen);
   t->bigfac   = t->dlen / (double)(1ll << 52);
   double ilen = 1.0 / t->dlen;
 
@@ -1433,6 +1451,12 @@ long long orc_compute_grav_accelerations(const orc_params *p, int n, const doubl
 
 long long orc_group_potential_bh(int n, const double *pos, const double *mass, double h, double theta, double *pot)
 {
+  return orc_group_potential_bh_cube(n, pos, mass, h, theta, NULL, pot);
+}
+
+long long orc_group_potential_bh_cube(int n, const double *pos, const double *mass, double h, double theta, const double *cube,
+                                      double *pot)
+{
   orc_params p;
   memset(&p, 0, sizeof(p));
   p.periodic = 0;
@@ -1441,7 +1465,7 @@ long long orc_group_potential_bh(int n, const double *pos, const double *mass, d
   p.soft[0]  = h / 2.8;
   orc_tree *t = NULL;
   double af   = n < 1000 ? 10.0 : 0.7;
-  while(orc_tree_build(&p, n, pos, mass, NULL, af, &t) < 0)
+  while(orc_tree_build_cube(&p, n, pos, mass, NULL, af, cube, &t) < 0)
     af *= 1.15;
   if(!t)
     return 0;
@@ -1498,4 +1522,229 @@ long long orc_group_potential_bh(int n, const double *pos, const double *mass, d
     }
   orc_tree_free(t);
   return total;
+}
+
+/* ------------------------------------------------------------------ */
+/* SUBFIND unbinding                                                   */
+/* ------------------------------------------------------------------ */
+
+void orc_loctree_extent(int n, const double *pos, double cube[4])
+{
+  /* subfind_loctree_findExtent, subfind_loctree.c:101-160: the largest extent of the
+   * bounding box times 1.001, centred on the box */
+  double xmin[3], xmax[3];
+  for(int j = 0; j < 3; j++)
+    {
+      xmin[j] = 1.0e306; /* MAX_REAL_NUMBER */
+      xmax[j] = -1.0e306;
+    }
+  for(int i = 0; i < n; i++)
+    for(int j = 0; j < 3; j++)
+      {
+        if(xmin[j] > pos[3 * i + j])
+          xmin[j] = pos[3 * i + j];
+        if(xmax[j] < pos[3 * i + j])
+          xmax[j] = pos[3 * i + j];
+      }
+  double len = 0;
+  for(int j = 0; j < 3; j++)
+    if(xmax[j] - xmin[j] > len)
+      len = xmax[j] - xmin[j];
+  len *= 1.001;
+  /* a group whose particles all coincide has no extent; any cube then serves */
+  if(!(len > 0))
+    len = 1.0;
+  for(int j = 0; j < 3; j++)
+    cube[j] = n > 0 ? 0.5 * (xmin[j] + xmax[j]) : 0.0;
+  cube[3] = len;
+}
+
+static int cmp_desc(const void *a, const void *b)
+{
+  /* subfind_compare_binding_energy: largest first */
+  double x = *(const double *)a, y = *(const double *)b;
+  return x > y ? -1 : (x < y ? 1 : 0);
+}
+
+int orc_subfind_unbind(const orc_unbind_params *up, int n, const double *pos, const double *vel, const double *mass,
+                       const double *utherm, const uint8_t *type, double eps, const double *cube_in, uint8_t *bound,
+                       double *potential, double *bnd_energy_out, int *len_non_gas_out, int *iterations_out)
+{
+  /* subfind_unbind, subfind_serial.c:429-637, for one group of an isolated set
+   * (positions need no nearest-image wrap).  The local tree's root cube is kept
+   * for every iteration (subfind_serial.c:107, subfind_loctree.c:192-206). */
+  const double G        = up->G;
+  const double cf_atime = up->comoving ? up->time_a : 1.0;
+  const double vel_to_phys = up->comoving ? 1.0 / up->time_a : 1.0; /* :441-452 */
+  const double atime       = up->comoving ? up->time_a : 1.0;
+  const double H_of_a      = up->comoving ? up->hubble_a : 0.0;
+  const double h           = 2.8 * eps; /* ForceSoftening, grav_softening.c:90-95 */
+  double cube[4];
+  if(cube_in)
+    memcpy(cube, cube_in, sizeof(cube));
+  else
+    orc_loctree_extent(n, pos, cube);
+
+  int *ud        = (int *)malloc(sizeof(int) * (n > 0 ? n : 1));
+  double *pot    = (double *)calloc(n > 0 ? n : 1, sizeof(double));
+  double *E      = (double *)calloc(n > 0 ? n : 1, sizeof(double));
+  double *sorted = (double *)malloc(sizeof(double) * (n > 0 ? n : 1));
+  double *spos   = (double *)malloc(sizeof(double) * 3 * (n > 0 ? n : 1));
+  double *smass  = (double *)malloc(sizeof(double) * (n > 0 ? n : 1));
+  double *spot   = (double *)malloc(sizeof(double) * (n > 0 ? n : 1));
+  for(int i = 0; i < n; i++)
+    ud[i] = i;
+  int len = n, phaseflag = 0, iter = 0, unbound = 0, len_non_gas = 0, rc = 0;
+  double weakly_bound_limit = 0, minpot = 0, ppos[3] = {0, 0, 0};
+  if(n > 0)
+    do
+      {
+        if(phaseflag == 0)
+          {
+            /* potential of every member with a tree over the current members (:458-477) */
+            for(int i = 0; i < len; i++)
+              {
+                int p = ud[i];
+                for(int j = 0; j < 3; j++)
+                  spos[3 * i + j] = pos[3 * p + j];
+                smass[i] = mass[p];
+              }
+            orc_group_potential_bh_cube(len, spos, smass, h, up->theta, cube, spot);
+            int minindex = -1;
+            minpot       = 1.0e30;
+            for(int i = 0; i < len; i++)
+              {
+                int p  = ud[i];
+                pot[p] = G / cf_atime * spot[i];
+                if(pot[p] < minpot || minindex == -1)
+                  {
+                    minpot   = pot[p];
+                    minindex = p;
+                  }
+              }
+            for(int j = 0; j < 3; j++)
+              ppos[j] = pos[3 * minindex + j];
+          }
+        else
+          {
+            /* :491-497: the reference evaluates the potential of the weakly bound
+             * members again but keeps the old value and scales it by G/a once more
+             * (the new value is discarded), so only that scaling is restated */
+            for(int i = 0; i < len; i++)
+              {
+                int p = ud[i];
+                if(E[p] >= weakly_bound_limit)
+                  pot[p] *= G / cf_atime;
+              }
+          }
+
+        /* bulk velocity and centre of mass (:501-532) */
+        double v[3] = {0, 0, 0}, s[3] = {0, 0, 0}, TotMass = 0;
+        for(int i = 0; i < len; i++)
+          {
+            int p = ud[i];
+            for(int j = 0; j < 3; j++)
+              {
+                double ddxx = pos[3 * p + j] - ppos[j];
+                s[j] += mass[p] * ddxx;
+                v[j] += mass[p] * vel[3 * p + j];
+              }
+            TotMass += mass[p];
+          }
+        for(int j = 0; j < 3; j++)
+          {
+            v[j] /= TotMass;
+            s[j] /= TotMass;
+            s[j] += ppos[j];
+          }
+
+        /* binding energies (:534-556) */
+        for(int i = 0; i < len; i++)
+          {
+            int p = ud[i];
+            double dv[3];
+            for(int j = 0; j < 3; j++)
+              {
+                dv[j]     = vel_to_phys * (vel[3 * p + j] - v[j]);
+                double dx = atime * (pos[3 * p + j] - s[j]);
+                dv[j] += H_of_a * dx;
+              }
+            E[p] = pot[p] + 0.5 * (dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
+            E[p] += G / cf_atime * mass[p] / (h / 2.8); /* self-energy */
+            if(type && type[p] == 0 && utherm)
+              E[p] += utherm[p];
+            sorted[i] = E[p];
+          }
+
+        qsort(sorted, len, sizeof(double), cmp_desc); /* :558 */
+        double energy_limit = sorted[(int)(0.25 * len)];
+        int i;
+        for(i = 0, unbound = 0; i < len - 1; i++)
+          {
+            if(sorted[i] > 0)
+              unbound++;
+            else
+              unbound--;
+            if(unbound <= 0)
+              break;
+          }
+        weakly_bound_limit = sorted[i];
+
+        /* omit unbound members, at most a quarter of them (:574-590) */
+        for(i = 0, unbound = 0, len_non_gas = 0; i < len; i++)
+          {
+            int p = ud[i];
+            if(E[p] > 0 && E[p] > energy_limit)
+              {
+                unbound++;
+                ud[i] = ud[len - 1];
+                i--;
+                len--;
+              }
+            else if(!type || type[p] != 0)
+              len_non_gas++;
+          }
+
+        if(len < up->des_link_ngb)
+          break;
+
+        if(phaseflag == 0)
+          {
+            if(unbound > 0)
+              phaseflag = 1;
+          }
+        else if(unbound == 0)
+          {
+            phaseflag = 0; /* one more pass over all members */
+            unbound   = 1;
+          }
+
+        if(iter++ > up->max_iter)
+          {
+            rc = -1;
+            break;
+          }
+      }
+    while(unbound > 0);
+
+  for(int q = 0; q < n; q++)
+    bound[q] = 0;
+  for(int q = 0; q < len; q++)
+    bound[ud[q]] = 1;
+  if(potential)
+    memcpy(potential, pot, sizeof(double) * n);
+  if(bnd_energy_out)
+    memcpy(bnd_energy_out, E, sizeof(double) * n);
+  if(len_non_gas_out)
+    *len_non_gas_out = len_non_gas;
+  if(iterations_out)
+    *iterations_out = iter;
+  free(ud);
+  free(pot);
+  free(E);
+  free(sorted);
+  free(spos);
+  free(smass);
+  free(spot);
+  return rc < 0 ? rc : len;
 }

@@ -56,6 +56,9 @@ void orc_domain_extent(const orc_params *p, int n, const double *pos, double cor
  * Returns number of nodes, or -1 when alloc_factor*n nodes are too few. */
 int orc_tree_build(const orc_params *p, int n, const double *pos, const double *mass, const uint8_t *stype, double alloc_factor,
                    orc_tree **out);
+/* Same with a given root cube {centre[3], len} (NULL: the domain cube of domain_box.c) */
+int orc_tree_build_cube(const orc_params *p, int n, const double *pos, const double *mass, const uint8_t *stype,
+                        double alloc_factor, const double *cube, orc_tree **out);
 void orc_tree_free(orc_tree *t);
 int orc_tree_numnodes(const orc_tree *t);
 /* Copy node data: center[3M], len[M], com[3M], mass[M], level[M], npart[M] (any may be NULL) */
@@ -111,6 +114,32 @@ void orc_direct_summation(const orc_params *p, int n, const double *pos, const d
 /* Unbinding potential of one isolated group with a local BH tree: subfind_loctree.c:564-730 (potential only,
  * ErrTolThetaSubfind criterion, spline softening, G=1).  Returns interactions. */
 long long orc_group_potential_bh(int n, const double *pos, const double *mass, double h, double theta, double *pot);
+/* Same with the local tree's root cube {centre[3], len} given (NULL: isolated domain cube) */
+long long orc_group_potential_bh_cube(int n, const double *pos, const double *mass, double h, double theta, const double *cube,
+                                      double *pot);
+
+/* subfind_loctree_findExtent, subfind_loctree.c:101-160: cube[4] = {centre[3], 1.001 x largest extent} */
+void orc_loctree_extent(int n, const double *pos, double cube[4]);
+
+typedef struct
+{
+  double G;          /* All.G */
+  double time_a;     /* All.Time (scale factor when comoving) */
+  double hubble_a;   /* hubble_function(All.Time) */
+  int comoving;      /* All.ComovingIntegrationOn */
+  double theta;      /* All.ErrTolThetaSubfind */
+  int des_link_ngb;  /* All.DesLinkNgb */
+  int max_iter;      /* MAXITER, allvars.h:482 */
+} orc_unbind_params;
+
+/* subfind_unbind, subfind_serial.c:429-637, one isolated group; eps = Plummer-equivalent softening
+ * (ForceSoftening/2.8).  cube: local-tree root {centre[3], len} or NULL for the group's own extent.
+ * utherm/type may be NULL (no gas).  bound[n] = 1 for the members left.  potential / bnd_energy (n, may
+ * be NULL): PS.Potential and PS.BindingEnergy as last computed (removed members keep their last values).
+ * Returns the number left, or -1 past max_iter iterations. */
+int orc_subfind_unbind(const orc_unbind_params *up, int n, const double *pos, const double *vel, const double *mass,
+                       const double *utherm, const uint8_t *type, double eps, const double *cube, uint8_t *bound,
+                       double *potential, double *bnd_energy, int *len_non_gas, int *iterations);
 
 /* Textbook in-place complex FFT of a cube (n^3, n power of two); sign -1 forward (FFTW_FORWARD), +1 backward; unnormalised */
 void orc_fft3d(int n, double *re, double *im, int sign);
