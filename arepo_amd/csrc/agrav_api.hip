@@ -351,6 +351,7 @@ void agrav_destroy(agrav_ctx *c)
   c->d_sdom.release();
   c->d_shd.release();
   c->d_groups.release();
+  c->d_gsets.release();
   c->d_at.release();
   c->d_pt.release();
   c->d_nint.release();

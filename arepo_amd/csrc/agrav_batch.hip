@@ -71,21 +71,75 @@ static void make_groups(const agrav_ctx *c, const std::vector<int> &sets, const 
     }
 }
 
+struct ClassTuning
+{
+  int gs[AGRAV_BATCH_CLASSES], lb[AGRAV_BATCH_CLASSES];
+};
+
+// the walk groups of one set per workgroup: set q of the size order starts at group
+// sg[q].y and is cut into groups of its class's size
+__global__ void k_expand_groups(const int2 *__restrict__ sg, const int64_t *__restrict__ off, ClassTuning ct, int4 *groups)
+{
+  const int s = sg[blockIdx.x].x, base = sg[blockIdx.x].y;
+  const int64_t f = off[s], n = off[s + 1] - f;
+  int k = AGRAV_BATCH_CLASSES - 1;
+  if(n <= 128)
+    k = 0;
+  else if(n <= 2048)
+    k = 1;
+  else if(n <= 32768)
+    k = 2;
+  const int g = ct.gs[k], ng = (int)((n + g - 1) / g);
+  for(int j = threadIdx.x; j < ng; j += blockDim.x)
+    {
+      const int64_t p = (int64_t)j * g;
+      groups[base + j] = make_int4((int)(f + p), (int)(n - p < g ? n - p : g), s, ct.lb[k]);
+    }
+}
+
 agrav_status batch_groups(agrav_ctx *c)
 {
   if(!c->batch_groups_dirty)
     return AGRAV_OK;
-  std::vector<int> sets(c->batch_nsets);
-  std::iota(sets.begin(), sets.end(), 0);
-  std::vector<int4> g;
-  make_groups(c, sets, c->batch_gs, c->batch_lb, g);
+  // the same table as make_groups, written on the device: the host only orders the sets
+  // (largest first) and places each set's first group
+  static_assert(sizeof(kClassMax) / sizeof(kClassMax[0]) == 4, "the expansion kernel knows four classes");
+  const int nsets = c->batch_nsets;
+  std::vector<int> order(nsets);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return c->h_soff[a + 1] - c->h_soff[a] > c->h_soff[b + 1] - c->h_soff[b];
+  });
+  // kept in the context: the upload below is asynchronous
+  std::vector<int2> &sg = c->h_gsets;
+  sg.clear();
+  sg.reserve(nsets);
+  int64_t total = 0;
+  for(int s : order)
+    {
+      const int64_t n = c->h_soff[s + 1] - c->h_soff[s];
+      if(n == 0)
+        continue;
+      sg.push_back(make_int2(s, (int)total));
+      const int g = c->batch_gs[size_class(n)];
+      total += (n + g - 1) / g;
+    }
   agrav_status st;
-  if((st = c->d_groups.ensure(g.size())) != AGRAV_OK)
+  if((st = c->d_groups.ensure(total)) != AGRAV_OK || (st = c->d_gsets.ensure(sg.size())) != AGRAV_OK)
     return st;
-  if(!g.empty())
-    AG_HIP(hipMemcpyAsync(c->d_groups.p, g.data(), g.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream));
-  AG_HIP(hipStreamSynchronize(c->stream));
-  c->batch_ngroups      = (int)g.size();
+  if(!sg.empty())
+    {
+      ClassTuning ct;
+      for(int k = 0; k < AGRAV_BATCH_CLASSES; k++)
+        {
+          ct.gs[k] = c->batch_gs[k];
+          ct.lb[k] = c->batch_lb[k];
+        }
+      AG_HIP(hipMemcpyAsync(c->d_gsets.p, sg.data(), sg.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream));
+      k_expand_groups<<<(unsigned)sg.size(), 256, 0, c->stream>>>(c->d_gsets.p, c->d_soff.p, ct, c->d_groups.p);
+      AG_HIP(hipGetLastError());
+    }
+  c->batch_ngroups      = (int)total;
   c->batch_groups_dirty = false;
   return AGRAV_OK;
 }

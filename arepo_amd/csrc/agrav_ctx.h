@@ -63,17 +63,22 @@ struct PmState
 // device buffers of the unbinding loop (agrav_unbind.hip), kept between calls
 struct UnbindBufs
 {
-  DBuf<double> pos, vel, mass, uth, pot, E, cube, epsr, wbl, ppos;
+  DBuf<double> pos, vel, mass, uth, pot, E, cube, epsr, wbl, lim, ppos, sv, cpart, cmin;
+  DBuf<long long> cimin;
   DBuf<uint8_t> type, alive;
-  DBuf<int> gid, flag, dst, cidx, glist, gphase, res, cnt;
+  DBuf<int> gid, flag, dst, cidx, glist, gphase, res, cnt, glen, cbeg, ccnt;
+  DBuf<int2> chunk;
   DBuf<int64_t> off, soff;
   DBuf<char> scan_tmp;
+  std::vector<int2> h_chunk;  // host copy of chunk (source of its asynchronous upload)
   void release()
   {
     pos.release(), vel.release(), mass.release(), uth.release(), pot.release(), E.release();
-    cube.release(), epsr.release(), wbl.release(), ppos.release(), type.release(), alive.release();
+    cube.release(), epsr.release(), wbl.release(), lim.release(), ppos.release(), sv.release();
+    cpart.release(), cmin.release(), cimin.release(), type.release(), alive.release();
     gid.release(), flag.release(), dst.release(), cidx.release(), glist.release(), gphase.release();
-    res.release(), cnt.release(), off.release(), soff.release(), scan_tmp.release();
+    res.release(), cnt.release(), glen.release(), cbeg.release(), ccnt.release(), chunk.release();
+    off.release(), soff.release(), scan_tmp.release();
   }
 };
 
@@ -115,6 +120,8 @@ struct agrav_ctx
   agrav::DBuf<double> d_shd;          // softening length h = 2.8 eps per set
   const double *batch_cubes = nullptr; // device {centre[3], len} per set: given root cubes, or null
   agrav::DBuf<int4> d_groups;         // walk groups {first target, count, set, leaf bucket}
+  agrav::DBuf<int2> d_gsets;          // {set, first walk group} of each non-empty set, largest first
+  std::vector<int2> h_gsets;          // host copy of d_gsets (source of its asynchronous upload)
   int batch_ngroups = 0;
   bool batch_groups_dirty = true;
   int batch_gs[4] = {64, 64, 64, 64}; // group size per set-size class

@@ -162,17 +162,27 @@ __device__ inline double okey_inv(uint64_t k)
 struct UnbArgs
 {
   const int64_t *off;
-  const int *glist, *gphase;
+  const int *gphase;               // per group: 0 full pass, 1 energies only, 2 done
+  const int2 *chunk;               // {group, first member} per chunk of at most CH members
+  const int *cbeg;                 // first chunk of every group (ngroups + 1)
   const double *pos, *vel, *mass, *uth, *epsr;
   const uint8_t *type;
-  double *pot, *E, *wbl, *ppos;
+  double *pot, *E, *wbl, *lim, *ppos, *sv;
   uint8_t *alive;
+  double *cpart;                   // per chunk: 7 partial sums (centre of mass, bulk velocity, mass)
+  double *cmin;                    // per chunk: minimum potential ...
+  long long *cimin;                //            ... and its member (-1: none)
+  int *ccnt;                       // per chunk: two counts
+  int *glen;                       // per group: {len, P = #(E > 0)}
   int *res;                        // per group {unbound, len left, len_non_gas}
   double g_a, vel_to_phys, atime, hubble;
 };
 
-// fixed-order block sum of NV doubles (wave butterflies, then the waves in order)
-template <int NV> __device__ inline void block_sum(double (&v)[NV], double (*s_red)[8])
+static constexpr int CB = 256;          // threads of the per-chunk kernels
+static constexpr int CH = 16 * CB;      // members per chunk
+
+// fixed-order sum over the block of NV doubles (wave butterflies, then the waves in order)
+template <int NT, int NV> __device__ inline void block_sum(double (&v)[NV], double (*s_red)[8])
 {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for(int q = 0; q < NV; q++)
@@ -185,93 +195,109 @@ template <int NV> __device__ inline void block_sum(double (&v)[NV], double (*s_r
   for(int q = 0; q < NV; q++)
     {
       double t = s_red[0][q];
-      for(int k = 1; k < UW; k++)
+      for(int k = 1; k < NT / 64; k++)
         t += s_red[k][q];
       v[q] = t;
     }
   __syncthreads();
 }
 
-// One pass of subfind_unbind (subfind_serial.c:455-607) for one group per workgroup.
-__global__ __launch_bounds__(UB) void k_unb_pass(UnbArgs a)
+// (minimum, member) pairs: the smaller potential, the first member on ties, -1 = none
+__device__ inline void min_merge(double &best, long long &bi, double ob, long long oi)
 {
-  __shared__ double s_red[UW][8];
-  __shared__ double s_pmin[UW];
-  __shared__ long long s_imin[UW];
-  __shared__ int s_hist[2][256];
-  __shared__ unsigned long long s_pref[2];
-  __shared__ int s_rank[2];
-  const int g = a.glist[blockIdx.x];
-  const int64_t f = a.off[g], e = a.off[g + 1];
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const int phase = a.gphase[g];
-
-  double px, py, pz;
-  if(phase == 1)
+  if(oi >= 0 && (bi < 0 || ob < best || (ob == best && oi < bi)))
     {
-      // :491-497: the weakly bound members' potentials are scaled by G/a again (the
-      // reference evaluates them anew but keeps the old value)
-      const double lim = a.wbl[g];
-      for(int64_t i = f + t; i < e; i += UB)
-        if(a.alive[i] && a.E[i] >= lim)
-          a.pot[i] *= a.g_a;
-      px = a.ppos[3 * g];
-      py = a.ppos[3 * g + 1];
-      pz = a.ppos[3 * g + 2];
+      best = ob;
+      bi   = oi;
     }
-  else
-    {
-      // member of minimum potential (:460-471): the first one in member order on ties
-      double best  = 1.0e30;
-      long long bi = -1;
-      for(int64_t i = f + t; i < e; i += UB)
-        if(a.alive[i])
-          {
-            const double p = a.pot[i];
-            if(bi < 0 || p < best)
-              {
-                best = p;
-                bi   = i;
-              }
-          }
-      for(int o = 32; o > 0; o >>= 1)
-        {
-          const double ob     = __shfl_xor(best, o, 64);
-          const long long oi  = __shfl_xor(bi, o, 64);
-          const bool take     = oi >= 0 && (bi < 0 || ob < best || (ob == best && oi < bi));
-          best                = take ? ob : best;
-          bi                  = take ? oi : bi;
-        }
-      if(lane == 0)
-        {
-          s_pmin[w] = best;
-          s_imin[w] = bi;
-        }
-      __syncthreads();
-      best = s_pmin[0];
-      bi   = s_imin[0];
-      for(int k = 1; k < UW; k++)
-        if(s_imin[k] >= 0 && (bi < 0 || s_pmin[k] < best || (s_pmin[k] == best && s_imin[k] < bi)))
-          {
-            best = s_pmin[k];
-            bi   = s_imin[k];
-          }
-      // a group in the loop always has a member left (the removal keeps the rank-0.25 len one)
-      px = bi >= 0 ? a.pos[3 * bi] : 0.0;
-      py = bi >= 0 ? a.pos[3 * bi + 1] : 0.0;
-      pz = bi >= 0 ? a.pos[3 * bi + 2] : 0.0;
-      __syncthreads();
-      if(t == 0)
-        {
-          a.ppos[3 * g]     = px;
-          a.ppos[3 * g + 1] = py;
-          a.ppos[3 * g + 2] = pz;
-        }
-    }
+}
 
-  // bulk velocity and centre of mass (:501-532)
+__device__ inline void wave_min(double &best, long long &bi)
+{
+  for(int o = 32; o > 0; o >>= 1)
+    min_merge(best, bi, __shfl_xor(best, o, 64), __shfl_xor(bi, o, 64));
+}
+
+// the members of a chunk of an active group: [f, e), or an empty range
+__device__ inline int chunk_range(const UnbArgs &a, int c, int64_t &f, int64_t &e)
+{
+  const int2 ch = a.chunk[c];
+  f             = ch.y;
+  e             = f + CH < a.off[ch.x + 1] ? f + CH : a.off[ch.x + 1];
+  return ch.x;
+}
+
+// 1. energy-only pass (:491-497): the weakly bound members' potentials are scaled by
+// G/a again (the reference evaluates them anew but keeps the old value); full pass:
+// each chunk's member of minimum potential (:460-471)
+__global__ __launch_bounds__(CB) void k_uc_min(UnbArgs a)
+{
+  __shared__ double s_min[CB / 64];
+  __shared__ long long s_imin[CB / 64];
+  int64_t f, e;
+  const int g = chunk_range(a, blockIdx.x, f, e), ph = a.gphase[g];
+  if(ph == 2)
+    return;
+  if(ph == 1)
+    {
+      if(a.g_a != 1.0)
+        {
+          const double lim = a.wbl[g];
+          for(int64_t i = f + threadIdx.x; i < e; i += CB)
+            if(a.alive[i] && a.E[i] >= lim)
+              a.pot[i] *= a.g_a;
+        }
+      return;
+    }
+  double best  = 0;
+  long long bi = -1;
+  for(int64_t i = f + threadIdx.x; i < e; i += CB)
+    if(a.alive[i])
+      min_merge(best, bi, a.pot[i], i);
+  wave_min(best, bi);
+  if((threadIdx.x & 63) == 0)
+    {
+      s_min[threadIdx.x >> 6]  = best;
+      s_imin[threadIdx.x >> 6] = bi;
+    }
+  __syncthreads();
+  if(threadIdx.x == 0)
+    {
+      for(int w = 1; w < CB / 64; w++)
+        min_merge(best, bi, s_min[w], s_imin[w]);
+      a.cmin[blockIdx.x]  = best;
+      a.cimin[blockIdx.x] = bi;
+    }
+}
+
+// 2. per group (one wavefront): the member of minimum potential over its chunks in order
+__global__ void k_ug_min(UnbArgs a, int ng)
+{
+  const int g = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if(g >= ng || a.gphase[g] != 0)
+    return;
+  double best  = 0;
+  long long bi = -1;
+  for(int c = a.cbeg[g] + lane; c < a.cbeg[g + 1]; c += 64)
+    min_merge(best, bi, a.cmin[c], a.cimin[c]);
+  wave_min(best, bi);
+  // a group in the loop always has a member left (the removal keeps the rank-0.25 len one)
+  if(lane == 0)
+    for(int d = 0; d < 3; d++)
+      a.ppos[3 * g + d] = bi >= 0 ? a.pos[3 * bi + d] : 0.0;
+}
+
+// 3. bulk velocity and centre of mass (:501-532): chunk partial sums
+__global__ __launch_bounds__(CB) void k_uc_sums(UnbArgs a)
+{
+  __shared__ double s_red[CB / 64][8];
+  int64_t f, e;
+  const int g = chunk_range(a, blockIdx.x, f, e);
+  if(a.gphase[g] == 2)
+    return;
+  const double px = a.ppos[3 * g], py = a.ppos[3 * g + 1], pz = a.ppos[3 * g + 2];
   double v[7] = {0, 0, 0, 0, 0, 0, 0};
-  for(int64_t i = f + t; i < e; i += UB)
+  for(int64_t i = f + threadIdx.x; i < e; i += CB)
     if(a.alive[i])
       {
         const double m = a.mass[i];
@@ -283,14 +309,49 @@ __global__ __launch_bounds__(UB) void k_unb_pass(UnbArgs a)
         v[5] += m * a.vel[3 * i + 2];
         v[6] += m;
       }
-  block_sum<7>(v, s_red);
-  const double tot = v[6];
-  const double vx = v[3] / tot, vy = v[4] / tot, vz = v[5] / tot;
-  const double sx = v[0] / tot + px, sy = v[1] / tot + py, sz = v[2] / tot + pz;
+  block_sum<CB, 7>(v, s_red);
+  if(threadIdx.x < 7)
+    a.cpart[8 * (int64_t)blockIdx.x + threadIdx.x] = v[threadIdx.x];
+}
 
-  // binding energies (:534-556), with the self-energy G/a m / eps added back
-  int nl = 0, npos = 0;
-  for(int64_t i = f + t; i < e; i += UB)
+// 4. per group (one wavefront): s = ppos + sum m (x - ppos) / M, v = sum m vel / M
+__global__ void k_ug_sums(UnbArgs a, int ng)
+{
+  const int g = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if(g >= ng || a.gphase[g] == 2)
+    return;
+  double v[7] = {0, 0, 0, 0, 0, 0, 0};
+  for(int c = a.cbeg[g] + lane; c < a.cbeg[g + 1]; c += 64)
+    for(int q = 0; q < 7; q++)
+      v[q] += a.cpart[8 * (int64_t)c + q];
+  for(int q = 0; q < 7; q++)
+    for(int o = 32; o > 0; o >>= 1)
+      v[q] += __shfl_xor(v[q], o, 64);
+  if(lane == 0)
+    {
+      const double tot = v[6];
+      for(int d = 0; d < 3; d++)
+        {
+          a.sv[6 * g + d]     = v[d] / tot + a.ppos[3 * g + d];
+          a.sv[6 * g + 3 + d] = v[3 + d] / tot;
+        }
+    }
+}
+
+// 5. binding energies (:534-556), with the self-energy G/a m / eps added back; chunk
+// counts of members and of positive energies
+__global__ __launch_bounds__(CB) void k_uc_energy(UnbArgs a)
+{
+  __shared__ double s_red[CB / 64][8];
+  int64_t f, e;
+  const int g = chunk_range(a, blockIdx.x, f, e);
+  if(a.gphase[g] == 2)
+    return;
+  const double sx = a.sv[6 * g], sy = a.sv[6 * g + 1], sz = a.sv[6 * g + 2];
+  const double vx = a.sv[6 * g + 3], vy = a.sv[6 * g + 4], vz = a.sv[6 * g + 5];
+  const double eps = a.epsr[g];
+  double cnt[2] = {0, 0};
+  for(int64_t i = f + threadIdx.x; i < e; i += CB)
     if(a.alive[i])
       {
         double dvx = a.vel_to_phys * (a.vel[3 * i] - vx);
@@ -300,24 +361,71 @@ __global__ __launch_bounds__(UB) void k_unb_pass(UnbArgs a)
         dvy += a.hubble * (a.atime * (a.pos[3 * i + 1] - sy));
         dvz += a.hubble * (a.atime * (a.pos[3 * i + 2] - sz));
         double en = a.pot[i] + 0.5 * (dvx * dvx + dvy * dvy + dvz * dvz);
-        en += a.g_a * a.mass[i] / a.epsr[g];
+        en += a.g_a * a.mass[i] / eps;
         if(a.type && a.uth && a.type[i] == 0)
           en += a.uth[i];
         a.E[i] = en;
-        nl++;
-        npos += en > 0;
+        cnt[0] += 1.0;
+        cnt[1] += en > 0 ? 1.0 : 0.0;
       }
-  double cnt[2] = {(double)nl, (double)npos};
-  block_sum<2>(cnt, s_red);
-  const int len = (int)cnt[0], P = (int)cnt[1];
+  block_sum<CB, 2>(cnt, s_red);
+  if(threadIdx.x == 0)
+    {
+      a.ccnt[2 * blockIdx.x]     = (int)cnt[0];
+      a.ccnt[2 * blockIdx.x + 1] = (int)cnt[1];
+    }
+}
 
-  // energy_limit = 0.25 len-th largest energy (:560); weakly_bound_limit = the
-  // energy where the running count of (E > 0) minus (E <= 0) over the sorted list
-  // first drops to zero (:562-572), i.e. the min(max(2P - 1, 0), len - 1)-th largest
+// 6. per group (one wavefront): members and positive energies
+__global__ void k_ug_count(UnbArgs a, int ng)
+{
+  const int g = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if(g >= ng || a.gphase[g] == 2)
+    return;
+  int n = 0, p = 0;
+  for(int c = a.cbeg[g] + lane; c < a.cbeg[g + 1]; c += 64)
+    {
+      n += a.ccnt[2 * c];
+      p += a.ccnt[2 * c + 1];
+    }
+  for(int o = 32; o > 0; o >>= 1)
+    {
+      n += __shfl_xor(n, o, 64);
+      p += __shfl_xor(p, o, 64);
+    }
+  if(lane == 0)
+    {
+      a.glen[2 * g]     = n;
+      a.glen[2 * g + 1] = p;
+    }
+}
+
+// 7. energy_limit = 0.25 len-th largest energy (:560); weakly_bound_limit = the energy
+// where the running count of (E > 0) minus (E <= 0) over the sorted list first drops
+// to zero (:562-572), i.e. the min(max(2P - 1, 0), len - 1)-th largest.  With at most
+// 0.25 len positive energies the energy limit is <= 0 and the removal is exactly
+// {E > 0}; the weakly bound limit only scales potentials by G/a, which is 1 in
+// physical runs with G = 1.  Either order statistic is selected (exact 8-bit radix
+// select on order-preserving keys, one workgroup per group) only when it changes
+// the outcome.
+__global__ __launch_bounds__(UB) void k_ug_select(UnbArgs a, const int *__restrict__ glist)
+{
+  __shared__ int s_hist[2][256];
+  __shared__ unsigned long long s_pref[2];
+  __shared__ int s_rank[2];
+  const int g = glist[blockIdx.x];
+  const int len = a.glen[2 * g], P = a.glen[2 * g + 1];
+  const int k1        = (int)(0.25 * len);
+  const bool need_lim = P > k1;
+  const bool need_wbl = a.g_a != 1.0;
+  if(!need_lim && !need_wbl)
+    return;
+  const int64_t f = a.off[g], e = a.off[g + 1];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   if(t < 2)
     {
       s_pref[t] = 0;
-      s_rank[t] = t == 0 ? (int)(0.25 * len) : min(max(2 * P - 1, 0), len - 1);
+      s_rank[t] = t == 0 ? k1 : min(max(2 * P - 1, 0), len - 1);
     }
   for(int shift = 56; shift >= 0; shift -= 8)
     {
@@ -331,9 +439,9 @@ __global__ __launch_bounds__(UB) void k_unb_pass(UnbArgs a)
           {
             const uint64_t k = okey(a.E[i]);
             const int d      = (int)((k >> shift) & 255);
-            if((k & mask) == p0)
+            if(need_lim && (k & mask) == p0)
               atomicAdd(&s_hist[0][d], 1);
-            if((k & mask) == p1)
+            if(need_wbl && (k & mask) == p1)
               atomicAdd(&s_hist[1][d], 1);
           }
       __syncthreads();
@@ -353,31 +461,85 @@ __global__ __launch_bounds__(UB) void k_unb_pass(UnbArgs a)
         }
       __syncthreads();
     }
-  const double energy_limit = okey_inv(s_pref[0]);
+  if(t == 0)
+    {
+      a.lim[g] = need_lim ? okey_inv(s_pref[0]) : 0.0;
+      a.wbl[g] = need_wbl ? okey_inv(s_pref[1]) : 0.0;
+    }
+}
 
-  // omit unbound members, at most a quarter of them (:574-590)
-  int unb = 0, lng = 0;
-  for(int64_t i = f + t; i < e; i += UB)
+// 8. omit unbound members, at most a quarter of them (:574-590); chunk counts of the
+// removed members and of the non-gas members left
+__global__ __launch_bounds__(CB) void k_uc_remove(UnbArgs a)
+{
+  __shared__ double s_red[CB / 64][8];
+  int64_t f, e;
+  const int g = chunk_range(a, blockIdx.x, f, e);
+  if(a.gphase[g] == 2)
+    return;
+  const int len = a.glen[2 * g], P = a.glen[2 * g + 1];
+  const double lim = P > (int)(0.25 * len) ? a.lim[g] : 0.0;
+  double cnt[2] = {0, 0};
+  for(int64_t i = f + threadIdx.x; i < e; i += CB)
     if(a.alive[i])
       {
         const double en = a.E[i];
-        if(en > 0 && en > energy_limit)
+        if(en > 0 && en > lim)
           {
             a.alive[i] = 0;
-            unb++;
+            cnt[0] += 1.0;
           }
         else if(!a.type || a.type[i] != 0)
-          lng++;
+          cnt[1] += 1.0;
       }
-  double c2[2] = {(double)unb, (double)lng};
-  block_sum<2>(c2, s_red);
-  if(t == 0)
+  block_sum<CB, 2>(cnt, s_red);
+  if(threadIdx.x == 0)
     {
-      a.res[3 * g]     = (int)c2[0];
-      a.res[3 * g + 1] = len - (int)c2[0];
-      a.res[3 * g + 2] = (int)c2[1];
-      a.wbl[g]         = okey_inv(s_pref[1]);
+      a.ccnt[2 * blockIdx.x]     = (int)cnt[0];
+      a.ccnt[2 * blockIdx.x + 1] = (int)cnt[1];
     }
+}
+
+// 9. per group (one wavefront): {unbound, len left, len_non_gas}
+__global__ void k_ug_result(UnbArgs a, int ng)
+{
+  const int g = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if(g >= ng || a.gphase[g] == 2)
+    return;
+  int u = 0, l = 0;
+  for(int c = a.cbeg[g] + lane; c < a.cbeg[g + 1]; c += 64)
+    {
+      u += a.ccnt[2 * c];
+      l += a.ccnt[2 * c + 1];
+    }
+  for(int o = 32; o > 0; o >>= 1)
+    {
+      u += __shfl_xor(u, o, 64);
+      l += __shfl_xor(l, o, 64);
+    }
+  if(lane == 0)
+    {
+      a.res[3 * g]     = u;
+      a.res[3 * g + 1] = a.glen[2 * g] - u;
+      a.res[3 * g + 2] = l;
+    }
+}
+
+// one pass of subfind_unbind (subfind_serial.c:455-590) for every active group
+static agrav_status unbind_pass(const UnbArgs &a, int ng, int nchunks, int nactive, const int *glist, hipStream_t s)
+{
+  const unsigned gb = (unsigned)((ng + 3) / 4);
+  k_uc_min<<<nchunks, CB, 0, s>>>(a);
+  k_ug_min<<<gb, 256, 0, s>>>(a, ng);
+  k_uc_sums<<<nchunks, CB, 0, s>>>(a);
+  k_ug_sums<<<gb, 256, 0, s>>>(a, ng);
+  k_uc_energy<<<nchunks, CB, 0, s>>>(a);
+  k_ug_count<<<gb, 256, 0, s>>>(a, ng);
+  k_ug_select<<<nactive, UB, 0, s>>>(a, glist);
+  k_uc_remove<<<nchunks, CB, 0, s>>>(a);
+  k_ug_result<<<gb, 256, 0, s>>>(a, ng);
+  AG_HIP(hipGetLastError());
+  return AGRAV_OK;
 }
 
 // the potential solve of the members of every phase-0 group, through the batch path
@@ -479,7 +641,23 @@ agrav_status agrav_subfind_unbind(agrav_ctx *c, int ngroups, const int64_t *offs
      (st = u.glist.ensure(ngroups)) || (st = u.gphase.ensure(ngroups)) || (st = u.res.ensure(3 * ngroups)) ||
      (st = u.cnt.ensure(4)) || (st = u.off.ensure(ngroups + 1)) || (st = u.soff.ensure(ngroups + 1)) ||
      (st = u.scan_tmp.ensure(scan_temp_bytes(nn) + 4096)) || (st = c->d_shd.ensure(ngroups)) ||
-     (st = c->d_sdom.ensure(ngroups)))
+     (st = c->d_sdom.ensure(ngroups)) || (st = u.lim.ensure(ngroups)) || (st = u.sv.ensure(6 * ngroups)) ||
+     (st = u.glen.ensure(2 * ngroups)) || (st = u.cbeg.ensure(ngroups + 1)))
+    return st;
+  // chunks of at most CH members, each inside one group
+  std::vector<int> cbeg(ngroups + 1);
+  std::vector<int2> &chunks = u.h_chunk;
+  chunks.clear();
+  for(int g = 0; g < ngroups; g++)
+    {
+      cbeg[g] = (int)chunks.size();
+      for(int64_t q = offsets[g]; q < offsets[g + 1]; q += CH)
+        chunks.push_back(make_int2(g, (int)q));
+    }
+  cbeg[ngroups]     = (int)chunks.size();
+  const int nchunks = (int)chunks.size();
+  if((st = u.chunk.ensure(nchunks)) || (st = u.cpart.ensure(8 * (size_t)nchunks)) || (st = u.cmin.ensure(nchunks)) ||
+     (st = u.cimin.ensure(nchunks)) || (st = u.ccnt.ensure(2 * (size_t)nchunks)))
     return st;
   if(utherm && (st = u.uth.ensure(nn)))
     return st;
@@ -495,6 +673,9 @@ agrav_status agrav_subfind_unbind(agrav_ctx *c, int ngroups, const int64_t *offs
       epsr[g] = h[g] / 2.8;
     }
   AG_HIP(hipMemcpyAsync(u.off.p, offsets, (ngroups + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
+  AG_HIP(hipMemcpyAsync(u.cbeg.p, cbeg.data(), (ngroups + 1) * sizeof(int), hipMemcpyHostToDevice, s));
+  if(nchunks > 0)
+    AG_HIP(hipMemcpyAsync(u.chunk.p, chunks.data(), nchunks * sizeof(int2), hipMemcpyHostToDevice, s));
   AG_HIP(hipMemcpyAsync(c->d_shd.p, h.data(), ngroups * sizeof(double), hipMemcpyHostToDevice, s));
   AG_HIP(hipMemcpyAsync(u.epsr.p, epsr.data(), ngroups * sizeof(double), hipMemcpyHostToDevice, s));
   if(n > 0)
@@ -568,11 +749,14 @@ agrav_status agrav_subfind_unbind(agrav_ctx *c, int ngroups, const int64_t *offs
           solves++;
         }
       (void)hipEventRecord(c->ev[6], s);
-      UnbArgs a{u.off.p,  u.glist.p, u.gphase.p, u.pos.p, u.vel.p,   u.mass.p, utherm ? u.uth.p : nullptr, u.epsr.p,
-                type ? u.type.p : nullptr, u.pot.p, u.E.p, u.wbl.p, u.ppos.p, u.alive.p, u.res.p, g_a,
-                up->comoving ? 1.0 / up->time_a : 1.0, cf_atime, up->comoving ? up->hubble_a : 0.0};
-      k_unb_pass<<<(unsigned)glist.size(), UB, 0, s>>>(a);
-      if(hipGetLastError() != hipSuccess || hipEventRecord(c->ev[4], s) != hipSuccess ||
+      UnbArgs a{u.off.p,   u.gphase.p, u.chunk.p, u.cbeg.p, u.pos.p,  u.vel.p,  u.mass.p,
+                utherm ? u.uth.p : nullptr, u.epsr.p, type ? u.type.p : nullptr, u.pot.p, u.E.p, u.wbl.p,
+                u.lim.p,   u.ppos.p,   u.sv.p,    u.alive.p, u.cpart.p, u.cmin.p, u.cimin.p, u.ccnt.p,
+                u.glen.p,  u.res.p,    g_a,       up->comoving ? 1.0 / up->time_a : 1.0, cf_atime,
+                up->comoving ? up->hubble_a : 0.0};
+      if((st = unbind_pass(a, ngroups, nchunks, (int)glist.size(), u.glist.p, s)) != AGRAV_OK)
+        break;
+      if(hipEventRecord(c->ev[4], s) != hipSuccess ||
          hipMemcpyAsync(res.data(), u.res.p, 3 * ngroups * sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
          hipStreamSynchronize(s) != hipSuccess)
         {
