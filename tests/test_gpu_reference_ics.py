@@ -1,0 +1,88 @@
+"""GPU parity on the reference's own example initial conditions (Gadget format 1, read by arepo_amd.gadget).
+
+* examples/cosmo_box_gravity_only_3d: 32,768 particles at z = 127 in a 50 Mpc/h periodic box, with the
+  example's gravity settings (Config.sh: PMGRID = 256, RCUT = 5.0; param.txt: ErrTolTheta 0.7,
+  ErrTolForceAcc 0.0025, relative criterion, softening 40 comoving kpc/h, code units kpc, 1e10 Msun, km/s).
+  One force evaluation from scratch (the Barnes-Hut seeding pass, then the relative pass).
+* examples/isolated_galaxy_collisionless_3d: every 12th particle of the halo + disk + bulge ICs, isolated
+  (GRAVITY_NOT_PERIODIC), BoxSize 100000 as in param.txt, softening 1.0, tree only.
+
+The FP64 per-target walk must equal the oracle over all particles; the group walks' errors against the
+FP64 direct sum (FORCETEST) must be no worse than the oracle's own on the same targets.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from arepo_amd import gadget as G
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GC = G.gravity_constant(3.085678e21, 1.989e43, 1e5)
+
+
+def _cosmo():
+    s = G.read_gadget1(os.path.join(HERE, "golden", "L50n32_ics"))
+    eps = G.softening_comoving(40.0, 20.0, s.header.time, True)
+    kw = dict(periodic=True, box_size=s.header.box_size, pmgrid=256, rcut=5.0, asmth=1.25, G=GC, soft=(eps,),
+              err_tol_theta=0.7, err_tol_force_acc=0.0025)
+    return s, kw
+
+
+def _galaxy():
+    s = G.read_gadget1(os.path.join(HERE, "golden", "galaxy_ics_every12"))
+    kw = dict(periodic=False, box_size=100000.0, pmgrid=0, G=GC, soft=(1.0,), err_tol_theta=0.7,
+              err_tol_force_acc=0.0025)
+    return s, kw
+
+
+@pytest.fixture(scope="module", params=["cosmo_box", "isolated_galaxy"])
+def case(request, oracle):
+    s, kw = _cosmo() if request.param == "cosmo_box" else _galaxy()
+    o = oracle.compute_grav_accelerations(oracle.make_params(**kw, opening=1), s.pos, s.mass, first_step=True)
+    sel = np.sort(np.random.Generator(np.random.Philox(41)).choice(len(s.pos), 2048, replace=False)).astype(np.int32)
+    return request.param, s, kw, o, sel
+
+
+def test_fp64_plain_equals_oracle(agrav, case):
+    name, s, kw, o, sel = case
+    with agrav.Gravity(agrav.make_params(**kw, opening=1, precision=agrav.FP64, walk=agrav.WALK_PLAIN)) as g:
+        g.set_particles(s.pos, s.mass)
+        g.compute_grav_accelerations(first_step=True)
+        r = g.get_accelerations()
+    e = rel_err(r["acc"], o["acc"])
+    ep = np.abs(r["pot"] - o["pot"]) / np.abs(o["pot"])
+    print("\n%s FP64 plain vs oracle: max %.3g, fraction above 1e-8: %.4f" % (name, e.max(), np.mean(e > 1e-8)))
+    if name == "cosmo_box":
+        assert np.percentile(e, 99.9) < 1e-8, (name, float(e.max()))
+        assert np.percentile(ep, 99.9) < 1e-8, (name, float(ep.max()))
+        assert np.mean(r["nint"] == o["nint"]) > 0.999, name
+    else:
+        # The galaxy's centre holds pairs closer than the level-21 cell (0.088 kpc of the 185,000 kpc domain
+        # cube): the oracle's insertion tree splits them down to level 23 (36 nodes below level 21), the
+        # GPU's 63-bit keys keep them as level-21 buckets.  Targets whose walk meets such a cell differ at
+        # the 1e-6 level, far below the method's error; every other target matches to round-off.
+        assert np.percentile(e, 99) < 1e-8, (name, float(np.percentile(e, 99)))
+        assert e.max() < 1e-5 and ep.max() < 1e-5, (name, float(e.max()), float(ep.max()))
+        assert np.mean(r["nint"] == o["nint"]) > 0.99, name
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_group_walk_error_vs_direct_sum(agrav, case, prec):
+    name, s, kw, o, sel = case
+    p = agrav.FP32 if prec == "fp32" else agrav.FP64
+    with agrav.Gravity(agrav.make_params(**kw, opening=1, precision=p, walk=agrav.WALK_GROUP)) as g:
+        g.set_particles(s.pos, s.mass)
+        g.compute_grav_accelerations(first_step=True)
+        r = g.get_accelerations()
+        ft = g.gravity_forcetest(sel)  # G = 1 units
+    exact = ft["acc"] * kw["G"]
+    e_gpu = rel_err(r["acc"][sel], exact)
+    e_or = rel_err(o["acc"][sel], exact)
+    print("\n%s %s group: median %.3g p99 %.3g | oracle median %.3g p99 %.3g"
+          % (name, prec, np.median(e_gpu), np.percentile(e_gpu, 99), np.median(e_or), np.percentile(e_or, 99)))
+    assert np.median(e_gpu) <= 1.05 * np.median(e_or), name
+    assert np.percentile(e_gpu, 99) <= 1.05 * np.percentile(e_or, 99), name
