@@ -300,6 +300,9 @@ void agrav_destroy(agrav_ctx *c)
   c->d_oldacc.release();
   c->d_key.release();
   c->d_key2.release();
+  c->d_key_lo.release();
+  c->d_key_lo_s.release();
+  c->d_ties.release();
   c->d_idx.release();
   c->d_idx2.release();
   c->d_px.release();

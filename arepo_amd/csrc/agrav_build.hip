@@ -262,14 +262,20 @@ __device__ inline uint64_t int52(double x, const Domain &dom, int d)
   return (uint64_t)__double_as_longlong(v) & 0xFFFFFFFFFFFFFull;
 }
 
+// Keys of 126 bits in two 63-bit words {keys, keys_lo}: the set index in the top
+// sbits bits, then `levels` Peano-Hilbert digits (3 bits each, coarsest first),
+// then zeros.  The octree splits cells down to `levels` (AREPO's insertion tree
+// splits down to MAX_TREE_LEVEL, forcetree.c:861-877); only particles sharing a
+// cell below the first word need the second one.
 __global__ void k_keys(const double *__restrict__ pos, int64_t n, BuildGeo geo, int periodic, double box,
-                       uint64_t *__restrict__ keys, int *__restrict__ idx, Domain *dom_w)
+                       uint64_t *__restrict__ keys, uint64_t *__restrict__ keys_lo, int *__restrict__ idx, Domain *dom_w)
 {
   int64_t i = (int64_t)blockIdx.x * B + threadIdx.x;
   if(i >= n)
     return;
   const int set    = geo.pset_in ? geo.pset_in[i] : 0;
   const Domain dom = geo.dom[set];
+  const int L      = geo.levels;
   uint32_t c[3];
   bool bad = false;
   for(int d = 0; d < 3; d++)
@@ -279,16 +285,170 @@ __global__ void k_keys(const double *__restrict__ pos, int64_t n, BuildGeo geo, 
         x = wrap_box(x, box);
       if(!(x >= dom.corner[d] && x < dom.corner[d] + dom.len))
         bad = true;
-      c[d] = (uint32_t)(int52(x, dom, d) >> (52 - AG_KEY_LEVELS));
+      c[d] = (uint32_t)(int52(x, dom, d) >> (52 - L));
     }
   if(bad)
     {
       atomicExch(&dom_w->out_of_box, 1);
       c[0] = c[1] = c[2] = 0;
     }
-  const int hb = 3 * geo.levels;  // Hilbert bits kept; their top digits are the coarser cells
-  keys[i]      = ((uint64_t)set << hb) | (hilbert_key21(c[0], c[1], c[2]) >> (3 * AG_KEY_LEVELS - hb));
-  idx[i]       = (int)i;
+  hilbert_transpose(c, L);
+  unsigned __int128 k = (unsigned __int128)(unsigned)set;
+  for(int b = L - 1; b >= 0; b--)
+    k = (k << 3) | (((c[0] >> b) & 1u) << 2) | (((c[1] >> b) & 1u) << 1) | ((c[2] >> b) & 1u);
+  k <<= (126 - geo.sbits - 3 * L);
+  keys[i]    = (uint64_t)(k >> 63);
+  keys_lo[i] = (uint64_t)k & ((1ull << 63) - 1);
+  idx[i]     = (int)i;
+}
+
+// runs of equal first key words in the sorted order (particles sharing a cell at
+// the first word's depth): the start of each run, one atomic per wavefront
+__global__ void k_tie_runs(const uint64_t *__restrict__ keys, int n, int *runs, int *nruns)
+{
+  const int k     = blockIdx.x * B + threadIdx.x;
+  const bool head = k + 1 < n && keys[k] == keys[k + 1] && (k == 0 || keys[k - 1] != keys[k]);
+  const uint64_t m = __ballot(head);
+  if(!m)
+    return;
+  const int lane = threadIdx.x & 63;
+  int base       = 0;
+  if(lane == __ffsll((unsigned long long)m) - 1)
+    base = atomicAdd(nruns, __popcll(m));
+  base = __shfl(base, __ffsll((unsigned long long)m) - 1, 64);
+  if(head)
+    runs[base + __popcll(m & ((1ull << lane) - 1))] = k;
+}
+
+__device__ inline bool tie_less(uint64_t ka, int va, uint64_t kb, int vb);
+
+// Short runs (the usual case: a few particles sharing a deep cell), one thread
+// each: the run's second key words, then an insertion sort by (second word,
+// caller index) in place.  Runs longer than kShortRun go to the workgroup sort.
+static constexpr int kShortRun = 32;
+__global__ void k_tie_sort_short(const uint64_t *__restrict__ keys, const uint64_t *__restrict__ keys_lo, int n,
+                                 const int *__restrict__ runs, const int *__restrict__ nruns, int *idx, uint64_t *lo_s,
+                                 int *long_runs, int *nlong)
+{
+  const int nr = *nruns;
+  for(int r = blockIdx.x * B + threadIdx.x; r < nr; r += gridDim.x * B)
+    {
+      const int k0 = runs[r];
+      int e        = k0 + 1;
+      while(e < n && e - k0 <= kShortRun && keys[e] == keys[k0])
+        e++;
+      if(e - k0 > kShortRun)
+        {
+          long_runs[atomicAdd(nlong, 1)] = k0;
+          continue;
+        }
+      for(int t = k0; t < e; t++)
+        lo_s[t] = keys_lo[idx[t]];
+      for(int t = k0 + 1; t < e; t++)
+        {
+          const uint64_t kk = lo_s[t];
+          const int vv      = idx[t];
+          int j             = t - 1;
+          while(j >= k0 && tie_less(kk, vv, lo_s[j], idx[j]))
+            {
+              lo_s[j + 1] = lo_s[j];
+              idx[j + 1]  = idx[j];
+              j--;
+            }
+          lo_s[j + 1] = kk;
+          idx[j + 1]  = vv;
+        }
+    }
+}
+
+__device__ inline bool tie_less(uint64_t ka, int va, uint64_t kb, int vb) { return ka < kb || (ka == kb && va < vb); }
+
+// Sort every run by its second key words, ties by caller index: the order a stable
+// sort of the full 126-bit keys gives.  One workgroup per run; a bitonic network
+// whose every compare-exchange puts the smaller pair first, so that the padding
+// to a power of two (virtual +inf entries) never moves; in LDS up to 1024
+// entries, in place in global memory beyond.  Also the sorted second words.
+__global__ __launch_bounds__(256) void k_tie_sort(const uint64_t *__restrict__ keys, const uint64_t *__restrict__ keys_lo, int n,
+                                                  const int *__restrict__ runs, const int *__restrict__ nruns, int *idx,
+                                                  uint64_t *lo_s)
+{
+  __shared__ uint64_t s_k[1024];
+  __shared__ int s_v[1024];
+  __shared__ int s_end;
+  const int nr = *nruns;
+  for(int r = blockIdx.x; r < nr; r += gridDim.x)
+    {
+      const int k0 = runs[r];
+      if(threadIdx.x == 0)
+        {
+          int e = k0 + 1;
+          while(e < n && keys[e] == keys[k0])
+            e++;
+          s_end = e;
+        }
+      __syncthreads();
+      const int len = s_end - k0;
+      int P         = 1;
+      while(P < len)
+        P <<= 1;
+      const bool lds = P <= 1024;
+      uint64_t *gk   = lo_s + k0;
+      int *gv        = idx + k0;
+      for(int t = threadIdx.x; t < P; t += 256)
+        {
+          const uint64_t kk = t < len ? keys_lo[idx[k0 + t]] : ~0ull;
+          if(lds)
+            {
+              s_k[t] = kk;
+              s_v[t] = t < len ? idx[k0 + t] : 0x7fffffff;
+            }
+          else if(t < len)
+            gk[t] = kk;
+        }
+      __syncthreads();
+      uint64_t *K = lds ? s_k : gk;
+      int *V      = lds ? s_v : gv;
+      for(int size = 2; size <= P; size <<= 1)
+        for(int j = size >> 1; j > 0; j >>= 1)
+          {
+            for(int t = threadIdx.x; t < (P >> 1); t += 256)
+              {
+                // first step of each merge: mirror pairs inside blocks of `size`; then pairs at distance j
+                int a, b;
+                if(j == (size >> 1))
+                  {
+                    const int half = size >> 1, blk = t / half, o = t % half;
+                    a = blk * size + o;
+                    b = blk * size + size - 1 - o;
+                  }
+                else
+                  {
+                    a = 2 * j * (t / j) + (t % j);
+                    b = a + j;
+                  }
+                if(b < (lds ? P : len))
+                  {
+                    const uint64_t ka = K[a], kb = K[b];
+                    const int va = V[a], vb = V[b];
+                    if(tie_less(kb, vb, ka, va))
+                      {
+                        K[a] = kb;
+                        K[b] = ka;
+                        V[a] = vb;
+                        V[b] = va;
+                      }
+                  }
+              }
+            __syncthreads();
+          }
+      if(lds)
+        for(int t = threadIdx.x; t < len; t += 256)
+          {
+            gk[t] = s_k[t];
+            gv[t] = s_v[t];
+          }
+      __syncthreads();
+    }
 }
 
 __global__ void k_gather(const int *__restrict__ idx, const double *__restrict__ pos, const double *__restrict__ mass,
@@ -327,16 +487,26 @@ __global__ void k_gather(const int *__restrict__ idx, const double *__restrict__
 // 3./4. octree cubes from the sorted keys
 // ---------------------------------------------------------------------------
 
-// number of equal leading bits of keys i and j (keys use bits 62..0); identical
-// keys count as longer than any key prefix; -1 beyond the ends
-__device__ inline int kdelta(const uint64_t *__restrict__ k, int n, int i, int j)
+// number of equal leading bits of the 126-bit keys of sorted particles i and j
+// (each word uses bits 62..0; second words are read only where the first ones
+// are equal, i.e. inside a tie run); identical keys count as longer than any
+// key prefix; -1 beyond the ends
+__device__ inline int key_lcp2(const uint64_t *__restrict__ k, const uint64_t *__restrict__ klo, int i, int j)
+{
+  uint64_t a = k[i], b = k[j];
+  if(a != b)
+    return __clzll((long long)(a ^ b)) - 1;
+  a = klo[i];
+  b = klo[j];
+  return a == b ? 126 : 63 + __clzll((long long)(a ^ b)) - 1;
+}
+
+__device__ inline int kdelta(const uint64_t *__restrict__ k, const uint64_t *__restrict__ klo, int n, int i, int j)
 {
   if(j < 0 || j >= n)
     return -1;
-  uint64_t a = k[i], b = k[j];
-  if(a == b)
-    return 64 + __clz((unsigned)(i ^ j));
-  return __clzll((long long)(a ^ b)) - 1;  // keys use bits 62..0
+  const int l = key_lcp2(k, klo, i, j);
+  return l < 126 ? l : 128 + __clz((unsigned)(i ^ j));
 }
 
 
@@ -346,18 +516,18 @@ __device__ inline int kdelta(const uint64_t *__restrict__ k, int n, int i, int j
 // holding two or more) when f+1 shares it.  Also the cubes per particle, and per
 // block the cubes per level (ballots, then one row of partial counts per block:
 // global atomics on the same few words would serialise).
-__global__ __launch_bounds__(B) void k_lvlmask(const uint64_t *__restrict__ keys, int n, int sbits, int levels, unsigned *lvlmask,
-                                               int *pcnt, int *lvlpart)
+__global__ __launch_bounds__(B) void k_lvlmask(const uint64_t *__restrict__ keys, const uint64_t *__restrict__ klo, int n, int sbits,
+                                               int levels, unsigned *lvlmask, int *pcnt, int *lvlpart)
 {
-  __shared__ int s_cnt[AG_KEY_LEVELS + 1];
-  if(threadIdx.x <= AG_KEY_LEVELS)
+  __shared__ int s_cnt[AG_MAX_LEVELS + 1];
+  if(threadIdx.x <= AG_MAX_LEVELS)
     s_cnt[threadIdx.x] = 0;
   __syncthreads();
   const int f = blockIdx.x * B + threadIdx.x;
   unsigned m  = 0;
   if(f < n)
     {
-      const int a = kdelta(keys, n, f, f - 1), b = kdelta(keys, n, f, f + 1);  // -1 beyond the ends
+      const int a = kdelta(keys, klo, n, f, f - 1), b = kdelta(keys, klo, n, f, f + 1);  // -1 beyond the ends
       const int lo = a < sbits ? 0 : (a - sbits) / 3 + 1;
       const int hi = b < sbits ? -1 : min((b - sbits) / 3, levels);
       for(int L = lo; L <= hi; L++)
@@ -372,8 +542,8 @@ __global__ __launch_bounds__(B) void k_lvlmask(const uint64_t *__restrict__ keys
         atomicAdd(&s_cnt[L], c);
     }
   __syncthreads();
-  if(threadIdx.x <= AG_KEY_LEVELS)
-    lvlpart[(int64_t)blockIdx.x * (AG_KEY_LEVELS + 1) + threadIdx.x] = s_cnt[threadIdx.x];
+  if(threadIdx.x <= AG_MAX_LEVELS)
+    lvlpart[(int64_t)blockIdx.x * (AG_MAX_LEVELS + 1) + threadIdx.x] = s_cnt[threadIdx.x];
 }
 
 // cubes per level: block L sums column L of the partial counts
@@ -387,7 +557,7 @@ __global__ __launch_bounds__(B) void k_level_totals(const int *__restrict__ lvlp
       int v[8];
 #pragma unroll
       for(int u = 0; u < 8; u++)
-        v[u] = r0 + u < nrows ? lvlpart[(int64_t)(r0 + u) * (AG_KEY_LEVELS + 1) + L] : 0;
+        v[u] = r0 + u < nrows ? lvlpart[(int64_t)(r0 + u) * (AG_MAX_LEVELS + 1) + L] : 0;
 #pragma unroll
       for(int u = 0; u < 8; u++)
         c += v[u];
@@ -421,7 +591,6 @@ __device__ inline int pnext_of(int p, int n, const unsigned *__restrict__ lvlmas
   return lvlmask[q] ? n + pbase[q] : q;
 }
 
-__device__ inline int key_lcp(uint64_t a, uint64_t b) { return a == b ? 64 : __clzll((long long)(a ^ b)) - 1; }
 
 // Place the cubes starting at each particle in pre-order (first particle, then
 // level) and derive their geometry and first link.
@@ -470,9 +639,9 @@ __global__ void k_place(int n, const unsigned *__restrict__ lvlmask, const int *
 // The last particle of every cube: the end of the run of sorted keys that share
 // the cube's prefix, by a galloping search from its first particle; then the
 // particle range and the sibling link (the entry after the cube).
-__global__ void k_cube_last(int m, int n, const uint64_t *__restrict__ keys, const int *__restrict__ nfirst,
-                            const int *__restrict__ nlevel, const unsigned *__restrict__ lvlmask, const int *__restrict__ pbase,
-                            int sbits, int *nlast, int2 *nrange, int *sibling)
+__global__ void k_cube_last(int m, int n, const uint64_t *__restrict__ keys, const uint64_t *__restrict__ klo,
+                            const int *__restrict__ nfirst, const int *__restrict__ nlevel, const unsigned *__restrict__ lvlmask,
+                            const int *__restrict__ pbase, int sbits, int *nlast, int2 *nrange, int *sibling)
 {
   const int j = blockIdx.x * B + threadIdx.x;
   if(j >= m)
@@ -483,9 +652,8 @@ __global__ void k_cube_last(int m, int n, const uint64_t *__restrict__ keys, con
     end = n - 1;  // the root of a single set holds every particle
   else
     {
-      const uint64_t kf = keys[f];
-      int step          = 1;
-      while(end + step < n && key_lcp(kf, keys[end + step]) >= q)
+      int step = 1;
+      while(end + step < n && key_lcp2(keys, klo, f, end + step) >= q)
         {
           end += step;
           step <<= 1;
@@ -494,7 +662,7 @@ __global__ void k_cube_last(int m, int n, const uint64_t *__restrict__ keys, con
       while(hi - end > 1)
         {
           const int mid = (end + hi) >> 1;
-          if(key_lcp(kf, keys[mid]) >= q)
+          if(key_lcp2(keys, klo, f, mid) >= q)
             end = mid;
           else
             hi = mid;
@@ -931,7 +1099,7 @@ agrav_status build_tree(agrav_ctx *c)
 
   // 1. domain (one cube, or one per set of a batch)
   const int nsets = c->batch_nsets;
-  BuildGeo geo{c->d_dom.p, nullptr, nullptr, 0, AG_KEY_LEVELS};
+  BuildGeo geo{c->d_dom.p, nullptr, nullptr, 0, AG_MAX_LEVELS};
   if(!c->par.periodic && nsets == 0)
     {
       unsigned g = nblk(n);
@@ -948,8 +1116,8 @@ agrav_status build_tree(agrav_ctx *c)
       int bits = 0;
       while((1ll << bits) < nsets)
         bits++;
-      geo.levels  = (63 - bits) / 3;
-      geo.sbits   = 63 - 3 * geo.levels;
+      geo.levels  = std::min(AG_MAX_LEVELS, (126 - bits) / 3);
+      geo.sbits   = bits;
       geo.dom     = c->d_sdom.p;
       geo.pset_in = c->d_pset_in.p;
       geo.pset    = c->d_pset.p;
@@ -963,13 +1131,26 @@ agrav_status build_tree(agrav_ctx *c)
   c->geo_levels = geo.levels;
 
   // 2. keys and sort
-  if((st = c->d_key.ensure(n)) || (st = c->d_key2.ensure(n)) || (st = c->d_idx.ensure(n)) || (st = c->d_idx2.ensure(n)))
+  if((st = c->d_key.ensure(n)) || (st = c->d_key2.ensure(n)) || (st = c->d_idx.ensure(n)) || (st = c->d_idx2.ensure(n)) ||
+     (st = c->d_key_lo.ensure(n)) || (st = c->d_key_lo_s.ensure(n)) || (st = c->d_ties.ensure(2 * (n / 2 + 1))))
     return st;
-  k_keys<<<nblk(n), B, 0, s>>>(c->d_pos_in.p, n, geo, c->par.periodic, c->par.box_size, c->d_key.p, c->d_idx.p, c->d_dom.p);
+  k_keys<<<nblk(n), B, 0, s>>>(c->d_pos_in.p, n, geo, c->par.periodic, c->par.box_size, c->d_key.p, c->d_key_lo.p, c->d_idx.p,
+                               c->d_dom.p);
   AG_HIP(hipGetLastError());
   if((st = radix_sort_u64(c->d_key.p, c->d_idx.p, c->d_key2.p, c->d_idx2.p, n, 63, c->d_temp.p, d_perr, s)) != AGRAV_OK)
     return st;
-  // sorted keys/perm are now in d_key2/d_idx2
+  // sorted first key words and perm are now in d_key2/d_idx2; runs of equal first
+  // words are put in the order of their second words
+  int *d_nties = c->d_counters.p + 60;  // cleared by the upload at the start
+  if(n > 1)
+    {
+      int *d_nlong = c->d_counters.p + 61, *d_long = c->d_ties.p + (n / 2 + 1);
+      k_tie_runs<<<nblk(n), B, 0, s>>>(c->d_key2.p, (int)n, c->d_ties.p, d_nties);
+      k_tie_sort_short<<<512, B, 0, s>>>(c->d_key2.p, c->d_key_lo.p, (int)n, c->d_ties.p, d_nties, c->d_idx2.p,
+                                         c->d_key_lo_s.p, d_long, d_nlong);
+      k_tie_sort<<<256, 256, 0, s>>>(c->d_key2.p, c->d_key_lo.p, (int)n, d_long, d_nlong, c->d_idx2.p, c->d_key_lo_s.p);
+      AG_HIP(hipGetLastError());
+    }
 
   if((st = c->d_px.ensure(n)) || (st = c->d_py.ensure(n)) || (st = c->d_pz.ensure(n)) || (st = c->d_pm.ensure(n)) ||
      (st = c->d_prec.ensure(n)) || (st = c->d_pnext.ensure(n)) || (st = c->d_oldacc_s.ensure(n)))
@@ -988,12 +1169,13 @@ agrav_status build_tree(agrav_ctx *c)
     return st;
   if(N > 1)
     {
-      int *lvlpart = c->d_bcnt.p;  // nblk(N) rows of 22 partial counts
-      if((st = c->d_bcnt.ensure((size_t)nblk(N) * (AG_KEY_LEVELS + 1))) != AGRAV_OK)
+      int *lvlpart = c->d_bcnt.p;  // nblk(N) rows of AG_MAX_LEVELS + 1 partial counts
+      if((st = c->d_bcnt.ensure((size_t)nblk(N) * (AG_MAX_LEVELS + 1))) != AGRAV_OK)
         return st;
       lvlpart = c->d_bcnt.p;
-      k_lvlmask<<<nblk(N), B, 0, s>>>(c->d_key2.p, N, geo.sbits, geo.levels, c->d_lvlmask.p, c->d_pcnt.p, lvlpart);
-      k_level_totals<<<AG_KEY_LEVELS + 1, B, 0, s>>>(lvlpart, (int)nblk(N), d_lvlcnt);
+      k_lvlmask<<<nblk(N), B, 0, s>>>(c->d_key2.p, c->d_key_lo_s.p, N, geo.sbits, geo.levels, c->d_lvlmask.p, c->d_pcnt.p,
+                                      lvlpart);
+      k_level_totals<<<AG_MAX_LEVELS + 1, B, 0, s>>>(lvlpart, (int)nblk(N), d_lvlcnt);
     }
   else
     k_single_root<<<1, 1, 0, s>>>(c->d_lvlmask.p, c->d_pcnt.p, d_lvlcnt);
@@ -1001,10 +1183,11 @@ agrav_status build_tree(agrav_ctx *c)
   if((st = exclusive_scan_i32(c->d_pcnt.p, c->d_pbase.p, N, d_total, c->d_temp.p, d_perr, s)) != AGRAV_OK)
     return st;
   // the one host synchronisation of the octree: node count and cubes per level
-  int hc[24];
+  static_assert(14 + AG_MAX_LEVELS + 1 <= 48, "cubes per level fit below the walk counters");
+  int hc[2 + AG_MAX_LEVELS + 1];
   AG_HIP(hipMemcpyAsync(hc, d_total, sizeof(hc), hipMemcpyDeviceToHost, s));
   AG_HIP(hipStreamSynchronize(s));
-  for(int L = 0; L <= AG_KEY_LEVELS; L++)
+  for(int L = 0; L <= AG_MAX_LEVELS; L++)
     c->lvlcnt_h[L] = hc[2 + L];
   if(hc[1])
     return fail(c, AGRAV_ERR_HIP, "sort/scan look-back did not complete");
@@ -1023,8 +1206,8 @@ agrav_status build_tree(agrav_ctx *c)
     return st;
   k_place<<<nblk(N), B, 0, s>>>(N, c->d_lvlmask.p, c->d_pbase.p, c->d_px.p, c->d_py.p, c->d_pz.p, geo, c->d_nfirst.p,
                                 c->d_nlevel.p, c->d_cx.p, c->d_cy.p, c->d_cz.p, c->d_len.p, c->d_nextnode.p);
-  k_cube_last<<<nblk(M), B, 0, s>>>(M, N, c->d_key2.p, c->d_nfirst.p, c->d_nlevel.p, c->d_lvlmask.p, c->d_pbase.p, geo.sbits,
-                                    c->d_nlast.p, c->d_nrange.p, c->d_sibling.p);
+  k_cube_last<<<nblk(M), B, 0, s>>>(M, N, c->d_key2.p, c->d_key_lo_s.p, c->d_nfirst.p, c->d_nlevel.p, c->d_lvlmask.p,
+                                    c->d_pbase.p, geo.sbits, c->d_nlast.p, c->d_nrange.p, c->d_sibling.p);
   k_pnext<<<nblk(N), B, 0, s>>>(N, c->d_lvlmask.p, c->d_pbase.p, c->d_pnext.p);
   if(nsets > 0)
     {
@@ -1048,7 +1231,7 @@ agrav_status build_tree(agrav_ctx *c)
   k_fill_children<<<nblk(M), B, 0, s>>>(M, N, c->d_nextnode.p, c->d_sibling.p, c->d_pnext.p, c->d_cbeg.p, c->d_ccnt.p,
                                          c->d_nrange.p, c->d_prec.p, c->d_child.p, c->d_cnn.p, c->d_meta.p, c->d_cprec.p);
   int mtot = 0;
-  for(int L = 0; L <= AG_KEY_LEVELS; L++)
+  for(int L = 0; L <= AG_MAX_LEVELS; L++)
     mtot += c->lvlcnt_h[L];
   if(mtot != M)
     return fail(c, AGRAV_ERR_CAPACITY, "inconsistent octree level counts");

@@ -111,7 +111,7 @@ struct agrav_ctx
 
   // batch of independent isolated sets (batch_nsets = 0: one particle set)
   int batch_nsets = 0;
-  int geo_sbits = 0, geo_levels = AG_KEY_LEVELS;
+  int geo_sbits = 0, geo_levels = AG_MAX_LEVELS;
   std::vector<int64_t> h_soff;       // set offsets (nsets + 1), caller and sorted order alike
   agrav::DBuf<int64_t> d_soff;
   agrav::DBuf<int> d_pset_in, d_pset; // set of each particle in caller / sorted order
@@ -130,6 +130,9 @@ struct agrav_ctx
 
   // keys and sorted particles
   agrav::DBuf<uint64_t> d_key, d_key2;
+  agrav::DBuf<uint64_t> d_key_lo, d_key_lo_s; // second key words: caller order, sorted order (tie runs only)
+  agrav::DBuf<int> d_ties;                    // first sorted position of every run of equal first words, then
+                                              // of the runs too long for one thread
   agrav::DBuf<int> d_idx, d_idx2;
   agrav::DBuf<double> d_px, d_py, d_pz, d_pm;
   agrav::DBuf<uint8_t> d_ps;
@@ -161,7 +164,7 @@ struct agrav_ctx
   agrav::DBuf<double4> d_mom_hi;
   agrav::DBuf<float4> d_mom_lo;
   agrav::DBuf<int> d_tcnt;
-  int lvlcnt_h[AG_KEY_LEVELS + 2];
+  int lvlcnt_h[AG_MAX_LEVELS + 2];
 
   // walk outputs in sorted order (G=1 units) and per-target counts
   agrav::DBuf<double> d_at, d_pt;

@@ -19,7 +19,8 @@
 #include "agrav.h"
 
 #define AG_NTAB 127
-#define AG_KEY_LEVELS 21
+#define AG_KEY_LEVELS 21  // Hilbert levels of one set in the first 63-bit key word
+#define AG_MAX_LEVELS 30  // deepest octree level, AREPO's MAX_TREE_LEVEL (forcetree.h:106)
 #define AG_MAXSOFT AGRAV_MAX_SOFT_TYPES
 
 namespace agrav
@@ -171,13 +172,13 @@ size_t sort_temp_bytes(int64_t n);
 agrav_status radix_sort_u64(uint64_t *keys_in, int *vals_in, uint64_t *keys_out, int *vals_out, int64_t n, int key_bits,
                             void *d_temp, int *d_err, hipStream_t s);
 
-__host__ __device__ inline uint64_t hilbert_key21(uint32_t x, uint32_t y, uint32_t z)
+// 3-D Hilbert index by Skilling's transpose method ("Programming the Hilbert
+// curve", AIP Conf. Proc. 707, 381 (2004)) on b-bit coordinates: undo excess
+// work, then Gray-encode.  Interleaving the transposed coordinates MSB first
+// gives the key; its top 3L bits are the key of the coordinates' top L bits.
+__host__ __device__ inline void hilbert_transpose(uint32_t X[3], int b)
 {
-  // 3-D Hilbert index by Skilling's transpose method ("Programming the
-  // Hilbert curve", AIP Conf. Proc. 707, 381 (2004)): undo excess work,
-  // Gray-encode, then interleave the transposed coordinates MSB first.
-  uint32_t X[3] = {x, y, z};
-  const uint32_t M = 1u << (AG_KEY_LEVELS - 1);
+  const uint32_t M = 1u << (b - 1);
   for(uint32_t Q = M; Q > 1; Q >>= 1)
     {
       uint32_t P = Q - 1;
@@ -202,6 +203,12 @@ __host__ __device__ inline uint64_t hilbert_key21(uint32_t x, uint32_t y, uint32
   X[0] ^= t;
   X[1] ^= t;
   X[2] ^= t;
+}
+
+__host__ __device__ inline uint64_t hilbert_key21(uint32_t x, uint32_t y, uint32_t z)
+{
+  uint32_t X[3] = {x, y, z};
+  hilbert_transpose(X, AG_KEY_LEVELS);
   uint64_t key = 0;
   for(int b = AG_KEY_LEVELS - 1; b >= 0; b--)
     {

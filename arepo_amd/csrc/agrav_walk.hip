@@ -690,7 +690,7 @@ __device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, co
   // single entry (a depth-first step) adds at most 7 per tree level, which the
   // reserve holds down to the deepest level.  Deep walks (large isolated sets) then
   // narrow instead of overflowing.
-  constexpr int kReserve = 8 * (AG_KEY_LEVELS + 2);
+  constexpr int kReserve = 8 * (AG_MAX_LEVELS + 2);
   // split group: all waves expand the same entries (only wave 0 keeps their
   // interactions) until the stack holds kSplitMin entries; then each wave keeps every
   // nsplit-th of them and walks on alone

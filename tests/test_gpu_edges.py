@@ -3,10 +3,13 @@
 * empty particle sets (a rank with no particles is legal in the reference);
 * coincident particles: AREPO splits cells down to level 30 and then puts
   coincident points into free slots (forcetree.c:861-877); this library keeps
-  a cell still shared at level 21 as one multi-particle leaf.  Accelerations
+  a cell still shared at level 30 as one multi-particle leaf.  Accelerations
   and potentials equal the oracle's anyway (all members of such a cell sit at
   one point, so a deep node and its particles contribute the same pair terms);
   only interaction counts may differ;
+* a dense clump inside one level-21 cell: its particles tie in the first key
+  word and are ordered by the second (runs longer than 1024 are sorted in
+  global memory); the octree below level 21 must be the oracle's;
 * several softening types (the node softening test, forcetree_walk.c:294-320);
 * positions one box outside a periodic box, wrapped once as forcetree.c:404-409;
 * device-resident input (agrav_set_particles_device) equal to host input.
@@ -220,3 +223,35 @@ def test_pm_without_tree_rejects_positions_outside_box(agrav, bad):
         g.long_range_force()
         pm = g.get_components(tree=False, pm=True)["acc_pm"]
         assert np.all(np.isfinite(pm))
+
+
+@pytest.mark.parametrize("nclump", [300, 1500])
+def test_clump_below_level_21_matches_oracle(agrav, oracle, nclump):
+    # uniform particles in an isolated domain, plus a clump of nclump particles filling the middle fifth
+    # of one level-21 cell of that domain (1500 > 1024: the tie run is sorted in global memory)
+    g = np.random.Generator(np.random.Philox(123))
+    nu = 4000
+    uni = g.random((nu, 3))
+    p = oracle.make_params(periodic=False, box_size=0.0, pmgrid=0, soft=(1e-9,), err_tol_theta=0.7,
+                           err_tol_force_acc=0.0025)
+    corner, center, length = oracle.domain_extent(p, uni)
+    cell = length / 2 ** 21
+    origin = corner + (np.array([2 ** 20, 2 ** 20 - 7, 2 ** 20 + 5]) * cell)
+    clump = origin + (0.4 + 0.2 * g.random((nclump, 3))) * cell
+    pos = np.concatenate([uni, clump])
+    mass = np.full(len(pos), 1.0 / len(pos))
+    kw = dict(periodic=False, box_size=0.0, pmgrid=0, soft=(1e-9,), err_tol_theta=0.7, err_tol_force_acc=0.0025)
+    o = oracle.compute_grav_accelerations(oracle.make_params(**kw, opening=1), pos, mass, first_step=True)
+    with agrav.Gravity(agrav.make_params(**kw, opening=1, precision=agrav.FP64, walk=agrav.WALK_PLAIN)) as gr:
+        gr.set_particles(pos, mass)
+        gr.compute_grav_accelerations(first_step=True)
+        r = gr.get_accelerations()
+        tree = gr.get_tree()
+    ot = oracle.Tree(oracle.make_params(**kw), pos, mass).export()
+    # the same cubes as the oracle's insertion tree, down to its deepest level (above 21 here)
+    assert ot["level"].max() > 24
+    assert len(tree["level"]) == len(ot["level"]) and tree["level"].max() == ot["level"].max()
+    assert np.array_equal(np.sort(tree["len"]), np.sort(ot["len"]))
+    e = rel_err(r["acc"], o["acc"])
+    assert np.percentile(e, 99.9) < 1e-8, float(e.max())
+    assert np.mean(r["nint"] == o["nint"]) > 0.999
