@@ -91,7 +91,7 @@ def lib():
 
 EXPORTED = [
     "agrav_status_string", "agrav_last_error", "agrav_default_params", "agrav_create", "agrav_destroy",
-    "agrav_set_softenings", "agrav_set_particles", "agrav_set_particles_device", "agrav_set_old_acc",
+    "agrav_set_softenings", "agrav_set_particles", "agrav_set_particles_device", "agrav_set_old_acc", "agrav_set_active",
     "agrav_construct_forcetree", "agrav_gravity_tree", "agrav_long_range_force", "agrav_gravity_force_finalize",
     "agrav_compute_grav_accelerations", "agrav_get_accelerations", "agrav_get_accelerations_device",
     "agrav_get_components", "agrav_gravity_forcetest", "agrav_gravity_direct", "agrav_get_timings", "agrav_get_tree",
@@ -165,6 +165,7 @@ def _declare(L):
     L.agrav_set_particles.argtypes = [vp, C.c_int64, vp, vp, vp]
     L.agrav_set_particles_device.argtypes = [vp, C.c_int64, vp, vp, vp]
     L.agrav_set_old_acc.argtypes = [vp, vp]
+    L.agrav_set_active.argtypes = [vp, vp, C.c_int64]
     L.agrav_construct_forcetree.argtypes = [vp, C.POINTER(C.c_int)]
     L.agrav_gravity_tree.argtypes = [vp, C.c_int]
     L.agrav_long_range_force.argtypes = [vp]
@@ -342,6 +343,15 @@ class Gravity:
                                                 _ptr(it), _ptr(pot), _ptr(en)))
         return dict(bound=bound[:n].astype(bool), len=ln, len_non_gas=lng, iterations=it, potential=pot[:n],
                     energy=en[:n])
+
+    def set_active(self, active=None):
+        """Targets of the next walk/finalize (gravity_tree's TargetList); None: every particle."""
+        if active is None:
+            self._check(self.L.agrav_set_active(self.h, None, -1))
+            return
+        a = np.ascontiguousarray(active, dtype=np.int32)
+        buf = a if len(a) else np.zeros(1, np.int32)  # an empty list is a list, not "every particle"
+        self._check(self.L.agrav_set_active(self.h, _ptr(buf), len(a)))
 
     def set_old_acc(self, old_acc):
         oa = np.ascontiguousarray(old_acc, dtype=np.float32)

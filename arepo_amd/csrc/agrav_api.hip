@@ -33,13 +33,15 @@ agrav_status fail(agrav_ctx *c, agrav_status s, const std::string &m)
 
 static constexpr int B = 256;
 
-// gravity_force_finalize, accel.c:204-347 (no comoving terms, OmegaLambda = 0)
-__global__ void k_finalize(int64_t n, FinalOut f, const double *__restrict__ at, const double *__restrict__ pt,
-                           const int *__restrict__ nint)
+// gravity_force_finalize, accel.c:204-347 (no comoving terms, OmegaLambda = 0), for
+// every particle or for the active targets (sorted positions in tmap)
+__global__ void k_finalize(int64_t n, const int *__restrict__ tmap, FinalOut f, const double *__restrict__ at,
+                           const double *__restrict__ pt, const int *__restrict__ nint)
 {
-  int64_t k = (int64_t)blockIdx.x * B + threadIdx.x;
-  if(k >= n)
+  int64_t j = (int64_t)blockIdx.x * B + threadIdx.x;
+  if(j >= n)
     return;
+  const int64_t k = tmap ? tmap[j] : j;
   finalize_one(f, k, at[3 * k], at[3 * k + 1], at[3 * k + 2], pt[k], nint[k]);
 }
 
@@ -51,6 +53,14 @@ agrav_status final_out(agrav_ctx *c, FinalOut &f)
   if((st = c->d_acc.ensure(3 * (n ? n : 1))) || (st = c->d_pot.ensure(n ? n : 1)) || (st = c->d_nint_out.ensure(n ? n : 1)) ||
      (st = c->d_oldacc.ensure(n ? n : 1)))
     return st;
+  if(c->n_active >= 0 && !c->outputs_zero && n > 0)
+    {
+      // inactive particles keep their outputs; after set_particles they have none yet
+      AG_HIP(hipMemsetAsync(c->d_acc.p, 0, 3 * n * sizeof(double), c->stream));
+      AG_HIP(hipMemsetAsync(c->d_pot.p, 0, n * sizeof(double), c->stream));
+      AG_HIP(hipMemsetAsync(c->d_nint_out.p, 0, n * sizeof(int), c->stream));
+    }
+  c->outputs_zero = true;
   const bool pm = c->par.pmgrid > 0;
   double pot_const = 0;
   if(pm && c->par.periodic)
@@ -86,8 +96,10 @@ agrav_status finalize(agrav_ctx *c)
   if((st = final_out(c, f)) != AGRAV_OK)
     return st;
   AG_HIP(hipEventRecord(c->ev[0], c->stream));
-  if(n > 0)
-    k_finalize<<<(unsigned)((n + B - 1) / B), B, 0, c->stream>>>(n, f, c->d_at.p, c->d_pt.p, c->d_nint.p);
+  const int64_t nt = c->n_active >= 0 ? c->n_active : n;
+  if(nt > 0)
+    k_finalize<<<(unsigned)((nt + B - 1) / B), B, 0, c->stream>>>(nt, c->n_active >= 0 ? c->d_tmap.p : nullptr, f, c->d_at.p,
+                                                                  c->d_pt.p, c->d_nint.p);
   AG_HIP(hipGetLastError());
   AG_HIP(hipEventRecord(c->ev[1], c->stream));
   AG_HIP(hipEventSynchronize(c->ev[1]));
@@ -369,6 +381,11 @@ void agrav_destroy(agrav_ctx *c)
   c->d_nint_out.release();
   c->d_temp.release();
   c->d_dom.release();
+  c->d_active.release();
+  c->d_tmap.release();
+  c->d_aflag.release();
+  c->d_apos.release();
+  c->d_ainv.release();
   c->unb.release();
   if(c->stream)
     (void)hipStreamDestroy(c->stream);
@@ -454,6 +471,9 @@ static agrav_status set_particles_common(agrav_ctx *c, int64_t n, const double *
   c->mass_total     = mt;
   c->n              = n;
   c->batch_nsets    = 0;
+  c->n_active       = -1;
+  c->tmap_dirty     = true;
+  c->outputs_zero   = false;
   c->have_particles = true;
   c->have_oldacc    = false;
   c->have_tree = c->have_tree_force = c->have_pm = c->have_final = false;
@@ -504,9 +524,40 @@ agrav_status agrav_construct_forcetree(agrav_ctx *c, int *numnodes)
   AG_HIP(hipEventElapsedTime(&ms, c->ev[7], c->ev[1]));
   c->tim.treebuild = ms;
   c->tim.numnodes  = c->m;
+  c->tmap_dirty    = true;  // the sorted positions of the active targets changed
   if(numnodes)
     *numnodes = c->m;
   c->have_tree_force = false;
+  return AGRAV_OK;
+}
+
+agrav_status agrav_set_active(agrav_ctx *c, const int32_t *active, int64_t n_active)
+{
+  if(!c)
+    return AGRAV_ERR_BAD_PARAM;
+  if(!c->have_particles)
+    return fail(c, AGRAV_ERR_STATE, "set_active before set_particles");
+  if(!active || n_active < 0)
+    {
+      c->n_active   = -1;
+      c->tmap_dirty = true;
+      return AGRAV_OK;
+    }
+  if(c->batch_nsets > 0)
+    return fail(c, AGRAV_ERR_BAD_PARAM, "active lists select targets of one particle set, not of a batch");
+  if(n_active > c->n)
+    return fail(c, AGRAV_ERR_BAD_PARAM, "more active particles than particles");
+  agrav_status st;
+  if((st = c->d_active.ensure(n_active ? n_active : 1)) != AGRAV_OK)
+    return st;
+  if(n_active > 0)
+    {
+      AG_HIP(hipMemcpyAsync(c->d_active.p, active, n_active * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+      AG_HIP(hipStreamSynchronize(c->stream));
+    }
+  c->n_active        = n_active;
+  c->tmap_dirty      = true;
+  c->have_tree_force = c->have_final = false;
   return AGRAV_OK;
 }
 

@@ -223,6 +223,8 @@ agrav_status agrav_batch_set_particles(agrav_ctx *c, int nsets, const int64_t *o
   c->have_stype         = false;
   c->batch_nsets        = nsets;
   c->batch_groups_dirty = true;
+  c->n_active           = -1;
+  c->tmap_dirty         = true;
   c->mass_total         = mt;
   c->n                  = n;
   c->have_particles     = true;

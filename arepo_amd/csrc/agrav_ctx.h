@@ -184,6 +184,15 @@ struct agrav_ctx
   bool final_fused  = false;  // the last walk wrote them for every target
   agrav::FinalOut fin{};      // what the walk kernel writes (perm null: nothing)
 
+  // active targets of the walk and the finalize (TimeBinsGravity.ActiveParticleList);
+  // n_active < 0: every particle
+  int64_t n_active = -1;
+  agrav::DBuf<int> d_active;      // caller indices as given
+  agrav::DBuf<int> d_tmap;        // their sorted positions, in Hilbert order
+  agrav::DBuf<int> d_aflag, d_apos, d_ainv;
+  bool tmap_dirty   = true;       // the list or the sorted order changed
+  bool outputs_zero = false;      // the caller-order outputs hold values for every particle
+
   // final results in caller order
   agrav::DBuf<double> d_acc, d_pot;
   agrav::DBuf<int> d_nint_out;
@@ -210,6 +219,7 @@ namespace agrav
 agrav_status fail(agrav_ctx *c, agrav_status s, const std::string &m);
 agrav_status build_tree(agrav_ctx *c);
 agrav_status walk_tree(agrav_ctx *c, int use_bh);
+agrav_status active_targets(agrav_ctx *c);
 agrav_status pm_init(agrav_ctx *c);
 void pm_free(agrav_ctx *c);
 agrav_status pm_force(agrav_ctx *c);

@@ -574,6 +574,7 @@ static agrav_status unbind_potential(agrav_ctx *c, int ng, int64_t n, double g_a
   AG_HIP(hipMemcpyAsync(c->d_soff.p, u.soff.p, (ng + 1) * sizeof(int64_t), hipMemcpyDeviceToDevice, s));
   c->h_soff             = soff;
   c->batch_nsets        = ng;
+  c->n_active           = -1;
   c->have_stype         = false;  // one softening per group
   c->batch_groups_dirty = true;
   c->n                  = m;

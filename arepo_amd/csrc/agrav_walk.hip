@@ -144,8 +144,8 @@ template <bool PM, bool EWALD>
 __global__ __launch_bounds__(256) void k_walk_plain(TreeView tv, WalkConst wc, const double *__restrict__ hsoft,
                                                     const float *__restrict__ oldacc_s, const float *__restrict__ tabs,
                                                     const double *__restrict__ ewald, int ntarget, const int *__restrict__ tlist,
-                                                    int tl_group, BatchView bv, double *at, double *pt, int *nint,
-                                                    unsigned long long *isum)
+                                                    int tl_group, const int *__restrict__ tmap, int ntarg, BatchView bv, double *at,
+                                                    double *pt, int *nint, unsigned long long *isum)
 {
   __shared__ float s_tab[256];
   s_tab[threadIdx.x] = tabs[threadIdx.x];
@@ -162,14 +162,12 @@ __global__ __launch_bounds__(256) void k_walk_plain(TreeView tv, WalkConst wc, c
         }
       else if(tlist)
         {
-          // targets of failed groups: tl_group particles per group id
-          int g = tlist[t / tl_group];
-          i     = g * tl_group + (t % tl_group);
-          if(i >= tv.n)
-            i = -1;
+          // targets of failed groups: tl_group consecutive targets per group id
+          const int j = tlist[t / tl_group] * tl_group + (t % tl_group);
+          i           = j < ntarg ? (tmap ? tmap[j] : j) : -1;
         }
       else
-        i = t;
+        i = tmap ? tmap[t] : t;  // every particle, or the active-target list
     }
   int nin = 0;
   if(i >= 0)
@@ -609,8 +607,9 @@ template <typename T> struct GroupAcc
 template <typename T, bool PM, bool EWALD, bool MSOFT>
 __device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, const WalkConst &wc, const double *__restrict__ hsoft,
                                   const float *__restrict__ oldacc_s, const float *__restrict__ ewald_f,
-                                  const double *__restrict__ ewald_d, int p0, int np, int root, T unit, int leaf_bucket,
-                                  int nsplit, int wsplit, GroupAcc<T> &out, unsigned long long (&cnt)[3])
+                                  const double *__restrict__ ewald_d, const int *__restrict__ tmap, int p0, int np, int root,
+                                  T unit, int leaf_bucket, int nsplit, int wsplit, GroupAcc<T> &out,
+                                  unsigned long long (&cnt)[3])
 {
   int *s_stack     = lds.stack;
   Vec4<T> *s_list  = lds.list;
@@ -621,7 +620,8 @@ __device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, co
 
   const int N  = tv.n;
   const bool has = lane < np;
-  const int pi   = p0 + (has ? lane : 0);
+  // the lane's target: Hilbert-consecutive particles, or entries of the active-target list
+  const int pi   = tmap ? tmap[p0 + (has ? lane : 0)] : p0 + (has ? lane : 0);
 
   // ---- targets relative to the group centre
   T tx, ty, tz, hx, hy, hz;
@@ -666,6 +666,18 @@ __device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, co
 
   const T rcut = (T)wc.rcut, rcut2 = (T)wc.rcut2, theta2 = (T)wc.theta2;
   const T asmthfac = (T)wc.asmthfac, asmthinv2 = (T)wc.asmthinv2;
+  // TreePM pairs take their periodic image relative to the group centre, which is every
+  // target's nearest image of every source within Rcut only while the group's
+  // half-extent plus Rcut stays below half a box.  Groups of Hilbert-consecutive
+  // particles are far smaller; a wider group (sparse active targets) is reported as
+  // failed before its walk starts (every wavefront of the group decides alike) and is
+  // walked per target by the caller.
+  if(PM && wc.periodic)
+    {
+      const T hmax = hx > hy ? (hx > hz ? hx : hz) : (hy > hz ? hy : hz);
+      if(hmax + rcut >= (T)(0.49 * wc.box))
+        return false;
+    }
 
   // ---- walk
   int sp = 1;
@@ -1038,8 +1050,8 @@ template <typename T, bool PM, bool EWALD, bool MSOFT, int W>
 __global__ __launch_bounds__(64 * W) void k_walk_group(TreeView tv, WalkConst wc, const double *__restrict__ hsoft,
                                                        const float *__restrict__ oldacc_s, const float *__restrict__ tabs,
                                                        const float *__restrict__ ewald_f, const double *__restrict__ ewald_d, int gs,
-                                                       double *at, double *pt, int *nint, int *failed, int *nfailed,
-                                                       unsigned long long *isum, FinalOut fin)
+                                                       const int *__restrict__ tmap, int ntarg, double *at, double *pt, int *nint,
+                                                       int *failed, int *nfailed, unsigned long long *isum, FinalOut fin)
 {
   __shared__ WalkLds<T, MSOFT> lds[W];
   __shared__ int s_fail;
@@ -1049,13 +1061,13 @@ __global__ __launch_bounds__(64 * W) void k_walk_group(TreeView tv, WalkConst wc
     s_fail = 0;
   const int g  = xcd_block(blockIdx.x, gridDim.x);
   const int p0 = g * gs;
-  const int np = min(gs, tv.n - p0);
+  const int np = min(gs, ntarg - p0);
   unsigned long long cnt[3] = {0, 0, 0};
   GroupAcc<T> acc;
-  const bool ok = walk_group<T, PM, EWALD, MSOFT>(lds[wv], tv, wc, hsoft, oldacc_s, ewald_f, ewald_d, p0, np, tv.n, (T)wc.fix_unit,
-                                                  wc.leaf_bucket, W, wv, acc, cnt);
+  const bool ok = walk_group<T, PM, EWALD, MSOFT>(lds[wv], tv, wc, hsoft, oldacc_s, ewald_f, ewald_d, tmap, p0, np, tv.n,
+                                                  (T)wc.fix_unit, wc.leaf_bucket, W, wv, acc, cnt);
   flush_counts(cnt, isum);
-  const int64_t pi = (int64_t)p0 + lane;
+  const int64_t pi = lane < np ? (tmap ? tmap[p0 + lane] : p0 + lane) : 0;
   if(W == 1)
     {
       if(ok && lane < np)
@@ -1143,8 +1155,8 @@ __global__ __launch_bounds__(64) void k_walk_batch(TreeView tv, WalkConst wc, Ba
       const int4 d = bv.groups[g];
       const T unit = (T)bv.dom[d.z].fix_unit;
       GroupAcc<T> acc;
-      if(walk_group<T, false, false, false>(lds, tv, wc, bv.sh + d.z, oldacc_s, nullptr, nullptr, d.x, d.y, bv.sroot[d.z], unit, d.w,
-                                            1, 0, acc, cnt))
+      if(walk_group<T, false, false, false>(lds, tv, wc, bv.sh + d.z, oldacc_s, nullptr, nullptr, nullptr, d.x, d.y, bv.sroot[d.z],
+                                            unit, d.w, 1, 0, acc, cnt))
         {
           if((int)threadIdx.x < d.y)
             {
@@ -1198,11 +1210,17 @@ WalkConst walk_const(const agrav_ctx *c, int use_bh)
   return wc;
 }
 
+// the walk's targets: every particle, or the active-target list (sorted positions in
+// Hilbert order)
+static const int *target_map(const agrav_ctx *c) { return c->n_active >= 0 ? c->d_tmap.p : nullptr; }
+static int target_count(const agrav_ctx *c) { return c->n_active >= 0 ? (int)c->n_active : (int)c->n; }
+
 template <typename T, bool PM, bool EW, bool MS, int W>
 static void launch_group_w(int ngroups, hipStream_t s, const TreeView &tv, const WalkConst &wc, agrav_ctx *c, int gs)
 {
   k_walk_group<T, PM, EW, MS, W><<<ngroups, 64 * W, 0, s>>>(tv, wc, c->d_soft.p, c->d_oldacc_s.p, c->d_tab.p, c->d_ewald_f.p,
-                                                            c->d_ewald.p, gs, c->d_at.p, c->d_pt.p, c->d_nint.p, c->d_failed.p,
+                                                            c->d_ewald.p, gs, target_map(c), target_count(c), c->d_at.p,
+                                                            c->d_pt.p, c->d_nint.p, c->d_failed.p,
                                                             reinterpret_cast<int *>(c->d_isum.p + 3), c->d_isum.p, c->fin);
 }
 
@@ -1252,15 +1270,17 @@ static void launch_plain(bool pm, bool ew, int nt, const int *tlist, int tl_grou
   unsigned g = (unsigned)((nt + 255) / 256);
   if(g == 0)
     return;
+  const int *tm  = bv.pset ? nullptr : target_map(c);
+  const int ntar = bv.pset ? (int)c->n : target_count(c);
   if(pm)
     k_walk_plain<true, false><<<g, 256, 0, s>>>(tv, wc, c->d_soft.p, c->d_oldacc_s.p, c->d_tab.p, c->d_ewald.p, nt, tlist,
-                                                 tl_group, bv, c->d_at.p, c->d_pt.p, c->d_nint.p, c->d_isum.p);
+                                                 tl_group, tm, ntar, bv, c->d_at.p, c->d_pt.p, c->d_nint.p, c->d_isum.p);
   else if(ew)
     k_walk_plain<false, true><<<g, 256, 0, s>>>(tv, wc, c->d_soft.p, c->d_oldacc_s.p, c->d_tab.p, c->d_ewald.p, nt, tlist,
-                                                 tl_group, bv, c->d_at.p, c->d_pt.p, c->d_nint.p, c->d_isum.p);
+                                                 tl_group, tm, ntar, bv, c->d_at.p, c->d_pt.p, c->d_nint.p, c->d_isum.p);
   else
     k_walk_plain<false, false><<<g, 256, 0, s>>>(tv, wc, c->d_soft.p, c->d_oldacc_s.p, c->d_tab.p, c->d_ewald.p, nt, tlist,
-                                                  tl_group, bv, c->d_at.p, c->d_pt.p, c->d_nint.p, c->d_isum.p);
+                                                  tl_group, tm, ntar, bv, c->d_at.p, c->d_pt.p, c->d_nint.p, c->d_isum.p);
 }
 
 BatchView batch_view(const agrav_ctx *c)
@@ -1337,6 +1357,75 @@ agrav_status walk_batch_groups(agrav_ctx *c, int use_bh, const int4 *groups, int
   return AGRAV_OK;
 }
 
+// active targets (caller indices) -> their sorted positions in Hilbert order: a flag
+// per sorted position, a scan, a scatter; a repeated or out-of-range index is an error
+__global__ void k_flag_active(const int *__restrict__ active, int64_t na, int64_t n, const int *__restrict__ inv, int *flag,
+                              int *bad)
+{
+  const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if(j >= na)
+    return;
+  const int i = active[j];
+  if(i < 0 || i >= n)
+    {
+      atomicOr(bad, 1);
+      return;
+    }
+  if(atomicExch(&flag[inv[i]], 1) != 0)
+    atomicOr(bad, 2);
+}
+
+__global__ void k_inverse_perm(const int *__restrict__ perm, int64_t n, int *inv)
+{
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if(k < n)
+    inv[perm[k]] = (int)k;
+}
+
+__global__ void k_fill_tmap(const int *__restrict__ flag, const int *__restrict__ pos, int64_t n, int *tmap)
+{
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if(k < n && flag[k])
+    tmap[pos[k]] = (int)k;
+}
+
+agrav_status active_targets(agrav_ctx *c)
+{
+  if(!c->tmap_dirty)
+    return AGRAV_OK;
+  const int64_t n = c->n, na = c->n_active;
+  hipStream_t s   = c->stream;
+  agrav_status st;
+  if((st = c->d_tmap.ensure(na ? na : 1)) || (st = c->d_aflag.ensure(n ? n : 1)) || (st = c->d_apos.ensure(n ? n : 1)) ||
+     (st = c->d_ainv.ensure(n ? n : 1)) || (st = c->d_temp.ensure(scan_temp_bytes(n) + 4096)))
+    return st;
+  int *d_bad = c->d_counters.p + 102, *d_err = c->d_counters.p + 103;
+  AG_HIP(hipMemsetAsync(d_bad, 0, 2 * sizeof(int), s));
+  if(na > 0)
+    {
+      const unsigned gn = (unsigned)((n + 255) / 256), ga = (unsigned)((na + 255) / 256);
+      AG_HIP(hipMemsetAsync(c->d_aflag.p, 0, n * sizeof(int), s));
+      k_inverse_perm<<<gn, 256, 0, s>>>(c->d_idx2.p, n, c->d_ainv.p);
+      k_flag_active<<<ga, 256, 0, s>>>(c->d_active.p, na, n, c->d_ainv.p, c->d_aflag.p, d_bad);
+      AG_HIP(hipGetLastError());
+      if((st = exclusive_scan_i32(c->d_aflag.p, c->d_apos.p, n, nullptr, c->d_temp.p, d_err, s)) != AGRAV_OK)
+        return st;
+      k_fill_tmap<<<gn, 256, 0, s>>>(c->d_aflag.p, c->d_apos.p, n, c->d_tmap.p);
+      AG_HIP(hipGetLastError());
+    }
+  int bad[2];
+  AG_HIP(hipMemcpyAsync(bad, d_bad, sizeof(bad), hipMemcpyDeviceToHost, s));
+  AG_HIP(hipStreamSynchronize(s));
+  if(bad[0] & 1)
+    return fail(c, AGRAV_ERR_BAD_PARAM, "active index out of range");
+  if(bad[0] & 2)
+    return fail(c, AGRAV_ERR_BAD_PARAM, "active list repeats a particle");
+  if(bad[1])
+    return fail(c, AGRAV_ERR_HIP, "active-target scan look-back did not complete");
+  c->tmap_dirty = false;
+  return AGRAV_OK;
+}
+
 agrav_status walk_tree(agrav_ctx *c, int use_bh)
 {
   if(!c->have_tree)
@@ -1369,6 +1458,17 @@ agrav_status walk_tree(agrav_ctx *c, int use_bh)
       if((st = c->d_ewald.ensure(1)) || (st = c->d_ewald_f.ensure(1)))
         return st;
     }
+  // the active-target list in Hilbert order, after each build or change of the list
+  if(c->n_active >= 0 && (st = active_targets(c)) != AGRAV_OK)
+    return st;
+  const int64_t nt = c->n_active >= 0 ? c->n_active : n;
+  if(nt == 0)
+    {
+      c->tim.treewalk = c->tim.walk_kernel = 0;
+      c->tim.interactions                  = 0;
+      c->have_tree_force                   = true;
+      return AGRAV_OK;
+    }
   c->tim.walk_retries = 0;
   unsigned long long cnt[4] = {0, 0, 0, 0};
   bool have_cnt       = false;  // the counters are read already (group walk, no re-walk)
@@ -1384,7 +1484,7 @@ agrav_status walk_tree(agrav_ctx *c, int use_bh)
   else if(c->par.walk == AGRAV_WALK_PLAIN)
     {
       AG_HIP(hipEventRecord(c->ev[2], s));
-      launch_plain(pm, ew, (int)n, nullptr, 1, s, tv, wc, c, batch_view(c));
+      launch_plain(pm, ew, (int)nt, nullptr, 1, s, tv, wc, c, batch_view(c));
       AG_HIP(hipGetLastError());
       AG_HIP(hipEventRecord(c->ev[3], s));
     }
@@ -1398,14 +1498,14 @@ agrav_status walk_tree(agrav_ctx *c, int use_bh)
       const bool isolated = !pm && !ew;
       int gs = c->par.group_size;
       if(gs <= 0)
-        gs = (isolated && n < (1 << 19)) ? 16 : 64;
-      int ngroups = (int)((n + gs - 1) / gs);
+        gs = (isolated && nt < (1 << 19)) ? 16 : 64;
+      int ngroups = (int)((nt + gs - 1) / gs);
       if((st = c->d_failed.ensure(ngroups)) != AGRAV_OK)
         return st;
       // wavefronts per group: small sets have too few groups to fill the device
       int waves = c->par.group_waves;
       if(waves <= 0)
-        waves = isolated ? (n < (1 << 17) ? 4 : 1) : (n < (1 << 18) ? 4 : (n < (1 << 20) ? 2 : 1));
+        waves = isolated ? (nt < (1 << 17) ? 4 : 1) : (nt < (1 << 18) ? 4 : (nt < (1 << 20) ? 2 : 1));
       c->tim_group_waves = c->par.precision == AGRAV_FP32 ? waves : 1;
       // within a full step the walk's epilogue also writes the final outputs (the PM
       // has run, in sorted order); the finalize kernel then has nothing left to do

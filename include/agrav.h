@@ -132,6 +132,16 @@ agrav_status agrav_set_particles_device(agrav_ctx *ctx, int64_t n, const double 
 /* P[i].OldAcc (accel.c:237-255), host pointer, caller order. */
 agrav_status agrav_set_old_acc(agrav_ctx *ctx, const float *old_acc);
 
+/* The active particles, gravity_tree's TargetList (gravtree.c:494-520, built from
+ * TimeBinsGravity.ActiveParticleList, allvars.h:664): caller indices, each at most
+ * once.  The next agrav_gravity_tree, agrav_gravity_force_finalize and
+ * agrav_compute_grav_accelerations evaluate and write only these; the tree is built
+ * from every particle and the long-range PM force covers every particle, as in the
+ * reference.  Outputs of inactive particles keep their last values (zero after
+ * set_particles).  active = NULL or n_active < 0 makes every particle active again
+ * (the default); set_particles resets the list.  Not for batches. */
+agrav_status agrav_set_active(agrav_ctx *ctx, const int32_t *active, int64_t n_active);
+
 /* construct_forcetree (forcetree.c:110-155): domain cube, Peano-Hilbert keys,
  * sort, octree, bottom-up moments.  numnodes may be NULL. */
 agrav_status agrav_construct_forcetree(agrav_ctx *ctx, int *numnodes);
