@@ -97,7 +97,7 @@ EXPORTED = [
     "agrav_get_components", "agrav_gravity_forcetest", "agrav_gravity_direct", "agrav_get_timings", "agrav_get_tree",
     "agrav_get_order", "agrav_get_tables", "agrav_hilbert_key21",
     "agrav_batch_set_particles", "agrav_batch_autotune", "agrav_batch_get_tuning", "agrav_batch_set_tuning",
-    "agrav_subfind_unbind", "agrav_server_create", "agrav_server_submit", "agrav_server_wait", "agrav_server_poll", "agrav_server_stats",
+    "agrav_subfind_unbind", "agrav_gravity_timesteps", "agrav_server_create", "agrav_server_submit", "agrav_server_wait", "agrav_server_poll", "agrav_server_stats",
     "agrav_server_destroy",
 ]
 
@@ -117,6 +117,17 @@ class BatchTuning(C.Structure):
 
     def as_dict(self):
         return {k: (list(getattr(self, k)) if k != "tuned" else self.tuned) for k, _ in self._fields_}
+
+
+class TimestepParams(C.Structure):
+    _fields_ = [
+        ("err_tol_int_accuracy", C.c_double),
+        ("max_size_timestep", C.c_double),
+        ("min_size_timestep", C.c_double),
+        ("time_a", C.c_double),
+        ("hubble_a", C.c_double),
+        ("comoving", C.c_int),
+    ]
 
 
 class UnbindParams(C.Structure):
@@ -184,6 +195,8 @@ def _declare(L):
     L.agrav_batch_autotune.argtypes = [vp, C.c_int64, C.POINTER(BatchTuning)]
     L.agrav_batch_get_tuning.argtypes = [vp, C.POINTER(BatchTuning)]
     L.agrav_batch_set_tuning.argtypes = [vp, C.POINTER(BatchTuning)]
+    L.agrav_gravity_timesteps.argtypes = [vp, C.POINTER(TimestepParams), vp, C.POINTER(C.c_double),
+                                          C.POINTER(C.c_int64)]
     L.agrav_subfind_unbind.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp, C.POINTER(UnbindParams), vp, vp,
                                        vp, vp, vp, vp]
     L.agrav_server_create.argtypes = [C.POINTER(Params), C.c_int, C.c_size_t, C.POINTER(vp)]
@@ -352,6 +365,15 @@ class Gravity:
         a = np.ascontiguousarray(active, dtype=np.int32)
         buf = a if len(a) else np.zeros(1, np.int32)  # an empty list is a list, not "every particle"
         self._check(self.L.agrav_set_active(self.h, _ptr(buf), len(a)))
+
+    def gravity_timesteps(self, eta=0.025, max_dt=0.01, min_dt=0.0, time_a=1.0, hubble_a=0.0, comoving=False):
+        """dt per particle (get_timestep_gravity) and the PM displacement constraint after a step."""
+        tp = TimestepParams(eta, max_dt, min_dt, time_a, hubble_a, int(comoving))
+        dt = np.zeros(max(self.n, 1))
+        dd = C.c_double(0)
+        nb = C.c_int64(0)
+        self._check(self.L.agrav_gravity_timesteps(self.h, C.byref(tp), _ptr(dt), C.byref(dd), C.byref(nb)))
+        return dt[:self.n], dd.value, nb.value
 
     def set_old_acc(self, old_acc):
         oa = np.ascontiguousarray(old_acc, dtype=np.float32)

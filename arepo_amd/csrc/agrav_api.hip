@@ -111,6 +111,73 @@ agrav_status finalize(agrav_ctx *c)
   return AGRAV_OK;
 }
 
+// timestep criteria after a step (timestep.c:284-344, :525-565): the physical
+// acceleration a2inv |a|, dt = sqrt(2 ErrTolIntAccuracy a ForceSoftening / 2.8 / ac) H(a)
+struct TsArgs
+{
+  double eta2a;  // 2 ErrTolIntAccuracy cf_atime
+  double a2inv, hub, maxdt, mindt;
+};
+
+// find_long_range_step_constraint: the minimum over every particle of the dt of its PM
+// acceleration (positive doubles order like their bit patterns: one atomic minimum)
+__global__ void k_pm_dtmin(int64_t n, const double *__restrict__ acc_pm, const uint8_t *__restrict__ st, const double *__restrict__ fsoft,
+                           TsArgs a, unsigned long long *dtmin)
+{
+  __shared__ unsigned long long s_min[B / 64];
+  const int64_t i        = (int64_t)blockIdx.x * B + threadIdx.x;
+  unsigned long long v   = ~0ull;
+  if(i < n)
+    {
+      const double ax = a.a2inv * acc_pm[3 * i], ay = a.a2inv * acc_pm[3 * i + 1], az = a.a2inv * acc_pm[3 * i + 2];
+      double ac       = sqrt(ax * ax + ay * ay + az * az);
+      if(ac < 1e-37)  // MIN_FLOAT_NUMBER
+        ac = 1e-37;
+      const double d = sqrt(a.eta2a * fsoft[st ? st[i] : 0] / (2.8 * ac)) * a.hub;
+      v              = (unsigned long long)__double_as_longlong(d);
+    }
+  for(int o = 32; o > 0; o >>= 1)
+    {
+      const unsigned long long w = __shfl_xor(v, o, 64);
+      v                          = w < v ? w : v;
+    }
+  if((threadIdx.x & 63) == 0)
+    s_min[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if(threadIdx.x == 0)
+    {
+      for(int k = 1; k < B / 64; k++)
+        v = s_min[k] < v ? s_min[k] : v;
+      atomicMin(dtmin, v);
+    }
+}
+
+// get_timestep_gravity before the integer timeline, caller order
+__global__ void k_grav_dt(int64_t n, const double *__restrict__ acc, const double *__restrict__ mass, const uint8_t *__restrict__ st,
+                          const double *__restrict__ fsoft, TsArgs a, int have_pm, double dtdisp, double *dt, int *below)
+{
+  const int64_t i = (int64_t)blockIdx.x * B + threadIdx.x;
+  if(i >= n)
+    return;
+  const double ax = a.a2inv * acc[3 * i], ay = a.a2inv * acc[3 * i + 1], az = a.a2inv * acc[3 * i + 2];
+  double ac       = sqrt(ax * ax + ay * ay + az * az);
+  if(ac == 0)
+    ac = 1.0e-30;
+  double d = sqrt(a.eta2a * fsoft[st ? st[i] : 0] / 2.8 / ac) * a.hub;
+  if(mass[i] == 0)
+    d = a.maxdt;  // a swallowed particle (timestep.c:328-329)
+  if(d >= a.maxdt)
+    d = a.maxdt;
+  if(d < a.mindt)
+    {
+      atomicAdd(below, 1);
+      d = a.mindt;
+    }
+  if(have_pm && d >= dtdisp)
+    d = dtdisp;
+  dt[i] = d;
+}
+
 // softening types of a device-resident input must index the softening table
 __global__ void k_check_stype(const uint8_t *__restrict__ st, int64_t n, int nsoft, int *bad)
 {
@@ -558,6 +625,74 @@ agrav_status agrav_set_active(agrav_ctx *c, const int32_t *active, int64_t n_act
   c->n_active        = n_active;
   c->tmap_dirty      = true;
   c->have_tree_force = c->have_final = false;
+  return AGRAV_OK;
+}
+
+agrav_status agrav_gravity_timesteps(agrav_ctx *c, const agrav_timestep_params *tp, double *dt, double *dt_displacement,
+                                     int64_t *n_below_min)
+{
+  if(!c || !tp || !(tp->err_tol_int_accuracy > 0) || !(tp->max_size_timestep > 0) || tp->min_size_timestep < 0)
+    return AGRAV_ERR_BAD_PARAM;
+  if(tp->comoving && !(tp->time_a > 0))
+    return fail(c, AGRAV_ERR_BAD_PARAM, "comoving timesteps need a positive scale factor");
+  if(!c->have_final)
+    return fail(c, AGRAV_ERR_STATE, "gravity_timesteps before the accelerations of a step");
+  if(c->batch_nsets > 0)
+    return fail(c, AGRAV_ERR_BAD_PARAM, "timesteps are for one particle set, not for a batch");
+  const int64_t n = c->n;
+  hipStream_t s   = c->stream;
+  TsArgs a;
+  a.eta2a = 2.0 * tp->err_tol_int_accuracy * (tp->comoving ? tp->time_a : 1.0);
+  a.a2inv = tp->comoving ? 1.0 / (tp->time_a * tp->time_a) : 1.0;
+  a.hub   = tp->comoving ? tp->hubble_a : 1.0;
+  a.maxdt = tp->max_size_timestep;
+  a.mindt = tp->min_size_timestep;
+  agrav_status st;
+  if((st = c->d_counters.ensure(128)) || (st = c->d_temp.ensure(n ? 8 * n : 8)))
+    return st;
+  unsigned long long *d_min = reinterpret_cast<unsigned long long *>(c->d_counters.p + 104);  // 8-byte aligned
+  int *d_below              = c->d_counters.p + 106;
+  const bool pm             = c->par.pmgrid > 0 && c->have_pm;
+  double dtdisp             = tp->max_size_timestep;
+  unsigned long long init   = ~0ull;
+  int zero                  = 0;
+  AG_HIP(hipMemcpyAsync(d_min, &init, sizeof(init), hipMemcpyHostToDevice, s));
+  AG_HIP(hipMemcpyAsync(d_below, &zero, sizeof(zero), hipMemcpyHostToDevice, s));
+  const unsigned g = (unsigned)((n + B - 1) / B);
+  if(pm && n > 0)
+    {
+      // the PM accelerations are in sorted order when the tree of the step was current
+      const uint8_t *stp = c->pm_sorted ? (c->soft.nsoft > 1 || c->have_stype ? c->d_ps.p : nullptr)
+                                        : (c->have_stype ? c->d_stype_in.p : nullptr);
+      k_pm_dtmin<<<g, B, 0, s>>>(n, c->d_acc_pm.p, stp, c->d_soft.p, a, d_min);
+      AG_HIP(hipGetLastError());
+      unsigned long long bits = 0;
+      AG_HIP(hipMemcpyAsync(&bits, d_min, sizeof(bits), hipMemcpyDeviceToHost, s));
+      AG_HIP(hipStreamSynchronize(s));
+      double m = 0;
+      memcpy(&m, &bits, sizeof(m));
+      dtdisp = 2.0 * m;  // one timebin higher (timestep.c:558)
+      if(dtdisp > tp->max_size_timestep)
+        dtdisp = tp->max_size_timestep;
+      if(dtdisp < tp->min_size_timestep)
+        dtdisp = tp->min_size_timestep;
+    }
+  double *d_dt = reinterpret_cast<double *>(c->d_temp.p);
+  if(n > 0)
+    {
+      k_grav_dt<<<g, B, 0, s>>>(n, c->d_acc.p, c->d_mass_in.p, c->have_stype ? c->d_stype_in.p : nullptr, c->d_soft.p, a, pm ? 1 : 0,
+                                dtdisp, d_dt, d_below);
+      AG_HIP(hipGetLastError());
+      if(dt)
+        AG_HIP(hipMemcpyAsync(dt, d_dt, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    }
+  int below = 0;
+  AG_HIP(hipMemcpyAsync(&below, d_below, sizeof(below), hipMemcpyDeviceToHost, s));
+  AG_HIP(hipStreamSynchronize(s));
+  if(dt_displacement)
+    *dt_displacement = dtdisp;
+  if(n_below_min)
+    *n_below_min = below;
   return AGRAV_OK;
 }
 

@@ -167,6 +167,30 @@ agrav_status agrav_compute_grav_accelerations(agrav_ctx *ctx, int first_step);
 agrav_status agrav_get_accelerations(agrav_ctx *ctx, double *acc, double *pot, int32_t *nint, float *old_acc);
 /* Device pointers to the same results (valid until the next call on ctx). */
 agrav_status agrav_get_accelerations_device(agrav_ctx *ctx, const double **acc, const double **pot);
+/* Timestep criteria from the accelerations of the last step, before AREPO's integer
+ * timeline: find_long_range_step_constraint (timestep.c:525-565; PMGRID only, every
+ * particle counted as collisionless) gives dt_displacement = 2 x the minimum over
+ * particles of sqrt(2 ErrTolIntAccuracy a ForceSoftening / 2.8 / |a_PM,phys|) H(a),
+ * clamped to [MinSizeTimestep, MaxSizeTimestep]; get_timestep_gravity
+ * (timestep.c:284-344) gives dt[i] (caller order, may be NULL) from the total
+ * acceleration (GravAccel + GravPM), capped at MaxSizeTimestep and, with PMGRID, at
+ * dt_displacement.  Comoving: a2inv = 1/a^2 on accelerations, a and H(a) as in
+ * set_cosmo_factors_for_current_time (timestep.c:77-90).  A massless particle gets
+ * MaxSizeTimestep.  A dt below MinSizeTimestep is raised to it and counted in
+ * n_below_min (the reference stops there unless NOSTOP_WHEN_BELOW_MINTIMESTEP). */
+typedef struct
+{
+  double err_tol_int_accuracy; /* ErrTolIntAccuracy */
+  double max_size_timestep;    /* MaxSizeTimestep */
+  double min_size_timestep;    /* MinSizeTimestep */
+  double time_a;               /* All.Time, comoving only */
+  double hubble_a;             /* hubble_function(All.Time), comoving only */
+  int comoving;                /* ComovingIntegrationOn */
+} agrav_timestep_params;
+
+agrav_status agrav_gravity_timesteps(agrav_ctx *ctx, const agrav_timestep_params *tp, double *dt, double *dt_displacement,
+                                     int64_t *n_below_min);
+
 /* Raw parts: tree force (G=1 units) and PM force, caller order, host pointers (may be NULL). */
 agrav_status agrav_get_components(agrav_ctx *ctx, double *acc_tree, double *pot_tree, double *acc_pm, double *pot_pm);
 

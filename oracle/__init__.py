@@ -66,6 +66,33 @@ class UnbindParams(C.Structure):
     ]
 
 
+class TimestepParams(C.Structure):
+    _fields_ = [
+        ("err_tol_int_accuracy", C.c_double),
+        ("max_size_timestep", C.c_double),
+        ("min_size_timestep", C.c_double),
+        ("time_a", C.c_double),
+        ("hubble_a", C.c_double),
+        ("comoving", C.c_int),
+    ]
+
+
+def gravity_timesteps(p, acc, mass, acc_pm=None, stype=None, eta=0.025, max_dt=0.01, min_dt=0.0, time_a=1.0,
+                      hubble_a=0.0, comoving=False):
+    """get_timestep_gravity's dt per particle and the PM displacement constraint (timestep.c:284-344, :525-565)."""
+    tp = TimestepParams(eta, max_dt, min_dt, time_a, hubble_a, int(comoving))
+    acc = np.ascontiguousarray(acc, dtype=np.float64).reshape(-1, 3)
+    n = len(acc)
+    mass = np.ascontiguousarray(mass, dtype=np.float64)
+    ap = None if acc_pm is None else np.ascontiguousarray(acc_pm, dtype=np.float64).reshape(-1, 3)
+    st = None if stype is None else np.ascontiguousarray(stype, dtype=np.uint8)
+    dt = np.zeros(max(n, 1))
+    dd = C.c_double(0)
+    below = lib().orc_gravity_timesteps(C.byref(tp), C.byref(p), n, acc, None if ap is None else ap.ctypes.data, mass,
+                                        None if st is None else st.ctypes.data, dt, C.byref(dd))
+    return dt[:n], dd.value, below
+
+
 def make_unbind_params(G=1.0, time_a=1.0, hubble_a=0.0, comoving=False, theta=0.7, des_link_ngb=20,
                        max_iter=300000):
     return UnbindParams(G, time_a, hubble_a, int(comoving), theta, des_link_ngb, max_iter)
@@ -131,6 +158,9 @@ def _declare(L):
                                      C.c_double, C.c_void_p, _u8, _dp, _dp, C.POINTER(C.c_int),
                                      C.POINTER(C.c_int)]
     L.orc_subfind_unbind.restype = C.c_int
+    L.orc_gravity_timesteps.argtypes = [C.POINTER(TimestepParams), C.POINTER(Params), C.c_int, _dp, C.c_void_p, _dp,
+                                        C.c_void_p, _dp, C.POINTER(C.c_double)]
+    L.orc_gravity_timesteps.restype = C.c_int
 
 
 def _pos(pos):

@@ -1748,3 +1748,66 @@ int orc_subfind_unbind(const orc_unbind_params *up, int n, const double *pos, co
   free(spot);
   return rc < 0 ? rc : len;
 }
+
+/* ------------------------------------------------------------------ */
+/* Gravity timesteps                                                   */
+/* ------------------------------------------------------------------ */
+
+int orc_gravity_timesteps(const orc_timestep_params *tp, const orc_params *p, int n, const double *acc, const double *acc_pm,
+                          const double *mass, const uint8_t *stype, double *dt, double *dt_displacement)
+{
+  double fs[ORC_MAXSOFT + 1];
+  orc_set_softenings(p, fs);
+  const double a2inv = tp->comoving ? 1.0 / (tp->time_a * tp->time_a) : 1.0;
+  const double atime = tp->comoving ? tp->time_a : 1.0;
+  const double hub   = tp->comoving ? tp->hubble_a : 1.0;
+  /* find_long_range_step_constraint, timestep.c:525-565 (every particle collisionless) */
+  double dtdisp = tp->max_size_timestep;
+  if(acc_pm)
+    {
+      double dtmin = 1e306; /* MAX_DOUBLE_NUMBER */
+      for(int i = 0; i < n; i++)
+        {
+          double ax = a2inv * acc_pm[3 * i], ay = a2inv * acc_pm[3 * i + 1], az = a2inv * acc_pm[3 * i + 2];
+          double ac = sqrt(ax * ax + ay * ay + az * az);
+          if(ac < 1e-37) /* MIN_FLOAT_NUMBER */
+            ac = 1e-37;
+          double d = sqrt(2.0 * tp->err_tol_int_accuracy * atime * fs[stype ? stype[i] : 0] / (2.8 * ac));
+          d *= hub;
+          if(d < dtmin)
+            dtmin = d;
+        }
+      dtmin *= 2.0;
+      dtdisp = dtmin;
+      if(dtdisp > tp->max_size_timestep)
+        dtdisp = tp->max_size_timestep;
+      if(dtdisp < tp->min_size_timestep)
+        dtdisp = tp->min_size_timestep;
+    }
+  if(dt_displacement)
+    *dt_displacement = dtdisp;
+  /* get_timestep_gravity, timestep.c:284-344, before the integer timeline */
+  int below = 0;
+  for(int i = 0; i < n; i++)
+    {
+      double ax = a2inv * acc[3 * i], ay = a2inv * acc[3 * i + 1], az = a2inv * acc[3 * i + 2];
+      double ac = sqrt(ax * ax + ay * ay + az * az);
+      if(ac == 0)
+        ac = 1.0e-30;
+      double d = sqrt(2 * tp->err_tol_int_accuracy * atime * fs[stype ? stype[i] : 0] / 2.8 / ac);
+      d *= hub;
+      if(mass[i] == 0)
+        d = tp->max_size_timestep;
+      if(d >= tp->max_size_timestep)
+        d = tp->max_size_timestep;
+      if(d < tp->min_size_timestep)
+        {
+          below++;
+          d = tp->min_size_timestep;
+        }
+      if(acc_pm && d >= dtdisp)
+        d = dtdisp;
+      dt[i] = d;
+    }
+  return below;
+}

@@ -141,6 +141,22 @@ int orc_subfind_unbind(const orc_unbind_params *up, int n, const double *pos, co
                        const double *utherm, const uint8_t *type, double eps, const double *cube, uint8_t *bound,
                        double *potential, double *bnd_energy, int *len_non_gas, int *iterations);
 
+typedef struct
+{
+  double err_tol_int_accuracy; /* All.ErrTolIntAccuracy */
+  double max_size_timestep;    /* All.MaxSizeTimestep */
+  double min_size_timestep;    /* All.MinSizeTimestep */
+  double time_a;               /* All.Time when comoving */
+  double hubble_a;             /* hubble_function(All.Time) when comoving */
+  int comoving;                /* All.ComovingIntegrationOn */
+} orc_timestep_params;
+
+/* find_long_range_step_constraint (timestep.c:525-565) over acc_pm (NULL: no PM) and get_timestep_gravity's
+ * dt (timestep.c:284-344) before the integer timeline, for n particles with total acceleration acc
+ * (GravAccel + GravPM).  A dt below MinSizeTimestep is raised to it and counted (the return value). */
+int orc_gravity_timesteps(const orc_timestep_params *tp, const orc_params *p, int n, const double *acc, const double *acc_pm,
+                          const double *mass, const uint8_t *stype, double *dt, double *dt_displacement);
+
 /* Textbook in-place complex FFT of a cube (n^3, n power of two); sign -1 forward (FFTW_FORWARD), +1 backward; unnormalised */
 void orc_fft3d(int n, double *re, double *im, int sign);
 
