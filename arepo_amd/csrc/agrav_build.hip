@@ -4,20 +4,22 @@
 // and then computes moments with a recursive depth-first pass (:1217-1429).
 // Here the same octree -- every cube that holds two or more particles, single
 // particles hanging in the child slots -- is derived in parallel from sorted
-// 63-bit Peano-Hilbert keys:
+// Peano-Hilbert keys:
 //   1. domain cube (domain_box.c:107-197) and keys from AREPO's 52-bit integer
-//      coordinates (forcetree.c:493-505), top 21 bits per axis;
-//   2. LSD radix sort of the keys (Hilbert order, like domain.c:140);
-//   3. binary radix tree over the sorted keys (Karras, HPG 2012);
-//   4. each binary node whose prefix crosses 3-bit digit boundaries yields the
-//      octree cubes in between; cubes are placed in pre-order (first particle,
-//      then level) so AREPO's nextnode/sibling threading (forcetree.c:1288-1330)
-//      follows from array positions;
-//   5. moments level by level, deepest first, in FP64.
+//      coordinates (forcetree.c:493-505): 30 levels (MAX_TREE_LEVEL,
+//      forcetree.h:106) in two 63-bit words;
+//   2. LSD radix sort of the first words (Hilbert order, like domain.c:140), then
+//      runs of equal first words ordered by the second words;
+//   3. the levels at which cubes start at each particle, from its neighbours'
+//      keys; cubes are placed in pre-order (first particle, then level) so
+//      AREPO's nextnode/sibling threading (forcetree.c:1288-1330) follows from
+//      array positions;
+//   4. moments of every cube at once from double-double prefix sums over the
+//      Hilbert-ordered particles.
 // Cube centres and lengths use AREPO's integer formula (forcetree.c:903-906),
-// so node geometry is bit-identical to the reference's.  Cells of the
-// 21st level that still hold several particles become buckets whose
-// particles are direct children (AREPO would split them further, up to level 30).
+// so node geometry is bit-identical to the reference's.  Cells of level 30 that
+// still hold several particles become buckets whose particles are direct
+// children (AREPO goes on putting them into free slots, forcetree.c:861-877).
 #include "agrav_ctx.h"
 
 #include <algorithm>
