@@ -803,7 +803,7 @@ __device__ inline DD4 particle_dd4(double m, double x, double y, double z)
   return r;
 }
 
-static constexpr int kMomItems = 8;
+static constexpr int kMomItems = 4;
 static constexpr int kMomTile  = B * kMomItems;
 
 // The prefix sums restart at every set of a batch, over tiles aligned to the set's
@@ -844,23 +844,79 @@ __device__ inline DD4 block_excl_scan_dd4(DD4 v, DD4 *s_buf)
   return r;
 }
 
-// 1. per tile: the sums of its (m, m x, m y, m z)
+__device__ inline DD4 dd4_shfl_up(const DD4 &a, int d)
+{
+  DD4 r;
+#pragma unroll
+  for(int k = 0; k < 4; k++)
+    r.v[k] = DD{__shfl_up(a.v[k].hi, d, 64), __shfl_up(a.v[k].lo, d, 64)};
+  return r;
+}
+
+__device__ inline DD4 dd4_shfl_xor(const DD4 &a, int d)
+{
+  DD4 r;
+#pragma unroll
+  for(int k = 0; k < 4; k++)
+    r.v[k] = DD{__shfl_xor(a.v[k].hi, d, 64), __shfl_xor(a.v[k].lo, d, 64)};
+  return r;
+}
+
+// exclusive scan of one value per thread over the block: inclusive scans inside each
+// wavefront by shuffles, then the wavefront totals in order through LDS (one barrier)
+__device__ inline DD4 block_excl_scan_dd4_waves(DD4 v, DD4 *s_wave)
+{
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  DD4 inc = v;
+#pragma unroll
+  for(int d = 1; d < 64; d <<= 1)
+    {
+      const DD4 o = dd4_shfl_up(inc, d);
+      if(lane >= d)
+        inc = dd4_add(o, inc);
+    }
+  if(lane == 63)
+    s_wave[w] = inc;
+  __syncthreads();
+  DD4 base = dd4_zero();
+  for(int k = 0; k < w; k++)
+    base = dd4_add(base, s_wave[k]);
+  DD4 ex = dd4_shfl_up(inc, 1);
+  if(lane == 0)
+    ex = dd4_zero();
+  return dd4_add(base, ex);
+}
+
+// 1. per tile: the sums of its (m, m x, m y, m z): coalesced reads, then a fixed-order
+//    reduction (wave butterflies, then the wavefronts in order)
 __global__ __launch_bounds__(B) void k_mom_tile_sums(MomTiles mt, const double *__restrict__ px, const double *__restrict__ py,
                                                      const double *__restrict__ pz, const double *__restrict__ pm, DD4 *tsum)
 {
-  __shared__ DD4 s_buf[B];
+  __shared__ DD4 s_wave[B / 64];
   const int4 tl = mom_tile(mt, blockIdx.x, gridDim.x);
-  const int l0  = threadIdx.x * kMomItems;
   DD4 acc       = dd4_zero();
   for(int u = 0; u < kMomItems; u++)
-    if(l0 + u < tl.y)
-      {
-        const int i = tl.x + l0 + u;
-        acc         = dd4_add(acc, particle_dd4(pm[i], px[i], py[i], pz[i]));
-      }
-  const DD4 ex = block_excl_scan_dd4(acc, s_buf);
-  if(threadIdx.x == B - 1)
-    tsum[blockIdx.x] = dd4_add(ex, acc);
+    {
+      const int l = threadIdx.x + u * B;
+      if(l < tl.y)
+        {
+          const int i = tl.x + l;
+          acc         = dd4_add(acc, particle_dd4(pm[i], px[i], py[i], pz[i]));
+        }
+    }
+#pragma unroll
+  for(int d = 32; d > 0; d >>= 1)
+    acc = dd4_add(acc, dd4_shfl_xor(acc, d));
+  if((threadIdx.x & 63) == 0)
+    s_wave[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if(threadIdx.x == 0)
+    {
+      DD4 t = s_wave[0];
+      for(int k = 1; k < B / 64; k++)
+        t = dd4_add(t, s_wave[k]);
+      tsum[blockIdx.x] = t;
+    }
 }
 
 // 2. per set (one block each): exclusive prefix of its tile sums, in place, and the
@@ -912,38 +968,65 @@ __global__ __launch_bounds__(B) void k_mom_settot_scan(int nsets, const DD4 *__r
 }
 
 // 3. the prefix before every particle, and each set's total after its last particle:
-//    high parts in FP64, low parts in FP32
+//    high parts in FP64, low parts in FP32.  The tile passes through LDS: coalesced
+//    loads of {m, x, y, z}, a serial run of kMomItems entries per thread with the
+//    block scan between the runs, the prefix written over each input slot once it is
+//    read, then coalesced stores.  Entry l sits at slot l + l / kMomItems: the pad
+//    slot per run spreads the threads' strided slots over the LDS banks.
+__device__ inline int mom_slot(int l) { return l + l / kMomItems; }
+
 __global__ __launch_bounds__(B) void k_mom_prefix(MomTiles mt, const double *__restrict__ px, const double *__restrict__ py,
                                                   const double *__restrict__ pz, const double *__restrict__ pm,
                                                   const DD4 *__restrict__ toff, double4 *phi, float4 *plo)
 {
-  __shared__ DD4 s_buf[B];
+  static constexpr int kSlots = kMomTile + kMomTile / kMomItems;
+  __shared__ double4 s_q[kSlots];
+  __shared__ float4 s_lo[kSlots];
+  __shared__ DD4 s_wave[B / 64];
   const int4 tl = mom_tile(mt, blockIdx.x, gridDim.x);
-  const int l0  = threadIdx.x * kMomItems;
-  DD4 v[kMomItems];
-  DD4 acc = dd4_zero();
+  for(int l = threadIdx.x; l < tl.y; l += B)
+    {
+      const int i       = tl.x + l;
+      s_q[mom_slot(l)]  = make_double4(pm[i], px[i], py[i], pz[i]);
+    }
+  __syncthreads();
+  const int l0 = threadIdx.x * kMomItems;
+  DD4 acc      = dd4_zero();
+#pragma unroll
+  for(int u = 0; u < kMomItems; u++)
+    if(l0 + u < tl.y)
+      {
+        const double4 q = s_q[mom_slot(l0 + u)];
+        acc             = dd4_add(acc, particle_dd4(q.x, q.y, q.z, q.w));
+      }
+  DD4 run = dd4_add(toff[blockIdx.x], block_excl_scan_dd4_waves(acc, s_wave));
 #pragma unroll
   for(int u = 0; u < kMomItems; u++)
     {
-      const int i = tl.x + l0 + u;
-      v[u]        = l0 + u < tl.y ? particle_dd4(pm[i], px[i], py[i], pz[i]) : dd4_zero();
-      acc         = dd4_add(acc, v[u]);
-    }
-  DD4 run = dd4_add(toff[blockIdx.x], block_excl_scan_dd4(acc, s_buf));
-  // one writer per entry: the prefix before a particle by the thread holding it, the
-  // set's total by the thread holding the set's last particle
-#pragma unroll
-  for(int u = 0; u <= kMomItems; u++)
-    {
       const int l = l0 + u;
-      if((l < tl.y && u < kMomItems) || (l == tl.y && u > 0 && tl.w))
+      if(l < tl.y)
         {
-          const int64_t e = (int64_t)tl.x + l + tl.z;
-          phi[e]          = make_double4(run.v[0].hi, run.v[1].hi, run.v[2].hi, run.v[3].hi);
-          plo[e]          = make_float4((float)run.v[0].lo, (float)run.v[1].lo, (float)run.v[2].lo, (float)run.v[3].lo);
+          const int k      = mom_slot(l);
+          const double4 q  = s_q[k];
+          s_q[k]           = make_double4(run.v[0].hi, run.v[1].hi, run.v[2].hi, run.v[3].hi);
+          s_lo[k]          = make_float4((float)run.v[0].lo, (float)run.v[1].lo, (float)run.v[2].lo, (float)run.v[3].lo);
+          run              = dd4_add(run, particle_dd4(q.x, q.y, q.z, q.w));
+          if(l == tl.y - 1 && tl.w)
+            {
+              // the set's total, one entry past its last particle
+              const int64_t e = (int64_t)tl.x + tl.y + tl.z;
+              phi[e]          = make_double4(run.v[0].hi, run.v[1].hi, run.v[2].hi, run.v[3].hi);
+              plo[e]          = make_float4((float)run.v[0].lo, (float)run.v[1].lo, (float)run.v[2].lo, (float)run.v[3].lo);
+            }
         }
-      if(u < kMomItems)
-        run = dd4_add(run, v[u]);
+    }
+  __syncthreads();
+  for(int l = threadIdx.x; l < tl.y; l += B)
+    {
+      const int64_t e = (int64_t)tl.x + l + tl.z;
+      const int k     = mom_slot(l);
+      phi[e]          = s_q[k];
+      plo[e]          = s_lo[k];
     }
 }
 

@@ -471,7 +471,8 @@ __device__ inline void tile_compute_f32(const Vec4<float> *__restrict__ s_list, 
           const int ti   = (int)te;
           valid          = ti < AG_NTAB;
           const int tc   = min(ti, AG_NTAB);
-          const float w  = te - (float)tc;
+          // te - ti exactly (te >= 0); for te past the table the pair is invalid anyway
+          const float w   = __builtin_amdgcn_fractf(te);
           const float2 fd = ftab[tc];
           ff = fd.x + w * fd.y;  // force table is pre-scaled by asmthinv2
           if(POT)
