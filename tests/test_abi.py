@@ -48,6 +48,19 @@ def test_create_without_gpu_fails_loudly(agrav):
     assert ei.value.code in (agrav.ERR_NO_DEVICE, agrav.ERR_HIP)
 
 
+@pytest.mark.parametrize("bad", [
+    dict(periodic=False, box_size=0.0, pmgrid=32),  # isolated TreePM: the reference leaves Asmth/Rcut unset
+    dict(periodic=True, box_size=1.0, pmgrid=48),   # not a power of two
+    dict(periodic=True, box_size=1.0, pmgrid=32, rcut=0.0),
+    dict(periodic=True, box_size=0.0),
+])
+def test_invalid_params_rejected_before_device(agrav, bad):
+    # checked before the device is touched, so the same code comes back with or without a GPU
+    with pytest.raises(agrav.AgravError) as ei:
+        agrav.Gravity(agrav.make_params(**bad))
+    assert ei.value.code == agrav.ERR_BAD_PARAM
+
+
 def _cells(level):
     """All cells of a 2^level grid, keyed by the top `level` bits of the 21-bit key coordinates."""
     n = 1 << level
