@@ -92,6 +92,7 @@ def lib():
 EXPORTED = [
     "agrav_status_string", "agrav_last_error", "agrav_default_params", "agrav_create", "agrav_destroy",
     "agrav_set_softenings", "agrav_set_particles", "agrav_set_particles_device", "agrav_set_old_acc", "agrav_set_active",
+    "agrav_set_partial_tree",
     "agrav_construct_forcetree", "agrav_gravity_tree", "agrav_long_range_force", "agrav_gravity_force_finalize",
     "agrav_compute_grav_accelerations", "agrav_get_accelerations", "agrav_get_accelerations_device",
     "agrav_get_components", "agrav_gravity_forcetest", "agrav_gravity_direct", "agrav_get_timings", "agrav_get_tree",
@@ -177,6 +178,7 @@ def _declare(L):
     L.agrav_set_particles_device.argtypes = [vp, C.c_int64, vp, vp, vp]
     L.agrav_set_old_acc.argtypes = [vp, vp]
     L.agrav_set_active.argtypes = [vp, vp, C.c_int64]
+    L.agrav_set_partial_tree.argtypes = [vp, C.c_int, C.c_int]
     L.agrav_construct_forcetree.argtypes = [vp, C.POINTER(C.c_int)]
     L.agrav_gravity_tree.argtypes = [vp, C.c_int]
     L.agrav_long_range_force.argtypes = [vp]
@@ -366,6 +368,10 @@ class Gravity:
         buf = a if len(a) else np.zeros(1, np.int32)  # an empty list is a list, not "every particle"
         self._check(self.L.agrav_set_active(self.h, _ptr(buf), len(a)))
 
+    def set_partial_tree(self, partial=True, set_oldacc=True):
+        """HIERARCHICAL_GRAVITY: builds insert only the active particles; set_oldacc=False keeps OldAcc."""
+        self._check(self.L.agrav_set_partial_tree(self.h, int(bool(partial)), int(bool(set_oldacc))))
+
     def gravity_timesteps(self, eta=0.025, max_dt=0.01, min_dt=0.0, time_a=1.0, hubble_a=0.0, comoving=False):
         """dt per particle (get_timestep_gravity) and the PM displacement constraint after a step."""
         tp = TimestepParams(eta, max_dt, min_dt, time_a, hubble_a, int(comoving))
@@ -456,10 +462,12 @@ class Gravity:
         return dict(center=center, len=ln, com=com, mass=mass, level=level, npart=npart)
 
     def get_order(self):
-        perm = np.zeros(self.n, np.int32)
-        keys = np.zeros(self.n, np.uint64)
+        """(perm, keys) of the tree's particles (the active ones after a partial tree)."""
+        perm = np.full(max(self.n, 1), -1, np.int32)
+        keys = np.zeros(max(self.n, 1), np.uint64)
         self._check(self.L.agrav_get_order(self.h, _ptr(perm), _ptr(keys)))
-        return perm, keys
+        k = int(np.count_nonzero(perm >= 0))
+        return perm[:k], keys[:k]
 
     def get_tables(self, ewald_tree=False, ewald_forcetest=False):
         tf = np.zeros(128, np.float32)

@@ -69,8 +69,9 @@ agrav_status final_out(agrav_ctx *c, FinalOut &f)
       double L     = c->par.box_size;
       pot_const    = c->mass_total * M_PI / (asmth * asmth * L * L * L);  // accel.c:269-274
     }
+  // OldAcc is left as it is on request (accel.c:231-232: set on the highest occupied timebin only)
   f = FinalOut{c->d_idx2.p, pm ? c->d_acc_pm.p : nullptr, pm ? c->d_pot_pm.p : nullptr, c->pm_sorted ? 1 : 0, c->par.G,
-               pot_const, c->d_acc.p, c->d_pot.p, c->d_nint_out.p, c->d_oldacc.p, c->d_oldacc_s.p};
+               pot_const, c->d_acc.p, c->d_pot.p, c->d_nint_out.p, c->keep_oldacc ? nullptr : c->d_oldacc.p, c->d_oldacc_s.p};
   return AGRAV_OK;
 }
 
@@ -88,7 +89,7 @@ agrav_status finalize(agrav_ctx *c)
       // the walk's epilogue has written the outputs of every target
       c->final_fused  = false;
       c->tim.finalize = 0;
-      c->have_oldacc  = true;
+      c->have_oldacc  = c->have_oldacc || !c->keep_oldacc;
       c->have_final   = true;
       return AGRAV_OK;
     }
@@ -96,17 +97,18 @@ agrav_status finalize(agrav_ctx *c)
   if((st = final_out(c, f)) != AGRAV_OK)
     return st;
   AG_HIP(hipEventRecord(c->ev[0], c->stream));
-  const int64_t nt = c->n_active >= 0 ? c->n_active : n;
+  const bool tm    = c->n_active >= 0 && !c->tree_partial;
+  const int64_t nt = c->tree_partial ? c->n_tree : c->n_active >= 0 ? c->n_active : n;
   if(nt > 0)
-    k_finalize<<<(unsigned)((nt + B - 1) / B), B, 0, c->stream>>>(nt, c->n_active >= 0 ? c->d_tmap.p : nullptr, f, c->d_at.p,
-                                                                  c->d_pt.p, c->d_nint.p);
+    k_finalize<<<(unsigned)((nt + B - 1) / B), B, 0, c->stream>>>(nt, tm ? c->d_tmap.p : nullptr, f, c->d_at.p, c->d_pt.p,
+                                                                  c->d_nint.p);
   AG_HIP(hipGetLastError());
   AG_HIP(hipEventRecord(c->ev[1], c->stream));
   AG_HIP(hipEventSynchronize(c->ev[1]));
   float ms = 0;
   AG_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
   c->tim.finalize = ms;
-  c->have_oldacc  = true;
+  c->have_oldacc  = c->have_oldacc || !c->keep_oldacc;
   c->have_final   = true;
   return AGRAV_OK;
 }
@@ -443,6 +445,8 @@ void agrav_destroy(agrav_ctx *c)
   c->d_isum.release();
   c->d_acc_pm.release();
   c->d_pot_pm.release();
+  c->d_acc_pm2.release();
+  c->d_pot_pm2.release();
   c->d_acc.release();
   c->d_pot.release();
   c->d_nint_out.release();
@@ -539,6 +543,8 @@ static agrav_status set_particles_common(agrav_ctx *c, int64_t n, const double *
   c->n              = n;
   c->batch_nsets    = 0;
   c->n_active       = -1;
+  c->active_version++;
+  c->tree_partial   = false;
   c->tmap_dirty     = true;
   c->outputs_zero   = false;
   c->have_particles = true;
@@ -604,6 +610,7 @@ agrav_status agrav_set_active(agrav_ctx *c, const int32_t *active, int64_t n_act
     return AGRAV_ERR_BAD_PARAM;
   if(!c->have_particles)
     return fail(c, AGRAV_ERR_STATE, "set_active before set_particles");
+  c->active_version++;
   if(!active || n_active < 0)
     {
       c->n_active   = -1;
@@ -625,6 +632,15 @@ agrav_status agrav_set_active(agrav_ctx *c, const int32_t *active, int64_t n_act
   c->n_active        = n_active;
   c->tmap_dirty      = true;
   c->have_tree_force = c->have_final = false;
+  return AGRAV_OK;
+}
+
+agrav_status agrav_set_partial_tree(agrav_ctx *c, int partial, int set_oldacc)
+{
+  if(!c)
+    return AGRAV_ERR_BAD_PARAM;
+  c->partial_tree = partial != 0;
+  c->keep_oldacc  = set_oldacc == 0;
   return AGRAV_OK;
 }
 
@@ -731,8 +747,12 @@ agrav_status agrav_compute_grav_accelerations(agrav_ctx *c, int first_step)
   // the PM does not depend on the tree; running it after the build lets it use the
   // Hilbert-ordered particles (the reference calls long_range_force first)
   agrav_status st = agrav_construct_forcetree(c, nullptr);
-  if(st == AGRAV_OK && c->par.pmgrid > 0)
+  // a partial-tree step keeps the long-range force of the last PM while the particles
+  // are unchanged, as GravPM between PM steps (do_gravity_hydro.c:105-123)
+  if(st == AGRAV_OK && c->par.pmgrid > 0 && !(c->tree_partial && c->have_pm))
     st = pm_force(c);
+  else if(st == AGRAV_OK && c->par.pmgrid > 0 && (st = pm_time_settle(c)) == AGRAV_OK)
+    c->tim.pm_grav = 0;  // no PM in this call
   // the walks of a full step write the final outputs from their epilogue
   c->fuse_final = true;
   const bool seed = c->par.type_of_opening_criterion == 1 && first_step && c->par.err_tol_theta > 0;
@@ -798,15 +818,17 @@ agrav_status agrav_get_components(agrav_ctx *c, double *acc_tree, double *pot_tr
     return fail(c, AGRAV_ERR_STATE, "no tree force");
   if((acc_pm || pot_pm) && !c->have_pm)
     return fail(c, AGRAV_ERR_STATE, "no PM force");
-  if(acc_tree || pot_tree)
+  if((acc_tree || pot_tree) && c->n_tree > 0)
     {
-      std::vector<double> at(3 * n), pt(n);
-      std::vector<int> perm(n);
-      AG_HIP(hipMemcpyAsync(at.data(), c->d_at.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost, s));
-      AG_HIP(hipMemcpyAsync(pt.data(), c->d_pt.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
-      AG_HIP(hipMemcpyAsync(perm.data(), c->d_idx2.p, n * sizeof(int), hipMemcpyDeviceToHost, s));
+      // a partial tree holds, and writes, only its own particles
+      const int64_t nt = c->n_tree;
+      std::vector<double> at(3 * nt), pt(nt);
+      std::vector<int> perm(nt);
+      AG_HIP(hipMemcpyAsync(at.data(), c->d_at.p, 3 * nt * sizeof(double), hipMemcpyDeviceToHost, s));
+      AG_HIP(hipMemcpyAsync(pt.data(), c->d_pt.p, nt * sizeof(double), hipMemcpyDeviceToHost, s));
+      AG_HIP(hipMemcpyAsync(perm.data(), c->d_idx2.p, nt * sizeof(int), hipMemcpyDeviceToHost, s));
       AG_HIP(hipStreamSynchronize(s));
-      for(int64_t k = 0; k < n; k++)
+      for(int64_t k = 0; k < nt; k++)
         {
           int i = perm[k];
           if(acc_tree)
@@ -933,7 +955,7 @@ agrav_status agrav_get_order(agrav_ctx *c, int32_t *perm, uint64_t *keys)
     return AGRAV_ERR_BAD_PARAM;
   if(!c->have_tree)
     return fail(c, AGRAV_ERR_STATE, "no tree");
-  const int64_t n = c->n;
+  const int64_t n = c->n_tree;
   if(n == 0)
     return AGRAV_OK;
   if(perm)

@@ -269,12 +269,13 @@ __device__ inline uint64_t int52(double x, const Domain &dom, int d)
 // then zeros.  The octree splits cells down to `levels` (AREPO's insertion tree
 // splits down to MAX_TREE_LEVEL, forcetree.c:861-877); only particles sharing a
 // cell below the first word need the second one.
-__global__ void k_keys(const double *__restrict__ pos, int64_t n, BuildGeo geo, int periodic, double box,
-                       uint64_t *__restrict__ keys, uint64_t *__restrict__ keys_lo, int *__restrict__ idx, Domain *dom_w)
+__global__ void k_keys(const double *__restrict__ pos, const int *__restrict__ src, int64_t n, BuildGeo geo, int periodic,
+                       double box, uint64_t *__restrict__ keys, uint64_t *__restrict__ keys_lo, int *__restrict__ idx, Domain *dom_w)
 {
-  int64_t i = (int64_t)blockIdx.x * B + threadIdx.x;
-  if(i >= n)
+  const int64_t k = (int64_t)blockIdx.x * B + threadIdx.x;
+  if(k >= n)
     return;
+  const int64_t i = src ? (int64_t)src[k] : k;  // caller index (src: a partial tree's checked list)
   const int set    = geo.pset_in ? geo.pset_in[i] : 0;
   const Domain dom = geo.dom[set];
   const int L      = geo.levels;
@@ -295,13 +296,13 @@ __global__ void k_keys(const double *__restrict__ pos, int64_t n, BuildGeo geo, 
       c[0] = c[1] = c[2] = 0;
     }
   hilbert_transpose(c, L);
-  unsigned __int128 k = (unsigned __int128)(unsigned)set;
+  unsigned __int128 key = (unsigned __int128)(unsigned)set;
   for(int b = L - 1; b >= 0; b--)
-    k = (k << 3) | (((c[0] >> b) & 1u) << 2) | (((c[1] >> b) & 1u) << 1) | ((c[2] >> b) & 1u);
-  k <<= (126 - geo.sbits - 3 * L);
-  keys[i]    = (uint64_t)(k >> 63);
-  keys_lo[i] = (uint64_t)k & ((1ull << 63) - 1);
-  idx[i]     = (int)i;
+    key = (key << 3) | (((c[0] >> b) & 1u) << 2) | (((c[1] >> b) & 1u) << 1) | ((c[2] >> b) & 1u);
+  key <<= (126 - geo.sbits - 3 * L);
+  keys[k]    = (uint64_t)(key >> 63);
+  keys_lo[k] = (uint64_t)key & ((1ull << 63) - 1);
+  idx[k]     = (int)i;
 }
 
 // runs of equal first key words in the sorted order (particles sharing a cell at
@@ -1145,12 +1146,88 @@ __global__ void k_set_roots(int nsets, int n, const int64_t *__restrict__ off, c
 // host driver
 // ---------------------------------------------------------------------------
 
+// A partial tree reads positions through the active list, so the list is checked
+// before anything indexes with it: every entry in range and listed once.
+__global__ void k_check_active(const int *__restrict__ active, int64_t na, int64_t n, int *flag, int *bad)
+{
+  const int64_t j = (int64_t)blockIdx.x * B + threadIdx.x;
+  if(j >= na)
+    return;
+  const int i = active[j];
+  if(i < 0 || i >= n)
+    {
+      atomicOr(bad, 1);
+      return;
+    }
+  if(atomicExch(&flag[i], 1) != 0)
+    atomicOr(bad, 2);
+}
+
+// PM results kept in the sorted order of the tree about to be replaced, back to caller order
+__global__ void k_unsort_pm(const int *__restrict__ perm, int64_t n, const double *__restrict__ acc_s,
+                            const double *__restrict__ pot_s, double *acc, double *pot)
+{
+  const int64_t k = (int64_t)blockIdx.x * B + threadIdx.x;
+  if(k >= n)
+    return;
+  const int64_t i = perm[k];
+  acc[3 * i]      = acc_s[3 * k];
+  acc[3 * i + 1]  = acc_s[3 * k + 1];
+  acc[3 * i + 2]  = acc_s[3 * k + 2];
+  pot[i]          = pot_s[k];
+}
+
+static agrav_status partial_tree_prepare(agrav_ctx *c)
+{
+  hipStream_t s    = c->stream;
+  const int64_t n  = c->n, na = c->n_active;
+  agrav_status st;
+  if((st = c->d_counters.ensure(128)) || (st = c->d_aflag.ensure(n ? n : 1)))
+    return st;
+  int *d_bad = c->d_counters.p + 102;
+  AG_HIP(hipMemsetAsync(d_bad, 0, sizeof(int), s));
+  if(na > 0)
+    {
+      AG_HIP(hipMemsetAsync(c->d_aflag.p, 0, n * sizeof(int), s));
+      k_check_active<<<nblk(na), B, 0, s>>>(c->d_active.p, na, n, c->d_aflag.p, d_bad);
+      AG_HIP(hipGetLastError());
+    }
+  if(c->have_pm && c->pm_sorted && c->n_tree == n && n > 0)
+    {
+      if((st = c->d_acc_pm2.ensure(3 * n)) || (st = c->d_pot_pm2.ensure(n)))
+        return st;
+      k_unsort_pm<<<nblk(n), B, 0, s>>>(c->d_idx2.p, n, c->d_acc_pm.p, c->d_pot_pm.p, c->d_acc_pm2.p, c->d_pot_pm2.p);
+      AG_HIP(hipGetLastError());
+      std::swap(c->d_acc_pm, c->d_acc_pm2);
+      std::swap(c->d_pot_pm, c->d_pot_pm2);
+      c->pm_sorted = false;
+    }
+  int bad = 0;
+  AG_HIP(hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, s));
+  AG_HIP(hipStreamSynchronize(s));
+  if(bad & 1)
+    return fail(c, AGRAV_ERR_BAD_PARAM, "active index out of range");
+  if(bad & 2)
+    return fail(c, AGRAV_ERR_BAD_PARAM, "active list repeats a particle");
+  return AGRAV_OK;
+}
+
 agrav_status build_tree(agrav_ctx *c)
 {
-  hipStream_t s   = c->stream;
-  const int64_t n = c->n;
-  const int N     = (int)n;
+  hipStream_t s = c->stream;
+  // HIERARCHICAL_GRAVITY's partial tree (forcetree.c:91-95, :166-171): only the active
+  // particles are inserted, in a domain cube that still spans every particle
+  const bool part    = c->partial_tree && c->n_active >= 0 && c->batch_nsets == 0;
+  const int64_t nall = c->n;
+  const int64_t n    = part ? c->n_active : nall;
+  const int N        = (int)n;
   agrav_status st;
+  c->have_tree = false;
+  if(part && (st = partial_tree_prepare(c)) != AGRAV_OK)
+    return st;
+  c->tree_partial        = part;
+  c->tree_active_version = c->active_version;
+  c->n_tree              = n;
   if(n <= 0)
     {
       c->m         = 0;
@@ -1187,12 +1264,12 @@ agrav_status build_tree(agrav_ctx *c)
   BuildGeo geo{c->d_dom.p, nullptr, nullptr, 0, AG_MAX_LEVELS};
   if(!c->par.periodic && nsets == 0)
     {
-      unsigned g = nblk(n);
+      unsigned g = nblk(nall);
       if(g > 1024)
         g = 1024;
-      k_extent<<<g, B, 0, s>>>(c->d_pos_in.p, n, d_mm);
+      k_extent<<<g, B, 0, s>>>(c->d_pos_in.p, nall, d_mm);
     }
-  k_domain_finish<<<1, 1, 0, s>>>(d_mm, c->d_dom.p, c->par.periodic, c->par.box_size, n);
+  k_domain_finish<<<1, 1, 0, s>>>(d_mm, c->d_dom.p, c->par.periodic, c->par.box_size, nall);
   AG_HIP(hipGetLastError());
   if(nsets > 0)
     {
@@ -1219,8 +1296,8 @@ agrav_status build_tree(agrav_ctx *c)
   if((st = c->d_key.ensure(n)) || (st = c->d_key2.ensure(n)) || (st = c->d_idx.ensure(n)) || (st = c->d_idx2.ensure(n)) ||
      (st = c->d_key_lo.ensure(n)) || (st = c->d_key_lo_s.ensure(n)) || (st = c->d_ties.ensure(2 * (n / 2 + 1))))
     return st;
-  k_keys<<<nblk(n), B, 0, s>>>(c->d_pos_in.p, n, geo, c->par.periodic, c->par.box_size, c->d_key.p, c->d_key_lo.p, c->d_idx.p,
-                               c->d_dom.p);
+  k_keys<<<nblk(n), B, 0, s>>>(c->d_pos_in.p, part ? c->d_active.p : nullptr, n, geo, c->par.periodic, c->par.box_size,
+                               c->d_key.p, c->d_key_lo.p, c->d_idx.p, c->d_dom.p);
   AG_HIP(hipGetLastError());
   if((st = radix_sort_u64(c->d_key.p, c->d_idx.p, c->d_key2.p, c->d_idx2.p, n, 63, c->d_temp.p, d_perr, s)) != AGRAV_OK)
     return st;
@@ -1402,7 +1479,7 @@ agrav_status build_tree(agrav_ctx *c)
 TreeView tree_view(const agrav_ctx *c)
 {
   TreeView v;
-  v.n        = (int)c->n;
+  v.n        = (int)c->n_tree;
   v.m        = c->m;
   v.px       = c->d_px.p;
   v.py       = c->d_py.p;

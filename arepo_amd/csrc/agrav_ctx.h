@@ -177,6 +177,7 @@ struct agrav_ctx
   // PM
   agrav::PmState pm;
   agrav::DBuf<double> d_acc_pm, d_pot_pm;  // caller order, or sorted order when pm_sorted
+  agrav::DBuf<double> d_acc_pm2, d_pot_pm2;  // caller-order copies made before a partial tree replaces the order
   bool pm_sorted = false;
   bool pm_time_pending = false;  // the last PM's events are recorded, its time not yet read
   // finalize fused into the group walk's epilogue (set by compute_grav_accelerations)
@@ -192,6 +193,15 @@ struct agrav_ctx
   agrav::DBuf<int> d_aflag, d_apos, d_ainv;
   bool tmap_dirty   = true;       // the list or the sorted order changed
   bool outputs_zero = false;      // the caller-order outputs hold values for every particle
+  int64_t active_version = 0;     // bumped by every new list
+
+  // HIERARCHICAL_GRAVITY (agrav_set_partial_tree): builds insert only the active
+  // particles; the finalize may leave OldAcc as it is
+  bool partial_tree  = false;     // the setting
+  bool keep_oldacc   = false;
+  bool tree_partial  = false;     // the current tree holds the active particles only
+  int64_t n_tree     = 0;         // particles in the current tree
+  int64_t tree_active_version = -1;
 
   // final results in caller order
   agrav::DBuf<double> d_acc, d_pot;

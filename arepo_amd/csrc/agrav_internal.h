@@ -123,7 +123,7 @@ struct FinalOut
   double G, pot_const;
   double *acc, *pot;
   int *nint;
-  float *oldacc, *oldacc_s;  // caller order, sorted order
+  float *oldacc, *oldacc_s;  // caller order (null: OldAcc is kept), sorted order
 };
 
 __device__ inline void finalize_one(const FinalOut &f, int64_t k, double ax, double ay, double az, double pt, int ni)
@@ -139,9 +139,12 @@ __device__ inline void finalize_one(const FinalOut &f, int64_t k, double ax, dou
       pp              = f.pot_pm[q];
     }
   const double ox = ax + p0 / f.G, oy = ay + p1 / f.G, oz = az + p2 / f.G;
-  const float oa  = (float)sqrt(ox * ox + oy * oy + oz * oz);
-  f.oldacc[i]     = oa;
-  f.oldacc_s[k]   = oa;
+  if(f.oldacc)
+    {
+      const float oa = (float)sqrt(ox * ox + oy * oy + oz * oz);
+      f.oldacc[i]    = oa;
+      f.oldacc_s[k]  = oa;
+    }
   f.acc[3 * (int64_t)i + 0] = ax * f.G + p0;
   f.acc[3 * (int64_t)i + 1] = ay * f.G + p1;
   f.acc[3 * (int64_t)i + 2] = az * f.G + p2;

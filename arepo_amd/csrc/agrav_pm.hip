@@ -454,7 +454,7 @@ template <typename R> static agrav_status pm_run(agrav_ctx *c)
   AG_HIP(hipMemsetAsync(p.rho_fix.p, 0, g3 * sizeof(unsigned long long), s));
   // with a current tree the particles are available in Hilbert order: tiled deposit and
   // coherent readout, results kept in sorted order (the finalize kernel maps them back)
-  const bool sorted = c->have_tree;
+  const bool sorted = c->have_tree && !c->tree_partial;  // a partial tree lacks the inactive particles
   if(n > 0 && sorted)
     k_cic_deposit_sorted<<<nb(n), B, 0, s>>>(c->d_px.p, c->d_py.p, c->d_pz.p, c->d_pm.p, n, G, L, scale, p.rho_fix.p);
   int *d_bad = nullptr;

@@ -1212,9 +1212,12 @@ WalkConst walk_const(const agrav_ctx *c, int use_bh)
 }
 
 // the walk's targets: every particle, or the active-target list (sorted positions in
-// Hilbert order)
-static const int *target_map(const agrav_ctx *c) { return c->n_active >= 0 ? c->d_tmap.p : nullptr; }
-static int target_count(const agrav_ctx *c) { return c->n_active >= 0 ? (int)c->n_active : (int)c->n; }
+// Hilbert order); a partial tree's targets are its own particles
+static const int *target_map(const agrav_ctx *c) { return c->n_active >= 0 && !c->tree_partial ? c->d_tmap.p : nullptr; }
+static int target_count(const agrav_ctx *c)
+{
+  return c->tree_partial ? (int)c->n_tree : c->n_active >= 0 ? (int)c->n_active : (int)c->n;
+}
 
 template <typename T, bool PM, bool EW, bool MS, int W>
 static void launch_group_w(int ngroups, hipStream_t s, const TreeView &tv, const WalkConst &wc, agrav_ctx *c, int gs)
@@ -1459,10 +1462,12 @@ agrav_status walk_tree(agrav_ctx *c, int use_bh)
       if((st = c->d_ewald.ensure(1)) || (st = c->d_ewald_f.ensure(1)))
         return st;
     }
+  if(c->tree_partial && c->tree_active_version != c->active_version)
+    return fail(c, AGRAV_ERR_STATE, "the active list changed after the partial tree was built");
   // the active-target list in Hilbert order, after each build or change of the list
-  if(c->n_active >= 0 && (st = active_targets(c)) != AGRAV_OK)
+  if(c->n_active >= 0 && !c->tree_partial && (st = active_targets(c)) != AGRAV_OK)
     return st;
-  const int64_t nt = c->n_active >= 0 ? c->n_active : n;
+  const int64_t nt = target_count(c);
   if(nt == 0)
     {
       c->tim.treewalk = c->tim.walk_kernel = 0;
@@ -1499,7 +1504,19 @@ agrav_status walk_tree(agrav_ctx *c, int use_bh)
       const bool isolated = !pm && !ew;
       int gs = c->par.group_size;
       if(gs <= 0)
-        gs = (isolated && nt < (1 << 19)) ? 16 : 64;
+        {
+          gs = (isolated && nt < (1 << 19)) ? 16 : 64;
+          // a sparse active list spreads a group over a larger volume (more nodes opened)
+          // and gives fewer groups: about 256 N_active / N targets per group, 8 at least
+          // (2M-particle clustered TreePM box, 3 % active: walk 1.7 ms against 5.0 ms)
+          if(nt < c->n_tree / 2)
+            {
+              int sg = 64;
+              while(sg > 8 && (int64_t)sg * c->n_tree > 256 * nt)
+                sg >>= 1;
+              gs = std::min(gs, sg);
+            }
+        }
       int ngroups = (int)((nt + gs - 1) / gs);
       if((st = c->d_failed.ensure(ngroups)) != AGRAV_OK)
         return st;

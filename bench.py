@@ -43,10 +43,10 @@ def rel_err(a, b):
     return np.linalg.norm(a - b, axis=1) / np.maximum(np.linalg.norm(b, axis=1), 1e-300)
 
 
-def make_params(A, precision, walk, eps, pmgrid=128, pot=False):
+def make_params(A, precision, walk, eps, pmgrid=128, pot=False, group_size=64):
     return A.make_params(periodic=True, box_size=1.0, pmgrid=pmgrid, asmth=1.25, rcut=4.5, G=1.0,
                          err_tol_theta=0.7, err_tol_force_acc=0.0025, opening=1, soft=(eps,),
-                         precision=precision, walk=walk, group_size=64, leaf_bucket=4, eval_potential=pot)
+                         precision=precision, walk=walk, group_size=group_size, leaf_bucket=4, eval_potential=pot)
 
 
 def run_path(A, pos, mass, eps, precision, walk, steps, warmup, old_acc=None, pot=False):
@@ -70,6 +70,37 @@ def run_path(A, pos, mass, eps, precision, walk, steps, warmup, old_acc=None, po
     res = g.get_accelerations()
     g.close()
     return dict(total_s=total, step_ms=step_ms, timings=tim), res
+
+
+def substep_run(A, pos, mass, eps, old_acc, fractions, steps):
+    """Hierarchical sub-steps on the same particle set: for each active fraction, a seeded random active
+    list, then `steps` timed compute_grav_accelerations calls (FP32 group walk) with (a) the partial tree of
+    HIERARCHICAL_GRAVITY (builds insert only the active particles, OldAcc kept) and (b) the tree of every
+    particle with the active list as walk targets.  (b) runs the PM of every particle on each call; (a) keeps
+    the last one while the particles are unchanged, as GravPM between PM steps."""
+    out = {}
+    n = len(pos)
+    for f in fractions:
+        act = np.sort(np.random.Generator(np.random.Philox(97)).choice(n, int(n * f), replace=False)).astype(np.int32)
+        row = {"active": int(len(act))}
+        for mode in ("partial_tree", "full_tree_active_targets"):
+            g = A.Gravity(make_params(A, A.FP32, A.WALK_GROUP, eps, group_size=0))  # group size by target density
+            g.set_particles(pos, mass)
+            g.set_old_acc(old_acc)
+            if mode == "partial_tree":
+                g.set_partial_tree(True, set_oldacc=False)
+            g.set_active(act)
+            g.compute_grav_accelerations(first_step=False)  # warmup
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                g.compute_grav_accelerations(first_step=False)
+            dt = (time.perf_counter() - t0) / steps
+            tim = g.timings()
+            g.close()
+            row[mode] = {"ms_per_step": 1e3 * dt, "treebuild_ms": tim["treebuild"], "treewalk_ms": tim["treewalk"],
+                         "pm_ms": tim["pm_grav"], "interactions_per_active": tim["interactions"] / max(len(act), 1)}
+        out["%g" % f] = row
+    return out
 
 
 def serving_run(A, precision, walk, slots, rate, seconds, seed, max_requests=None):
@@ -389,6 +420,7 @@ def main():
     ap.add_argument("--no-errors", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--copy-probe", action="store_true")
+    ap.add_argument("--substep-steps", type=int, default=5, help="timed calls per active fraction (0: skip)")
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--batch-sets", type=int, default=4096)
@@ -499,6 +531,12 @@ def main():
     rpot, _ = run_path(A, pos, mass, eps, A.FP32, A.WALK_GROUP, 3, 0, old_acc=res32["old_acc"], pot=True)
     out["fp32_group_with_potential"] = {"ms_per_step": 1e3 * rpot["total_s"] / 3,
                                         "walk_ms": rpot["timings"]["treewalk"]}
+
+    if args.substep_steps > 0:
+        log("hierarchical sub-steps: partial tree vs full tree with active targets")
+        out["hierarchical_substeps"] = substep_run(A, pos, mass, eps, res32["old_acc"], (0.5, 0.125, 0.03125),
+                                                   args.substep_steps)
+        log("sub-steps: %s" % json.dumps(out["hierarchical_substeps"]))
 
     res64p = None
     if not args.no_fp64:

@@ -142,6 +142,20 @@ agrav_status agrav_set_old_acc(agrav_ctx *ctx, const float *old_acc);
  * (the default); set_particles resets the list.  Not for batches. */
 agrav_status agrav_set_active(agrav_ctx *ctx, const int32_t *active, int64_t n_active);
 
+/* HIERARCHICAL_GRAVITY's partial tree (forcetree.c:91-95, :166-171; the driver
+ * do_gravity_hydro.c:126-266 calls compute_grav_accelerations(timebin,
+ * FLAG_PARTIAL_TREE) per timebin).  partial = 1: while an active list is set, the
+ * next builds insert only the active particles, in the domain cube of every
+ * particle, so the walk gives the forces among the active particles; the
+ * long-range PM force still covers every particle, and compute_grav_accelerations
+ * reuses the last one while the particles are unchanged (GravPM between PM steps,
+ * do_gravity_hydro.c:105-123).  A list changed after such a
+ * build makes the walk fail (AGRAV_ERR_STATE) until the next build.
+ * set_oldacc = 0: gravity_force_finalize leaves OldAcc as it is, as on every
+ * timebin below the highest occupied one (accel.c:231-232).  Defaults: partial
+ * = 0, set_oldacc = 1.  The setting stays across set_particles. */
+agrav_status agrav_set_partial_tree(agrav_ctx *ctx, int partial, int set_oldacc);
+
 /* construct_forcetree (forcetree.c:110-155): domain cube, Peano-Hilbert keys,
  * sort, octree, bottom-up moments.  numnodes may be NULL. */
 agrav_status agrav_construct_forcetree(agrav_ctx *ctx, int *numnodes);
@@ -191,7 +205,8 @@ typedef struct
 agrav_status agrav_gravity_timesteps(agrav_ctx *ctx, const agrav_timestep_params *tp, double *dt, double *dt_displacement,
                                      int64_t *n_below_min);
 
-/* Raw parts: tree force (G=1 units) and PM force, caller order, host pointers (may be NULL). */
+/* Raw parts: tree force (G=1 units) and PM force, caller order, host pointers (may be NULL).
+ * After a partial tree the tree parts are written for the tree's particles only. */
 agrav_status agrav_get_components(agrav_ctx *ctx, double *acc_tree, double *pot_tree, double *acc_pm, double *pot_pm);
 
 /* gravity_forcetest / gravity_forcetest_evaluate (gravtree_forcetest.c:257-648):
@@ -213,7 +228,8 @@ agrav_status agrav_get_timings(agrav_ctx *ctx, agrav_timings *t);
  * numnodes entries; any may be NULL).  Node 0 is the root; order is pre-order. */
 agrav_status agrav_get_tree(agrav_ctx *ctx, int *numnodes, double *center, double *len, double *com, double *mass, int32_t *level,
                             int32_t *npart);
-/* Particle permutation: sorted position k holds caller index perm[k]; keys[k] its 63-bit Hilbert key. */
+/* Particle permutation: sorted position k holds caller index perm[k]; keys[k] its 63-bit Hilbert key.
+ * One entry per particle of the tree (the active ones after a partial tree). */
 agrav_status agrav_get_order(agrav_ctx *ctx, int32_t *perm, uint64_t *keys);
 
 /* The tables the kernels use: short-range (forcetree_walk.c:72-85), 2 x 128 floats;
