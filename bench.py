@@ -498,6 +498,10 @@ def main():
                      "finalize": tim["finalize"]},
         "numnodes": tim["numnodes"],
         "walk_fallback_groups": tim["walk_retries"],
+        # group-walk work per step: stack entries evaluated, and tile entries each streamed by all 64 lanes
+        # (interactions / (64 x tile entries) is the fraction of tile lane work that is an interaction)
+        "walk_popped": tim["walk_popped"],
+        "tile_entries": tim["tile_entries"],
     }
 
     # roofline of the dominant kernel (the walk): algorithmic bytes = one 16-byte source record per
