@@ -531,6 +531,19 @@ def main():
                                      "l2_hit_rate": prof.get("l2_hit_rate")},
                        "profiled": os.path.relpath(PMC_PROFILE, ROOT) if prof else None}
 
+    # whole-step algorithmic bytes, the per-solve formula of the path's scope table:
+    # I*16 + N*52 + tree (keys+indices, 4 passes read+write; 48 B per node read+write)
+    # + PM (deposit and readout per particle, 24 passes of 4 B over the G^3 mesh)
+    G = 128
+    b_tree = n * 12 * 2 * 4 + tim["numnodes"] * 48 * 2
+    b_pm = n * 32 + n * 40 + G ** 3 * 4 * 24
+    b_step = inter * 16 + n * 52 + b_tree + b_pm
+    out["roofline_step"] = {"bytes_per_step": b_step, "rate_GBps": b_step / (ms_per_step * 1e-3) / 1e9,
+                            "frac_of_hbm_peak": b_step / (ms_per_step * 1e-3) / HBM_PEAK,
+                            "cache_amplified": True,
+                            "note": "I*16 B dominates; node and particle records are reused from L2 and LDS, so this "
+                                    "is not a measured HBM rate (roofline.traffic is)"}
+
     # the same path with EVALPOTENTIAL (potentials summed too)
     rpot, _ = run_path(A, pos, mass, eps, A.FP32, A.WALK_GROUP, 3, 0, old_acc=res32["old_acc"], pot=True)
     out["fp32_group_with_potential"] = {"ms_per_step": 1e3 * rpot["total_s"] / 3,
