@@ -992,6 +992,12 @@ agrav_status agrav_get_tables(agrav_ctx *c, float *tab_force, float *tab_pot, do
   return AGRAV_OK;
 }
 
-uint64_t agrav_hilbert_key21(uint32_t x, uint32_t y, uint32_t z) { return hilbert_key21(x, y, z); }
+uint64_t agrav_hilbert_key21(uint32_t x, uint32_t y, uint32_t z)
+{
+  static const uint8_t tab[192] = AG_HILBERT_TABLE;
+  uint64_t top = 0, low = 0;
+  hilbert_digits(x, y, z, AG_KEY_LEVELS, tab, top, low);
+  return low;
+}
 
 }  // extern "C"

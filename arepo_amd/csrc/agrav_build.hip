@@ -269,9 +269,15 @@ __device__ inline uint64_t int52(double x, const Domain &dom, int d)
 // then zeros.  The octree splits cells down to `levels` (AREPO's insertion tree
 // splits down to MAX_TREE_LEVEL, forcetree.c:861-877); only particles sharing a
 // cell below the first word need the second one.
+__constant__ uint8_t c_hilbert_tab[192] = AG_HILBERT_TABLE;
+
 __global__ void k_keys(const double *__restrict__ pos, const int *__restrict__ src, int64_t n, BuildGeo geo, int periodic,
                        double box, uint64_t *__restrict__ keys, uint64_t *__restrict__ keys_lo, int *__restrict__ idx, Domain *dom_w)
 {
+  __shared__ uint8_t s_tab[192];  // the curve's state table, read once per level per particle
+  if(threadIdx.x < 192)
+    s_tab[threadIdx.x] = c_hilbert_tab[threadIdx.x];
+  __syncthreads();
   const int64_t k = (int64_t)blockIdx.x * B + threadIdx.x;
   if(k >= n)
     return;
@@ -295,10 +301,10 @@ __global__ void k_keys(const double *__restrict__ pos, const int *__restrict__ s
       atomicExch(&dom_w->out_of_box, 1);
       c[0] = c[1] = c[2] = 0;
     }
-  hilbert_transpose(c, L);
-  unsigned __int128 key = (unsigned __int128)(unsigned)set;
-  for(int b = L - 1; b >= 0; b--)
-    key = (key << 3) | (((c[0] >> b) & 1u) << 2) | (((c[1] >> b) & 1u) << 1) | ((c[2] >> b) & 1u);
+  uint64_t top, low;
+  hilbert_digits(c[0], c[1], c[2], L, s_tab, top, low);
+  const int nlow = L < AG_KEY_LEVELS ? L : AG_KEY_LEVELS;
+  unsigned __int128 key = ((unsigned __int128)(unsigned)set << (3 * L)) | ((unsigned __int128)top << (3 * nlow)) | low;
   key <<= (126 - geo.sbits - 3 * L);
   keys[k]    = (uint64_t)(key >> 63);
   keys_lo[k] = (uint64_t)key & ((1ull << 63) - 1);

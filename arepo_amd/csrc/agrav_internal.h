@@ -175,49 +175,53 @@ size_t sort_temp_bytes(int64_t n);
 agrav_status radix_sort_u64(uint64_t *keys_in, int *vals_in, uint64_t *keys_out, int *vals_out, int64_t n, int key_bits,
                             void *d_temp, int *d_err, hipStream_t s);
 
-// 3-D Hilbert index by Skilling's transpose method ("Programming the Hilbert
-// curve", AIP Conf. Proc. 707, 381 (2004)) on b-bit coordinates: undo excess
-// work, then Gray-encode.  Interleaving the transposed coordinates MSB first
-// gives the key; its top 3L bits are the key of the coordinates' top L bits.
-__host__ __device__ inline void hilbert_transpose(uint32_t X[3], int b)
-{
-  const uint32_t M = 1u << (b - 1);
-  for(uint32_t Q = M; Q > 1; Q >>= 1)
-    {
-      uint32_t P = Q - 1;
-      for(int i = 0; i < 3; i++)
-        {
-          if(X[i] & Q)
-            X[0] ^= P;
-          else
-            {
-              uint32_t t = (X[0] ^ X[i]) & P;
-              X[0] ^= t;
-              X[i] ^= t;
-            }
-        }
-    }
-  X[1] ^= X[0];
-  X[2] ^= X[1];
-  uint32_t t = 0;
-  for(uint32_t Q = M; Q > 1; Q >>= 1)
-    if(X[2] & Q)
-      t ^= Q - 1;
-  X[0] ^= t;
-  X[1] ^= t;
-  X[2] ^= t;
-}
+// 3-D Hilbert index of Skilling's transpose method ("Programming the Hilbert
+// curve", AIP Conf. Proc. 707, 381 (2004)) on b-bit coordinates, evaluated as a
+// state machine: each level's key digit depends on that level's octant
+// (x bit << 2 | y bit << 1 | z bit) and on an orientation state carried down from
+// the levels above.  Entry [8 s + o] = (next state << 3) | digit, start state 0;
+// the 24 states and their transitions were derived from the transpose method
+// (arepo_amd/gen_hilbert_table.py, which also checks the table against it for
+// b = 1..30).  A key's top 3L bits are the key of the coordinates' top L bits.
+#define AG_HILBERT_TABLE                                                                                          \
+  {                                                                                                              \
+      8,  17,  27,   2,  39,  46,  52,   5,  56,  71,  73,  86,  91,  20,  10,  13, \
+     48,   1, 103, 110, 115,  18,  12,  21, 126, 129,  29,  26,  79,  80, 140,   3, \
+    148,  43,  37,  34, 127, 128,  78,  81, 156,  45,  35,  42,  31,   6, 160, 105, \
+     72,  87, 139,   4,  57,  70,  50,  53,   0, 171, 111,  76,  49,  58, 102,  61, \
+    180, 143,  83, 184,  69,  54,  66,  97,  16, 123,   9,  74,  47,  60,  38,  77, \
+    132,  95,  85,  14,  67, 144,  82,  33, 142,  55, 185,  96,  93, 116,  90,  11, \
+    188, 107, 175, 176, 101,  98,  62,  65, 164, 109, 119,  22,  99, 106, 152,  41, \
+    174, 177,  63,  64, 117, 114,  92,  19,  30, 125, 161, 122,   7, 172, 104,  75, \
+    130,  25, 133, 166, 179, 136,  84, 191,  94,  15, 141,  28, 145,  32, 138,  51, \
+    146, 155, 149,  36, 137,  24, 190, 167, 154, 157, 147,  44, 169, 182, 120, 135, \
+    162, 165, 121, 134, 187, 108, 168, 183, 118, 173,  23, 124, 153, 170,  40,  59, \
+    178, 113, 131,  88, 181, 158,  68, 151, 186, 163,  89, 112, 189, 100, 150, 159 \
+  }
 
-__host__ __device__ inline uint64_t hilbert_key21(uint32_t x, uint32_t y, uint32_t z)
+// digits of the L-level key of L-bit coordinates: the last min(L, 21) levels in
+// `low`, the levels above them in `top` (key = top << 3 min(L, 21) | low)
+__host__ __device__ inline void hilbert_digits(uint32_t x, uint32_t y, uint32_t z, int L, const uint8_t *tab, uint64_t &top,
+                                               uint64_t &low)
 {
-  uint32_t X[3] = {x, y, z};
-  hilbert_transpose(X, AG_KEY_LEVELS);
-  uint64_t key = 0;
-  for(int b = AG_KEY_LEVELS - 1; b >= 0; b--)
+  unsigned s = 0;
+  uint64_t a = 0, c = 0;
+  const int lo_levels = L < AG_KEY_LEVELS ? L : AG_KEY_LEVELS;
+  int l = L - 1;
+  for(; l >= lo_levels; l--)
     {
-      key = (key << 3) | ((uint64_t)((X[0] >> b) & 1) << 2) | ((uint64_t)((X[1] >> b) & 1) << 1) | (uint64_t)((X[2] >> b) & 1);
+      const unsigned e = tab[8 * s + ((((x >> l) & 1u) << 2) | (((y >> l) & 1u) << 1) | ((z >> l) & 1u))];
+      a = (a << 3) | (e & 7u);
+      s = e >> 3;
     }
-  return key;
+  for(; l >= 0; l--)
+    {
+      const unsigned e = tab[8 * s + ((((x >> l) & 1u) << 2) | (((y >> l) & 1u) << 1) | ((z >> l) & 1u))];
+      c = (c << 3) | (e & 7u);
+      s = e >> 3;
+    }
+  top = a;
+  low = c;
 }
 
 }  // namespace agrav

@@ -97,6 +97,22 @@ def test_hilbert_key_prefix_property(agrav):
         assert ka == kb
 
 
+def test_hilbert_table_equals_transpose_method(agrav):
+    # the library evaluates the curve with a 24-state table; Skilling's transpose method, written out
+    # independently in the table's generator, gives the same key for every point
+    from arepo_amd.gen_hilbert_table import derive, table_key, transpose_key
+    g = np.random.Generator(np.random.Philox(3))
+    pts = g.integers(0, 1 << 21, (2000, 3))
+    pts[:8] = [[0, 0, 0], [(1 << 21) - 1] * 3, [1, 0, 0], [0, 1, 0], [0, 0, 1], [1 << 20, 0, 0], [0, 1 << 20, 0],
+               [0, 0, 1 << 20]]
+    for x, y, z in pts:
+        assert agrav.hilbert_key21(x, y, z) == transpose_key(int(x), int(y), int(z), 21)
+    # the generator's own table, for the deeper levels of the 30-level keys
+    tab = derive(depth=3)
+    for x, y, z in g.integers(0, 1 << 30, (300, 3)):
+        assert table_key(tab, int(x), int(y), int(z), 30) == transpose_key(int(x), int(y), int(z), 30)
+
+
 def test_hilbert_key_fits_63_bits(agrav):
     m = (1 << 21) - 1
     for c in [(0, 0, 0), (m, m, m), (m, 0, 0), (0, m, 0), (0, 0, m)]:
