@@ -591,8 +591,7 @@ agrav_status agrav_construct_forcetree(agrav_ctx *c, int *numnodes)
       c->have_tree = false;
       return st;
     }
-  AG_HIP(hipEventRecord(c->ev[1], c->stream));
-  AG_HIP(hipEventSynchronize(c->ev[1]));
+  AG_HIP(hipEventSynchronize(c->ev[1]));  // recorded by the build (already passed when it read back)
   float ms = 0;
   AG_HIP(hipEventElapsedTime(&ms, c->ev[7], c->ev[1]));
   c->tim.treebuild = ms;

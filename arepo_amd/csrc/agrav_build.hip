@@ -1238,6 +1238,7 @@ agrav_status build_tree(agrav_ctx *c)
     {
       c->m         = 0;
       c->have_tree = true;
+      AG_HIP(hipEventRecord(c->ev[1], s));  // end of the build (timed by the caller)
       return AGRAV_OK;
     }
   if(n >= (1ll << 30))
@@ -1465,6 +1466,9 @@ agrav_status build_tree(agrav_ctx *c)
                                          c->d_maxsoft.p, c->d_recA.p, c->d_recB.p, geo.dom, geo.pset);
   AG_HIP(hipGetLastError());
 
+  // end of the build, recorded before the read-back so that timing it needs no second
+  // round trip to the device
+  AG_HIP(hipEventRecord(c->ev[1], s));
   // out-of-box check (forcetree.c:432-438 aborts; here it is an error code)
   int cnow[64];
   AG_HIP(hipMemcpyAsync(cnow, c->d_counters.p, sizeof(cnow), hipMemcpyDeviceToHost, s));

@@ -1535,6 +1535,7 @@ agrav_status walk_tree(agrav_ctx *c, int use_bh)
         dispatch_group<double>(pm, ew, ms, ngroups, s, tv, wc, c, gs, 1);
       AG_HIP(hipGetLastError());
       AG_HIP(hipEventRecord(c->ev[3], s));
+      AG_HIP(hipEventRecord(c->ev[4], s));  // the walk's end, unless groups are re-walked below
       // groups whose LDS stack overflowed (their count is isum[3]) are re-walked one
       // lane per target
       AG_HIP(hipMemcpyAsync(cnt, c->d_isum.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
@@ -1551,10 +1552,13 @@ agrav_status walk_tree(agrav_ctx *c, int use_bh)
       c->final_fused = c->fin.perm != nullptr && nf == 0;
       c->fin.perm    = nullptr;
     }
-  AG_HIP(hipEventRecord(c->ev[4], s));
   if(!have_cnt)
-    AG_HIP(hipMemcpyAsync(cnt, c->d_isum.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
-  AG_HIP(hipStreamSynchronize(s));
+    {
+      // other walks, or a group walk with re-walked groups: one more read-back
+      AG_HIP(hipEventRecord(c->ev[4], s));
+      AG_HIP(hipMemcpyAsync(cnt, c->d_isum.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
+      AG_HIP(hipStreamSynchronize(s));
+    }
   const unsigned long long tot = cnt[0];
   c->tim.walk_popped           = (long long)cnt[1];
   c->tim.tile_entries          = (long long)cnt[2];
