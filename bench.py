@@ -403,8 +403,15 @@ def cpu_baseline(pos, mass, eps, old_acc, sel, threads):
     t3 = time.perf_counter()
     t_walk_full = (t3 - t2) * n / sample
     t_total = (t1 - t0) + (t2 - t1) + t_walk_full
+    # one core: the PM and the build are serial already; the walk of every 16th sample target on one thread
+    sub = sel[::16]
+    t4 = time.perf_counter()
+    tree.walk(False, old_acc=old_acc, targets=sub, nthreads=1)
+    t5 = time.perf_counter()
+    t_total_1 = (t1 - t0) + (t2 - t1) + (t5 - t4) * n / len(sub)
     return dict(value=n / t_total, t_pm=t1 - t0, t_build=t2 - t1, t_walk_sample=t3 - t2, sample=sample,
-                ia_per_part=tot / sample, sel=sel, acc_tree=acc, pot_tree=pot, acc_pm=acc_pm, params=p)
+                ia_per_part=tot / sample, sel=sel, acc_tree=acc, pot_tree=pot, acc_pm=acc_pm, params=p,
+                value_1core=n / t_total_1, sample_1core=len(sub), t_walk_sample_1core=t5 - t4)
 
 
 def main():
@@ -579,6 +586,10 @@ def main():
                                          "targets scaled to N (build %.2f s, pm %.2f s, walk sample %.2f s)"
                                          % (cpu["sample"], cpu["t_build"], cpu["t_pm"], cpu["t_walk_sample"])}
         out["speedup_vs_cpu"] = value / cpu["value"]
+        out["cpu_baseline_1core"] = {"value": cpu["value_1core"], "unit": "particles/s", "cores": 1, "kind": "port",
+                                     "sample": "the same build and PM; one-thread walk of %d of the sample targets "
+                                               "(%.2f s) scaled to N" % (cpu["sample_1core"],
+                                                                        cpu["t_walk_sample_1core"])}
 
     if not args.no_errors:
         log("FP64 direct summation (Ewald) on %d targets" % len(sel))
