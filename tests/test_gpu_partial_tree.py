@@ -3,7 +3,8 @@ particles of the active list (forcetree.c:91-95, :166-171) in the domain cube of
 finalize may keep OldAcc (accel.c:231-232).
 
 * The tree of a sorted active subset is the tree of that subset alone whenever the domain cube is the same
-  (a periodic box; or an isolated set whose subset holds the extreme particles of every axis): the walk's
+  (a periodic box, TreePM or tree-only with Ewald, with one or two softening types; or an isolated set whose
+  subset holds the extreme particles of every axis): the walk's
   tree forces, potentials and interaction counts must equal those of a context holding only the subset,
   bit for bit, for the FP64 per-target walk and the FP32 group walk.  The FP64 walk also equals the
   oracle's walk of the subset's own tree.
@@ -30,6 +31,11 @@ def _oldacc(n, seed):
 
 
 def _case(name):
+    if name == "ewald":
+        # periodic tree-only: Ewald correction per interaction, domain = the box
+        pos, mass, eps = ics.clustered_box(16384, nhalo=8, seed=14)
+        kw = dict(periodic=True, box_size=1.0, pmgrid=0, soft=(eps,), err_tol_theta=0.7, err_tol_force_acc=0.0025)
+        return pos, mass, kw, _sel(len(pos), 3000, 15)
     if name == "treepm":
         pos, mass, eps = ics.clustered_box(32768, nhalo=16, seed=8)
         kw = dict(periodic=True, box_size=1.0, pmgrid=32, soft=(eps,), err_tol_theta=0.7, err_tol_force_acc=0.0025)
@@ -47,13 +53,18 @@ WALKS = [("plain_fp64", "WALK_PLAIN", "FP64"), ("group_fp32", "WALK_GROUP", "FP3
 
 
 @pytest.mark.parametrize("walk", WALKS, ids=[w[0] for w in WALKS])
-@pytest.mark.parametrize("name", ["treepm", "isolated"])
+@pytest.mark.parametrize("name", ["treepm", "isolated", "ewald", "treepm_2soft"])
 def test_partial_tree_equals_tree_of_the_subset(agrav, name, walk):
-    pos, mass, kw, sel = _case(name)
+    pos, mass, kw, sel = _case(name.replace("_2soft", ""))
+    stype = None
+    if name.endswith("_2soft"):
+        # two softening types (forcetree_walk.c:294-320 opens nodes with a larger softening nearby)
+        kw = dict(kw, soft=(kw["soft"][0], 3 * kw["soft"][0]))
+        stype = (np.random.Generator(np.random.Philox(16)).random(len(pos)) < 0.3).astype(np.uint8)
     oa = _oldacc(len(pos), 12)
     p = agrav.make_params(**kw, opening=1, precision=getattr(agrav, walk[2]), walk=getattr(agrav, walk[1]))
     with agrav.Gravity(p) as g:
-        g.set_particles(pos, mass)
+        g.set_particles(pos, mass, softtype=stype)
         g.set_old_acc(oa)
         g.set_partial_tree(True)
         g.set_active(sel)
@@ -62,7 +73,7 @@ def test_partial_tree_equals_tree_of_the_subset(agrav, name, walk):
         out = g.get_accelerations()
         perm, _ = g.get_order()
     with agrav.Gravity(p) as g:
-        g.set_particles(pos[sel], mass[sel])
+        g.set_particles(pos[sel], mass[sel], softtype=None if stype is None else stype[sel])
         g.set_old_acc(oa[sel])
         g.compute_grav_accelerations(first_step=False)
         alone = g.get_components(tree=True)
