@@ -442,7 +442,7 @@ def main():
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU only: the gravity step of one particle set runs on a single MI355X "
-              "(multi-GPU sharding is described in DESIGN.md, not run); use --gpus 1", file=sys.stderr)
+              "(the path is not sharded across GPUs here); use --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     from arepo_amd import agrav as A

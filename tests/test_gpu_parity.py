@@ -1,7 +1,7 @@
 """GPU parity tests: the HIP path (through the C ABI) against the CPU oracle.
 
 Bars: integer/topology data bit-exact; FP64 paths within 1e-9 relative of
-the oracle (summation order differs); FP32 paths judged by their error
+the oracle (summation order differs); FP32 paths assessed by their error
 against FP64 direct summation, which must not exceed the oracle's own
 error by more than the stated margin.
 """
