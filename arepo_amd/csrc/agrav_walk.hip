@@ -372,44 +372,90 @@ template <typename T> __device__ inline T nearest_image(T d, T box, T half)
   return d > half ? d - box : (d < -half ? d + box : d);
 }
 
-template <typename T, bool PM, bool EWALD, bool MSOFT>
-__device__ inline void tile_compute_generic(const Vec4<T> *__restrict__ s_list, const T *__restrict__ s_h, int nl, T tx, T ty, T tz, T hi,
-                                    const float *__restrict__ s_tab, const WalkConst &wc, T asmthfac, T asmthinv2,
-                                    const float *__restrict__ ewald_f, const double *__restrict__ ewald_d, T &ax, T &ay, T &az, T &pp,
-                                    T &cax, T &cay, T &caz, T &cpp, int &nin)
+// FP64 tile loop (FP64 group walks): 1/r from the hardware reciprocal square root
+// (about 2^-23 relative) refined by two Newton steps, which reach FP64 rounding,
+// instead of a square root and a division per pair; the pair kernel is otherwise
+// pair_kernel's (forcetree_walk.c:364-487).  The tiny addend keeps 1/r finite for
+// the target itself (r2 = 0), which then takes the softened branch with r = 0.
+template <bool PM, bool EWALD, bool MSOFT>
+__device__ inline void tile_compute_f64(const Vec4<double> *__restrict__ s_list, const double *__restrict__ s_h, int nl_in,
+                                        double tx, double ty, double tz, double hi, const float *__restrict__ s_tab,
+                                        const WalkConst &wc, double asmthfac, double asmthinv2, const double *__restrict__ ewald_d,
+                                        double &ax, double &ay, double &az, double &pp, double &cax, double &cay, double &caz,
+                                        double &cpp, int &nin)
 {
-  T fx = 0, fy = 0, fz = 0, fp = 0;
-  const T box = (T)wc.box, half = (T)wc.half_box;
+  double fx = 0, fy = 0, fz = 0, fp = 0;
+  const int nl       = __builtin_amdgcn_readfirstlane(nl_in);
+  const double box = wc.box, half = wc.half_box;
   for(int k = 0; k < nl; k++)
     {
-      Vec4<T> e = s_list[k];
-      T dx = e.x - tx, dy = e.y - ty, dz = e.z - tz;
+      const Vec4<double> e = s_list[k];
+      double dx = e.x - tx, dy = e.y - ty, dz = e.z - tz;
       if(EWALD)
         {
           dx = nearest_image(dx, box, half);
           dy = nearest_image(dy, box, half);
           dz = nearest_image(dz, box, half);
         }
-      T r2   = dx * dx + dy * dy + dz * dz;
-      T hmax = hi;
+      const double r2  = dx * dx + dy * dy + dz * dz;
+      const double r2s = r2 + 1e-200;
+      double y         = __builtin_amdgcn_rsq(r2s);
+#pragma unroll
+      for(int it = 0; it < 2; it++)
+        y = fma(0.5 * y, fma(-r2s * y, y, 1.0), y);
+      const double r = r2 * y;
+      double hmax    = hi;
       if(MSOFT)
         hmax = s_h[k] > hi ? s_h[k] : hi;
-      T r = sqrt(r2);
-      T fac, wp;
-      if(e.w == (T)0 || !pair_kernel<T, PM>(r, hmax, s_tab, asmthfac, asmthinv2, fac, wp))
+      if(e.w == 0.0)
         continue;
+      double ff = 0, fpt = 0;
+      if(PM)
+        {
+          const double te = asmthfac * r;
+          const int ti    = (int)te;
+          if(ti >= AG_NTAB)
+            continue;
+          const double w = te - (double)ti;
+          ff             = (1.0 - w) * (double)s_tab[ti] + w * (double)s_tab[ti + 1];
+          fpt            = (1.0 - w) * (double)s_tab[128 + ti] + w * (double)s_tab[128 + ti + 1];
+        }
+      double fac, wp;
+      if(r >= hmax)
+        {
+          const double rinv3 = y * y * y;
+          fac                = PM ? rinv3 + y * ff * asmthinv2 : rinv3;
+          wp                 = PM ? y * fpt : -y;
+        }
+      else
+        {
+          // spline softening (allvars.h:557-573), rare: exact divisions
+          const double h_inv = 1.0 / hmax, h3_inv = h_inv * h_inv * h_inv;
+          const double u = r * h_inv, u2 = u * u;
+          if(u < 0.5)
+            {
+              fac = h3_inv * (SF1 + u2 * (SF2 * u + SF3));
+              wp  = h_inv * (SF4 + u2 * (SF5 + u2 * (SF6 * u + SF7)));
+            }
+          else
+            {
+              const double u3 = u2 * u;
+              fac = h3_inv * (SF8 + SF9 * u + SF10 * u2 + SF11 * u3 + SF12 / u3);
+              wp  = h_inv * (SF13 + SF14 / u + u2 * (SF1 + u * (SF15 + u * (SF16 + SF17 * u))));
+            }
+          if(PM && r > 0.0)
+            {
+              fac += y * ff * asmthinv2;
+              wp += y * (fpt + 1.0);
+            }
+        }
       fp += e.w * wp;
       fac *= e.w;
       fx += dx * fac;
       fy += dy * fac;
       fz += dz * fac;
       if(EWALD)
-        {
-          if constexpr(sizeof(T) == 4)
-            ewald_add<T, float>(ewald_f, (T)wc.ewald_fac, dx, dy, dz, e.w, fx, fy, fz, fp);
-          else
-            ewald_add<T, double>(ewald_d, (T)wc.ewald_fac, dx, dy, dz, e.w, fx, fy, fz, fp);
-        }
+        ewald_add<double, double>(ewald_d, wc.ewald_fac, dx, dy, dz, e.w, fx, fy, fz, fp);
       nin++;
     }
   // tile sums join the running totals through compensated additions
@@ -551,8 +597,8 @@ __device__ inline void tile_compute(const Vec4<T> *__restrict__ s_list, const T 
                                                   ay, az, pp, cax, cay, caz, cpp, nin);
     }
   else
-    tile_compute_generic<T, PM, EWALD, MSOFT>(s_list, s_h, nl, tx, ty, tz, hi, s_tab, wc, asmthfac, asmthinv2, ewald_f, ewald_d,
-                                              ax, ay, az, pp, cax, cay, caz, cpp, nin);
+    tile_compute_f64<PM, EWALD, MSOFT>(s_list, s_h, nl, tx, ty, tz, hi, s_tab, wc, asmthfac, asmthinv2, ewald_d, ax, ay, az, pp, cax,
+                                       cay, caz, cpp, nin);
 }
 
 // LDS of one group walk (one wavefront per block)
