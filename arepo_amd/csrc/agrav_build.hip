@@ -307,7 +307,7 @@ __global__ void k_keys(const double *__restrict__ pos, const int *__restrict__ s
   unsigned __int128 key = ((unsigned __int128)(unsigned)set << (3 * L)) | ((unsigned __int128)top << (3 * nlow)) | low;
   key <<= (126 - geo.sbits - 3 * L);
   keys[k]    = (uint64_t)(key >> 63);
-  keys_lo[k] = (uint64_t)key & ((1ull << 63) - 1);
+  keys_lo[i] = (uint64_t)key & ((1ull << 63) - 1);  // by caller index: the tie-run sorts read keys_lo[idx[.]]
   idx[k]     = (int)i;
 }
 
@@ -1301,7 +1301,7 @@ agrav_status build_tree(agrav_ctx *c)
 
   // 2. keys and sort
   if((st = c->d_key.ensure(n)) || (st = c->d_key2.ensure(n)) || (st = c->d_idx.ensure(n)) || (st = c->d_idx2.ensure(n)) ||
-     (st = c->d_key_lo.ensure(n)) || (st = c->d_key_lo_s.ensure(n)) || (st = c->d_ties.ensure(2 * (n / 2 + 1))))
+     (st = c->d_key_lo.ensure(nall)) || (st = c->d_key_lo_s.ensure(n)) || (st = c->d_ties.ensure(2 * (n / 2 + 1))))
     return st;
   k_keys<<<nblk(n), B, 0, s>>>(c->d_pos_in.p, part ? c->d_active.p : nullptr, n, geo, c->par.periodic, c->par.box_size,
                                c->d_key.p, c->d_key_lo.p, c->d_idx.p, c->d_dom.p);

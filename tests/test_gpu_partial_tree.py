@@ -187,3 +187,44 @@ def test_partial_tree_lists_are_checked(agrav):
         g.compute_grav_accelerations(first_step=True)
         r = g.get_accelerations()
         assert np.isfinite(r["acc"][sel[:2000]]).all() and (r["nint"][sel[:2000]] > 0).all()
+
+
+@pytest.mark.parametrize("walk", WALKS, ids=[w[0] for w in WALKS])
+def test_partial_tree_with_deep_tie_runs(agrav, walk):
+    # uniform particles in the unit periodic box plus clumps inside single level-21 cells (first key words
+    # equal, order from the second words): the active half of the clumps forms tie runs of ~1200 (global
+    # bitonic sort), ~150 (LDS) and 2 (per-thread); the partial build is the context's first build
+    g = np.random.Generator(np.random.Philox(31))
+    uni = g.random((6000, 3))
+    ln = 1.00001
+    corner = 0.5 - 0.5 * ln
+    cell = ln / 2 ** 21
+    parts = [uni]
+    for j, (nc, off) in enumerate(((2400, (2 ** 20, 2 ** 20 - 7, 2 ** 20 + 5)), (300, (2 ** 19 + 3, 2 ** 18, 2 ** 20 - 1)))):
+        origin = corner + np.array(off) * cell
+        parts.append(origin + (0.4 + 0.2 * g.random((nc, 3))) * cell)
+    pos = np.concatenate(parts)
+    pos[10:20] = pos[20:30]  # coincident pairs
+    mass = g.uniform(0.5, 1.5, len(pos)) / len(pos)
+    sel = np.arange(0, len(pos), 2, dtype=np.int32)
+    kw = dict(periodic=True, box_size=1.0, pmgrid=0, soft=(1e-9,), err_tol_theta=0.7, err_tol_force_acc=0.0025)
+    p = agrav.make_params(**kw, opening=0, precision=getattr(agrav, walk[2]), walk=getattr(agrav, walk[1]))
+    with agrav.Gravity(p) as gr:
+        gr.set_particles(pos, mass)
+        gr.set_partial_tree(True)
+        gr.set_active(sel)
+        gr.compute_grav_accelerations(first_step=True)
+        part = gr.get_components(tree=True)
+        perm, keys = gr.get_order()
+        nodes = gr.get_tree()
+    with agrav.Gravity(p) as gr:
+        gr.set_particles(pos[sel], mass[sel])
+        gr.compute_grav_accelerations(first_step=True)
+        alone = gr.get_components(tree=True)
+        perm_alone, keys_alone = gr.get_order()
+        nodes_alone = gr.get_tree()
+    assert nodes["level"].max() > 21  # cells split below the first key word
+    assert np.array_equal(keys, keys_alone) and np.array_equal(perm, sel[perm_alone])
+    assert np.array_equal(nodes["len"], nodes_alone["len"]) and np.array_equal(nodes["npart"], nodes_alone["npart"])
+    assert np.array_equal(part["acc_tree"][sel], alone["acc_tree"])
+    assert np.array_equal(part["pot_tree"][sel], alone["pot_tree"])
