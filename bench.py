@@ -428,7 +428,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--substep-steps", type=int, default=5, help="timed calls per active fraction (0: skip)")
-    ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--serve-seconds", type=float, default=60.0, help="length of the Poisson request stream (the serving workload: 60 s)")
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--batch-sets", type=int, default=4096)
     ap.add_argument("--batch-steps", type=int, default=2)
