@@ -311,12 +311,38 @@ __global__ void k_keys(const double *__restrict__ pos, const int *__restrict__ s
   idx[k]     = (int)i;
 }
 
-// runs of equal first key words in the sorted order (particles sharing a cell at
-// the first word's depth): the start of each run, one atomic per wavefront
-__global__ void k_tie_runs(const uint64_t *__restrict__ keys, int n, int *runs, int *nruns)
+// Tie runs.  The radix sort orders the first key words from bit tshift up (tshift = 0:
+// all of them; tshift = 15 for one set, whose second words have their low 36 bits
+// zero).  A run of keys equal in those bits is then ordered by a composite secondary
+// key: the first word's low tshift bits on top of the second word shifted down by
+// tshift, which loses nothing; after the run is sorted both words are written back.
+__device__ inline uint64_t tie_sec(uint64_t k1, uint64_t klo, int tshift)
+{
+  return tshift ? (((k1 & ((1ull << tshift) - 1)) << (63 - tshift)) | (klo >> tshift)) : klo;
+}
+
+// the first word (from the run's shared high bits) and the second word of a composite key
+__device__ inline void tie_unpack(uint64_t sec, uint64_t k1_run, int tshift, uint64_t &k1, uint64_t &klo)
+{
+  if(tshift)
+    {
+      k1  = (k1_run & ~((1ull << tshift) - 1)) | (sec >> (63 - tshift));
+      klo = (sec << tshift) & ((1ull << 63) - 1);
+    }
+  else
+    {
+      k1  = k1_run;
+      klo = sec;
+    }
+}
+
+// runs of first key words equal from bit tshift up in the sorted order (particles
+// sharing a cell at that depth): the start of each run, one atomic per wavefront
+__global__ void k_tie_runs(const uint64_t *__restrict__ keys, int n, int tshift, int *runs, int *nruns)
 {
   const int k     = blockIdx.x * B + threadIdx.x;
-  const bool head = k + 1 < n && keys[k] == keys[k + 1] && (k == 0 || keys[k - 1] != keys[k]);
+  const bool head = k + 1 < n && (keys[k] >> tshift) == (keys[k + 1] >> tshift) &&
+                    (k == 0 || (keys[k - 1] >> tshift) != (keys[k] >> tshift));
   const uint64_t m = __ballot(head);
   if(!m)
     return;
@@ -335,16 +361,17 @@ __device__ inline bool tie_less(uint64_t ka, int va, uint64_t kb, int vb);
 // each: the run's second key words, then an insertion sort by (second word,
 // caller index) in place.  Runs longer than kShortRun go to the workgroup sort.
 static constexpr int kShortRun = 32;
-__global__ void k_tie_sort_short(const uint64_t *__restrict__ keys, const uint64_t *__restrict__ keys_lo, int n,
+__global__ void k_tie_sort_short(uint64_t *keys, const uint64_t *__restrict__ keys_lo, int n, int tshift,
                                  const int *__restrict__ runs, const int *__restrict__ nruns, int *idx, uint64_t *lo_s,
                                  int *long_runs, int *nlong)
 {
   const int nr = *nruns;
   for(int r = blockIdx.x * B + threadIdx.x; r < nr; r += gridDim.x * B)
     {
-      const int k0 = runs[r];
-      int e        = k0 + 1;
-      while(e < n && e - k0 <= kShortRun && keys[e] == keys[k0])
+      const int k0      = runs[r];
+      const uint64_t kr = keys[k0];
+      int e             = k0 + 1;
+      while(e < n && e - k0 <= kShortRun && (keys[e] >> tshift) == (kr >> tshift))
         e++;
       if(e - k0 > kShortRun)
         {
@@ -352,7 +379,7 @@ __global__ void k_tie_sort_short(const uint64_t *__restrict__ keys, const uint64
           continue;
         }
       for(int t = k0; t < e; t++)
-        lo_s[t] = keys_lo[idx[t]];
+        lo_s[t] = tie_sec(keys[t], keys_lo[idx[t]], tshift);
       for(int t = k0 + 1; t < e; t++)
         {
           const uint64_t kk = lo_s[t];
@@ -367,6 +394,14 @@ __global__ void k_tie_sort_short(const uint64_t *__restrict__ keys, const uint64
           lo_s[j + 1] = kk;
           idx[j + 1]  = vv;
         }
+      if(tshift)
+        for(int t = k0; t < e; t++)
+          {
+            uint64_t k1, klo;
+            tie_unpack(lo_s[t], kr, tshift, k1, klo);
+            keys[t] = k1;
+            lo_s[t] = klo;
+          }
     }
 }
 
@@ -377,7 +412,7 @@ __device__ inline bool tie_less(uint64_t ka, int va, uint64_t kb, int vb) { retu
 // whose every compare-exchange puts the smaller pair first, so that the padding
 // to a power of two (virtual +inf entries) never moves; in LDS up to 1024
 // entries, in place in global memory beyond.  Also the sorted second words.
-__global__ __launch_bounds__(256) void k_tie_sort(const uint64_t *__restrict__ keys, const uint64_t *__restrict__ keys_lo, int n,
+__global__ __launch_bounds__(256) void k_tie_sort(uint64_t *keys, const uint64_t *__restrict__ keys_lo, int n, int tshift,
                                                   const int *__restrict__ runs, const int *__restrict__ nruns, int *idx,
                                                   uint64_t *lo_s)
 {
@@ -387,11 +422,12 @@ __global__ __launch_bounds__(256) void k_tie_sort(const uint64_t *__restrict__ k
   const int nr = *nruns;
   for(int r = blockIdx.x; r < nr; r += gridDim.x)
     {
-      const int k0 = runs[r];
+      const int k0      = runs[r];
+      const uint64_t kr = keys[k0];
       if(threadIdx.x == 0)
         {
           int e = k0 + 1;
-          while(e < n && keys[e] == keys[k0])
+          while(e < n && (keys[e] >> tshift) == (kr >> tshift))
             e++;
           s_end = e;
         }
@@ -405,7 +441,7 @@ __global__ __launch_bounds__(256) void k_tie_sort(const uint64_t *__restrict__ k
       int *gv        = idx + k0;
       for(int t = threadIdx.x; t < P; t += 256)
         {
-          const uint64_t kk = t < len ? keys_lo[idx[k0 + t]] : ~0ull;
+          const uint64_t kk = t < len ? tie_sec(keys[k0 + t], keys_lo[idx[k0 + t]], tshift) : ~0ull;
           if(lds)
             {
               s_k[t] = kk;
@@ -455,6 +491,15 @@ __global__ __launch_bounds__(256) void k_tie_sort(const uint64_t *__restrict__ k
           {
             gk[t] = s_k[t];
             gv[t] = s_v[t];
+          }
+      __syncthreads();
+      if(tshift)
+        for(int t = threadIdx.x; t < len; t += 256)
+          {
+            uint64_t k1, klo;
+            tie_unpack(gk[t], kr, tshift, k1, klo);
+            keys[k0 + t] = k1;
+            gk[t]        = klo;
           }
       __syncthreads();
     }
@@ -1306,7 +1351,10 @@ agrav_status build_tree(agrav_ctx *c)
   k_keys<<<nblk(n), B, 0, s>>>(c->d_pos_in.p, part ? c->d_active.p : nullptr, n, geo, c->par.periodic, c->par.box_size,
                                c->d_key.p, c->d_key_lo.p, c->d_idx.p, c->d_dom.p);
   AG_HIP(hipGetLastError());
-  if((st = radix_sort_u64(c->d_key.p, c->d_idx.p, c->d_key2.p, c->d_idx2.p, n, 63, c->d_temp.p, d_perr, s)) != AGRAV_OK)
+  // one set: the first words are sorted from bit 15 up (6 passes instead of 8); runs
+  // sharing a level-16 cell are finished by the tie-run sorts
+  const int tshift = nsets == 0 ? 15 : 0;
+  if((st = radix_sort_u64(c->d_key.p, c->d_idx.p, c->d_key2.p, c->d_idx2.p, n, 63, c->d_temp.p, d_perr, s, tshift)) != AGRAV_OK)
     return st;
   // sorted first key words and perm are now in d_key2/d_idx2; runs of equal first
   // words are put in the order of their second words
@@ -1314,10 +1362,10 @@ agrav_status build_tree(agrav_ctx *c)
   if(n > 1)
     {
       int *d_nlong = c->d_counters.p + 61, *d_long = c->d_ties.p + (n / 2 + 1);
-      k_tie_runs<<<nblk(n), B, 0, s>>>(c->d_key2.p, (int)n, c->d_ties.p, d_nties);
-      k_tie_sort_short<<<512, B, 0, s>>>(c->d_key2.p, c->d_key_lo.p, (int)n, c->d_ties.p, d_nties, c->d_idx2.p,
+      k_tie_runs<<<nblk(n), B, 0, s>>>(c->d_key2.p, (int)n, tshift, c->d_ties.p, d_nties);
+      k_tie_sort_short<<<512, B, 0, s>>>(c->d_key2.p, c->d_key_lo.p, (int)n, tshift, c->d_ties.p, d_nties, c->d_idx2.p,
                                          c->d_key_lo_s.p, d_long, d_nlong);
-      k_tie_sort<<<256, 256, 0, s>>>(c->d_key2.p, c->d_key_lo.p, (int)n, d_long, d_nlong, c->d_idx2.p, c->d_key_lo_s.p);
+      k_tie_sort<<<256, 256, 0, s>>>(c->d_key2.p, c->d_key_lo.p, (int)n, tshift, d_long, d_nlong, c->d_idx2.p, c->d_key_lo_s.p);
       AG_HIP(hipGetLastError());
     }
 

@@ -170,10 +170,11 @@ void set_error_msg(const std::string &m);
 size_t scan_temp_bytes(int64_t n);
 agrav_status exclusive_scan_i32(const int *d_in, int *d_out, int64_t n, int *d_total, void *d_temp, int *d_err, hipStream_t s);
 size_t sort_temp_bytes(int64_t n);
-// LSD radix sort of 64-bit keys with 32-bit values; bits [0, key_bits) are sorted.
-// Results end up in keys_out/vals_out.
+// LSD radix sort of 64-bit keys with 32-bit values, stable; bits [lo_bit, key_bits)
+// are sorted (keys equal in those bits keep their input order).  Results end up in
+// keys_out/vals_out.
 agrav_status radix_sort_u64(uint64_t *keys_in, int *vals_in, uint64_t *keys_out, int *vals_out, int64_t n, int key_bits,
-                            void *d_temp, int *d_err, hipStream_t s);
+                            void *d_temp, int *d_err, hipStream_t s, int lo_bit = 0);
 
 // 3-D Hilbert index of Skilling's transpose method ("Programming the Hilbert
 // curve", AIP Conf. Proc. 707, 381 (2004)) on b-bit coordinates, evaluated as a

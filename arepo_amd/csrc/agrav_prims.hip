@@ -279,7 +279,7 @@ static_assert(kSortBlock == 256, "the scatter gives each thread one of the 256 d
 // digit histograms of all passes at once: per-block partials, one row per block
 static constexpr int kHistBlocks = 256;
 
-__global__ __launch_bounds__(kSortBlock) void k_radix_hist_all(const uint64_t *__restrict__ keys, int64_t n, int passes,
+__global__ __launch_bounds__(kSortBlock) void k_radix_hist_all(const uint64_t *__restrict__ keys, int64_t n, int passes, int lo_bit,
                                                                int *__restrict__ part)
 {
   __shared__ int s_h[8][256];
@@ -290,7 +290,7 @@ __global__ __launch_bounds__(kSortBlock) void k_radix_hist_all(const uint64_t *_
     {
       const uint64_t key = keys[k];
       for(int p = 0; p < passes; p++)
-        atomicAdd(&s_h[p][(key >> (8 * p)) & 255], 1);
+        atomicAdd(&s_h[p][(key >> (lo_bit + 8 * p)) & 255], 1);
     }
   __syncthreads();
   for(int q = threadIdx.x; q < passes * 256; q += kSortBlock)
@@ -458,19 +458,19 @@ size_t sort_temp_bytes(int64_t n)
 }
 
 agrav_status radix_sort_u64(uint64_t *keys_in, int *vals_in, uint64_t *keys_out, int *vals_out, int64_t n, int key_bits,
-                            void *d_temp, int *d_err, hipStream_t s)
+                            void *d_temp, int *d_err, hipStream_t s, int lo_bit)
 {
   if(n <= 0)
     return AGRAV_OK;
   const int64_t nt  = (n + kSortTile - 1) / kSortTile;
-  const int passes  = (key_bits + 7) / 8;
+  const int passes  = (key_bits - lo_bit + 7) / 8;
   uint32_t *states  = (uint32_t *)d_temp;                     // passes x (nt x 256 + 1)
   const int64_t per = nt * 256 + 1;
   int *part         = (int *)(states + (size_t)passes * per);  // kHistBlocks x 2048
   int *dbase        = part + (size_t)kHistBlocks * 2048;       // passes x 256
   AG_HIP(hipMemsetAsync(states, 0, (size_t)passes * per * sizeof(uint32_t), s));
   const int hb = (int)(nt < kHistBlocks ? nt : kHistBlocks);
-  k_radix_hist_all<<<hb, kSortBlock, 0, s>>>(keys_in, n, passes, part);
+  k_radix_hist_all<<<hb, kSortBlock, 0, s>>>(keys_in, n, passes, lo_bit, part);
   k_radix_digit_base<<<passes, kSortBlock, 0, s>>>(part, hb, dbase);
   AG_HIP(hipGetLastError());
   uint64_t *ka = keys_in, *kb = keys_out;
@@ -479,7 +479,7 @@ agrav_status radix_sort_u64(uint64_t *keys_in, int *vals_in, uint64_t *keys_out,
     {
       uint32_t *st = states + (size_t)p * per;
       int *ticket  = (int *)(st + nt * 256);
-      k_radix_onesweep<<<(unsigned)nt, kSortBlock, 0, s>>>(ka, va, kb, vb, n, 8 * p, dbase + 256 * p, st, ticket, d_err);
+      k_radix_onesweep<<<(unsigned)nt, kSortBlock, 0, s>>>(ka, va, kb, vb, n, lo_bit + 8 * p, dbase + 256 * p, st, ticket, d_err);
       AG_HIP(hipGetLastError());
       uint64_t *tk = ka;
       ka           = kb;
