@@ -471,8 +471,9 @@ __device__ inline void tile_compute_f64(const Vec4<double> *__restrict__ s_list,
 // arithmetic is the reference's pair kernel (forcetree_walk.c:364-487).
 // Table layout (FP32 walks): float2 {f_i, f_(i+1) - f_i} for i = 0..127 with
 // entry 127 = {f_127, 0}, so an index clamped to 127 reads in bounds, then the
-// potential table p_0..p_127 and a copy of p_127.  Zero-mass entries never
-// enter the tile, so validity is only the table test.
+// potential table as float2 {p_i, p_(i+1) - p_i} in the same way (one read at a
+// fixed offset from the force entry).  Zero-mass entries never enter the tile, so
+// validity is only the table test.
 static constexpr float kTinyR2 = 1e-30f;
 
 template <bool PM, bool EWALD, bool MSOFT, bool POT>
@@ -484,7 +485,7 @@ __device__ inline void tile_compute_f32(const Vec4<float> *__restrict__ s_list, 
   float fx = 0, fy = 0, fz = 0, fp = 0;
   const int nl         = __builtin_amdgcn_readfirstlane(nl_in);  // wave-uniform: scalar loop control
   const float2 *ftab   = reinterpret_cast<const float2 *>(s_tab);
-  const float *ptab    = s_tab + 256;
+  const float2 *ptab   = reinterpret_cast<const float2 *>(s_tab) + 128;
   const float hi_inv   = __builtin_amdgcn_rcpf(hi);
   const float box = (float)wc.box, half = (float)wc.half_box;
   int cnt = 0;
@@ -523,8 +524,8 @@ __device__ inline void tile_compute_f32(const Vec4<float> *__restrict__ s_list, 
           ff = fd.x + w * fd.y;  // force table is pre-scaled by asmthinv2
           if(POT)
             {
-              const float p0 = ptab[tc], p1 = ptab[tc + 1];
-              fpt = p0 + w * (p1 - p0);
+              const float2 pd = ptab[tc];
+              fpt             = pd.x + w * pd.y;
             }
         }
       const float rinv2 = rinv * rinv;
@@ -604,7 +605,7 @@ __device__ inline void tile_compute(const Vec4<T> *__restrict__ s_list, const T 
 // LDS of one group walk (one wavefront per block)
 template <typename T, bool MSOFT> struct WalkLds
 {
-  alignas(16) float tab[388];  // short-range tables: FP64 walks [force 128 | potential 128], FP32 walks see tile_compute_f32
+  alignas(16) float tab[512];  // short-range tables: FP64 walks [force 128 | potential 128], FP32 walks see tile_compute_f32
   int stack[kStackCap];
   Vec4<T> list[kListCap];
   T h[MSOFT ? kListCap : 1];
@@ -623,10 +624,10 @@ template <typename T, bool PM> __device__ inline void load_tables(float *s_tab, 
           const float f0 = tabs[q] * tscale;
           const float f1 = q < 127 ? tabs[q + 1] * tscale : f0;
           ftab[q]        = make_float2(f0, f1 - f0);
-          s_tab[256 + q] = tabs[128 + q];
+          const float p0 = tabs[128 + q];
+          const float p1 = q < 127 ? tabs[128 + q + 1] : p0;
+          ftab[128 + q]  = make_float2(p0, p1 - p0);
         }
-      if((threadIdx.x & 63) == 0)
-        s_tab[384] = tabs[255];
     }
   else
     for(int q = threadIdx.x & 63; q < 256; q += 64)
