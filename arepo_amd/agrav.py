@@ -68,6 +68,7 @@ class Timings(C.Structure):
         ("unbind_solves", C.c_int),
         ("unbind_solve_ms", C.c_double),
         ("unbind_pass_ms", C.c_double),
+        ("device_bytes", C.c_longlong),
     ]
 
     def as_dict(self):
@@ -99,7 +100,7 @@ EXPORTED = [
     "agrav_get_order", "agrav_get_tables", "agrav_hilbert_key21",
     "agrav_batch_set_particles", "agrav_batch_autotune", "agrav_batch_get_tuning", "agrav_batch_set_tuning",
     "agrav_subfind_unbind", "agrav_gravity_timesteps", "agrav_server_create", "agrav_server_submit", "agrav_server_wait", "agrav_server_poll", "agrav_server_stats",
-    "agrav_server_destroy",
+    "agrav_server_destroy", "agrav_solve_footprint",
 ]
 
 
@@ -161,7 +162,8 @@ class RequestResult(C.Structure):
 class ServerStats(C.Structure):
     _fields_ = [("completed", C.c_longlong), ("failed", C.c_longlong), ("queued", C.c_int), ("running", C.c_int),
                 ("peak_running", C.c_int), ("slots", C.c_int), ("hbm_in_use", C.c_size_t), ("hbm_peak", C.c_size_t),
-                ("hbm_budget", C.c_size_t)]
+                ("hbm_budget", C.c_size_t), ("hbm_resident", C.c_size_t), ("evictions", C.c_longlong),
+                ("model_exceeded", C.c_longlong)]
 
 
 def _declare(L):
@@ -202,6 +204,8 @@ def _declare(L):
     L.agrav_subfind_unbind.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp, C.POINTER(UnbindParams), vp, vp,
                                        vp, vp, vp, vp]
     L.agrav_server_create.argtypes = [C.POINTER(Params), C.c_int, C.c_size_t, C.POINTER(vp)]
+    L.agrav_solve_footprint.restype = C.c_size_t
+    L.agrav_solve_footprint.argtypes = [C.POINTER(Params), C.c_int64]
     L.agrav_server_submit.argtypes = [vp, C.POINTER(Request), C.POINTER(C.c_uint64)]
     L.agrav_server_wait.argtypes = [vp, C.c_uint64, C.POINTER(RequestResult)]
     L.agrav_server_poll.argtypes = [vp, C.c_uint64, C.POINTER(C.c_int)]
@@ -211,7 +215,7 @@ def _declare(L):
     L.agrav_hilbert_key21.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
     for name in EXPORTED:
         if name not in ("agrav_status_string", "agrav_last_error", "agrav_hilbert_key21", "agrav_destroy",
-                        "agrav_default_params", "agrav_server_destroy"):
+                        "agrav_default_params", "agrav_server_destroy", "agrav_solve_footprint"):
             getattr(L, name).restype = C.c_int
     L.agrav_destroy.restype = None
     L.agrav_server_destroy.restype = None
@@ -408,14 +412,18 @@ class Gravity:
         self._check(self.L.agrav_compute_grav_accelerations(self.h, int(first_step)))
 
     # -- outputs ------------------------------------------------------------
-    def get_accelerations(self):
+    def get_accelerations(self, fields=("acc", "pot", "nint", "old_acc"), out=None):
+        """The results in caller order; only `fields` are copied back (into the arrays of `out` when given)."""
         n = self.n
-        acc = np.zeros((n, 3))
-        pot = np.zeros(n)
-        nint = np.zeros(n, np.int32)
-        oa = np.zeros(n, np.float32)
-        self._check(self.L.agrav_get_accelerations(self.h, _ptr(acc), _ptr(pot), _ptr(nint), _ptr(oa)))
-        return dict(acc=acc, pot=pot, nint=nint, old_acc=oa)
+        out = dict(out or {})
+        make = dict(acc=lambda: np.empty((n, 3)), pot=lambda: np.empty(n), nint=lambda: np.empty(n, np.int32),
+                    old_acc=lambda: np.empty(n, np.float32))
+        for k in fields:
+            if out.get(k) is None:
+                out[k] = make[k]()
+        self._check(self.L.agrav_get_accelerations(self.h, *(_ptr(out.get(k)) if k in fields else None
+                                                             for k in ("acc", "pot", "nint", "old_acc"))))
+        return {k: (out[k] if k in fields else None) for k in ("acc", "pot", "nint", "old_acc")}
 
     def get_components(self, tree=True, pm=False):
         n = self.n
@@ -476,6 +484,11 @@ class Gravity:
         ef = np.zeros(4 * 129 ** 3) if ewald_forcetest else None
         self._check(self.L.agrav_get_tables(self.h, _ptr(tf), _ptr(tp), _ptr(et), _ptr(ef)))
         return dict(tab_force=tf, tab_pot=tp, ewald_tree=et, ewald_forcetest=ef)
+
+
+def solve_footprint(params: Params, n):
+    """The serving footprint model: device bytes one solve of n particles adds to a fresh context."""
+    return int(lib().agrav_solve_footprint(C.byref(params), int(n)))
 
 
 def hilbert_key21(x, y, z):

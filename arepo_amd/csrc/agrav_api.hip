@@ -12,6 +12,15 @@ namespace agrav
 {
 
 static thread_local std::string g_last_msg;
+thread_local DBufBase **dbuf_registry = nullptr;
+
+size_t device_bytes(const agrav_ctx *c)
+{
+  size_t b = 0;
+  for(const DBufBase *d = c->bufs; d; d = d->reg_next)
+    b += d->bytes;
+  return b;
+}
 
 void set_error_msg(const std::string &m) { g_last_msg = m; }
 
@@ -313,8 +322,12 @@ agrav_status agrav_create(const agrav_params *p, agrav_ctx **out)
       set_error_msg("no HIP device available");
       return AGRAV_ERR_NO_DEVICE;
     }
-  agrav_ctx *c = new agrav_ctx();
-  c->par       = *p;
+  DBufBase *reg = nullptr;
+  dbuf_registry = &reg;
+  agrav_ctx *c  = new agrav_ctx();
+  dbuf_registry = nullptr;
+  c->bufs       = reg;
+  c->par        = *p;
   if(hipGetDevice(&c->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
     {
       delete c;
@@ -354,6 +367,14 @@ agrav_status agrav_create(const agrav_params *p, agrav_ctx **out)
       set_error_msg(c->err);
       agrav_destroy(c);
       return st;
+    }
+  // ewald_init (begrun.c:157, forcetree_ewald.c:89-252): periodic tree-only
+  if(c->par.periodic && c->par.pmgrid == 0 &&
+     ((st = ewald_tree_table(c)) != AGRAV_OK || hipStreamSynchronize(c->stream) != hipSuccess))
+    {
+      set_error_msg(c->err);
+      agrav_destroy(c);
+      return st != AGRAV_OK ? st : AGRAV_ERR_HIP;
     }
   *out = c;
   return AGRAV_OK;
@@ -487,6 +508,26 @@ static agrav_status set_particles_common(agrav_ctx *c, int64_t n, const double *
     return AGRAV_ERR_BAD_PARAM;
   if(n >= (1ll << 30))
     return fail(c, AGRAV_ERR_CAPACITY, "particle count exceeds 2^30 per context");
+  // host input is checked before any state changes
+  if(stype && kind == hipMemcpyHostToDevice)
+    for(int64_t i = 0; i < n; i++)
+      if(stype[i] >= c->soft.nsoft)
+        return fail(c, AGRAV_ERR_BAD_PARAM, "softening type out of range");
+  // from here on the old particle set is gone: a failure below leaves the context
+  // without particles (every later call reports AGRAV_ERR_STATE until a
+  // set_particles succeeds), never with a mix of old and new buffers
+  c->n              = 0;
+  c->n_tree         = 0;
+  c->batch_nsets    = 0;
+  c->n_active       = -1;
+  c->active_version++;
+  c->tree_partial   = false;
+  c->tmap_dirty     = true;
+  c->outputs_zero   = false;
+  c->have_particles = false;
+  c->have_stype     = false;
+  c->have_oldacc    = false;
+  c->have_tree = c->have_tree_force = c->have_pm = c->have_final = false;
   agrav_status st;
   const int64_t nn = n ? n : 1;
   if((st = c->d_pos_in.ensure(3 * nn)) || (st = c->d_mass_in.ensure(nn)) || (st = c->d_oldacc.ensure(nn)))
@@ -497,15 +538,10 @@ static agrav_status set_particles_common(agrav_ctx *c, int64_t n, const double *
       AG_HIP(hipMemcpyAsync(c->d_pos_in.p, pos, 3 * n * sizeof(double), kind, s));
       AG_HIP(hipMemcpyAsync(c->d_mass_in.p, mass, n * sizeof(double), kind, s));
     }
-  c->have_stype = stype != nullptr;
   if(stype)
     {
       if((st = c->d_stype_in.ensure(nn)))
         return st;
-      if(kind == hipMemcpyHostToDevice)
-        for(int64_t i = 0; i < n; i++)
-          if(stype[i] >= c->soft.nsoft)
-            return fail(c, AGRAV_ERR_BAD_PARAM, "softening type out of range");
       AG_HIP(hipMemcpyAsync(c->d_stype_in.p, stype, n, kind, s));
       if(kind != hipMemcpyHostToDevice && n > 0)
         {
@@ -541,15 +577,8 @@ static agrav_status set_particles_common(agrav_ctx *c, int64_t n, const double *
   AG_HIP(hipStreamSynchronize(s));
   c->mass_total     = mt;
   c->n              = n;
-  c->batch_nsets    = 0;
-  c->n_active       = -1;
-  c->active_version++;
-  c->tree_partial   = false;
-  c->tmap_dirty     = true;
-  c->outputs_zero   = false;
+  c->have_stype     = stype != nullptr;
   c->have_particles = true;
-  c->have_oldacc    = false;
-  c->have_tree = c->have_tree_force = c->have_pm = c->have_final = false;
   return AGRAV_OK;
 }
 
@@ -885,7 +914,8 @@ agrav_status agrav_get_timings(agrav_ctx *c, agrav_timings *t)
   agrav_status st = pm_time_settle(c);
   if(st != AGRAV_OK)
     return st;
-  *t = c->tim;
+  *t              = c->tim;
+  t->device_bytes = (long long)device_bytes(c);
   return AGRAV_OK;
 }
 

@@ -185,6 +185,20 @@ agrav_status agrav_batch_set_particles(agrav_ctx *c, int nsets, const int64_t *o
   if(n >= (1ll << 30))
     return fail(c, AGRAV_ERR_CAPACITY, "particle count exceeds 2^30 per context");
   const int64_t nn = n ? n : 1;
+  // the old particle set is gone from here on: a failure below leaves the context
+  // without particles, never with a mix of old and new buffers
+  c->n              = 0;
+  c->n_tree         = 0;
+  c->batch_nsets    = 0;
+  c->n_active       = -1;
+  c->active_version++;
+  c->tree_partial   = false;
+  c->tmap_dirty     = true;
+  c->outputs_zero   = false;
+  c->have_particles = false;
+  c->have_stype     = false;
+  c->have_oldacc    = false;
+  c->have_tree = c->have_tree_force = c->have_pm = c->have_final = false;
   agrav_status st;
   if((st = c->d_pos_in.ensure(3 * nn)) || (st = c->d_mass_in.ensure(nn)) || (st = c->d_oldacc.ensure(nn)) ||
      (st = c->d_pset_in.ensure(nn)) || (st = c->d_soff.ensure(nsets + 1)) || (st = c->d_shd.ensure(nsets)) ||

@@ -1249,8 +1249,8 @@ static agrav_status partial_tree_prepare(agrav_ctx *c)
         return st;
       k_unsort_pm<<<nblk(n), B, 0, s>>>(c->d_idx2.p, n, c->d_acc_pm.p, c->d_pot_pm.p, c->d_acc_pm2.p, c->d_pot_pm2.p);
       AG_HIP(hipGetLastError());
-      std::swap(c->d_acc_pm, c->d_acc_pm2);
-      std::swap(c->d_pot_pm, c->d_pot_pm2);
+      c->d_acc_pm.swap_storage(c->d_acc_pm2);
+      c->d_pot_pm.swap_storage(c->d_pot_pm2);
       c->pm_sorted = false;
     }
   int bad = 0;
@@ -1410,8 +1410,14 @@ agrav_status build_tree(agrav_ctx *c)
     return fail(c, AGRAV_ERR_HIP, "sort/scan look-back did not complete");
   const int mt = hc[0];
   // capacity check in the spirit of TreeAllocFactor (forcetree.c:186-228): the
-  // exact count is known here, so the node arrays are sized to it.
-  if(mt <= 0 || mt > 24 * N + 1)
+  // exact count is known here, so the node arrays are sized to it.  The cubes per
+  // level are counted without overflow (each at most N); their 64-bit sum must
+  // equal the 32-bit scan total, and nodes and particles share one int32 entry
+  // space (entries >= N are nodes).
+  int64_t mt64 = 0;
+  for(int L = 0; L <= AG_MAX_LEVELS; L++)
+    mt64 += c->lvlcnt_h[L];
+  if(mt <= 0 || (int64_t)mt != mt64 || mt64 > 24 * (int64_t)N + 1 || mt64 + (int64_t)N >= ((int64_t)1 << 31))
     return fail(c, AGRAV_ERR_CAPACITY, "octree node count out of range");
   const int M = mt;
   if((st = c->d_nfirst.ensure(M)) || (st = c->d_nlast.ensure(M)) || (st = c->d_nlevel.ensure(M)) || (st = c->d_sx.ensure(M)) ||

@@ -6,6 +6,7 @@
 
 #include <rocfft/rocfft.h>
 
+#include <utility>
 #include <vector>
 
 #include "agrav_internal.h"
@@ -13,7 +14,29 @@
 namespace agrav
 {
 
-template <class T> struct DBuf
+// Every device buffer of a context registers itself in the context's list while
+// the context is constructed (agrav_create sets dbuf_registry for that), so the
+// HBM a context holds is known exactly: the sum of `bytes` over the list.
+struct DBufBase;
+extern thread_local DBufBase **dbuf_registry;
+
+struct DBufBase
+{
+  DBufBase *reg_next = nullptr;
+  size_t bytes       = 0;  // device bytes currently allocated
+  DBufBase()
+  {
+    if(dbuf_registry)
+      {
+        reg_next       = *dbuf_registry;
+        *dbuf_registry = this;
+      }
+  }
+  DBufBase(const DBufBase &)            = delete;
+  DBufBase &operator=(const DBufBase &) = delete;
+};
+
+template <class T> struct DBuf : DBufBase
 {
   T *p       = nullptr;
   size_t cap = 0;  // elements
@@ -21,10 +44,7 @@ template <class T> struct DBuf
   {
     if(count <= cap && p)
       return AGRAV_OK;
-    if(p)
-      (void)hipFree(p);
-    p         = nullptr;
-    cap       = 0;
+    release();
     size_t nb = (count ? count : 1) * sizeof(T);
     if(hipMalloc((void **)&p, nb) != hipSuccess)
       {
@@ -33,15 +53,24 @@ template <class T> struct DBuf
         set_error_msg("hipMalloc failed");
         return AGRAV_ERR_OOM;
       }
-    cap = count ? count : 1;
+    cap   = count ? count : 1;
+    bytes = nb;
     return AGRAV_OK;
   }
   void release()
   {
     if(p)
       (void)hipFree(p);
-    p   = nullptr;
-    cap = 0;
+    p     = nullptr;
+    cap   = 0;
+    bytes = 0;
+  }
+  // exchange the allocations of two buffers (each stays in its own registry)
+  void swap_storage(DBuf &o)
+  {
+    std::swap(p, o.p);
+    std::swap(cap, o.cap);
+    std::swap(bytes, o.bytes);
   }
 };
 
@@ -218,6 +247,9 @@ struct agrav_ctx
   // unbinding loop
   agrav::UnbindBufs unb;
 
+  // every DBuf member above, linked at construction (device_bytes sums them)
+  agrav::DBufBase *bufs = nullptr;
+
   // timing
   hipEvent_t ev[8] = {};
   agrav_timings tim{};
@@ -227,6 +259,7 @@ struct agrav_ctx
 namespace agrav
 {
 agrav_status fail(agrav_ctx *c, agrav_status s, const std::string &m);
+size_t device_bytes(const agrav_ctx *c);
 agrav_status build_tree(agrav_ctx *c);
 agrav_status walk_tree(agrav_ctx *c, int use_bh);
 agrav_status active_targets(agrav_ctx *c);

@@ -411,6 +411,7 @@ struct DirectAllArgs
 {
   const double *pos, *mass, *hsoft;
   const uint8_t *stype;
+  int nsoft;  // softening types in the table; a larger type gives NaN results
   int n, periodic, pm, ewald;
   double box, half, asmthfac, asmthinv2, ewald_fac;
   const float *tab;
@@ -430,7 +431,10 @@ __global__ __launch_bounds__(B) void k_direct(DirectAllArgs a)
   const int ii     = has ? i : 0;
   const double px = wrapb(a.pos[3 * (int64_t)ii], a.periodic, a.box), py = wrapb(a.pos[3 * (int64_t)ii + 1], a.periodic, a.box),
                pz = wrapb(a.pos[3 * (int64_t)ii + 2], a.periodic, a.box);
-  const double h_i = a.hsoft[a.stype ? a.stype[ii] : 0];
+  // softening types index the table only in range; a type out of range makes
+  // the results NaN (through the target's outputs or the source's mass)
+  const int t_i    = a.stype ? a.stype[ii] : 0;
+  const double h_i = a.hsoft[t_i < a.nsoft ? t_i : 0];
   double ax = 0, ay = 0, az = 0, pot = 0;
   const int j0 = chunk * a.chunk, j1 = min(a.n, j0 + a.chunk);
   for(int tb = j0; tb < j1; tb += B)
@@ -439,11 +443,12 @@ __global__ __launch_bounds__(B) void k_direct(DirectAllArgs a)
       __syncthreads();
       if(j < j1)
         {
+          const int t_j    = a.stype ? a.stype[j] : 0;
           s_x[threadIdx.x] = wrapb(a.pos[3 * (int64_t)j], a.periodic, a.box);
           s_y[threadIdx.x] = wrapb(a.pos[3 * (int64_t)j + 1], a.periodic, a.box);
           s_z[threadIdx.x] = wrapb(a.pos[3 * (int64_t)j + 2], a.periodic, a.box);
-          s_m[threadIdx.x] = a.mass[j];
-          s_h[threadIdx.x] = a.hsoft[a.stype ? a.stype[j] : 0];
+          s_m[threadIdx.x] = t_j < a.nsoft ? a.mass[j] : __builtin_nan("");
+          s_h[threadIdx.x] = a.hsoft[t_j < a.nsoft ? t_j : 0];
         }
       __syncthreads();
       int cnt = min(B, j1 - tb);
@@ -534,6 +539,8 @@ __global__ __launch_bounds__(B) void k_direct(DirectAllArgs a)
     }
   if(has)
     {
+      if(t_i >= a.nsoft)
+        ax = ay = az = pot = __builtin_nan("");
       double *o = a.part + ((size_t)chunk * a.n + i) * 4;
       o[0]      = ax;
       o[1]      = ay;
@@ -767,6 +774,39 @@ agrav_status forcetest(agrav_ctx *c, int ns, const int *sel_h, double *acc, doub
   return AGRAV_OK;
 }
 
+// k_direct over n particles given by device pointers, partial sums into part
+// ([nchunk][n][4]); the context supplies the softening and short-range tables
+// and, for a periodic tree-only context, the Ewald table (ready beforehand)
+static agrav_status direct_launch(agrav_ctx *c, int64_t n, const double *pos, const double *mass, const uint8_t *stype,
+                                  double *part, int nchunk, int chunk, hipStream_t s)
+{
+  const bool ew = c->par.periodic && c->par.pmgrid == 0;
+  WalkConst wc  = walk_const(c, 0);
+  DirectAllArgs a{};
+  a.pos       = pos;
+  a.mass      = mass;
+  a.hsoft     = c->d_soft.p;
+  a.stype     = stype;
+  a.nsoft     = c->soft.nsoft;
+  a.n         = (int)n;
+  a.periodic  = c->par.periodic;
+  a.pm        = c->par.pmgrid > 0;
+  a.ewald     = ew;
+  a.box       = c->par.box_size;
+  a.half      = 0.5 * c->par.box_size;
+  a.asmthfac  = wc.asmthfac;
+  a.asmthinv2 = wc.asmthinv2;
+  a.ewald_fac = wc.ewald_fac;
+  a.tab       = c->d_tab.p;
+  a.ewtab     = ew ? c->d_ewald.p : c->d_soft.p;  // read only when ewald
+  a.chunk     = chunk;
+  a.part      = part;
+  dim3 grid((unsigned)((n + B - 1) / B), nchunk);
+  k_direct<<<grid, B, 0, s>>>(a);
+  AG_HIP(hipGetLastError());
+  return AGRAV_OK;
+}
+
 agrav_status direct_all(agrav_ctx *c, double *acc, double *pot)
 {
   if(!c->have_particles)
@@ -781,8 +821,6 @@ agrav_status direct_all(agrav_ctx *c, double *acc, double *pot)
   if(ew && !c->ewald_ready)
     if((st = ewald_tree_table(c)) != AGRAV_OK)
       return st;
-  if(!ew && (st = c->d_ewald.ensure(1)))
-    return st;
   hipStream_t s = c->stream;
   int tblocks   = (int)((n + B - 1) / B);
   int nchunk    = pick_chunks(tblocks, n);
@@ -794,27 +832,13 @@ agrav_status direct_all(agrav_ctx *c, double *acc, double *pot)
       dout.release();
       return st;
     }
-  WalkConst wc = walk_const(c, 0);
-  DirectAllArgs a{};
-  a.pos       = c->d_pos_in.p;
-  a.mass      = c->d_mass_in.p;
-  a.hsoft     = c->d_soft.p;
-  a.stype     = c->have_stype ? c->d_stype_in.p : nullptr;
-  a.n         = (int)n;
-  a.periodic  = c->par.periodic;
-  a.pm        = c->par.pmgrid > 0;
-  a.ewald     = ew;
-  a.box       = c->par.box_size;
-  a.half      = 0.5 * c->par.box_size;
-  a.asmthfac  = wc.asmthfac;
-  a.asmthinv2 = wc.asmthinv2;
-  a.ewald_fac = wc.ewald_fac;
-  a.tab       = c->d_tab.p;
-  a.ewtab     = c->d_ewald.p;
-  a.chunk     = chunk;
-  a.part      = dpart.p;
-  dim3 grid(tblocks, nchunk);
-  k_direct<<<grid, B, 0, s>>>(a);
+  if((st = direct_launch(c, n, c->d_pos_in.p, c->d_mass_in.p, c->have_stype ? c->d_stype_in.p : nullptr, dpart.p, nchunk,
+                         chunk, s)) != AGRAV_OK)
+    {
+      dpart.release();
+      dout.release();
+      return st;
+    }
   k_chunk_reduce4<<<tblocks, B, 0, s>>>(dpart.p, (int)n, nchunk, dout.p);
   AG_HIP(hipGetLastError());
   std::vector<double> h((size_t)n * 4);

@@ -4,12 +4,17 @@
 // Requests (one particle set each) are queued in arrival order.  A fixed number
 // of slots each own a resident solver context -- its own HIP stream and its own
 // device buffers, which stay allocated between requests -- and a worker thread.
-// A free slot admits the oldest queued request whose estimated HBM footprint fits
-// the remaining budget (first fit in arrival order), so requests that arrive
-// while others run start as soon as a slot and memory free up, and their device
-// work overlaps on the GPU through the slots' streams.  Every solve runs the same
-// deterministic kernels on its own buffers, so a result does not depend on what
-// else is in flight.
+// Admission bounds the HBM the slots hold.  Each slot is charged what its
+// context actually holds (the sum of its device buffers, measured after every
+// solve), or, while it solves, what the request's footprint model says the
+// buffers grow to.  A free slot admits the oldest queued request that fits the
+// budget (first fit in arrival order); when only the buffers idle slots keep
+// resident stand in the way, those contexts are replaced by fresh ones.  So
+// requests that arrive while others run start as soon as a slot and memory free
+// up, and their device work overlaps on the GPU through the slots' streams.
+// Every solve runs the same deterministic kernels on its own buffers with the
+// same softening table, so a result does not depend on what else is in flight
+// or on what the slot solved before.
 //
 // The reference solves independent sets one after another on one core
 // (subfind_unbind, subfind_serial.c:429-560); this is the GPU serving analogue.
@@ -33,7 +38,7 @@ struct Job
   uint64_t ticket;
   agrav_request req;
   clk::time_point t_submit, t_start, t_end;
-  size_t footprint = 0;
+  size_t footprint = 0;  // device bytes of a context that has solved this request (model)
   agrav_status status = AGRAV_OK;
   bool done           = false;
   int slot            = -1;
@@ -44,16 +49,19 @@ struct Job
 struct agrav_server
 {
   agrav_params par{};
-  size_t budget = 0, in_use = 0, peak = 0;
+  size_t budget = 0, peak = 0;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
   std::deque<Job *> queue;
   std::unordered_map<uint64_t, Job *> jobs;
   std::vector<std::thread> workers;
   std::vector<agrav_ctx *> slots;
+  std::vector<size_t> resident;  // device bytes each slot's context holds (measured after every solve)
+  std::vector<Job *> busy;       // the job each slot runs, or null
+  size_t base = 0;               // device bytes of a fresh context (tables, PM meshes)
   uint64_t next_ticket = 1;
   bool stop            = false;
-  long long completed = 0, failed = 0;
+  long long completed = 0, failed = 0, evictions = 0, model_exceeded = 0;
   int running = 0, peak_running = 0;
   std::string err;
 };
@@ -61,16 +69,105 @@ struct agrav_server
 namespace
 {
 
-// HBM footprint of one solve: particles (inputs, sorted copies, keys, results),
-// octree (about 0.5 nodes per particle for clustered sets, 1 worst case) and scratch
-size_t footprint_of(int64_t n) { return (size_t)(n > 0 ? n : 1) * 720 + ((size_t)64 << 20); }
+// What a slot is charged: its resident buffers, or what its running job will
+// grow them to (buffers only grow while a context lives).
+size_t charge(const agrav_server *S, int s)
+{
+  size_t r = S->resident[s];
+  if(S->busy[s] && S->busy[s]->footprint > r)
+    r = S->busy[s]->footprint;
+  return r;
+}
 
-void run_job(agrav_ctx *c, Job *j)
+size_t in_use(const agrav_server *S)
+{
+  size_t t = 0;
+  for(size_t s = 0; s < S->slots.size(); s++)
+    t += charge(S, (int)s);
+  return t;
+}
+
+// Drop an idle slot's resident buffers: the context is replaced by a fresh one
+// (its worker is waiting on the server mutex, which the caller holds).
+bool evict(agrav_server *S, int s)
+{
+  agrav_ctx *c = S->slots[s];
+  if(c)
+    agrav_destroy(c);
+  S->slots[s]    = nullptr;
+  S->resident[s] = 0;
+  agrav_ctx *f   = nullptr;
+  if(agrav_create(&S->par, &f) != AGRAV_OK)
+    return false;
+  S->slots[s]    = f;
+  S->resident[s] = agrav::device_bytes(f);
+  S->evictions++;
+  return true;
+}
+
+bool fits(const agrav_server *S, int s, const Job *j)
+{
+  const size_t mine = S->resident[s];  // slot s is idle: charged its resident bytes
+  return in_use(S) - mine + (mine > j->footprint ? mine : j->footprint) <= S->budget;
+}
+
+// First queued job (arrival order) that slot s can admit within the budget,
+// evicting idle slots' buffers (its own first, then the largest) when that makes
+// it fit.  Null when every queued job must wait for a running job to end.
+Job *pick_job(agrav_server *S, int s)
+{
+  if(!S->slots[s] && !evict(S, s))
+    return nullptr;
+  for(auto it = S->queue.begin(); it != S->queue.end(); ++it)
+    {
+      Job *j = *it;
+      if(!fits(S, s, j))
+        {
+          // the least the server can hold with j on slot s: every other idle
+          // slot back to a fresh context, the running ones as charged
+          size_t floor = j->footprint;
+          for(size_t o = 0; o < S->slots.size(); o++)
+            if((int)o != s)
+              floor += S->busy[o] ? charge(S, (int)o) : S->base;
+          if(floor > S->budget)
+            continue;  // fits only after a running job ends
+          if(S->resident[s] > j->footprint && !evict(S, s))
+            return nullptr;
+          while(!fits(S, s, j))
+            {
+              int victim = -1;
+              for(size_t o = 0; o < S->slots.size(); o++)
+                if((int)o != s && !S->busy[o] && S->resident[o] > S->base &&
+                   (victim < 0 || S->resident[o] > S->resident[victim]))
+                  victim = (int)o;
+              if(victim < 0 || !evict(S, victim))
+                return nullptr;
+            }
+        }
+      S->queue.erase(it);
+      return j;
+    }
+  return nullptr;
+}
+
+void run_job(agrav_server *S, agrav_ctx *c, Job *j)
 {
   const agrav_request &r = j->req;
-  agrav_status s         = AGRAV_OK;
-  if(r.softening > 0 && r.softening != c->par.softening[0])
-    s = agrav_set_softenings(c, 1, &r.softening);
+  agrav_status s         = c ? AGRAV_OK : AGRAV_ERR_STATE;
+  // the effective softening table: the server's, with type 0 replaced by the
+  // request's value when it gives one; set whenever the slot's table differs, so
+  // a result never depends on what the slot solved before
+  double eps[AGRAV_MAX_SOFT_TYPES];
+  const int ns = S->par.n_soft_types;
+  for(int k = 0; k < ns; k++)
+    eps[k] = S->par.softening[k];
+  if(r.softening > 0)
+    eps[0] = r.softening;
+  bool same = c && c->par.n_soft_types == ns;
+  for(int k = 0; same && k < ns; k++)
+    same = c->par.softening[k] == eps[k];
+  if(s == AGRAV_OK && !same)
+    s = agrav_set_softenings(c, ns, eps);
   if(s == AGRAV_OK)
     s = agrav_set_particles(c, r.n, r.pos, r.mass, r.softtype);
   if(s == AGRAV_OK)
@@ -82,42 +179,62 @@ void run_job(agrav_ctx *c, Job *j)
 
 void worker(agrav_server *S, int slot)
 {
-  agrav_ctx *c = S->slots[slot];
   std::unique_lock<std::mutex> lk(S->mu);
   for(;;)
     {
       Job *pick = nullptr;
-      S->cv_work.wait(lk, [&] {
-        if(S->stop)
-          return true;
-        for(Job *j : S->queue)
-          if(S->in_use + j->footprint <= S->budget || S->running == 0)
-            return true;
-        return false;
-      });
-      if(S->stop && S->queue.empty())
-        return;
-      for(auto it = S->queue.begin(); it != S->queue.end(); ++it)
-        if(S->in_use + (*it)->footprint <= S->budget || S->running == 0)
-          {
-            pick = *it;
-            S->queue.erase(it);
+      for(;;)
+        {
+          if(S->queue.empty())
+            {
+              if(S->stop)
+                return;
+              S->cv_work.wait(lk);
+              continue;
+            }
+          if((pick = pick_job(S, slot)))
             break;
-          }
-      if(!pick)
-        continue;
-      S->in_use += pick->footprint;
-      S->peak = S->in_use > S->peak ? S->in_use : S->peak;
+          if(S->running == 0)
+            {
+              // nothing will free memory (a fresh context could not be made):
+              // the oldest request fails instead of waiting for ever
+              Job *j = S->queue.front();
+              S->queue.pop_front();
+              j->status = AGRAV_ERR_OOM;
+              j->t_start = j->t_end = clk::now();
+              j->done               = true;
+              S->failed++;
+              S->cv_done.notify_all();
+              continue;
+            }
+          // every queued job waits for memory that a running job holds; its end
+          // notifies cv_work (with nothing running, every job fits: submit
+          // rejects footprints above the budget)
+          S->cv_work.wait(lk);
+        }
+      S->busy[slot] = pick;
       S->running++;
       S->peak_running = S->running > S->peak_running ? S->running : S->peak_running;
-      pick->slot    = slot;
-      pick->t_start = clk::now();
+      const size_t u  = in_use(S);
+      S->peak         = u > S->peak ? u : S->peak;
+      pick->slot      = slot;
+      pick->t_start   = clk::now();
+      agrav_ctx *c    = S->slots[slot];
       lk.unlock();
-      run_job(c, pick);
+      run_job(S, c, pick);
+      const size_t now = c ? agrav::device_bytes(c) : 0;
       lk.lock();
       pick->t_end = clk::now();
       pick->done  = true;
-      S->in_use -= pick->footprint;
+      // the model is an upper bound of what a solve leaves resident; count the
+      // solves that exceed it (their real bytes are charged from now on)
+      const size_t bound = pick->footprint > S->resident[slot] ? pick->footprint : S->resident[slot];
+      if(now > bound)
+        S->model_exceeded++;
+      S->resident[slot] = now;
+      S->busy[slot]     = nullptr;
+      const size_t u2   = in_use(S);
+      S->peak           = u2 > S->peak ? u2 : S->peak;
       S->running--;
       if(pick->status == AGRAV_OK)
         S->completed++;
@@ -131,6 +248,22 @@ void worker(agrav_server *S, int slot)
 double ms_between(clk::time_point a, clk::time_point b) { return std::chrono::duration<double, std::milli>(b - a).count(); }
 
 }  // namespace
+
+namespace agrav
+{
+// Device bytes one solve of n particles adds to a fresh context: about 300 B per
+// particle of inputs, sorted copies, keys, prefix sums and results, about 175 B
+// per octree node (at most 1.5 nodes per particle assumed; clustered sets have
+// 0.5-0.7), the sort's scratch and a fixed reserve for the small buffers.
+size_t solve_footprint(const agrav_params *p, int64_t n)
+{
+  const size_t nn = (size_t)(n > 0 ? n : 1);
+  size_t per      = 720;
+  if(p && p->pmgrid > 0)
+    per += 32;  // PM accelerations and potentials per particle
+  return nn * per + ((size_t)16 << 20);
+}
+}  // namespace agrav
 
 extern "C" {
 
@@ -154,6 +287,17 @@ agrav_status agrav_server_create(const agrav_params *p, int slots, size_t hbm_bu
           return st;
         }
       S->slots.push_back(c);
+      S->resident.push_back(agrav::device_bytes(c));
+      S->busy.push_back(nullptr);
+    }
+  S->base = S->resident[0];
+  if((size_t)slots * S->base > S->budget)
+    {
+      for(agrav_ctx *d : S->slots)
+        agrav_destroy(d);
+      delete S;
+      agrav::set_error_msg("the HBM budget does not hold the slots' fresh contexts");
+      return AGRAV_ERR_CAPACITY;
     }
   for(int i = 0; i < slots; i++)
     S->workers.emplace_back(worker, S, i);
@@ -167,7 +311,7 @@ agrav_status agrav_server_submit(agrav_server *S, const agrav_request *r, uint64
     return AGRAV_ERR_BAD_PARAM;
   Job *j       = new Job();
   j->req       = *r;
-  j->footprint = footprint_of(r->n);
+  j->footprint = S->base + agrav::solve_footprint(&S->par, r->n);
   j->t_submit  = clk::now();
   {
     std::lock_guard<std::mutex> lk(S->mu);
@@ -175,6 +319,13 @@ agrav_status agrav_server_submit(agrav_server *S, const agrav_request *r, uint64
       {
         delete j;
         return AGRAV_ERR_STATE;
+      }
+    // with every other slot at a fresh context, the request must fit the budget
+    if(j->footprint + (S->slots.size() - 1) * S->base > S->budget)
+      {
+        delete j;
+        agrav::set_error_msg("the request's HBM footprint exceeds the server's budget");
+        return AGRAV_ERR_CAPACITY;
       }
     j->ticket = S->next_ticket++;
     S->jobs[j->ticket] = j;
@@ -234,12 +385,20 @@ agrav_status agrav_server_stats(agrav_server *S, agrav_server_info *st)
   st->queued          = (int)S->queue.size();
   st->running         = S->running;
   st->peak_running    = S->peak_running;
-  st->hbm_in_use      = S->in_use;
+  st->hbm_in_use      = in_use(S);
   st->hbm_peak        = S->peak;
   st->hbm_budget      = S->budget;
   st->slots           = (int)S->slots.size();
+  size_t res          = 0;
+  for(size_t s = 0; s < S->slots.size(); s++)
+    res += S->resident[s];
+  st->hbm_resident    = res;
+  st->evictions       = S->evictions;
+  st->model_exceeded  = S->model_exceeded;
   return AGRAV_OK;
 }
+
+size_t agrav_solve_footprint(const agrav_params *p, int64_t n) { return agrav::solve_footprint(p, n); }
 
 void agrav_server_destroy(agrav_server *S)
 {
@@ -253,7 +412,8 @@ void agrav_server_destroy(agrav_server *S)
   for(auto &t : S->workers)
     t.join();
   for(agrav_ctx *c : S->slots)
-    agrav_destroy(c);
+    if(c)
+      agrav_destroy(c);
   for(auto &kv : S->jobs)
     delete kv.second;
   delete S;

@@ -49,8 +49,9 @@ def make_params(A, precision, walk, eps, pmgrid=128, pot=False, group_size=64):
                          precision=precision, walk=walk, group_size=group_size, leaf_bucket=4, eval_potential=pot)
 
 
-def run_path(A, pos, mass, eps, precision, walk, steps, warmup, old_acc=None, pot=False):
-    """Warm up, then time `steps` full solves; returns timing dict and the last result."""
+def run_path(A, pos, mass, eps, precision, walk, steps, warmup, old_acc=None, pot=False, components=False):
+    """Warm up, then time `steps` full solves; returns timing dict and the last result (with the tree and
+    PM parts of the acceleration when `components`)."""
     g = A.Gravity(make_params(A, precision, walk, eps, pot=pot))
     g.set_particles(pos, mass)
     if old_acc is not None:
@@ -68,8 +69,41 @@ def run_path(A, pos, mass, eps, precision, walk, steps, warmup, old_acc=None, po
     total = time.perf_counter() - t0
     tim = g.timings()
     res = g.get_accelerations()
+    if components:
+        res.update(g.get_components(tree=True, pm=True))
     g.close()
     return dict(total_s=total, step_ms=step_ms, timings=tim), res
+
+
+def host_io_run(A, pos, mass, eps, old_acc, steps):
+    """The drop-in as AREPO's glue drives it every step: positions, masses and softening types from host
+    arrays (agrav_set_particles), OldAcc from the host (agrav_set_old_acc), the step, and the
+    accelerations and OldAcc back to host arrays (agrav_get_accelerations); pageable host memory, as
+    the glue's myrealloc buffers.  Times each part per step."""
+    g = A.Gravity(make_params(A, A.FP32, A.WALK_GROUP, eps))
+    st = np.zeros(len(pos), np.uint8)
+    oa = old_acc.copy()
+    out = {"acc": np.empty((len(pos), 3)), "old_acc": np.empty(len(pos), np.float32)}
+    t = {"h2d": [], "device": [], "d2h": [], "total": []}
+    for k in range(steps + 1):
+        t0 = time.perf_counter()
+        g.set_particles(pos, mass, softtype=st)
+        g.set_old_acc(oa)
+        t1 = time.perf_counter()
+        g.compute_grav_accelerations(first_step=False)
+        t2 = time.perf_counter()
+        r = g.get_accelerations(fields=("acc", "old_acc"), out=out)
+        t3 = time.perf_counter()
+        oa = r["old_acc"]
+        if k > 0:  # the first call is the warm-up
+            for key, v in (("h2d", t1 - t0), ("device", t2 - t1), ("d2h", t3 - t2), ("total", t3 - t0)):
+                t[key].append(1e3 * v)
+    g.close()
+    med = {k: float(np.median(v)) for k, v in t.items()}
+    return {"steps": steps, "ms_per_step": med["total"], "particles_per_s": len(pos) / (med["total"] * 1e-3),
+            "h2d_ms": med["h2d"], "device_ms": med["device"], "d2h_ms": med["d2h"],
+            "bytes_in": int(len(pos) * (24 + 8 + 1 + 4)), "bytes_out": int(len(pos) * (24 + 4)),
+            "host_memory": "pageable numpy arrays"}
 
 
 def substep_run(A, pos, mass, eps, old_acc, fractions, steps):
@@ -362,7 +396,7 @@ def profiled_counters():
     got = {}
     try:
         for line in open(PMC_PROFILE):
-            for key in ("traffic_bytes_per_launch", "valu_issue_fraction", "l2_hit_rate"):
+            for key in ("traffic_bytes_per_launch", "valu_issue_fraction", "l2_hit_rate", "kernel_ms"):
                 if line.startswith(key + ":"):
                     got[key] = float(line.split(":")[1])
     except OSError:
@@ -472,7 +506,7 @@ def main():
 
     # FP32 group walk: the measured path
     log("FP32 group walk: warmup %d, steps %d" % (args.warmup, args.steps))
-    r32, res32 = run_path(A, pos, mass, eps, A.FP32, A.WALK_GROUP, args.steps, args.warmup)
+    r32, res32 = run_path(A, pos, mass, eps, A.FP32, A.WALK_GROUP, args.steps, args.warmup, components=True)
     tim = r32["timings"]
     ms_per_step = 1e3 * r32["total_s"] / args.steps
     value = n * args.steps / r32["total_s"]
@@ -518,6 +552,13 @@ def main():
     alg_rate = alg_bytes / walk_s if walk_s > 0 else 0.0
     prof = profiled_counters()
     traffic = prof.get("traffic_bytes_per_launch")
+    # the counters come from a committed profile of this kernel at this workload: they stand for this run
+    # only while its kernel time agrees with the profile's
+    prof_ms = prof.get("kernel_ms")
+    stale = prof_ms is not None and abs(tim["walk_kernel"] - prof_ms) > 0.1 * prof_ms
+    if stale:
+        log("WARNING: walk kernel %.3f ms here, %.3f ms in %s: the profiled traffic may be stale"
+            % (tim["walk_kernel"], prof_ms, os.path.relpath(PMC_PROFILE, ROOT)))
     # Node and particle records are reused from L2 and from the LDS tile by all lanes of a group, so
     # the algorithmic rate can pass the HBM peak.  Then the rate of the measured HBM traffic (rocprofv3
     # FETCH_SIZE + WRITE_SIZE of this kernel at this workload) is the roofline figure, flagged
@@ -536,7 +577,8 @@ def main():
                                      "fp32_frac": flops / walk_s / FP32_PEAK if walk_s > 0 else 0.0,
                                      "valu_issue_fraction_of_kernel_time": prof.get("valu_issue_fraction"),
                                      "l2_hit_rate": prof.get("l2_hit_rate")},
-                       "profiled": os.path.relpath(PMC_PROFILE, ROOT) if prof else None}
+                       "profiled": os.path.relpath(PMC_PROFILE, ROOT) if prof else None,
+                       "profiled_kernel_ms": prof_ms, "profile_stale": bool(stale)}
 
     # whole-step algorithmic bytes, the per-solve formula of the path's scope table:
     # I*16 + N*52 + tree (keys+indices, 4 passes read+write; 48 B per node read+write)
@@ -565,7 +607,8 @@ def main():
     res64p = None
     if not args.no_fp64:
         log("FP64 plain walk (one lane per target)")
-        r64p, res64p = run_path(A, pos, mass, eps, A.FP64, A.WALK_PLAIN, 1, 0, old_acc=res32["old_acc"])
+        r64p, res64p = run_path(A, pos, mass, eps, A.FP64, A.WALK_PLAIN, 1, 0, old_acc=res32["old_acc"],
+                                components=True)
         log("FP64 group walk")
         r64g, res64g = run_path(A, pos, mass, eps, A.FP64, A.WALK_GROUP, 2, 0, old_acc=res32["old_acc"])
         out["fp64_plain"] = {"ms_per_step": 1e3 * r64p["total_s"], "walk_ms": r64p["timings"]["treewalk"],
@@ -573,6 +616,13 @@ def main():
         out["fp64_group"] = {"ms_per_step": 1e3 * r64g["total_s"] / 2, "walk_ms": r64g["timings"]["treewalk"],
                              "interactions": r64g["timings"]["interactions"]}
         out["speedup_fp32_vs_fp64_plain"] = out["fp64_plain"]["ms_per_step"] / ms_per_step
+        # the interactions the reference's per-particle walk makes (the FP64 plain count), per second of
+        # the FP32 step: the group walk evaluates more pairs, each far cheaper
+        out["reference_equivalent_interactions_per_s"] = r64p["timings"]["interactions"] / (ms_per_step * 1e-3)
+
+    log("drop-in with host arrays (set_particles + set_old_acc + step + get_accelerations)")
+    out["fp32_group_host_io"] = host_io_run(A, pos, mass, eps, res32["old_acc"], max(3, min(args.steps, 10)))
+    log("host io: %s" % json.dumps(out["fp32_group_host_io"]))
 
     cpu = None
     if not args.no_cpu:
@@ -600,16 +650,28 @@ def main():
         log("direct sum took %.1f s" % (time.perf_counter() - t0))
         g.close()
         errs = {}
+        a_norm = np.linalg.norm(ft["acc"], axis=1)
+
+        def stat(e):
+            return {"median": float(np.median(e)), "p99": float(np.percentile(e, 99))}
+
+        def split(acc_tree, acc_pm):
+            # the tree part against FORCETEST's exact short-range sum, the PM part against the long-range
+            # rest (gravtree_forcetest.c:629-638), both relative to |a|
+            return {"tree": stat(np.linalg.norm(acc_tree - ft["acc_short"], axis=1) / a_norm),
+                    "pm": stat(np.linalg.norm(acc_pm - (ft["acc"] - ft["acc_short"]), axis=1) / a_norm)}
+
         for name, res in (("fp32_group", res32), ("fp64_plain", res64p)):
             if res is None:
                 continue
-            e = rel_err(res["acc"][sel], ft["acc"])
-            errs[name] = {"median": float(np.median(e)), "p99": float(np.percentile(e, 99))}
+            errs[name] = stat(rel_err(res["acc"][sel], ft["acc"]))
+            if res.get("acc_tree") is not None:
+                errs[name]["split"] = split(res["acc_tree"][sel], res["acc_pm"][sel])
         if cpu is not None:
             # the oracle's own TreePM error on its sample targets (tree part from the oracle, PM from the oracle)
             pos_in = np.searchsorted(cpu["sel"], sel)
-            e = rel_err(cpu["acc_tree"][pos_in] + cpu["acc_pm"][sel], ft["acc"])
-            errs["cpu_oracle"] = {"median": float(np.median(e)), "p99": float(np.percentile(e, 99))}
+            errs["cpu_oracle"] = stat(rel_err(cpu["acc_tree"][pos_in] + cpu["acc_pm"][sel], ft["acc"]))
+            errs["cpu_oracle"]["split"] = split(cpu["acc_tree"][pos_in], cpu["acc_pm"][sel])
         out["force_error_vs_direct"] = errs
         out["force_error_sample"] = int(args.err_sample)
 

@@ -102,6 +102,7 @@ typedef struct
   int unbind_solves;      /* agrav_subfind_unbind: batched potential solves (tree build + walk) */
   double unbind_solve_ms; /* agrav_subfind_unbind: device time of those solves */
   double unbind_pass_ms;  /* agrav_subfind_unbind: device time of the per-group pass kernels */
+  long long device_bytes; /* HBM held by the context's device buffers now (they are kept between calls) */
 } agrav_timings;
 
 typedef struct agrav_ctx agrav_ctx;
@@ -222,6 +223,7 @@ agrav_status agrav_gravity_forcetest(agrav_ctx *ctx, int ns, const int32_t *sel,
  * caller order (host pointers). */
 agrav_status agrav_gravity_direct(agrav_ctx *ctx, double *acc, double *pot);
 
+
 agrav_status agrav_get_timings(agrav_ctx *ctx, agrav_timings *t);
 
 /* Tree inspection for tests: node count and per-node data (host pointers of
@@ -323,10 +325,13 @@ agrav_status agrav_subfind_unbind(agrav_ctx *ctx, int ngroups, const int64_t *of
  * No counterpart in the reference, which solves independent sets one after
  * another (subfind_unbind, subfind_serial.c:429-560).  Requests are queued in
  * arrival order; `slots` resident contexts (own stream, own device buffers kept
- * between requests) admit the oldest request whose HBM footprint fits the
- * budget.  Each request runs compute_grav_accelerations(first_step=1) on its set
- * with the server's parameters (softening per request).  Results are identical
- * to a solve of the same set alone. */
+ * between requests) admit the oldest request that fits the HBM budget: each slot
+ * is charged the bytes its context holds, or its running request's footprint;
+ * idle contexts are replaced by fresh ones when their resident buffers alone
+ * keep a request out.  Each request runs compute_grav_accelerations(first_step=1)
+ * on its set with the server's parameters and softening table (type 0 from the
+ * request when it gives one).  Results are identical to a solve of the same set
+ * alone. */
 typedef struct agrav_server agrav_server;
 
 typedef struct
@@ -351,16 +356,26 @@ typedef struct
 {
   long long completed, failed;
   int queued, running, peak_running, slots;
-  size_t hbm_in_use, hbm_peak, hbm_budget;
+  size_t hbm_in_use;    /* what admission charges now: each slot's resident buffers, or its running request's footprint */
+  size_t hbm_peak;      /* the largest hbm_in_use so far (never above hbm_budget) */
+  size_t hbm_budget;
+  size_t hbm_resident;  /* device bytes the slots' contexts hold now (measured) */
+  long long evictions;  /* idle contexts replaced by fresh ones to admit a request */
+  long long model_exceeded; /* solves that left more resident than their footprint model */
 } agrav_server_info;
 
+/* The server's footprint model: device bytes one solve of n particles adds to a fresh context (inputs,
+ * sorted copies, keys, octree, results, scratch).  A request is charged a fresh context's bytes plus this. */
+size_t agrav_solve_footprint(const agrav_params *p, int64_t n);
+/* AGRAV_ERR_CAPACITY when hbm_budget_bytes (0 = 64 GiB) cannot hold `slots` fresh contexts. */
 agrav_status agrav_server_create(const agrav_params *p, int slots, size_t hbm_budget_bytes, agrav_server **out);
+/* AGRAV_ERR_CAPACITY when the request's footprint, with every other slot at a fresh context, exceeds the budget. */
 agrav_status agrav_server_submit(agrav_server *s, const agrav_request *r, uint64_t *ticket);
 /* Blocks until the request is done; releases its ticket. */
 agrav_status agrav_server_wait(agrav_server *s, uint64_t ticket, agrav_request_result *res);
 agrav_status agrav_server_poll(agrav_server *s, uint64_t ticket, int *done);
 agrav_status agrav_server_stats(agrav_server *s, agrav_server_info *st);
-/* Finishes the queued requests, then stops the workers and frees the slots. */
+/* Finishes the queued requests (they stay waitable until then), then stops the workers and frees the slots. */
 void agrav_server_destroy(agrav_server *s);
 
 /* Standalone 63-bit Peano-Hilbert key of 21-bit integer coordinates (host side, for checks). */

@@ -161,6 +161,9 @@ def _declare(L):
     L.orc_gravity_timesteps.argtypes = [C.POINTER(TimestepParams), C.POINTER(Params), C.c_int, _dp, C.c_void_p, _dp,
                                         C.c_void_p, _dp, C.POINTER(C.c_double)]
     L.orc_gravity_timesteps.restype = C.c_int
+    u64 = C.c_uint64
+    L.orc_peano_hilbert_key.argtypes = [u64, u64, u64, C.c_int, C.POINTER(u64), C.POINTER(u64)]
+    L.orc_peano_hilbert_key_inverse.argtypes = [u64, u64, C.c_int, C.POINTER(u64), C.POINTER(u64), C.POINTER(u64)]
 
 
 def _pos(pos):
@@ -409,3 +412,18 @@ def fft3d(re, im, sign):
     n = re.shape[0]
     lib().orc_fft3d(n, re, im, sign)
     return re, im
+
+
+def peano_hilbert_key(x, y, z, bits):
+    """peano_hilbert_key (peano.c:374-390): the key of integer coordinates as a Python int."""
+    hi, lo = C.c_uint64(), C.c_uint64()
+    lib().orc_peano_hilbert_key(int(x), int(y), int(z), int(bits), C.byref(hi), C.byref(lo))
+    return (hi.value << 64) | lo.value
+
+
+def peano_hilbert_key_inverse(key, bits):
+    """peano_hilbert_key_inverse (peano.c:446-496)."""
+    x, y, z = C.c_uint64(), C.c_uint64(), C.c_uint64()
+    lib().orc_peano_hilbert_key_inverse(int(key) >> 64, int(key) & ((1 << 64) - 1), int(bits), C.byref(x), C.byref(y),
+                                        C.byref(z))
+    return x.value, y.value, z.value

@@ -1811,3 +1811,138 @@ int orc_gravity_timesteps(const orc_timestep_params *tp, const orc_params *p, in
     }
   return below;
 }
+
+/* ------------------------------------------------------------------ */
+/* Peano-Hilbert keys, peano.c:334-496                                  */
+/* ------------------------------------------------------------------ */
+
+/* The curve's tables, restated from peano.c: for the key, the next orientation
+ * (rottable3, :334-350) and the key digit (subpix3, :352-362) of each of 48
+ * orientations and each octant pix = x bit << 2 | y bit << 1 | z bit; for the
+ * inverse, the octant order of 24 rotations (quadrants, :393-424) with the
+ * rotation maps and the sense flips (:426-433). */
+static const unsigned char ph_next_rotation[384] = {
+  36, 28, 25, 27, 10, 10, 25, 27, 29, 11, 24, 24, 37, 11, 26, 26,
+  8, 8, 25, 27, 30, 38, 25, 27, 9, 39, 24, 24, 9, 31, 26, 26,
+  40, 24, 44, 32, 40, 6, 44, 6, 25, 7, 33, 7, 41, 41, 45, 45,
+  4, 42, 4, 46, 26, 42, 34, 46, 43, 43, 47, 47, 5, 27, 5, 35,
+  33, 35, 36, 28, 33, 35, 2, 2, 32, 32, 29, 3, 34, 34, 37, 3,
+  33, 35, 0, 0, 33, 35, 30, 38, 32, 32, 1, 39, 34, 34, 1, 31,
+  24, 42, 32, 46, 14, 42, 14, 46, 43, 43, 47, 47, 25, 15, 33, 15,
+  40, 12, 44, 12, 40, 26, 44, 34, 13, 27, 13, 35, 41, 41, 45, 45,
+  28, 41, 28, 22, 38, 43, 38, 22, 42, 40, 23, 23, 29, 39, 29, 39,
+  41, 36, 20, 36, 43, 30, 20, 30, 37, 31, 37, 31, 42, 40, 21, 21,
+  28, 18, 28, 45, 38, 18, 38, 47, 19, 19, 46, 44, 29, 39, 29, 39,
+  16, 36, 45, 36, 16, 30, 47, 30, 37, 31, 37, 31, 17, 17, 46, 44,
+  12, 4, 1, 3, 34, 34, 1, 3, 5, 35, 0, 0, 13, 35, 2, 2,
+  32, 32, 1, 3, 6, 14, 1, 3, 33, 15, 0, 0, 33, 7, 2, 2,
+  16, 0, 20, 8, 16, 30, 20, 30, 1, 31, 9, 31, 17, 17, 21, 21,
+  28, 18, 28, 22, 2, 18, 10, 22, 19, 19, 23, 23, 29, 3, 29, 11,
+  9, 11, 12, 4, 9, 11, 26, 26, 8, 8, 5, 27, 10, 10, 13, 27,
+  9, 11, 24, 24, 9, 11, 6, 14, 8, 8, 25, 15, 10, 10, 25, 7,
+  0, 18, 8, 22, 38, 18, 38, 22, 19, 19, 23, 23, 1, 39, 9, 39,
+  16, 36, 20, 36, 16, 2, 20, 10, 37, 3, 37, 11, 17, 17, 21, 21,
+  4, 17, 4, 46, 14, 19, 14, 46, 18, 16, 47, 47, 5, 15, 5, 15,
+  17, 12, 44, 12, 19, 6, 44, 6, 13, 7, 13, 7, 18, 16, 45, 45,
+  4, 42, 4, 21, 14, 42, 14, 23, 43, 43, 22, 20, 5, 15, 5, 15,
+  40, 12, 21, 12, 40, 6, 23, 6, 13, 7, 13, 7, 41, 41, 22, 20};
+static const unsigned char ph_digit[384] = {
+  0, 7, 1, 6, 3, 4, 2, 5, 7, 4, 6, 5, 0, 3, 1, 2,
+  4, 3, 5, 2, 7, 0, 6, 1, 3, 0, 2, 1, 4, 7, 5, 6,
+  1, 0, 6, 7, 2, 3, 5, 4, 0, 3, 7, 4, 1, 2, 6, 5,
+  3, 2, 4, 5, 0, 1, 7, 6, 2, 1, 5, 6, 3, 0, 4, 7,
+  6, 1, 7, 0, 5, 2, 4, 3, 1, 2, 0, 3, 6, 5, 7, 4,
+  2, 5, 3, 4, 1, 6, 0, 7, 5, 6, 4, 7, 2, 1, 3, 0,
+  7, 6, 0, 1, 4, 5, 3, 2, 6, 5, 1, 2, 7, 4, 0, 3,
+  5, 4, 2, 3, 6, 7, 1, 0, 4, 7, 3, 0, 5, 6, 2, 1,
+  6, 7, 5, 4, 1, 0, 2, 3, 7, 0, 4, 3, 6, 1, 5, 2,
+  0, 1, 3, 2, 7, 6, 4, 5, 1, 6, 2, 5, 0, 7, 3, 4,
+  2, 3, 1, 0, 5, 4, 6, 7, 3, 4, 0, 7, 2, 5, 1, 6,
+  4, 5, 7, 6, 3, 2, 0, 1, 5, 2, 6, 1, 4, 3, 7, 0,
+  7, 0, 6, 1, 4, 3, 5, 2, 0, 3, 1, 2, 7, 4, 6, 5,
+  3, 4, 2, 5, 0, 7, 1, 6, 4, 7, 5, 6, 3, 0, 2, 1,
+  6, 7, 1, 0, 5, 4, 2, 3, 7, 4, 0, 3, 6, 5, 1, 2,
+  4, 5, 3, 2, 7, 6, 0, 1, 5, 6, 2, 1, 4, 7, 3, 0,
+  1, 6, 0, 7, 2, 5, 3, 4, 6, 5, 7, 4, 1, 2, 0, 3,
+  5, 2, 4, 3, 6, 1, 7, 0, 2, 1, 3, 0, 5, 6, 4, 7,
+  0, 1, 7, 6, 3, 2, 4, 5, 1, 2, 6, 5, 0, 3, 7, 4,
+  2, 3, 5, 4, 1, 0, 6, 7, 3, 0, 4, 7, 2, 1, 5, 6,
+  1, 0, 2, 3, 6, 7, 5, 4, 0, 7, 3, 4, 1, 6, 2, 5,
+  7, 6, 4, 5, 0, 1, 3, 2, 6, 1, 5, 2, 7, 0, 4, 3,
+  5, 4, 6, 7, 2, 3, 1, 0, 4, 3, 7, 0, 5, 2, 6, 1,
+  3, 2, 0, 1, 4, 5, 7, 6, 2, 5, 1, 6, 3, 4, 0, 7};
+static const int ph_quadrant[192] = {
+  0, 7, 1, 6, 3, 4, 2, 5, 7, 4, 6, 5, 0, 3, 1, 2,
+  4, 3, 5, 2, 7, 0, 6, 1, 3, 0, 2, 1, 4, 7, 5, 6,
+  1, 0, 6, 7, 2, 3, 5, 4, 0, 3, 7, 4, 1, 2, 6, 5,
+  3, 2, 4, 5, 0, 1, 7, 6, 2, 1, 5, 6, 3, 0, 4, 7,
+  6, 1, 7, 0, 5, 2, 4, 3, 1, 2, 0, 3, 6, 5, 7, 4,
+  2, 5, 3, 4, 1, 6, 0, 7, 5, 6, 4, 7, 2, 1, 3, 0,
+  7, 6, 0, 1, 4, 5, 3, 2, 6, 5, 1, 2, 7, 4, 0, 3,
+  5, 4, 2, 3, 6, 7, 1, 0, 4, 7, 3, 0, 5, 6, 2, 1,
+  6, 7, 5, 4, 1, 0, 2, 3, 7, 0, 4, 3, 6, 1, 5, 2,
+  0, 1, 3, 2, 7, 6, 4, 5, 1, 6, 2, 5, 0, 7, 3, 4,
+  2, 3, 1, 0, 5, 4, 6, 7, 3, 4, 0, 7, 2, 5, 1, 6,
+  4, 5, 7, 6, 3, 2, 0, 1, 5, 2, 6, 1, 4, 3, 7, 0};
+static const int ph_rotx_map[24] = {
+  4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15,
+  0, 1, 2, 3, 17, 18, 19, 16, 23, 20, 21, 22};
+static const int ph_roty_map[24] = {
+  1, 2, 3, 0, 16, 17, 18, 19, 11, 8, 9, 10,
+  22, 23, 20, 21, 14, 15, 12, 13, 4, 5, 6, 7};
+static const int ph_rotx[8] = {
+  3, 0, 0, 2, 2, 0, 0, 1};
+static const int ph_roty[8] = {
+  0, 1, 1, 2, 2, 3, 3, 0};
+static const int ph_sense[8] = {
+  -1, -1, -1, 1, 1, -1, -1, -1};
+
+/* peano_hilbert_key, peano.c:374-390: `bits` levels, most significant first;
+ * the key (up to 3 x 42 bits) is returned as its high and low 64-bit words. */
+void orc_peano_hilbert_key(uint64_t x, uint64_t y, uint64_t z, int bits, uint64_t *hi, uint64_t *lo)
+{
+  unsigned __int128 key = 0;
+  int rotation          = 0;
+  for(int l = bits - 1; l >= 0; l--)
+    {
+      const int pix = (int)((((x >> l) & 1) << 2) | (((y >> l) & 1) << 1) | ((z >> l) & 1));
+      key           = (key << 3) | ph_digit[8 * rotation + pix];
+      rotation      = ph_next_rotation[8 * rotation + pix];
+    }
+  *hi = (uint64_t)(key >> 64);
+  *lo = (uint64_t)key;
+}
+
+/* peano_hilbert_key_inverse, peano.c:446-496: the coordinates of a key, from
+ * the quadrant tables (a second description of the same curve). */
+void orc_peano_hilbert_key_inverse(uint64_t hi, uint64_t lo, int bits, uint64_t *x, uint64_t *y, uint64_t *z)
+{
+  int inv[24][8][3];
+  for(int r = 0; r < 24; r++)
+    for(int b = 0; b < 8; b++)
+      {
+        const int q = ph_quadrant[8 * r + b];
+        inv[r][q][0] = (b >> 2) & 1;
+        inv[r][q][1] = (b >> 1) & 1;
+        inv[r][q][2] = b & 1;
+      }
+  const unsigned __int128 key = ((unsigned __int128)hi << 64) | lo;
+  int rotation = 0, sense = 1;
+  uint64_t ox = 0, oy = 0, oz = 0;
+  for(int l = bits - 1; l >= 0; l--)
+    {
+      const int part = (int)((key >> (3 * l)) & 7);
+      const int quad = sense == 1 ? part : 7 - part;
+      ox             = (ox << 1) | (uint64_t)inv[rotation][quad][0];
+      oy             = (oy << 1) | (uint64_t)inv[rotation][quad][1];
+      oz             = (oz << 1) | (uint64_t)inv[rotation][quad][2];
+      sense *= ph_sense[quad];
+      for(int k = 0; k < ph_rotx[quad]; k++)
+        rotation = ph_rotx_map[rotation];
+      for(int k = 0; k < ph_roty[quad]; k++)
+        rotation = ph_roty_map[rotation];
+    }
+  *x = ox;
+  *y = oy;
+  *z = oz;
+}

@@ -160,6 +160,11 @@ int orc_gravity_timesteps(const orc_timestep_params *tp, const orc_params *p, in
 /* Textbook in-place complex FFT of a cube (n^3, n power of two); sign -1 forward (FFTW_FORWARD), +1 backward; unnormalised */
 void orc_fft3d(int n, double *re, double *im, int sign);
 
+/* peano_hilbert_key (peano.c:374-390): the `bits`-level key (bits <= 42) of integer
+ * coordinates, as high and low 64-bit words; and its inverse (peano.c:446-496). */
+void orc_peano_hilbert_key(uint64_t x, uint64_t y, uint64_t z, int bits, uint64_t *hi, uint64_t *lo);
+void orc_peano_hilbert_key_inverse(uint64_t hi, uint64_t lo, int bits, uint64_t *x, uint64_t *y, uint64_t *z);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,4 +1,13 @@
-"""GPU tests at the full sizes of the two single-set workloads besides the 128^3 benchmark.
+"""GPU tests at the full sizes of the single-set workloads.
+
+* 128^3 = 2,097,152 clustered particles (half uniform, half in 256 Hernquist
+  spheres) in the periodic unit box, TreePM with PMGRID = 128, ASMTH 1.25,
+  RCUT 4.5, ErrTolForceAcc 0.0025, softening 1/40 of the mean spacing (the
+  benchmark's workload).  OldAcc comes from a Barnes-Hut seeding pass.  The
+  FP64 plain walk + PM equals the oracle (its tree walk + PM) on 16,384
+  seeded targets; the FP32 group walk's error against the FP64 Ewald direct
+  sum (FORCETEST) is no worse than the oracle's, in total and in its tree
+  (short-range) part; the PM part of the error is printed beside it.
 
 * 32^3 = 32,768 uniform particles in the periodic unit box, tree-only with the
   Ewald correction, relative criterion (ErrTolForceAcc 0.0025) seeded by a
@@ -23,6 +32,62 @@ pytestmark = pytest.mark.gpu
 
 def _stats(e):
     return float(np.median(e)), float(np.percentile(e, 99))
+
+
+def test_clustered_128cubed_treepm(agrav, oracle):
+    from arepo_amd import ics
+    pos, mass, eps = ics.clustered_box(128 ** 3, seed=2)
+    n = len(pos)
+    sel = np.sort(np.random.Generator(np.random.Philox(99)).choice(n, 16384, replace=False)).astype(np.int32)
+    kw = dict(periodic=True, box_size=1.0, pmgrid=128, asmth=1.25, rcut=4.5, soft=(eps,), err_tol_theta=0.7,
+              err_tol_force_acc=0.0025, opening=1)
+    # OldAcc from the Barnes-Hut seeding pass (accel.c:62-72) of the FP64 per-target walk
+    with agrav.Gravity(agrav.make_params(**kw, precision=agrav.FP64, walk=agrav.WALK_PLAIN)) as g:
+        g.set_particles(pos, mass)
+        g.construct_forcetree()
+        g.long_range_force()
+        g.gravity_tree(use_bh=True)
+        g.gravity_force_finalize()
+        oa = g.get_accelerations()["old_acc"]
+        g.set_old_acc(oa)
+        g.compute_grav_accelerations(first_step=False)
+        r64 = g.get_accelerations()
+        c64 = g.get_components(tree=True, pm=True)
+        ft = g.gravity_forcetest(sel)
+    with agrav.Gravity(agrav.make_params(**kw, precision=agrav.FP32, walk=agrav.WALK_GROUP, leaf_bucket=4,
+                                         eval_potential=False)) as g:
+        g.set_particles(pos, mass)
+        g.set_old_acc(oa)
+        g.compute_grav_accelerations(first_step=False)
+        r32 = g.get_accelerations()
+        c32 = g.get_components(tree=True, pm=True)
+    # the oracle: the reference's PM and its relative walk of the selected targets with the same OldAcc
+    op = oracle.make_params(**kw)
+    acc_pm_o, _ = oracle.pm_periodic(op, pos, mass)
+    acc_t_o, _, nint_o, _ = oracle.Tree(op, pos, mass).walk(False, old_acc=oa, targets=sel, nthreads=16)
+    acc_o = acc_t_o + acc_pm_o[sel]
+    e = rel_err(r64["acc"][sel], acc_o)
+    assert np.percentile(e, 99.9) < 1e-8, (np.percentile(e, 99.9), e.max())
+    assert np.mean(r64["nint"][sel] == nint_o) > 0.999
+    assert np.percentile(rel_err(c64["acc_pm"][sel], acc_pm_o[sel]), 99.9) < 1e-8
+    # accuracy against the FP64 Ewald direct sum, in total and split: the tree part against the exact
+    # short-range sum, the PM part against the long-range rest (gravtree_forcetest.c:629-638), both
+    # relative to the total acceleration
+    a_norm = np.linalg.norm(ft["acc"], axis=1)
+
+    def split(acc_tree, acc_pm):
+        e_tree = np.linalg.norm(acc_tree - ft["acc_short"], axis=1) / a_norm
+        e_pm = np.linalg.norm(acc_pm - (ft["acc"] - ft["acc_short"]), axis=1) / a_norm
+        return _stats(e_tree), _stats(e_pm)
+
+    e_or, e_32 = rel_err(acc_o, ft["acc"]), rel_err(r32["acc"][sel], ft["acc"])
+    (t_or, pm_or), (t_32, pm_32) = split(acc_t_o, acc_pm_o[sel]), split(c32["acc_tree"][sel], c32["acc_pm"][sel])
+    print("\n128^3 TreePM: oracle median %.3g p99 %.3g (tree %.3g/%.3g, pm %.3g/%.3g) | FP32 group median %.3g p99 "
+          "%.3g (tree %.3g/%.3g, pm %.3g/%.3g)" % (*_stats(e_or), *t_or, *pm_or, *_stats(e_32), *t_32, *pm_32))
+    assert np.median(e_32) <= np.median(e_or) * 1.05 and np.percentile(e_32, 99) <= np.percentile(e_or, 99) * 1.05
+    assert t_32[0] <= t_or[0] * 1.05 and t_32[1] <= t_or[1] * 1.05
+    # the PM part is the reference's mesh force at FP32 on the mesh: it matches the oracle's
+    assert pm_32[1] <= pm_or[1] * 1.05
 
 
 def test_uniform_32cubed_periodic_ewald(agrav, oracle):

@@ -255,3 +255,43 @@ def test_clump_below_level_21_matches_oracle(agrav, oracle, nclump):
     e = rel_err(r["acc"], o["acc"])
     assert np.percentile(e, 99.9) < 1e-8, float(e.max())
     assert np.mean(r["nint"] == o["nint"]) > 0.999
+
+
+def test_rejected_set_particles_state(agrav):
+    """A set_particles rejected by its host-side checks leaves the context as it was
+    (the old set and its results stay valid); one that fails after the old set was
+    given up (a softening type beyond the table found on the device copy) leaves the
+    context without particles: later calls report a state error instead of reading
+    a mix of old and new buffers.  A good set afterwards works as on a fresh context."""
+    pos, mass = _clumped(3000, 21, periodic=True)
+    p = agrav.make_params(periodic=True, box_size=1.0, pmgrid=16, soft=(0.002, 0.004), precision=agrav.FP32,
+                          walk=agrav.WALK_GROUP)
+    bad = np.zeros(len(pos) + 500, dtype=np.uint8)
+    bad[7] = 5  # beyond the two softening types
+    pos2 = np.random.Generator(np.random.Philox(22)).random((len(pos) + 500, 3))
+    m2 = np.full(len(pos2), 1.0 / len(pos2))
+    dev_buf = _DeviceBuffers()
+    try:
+        with agrav.Gravity(p) as g:
+            g.set_particles(pos, mass)
+            g.compute_grav_accelerations(first_step=True)
+            first = g.get_accelerations()
+            with pytest.raises(agrav.AgravError) as ei:
+                g.set_particles(pos2, m2, softtype=bad)
+            assert ei.value.code == agrav.ERR_BAD_PARAM
+            kept = g.get_accelerations()
+            assert np.array_equal(kept["acc"], first["acc"])
+            with pytest.raises(agrav.AgravError) as ei:
+                g.set_particles_device(len(pos2), dev_buf.put(pos2), dev_buf.put(m2), dev_buf.put(bad))
+            assert ei.value.code == agrav.ERR_BAD_PARAM
+            for call in (lambda: g.compute_grav_accelerations(first_step=True), g.construct_forcetree,
+                         g.long_range_force, g.get_accelerations):
+                with pytest.raises(agrav.AgravError) as ei:
+                    call()
+                assert ei.value.code == agrav.ERR_STATE
+            g.set_particles(pos, mass)
+            g.compute_grav_accelerations(first_step=True)
+            again = g.get_accelerations()
+    finally:
+        dev_buf.free()
+    assert np.array_equal(again["acc"], first["acc"])
