@@ -176,28 +176,29 @@ size_t sort_temp_bytes(int64_t n);
 agrav_status radix_sort_u64(uint64_t *keys_in, int *vals_in, uint64_t *keys_out, int *vals_out, int64_t n, int key_bits,
                             void *d_temp, int *d_err, hipStream_t s, int lo_bit = 0);
 
-// 3-D Hilbert index of Skilling's transpose method ("Programming the Hilbert
-// curve", AIP Conf. Proc. 707, 381 (2004)) on b-bit coordinates, evaluated as a
-// state machine: each level's key digit depends on that level's octant
-// (x bit << 2 | y bit << 1 | z bit) and on an orientation state carried down from
-// the levels above.  Entry [8 s + o] = (next state << 3) | digit, start state 0;
-// the 24 states and their transitions were derived from the transpose method
-// (arepo_amd/gen_hilbert_table.py, which also checks the table against it for
-// b = 1..30).  A key's top 3L bits are the key of the coordinates' top L bits.
+// AREPO's 3-D Peano-Hilbert key (peano_hilbert_key, peano.c:374-390) on b-bit
+// coordinates, evaluated as a state machine: each level's key digit depends on
+// that level's octant (x bit << 2 | y bit << 1 | z bit) and on an orientation
+// state carried down from the levels above.  Entry [8 s + o] = (next state << 3)
+// | digit, start state 0.  The 24 states are those of the reference's curve
+// reachable from its start orientation; the table is derived from the curve's
+// quadrant description (peano.c:393-433) by arepo_amd/gen_hilbert_table.py, and
+// the keys are checked against the oracle's restatement of peano_hilbert_key.
+// A key's top 3L bits are the key of the coordinates' top L bits.
 #define AG_HILBERT_TABLE                                                                                          \
   {                                                                                                              \
-      8,  17,  27,   2,  39,  46,  52,   5,  56,  71,  73,  86,  91,  20,  10,  13, \
-     48,   1, 103, 110, 115,  18,  12,  21, 126, 129,  29,  26,  79,  80, 140,   3, \
-    148,  43,  37,  34, 127, 128,  78,  81, 156,  45,  35,  42,  31,   6, 160, 105, \
-     72,  87, 139,   4,  57,  70,  50,  53,   0, 171, 111,  76,  49,  58, 102,  61, \
-    180, 143,  83, 184,  69,  54,  66,  97,  16, 123,   9,  74,  47,  60,  38,  77, \
-    132,  95,  85,  14,  67, 144,  82,  33, 142,  55, 185,  96,  93, 116,  90,  11, \
-    188, 107, 175, 176, 101,  98,  62,  65, 164, 109, 119,  22,  99, 106, 152,  41, \
-    174, 177,  63,  64, 117, 114,  92,  19,  30, 125, 161, 122,   7, 172, 104,  75, \
-    130,  25, 133, 166, 179, 136,  84, 191,  94,  15, 141,  28, 145,  32, 138,  51, \
-    146, 155, 149,  36, 137,  24, 190, 167, 154, 157, 147,  44, 169, 182, 120, 135, \
-    162, 165, 121, 134, 187, 108, 168, 183, 118, 173,  23, 124, 153, 170,  40,  59, \
-    178, 113, 131,  88, 181, 158,  68, 151, 186, 163,  89, 112, 189, 100, 150, 159 \
+      8,  23,  25,  38,  43,  44,  26,  37,   0,  49,  63,  70,  75,  50,  76,  69, \
+     86,   7,  89,  56,  85, 100,  90,  99, 104, 115,   1,   2, 127, 116, 134, 133, \
+    140, 151,   5,   6, 139, 152, 130, 129, 138, 117,   3,   4, 137, 118,  96,  79, \
+    160,   9,  91,  10, 175, 102,  92, 101, 142, 113,  15,  16, 141, 114, 132, 131, \
+     84,  13, 183,  14,  83,  98, 184,  97,  82,  11,  93,  12,  81, 128,  94,  47, \
+     22, 167,  21,  68,  73, 168,  74,  67,  18,  51,  17, 176,  77,  52,  78, 191, \
+     20,  53,  19,  66, 135,  54,  40,  65,  24, 155, 143, 156, 161, 162, 182, 181, \
+     58,  57,  27, 144,  45,  46,  28, 159, 174, 173, 185, 186,  31, 148, 136, 147, \
+     60,  59,  29,  34, 103,  72,  30,  33,  62,  61, 111,  36,  41,  42, 120,  35, \
+    124,  39, 123, 112, 165, 166, 178, 177, 170, 169, 189, 190, 107,  32, 108, 119, \
+     48,  87, 187, 188, 105, 150, 106, 149, 126, 153, 125, 154,  55,  80, 180, 179, \
+    172, 171,  71,  88, 109, 146, 110, 145, 122, 157, 121, 158, 163, 164,  64,  95 \
   }
 
 // digits of the L-level key of L-bit coordinates: the last min(L, 21) levels in

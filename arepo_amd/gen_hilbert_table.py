@@ -1,74 +1,75 @@
-"""Derive the 24-state table of the library's 3-D Hilbert curve (run: python -m arepo_amd.gen_hilbert_table).
+"""Derive the 24-state table of AREPO's 3-D Peano-Hilbert curve (run: python -m arepo_amd.gen_hilbert_table).
 
-The curve is the one of Skilling's transpose method ("Programming the Hilbert curve", AIP Conf. Proc. 707, 381
-(2004)) on b-bit coordinates: undo excess work, Gray-encode, interleave MSB first.  Each level's key digit
-depends only on that level's octant and on an orientation state carried down from the levels above; this
-script finds the states by evaluating the transpose method on cells of increasing depth, checks that the
-transitions do not depend on the representative cell or on b, and prints the table the library uses:
-entry [8 s + o] = (next state << 3) | digit, start state 0.
+AREPO describes its curve twice in peano.c: by orientation/digit tables for the key (peano_hilbert_key,
+:334-390) and by the octant order of 24 rotations, two rotation maps and a sense flip for the inverse
+(peano_hilbert_key_inverse, :393-496).  This script starts from the second description, restated below:
+an orientation state is a (rotation, sense) pair; at each level the octant o = x bit << 2 | y bit << 1 | z bit
+falls in quadrant q = QUADRANT[rotation][o], the key digit is q (sense +1) or 7 - q (sense -1), and the
+rotation is stepped ROTX[q] times through ROTX_MAP and ROTY[q] times through ROTY_MAP while the sense takes
+the factor SENSE[q].  The states reachable from (0, +1) are numbered in breadth-first order (start state 0),
+and the table the library uses is printed: entry [8 s + o] = (next state << 3) | digit, one byte each.
+
+The library's keys are checked against the oracle's restatement of peano_hilbert_key (the first
+description) by tests/test_abi.py; the oracle's key and inverse are checked against each other by
+tests/test_oracle.py.
 """
-import itertools
 import random
 
+# peano.c:393-424, quadrants[24][2][2][2], flattened as [rotation][x bit][y bit][z bit]
+QUADRANT = [
+    0, 7, 1, 6, 3, 4, 2, 5, 7, 4, 6, 5, 0, 3, 1, 2, 4, 3, 5, 2, 7, 0, 6, 1, 3, 0, 2, 1, 4, 7, 5, 6,
+    1, 0, 6, 7, 2, 3, 5, 4, 0, 3, 7, 4, 1, 2, 6, 5, 3, 2, 4, 5, 0, 1, 7, 6, 2, 1, 5, 6, 3, 0, 4, 7,
+    6, 1, 7, 0, 5, 2, 4, 3, 1, 2, 0, 3, 6, 5, 7, 4, 2, 5, 3, 4, 1, 6, 0, 7, 5, 6, 4, 7, 2, 1, 3, 0,
+    7, 6, 0, 1, 4, 5, 3, 2, 6, 5, 1, 2, 7, 4, 0, 3, 5, 4, 2, 3, 6, 7, 1, 0, 4, 7, 3, 0, 5, 6, 2, 1,
+    6, 7, 5, 4, 1, 0, 2, 3, 7, 0, 4, 3, 6, 1, 5, 2, 0, 1, 3, 2, 7, 6, 4, 5, 1, 6, 2, 5, 0, 7, 3, 4,
+    2, 3, 1, 0, 5, 4, 6, 7, 3, 4, 0, 7, 2, 5, 1, 6, 4, 5, 7, 6, 3, 2, 0, 1, 5, 2, 6, 1, 4, 3, 7, 0,
+]
+# peano.c:426-433: rotation maps, rotation steps per quadrant, sense factor per quadrant
+ROTX_MAP = [4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 0, 1, 2, 3, 17, 18, 19, 16, 23, 20, 21, 22]
+ROTY_MAP = [1, 2, 3, 0, 16, 17, 18, 19, 11, 8, 9, 10, 22, 23, 20, 21, 14, 15, 12, 13, 4, 5, 6, 7]
+ROTX = [3, 0, 0, 2, 2, 0, 0, 1]
+ROTY = [0, 1, 1, 2, 2, 3, 3, 0]
+SENSE = [-1, -1, -1, +1, +1, -1, -1, -1]
 
-def transpose_key(x, y, z, b):
-    X = [x, y, z]
-    Q = 1 << (b - 1)
-    while Q > 1:
-        P = Q - 1
-        for i in range(3):
-            if X[i] & Q:
-                X[0] ^= P
-            else:
-                t = (X[0] ^ X[i]) & P
-                X[0] ^= t
-                X[i] ^= t
-        Q >>= 1
-    X[1] ^= X[0]
-    X[2] ^= X[1]
-    t = 0
-    Q = 1 << (b - 1)
-    while Q > 1:
-        if X[2] & Q:
-            t ^= Q - 1
-        Q >>= 1
-    X = [v ^ t for v in X]
-    k = 0
+
+def step(state, octant):
+    """One level of the curve: (digit, next state) of (rotation, sense) for octant x<<2 | y<<1 | z."""
+    rotation, sense = state
+    q = QUADRANT[8 * rotation + octant]
+    digit = q if sense == 1 else 7 - q
+    for _ in range(ROTX[q]):
+        rotation = ROTX_MAP[rotation]
+    for _ in range(ROTY[q]):
+        rotation = ROTY_MAP[rotation]
+    return digit, (rotation, sense * SENSE[q])
+
+
+def quadrant_key(x, y, z, b):
+    """The b-level key walked directly through the (rotation, sense) states."""
+    state, k = (0, 1), 0
     for l in range(b - 1, -1, -1):
-        k = (k << 3) | (((X[0] >> l) & 1) << 2) | (((X[1] >> l) & 1) << 1) | ((X[2] >> l) & 1)
+        d, state = step(state, (((x >> l) & 1) << 2) | (((y >> l) & 1) << 1) | ((z >> l) & 1))
+        k = (k << 3) | d
     return k
 
 
-def _digit_map(prefix, b):
-    """Key digit at level len(prefix)+1 of each octant inside the cell reached by `prefix` (octants from the top)."""
-    out = []
-    for o in range(8):
-        x = y = z = 0
-        for j, q in enumerate(list(prefix) + [o]):
-            sh = b - 1 - j
-            x |= ((q >> 2) & 1) << sh
-            y |= ((q >> 1) & 1) << sh
-            z |= (q & 1) << sh
-        out.append((transpose_key(x, y, z, b) >> (3 * (b - 1 - len(prefix)))) & 7)
-    return tuple(out)
-
-
-def derive(b=30, depth=4):
-    ids, rep = {}, {}
-    for d in range(depth + 1):
-        for pre in itertools.product(range(8), repeat=d):
-            m = _digit_map(pre, b)
-            if m not in ids:
-                ids[m] = len(ids)
-                rep[ids[m]] = list(pre)
-    digit = {s: m for m, s in ids.items()}
-    nxt = {s: [ids[_digit_map(rep[s] + [o], b)] for o in range(8)] for s in digit}
-    for d in range(depth):  # every representative agrees
-        for pre in itertools.product(range(8), repeat=d):
-            s = ids[_digit_map(pre, b)]
-            assert all(ids[_digit_map(list(pre) + [o], b)] == nxt[s][o] for o in range(8))
-    assert ids[_digit_map([], b)] == 0
-    return [(nxt[s][o] << 3) | digit[s][o] for s in range(len(digit)) for o in range(8)]
+def derive():
+    ids, order = {(0, 1): 0}, [(0, 1)]
+    i = 0
+    while i < len(order):
+        for o in range(8):
+            _, nxt = step(order[i], o)
+            if nxt not in ids:
+                ids[nxt] = len(order)
+                order.append(nxt)
+        i += 1
+    assert len(order) <= 32, "states must fit 5 bits"
+    tab = []
+    for s in order:
+        for o in range(8):
+            d, nxt = step(s, o)
+            tab.append((ids[nxt] << 3) | d)
+    return tab
 
 
 def table_key(tab, x, y, z, b):
@@ -83,9 +84,9 @@ def table_key(tab, x, y, z, b):
 if __name__ == "__main__":
     tab = derive()
     rng = random.Random(1)
-    for b in range(1, 31):
+    for b in range(1, 43):
         for _ in range(200):
             x, y, z = (rng.getrandbits(b) for _ in range(3))
-            assert table_key(tab, x, y, z, b) == transpose_key(x, y, z, b)
-    print("%d states, identical to the transpose method for b = 1..30" % (len(tab) // 8))
+            assert table_key(tab, x, y, z, b) == quadrant_key(x, y, z, b)
+    print("%d states reachable; the packed table equals the state walk for b = 1..42" % (len(tab) // 8))
     print(",\n".join("    " + ", ".join("%3d" % v for v in tab[i:i + 16]) for i in range(0, len(tab), 16)))

@@ -97,20 +97,23 @@ def test_hilbert_key_prefix_property(agrav):
         assert ka == kb
 
 
-def test_hilbert_table_equals_transpose_method(agrav):
-    # the library evaluates the curve with a 24-state table; Skilling's transpose method, written out
-    # independently in the table's generator, gives the same key for every point
-    from arepo_amd.gen_hilbert_table import derive, table_key, transpose_key
+def test_hilbert_key_equals_peano_hilbert_key(agrav, oracle):
+    # the library's 24-state table (derived from the curve's quadrant description) gives the reference's
+    # peano_hilbert_key (peano.c:374-390), restated in the oracle from its orientation/digit tables
     g = np.random.Generator(np.random.Philox(3))
-    pts = g.integers(0, 1 << 21, (2000, 3))
-    pts[:8] = [[0, 0, 0], [(1 << 21) - 1] * 3, [1, 0, 0], [0, 1, 0], [0, 0, 1], [1 << 20, 0, 0], [0, 1 << 20, 0],
-               [0, 0, 1 << 20]]
+    pts = g.integers(0, 1 << 21, (10000, 3))
+    m = (1 << 21) - 1
+    pts[:8] = [[0, 0, 0], [m, m, m], [1, 0, 0], [0, 1, 0], [0, 0, 1], [1 << 20, 0, 0], [0, 1 << 20, 0], [0, 0, 1 << 20]]
     for x, y, z in pts:
-        assert agrav.hilbert_key21(x, y, z) == transpose_key(int(x), int(y), int(z), 21)
-    # the generator's own table, for the deeper levels of the 30-level keys
-    tab = derive(depth=3)
-    for x, y, z in g.integers(0, 1 << 30, (300, 3)):
-        assert table_key(tab, int(x), int(y), int(z), 30) == transpose_key(int(x), int(y), int(z), 30)
+        assert agrav.hilbert_key21(x, y, z) == oracle.peano_hilbert_key(x, y, z, 21)
+    # the generator's packed table for the 30 levels of the octree keys, and the first 30 of the 42 levels
+    # of AREPO's domain keys (BITS_PER_DIMENSION, allvars.h:370-372)
+    from arepo_amd.gen_hilbert_table import derive, table_key
+    tab = derive()
+    for x, y, z in g.integers(0, 1 << 42, (500, 3), dtype=np.int64):
+        k42 = oracle.peano_hilbert_key(x, y, z, 42)
+        assert table_key(tab, int(x) >> 12, int(y) >> 12, int(z) >> 12, 30) == k42 >> 36
+        assert table_key(tab, int(x), int(y), int(z), 42) == k42
 
 
 def test_hilbert_key_fits_63_bits(agrav):

@@ -55,14 +55,14 @@ def test_tree_matches_oracle(agrav, oracle, periodic):
     assert np.all(np.diff(keys.astype(np.uint64)) >= 0)
     assert np.array_equal(np.sort(perm), np.arange(len(pos)))
     if periodic:
-        # unit box: the domain cube is 1.00001 box around the box centre (domain_box.c:161-168) and the 52-bit
-        # integer coordinate is the mantissa of (x - corner) / len + 1 (forcetree.c:493-505); the first key word
-        # holds the curve's top 21 levels, the host's key of the coordinates' top 21 bits
+        # unit box: the domain cube is 1.00001 box around the box centre (domain_box.c:161-168); AREPO's
+        # domain key is peano_hilbert_key of the top 42 mantissa bits of (x - corner) / len + 1
+        # (domain_toplevel.c:121-126, domain.c:361-370); the first key word holds its top 21 levels
         ln = 1.0 * 1.00001
         corner = 0.5 * (0.0 + 1.0) - 0.5 * ln
         v = (np.mod(pos[perm], 1.0) - corner) * (1.0 / ln) + 1.0
-        ic = (v.view(np.uint64) & np.uint64((1 << 52) - 1)) >> np.uint64(31)
-        want = np.array([agrav.hilbert_key21(int(a), int(b), int(c)) for a, b, c in ic], dtype=np.uint64)
+        ic = (v.view(np.uint64) & np.uint64((1 << 52) - 1)) >> np.uint64(10)
+        want = np.array([oracle.peano_hilbert_key(int(a), int(b), int(c), 42) >> 63 for a, b, c in ic], dtype=np.uint64)
         assert np.array_equal(keys.astype(np.uint64), want)
     ig = _tree_key(tg["center"], tg["len"])
     io = _tree_key(to["center"], to["len"])

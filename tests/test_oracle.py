@@ -255,3 +255,29 @@ def test_orc_compute_first_step_seeds_old_acc(oracle):
     r = oracle.compute_grav_accelerations(p, pos, mass, first_step=True)
     # OldAcc is |a_tree + a_PM/G| (G = 1 here, so |acc|)
     np.testing.assert_allclose(r["old_acc"], np.linalg.norm(r["acc"], axis=1), rtol=1e-6)
+
+
+def test_peano_hilbert_key_inverse_round_trip(oracle):
+    """The oracle's peano_hilbert_key (orientation/digit tables, peano.c:334-390) and its inverse
+    (quadrant tables, peano.c:393-496) describe one curve: the inverse of every key is its cell, for
+    every cell of 1-3 levels and for random 42-bit coordinates; the keys of a level are a bijection onto
+    0 .. 8^L - 1 and consecutive keys are face neighbours (the Hilbert property)."""
+    for L in (1, 2, 3):
+        n = 1 << L
+        keys = {}
+        for x in range(n):
+            for y in range(n):
+                for z in range(n):
+                    k = oracle.peano_hilbert_key(x, y, z, L)
+                    assert oracle.peano_hilbert_key_inverse(k, L) == (x, y, z)
+                    keys[k] = (x, y, z)
+        assert sorted(keys) == list(range(8 ** L))
+        cells = np.array([keys[k] for k in range(8 ** L)])
+        assert np.all(np.abs(np.diff(cells, axis=0)).sum(1) == 1)
+    g = np.random.Generator(np.random.Philox(8))
+    for x, y, z in g.integers(0, 1 << 42, (2000, 3), dtype=np.int64):
+        k = oracle.peano_hilbert_key(x, y, z, 42)
+        assert k < (1 << 126)
+        assert oracle.peano_hilbert_key_inverse(k, 42) == (int(x), int(y), int(z))
+    # the first level: subpix3[0] of peano.c:352 orders the octants x<<2|y<<1|z as 0, 7, 1, 6, 3, 4, 2, 5
+    assert [oracle.peano_hilbert_key(o >> 2, (o >> 1) & 1, o & 1, 1) for o in range(8)] == [0, 7, 1, 6, 3, 4, 2, 5]
