@@ -21,6 +21,7 @@ OK, ERR_BAD_PARAM, ERR_CAPACITY, ERR_OUT_OF_BOX, ERR_HIP, ERR_OOM, ERR_STATE, ER
 FP64, FP32 = 0, 1
 WALK_PLAIN, WALK_GROUP = 0, 1
 MAX_SOFT = 8
+DIRECT_NOALLOC_MAX_N = 16384  # AGRAV_DIRECT_NOALLOC_MAX_N
 
 
 class AgravError(RuntimeError):
@@ -96,7 +97,7 @@ EXPORTED = [
     "agrav_set_partial_tree",
     "agrav_construct_forcetree", "agrav_gravity_tree", "agrav_long_range_force", "agrav_gravity_force_finalize",
     "agrav_compute_grav_accelerations", "agrav_get_accelerations", "agrav_get_accelerations_device",
-    "agrav_get_components", "agrav_gravity_forcetest", "agrav_gravity_direct", "agrav_get_timings", "agrav_get_tree",
+    "agrav_get_components", "agrav_gravity_forcetest", "agrav_gravity_direct", "agrav_gravity_direct_noalloc", "agrav_get_timings", "agrav_get_tree",
     "agrav_get_order", "agrav_get_tables", "agrav_hilbert_key21",
     "agrav_batch_set_particles", "agrav_batch_autotune", "agrav_batch_get_tuning", "agrav_batch_set_tuning",
     "agrav_subfind_unbind", "agrav_gravity_timesteps", "agrav_server_create", "agrav_server_submit", "agrav_server_wait", "agrav_server_poll", "agrav_server_stats",
@@ -203,6 +204,7 @@ def _declare(L):
                                           C.POINTER(C.c_int64)]
     L.agrav_subfind_unbind.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp, C.POINTER(UnbindParams), vp, vp,
                                        vp, vp, vp, vp]
+    L.agrav_gravity_direct_noalloc.argtypes = [vp, C.c_int64, vp, vp, vp, vp, vp, vp]
     L.agrav_server_create.argtypes = [C.POINTER(Params), C.c_int, C.c_size_t, C.POINTER(vp)]
     L.agrav_solve_footprint.restype = C.c_size_t
     L.agrav_solve_footprint.argtypes = [C.POINTER(Params), C.c_int64]
@@ -449,6 +451,10 @@ class Gravity:
         pot = np.zeros(self.n)
         self._check(self.L.agrav_gravity_direct(self.h, _ptr(acc), _ptr(pot)))
         return acc, pot
+
+    def gravity_direct_noalloc(self, n, d_pos, d_mass, d_softtype, d_acc, d_pot, stream=None):
+        """agrav_gravity_direct_noalloc on device pointers (asynchronous on the context's stream by default)."""
+        self._check(self.L.agrav_gravity_direct_noalloc(self.h, int(n), d_pos, d_mass, d_softtype, d_acc, d_pot, stream))
 
     def timings(self):
         t = Timings()

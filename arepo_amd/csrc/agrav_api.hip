@@ -376,6 +376,13 @@ agrav_status agrav_create(const agrav_params *p, agrav_ctx **out)
       agrav_destroy(c);
       return st != AGRAV_OK ? st : AGRAV_ERR_HIP;
     }
+  // the device address of the fixed scratch array of agrav_gravity_direct_noalloc
+  if((st = noalloc_scratch_init(c)) != AGRAV_OK)
+    {
+      set_error_msg(c->err);
+      agrav_destroy(c);
+      return st;
+    }
   *out = c;
   return AGRAV_OK;
 }
@@ -905,6 +912,14 @@ agrav_status agrav_gravity_direct(agrav_ctx *c, double *acc, double *pot)
   if(!c)
     return AGRAV_ERR_BAD_PARAM;
   return direct_all(c, acc, pot);
+}
+
+agrav_status agrav_gravity_direct_noalloc(agrav_ctx *c, int64_t n, const double *d_pos, const double *d_mass,
+                                          const uint8_t *d_softtype, double *d_acc, double *d_pot, void *stream)
+{
+  if(!c)
+    return AGRAV_ERR_BAD_PARAM;
+  return direct_noalloc(c, n, d_pos, d_mass, d_softtype, d_acc, d_pot, stream ? (hipStream_t)stream : c->stream);
 }
 
 agrav_status agrav_get_timings(agrav_ctx *c, agrav_timings *t)

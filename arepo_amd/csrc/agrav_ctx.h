@@ -271,6 +271,9 @@ agrav_status ewald_tree_table(agrav_ctx *c);
 agrav_status ewald_forcetest_table(agrav_ctx *c);
 agrav_status forcetest(agrav_ctx *c, int ns, const int *sel, double *acc, double *acc_short, double *pot, double *pot_short);
 agrav_status direct_all(agrav_ctx *c, double *acc, double *pot);
+agrav_status noalloc_scratch_init(agrav_ctx *c);
+agrav_status direct_noalloc(agrav_ctx *c, int64_t n, const double *pos, const double *mass, const uint8_t *stype, double *acc,
+                            double *pot, hipStream_t s);
 agrav_status finalize(agrav_ctx *c);
 agrav_status final_out(agrav_ctx *c, FinalOut &f);
 WalkConst walk_const(const agrav_ctx *c, int use_bh);

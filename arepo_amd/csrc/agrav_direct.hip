@@ -15,6 +15,7 @@
 #include <math.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 namespace agrav
@@ -854,6 +855,87 @@ agrav_status direct_all(agrav_ctx *c, double *acc, double *pot)
       if(pot)
         pot[q] = h[q * 4 + 3];
     }
+  return AGRAV_OK;
+}
+
+// ---- gravity_direct without allocation -------------------------------------
+// For callers that cannot allocate device memory (code under HIP graph capture,
+// tight serving loops): the partial sums of k_direct go to `scratch`, a
+// file-scope device array of fixed size, instead of a buffer allocated per call.
+// Every call on a GPU shares that GPU's copy, so calls that overlap on one GPU
+// race.  Nothing in it needs clearing: k_direct writes every partial sum that
+// the reduction reads.
+//
+// Largest partial-sum array: pick_chunks gives at most ceil(n / 2048) chunks, so
+// n <= AGRAV_DIRECT_NOALLOC_MAX_N needs at most 8 x 16384 x 4 doubles (4 MiB).
+static constexpr size_t kNoallocDoubles = (size_t)((AGRAV_DIRECT_NOALLOC_MAX_N + 2047) / 2048) * AGRAV_DIRECT_NOALLOC_MAX_N * 4;
+static __device__ __align__(256) double scratch[kNoallocDoubles];
+
+// the device address of each GPU's copy of `scratch` (host code never uses the
+// array's name itself: on the host it is only a stand-in for the symbol)
+static constexpr int kMaxDevices = 64;
+static double *g_scratch_ptr[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+agrav_status noalloc_scratch_init(agrav_ctx *c)
+{
+  if(c->device < 0 || c->device >= kMaxDevices)
+    return fail(c, AGRAV_ERR_NO_DEVICE, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if(g_scratch_ptr[c->device])
+    return AGRAV_OK;
+  void *ptr = nullptr;
+  AG_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  if(!ptr)
+    return fail(c, AGRAV_ERR_HIP, "no device address for the scratch array");
+  g_scratch_ptr[c->device] = (double *)ptr;
+  return AGRAV_OK;
+}
+
+// sum the partials of each particle in chunk order (as k_chunk_reduce4) and
+// write GravAccel and Potential straight to the caller's arrays
+__global__ void k_chunk_reduce_out(const double *__restrict__ part, int n, int nchunk, double *acc, double *pot)
+{
+  int q = blockIdx.x * B + threadIdx.x;
+  if(q >= n)
+    return;
+  double v[4] = {0, 0, 0, 0};
+  for(int c = 0; c < nchunk; c++)
+    for(int k = 0; k < 4; k++)
+      v[k] += part[((size_t)c * n + q) * 4 + k];
+  acc[3 * (size_t)q]     = v[0];
+  acc[3 * (size_t)q + 1] = v[1];
+  acc[3 * (size_t)q + 2] = v[2];
+  if(pot)
+    pot[q] = v[3];
+}
+
+agrav_status direct_noalloc(agrav_ctx *c, int64_t n, const double *pos, const double *mass, const uint8_t *stype, double *acc,
+                            double *pot, hipStream_t s)
+{
+  if(n < 0 || (n > 0 && (!pos || !mass || !acc)))
+    return fail(c, AGRAV_ERR_BAD_PARAM, "gravity_direct_noalloc: null device pointer");
+  if(n > AGRAV_DIRECT_NOALLOC_MAX_N)
+    return fail(c, AGRAV_ERR_CAPACITY, "gravity_direct_noalloc: more than AGRAV_DIRECT_NOALLOC_MAX_N particles");
+  if(c->batch_nsets > 0)
+    return fail(c, AGRAV_ERR_BAD_PARAM, "gravity_direct_noalloc: not for a batch context");
+  if(c->par.periodic && c->par.pmgrid == 0 && !c->ewald_ready)
+    return fail(c, AGRAV_ERR_STATE, "gravity_direct_noalloc: the Ewald table is not ready");
+  double *part = (c->device >= 0 && c->device < kMaxDevices) ? g_scratch_ptr[c->device] : nullptr;
+  if(!part)
+    return fail(c, AGRAV_ERR_STATE, "gravity_direct_noalloc: scratch not set up on this device");
+  if(n == 0)
+    return AGRAV_OK;
+  const int tblocks = (int)((n + B - 1) / B);
+  const int nchunk  = pick_chunks(tblocks, n);
+  const int chunk   = (int)((n + nchunk - 1) / nchunk);
+  if((size_t)nchunk * (size_t)n * 4 > kNoallocDoubles)
+    return fail(c, AGRAV_ERR_CAPACITY, "gravity_direct_noalloc: partial sums exceed the scratch array");
+  agrav_status st;
+  if((st = direct_launch(c, n, pos, mass, stype, part, nchunk, chunk, s)) != AGRAV_OK)
+    return st;
+  k_chunk_reduce_out<<<tblocks, B, 0, s>>>(part, (int)n, nchunk, acc, pot);
+  AG_HIP(hipGetLastError());
   return AGRAV_OK;
 }
 

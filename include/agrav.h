@@ -223,6 +223,24 @@ agrav_status agrav_gravity_forcetest(agrav_ctx *ctx, int ns, const int32_t *sel,
  * caller order (host pointers). */
 agrav_status agrav_gravity_direct(agrav_ctx *ctx, double *acc, double *pot);
 
+/* agrav_gravity_direct for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  The same kernels and summation order as
+ * agrav_gravity_direct (the results are bitwise equal), but nothing is allocated:
+ * the partial sums live in a fixed 4 MiB device array of the library, set up by
+ * agrav_create on each device.  Inputs and outputs are caller-owned device
+ * pointers in caller order: d_pos[3n], d_mass[n], d_softtype[n] (or NULL: every
+ * particle type 0); d_acc[3n] and d_pot[n] (d_pot may be NULL) in G=1 units.  The
+ * context supplies the parameters and tables (its particles are not used).
+ * Asynchronous on `stream` (a hipStream_t; NULL = the context's stream): no
+ * allocation, no host synchronisation, no memset.  n <= AGRAV_DIRECT_NOALLOC_MAX_N,
+ * else AGRAV_ERR_CAPACITY.  Softening types are not checked on the host: a type
+ * >= n_soft_types makes the results NaN.  All calls on one GPU share its scratch
+ * array, so calls that overlap in time on one GPU race; serialise them (one
+ * stream, or events). */
+#define AGRAV_DIRECT_NOALLOC_MAX_N 16384
+agrav_status agrav_gravity_direct_noalloc(agrav_ctx *ctx, int64_t n, const double *d_pos, const double *d_mass,
+                                          const uint8_t *d_softtype, double *d_acc, double *d_pot, void *stream);
 
 agrav_status agrav_get_timings(agrav_ctx *ctx, agrav_timings *t);
 
