@@ -54,6 +54,20 @@ def make_params(periodic=True, box_size=1.0, pmgrid=0, asmth=1.25, rcut=4.5, G=1
     return p
 
 
+class HighRes(C.Structure):
+    """orc_highres: the PLACEHIGHRESREGION region (mask, enlarge in; the rest filled by the sizing)."""
+    _fields_ = [("mask", C.c_int), ("enlarge", C.c_double), ("valid", C.c_int), ("xmin", C.c_double * 3),
+                ("xmax", C.c_double * 3), ("corner", C.c_double * 3), ("upper", C.c_double * 3),
+                ("meshsize", C.c_double), ("asmth", C.c_double), ("rcut", C.c_double)]
+
+
+def make_highres(mask, enlarge=0.0):
+    r = HighRes()
+    r.mask = int(mask)
+    r.enlarge = float(enlarge)
+    return r
+
+
 class UnbindParams(C.Structure):
     _fields_ = [
         ("G", C.c_double),
@@ -161,6 +175,21 @@ def _declare(L):
     L.orc_gravity_timesteps.argtypes = [C.POINTER(TimestepParams), C.POINTER(Params), C.c_int, _dp, C.c_void_p, _dp,
                                         C.c_void_p, _dp, C.POINTER(C.c_double)]
     L.orc_gravity_timesteps.restype = C.c_int
+    HR = C.POINTER(HighRes)
+    L.orc_highres_regionsize.argtypes = [P, HR, C.c_int, _dp, _u8]
+    L.orc_highres_regionsize.restype = C.c_int
+    L.orc_highres_flags.argtypes = [P, HR, C.c_int, _dp, _u8, _u8]
+    L.orc_pm_highres.argtypes = [P, HR, C.c_int, _dp, _dp, _u8, _dp, _dp]
+    L.orc_pm_highres.restype = C.c_int
+    L.orc_tree_walk_hr.argtypes = [P, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                   C.c_void_p, _dp, _dp, _ip, C.c_int]
+    L.orc_tree_walk_hr.restype = C.c_longlong
+    L.orc_forcetest_hr.argtypes = [P, C.c_void_p, C.c_void_p, C.c_int, _dp, _dp, C.c_void_p, C.c_int, _ip, C.c_int,
+                                   C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, C.c_int]
+    L.orc_direct_summation_hr.argtypes = [P, C.c_void_p, C.c_void_p, C.c_int, _dp, _dp, C.c_void_p, _dp, _dp, C.c_int]
+    L.orc_compute_grav_accelerations_hr.argtypes = [P, C.c_void_p, C.c_void_p, C.c_int, _dp, _dp, C.c_void_p, C.c_int,
+                                                    _fp, _dp, _dp, _ip, C.c_int, _dp]
+    L.orc_compute_grav_accelerations_hr.restype = C.c_longlong
     u64 = C.c_uint64
     L.orc_peano_hilbert_key.argtypes = [u64, u64, u64, C.c_int, C.POINTER(u64), C.POINTER(u64)]
     L.orc_peano_hilbert_key_inverse.argtypes = [u64, u64, C.c_int, C.POINTER(u64), C.POINTER(u64), C.POINTER(u64)]
@@ -239,7 +268,8 @@ class Tree:
         return dict(center=center.reshape(-1, 3), len=ln, com=com.reshape(-1, 3), mass=mass, level=level,
                     npart=npart)
 
-    def walk(self, use_bh, old_acc=None, ewald_tab=None, targets=None, nthreads=0):
+    def walk(self, use_bh, old_acc=None, ewald_tab=None, targets=None, nthreads=0, highres=None, hrflag=None):
+        """highres/hrflag: the region and the per-particle flags of a high-resolution PM region."""
         nt = self.n if targets is None else len(targets)
         acc = np.zeros(3 * nt)
         pot = np.zeros(nt)
@@ -247,9 +277,12 @@ class Tree:
         oa = None if old_acc is None else np.ascontiguousarray(old_acc, dtype=np.float32)
         tg = None if targets is None else np.ascontiguousarray(targets, dtype=np.int32)
         tab = None if ewald_tab is None else np.ascontiguousarray(ewald_tab, dtype=np.float64)
-        total = lib().orc_tree_walk(C.byref(self.p), self.h, int(use_bh), None if oa is None else oa.ctypes.data,
-                                    None if tab is None else tab.ctypes.data, nt,
-                                    None if tg is None else tg.ctypes.data, acc, pot, nint, nthreads)
+        fl = None if hrflag is None else np.ascontiguousarray(hrflag, dtype=np.uint8)
+        total = lib().orc_tree_walk_hr(C.byref(self.p), self.h, int(use_bh), None if oa is None else oa.ctypes.data,
+                                       None if tab is None else tab.ctypes.data,
+                                       None if highres is None else C.addressof(highres),
+                                       None if fl is None else fl.ctypes.data, nt,
+                                       None if tg is None else tg.ctypes.data, acc, pot, nint, nthreads)
         return acc.reshape(-1, 3), pot, nint, total
 
 
@@ -311,7 +344,7 @@ def forcetest_ewald_lookup(tab, tew_n, box, d):
     return f
 
 
-def forcetest(p, pos, mass, sel, stype=None, ewald=0, tab=None, tew_n=0, nthreads=0):
+def forcetest(p, pos, mass, sel, stype=None, ewald=0, tab=None, tew_n=0, nthreads=0, highres=None, hrflag=None):
     """FORCETEST direct sum (G=1 units): returns acc, acc_short, pot, pot_short for the selected targets."""
     pos = _pos(pos)
     mass = np.ascontiguousarray(mass, dtype=np.float64)
@@ -323,12 +356,18 @@ def forcetest(p, pos, mass, sel, stype=None, ewald=0, tab=None, tew_n=0, nthread
     pots = np.zeros(ns)
     st = None if stype is None else np.ascontiguousarray(stype, dtype=np.uint8)
     tb = None if tab is None else np.ascontiguousarray(tab, dtype=np.float64)
-    lib().orc_forcetest(C.byref(p), len(pos), pos, mass, None if st is None else st.ctypes.data, ns, sel, ewald,
-                        None if tb is None else tb.ctypes.data, tew_n, acc, accs, pot, pots, nthreads)
+    fl = None if hrflag is None else np.ascontiguousarray(hrflag, dtype=np.uint8)
+    lib().orc_forcetest_hr(C.byref(p), None if highres is None else C.addressof(highres),
+                           None if fl is None else fl.ctypes.data, len(pos), pos, mass,
+                           None if st is None else st.ctypes.data, ns, sel, ewald,
+                           None if tb is None else tb.ctypes.data, tew_n, acc, accs, pot, pots, nthreads)
     return acc.reshape(-1, 3), accs.reshape(-1, 3), pot, pots
 
 
-def compute_grav_accelerations(p, pos, mass, stype=None, first_step=True, old_acc=None, nthreads=0):
+def compute_grav_accelerations(p, pos, mass, stype=None, first_step=True, old_acc=None, nthreads=0, highres=None,
+                               ptype=None):
+    """The step (accel.c:58-347); with `highres` (a HighRes, sized when its `valid` is 0) and the particle
+    types `ptype`, a high-resolution PM region as PLACEHIGHRESREGION."""
     pos = _pos(pos)
     n = len(pos)
     mass = np.ascontiguousarray(mass, dtype=np.float64)
@@ -338,22 +377,57 @@ def compute_grav_accelerations(p, pos, mass, stype=None, first_step=True, old_ac
     nint = np.zeros(n, np.int32)
     t = np.zeros(3)
     st = None if stype is None else np.ascontiguousarray(stype, dtype=np.uint8)
-    total = lib().orc_compute_grav_accelerations(C.byref(p), n, pos, mass, None if st is None else st.ctypes.data,
-                                                 int(first_step), oa, acc, pot, nint, nthreads, t)
+    ty = None if ptype is None else np.ascontiguousarray(ptype, dtype=np.uint8)
+    total = lib().orc_compute_grav_accelerations_hr(C.byref(p), None if highres is None else C.addressof(highres),
+                                                    None if ty is None else ty.ctypes.data, n, pos, mass,
+                                                    None if st is None else st.ctypes.data, int(first_step), oa, acc,
+                                                    pot, nint, nthreads, t)
     if total < 0:
         raise RuntimeError("oracle compute failed %d" % total)
     return dict(acc=acc.reshape(-1, 3), pot=pot, nint=nint, old_acc=oa, total=total,
                 t_treebuild=t[0], t_treewalk=t[1], t_pm=t[2])
 
 
-def direct_summation(p, pos, mass, stype=None, nthreads=0):
+def direct_summation(p, pos, mass, stype=None, nthreads=0, highres=None, hrflag=None):
     pos = _pos(pos)
     mass = np.ascontiguousarray(mass, dtype=np.float64)
     acc = np.zeros(3 * len(pos))
     pot = np.zeros(len(pos))
     st = None if stype is None else np.ascontiguousarray(stype, dtype=np.uint8)
-    lib().orc_direct_summation(C.byref(p), len(pos), pos, mass, None if st is None else st.ctypes.data, acc, pot,
-                               nthreads)
+    fl = None if hrflag is None else np.ascontiguousarray(hrflag, dtype=np.uint8)
+    lib().orc_direct_summation_hr(C.byref(p), None if highres is None else C.addressof(highres),
+                                  None if fl is None else fl.ctypes.data, len(pos), pos, mass,
+                                  None if st is None else st.ctypes.data, acc, pot, nthreads)
+    return acc.reshape(-1, 3), pot
+
+
+def highres_regionsize(p, r, pos, ptype):
+    """pm_init_regionsize for the high-resolution grid: fills the HighRes r in place."""
+    rc = lib().orc_highres_regionsize(C.byref(p), C.byref(r), len(pos), _pos(pos),
+                                      np.ascontiguousarray(ptype, dtype=np.uint8))
+    if rc != 0:
+        raise ValueError("no particle of the high-resolution types")
+    return r
+
+
+def highres_flags(p, r, pos, ptype):
+    fl = np.zeros(len(pos), np.uint8)
+    lib().orc_highres_flags(C.byref(p), C.byref(r), len(pos), _pos(pos), np.ascontiguousarray(ptype, dtype=np.uint8), fl)
+    return fl
+
+
+def pm_highres(p, r, pos, mass, ptype):
+    """The high-resolution PM force (G included) and potential; None when a high-resolution particle is
+    outside the region."""
+    pos = _pos(pos)
+    acc = np.zeros(3 * len(pos))
+    pot = np.zeros(len(pos))
+    rc = lib().orc_pm_highres(C.byref(p), C.byref(r), len(pos), pos, np.ascontiguousarray(mass, dtype=np.float64),
+                              np.ascontiguousarray(ptype, dtype=np.uint8), acc, pot)
+    if rc == 1:
+        return None
+    if rc != 0:
+        raise RuntimeError("orc_pm_highres failed %d" % rc)
     return acc.reshape(-1, 3), pot
 
 

@@ -767,6 +767,18 @@ typedef struct
   float tf[ORC_NTAB + 1], tp[ORC_NTAB + 1];
 } shortrange_ctx;
 
+/* the short-range constants of forcetree_walk.c:142-156 for split scale asmth and cut rcut (lengths) */
+static void shortrange_setup_len(const orc_params *p, double asmth, double rcut, shortrange_ctx *c)
+{
+  c->pm        = 1;
+  c->rcut      = rcut;
+  c->rcut2     = rcut * rcut;
+  double ai    = 0.5 / asmth;
+  c->asmthinv2 = ai * ai;
+  c->asmthfac  = ai * (ORC_NTAB / (p->rcut / 2.0));
+  orc_short_range_tables(p->rcut, c->tf, c->tp);
+}
+
 static void shortrange_setup(const orc_params *p, shortrange_ctx *c)
 {
   c->pm = p->pmgrid > 0;
@@ -970,8 +982,18 @@ static int walk_one(const orc_params *p, const orc_tree *t, const shortrange_ctx
 long long orc_tree_walk(const orc_params *p, const orc_tree *t, int use_bh, const float *old_acc, const double *ewald_tab,
                         int ntarget, const int *targets, double *acc, double *pot, int *nint, int nthreads)
 {
-  shortrange_ctx c;
+  return orc_tree_walk_hr(p, t, use_bh, old_acc, ewald_tab, NULL, NULL, ntarget, targets, acc, pot, nint, nthreads);
+}
+
+long long orc_tree_walk_hr(const orc_params *p, const orc_tree *t, int use_bh, const float *old_acc, const double *ewald_tab,
+                           const orc_highres *hr, const uint8_t *hrflag, int ntarget, const int *targets, double *acc,
+                           double *pot, int *nint, int nthreads)
+{
+  shortrange_ctx c, c1;
   shortrange_setup(p, &c);
+  /* forcetree_walk.c:143-150: a target in the high-resolution region uses Asmth[1] and Rcut[1] */
+  if(hr && hrflag)
+    shortrange_setup_len(p, hr->asmth, hr->rcut, &c1);
   long long total = 0;
   int nt          = targets ? ntarget : t->n;
 #pragma omp parallel for schedule(dynamic, 64) reduction(+ : total) num_threads(omp_threads(nthreads))
@@ -980,7 +1002,8 @@ long long orc_tree_walk(const orc_params *p, const orc_tree *t, int use_bh, cons
       int i = targets ? targets[q] : q;
       double a[3], pt;
       double aold = old_acc ? old_acc[i] : 0;
-      int ni = walk_one(p, t, &c, use_bh, aold, ewald_tab, &t->pos[3 * i], t->stype ? t->stype[i] : 0, a, &pt);
+      const shortrange_ctx *ci = (hr && hrflag && hrflag[i]) ? &c1 : &c;
+      int ni = walk_one(p, t, ci, use_bh, aold, ewald_tab, &t->pos[3 * i], t->stype ? t->stype[i] : 0, a, &pt);
       acc[3 * q + 0] = a[0];
       acc[3 * q + 1] = a[1];
       acc[3 * q + 2] = a[2];
@@ -1211,6 +1234,275 @@ int orc_pm_periodic(const orc_params *p, int n, const double *pos, const double 
 }
 
 /* ------------------------------------------------------------------ */
+/* high-resolution PM region (PLACEHIGHRESREGION), pm_nonperiodic.c    */
+/* ------------------------------------------------------------------ */
+
+static double wrap_box(const orc_params *p, double x)
+{
+  if(!p->periodic)
+    return x;
+  if(x < 0)
+    x += p->box_size;
+  if(x >= p->box_size)
+    x -= p->box_size;
+  return x;
+}
+
+int orc_highres_regionsize(const orc_params *p, orc_highres *r, int n, const double *pos, const uint8_t *type)
+{
+  /* pm_init_regionsize, pm_nonperiodic.c:135-245, for the high-resolution grid */
+  const int GRID = ORC_GRIDBOOST * p->pmgrid;
+  double xmin[3], xmax[3];
+  for(int j = 0; j < 3; j++)
+    {
+      xmin[j] = 1.0e36;
+      xmax[j] = -1.0e36;
+    }
+  int found = 0;
+  for(int i = 0; i < n; i++)
+    if((1 << type[i]) & r->mask)
+      {
+        found = 1;
+        for(int j = 0; j < 3; j++)
+          {
+            double x = wrap_box(p, pos[3 * i + j]);
+            if(x > xmax[j])
+              xmax[j] = x;
+            if(x < xmin[j])
+              xmin[j] = x;
+          }
+      }
+  if(!found)
+    return -1;
+  double size = xmax[0] - xmin[0];
+  for(int j = 1; j < 3; j++)
+    if(xmax[j] - xmin[j] > size)
+      size = xmax[j] - xmin[j];
+  if(r->enlarge > 0)
+    size *= r->enlarge; /* ENLARGEREGION */
+  for(int j = 0; j < 3; j++)
+    {
+      /* symmetrised onto the centre */
+      r->xmin[j] = (xmin[j] + xmax[j]) / 2 - size / 2;
+      r->xmax[j] = r->xmin[j] + size;
+    }
+  const double meshinner = size;
+  double tms             = size * 2.001 * GRID / ((double)(GRID - 2 - 8));
+  for(int j = 0; j < 3; j++)
+    {
+      r->corner[j] = r->xmin[j] - 2.0005 * tms / GRID;
+      r->upper[j]  = r->corner[j] + (GRID / 2 - 1) * (tms / GRID);
+    }
+  double asmth0, rcut0;
+  orc_asmth_rcut(p, &asmth0, &rcut0);
+  if(2 * tms / GRID < rcut0)
+    {
+      /* a high-resolution cell below half the coarse cut: the region is widened by the coarse cut */
+      tms = 2 * (meshinner + 2 * rcut0) * GRID / ((double)(GRID - 2));
+      for(int j = 0; j < 3; j++)
+        {
+          r->corner[j] = r->xmin[j] - 1.0001 * rcut0;
+          r->upper[j]  = r->corner[j] + (GRID / 2 - 1) * (tms / GRID);
+        }
+      if(2 * tms / GRID > rcut0)
+        {
+          tms = 2 * (meshinner + 2 * rcut0) * GRID / ((double)(GRID - 10));
+          for(int j = 0; j < 3; j++)
+            {
+              r->corner[j] = r->xmin[j] - 1.0001 * (rcut0 + 2 * tms / GRID);
+              r->upper[j]  = r->corner[j] + (GRID / 2 - 1) * (tms / GRID);
+            }
+        }
+    }
+  r->meshsize = tms;
+  r->asmth    = p->asmth * tms / GRID;
+  r->rcut     = p->rcut * r->asmth;
+  r->valid    = 1;
+  return 0;
+}
+
+int orc_highres_flag(const orc_params *p, const orc_highres *r, int type, const double *pos)
+{
+  /* pmforce_is_particle_high_res, pm_nonperiodic.c:344-371 */
+  if((1 << type) & r->mask)
+    return 1;
+  for(int j = 0; j < 3; j++)
+    {
+      double x = wrap_box(p, pos[j]);
+      if(x < r->xmin[j] || x > r->xmax[j])
+        return 0;
+    }
+  return 1;
+}
+
+void orc_highres_flags(const orc_params *p, const orc_highres *r, int n, const double *pos, const uint8_t *type, uint8_t *flags)
+{
+  for(int i = 0; i < n; i++)
+    flags[i] = (uint8_t)orc_highres_flag(p, r, type[i], &pos[3 * i]);
+}
+
+int orc_pm_highres(const orc_params *p, const orc_highres *r, int n, const double *pos, const double *mass, const uint8_t *type,
+                   double *acc_pm, double *pot_pm)
+{
+  /* pmforce_nonperiodic(1), pm_nonperiodic.c:1508-1775, with the kernel of pm_setup_nonperiodic_kernel
+   * (:1778-1987); accumulates into acc_pm (G included) and pot_pm for the high-resolution particles.
+   * Returns 1 when a high-resolution particle lies outside the region (the caller re-sizes it). */
+  const int GRID  = ORC_GRIDBOOST * p->pmgrid;
+  const int H     = GRID / 2;
+  const double fs = GRID / r->meshsize;  /* to_slab_fac */
+  for(int i = 0; i < n; i++)
+    if(orc_highres_flag(p, r, type[i], &pos[3 * i]))
+      for(int j = 0; j < 3; j++)
+        {
+          double x = wrap_box(p, pos[3 * i + j]);
+          if(x < r->xmin[j] || x > r->xmax[j])
+            return 1;
+        }
+  size_t G3   = (size_t)GRID * GRID * GRID;
+  double *re  = (double *)calloc(G3, sizeof(double));
+  double *im  = (double *)calloc(G3, sizeof(double));
+  double *kre = (double *)calloc(G3, sizeof(double));
+  double *kim = (double *)calloc(G3, sizeof(double));
+  double *fg  = (double *)calloc(G3, sizeof(double));
+  if(!re || !im || !kre || !kim || !fg)
+    return -2;
+#define IDX(x, y, z) (((size_t)(x)*GRID + (y)) * GRID + (z))
+  /* CIC deposit of every particle inside [Corner, UpperCorner) (:810-1170), no wrapping */
+  for(int i = 0; i < n; i++)
+    {
+      double x[3], d[3];
+      int s[3], in = 1;
+      for(int k = 0; k < 3 && in; k++)
+        {
+          x[k] = wrap_box(p, pos[3 * i + k]);
+          if(x[k] < r->corner[k] || x[k] >= r->upper[k])
+            in = 0;
+        }
+      if(!in)
+        continue;
+      for(int k = 0; k < 3; k++)
+        {
+          s[k] = (int)(fs * (x[k] - r->corner[k]));
+          d[k] = fs * (x[k] - r->corner[k]) - s[k];
+        }
+      double m = mass[i];
+      for(int c = 0; c < 8; c++)
+        {
+          int bx = (c >> 2) & 1, by = (c >> 1) & 1, bz = c & 1;
+          re[IDX(s[0] + bx, s[1] + by, s[2] + bz)] += m * (bx ? d[0] : 1.0 - d[0]) * (by ? d[1] : 1.0 - d[1]) * (bz ? d[2] : 1.0 - d[2]);
+        }
+    }
+  orc_fft3d(GRID, re, im, -1);
+  /* the kernel in real space on the whole grid (coordinates in units of the mesh size, wrapped to
+   * [-1/2, 1/2)): -(erfc(u Asmth[1] / Asmth[0]) - erfc(u)) / r with u = r / (2 ASMTH / GRID) */
+  double asmth0, rcut0;
+  orc_asmth_rcut(p, &asmth0, &rcut0);
+  const double ratio = r->asmth / asmth0;
+  for(int x = 0; x < GRID; x++)
+    for(int y = 0; y < GRID; y++)
+      for(int z = 0; z < GRID; z++)
+        {
+          double xx = (double)x / GRID, yy = (double)y / GRID, zz = (double)z / GRID;
+          if(xx >= 0.5)
+            xx -= 1.0;
+          if(yy >= 0.5)
+            yy -= 1.0;
+          if(zz >= 0.5)
+            zz -= 1.0;
+          double rr = sqrt(xx * xx + yy * yy + zz * zz);
+          double u  = 0.5 * rr / (p->asmth / GRID);
+          if(rr > 0)
+            kre[IDX(x, y, z)] = -(erfc(u * ratio) - erfc(u)) / rr;
+          else
+            kre[IDX(x, y, z)] = -(1 - ratio) / (sqrt(M_PI) * (p->asmth / GRID));
+        }
+  orc_fft3d(GRID, kre, kim, -1);
+  for(int x = 0; x < GRID; x++)
+    for(int y = 0; y < GRID; y++)
+      for(int z = 0; z < GRID; z++)
+        {
+          /* CIC deconvolution, twice (:1936-1981); frequencies above GRID / 2 are negative */
+          double kx = x > GRID / 2 ? x - GRID : x, ky = y > GRID / 2 ? y - GRID : y, kz = z > GRID / 2 ? z - GRID : z;
+          size_t ip = IDX(x, y, z);
+          if(kx * kx + ky * ky + kz * kz > 0)
+            {
+              double fx = 1, fy = 1, fz = 1;
+              if(kx != 0)
+                fx = sin(M_PI * kx / GRID) / (M_PI * kx / GRID);
+              if(ky != 0)
+                fy = sin(M_PI * ky / GRID) / (M_PI * ky / GRID);
+              if(kz != 0)
+                fz = sin(M_PI * kz / GRID) / (M_PI * kz / GRID);
+              double ff = 1 / (fx * fy * fz);
+              ff        = ff * ff * ff * ff;
+              kre[ip] *= ff;
+              kim[ip] *= ff;
+            }
+          double a = re[ip], b = im[ip];
+          re[ip] = a * kre[ip] - b * kim[ip];
+          im[ip] = a * kim[ip] + b * kre[ip];
+        }
+  orc_fft3d(GRID, re, im, +1); /* re: the potential (unnormalised) */
+  const double cell    = r->meshsize / GRID;
+  const double fac_pot = p->G / pow(r->meshsize, 4) * pow(cell, 3);
+  const double fac_fd  = fac_pot / (2 * cell);
+  for(int dim = -1; dim < 3; dim++)
+    {
+      const double *grid = re;
+      if(dim >= 0)
+        {
+          /* 4-point differences on the cells 2 .. GRID/2 - 3 of each axis (:1652-1700) */
+          memset(fg, 0, G3 * sizeof(double));
+          for(int x = 2; x < H - 2; x++)
+            for(int y = 2; y < H - 2; y++)
+              for(int z = 2; z < H - 2; z++)
+                {
+                  int l[3] = {x, y, z}, rt[3] = {x, y, z}, ll[3] = {x, y, z}, rr[3] = {x, y, z};
+                  l[dim] -= 1;
+                  rt[dim] += 1;
+                  ll[dim] -= 2;
+                  rr[dim] += 2;
+                  fg[IDX(x, y, z)] = fac_fd * ((4.0 / 3) * (re[IDX(l[0], l[1], l[2])] - re[IDX(rt[0], rt[1], rt[2])]) -
+                                               (1.0 / 6) * (re[IDX(ll[0], ll[1], ll[2])] - re[IDX(rr[0], rr[1], rr[2])]));
+                }
+          grid = fg;
+        }
+      /* CIC readout for the high-resolution particles (:1283-1500) */
+      for(int i = 0; i < n; i++)
+        {
+          if(!orc_highres_flag(p, r, type[i], &pos[3 * i]))
+            continue;
+          double x[3], d[3];
+          int s[3];
+          for(int k = 0; k < 3; k++)
+            {
+              x[k] = wrap_box(p, pos[3 * i + k]);
+              s[k] = (int)(fs * (x[k] - r->corner[k]));
+              d[k] = fs * (x[k] - r->corner[k]) - s[k];
+            }
+          double v = 0;
+          for(int c = 0; c < 8; c++)
+            {
+              int bx = (c >> 2) & 1, by = (c >> 1) & 1, bz = c & 1;
+              v += grid[IDX(s[0] + bx, s[1] + by, s[2] + bz)] * (bx ? d[0] : 1.0 - d[0]) * (by ? d[1] : 1.0 - d[1]) *
+                   (bz ? d[2] : 1.0 - d[2]);
+            }
+          if(dim < 0)
+            pot_pm[i] += v * fac_pot;
+          else
+            acc_pm[3 * i + dim] += v;
+        }
+    }
+#undef IDX
+  free(re);
+  free(im);
+  free(kre);
+  free(kim);
+  free(fg);
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
 /* FORCETEST direct summation                                          */
 /* ------------------------------------------------------------------ */
 
@@ -1218,11 +1510,18 @@ void orc_forcetest(const orc_params *p, int n, const double *pos, const double *
                    const int *sel, int ewald, const double *tab, int tew_n, double *acc, double *acc_short, double *pot,
                    double *pot_short, int nthreads)
 {
+  orc_forcetest_hr(p, NULL, NULL, n, pos, mass, stype, ns, sel, ewald, tab, tew_n, acc, acc_short, pot, pot_short, nthreads);
+}
+
+void orc_forcetest_hr(const orc_params *p, const orc_highres *hr, const uint8_t *hrflag, int n, const double *pos,
+                      const double *mass, const uint8_t *stype, int ns, const int *sel, int ewald, const double *tab, int tew_n,
+                      double *acc, double *acc_short, double *pot, double *pot_short, int nthreads)
+{
   double fs[ORC_MAXSOFT + 1];
   orc_set_softenings(p, fs);
-  double asmth = 0, rcut = 0;
+  double asmth0 = 0, rcut = 0;
   if(p->pmgrid > 0)
-    orc_asmth_rcut(p, &asmth, &rcut);
+    orc_asmth_rcut(p, &asmth0, &rcut);
   const double L   = p->box_size;
   const int per    = p->periodic;
   const double vol = L * L * L;
@@ -1231,6 +1530,8 @@ void orc_forcetest(const orc_params *p, int n, const double *pos, const double *
     {
       int i      = sel[q];
       double h_i = fs[stype ? stype[i] : 0];
+      /* gravtree_forcetest.c:480-483: Asmth[1] for a target in the high-resolution region */
+      const double asmth = (hr && hrflag && hrflag[i]) ? hr->asmth : asmth0;
       double a[3] = {0, 0, 0}, as[3] = {0, 0, 0}, pt = 0, pts = 0;
       for(int j = 0; j < n; j++)
         {
@@ -1315,9 +1616,18 @@ void orc_forcetest(const orc_params *p, int n, const double *pos, const double *
 void orc_direct_summation(const orc_params *p, int n, const double *pos, const double *mass, const uint8_t *stype, double *acc,
                           double *pot, int nthreads)
 {
-  /* force_evaluate_direct, forcetree_walk.c:557-708 */
-  shortrange_ctx c;
-  shortrange_setup(p, &c);
+  orc_direct_summation_hr(p, NULL, NULL, n, pos, mass, stype, acc, pot, nthreads);
+}
+
+void orc_direct_summation_hr(const orc_params *p, const orc_highres *hr, const uint8_t *hrflag, int n, const double *pos,
+                             const double *mass, const uint8_t *stype, double *acc, double *pot, int nthreads)
+{
+  /* force_evaluate_direct, forcetree_walk.c:557-708; Asmth[1] for a target in the high-resolution
+   * region (:576-581) */
+  shortrange_ctx c0, c1;
+  shortrange_setup(p, &c0);
+  if(hr && hrflag)
+    shortrange_setup_len(p, hr->asmth, hr->rcut, &c1);
   double fs[ORC_MAXSOFT + 1];
   orc_set_softenings(p, fs);
   const double L = p->box_size;
@@ -1326,6 +1636,7 @@ void orc_direct_summation(const orc_params *p, int n, const double *pos, const d
   for(int i = 0; i < n; i++)
     {
       double h_i = fs[stype ? stype[i] : 0];
+      const shortrange_ctx *cp = (hr && hrflag && hrflag[i]) ? &c1 : &c0;
       double ax = 0, ay = 0, az = 0, pt = 0;
       for(int j = 0; j < n; j++)
         {
@@ -1335,7 +1646,7 @@ void orc_direct_summation(const orc_params *p, int n, const double *pos, const d
           double dy   = nearest(pos[3 * j + 1] - pos[3 * i + 1], L, per);
           double dz   = nearest(pos[3 * j + 2] - pos[3 * i + 2], L, per);
           double r    = sqrt(dx * dx + dy * dy + dz * dz), fac, wp;
-          if(!pair_kernel(&c, r, hmax, &fac, &wp))
+          if(!pair_kernel(cp, r, hmax, &fac, &wp))
             continue;
           pt += mass[j] * wp;
           fac *= mass[j];
@@ -1367,8 +1678,19 @@ long long orc_compute_grav_accelerations(const orc_params *p, int n, const doubl
                                          int first_step, float *old_acc, double *acc, double *pot, int *nint, int nthreads,
                                          double *t_out)
 {
+  return orc_compute_grav_accelerations_hr(p, NULL, NULL, n, pos, mass, stype, first_step, old_acc, acc, pot, nint, nthreads,
+                                           t_out);
+}
+
+long long orc_compute_grav_accelerations_hr(const orc_params *p, orc_highres *hr, const uint8_t *type, int n, const double *pos,
+                                            const double *mass, const uint8_t *stype, int first_step, float *old_acc, double *acc,
+                                            double *pot, int *nint, int nthreads, double *t_out)
+{
   if(n <= 0)
     return 0;
+  uint8_t *hrflag = NULL;
+  if(hr && (!type || p->pmgrid <= 0 || !p->periodic))
+    return -3;
   double *acc_pm = (double *)calloc(3 * (size_t)n, sizeof(double));
   double *pot_pm = (double *)calloc((size_t)n, sizeof(double));
   double *at     = (double *)malloc(3 * sizeof(double) * n);
@@ -1395,7 +1717,27 @@ long long orc_compute_grav_accelerations(const orc_params *p, int n, const doubl
       int use_bh = (p->opening == 0) || (pass == 0 && passes == 2);
       double t0  = wall();
       if(p->pmgrid > 0 && pass == 0)
-        orc_pm_periodic(p, n, pos, mass, acc_pm, pot_pm);
+        {
+          orc_pm_periodic(p, n, pos, mass, acc_pm, pot_pm);
+          if(hr)
+            {
+              /* longrange.c:129-140: the high-resolution grid after the periodic one; its region is sized
+               * when unset and again when a high-resolution particle has left it */
+              if(!hr->valid && orc_highres_regionsize(p, hr, n, pos, type) != 0)
+                return -4;
+              int rc = orc_pm_highres(p, hr, n, pos, mass, type, acc_pm, pot_pm);
+              if(rc == 1)
+                {
+                  if(orc_highres_regionsize(p, hr, n, pos, type) != 0)
+                    return -4;
+                  rc = orc_pm_highres(p, hr, n, pos, mass, type, acc_pm, pot_pm);
+                }
+              if(rc != 0)
+                return -5;
+              hrflag = (uint8_t *)malloc((size_t)n);
+              orc_highres_flags(p, hr, n, pos, type, hrflag);
+            }
+        }
       double t1 = wall();
       if(pass == 0)
         tpm = t1 - t0;
@@ -1409,9 +1751,22 @@ long long orc_compute_grav_accelerations(const orc_params *p, int n, const doubl
             return -1;
         }
       double t2 = wall();
-      total     = orc_tree_walk(p, t, use_bh, oa, ewald, n, NULL, at, pt, nint, nthreads);
+      total     = orc_tree_walk_hr(p, t, use_bh, oa, ewald, hr, hrflag, n, NULL, at, pt, nint, nthreads);
       double t3 = wall();
       orc_tree_free(t);
+      /* the potential constant of accel.c:269-274 with the mass of each PM region (forcetree.c:440-456) */
+      double pot_const = 0;
+      if(p->pmgrid > 0 && p->periodic)
+        {
+          double m_high = 0;
+          for(int i = 0; hrflag && i < n; i++)
+            if(hrflag[i])
+              m_high += mass[i];
+          const double vol = p->box_size * p->box_size * p->box_size;
+          pot_const        = (m_total - m_high) * M_PI / (asmth * asmth * vol);
+          if(hrflag)
+            pot_const += m_high * M_PI / (hr->asmth * hr->asmth * vol);
+        }
       /* gravity_force_finalize, accel.c:204-347 */
       for(int i = 0; i < n; i++)
         {
@@ -1421,10 +1776,7 @@ long long orc_compute_grav_accelerations(const orc_params *p, int n, const doubl
           oa[i]     = (float)sqrt(ax * ax + ay * ay + az * az);
           for(int d = 0; d < 3; d++)
             acc[3 * i + d] = at[3 * i + d] * p->G + acc_pm[3 * i + d];
-          double pp = pt[i];
-          if(p->pmgrid > 0 && p->periodic)
-            pp += m_total * M_PI / (asmth * asmth * p->box_size * p->box_size * p->box_size);
-          pot[i] = pp * p->G + pot_pm[i];
+          pot[i] = (pt[i] + pot_const) * p->G + pot_pm[i];
         }
       tb = t2 - t1;
       tw = t3 - t2;
@@ -1440,6 +1792,7 @@ long long orc_compute_grav_accelerations(const orc_params *p, int n, const doubl
   free(at);
   free(pt);
   free(ewald);
+  free(hrflag);
   if(!old_acc)
     free(oa);
   return total;

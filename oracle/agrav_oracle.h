@@ -40,6 +40,19 @@ typedef struct
 
 typedef struct orc_tree orc_tree;
 
+/* The high-resolution PM region of PLACEHIGHRESREGION (pm_nonperiodic.c:135-245): mask and enlarge are
+ * inputs; orc_highres_regionsize fills the rest (All.Xmintot[1], All.Xmaxtot[1], All.Corner[1],
+ * All.UpperCorner[1], All.TotalMeshSize[1], All.Asmth[1], All.Rcut[1]). */
+#define ORC_GRIDBOOST 2 /* GRIDBOOST default, pm_nonperiodic.c:75-77 */
+typedef struct
+{
+  int mask;       /* PLACEHIGHRESREGION: bit t set for the high-resolution particle types t */
+  double enlarge; /* ENLARGEREGION (<= 0: not set) */
+  int valid;
+  double xmin[3], xmax[3], corner[3], upper[3];
+  double meshsize, asmth, rcut;
+} orc_highres;
+
 /* grav_softening.c:55-96: ForceSoftening[i] = 2.8 eps_i, sentinel 0 at [nsoft] */
 void orc_set_softenings(const orc_params *p, double *force_softening);
 
@@ -159,6 +172,32 @@ int orc_gravity_timesteps(const orc_timestep_params *tp, const orc_params *p, in
 
 /* Textbook in-place complex FFT of a cube (n^3, n power of two); sign -1 forward (FFTW_FORWARD), +1 backward; unnormalised */
 void orc_fft3d(int n, double *re, double *im, int sign);
+
+/* pm_init_regionsize for the high-resolution grid (pm_nonperiodic.c:135-245) from the particles of the
+ * mask's types; -1 when there are none */
+int orc_highres_regionsize(const orc_params *p, orc_highres *r, int n, const double *pos, const uint8_t *type);
+/* pmforce_is_particle_high_res (pm_nonperiodic.c:344-371) */
+int orc_highres_flag(const orc_params *p, const orc_highres *r, int type, const double *pos);
+void orc_highres_flags(const orc_params *p, const orc_highres *r, int n, const double *pos, const uint8_t *type, uint8_t *flags);
+/* pmforce_nonperiodic(1) with pm_setup_nonperiodic_kernel (pm_nonperiodic.c:1508-1987): adds the
+ * high-resolution PM force (G included) and potential of the high-resolution particles into acc_pm and
+ * pot_pm; returns 1 (nothing added) when one of them lies outside the region */
+int orc_pm_highres(const orc_params *p, const orc_highres *r, int n, const double *pos, const double *mass, const uint8_t *type,
+                   double *acc_pm, double *pot_pm);
+/* the walk, FORCETEST, direct summation and the step with a high-resolution region: a target with hrflag
+ * set uses Asmth[1] and Rcut[1] (forcetree_walk.c:143-150, gravtree_forcetest.c:480-483,
+ * forcetree_walk.c:576-581); the step sizes the region when hr->valid is 0 */
+long long orc_tree_walk_hr(const orc_params *p, const orc_tree *t, int use_bh, const float *old_acc, const double *ewald_tab,
+                           const orc_highres *hr, const uint8_t *hrflag, int ntarget, const int *targets, double *acc,
+                           double *pot, int *nint, int nthreads);
+void orc_forcetest_hr(const orc_params *p, const orc_highres *hr, const uint8_t *hrflag, int n, const double *pos,
+                      const double *mass, const uint8_t *stype, int ns, const int *sel, int ewald, const double *tab, int tew_n,
+                      double *acc, double *acc_short, double *pot, double *pot_short, int nthreads);
+void orc_direct_summation_hr(const orc_params *p, const orc_highres *hr, const uint8_t *hrflag, int n, const double *pos,
+                             const double *mass, const uint8_t *stype, double *acc, double *pot, int nthreads);
+long long orc_compute_grav_accelerations_hr(const orc_params *p, orc_highres *hr, const uint8_t *type, int n, const double *pos,
+                                            const double *mass, const uint8_t *stype, int first_step, float *old_acc, double *acc,
+                                            double *pot, int *nint, int nthreads, double *t_out);
 
 /* peano_hilbert_key (peano.c:374-390): the `bits`-level key (bits <= 42) of integer
  * coordinates, as high and low 64-bit words; and its inverse (peano.c:446-496). */
