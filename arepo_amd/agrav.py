@@ -50,6 +50,8 @@ class Params(C.Structure):
         ("leaf_bucket", C.c_int),
         ("eval_potential", C.c_int),
         ("group_waves", C.c_int),
+        ("placehighresregion", C.c_int),
+        ("enlargeregion", C.c_double),
     ]
 
 
@@ -94,6 +96,7 @@ def lib():
 EXPORTED = [
     "agrav_status_string", "agrav_last_error", "agrav_default_params", "agrav_create", "agrav_destroy",
     "agrav_set_softenings", "agrav_set_particles", "agrav_set_particles_device", "agrav_set_old_acc", "agrav_set_active",
+    "agrav_set_particle_types", "agrav_highres_region",
     "agrav_set_partial_tree",
     "agrav_construct_forcetree", "agrav_gravity_tree", "agrav_long_range_force", "agrav_gravity_force_finalize",
     "agrav_compute_grav_accelerations", "agrav_get_accelerations", "agrav_get_accelerations_device",
@@ -205,6 +208,8 @@ def _declare(L):
     L.agrav_subfind_unbind.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp, C.POINTER(UnbindParams), vp, vp,
                                        vp, vp, vp, vp]
     L.agrav_gravity_direct_noalloc.argtypes = [vp, C.c_int64, vp, vp, vp, vp, vp, vp]
+    L.agrav_set_particle_types.argtypes = [vp, vp]
+    L.agrav_highres_region.argtypes = [vp, C.c_int, vp]
     L.agrav_server_create.argtypes = [C.POINTER(Params), C.c_int, C.c_size_t, C.POINTER(vp)]
     L.agrav_solve_footprint.restype = C.c_size_t
     L.agrav_solve_footprint.argtypes = [C.POINTER(Params), C.c_int64]
@@ -226,7 +231,7 @@ def _declare(L):
 
 def make_params(periodic=True, box_size=1.0, pmgrid=0, asmth=1.25, rcut=4.5, G=1.0, err_tol_theta=0.7,
                 err_tol_force_acc=0.0025, opening=1, soft=(0.01,), precision=FP64, walk=WALK_GROUP, group_size=0,
-                leaf_bucket=8, eval_potential=True, group_waves=0):
+                leaf_bucket=8, eval_potential=True, group_waves=0, highres_mask=0, enlarge_region=0.0):
     p = Params()
     lib().agrav_default_params(C.byref(p))
     p.periodic = int(periodic)
@@ -247,6 +252,8 @@ def make_params(periodic=True, box_size=1.0, pmgrid=0, asmth=1.25, rcut=4.5, G=1
     p.leaf_bucket = leaf_bucket
     p.eval_potential = int(eval_potential)
     p.group_waves = int(group_waves)
+    p.placehighresregion = int(highres_mask)
+    p.enlargeregion = float(enlarge_region)
     return p
 
 
@@ -386,6 +393,16 @@ class Gravity:
         nb = C.c_int64(0)
         self._check(self.L.agrav_gravity_timesteps(self.h, C.byref(tp), _ptr(dt), C.byref(dd), C.byref(nb)))
         return dt[:self.n], dd.value, nb.value
+
+    def set_particle_types(self, ptype):
+        t = np.ascontiguousarray(ptype, dtype=np.uint8)
+        self._check(self.L.agrav_set_particle_types(self.h, _ptr(t)))
+
+    def highres_region(self, reset=False):
+        """The high-resolution PM region as a dict (xmin, xmax, corner, upper, meshsize, asmth, rcut)."""
+        r = np.zeros(15)
+        self._check(self.L.agrav_highres_region(self.h, int(reset), _ptr(r)))
+        return dict(xmin=r[0:3], xmax=r[3:6], corner=r[6:9], upper=r[9:12], meshsize=r[12], asmth=r[13], rcut=r[14])
 
     def set_old_acc(self, old_acc):
         oa = np.ascontiguousarray(old_acc, dtype=np.float32)

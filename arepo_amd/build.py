@@ -24,6 +24,7 @@ SOURCES = [
     "agrav_build.hip",
     "agrav_walk.hip",
     "agrav_pm.hip",
+    "agrav_pmhr.hip",
     "agrav_direct.hip",
     "agrav_batch.hip",
     "agrav_unbind.hip",

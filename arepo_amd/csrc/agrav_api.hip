@@ -74,9 +74,13 @@ agrav_status final_out(agrav_ctx *c, FinalOut &f)
   double pot_const = 0;
   if(pm && c->par.periodic)
     {
-      double asmth = c->par.asmth * c->par.box_size / c->par.pmgrid;
-      double L     = c->par.box_size;
-      pot_const    = c->mass_total * M_PI / (asmth * asmth * L * L * L);  // accel.c:269-274
+      // accel.c:269-274, with the mass of each PM region (forcetree.c:440-456)
+      const double asmth = c->par.asmth * c->par.box_size / c->par.pmgrid;
+      const double L     = c->par.box_size, vol = L * L * L;
+      const double m_hr  = hr_on(c) ? c->hr.m_high : 0.0;
+      pot_const          = (c->mass_total - m_hr) * M_PI / (asmth * asmth * vol);
+      if(hr_on(c))
+        pot_const += m_hr * M_PI / (c->hr.asmth * c->hr.asmth * vol);
     }
   // OldAcc is left as it is on request (accel.c:231-232: set on the highest occupied timebin only)
   f = FinalOut{c->d_idx2.p, pm ? c->d_acc_pm.p : nullptr, pm ? c->d_pot_pm.p : nullptr, c->pm_sorted ? 1 : 0, c->par.G,
@@ -301,6 +305,10 @@ static agrav_status check_params(const agrav_params *p)
     return AGRAV_ERR_BAD_PARAM;
   if(p->group_waves != 0 && p->group_waves != 1 && p->group_waves != 2 && p->group_waves != 4)
     return AGRAV_ERR_BAD_PARAM;
+  // PLACEHIGHRESREGION: a second grid inside a periodic TreePM box (longrange.c:57-59)
+  if(p->placehighresregion < 0 || p->placehighresregion > 255 ||
+     (p->placehighresregion != 0 && (!p->periodic || p->pmgrid <= 0)))
+    return AGRAV_ERR_BAD_PARAM;
   return AGRAV_OK;
 }
 
@@ -394,6 +402,7 @@ void agrav_destroy(agrav_ctx *c)
   if(c->stream)
     (void)hipStreamSynchronize(c->stream);
   pm_free(c);
+  hr_free(c);
   for(auto &e : c->ev)
     if(e)
       (void)hipEventDestroy(e);
@@ -533,6 +542,7 @@ static agrav_status set_particles_common(agrav_ctx *c, int64_t n, const double *
   c->outputs_zero   = false;
   c->have_particles = false;
   c->have_stype     = false;
+  c->have_ptype     = false;
   c->have_oldacc    = false;
   c->have_tree = c->have_tree_force = c->have_pm = c->have_final = false;
   agrav_status st;
@@ -597,6 +607,53 @@ agrav_status agrav_set_particles(agrav_ctx *c, int64_t n, const double *pos, con
 agrav_status agrav_set_particles_device(agrav_ctx *c, int64_t n, const double *pos, const double *mass, const uint8_t *stype)
 {
   return set_particles_common(c, n, pos, mass, stype, hipMemcpyDeviceToDevice);
+}
+
+agrav_status agrav_set_particle_types(agrav_ctx *c, const uint8_t *type)
+{
+  if(!c || !type)
+    return AGRAV_ERR_BAD_PARAM;
+  if(!c->have_particles)
+    return fail(c, AGRAV_ERR_STATE, "set_particle_types before set_particles");
+  agrav_status st;
+  if((st = c->d_ptype_in.ensure(c->n ? c->n : 1)))
+    return st;
+  if(c->n > 0)
+    AG_HIP(hipMemcpyAsync(c->d_ptype_in.p, type, c->n, hipMemcpyHostToDevice, c->stream));
+  AG_HIP(hipStreamSynchronize(c->stream));
+  c->have_ptype = true;
+  c->have_pm = c->have_tree_force = c->have_final = false;  // the high-resolution flags change
+  return AGRAV_OK;
+}
+
+agrav_status agrav_highres_region(agrav_ctx *c, int reset, double *region)
+{
+  if(!c)
+    return AGRAV_ERR_BAD_PARAM;
+  if(!hr_on(c))
+    return fail(c, AGRAV_ERR_STATE, "no high-resolution region: placehighresregion is 0");
+  if(reset)
+    {
+      c->hr.valid        = false;
+      c->hr.kernel_ready = false;
+      return AGRAV_OK;
+    }
+  if(!c->hr.valid)
+    return fail(c, AGRAV_ERR_STATE, "the high-resolution region is not sized yet");
+  if(region)
+    {
+      for(int j = 0; j < 3; j++)
+        {
+          region[j]     = c->hr.xmin[j];
+          region[3 + j] = c->hr.xmax[j];
+          region[6 + j] = c->hr.corner[j];
+          region[9 + j] = c->hr.upper[j];
+        }
+      region[12] = c->hr.meshsize;
+      region[13] = c->hr.asmth;
+      region[14] = c->hr.rcut;
+    }
+  return AGRAV_OK;
 }
 
 agrav_status agrav_set_old_acc(agrav_ctx *c, const float *old_acc)

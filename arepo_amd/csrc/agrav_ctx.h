@@ -89,6 +89,30 @@ struct PmState
   double mass_scale = 0;             // fixed-point units per mass unit
 };
 
+// The high-resolution PM region of PLACEHIGHRESREGION (agrav_pmhr.hip): its geometry
+// (pm_init_regionsize, pm_nonperiodic.c:135-245), the zero-padded grid of 2 PMGRID
+// cells with its rocFFT plans and the transformed two-scale kernel, kept between calls
+// while the region holds its particles.
+struct HrState
+{
+  bool valid = false;
+  double xmin[3] = {0, 0, 0}, xmax[3] = {0, 0, 0}, corner[3] = {0, 0, 0}, upper[3] = {0, 0, 0};
+  double meshsize = 0, asmth = 0, rcut = 0;
+  int grid = 0, prec = -1;
+  bool kernel_ready = false;
+  rocfft_plan fwd = nullptr, bwd = nullptr;
+  rocfft_execution_info info_f = nullptr, info_b = nullptr;
+  DBuf<char> work;
+  DBuf<unsigned long long> rho_fix;  // fixed-point mass on the padded grid
+  DBuf<char> real;                   // padded real grid (density, then potential)
+  DBuf<char> cplx;                   // its transform
+  DBuf<char> kern;                   // transformed kernel with the CIC deconvolution
+  DBuf<char> mesh;                   // {F_x, F_y, F_z, phi} over the inner half (GRID/2)^3
+  DBuf<unsigned long long> ext;      // extent of the region's particles: min[3], max[3] (ordered bits), count, outside
+  DBuf<double> mpart;                // per-block masses of the high-resolution particles
+  double m_high = 0;                 // mass of the high-resolution particles (forcetree.c:440-456)
+};
+
 // device buffers of the unbinding loop (agrav_unbind.hip), kept between calls
 struct UnbindBufs
 {
@@ -205,6 +229,14 @@ struct agrav_ctx
 
   // PM
   agrav::PmState pm;
+  // high-resolution region: particle types (caller order), the high-resolution flag of
+  // every particle (caller and sorted order), the walk's target lists split by it
+  agrav::HrState hr;
+  agrav::DBuf<uint8_t> d_ptype_in, d_hr, d_hr_s;
+  agrav::DBuf<int> d_hr_flag, d_hr_pos, d_hr_list;
+  bool have_ptype = false;
+  const int *walk_tmap = nullptr;  // the target list of the current walk pass (walk_nt >= 0)
+  int64_t walk_nt      = -1;
   agrav::DBuf<double> d_acc_pm, d_pot_pm;  // caller order, or sorted order when pm_sorted
   agrav::DBuf<double> d_acc_pm2, d_pot_pm2;  // caller-order copies made before a partial tree replaces the order
   bool pm_sorted = false;
@@ -267,6 +299,12 @@ agrav_status pm_init(agrav_ctx *c);
 void pm_free(agrav_ctx *c);
 agrav_status pm_force(agrav_ctx *c);
 agrav_status pm_time_settle(agrav_ctx *c);
+bool hr_on(const agrav_ctx *c);
+void hr_free(agrav_ctx *c);
+agrav_status hr_pm(agrav_ctx *c);
+agrav_status hr_flags(agrav_ctx *c);
+agrav_status hr_split_targets(agrav_ctx *c, const int *tmap, int64_t nt, const int **lo, int64_t *n_lo, const int **hi,
+                              int64_t *n_hi);
 agrav_status ewald_tree_table(agrav_ctx *c);
 agrav_status ewald_forcetest_table(agrav_ctx *c);
 agrav_status forcetest(agrav_ctx *c, int ns, const int *sel, double *acc, double *acc_short, double *pot, double *pot_short);

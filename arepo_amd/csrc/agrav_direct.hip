@@ -271,6 +271,8 @@ struct DirectArgs
   double box, half;
   int pm;
   double asmth, vol;
+  const uint8_t *hr;    // high-resolution flags (caller order) or null: Asmth[1] = hr_asmth for them
+  double hr_asmth;
   const double *ewtab;  // FORCETEST table (periodic)
   int chunk;            // sources per chunk
   double *part;         // [nchunk][ns][8]
@@ -306,6 +308,8 @@ __global__ __launch_bounds__(B) void k_forcetest(DirectArgs a)
   const double py = wrapb(a.pos[3 * (int64_t)i + 1], a.periodic, a.box);
   const double pz = wrapb(a.pos[3 * (int64_t)i + 2], a.periodic, a.box);
   const double h_i = a.hsoft[a.stype ? a.stype[i] : 0];
+  // gravtree_forcetest.c:480-483: Asmth[1] for a target in the high-resolution region
+  const double asmth = (a.hr && a.hr[i]) ? a.hr_asmth : a.asmth;
   double acc[3] = {0, 0, 0}, accs[3] = {0, 0, 0}, pot = 0, pots = 0;
   const int j0 = chunk * a.chunk;
   const int j1 = min(a.n, j0 + a.chunk);
@@ -362,14 +366,14 @@ __global__ __launch_bounds__(B) void k_forcetest(DirectArgs a)
             }
           if(a.pm)
             {
-              double u  = 0.5 / a.asmth * r;
+              double u  = 0.5 / asmth * r;
               double ec = erfc(u);
               double ff = ec + 2.0 * u / sqrt(M_PI) * exp(-u * u) - 1.0;
               double fs = fac + fac_newton * ff;
               accs[0] += dx * fs;
               accs[1] += dy * fs;
               accs[2] += dz * fs;
-              pots += wp + wp_newton * (ec - 1.0) + m * M_PI / (a.asmth * a.asmth * a.vol);
+              pots += wp + wp_newton * (ec - 1.0) + m * M_PI / (asmth * asmth * a.vol);
             }
           double fc[4] = {0, 0, 0, 0};
           if(a.periodic)
@@ -415,6 +419,8 @@ struct DirectAllArgs
   int nsoft;  // softening types in the table; a larger type gives NaN results
   int n, periodic, pm, ewald;
   double box, half, asmthfac, asmthinv2, ewald_fac;
+  const uint8_t *hr;  // high-resolution flags or null: those targets take the Asmth[1] constants below
+  double hr_asmthfac, hr_asmthinv2;
   const float *tab;
   const double *ewtab;
   int chunk;
@@ -436,6 +442,10 @@ __global__ __launch_bounds__(B) void k_direct(DirectAllArgs a)
   // the results NaN (through the target's outputs or the source's mass)
   const int t_i    = a.stype ? a.stype[ii] : 0;
   const double h_i = a.hsoft[t_i < a.nsoft ? t_i : 0];
+  // forcetree_walk.c:576-581: Asmth[1] for a target in the high-resolution region
+  const bool hr_i         = a.hr && a.hr[ii];
+  const double asmthfac  = hr_i ? a.hr_asmthfac : a.asmthfac;
+  const double asmthinv2 = hr_i ? a.hr_asmthinv2 : a.asmthinv2;
   double ax = 0, ay = 0, az = 0, pot = 0;
   const int j0 = chunk * a.chunk, j1 = min(a.n, j0 + a.chunk);
   for(int tb = j0; tb < j1; tb += B)
@@ -463,7 +473,7 @@ __global__ __launch_bounds__(B) void k_direct(DirectAllArgs a)
           double ff = 0, fp = 0, fac, wp;
           if(a.pm)
             {
-              double te = a.asmthfac * r;
+              double te = asmthfac * r;
               int ti    = (int)te;
               if(ti >= AG_NTAB)
                 continue;
@@ -474,7 +484,7 @@ __global__ __launch_bounds__(B) void k_direct(DirectAllArgs a)
           if(r >= hmax)
             {
               double rinv = 1.0 / r;
-              fac         = a.pm ? rinv * rinv * rinv + rinv * ff * a.asmthinv2 : rinv * rinv * rinv;
+              fac         = a.pm ? rinv * rinv * rinv + rinv * ff * asmthinv2 : rinv * rinv * rinv;
               wp          = a.pm ? rinv * fp : -rinv;
             }
           else
@@ -495,7 +505,7 @@ __global__ __launch_bounds__(B) void k_direct(DirectAllArgs a)
               if(a.pm && r > 0)
                 {
                   double rinv = 1.0 / r;
-                  fac += rinv * ff * a.asmthinv2;
+                  fac += rinv * ff * asmthinv2;
                   wp += rinv * (fp + 1.0);
                 }
             }
@@ -744,6 +754,13 @@ agrav_status forcetest(agrav_ctx *c, int ns, const int *sel_h, double *acc, doub
   a.pm       = c->par.pmgrid > 0;
   a.asmth    = a.pm ? c->par.asmth * c->par.box_size / c->par.pmgrid : 0;
   a.vol      = a.box * a.box * a.box;
+  if(hr_on(c))
+    {
+      if((st = hr_flags(c)) != AGRAV_OK)
+        return st;
+      a.hr       = c->d_hr.p;
+      a.hr_asmth = c->hr.asmth;
+    }
   a.ewtab    = c->d_ewald_ft.p;
   a.chunk    = chunk;
   a.part     = dpart.p;
@@ -800,6 +817,17 @@ static agrav_status direct_launch(agrav_ctx *c, int64_t n, const double *pos, co
   a.ewald_fac = wc.ewald_fac;
   a.tab       = c->d_tab.p;
   a.ewtab     = ew ? c->d_ewald.p : c->d_soft.p;  // read only when ewald
+  if(hr_on(c) && pos == c->d_pos_in.p)
+    {
+      // the context's own particles: their high-resolution flags
+      agrav_status st;
+      if((st = hr_flags(c)) != AGRAV_OK)
+        return st;
+      const double ai = 0.5 / c->hr.asmth;
+      a.hr            = c->d_hr.p;
+      a.hr_asmthinv2  = ai * ai;
+      a.hr_asmthfac   = ai * (AG_NTAB / (c->par.rcut / 2.0));
+    }
   a.chunk     = chunk;
   a.part      = part;
   dim3 grid((unsigned)((n + B - 1) / B), nchunk);
@@ -919,6 +947,8 @@ agrav_status direct_noalloc(agrav_ctx *c, int64_t n, const double *pos, const do
     return fail(c, AGRAV_ERR_CAPACITY, "gravity_direct_noalloc: more than AGRAV_DIRECT_NOALLOC_MAX_N particles");
   if(c->batch_nsets > 0)
     return fail(c, AGRAV_ERR_BAD_PARAM, "gravity_direct_noalloc: not for a batch context");
+  if(hr_on(c))
+    return fail(c, AGRAV_ERR_BAD_PARAM, "gravity_direct_noalloc: not for a context with a high-resolution region");
   if(c->par.periodic && c->par.pmgrid == 0 && !c->ewald_ready)
     return fail(c, AGRAV_ERR_STATE, "gravity_direct_noalloc: the Ewald table is not ready");
   double *part = (c->device >= 0 && c->device < kMaxDevices) ? g_scratch_ptr[c->device] : nullptr;

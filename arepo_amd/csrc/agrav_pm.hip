@@ -524,6 +524,9 @@ agrav_status pm_force(agrav_ctx *c)
     return st;
   AG_HIP(hipEventRecord(c->ev[5], c->stream));
   st = (c->par.precision == AGRAV_FP32) ? pm_run<float>(c) : pm_run<double>(c);
+  // the high-resolution grid after the periodic one (longrange.c:129-140)
+  if(st == AGRAV_OK && hr_on(c))
+    st = hr_pm(c);
   if(st != AGRAV_OK)
     return st;
   AG_HIP(hipEventRecord(c->ev[6], c->stream));

@@ -1260,9 +1260,17 @@ WalkConst walk_const(const agrav_ctx *c, int use_bh)
 
 // the walk's targets: every particle, or the active-target list (sorted positions in
 // Hilbert order); a partial tree's targets are its own particles
-static const int *target_map(const agrav_ctx *c) { return c->n_active >= 0 && !c->tree_partial ? c->d_tmap.p : nullptr; }
+// (during one pass of a walk split by the high-resolution flag: that pass's list)
+static const int *target_map(const agrav_ctx *c)
+{
+  if(c->walk_nt >= 0)
+    return c->walk_tmap;
+  return c->n_active >= 0 && !c->tree_partial ? c->d_tmap.p : nullptr;
+}
 static int target_count(const agrav_ctx *c)
 {
+  if(c->walk_nt >= 0)
+    return (int)c->walk_nt;
   return c->tree_partial ? (int)c->n_tree : c->n_active >= 0 ? (int)c->n_active : (int)c->n;
 }
 
@@ -1477,6 +1485,8 @@ agrav_status active_targets(agrav_ctx *c)
   return AGRAV_OK;
 }
 
+static agrav_status walk_pass(agrav_ctx *c, int use_bh, const WalkConst &wc, const TreeView &tv);
+
 agrav_status walk_tree(agrav_ctx *c, int use_bh)
 {
   if(!c->have_tree)
@@ -1497,7 +1507,7 @@ agrav_status walk_tree(agrav_ctx *c, int use_bh)
     use_bh = 1;
   const WalkConst wc = walk_const(c, use_bh);
   const TreeView tv  = tree_view(c);
-  const bool pm = wc.pm, ew = wc.ewald, ms = wc.multisoft;
+  const bool ew      = wc.ewald;
   if(ew && !c->ewald_ready)
     {
       if((st = ewald_tree_table(c)) != AGRAV_OK)
@@ -1522,6 +1532,52 @@ agrav_status walk_tree(agrav_ctx *c, int use_bh)
       c->have_tree_force                   = true;
       return AGRAV_OK;
     }
+  c->tim.walk_retries = 0;
+  if(!hr_on(c) || c->batch_nsets > 0)
+    return walk_pass(c, use_bh, wc, tv);
+  // a high-resolution region: its targets take the short-range force with Asmth[1] and
+  // Rcut[1] (forcetree_walk.c:143-150), the others with Asmth[0] and Rcut[0]; one pass
+  // over each list, each in Hilbert order
+  const int *lo = nullptr, *hi = nullptr;
+  int64_t n_lo = 0, n_hi = 0;
+  if((st = hr_split_targets(c, target_map(c), nt, &lo, &n_lo, &hi, &n_hi)) != AGRAV_OK)
+    return st;
+  WalkConst wc1   = wc;
+  const double ai = 0.5 / c->hr.asmth;
+  wc1.rcut        = c->hr.rcut;
+  wc1.rcut2       = c->hr.rcut * c->hr.rcut;
+  wc1.asmthinv2   = ai * ai;
+  wc1.asmthfac    = ai * (AG_NTAB / (c->par.rcut / 2.0));
+  c->walk_tmap = lo;
+  c->walk_nt   = n_lo;
+  st           = n_lo > 0 ? walk_pass(c, use_bh, wc, tv) : AGRAV_OK;
+  const bool fused0   = n_lo == 0 || c->final_fused;
+  const double t_walk = c->tim.treewalk, t_kernel = c->tim.walk_kernel;
+  const int retries   = c->tim.walk_retries;
+  if(st == AGRAV_OK && n_hi > 0)
+    {
+      c->walk_tmap = hi;
+      c->walk_nt   = n_hi;
+      AG_HIP(hipMemsetAsync(c->d_isum.p + 3, 0, sizeof(unsigned long long), s));  // the failed-group counter
+      st = walk_pass(c, use_bh, wc1, tv);
+      c->tim.treewalk += n_lo > 0 ? t_walk : 0;
+      c->tim.walk_kernel += n_lo > 0 ? t_kernel : 0;
+      c->tim.walk_retries += retries;
+      c->final_fused = c->final_fused && fused0;
+    }
+  c->walk_tmap = nullptr;
+  c->walk_nt   = -1;
+  return st;
+}
+
+// one walk over the targets of target_map / target_count with the constants wc; the
+// interaction counters accumulate over the passes of a step (each pass reads their totals)
+static agrav_status walk_pass(agrav_ctx *c, int use_bh, const WalkConst &wc, const TreeView &tv)
+{
+  hipStream_t s   = c->stream;
+  agrav_status st;
+  const bool pm = wc.pm, ew = wc.ewald, ms = wc.multisoft;
+  const int64_t nt = target_count(c);
   c->tim.walk_retries = 0;
   unsigned long long cnt[4] = {0, 0, 0, 0};
   bool have_cnt       = false;  // the counters are read already (group walk, no re-walk)
@@ -1591,7 +1647,7 @@ agrav_status walk_tree(agrav_ctx *c, int use_bh)
       have_cnt     = nf == 0;
       if(nf > 0)
         {
-          c->tim.walk_retries = nf;
+          c->tim.walk_retries += nf;
           launch_plain(pm, ew, nf * gs, c->d_failed.p, gs, s, tv, wc, c, BatchView{});
           AG_HIP(hipGetLastError());
         }

@@ -82,6 +82,10 @@ typedef struct
                                      4 below 2^18 and 2 below 2^20, else 1).  More waves cut the latency of
                                      small sets; results are deterministic for a given value but differ in
                                      the last bits between values */
+  int placehighresregion;         /* PLACEHIGHRESREGION (Template-Config.sh): bit t set for the particle types t that
+                                     define a second, high-resolution PM grid of 2 x PMGRID cells (pm_nonperiodic.c);
+                                     0 = none.  Periodic TreePM only; the types come from agrav_set_particle_types */
+  double enlargeregion;           /* ENLARGEREGION: the high-resolution region's extent factor (<= 0: not set) */
 } agrav_params;
 
 /* Per-stage device times (ms) of the last call, named like AREPO's cpu.txt
@@ -130,6 +134,18 @@ agrav_status agrav_set_particles(agrav_ctx *ctx, int64_t n, const double *pos, c
 /* Same with device pointers (already resident in HBM); copied device-to-device on the ctx stream. */
 agrav_status agrav_set_particles_device(agrav_ctx *ctx, int64_t n, const double *d_pos, const double *d_mass,
                                         const uint8_t *d_softtype);
+/* P[i].Type (allvars.h:1216-1274), host pointer, caller order: with placehighresregion, the types whose
+ * particles size the high-resolution region (pm_init_regionsize, pm_nonperiodic.c:135-245) and count as
+ * high-resolution particles wherever they are (pmforce_is_particle_high_res, :344-371).  Needed after
+ * every set_particles of a context with placehighresregion. */
+agrav_status agrav_set_particle_types(agrav_ctx *ctx, const uint8_t *type);
+/* The high-resolution region of the last PM step (All.Xmintot[1], All.Xmaxtot[1], All.Corner[1],
+ * All.UpperCorner[1], All.TotalMeshSize[1], All.Asmth[1], All.Rcut[1]): region[0..2] xmin, [3..5] xmax,
+ * [6..8] corner, [9..11] upper corner, [12] mesh size, [13] Asmth[1], [14] Rcut[1] (region may be NULL).
+ * AGRAV_ERR_STATE while it is not sized.  reset = 1 drops it: the next PM sizes it from the particles,
+ * as pm_init_regionsize does at start-up (long_range_init_regionsize, longrange.c:72-86); without a reset
+ * it is re-sized only when a high-resolution particle has left it (longrange.c:129-140). */
+agrav_status agrav_highres_region(agrav_ctx *ctx, int reset, double *region);
 /* P[i].OldAcc (accel.c:237-255), host pointer, caller order. */
 agrav_status agrav_set_old_acc(agrav_ctx *ctx, const float *old_acc);
 
