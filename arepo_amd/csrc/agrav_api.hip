@@ -356,10 +356,32 @@ agrav_status agrav_create(const agrav_params *p, agrav_ctx **out)
       c->tab_pot[i]   = (float)(-erfc(u));
       c->tab_force[i] = u > 0 ? (float)((erfc(u) + 2.0 * u / sqrt(M_PI) * exp(-u * u) - 1.0) / (u * u)) : 0.f;
     }
-  float tabs[256];
+  // [force 128 | potential 128] as above, then the FP32 TreePM group walk's tables on the
+  // same u grid, computed in FP64: the force table times du^2 (that walk works in units
+  // of the grid step du), T_i = tab_force[i] du^2, as {T_i, T_(i+1) - T_i} x 128, and
+  // {P_i, P_(i+1) - P_i} x 128 with P = -erfc(u); entry NTAB (beyond the cut) is zero
+  float tabs[256 + 512];
   memcpy(tabs, c->tab_force, 128 * sizeof(float));
   memcpy(tabs + 128, c->tab_pot, 128 * sizeof(float));
-  if((st = c->d_tab.ensure(256)) || (st = c->d_soft.ensure(AG_MAXSOFT + 1)))
+  {
+    const double du = (p->rcut / 2.0) / AG_NTAB;
+    auto tf = [&](int i) {
+      const double u = du * i;
+      return u > 0 ? (erfc(u) + 2.0 * u / sqrt(M_PI) * exp(-u * u) - 1.0) / (u * u) * du * du : 0.0;
+    };
+    for(int i = 0; i <= AG_NTAB; i++)
+      {
+        float *a = tabs + 256 + 2 * i, *b = tabs + 512 + 2 * i;
+        a[0] = a[1] = b[0] = b[1] = 0;
+        if(i == AG_NTAB)
+          continue;
+        a[0] = (float)tf(i);
+        a[1] = (float)(tf(i + 1) - tf(i));
+        b[0] = (float)(-erfc(du * i));
+        b[1] = (float)(-erfc(du * (i + 1)) + erfc(du * i));
+      }
+  }
+  if((st = c->d_tab.ensure(256 + 512)) || (st = c->d_soft.ensure(AG_MAXSOFT + 1)))
     {
       agrav_destroy(c);
       return st;
@@ -458,6 +480,9 @@ void agrav_destroy(agrav_ctx *c)
   c->d_child.release();
   c->d_meta.release();
   c->d_cprec.release();
+  c->d_nmeta.release();
+  c->d_nrecA.release();
+  c->d_nrecB.release();
   c->d_mom_tiles.release();
   c->d_mom_tab.release();
   c->d_mom_tbeg.release();

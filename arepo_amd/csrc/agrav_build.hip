@@ -775,6 +775,26 @@ __global__ void k_fill_children(int m, int n, const int *__restrict__ nextnode, 
   meta[j]       = make_int4(cbeg[j] + nn, o - cbeg[j] - nn, rg.x, min(nn, 15) | (min(rg.y, 255) << 4));
 }
 
+// The walk records of every node that is some node's child, copied to its position in
+// the parent's child list: the FP32 group walk pushes child-list positions (consecutive
+// for the children of one node, so no index is read to push them) and reads a popped
+// entry's records there.
+__global__ void k_node_by_position(int m, int n, const int *__restrict__ cbeg, const int *__restrict__ cnn,
+                                   const int *__restrict__ child, const int4 *__restrict__ meta, const uint4 *__restrict__ recA,
+                                   const uint4 *__restrict__ recB, int4 *nmeta, uint4 *nrecA, uint4 *nrecB)
+{
+  int j = blockIdx.x * B + threadIdx.x;
+  if(j >= m)
+    return;
+  for(int o = cbeg[j], e = cbeg[j] + cnn[j]; o < e; o++)
+    {
+      const int k = child[o] - n;
+      nmeta[o]    = meta[k];
+      nrecA[o]    = recA[k];
+      nrecB[o]    = recB[k];
+    }
+}
+
 // children of each node, by walking its threading from nextnode to sibling
 __global__ void k_count_children(int m, int n, const int *__restrict__ nextnode, const int *__restrict__ sibling,
                                  const int *__restrict__ pnext, int *ccnt, int limit, int *err)
@@ -1519,6 +1539,12 @@ agrav_status build_tree(agrav_ctx *c)
                                          c->d_cx.p, c->d_cy.p, c->d_cz.p, c->d_len.p, c->d_sx.p, c->d_sy.p, c->d_sz.p, c->d_nm.p,
                                          c->d_maxsoft.p, c->d_recA.p, c->d_recB.p, geo.dom, geo.pset);
   AG_HIP(hipGetLastError());
+  if((st = c->d_nmeta.ensure((size_t)M + n)) || (st = c->d_nrecA.ensure((size_t)M + n)) ||
+     (st = c->d_nrecB.ensure((size_t)M + n)))
+    return st;
+  k_node_by_position<<<nblk(M), B, 0, s>>>(M, N, c->d_cbeg.p, c->d_cnn.p, c->d_child.p, c->d_meta.p, c->d_recA.p, c->d_recB.p,
+                                            c->d_nmeta.p, c->d_nrecA.p, c->d_nrecB.p);
+  AG_HIP(hipGetLastError());
 
   // end of the build, recorded before the read-back so that timing it needs no second
   // round trip to the device
@@ -1572,6 +1598,9 @@ TreeView tree_view(const agrav_ctx *c)
   v.child    = c->d_child.p;
   v.meta     = c->d_meta.p;
   v.cprec    = c->d_cprec.p;
+  v.nmeta    = c->d_nmeta.p;
+  v.nrecA    = c->d_nrecA.p;
+  v.nrecB    = c->d_nrecB.p;
   return v;
 }
 

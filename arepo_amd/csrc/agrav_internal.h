@@ -79,6 +79,8 @@ struct TreeView
   const int4 *meta;        // per node {start and count of its particle children in child[], first particle,
                            //           node children | min(particles, 255) << 4}
   const uint4 *cprec;      // FP32 records of particle children, at their child[] positions
+  const int4 *nmeta;       // meta, recA, recB of node children, at their child[] positions
+  const uint4 *nrecA, *nrecB;
 };
 
 struct WalkConst

@@ -582,6 +582,102 @@ __device__ inline void tile_compute_f32(const Vec4<float> *__restrict__ s_list, 
   kahan_add(pp, cpp, fp);
 }
 
+// FP32 TreePM tile loop of one softening type, in table units: coordinates are
+// scaled by asmthfac, so r' = r2' rsq(r2') is the table position directly.  The pair
+// factor is fac = rinv (rinv^2 + T), T the interpolated short-range correction
+// (S(u) - 1) / u^2 du^2 (forcetree_walk.c:367-380 with u = r' du), so that fac r'^3 = S(u) =
+// erfc(u) + 2u/sqrt(pi) exp(-u^2).  The LDS table holds {T_i, T_(i+1) - T_i} and
+// {P_i, P_(i+1) - P_i} for the potential P = -erfc(u), zero at entry NTAB; a pair beyond
+// the cut (tabindex >= NTAB, forcetree_walk.c:372) adds nothing and counts nothing.  In these units a tile sum is the physical one times
+// asmthfac^-2 (accelerations) and asmthfac^-1 (potentials); the caller rescales.  The
+// spline branch (allvars.h:557-573, forcetree_walk.c:429-456) is entered only by tiles
+// that hold an entry within the softening length of the group (`soft`).
+template <bool POT, bool SOFT>
+__device__ inline void tile_loop_f32_pm(const Vec4<float> *__restrict__ s_list, int nl, float tx, float ty, float tz, float hs,
+                                        float hs_inv, const float *__restrict__ tab, float &fx, float &fy, float &fz,
+                                        float &fp, float &cnt)
+{
+  const float h3 = hs_inv * hs_inv * hs_inv;
+#pragma unroll 2  // deeper unrolling costs occupancy (VGPRs) and measured slower
+  for(int k = 0; k < nl; k++)
+    {
+      const Vec4<float> e = s_list[k];
+      const float dx = e.x - tx, dy = e.y - ty, dz = e.z - tz;
+      const float r2   = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, kTinyR2)));
+      const float rinv = __builtin_amdgcn_rsqf(r2);
+      const float te   = r2 * rinv;
+      const int tc     = min((int)te, AG_NTAB);
+      const float w    = __builtin_amdgcn_fractf(te);
+      // {T_i, dT_i} (b64 reads: fewer LDS bank conflicts than wider entries); rc = rinv
+      // within the cut and 0 beyond it, where the Newtonian term of fac must vanish too
+      const float2 q = reinterpret_cast<const float2 *>(tab)[tc];
+      const float ff = fmaf(w, q.y, q.x);
+      float fpt      = 0.f;
+      if(POT)
+        {
+          const float2 pq = reinterpret_cast<const float2 *>(tab)[128 + tc];
+          fpt             = fmaf(w, pq.y, pq.x);
+        }
+      const float rc = tc < AG_NTAB ? rinv : 0.f;
+      float fac = rc * fmaf(rinv, rinv, ff);
+      float wp  = POT ? rinv * fpt : 0.f;
+      if(SOFT && te < hs && tc < AG_NTAB)
+        {
+          // spline softening; u < 1 here
+          const float u  = te * hs_inv;
+          const float u2 = u * u;
+          float fs, ws;
+          if(u < 0.5f)
+            {
+              fs = h3 * ((float)SF1 + u2 * ((float)SF2 * u + (float)SF3));
+              ws = hs_inv * ((float)SF4 + u2 * ((float)SF5 + u2 * ((float)SF6 * u + (float)SF7)));
+            }
+          else
+            {
+              const float ui = __builtin_amdgcn_rcpf(u);
+              const float u3 = u2 * u;
+              fs = h3 * ((float)SF8 + (float)SF9 * u + (float)SF10 * u2 + (float)SF11 * u3 + (float)SF12 * ui * ui * ui);
+              ws = hs_inv * ((float)SF13 + (float)SF14 * ui + u2 * ((float)SF1 + u * ((float)SF15 + u * ((float)SF16 + (float)SF17 * u))));
+            }
+          if(r2 > kTinyR2)
+            {
+              fs += rinv * ff;
+              if(POT)
+                ws += rinv * (fpt + 1.0f);
+            }
+          fac = fs;
+          wp  = ws;
+        }
+      const float mf = e.w * fac;
+      fx = fmaf(dx, mf, fx);
+      fy = fmaf(dy, mf, fy);
+      fz = fmaf(dz, mf, fz);
+      if(POT)
+        fp = fmaf(e.w, wp, fp);
+      cnt = fmaf(rc, te, cnt);  // rinv r = 1 within the cut (to a few ulp), 0 beyond
+    }
+}
+
+template <bool POT>
+__device__ inline void tile_compute_f32_pm(const Vec4<float> *__restrict__ s_list, int nl_in, float tx, float ty, float tz, float hs,
+                                           bool soft, const float *__restrict__ s_tab, float &ax, float &ay, float &az, float &pp,
+                                           float &cax, float &cay, float &caz, float &cpp, int &nin)
+{
+  float fx = 0, fy = 0, fz = 0, fp = 0, cnt = 0;
+  const int nl          = __builtin_amdgcn_readfirstlane(nl_in);  // wave-uniform: scalar loop control
+  const float *tab      = s_tab;
+  const float hs_inv    = __builtin_amdgcn_rcpf(hs);
+  if(soft)
+    tile_loop_f32_pm<POT, true>(s_list, nl, tx, ty, tz, hs, hs_inv, tab, fx, fy, fz, fp, cnt);
+  else
+    tile_loop_f32_pm<POT, false>(s_list, nl, tx, ty, tz, hs, hs_inv, tab, fx, fy, fz, fp, cnt);
+  nin += (int)(cnt + 0.5f);  // a sum of tile-many terms 1 +- 2^-21: round to the count
+  kahan_add(ax, cax, fx);
+  kahan_add(ay, cay, fy);
+  kahan_add(az, caz, fz);
+  kahan_add(pp, cpp, fp);
+}
+
 template <typename T, bool PM, bool EWALD, bool MSOFT>
 __device__ inline void tile_compute(const Vec4<T> *__restrict__ s_list, const T *__restrict__ s_h, int nl, T tx, T ty, T tz, T hi,
                                     const float *__restrict__ s_tab, const WalkConst &wc, T asmthfac, T asmthinv2,
@@ -607,14 +703,24 @@ template <typename T, bool MSOFT> struct WalkLds
 {
   alignas(16) float tab[512];  // short-range tables: FP64 walks [force 128 | potential 128], FP32 walks see tile_compute_f32
   int stack[kStackCap];
+  int map[64];  // source lane of each slot of a particle-append chunk
   Vec4<T> list[kListCap];
   T h[MSOFT ? kListCap : 1];
 };
 
-// load the short-range tables (tabs = [force 128 | potential 128], entries 0..127)
-template <typename T, bool PM> __device__ inline void load_tables(float *s_tab, const float *__restrict__ tabs, const WalkConst &wc)
+// load the short-range tables (tabs = [force 128 | potential 128 | the FP32 TreePM
+// loop's {T_i, dT_i} x 128 | {P_i, dP_i} x 128])
+template <typename T, bool PM, bool FAST>
+__device__ inline void load_tables(float *s_tab, const float *__restrict__ tabs, const WalkConst &wc)
 {
-  if constexpr(sizeof(T) == 4)
+  if constexpr(FAST)
+    {
+      // {T_i, dT_i} x 128 then {P_i, dP_i} x 128 (the latter only when potentials are evaluated)
+      const int nq = wc.eval_pot ? 512 : 256;
+      for(int q = threadIdx.x & 63; q < nq; q += 64)
+        s_tab[q] = tabs[256 + q];
+    }
+  else if constexpr(sizeof(T) == 4)
     {
       // the FP32 tile loop uses the force table pre-multiplied by (0.5/Asmth)^2
       const float tscale = PM ? (float)wc.asmthinv2 : 1.0f;
@@ -660,6 +766,7 @@ __device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, co
                                   unsigned long long (&cnt)[3])
 {
   int *s_stack     = lds.stack;
+  int *s_map       = lds.map;
   Vec4<T> *s_list  = lds.list;
   T *s_h           = lds.h;
   const float *s_tab = lds.tab;
@@ -714,6 +821,17 @@ __device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, co
 
   const T rcut = (T)wc.rcut, rcut2 = (T)wc.rcut2, theta2 = (T)wc.theta2;
   const T asmthfac = (T)wc.asmthfac, asmthinv2 = (T)wc.asmthinv2;
+  // FP32 TreePM of one softening type: tile entries and targets in table units (see
+  // tile_compute_f32_pm), a tile marked `soft` when an entry lies within the
+  // softening length of the group's box
+  constexpr bool kFast = sizeof(T) == 4 && PM && !EWALD && !MSOFT;
+  // FP32 walks of one softening type keep node entries on the stack as positions in
+  // their parent's child list (e >= N: position e - N, records in nmeta/nrecA/nrecB),
+  // so that pushing the children of an opened node reads nothing; the top entry is a
+  // node index (e <= -2: node -e - 2).  Other walks push node indices (e >= N: node e - N).
+  constexpr bool kPos  = sizeof(T) == 4 && !MSOFT;
+  const T cs           = kFast ? asmthfac : (T)1;
+  const T hs2          = hi_min * hi_min * (T)1.01;
   // TreePM pairs take their periodic image relative to the group centre, which is every
   // target's nearest image of every source within Rcut only while the group's
   // half-extent plus Rcut stays below half a box.  Groups of Hilbert-consecutive
@@ -731,11 +849,28 @@ __device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, co
   int sp = 1;
   __builtin_amdgcn_wave_barrier();
   if(lane == 0)
-    s_stack[0] = root;
+    s_stack[0] = (kPos && root >= N) ? -(root - N) - 2 : root;
   int nl = 0;
   T ax = 0, ay = 0, az = 0, pp = 0, cax = 0, cay = 0, caz = 0, cpp = 0;
   int nin        = 0;
   unsigned long long n_popped = 0, n_tiled = 0;  // diagnostics: stack entries evaluated, tile entries computed
+  bool tile_soft       = false;
+  auto flush_tile      = [&]() {
+    if constexpr(kFast)
+      {
+        if(wc.eval_pot)
+          tile_compute_f32_pm<true>(s_list, nl, tx * cs, ty * cs, tz * cs, hi * cs, tile_soft, s_tab, ax, ay, az, pp, cax, cay, caz,
+                                    cpp, nin);
+        else
+          tile_compute_f32_pm<false>(s_list, nl, tx * cs, ty * cs, tz * cs, hi * cs, tile_soft, s_tab, ax, ay, az, pp, cax, cay,
+                                     caz, cpp, nin);
+      }
+    else
+      tile_compute<T, PM, EWALD, MSOFT>(s_list, s_h, nl, tx, ty, tz, hi, s_tab, wc, asmthfac, asmthinv2, ewald_f, ewald_d, ax, ay,
+                                        az, pp, cax, cay, caz, cpp, nin);
+    tile_soft = false;
+  };
+
   bool overflow  = false;
   int pushed     = 1;
   const int max_push = N + tv.m;  // each entry is pushed at most once per group
@@ -802,11 +937,18 @@ __device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, co
               rA[h]       = make_uint4(0, 0, 0, 0);
               rB[h]       = make_uint4(0, 0, 0, 0);
               rM[h]       = make_int4(0, 0, 0, 0);
-              if(e >= N)
+              if(kPos && e >= N)
                 {
-                  rM[h] = tv.meta[e - N];
-                  rA[h] = tv.recA[e - N];
-                  rB[h] = tv.recB[e - N];
+                  rM[h] = tv.nmeta[e - N];
+                  rA[h] = tv.nrecA[e - N];
+                  rB[h] = tv.nrecB[e - N];
+                }
+              else if(e >= N || e < -1)
+                {
+                  const int jn = e >= N ? e - N : -e - 2;
+                  rM[h]        = tv.meta[jn];
+                  rA[h]        = tv.recA[jn];
+                  rB[h]        = tv.recB[jn];
                 }
               else if(e >= 0)
                 rA[h] = tv.prec[e];
@@ -823,10 +965,11 @@ __device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, co
           int pbase = 0, pcnt = 0;
           int4 mt   = make_int4(0, 0, 0, 0);  // node meta {particle children start, count, first particle, cnn | count8 << 4}
           Vec4<T> item{0, 0, 0, 0};
-          T hj = 0;
-          if(e >= N)
+          T hj      = 0;
+          bool near = false;  // within the softening length of the group's box
+          if(e >= N || (kPos && e < -1))
             {
-              j = e - N;
+              j = e >= N ? e - N : -e - 2;  // node index (walks without kPos) or child-list position
               T m, ln, dcx, dcy, dcz, sxr, syr, szr;
               if constexpr(sizeof(T) == 4)
                 {
@@ -886,7 +1029,8 @@ __device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, co
                   // table test (tabindex < NTAB, forcetree_walk.c:372) for every target
                   if(PM && AG_NODE_RCUT_PRUNE && !open && r2 > rcut2 * (T)1.0001)
                     act = 0;
-                  item = Vec4<T>{sxr, syr, szr, m};
+                  item = Vec4<T>{sxr * cs, syr * cs, szr * cs, m};
+                  near = r2 < hs2;
                   if(open)
                     {
                       // a small opened node is expanded straight into its particles, which are
@@ -920,7 +1064,7 @@ __device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, co
                   m = tv.pm[e];
                 }
               act  = 3;
-              item = Vec4<T>{x, y, z, m};
+              item = Vec4<T>{x * cs, y * cs, z * cs, m};
               if(MSOFT)
                 hj = (T)hsoft[tv.ps[e]];
               if(PM)
@@ -930,8 +1074,10 @@ __device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, co
                   ddx   = ddx > 0 ? ddx : 0;
                   ddy   = ddy > 0 ? ddy : 0;
                   ddz   = ddz > 0 ? ddz : 0;
-                  if(ddx * ddx + ddy * ddy + ddz * ddz > rcut2 * (T)1.0001)
+                  const T dd2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                  if(dd2 > rcut2 * (T)1.0001)
                     act = 0;
+                  near = dd2 < hs2;
                 }
             }
 
@@ -939,15 +1085,16 @@ __device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, co
           const bool accp   = (act == 1 || act == 3) && item.w != (T)0 && keep_ia;
           const uint64_t bm = __ballot(accp);
           const int cnt     = __popcll(bm);
+          const bool soft_new = kFast && __ballot(accp && near) != 0;
           if(nl + cnt > kListCap)
             {
               __builtin_amdgcn_wave_barrier();
               n_tiled += nl;
-              tile_compute<T, PM, EWALD, MSOFT>(s_list, s_h, nl, tx, ty, tz, hi, s_tab, wc, asmthfac, asmthinv2, ewald_f, ewald_d,
-                                                ax, ay, az, pp, cax, cay, caz, cpp, nin);
+              flush_tile();
               nl = 0;
               __builtin_amdgcn_wave_barrier();
             }
+          tile_soft = tile_soft || soft_new;
           if(accp)
             {
               int slot     = nl + __popcll(bm & lt);
@@ -958,9 +1105,10 @@ __device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, co
           nl += cnt;
 
           // particles of small opened nodes (contiguous ranges) and the particle children of
-          // opened nodes go straight into the tile, one particle per lane and round so that a
-          // round never adds more than 64 entries; particles beyond Rcut of the whole group
-          // are dropped here
+          // opened nodes go straight into the tile: the ranges of all lanes are laid end to
+          // end and read 64 particles per chunk, one per lane, so that a chunk never adds
+          // more than 64 entries and every read of a chunk is in flight at once; particles
+          // beyond Rcut of the whole group are dropped here
           {
             int pc = 0;
             bool from_list = false;
@@ -974,17 +1122,31 @@ __device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, co
               }
             if(!keep_ia)
               pc = 0;
-            const int maxc = wave_max(pc);
-            for(int q = 0; q < maxc; q++)
+            const int pincl = wave_incl_scan_i(pc);
+            const int ptot  = __builtin_amdgcn_readlane(pincl, 63);
+            const int pst   = pincl - pc;  // first slot of this lane's range
+            for(int k0 = 0; k0 < ptot; k0 += 64)
               {
-                bool keep = q < pc;
+                // slot t of the chunk holds particle k0 + t: mark each slot with the lane whose
+                // range covers it, then read the range start, base and source from that lane
+                const int lo = pst - k0 > 0 ? pst - k0 : 0, hi = pst + pc - k0 < 64 ? pst + pc - k0 : 64;
+                for(int t = lo; t < hi; t++)
+                  s_map[t] = lane;
+                __builtin_amdgcn_wave_barrier();
+                bool keep   = k0 + lane < ptot;
+                const int L = keep ? s_map[lane] : 0;
+                __builtin_amdgcn_wave_barrier();  // the map is read before the next chunk writes it
+                const int q    = k0 + lane - __shfl(pst, L, 64);
+                const int pb   = __shfl(pbase, L, 64);
+                const bool fl  = __shfl((int)from_list, L, 64) != 0;
                 T x = 0, y = 0, z = 0, m = 0, hp = 0;
+                bool pnear = false;
                 if(keep)
                   {
                     if constexpr(sizeof(T) == 4)
                       {
                         // particle children come as records copied next to the child list
-                        const uint4 P = from_list ? tv.cprec[pbase + q] : tv.prec[pbase + q];
+                        const uint4 P = fl ? tv.cprec[pb + q] : tv.prec[pb + q];
                         x             = (T)(int)(P.x - cfx) * unit;
                         y             = (T)(int)(P.y - cfy) * unit;
                         z             = (T)(int)(P.z - cfz) * unit;
@@ -992,14 +1154,14 @@ __device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, co
                       }
                     else
                       {
-                        const int e2 = from_list ? tv.child[pbase + q] : pbase + q;
+                        const int e2 = fl ? tv.child[pb + q] : pb + q;
                         x            = nearest_d(tv.px[e2] - c0, wc);
                         y            = nearest_d(tv.py[e2] - c1, wc);
                         z            = nearest_d(tv.pz[e2] - c2, wc);
                         m            = tv.pm[e2];
                       }
                     if(MSOFT)
-                      hp = (T)hsoft[tv.ps[from_list ? tv.child[pbase + q] : pbase + q]];
+                      hp = (T)hsoft[tv.ps[fl ? tv.child[pb + q] : pb + q]];
                     keep = m != (T)0;
                     if(PM && keep)
                       {
@@ -1007,24 +1169,27 @@ __device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, co
                         ddx   = ddx > 0 ? ddx : 0;
                         ddy   = ddy > 0 ? ddy : 0;
                         ddz   = ddz > 0 ? ddz : 0;
-                        keep  = ddx * ddx + ddy * ddy + ddz * ddz <= rcut2 * (T)1.0001;
+                        const T dd2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                        keep        = dd2 <= rcut2 * (T)1.0001;
+                        pnear       = dd2 < hs2;
                       }
                   }
                 const uint64_t rb = __ballot(keep);
                 const int rcnt    = __popcll(rb);
+                const bool psoft  = kFast && __ballot(keep && pnear) != 0;
                 if(nl + rcnt > kListCap)
                   {
                     __builtin_amdgcn_wave_barrier();
                     n_tiled += nl;
-                    tile_compute<T, PM, EWALD, MSOFT>(s_list, s_h, nl, tx, ty, tz, hi, s_tab, wc, asmthfac, asmthinv2, ewald_f,
-                                                      ewald_d, ax, ay, az, pp, cax, cay, caz, cpp, nin);
+                    flush_tile();
                     nl = 0;
                     __builtin_amdgcn_wave_barrier();
                   }
+                tile_soft = tile_soft || psoft;
                 if(keep)
                   {
                     const int slot = nl + __popcll(rb & lt);
-                    s_list[slot]   = Vec4<T>{x, y, z, m};
+                    s_list[slot]   = Vec4<T>{x * cs, y * cs, z * cs, m};
                     if(MSOFT)
                       s_h[slot] = hp;
                   }
@@ -1048,8 +1213,29 @@ __device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, co
       int o = sp + incl - nch_all;
 #pragma unroll
       for(int h = 0; h < kPop; h++)
-        for(int k = 0; k < nchs[h]; k++)
-          s_stack[o++] = tv.child[cbs[h] + k];
+        {
+          const int nc = nchs[h];
+          if(kPos)
+            {
+              // node children are consecutive in the child list: push their positions
+              for(int k = 0; k < nc; k++)
+                s_stack[o + k] = N + cbs[h] + k;
+              o += nc;
+            }
+          else if(nc > 0)
+            {
+              // all child indices are read before the first is stored (one memory round trip)
+              int cv[8];
+#pragma unroll
+              for(int k = 0; k < 8; k++)
+                cv[k] = tv.child[cbs[h] + (k < nc ? k : nc - 1)];
+#pragma unroll
+              for(int k = 0; k < 8; k++)
+                if(k < nc)
+                  s_stack[o + k] = cv[k];
+              o += nc;
+            }
+        }
       sp += tot;
       __builtin_amdgcn_wave_barrier();
     }
@@ -1058,9 +1244,15 @@ __device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, co
     return false;
   __builtin_amdgcn_wave_barrier();
   n_tiled += nl;
-  tile_compute<T, PM, EWALD, MSOFT>(s_list, s_h, nl, tx, ty, tz, hi, s_tab, wc, asmthfac, asmthinv2, ewald_f, ewald_d, ax, ay, az,
-                                    pp, cax, cay, caz, cpp, nin);
+  flush_tile();
 
+  if constexpr(kFast)
+    {
+      // back from table units: accelerations x asmthfac^2, potentials x asmthfac
+      const T c2 = cs * cs;
+      ax *= c2, ay *= c2, az *= c2, cax *= c2, cay *= c2, caz *= c2;
+      pp *= cs, cpp *= cs;
+    }
   out = GroupAcc<T>{ax, ay, az, pp, cax, cay, caz, cpp, nin};
   unsigned long long v = has ? (unsigned long long)nin : 0ull;
   for(int off = 32; off > 0; off >>= 1)
@@ -1095,7 +1287,8 @@ __device__ inline int xcd_block(int b, int nb)
 // group's traversal is shared, see walk_group); wave 0 adds the waves' partial
 // sums in wave order, so the result is deterministic for a given W.
 template <typename T, bool PM, bool EWALD, bool MSOFT, int W>
-__global__ __launch_bounds__(64 * W) void k_walk_group(TreeView tv, WalkConst wc, const double *__restrict__ hsoft,
+// the FP32 TreePM walk of one softening type is held to 96 VGPRs (5 waves per SIMD): measured faster
+__global__ __launch_bounds__(64 * W, (sizeof(T) == 4 && PM && !EWALD && !MSOFT) ? 5 : 1) void k_walk_group(TreeView tv, WalkConst wc, const double *__restrict__ hsoft,
                                                        const float *__restrict__ oldacc_s, const float *__restrict__ tabs,
                                                        const float *__restrict__ ewald_f, const double *__restrict__ ewald_d, int gs,
                                                        const int *__restrict__ tmap, int ntarg, double *at, double *pt, int *nint,
@@ -1104,7 +1297,7 @@ __global__ __launch_bounds__(64 * W) void k_walk_group(TreeView tv, WalkConst wc
   __shared__ WalkLds<T, MSOFT> lds[W];
   __shared__ int s_fail;
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  load_tables<T, PM>(lds[wv].tab, tabs, wc);
+  load_tables<T, PM, sizeof(T) == 4 && PM && !EWALD && !MSOFT>(lds[wv].tab, tabs, wc);
   if(threadIdx.x == 0)
     s_fail = 0;
   const int g  = xcd_block(blockIdx.x, gridDim.x);
@@ -1190,7 +1383,7 @@ __global__ __launch_bounds__(64) void k_walk_batch(TreeView tv, WalkConst wc, Ba
                                                    int *nint, int *failed, int *nfailed, unsigned long long *isum)
 {
   __shared__ WalkLds<T, false> lds;
-  load_tables<T, false>(lds.tab, tabs, wc);
+  load_tables<T, false, false>(lds.tab, tabs, wc);
   unsigned long long cnt[3] = {0, 0, 0};
   for(;;)
     {
