@@ -316,16 +316,18 @@ static constexpr int kStackCap = 1024;
 #endif
 static constexpr int kListCap  = 128;
 
+// Inclusive prefix sum over the 64 lanes of a wavefront (all lanes active) with DPP
+// moves: shifts by 1, 2, 4, 8 within each row of 16 lanes, then the last lane of row
+// 0 (2) added to row 1 (3) and the last lane of row 1 to rows 2 and 3.  A lane whose
+// source lies outside its row, or a row outside the row mask, adds the `old` operand 0.
 __device__ inline int wave_incl_scan_i(int v)
 {
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for(int d = 1; d < 64; d <<= 1)
-    {
-      int t = __shfl_up(v, d, 64);
-      if(lane >= d)
-        v += t;
-    }
+  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);  // row_shr:1
+  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);  // row_shr:2
+  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);  // row_shr:4
+  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);  // row_shr:8
+  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15 into rows 1, 3
+  v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31 into rows 2, 3
   return v;
 }
 template <typename T> __device__ inline T wave_min(T v)
