@@ -705,7 +705,7 @@ template <typename T, bool MSOFT> struct WalkLds
 {
   alignas(16) float tab[512];  // short-range tables: FP64 walks [force 128 | potential 128], FP32 walks see tile_compute_f32
   int stack[kStackCap];
-  int map[64];  // source lane of each slot of a particle-append chunk
+  int map[64];  // list and index offset of each slot of a particle-append chunk
   Vec4<T> list[kListCap];
   T h[MSOFT ? kListCap : 1];
 };
@@ -1129,18 +1129,19 @@ __device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, co
             const int pst   = pincl - pc;  // first slot of this lane's range
             for(int k0 = 0; k0 < ptot; k0 += 64)
               {
-                // slot t of the chunk holds particle k0 + t: mark each slot with the lane whose
-                // range covers it, then read the range start, base and source from that lane
+                // slot t of the chunk holds particle k0 + t, at index pbase - pst + k0 + t of
+                // the list (child list or particles) of the lane whose range covers it: that
+                // lane marks the slot with pbase - pst + 512 (>= 1) and the list in bit 31
                 const int lo = pst - k0 > 0 ? pst - k0 : 0, hi = pst + pc - k0 < 64 ? pst + pc - k0 : 64;
+                const int mark = (pbase - pst + 512) | (from_list ? (int)0x80000000 : 0);
                 for(int t = lo; t < hi; t++)
-                  s_map[t] = lane;
+                  s_map[t] = mark;
                 __builtin_amdgcn_wave_barrier();
-                bool keep   = k0 + lane < ptot;
-                const int L = keep ? s_map[lane] : 0;
+                bool keep    = k0 + lane < ptot;
+                const int mk = keep ? s_map[lane] : 0;
                 __builtin_amdgcn_wave_barrier();  // the map is read before the next chunk writes it
-                const int q    = k0 + lane - __shfl(pst, L, 64);
-                const int pb   = __shfl(pbase, L, 64);
-                const bool fl  = __shfl((int)from_list, L, 64) != 0;
+                const bool fl = mk < 0;
+                const int pi2 = (mk & 0x7fffffff) - 512 + k0 + lane;
                 T x = 0, y = 0, z = 0, m = 0, hp = 0;
                 bool pnear = false;
                 if(keep)
@@ -1148,7 +1149,7 @@ __device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, co
                     if constexpr(sizeof(T) == 4)
                       {
                         // particle children come as records copied next to the child list
-                        const uint4 P = fl ? tv.cprec[pb + q] : tv.prec[pb + q];
+                        const uint4 P = fl ? tv.cprec[pi2] : tv.prec[pi2];
                         x             = (T)(int)(P.x - cfx) * unit;
                         y             = (T)(int)(P.y - cfy) * unit;
                         z             = (T)(int)(P.z - cfz) * unit;
@@ -1156,14 +1157,14 @@ __device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, co
                       }
                     else
                       {
-                        const int e2 = fl ? tv.child[pb + q] : pb + q;
+                        const int e2 = fl ? tv.child[pi2] : pi2;
                         x            = nearest_d(tv.px[e2] - c0, wc);
                         y            = nearest_d(tv.py[e2] - c1, wc);
                         z            = nearest_d(tv.pz[e2] - c2, wc);
                         m            = tv.pm[e2];
                       }
                     if(MSOFT)
-                      hp = (T)hsoft[tv.ps[fl ? tv.child[pb + q] : pb + q]];
+                      hp = (T)hsoft[tv.ps[fl ? tv.child[pi2] : pi2]];
                     keep = m != (T)0;
                     if(PM && keep)
                       {
@@ -1205,7 +1206,7 @@ __device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, co
 
       // push the children of opened nodes
       const int incl = wave_incl_scan_i(nch_all);
-      const int tot  = __shfl(incl, 63, 64);
+      const int tot  = __builtin_amdgcn_readlane(incl, 63);
       pushed += tot;
       if(sp + tot > kStackCap || pushed > max_push)
         {
