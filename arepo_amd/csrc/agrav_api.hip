@@ -532,6 +532,21 @@ agrav_status agrav_set_softenings(agrav_ctx *c, int nsoft, const double *eps)
   for(int i = 0; i < nsoft; i++)
     if(!(eps[i] > 0))
       return fail(c, AGRAV_ERR_BAD_PARAM, "softening must be positive");
+  if(c->have_stype && c->n > 0 && nsoft < c->soft.nsoft)
+    {
+      // a shorter table must still cover the types of the particles held
+      agrav_status st;
+      if((st = c->d_counters.ensure(128)))
+        return st;
+      int bad = 0, *d_bad = c->d_counters.p + 100;
+      AG_HIP(hipMemsetAsync(d_bad, 0, sizeof(int), c->stream));
+      k_check_stype<<<(unsigned)((c->n + 255) / 256), 256, 0, c->stream>>>(c->d_stype_in.p, c->n, nsoft, d_bad);
+      AG_HIP(hipGetLastError());
+      AG_HIP(hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      AG_HIP(hipStreamSynchronize(c->stream));
+      if(bad)
+        return fail(c, AGRAV_ERR_BAD_PARAM, "softening type out of range of the new table");
+    }
   c->par.n_soft_types = nsoft;
   for(int i = 0; i < nsoft; i++)
     c->par.softening[i] = eps[i];
@@ -795,7 +810,7 @@ agrav_status agrav_gravity_timesteps(agrav_ctx *c, const agrav_timestep_params *
   if(pm && n > 0)
     {
       // the PM accelerations are in sorted order when the tree of the step was current
-      const uint8_t *stp = c->pm_sorted ? (c->soft.nsoft > 1 || c->have_stype ? c->d_ps.p : nullptr)
+      const uint8_t *stp = c->pm_sorted ? (c->soft.nsoft > 1 ? c->d_ps.p : nullptr)
                                         : (c->have_stype ? c->d_stype_in.p : nullptr);
       k_pm_dtmin<<<g, B, 0, s>>>(n, c->d_acc_pm.p, stp, c->d_soft.p, a, d_min);
       AG_HIP(hipGetLastError());

@@ -1392,7 +1392,9 @@ agrav_status build_tree(agrav_ctx *c)
   if((st = c->d_px.ensure(n)) || (st = c->d_py.ensure(n)) || (st = c->d_pz.ensure(n)) || (st = c->d_pm.ensure(n)) ||
      (st = c->d_prec.ensure(n)) || (st = c->d_pnext.ensure(n)) || (st = c->d_oldacc_s.ensure(n)))
     return st;
-  const bool multisoft = c->soft.nsoft > 1 || c->have_stype;
+  // per-particle softening types matter only with several table entries: types are
+  // checked against the table, so with one entry every particle has type 0
+  const bool multisoft = c->soft.nsoft > 1;
   if(multisoft && (st = c->d_ps.ensure(n)))
     return st;
   k_gather<<<nblk(n), B, 0, s>>>(c->d_idx2.p, c->d_pos_in.p, c->d_mass_in.p, c->have_stype ? c->d_stype_in.p : nullptr,
@@ -1575,7 +1577,7 @@ TreeView tree_view(const agrav_ctx *c)
   v.py       = c->d_py.p;
   v.pz       = c->d_pz.p;
   v.pm       = c->d_pm.p;
-  v.ps       = (c->soft.nsoft > 1 || c->have_stype) ? c->d_ps.p : nullptr;
+  v.ps       = (c->soft.nsoft > 1) ? c->d_ps.p : nullptr;
   v.pnext    = c->d_pnext.p;
   v.prec     = c->d_prec.p;
   v.sx       = c->d_sx.p;

@@ -593,8 +593,10 @@ __device__ inline void tile_compute_f32(const Vec4<float> *__restrict__ s_list, 
 // the cut (tabindex >= NTAB, forcetree_walk.c:372) adds nothing and counts nothing.  In these units a tile sum is the physical one times
 // asmthfac^-2 (accelerations) and asmthfac^-1 (potentials); the caller rescales.  The
 // spline branch (allvars.h:557-573, forcetree_walk.c:429-456) is entered only by tiles
-// that hold an entry within the softening length of the group (`soft`).
-template <bool POT, bool SOFT>
+// that hold an entry within the softening length of the group (`soft`).  Without
+// CLAMP the table index is not clamped to NTAB: the caller has shown that every pair of
+// the group lies below 2 NTAB table units, and the LDS table is zero up to there.
+template <bool POT, bool SOFT, bool CLAMP>
 __device__ inline void tile_loop_f32_pm(const Vec4<float> *__restrict__ s_list, int nl, float tx, float ty, float tz, float hs,
                                         float hs_inv, const float *__restrict__ tab, float &fx, float &fy, float &fz,
                                         float &fp, float &cnt)
@@ -608,7 +610,7 @@ __device__ inline void tile_loop_f32_pm(const Vec4<float> *__restrict__ s_list, 
       const float r2   = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, kTinyR2)));
       const float rinv = __builtin_amdgcn_rsqf(r2);
       const float te   = r2 * rinv;
-      const int tc     = min((int)te, AG_NTAB);
+      const int tc     = CLAMP ? min((int)te, AG_NTAB) : (int)te;
       const float w    = __builtin_amdgcn_fractf(te);
       // {T_i, dT_i} (b64 reads: fewer LDS bank conflicts than wider entries); rc = rinv
       // within the cut and 0 beyond it, where the Newtonian term of fac must vanish too
@@ -662,17 +664,24 @@ __device__ inline void tile_loop_f32_pm(const Vec4<float> *__restrict__ s_list, 
 
 template <bool POT>
 __device__ inline void tile_compute_f32_pm(const Vec4<float> *__restrict__ s_list, int nl_in, float tx, float ty, float tz, float hs,
-                                           bool soft, const float *__restrict__ s_tab, float &ax, float &ay, float &az, float &pp,
+                                           bool soft, bool compact, const float *__restrict__ s_tab, float &ax, float &ay, float &az, float &pp,
                                            float &cax, float &cay, float &caz, float &cpp, int &nin)
 {
   float fx = 0, fy = 0, fz = 0, fp = 0, cnt = 0;
   const int nl          = __builtin_amdgcn_readfirstlane(nl_in);  // wave-uniform: scalar loop control
   const float *tab      = s_tab;
   const float hs_inv    = __builtin_amdgcn_rcpf(hs);
-  if(soft)
-    tile_loop_f32_pm<POT, true>(s_list, nl, tx, ty, tz, hs, hs_inv, tab, fx, fy, fz, fp, cnt);
+  if(!POT && compact)
+    {
+      if(soft)
+        tile_loop_f32_pm<false, true, false>(s_list, nl, tx, ty, tz, hs, hs_inv, tab, fx, fy, fz, fp, cnt);
+      else
+        tile_loop_f32_pm<false, false, false>(s_list, nl, tx, ty, tz, hs, hs_inv, tab, fx, fy, fz, fp, cnt);
+    }
+  else if(soft)
+    tile_loop_f32_pm<POT, true, true>(s_list, nl, tx, ty, tz, hs, hs_inv, tab, fx, fy, fz, fp, cnt);
   else
-    tile_loop_f32_pm<POT, false>(s_list, nl, tx, ty, tz, hs, hs_inv, tab, fx, fy, fz, fp, cnt);
+    tile_loop_f32_pm<POT, false, true>(s_list, nl, tx, ty, tz, hs, hs_inv, tab, fx, fy, fz, fp, cnt);
   nin += (int)(cnt + 0.5f);  // a sum of tile-many terms 1 +- 2^-21: round to the count
   kahan_add(ax, cax, fx);
   kahan_add(ay, cay, fy);
@@ -717,10 +726,10 @@ __device__ inline void load_tables(float *s_tab, const float *__restrict__ tabs,
 {
   if constexpr(FAST)
     {
-      // {T_i, dT_i} x 128 then {P_i, dP_i} x 128 (the latter only when potentials are evaluated)
-      const int nq = wc.eval_pot ? 512 : 256;
-      for(int q = threadIdx.x & 63; q < nq; q += 64)
-        s_tab[q] = tabs[256 + q];
+      // {T_i, dT_i} x 128, then {P_i, dP_i} x 128 when potentials are evaluated, else zeros
+      // (the force table read past NTAB by loops without an index clamp)
+      for(int q = threadIdx.x & 63; q < 512; q += 64)
+        s_tab[q] = (q < 256 || wc.eval_pot) ? tabs[256 + q] : 0.f;
     }
   else if constexpr(sizeof(T) == 4)
     {
@@ -834,6 +843,10 @@ __device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, co
   constexpr bool kPos  = sizeof(T) == 4 && !MSOFT;
   const T cs           = kFast ? asmthfac : (T)1;
   const T hs2          = hi_min * hi_min * (T)1.01;
+  // every tile entry lies within Rcut (x 1.00005) of the group's box, so no pair is
+  // farther than that plus the box diagonal: below 2 NTAB - 1 table units the loop needs
+  // no index clamp (the table is zero from NTAB to 2 NTAB)
+  const bool compact = kFast && (rcut * (T)1.0001 + (T)2 * sqrt(hx * hx + hy * hy + hz * hz)) * asmthfac < (T)(2 * AG_NTAB - 1);
   // TreePM pairs take their periodic image relative to the group centre, which is every
   // target's nearest image of every source within Rcut only while the group's
   // half-extent plus Rcut stays below half a box.  Groups of Hilbert-consecutive
@@ -861,11 +874,11 @@ __device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, co
     if constexpr(kFast)
       {
         if(wc.eval_pot)
-          tile_compute_f32_pm<true>(s_list, nl, tx * cs, ty * cs, tz * cs, hi * cs, tile_soft, s_tab, ax, ay, az, pp, cax, cay, caz,
-                                    cpp, nin);
+          tile_compute_f32_pm<true>(s_list, nl, tx * cs, ty * cs, tz * cs, hi * cs, tile_soft, false, s_tab, ax, ay, az, pp, cax,
+                                    cay, caz, cpp, nin);
         else
-          tile_compute_f32_pm<false>(s_list, nl, tx * cs, ty * cs, tz * cs, hi * cs, tile_soft, s_tab, ax, ay, az, pp, cax, cay,
-                                     caz, cpp, nin);
+          tile_compute_f32_pm<false>(s_list, nl, tx * cs, ty * cs, tz * cs, hi * cs, tile_soft, compact, s_tab, ax, ay, az, pp,
+                                     cax, cay, caz, cpp, nin);
       }
     else
       tile_compute<T, PM, EWALD, MSOFT>(s_list, s_h, nl, tx, ty, tz, hi, s_tab, wc, asmthfac, asmthinv2, ewald_f, ewald_d, ax, ay,
@@ -1447,7 +1460,7 @@ WalkConst walk_const(const agrav_ctx *c, int use_bh)
   wc.use_bh    = use_bh;
   wc.ewald     = c->par.periodic && c->par.pmgrid == 0;
   wc.ewald_fac = wc.ewald ? 2.0 * 64 / c->par.box_size : 0;
-  wc.multisoft = c->soft.nsoft > 1 || c->have_stype;
+  wc.multisoft = c->soft.nsoft > 1;
   wc.fix_unit  = (float)c->h_dom.fix_unit;
   wc.leaf_bucket = c->par.leaf_bucket > 0 ? c->par.leaf_bucket : 0;
   wc.eval_pot    = c->par.eval_potential ? 1 : 0;

@@ -132,6 +132,46 @@ def test_several_softening_types(agrav, oracle, case):
         assert np.percentile(eg, 99) <= 1.05 * np.percentile(eo, 99) + 2e-6, (name, np.percentile(eg, 99))
 
 
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_types_with_one_softening_equal_no_types(agrav, prec):
+    """With a one-entry softening table, per-particle types (all 0) change nothing: the same
+    results, bit for bit, as a set without types (the single-softening walk runs for both)."""
+    g0 = np.random.Generator(np.random.Philox(47))
+    n = 6000
+    pos = g0.random((n, 3))
+    pos[:1500] = 0.5 + 0.02 * (g0.random((1500, 3)) - 0.5)
+    mass = np.full(n, 1.0 / n)
+    kw = dict(periodic=True, box_size=1.0, pmgrid=32, soft=(0.002,), err_tol_force_acc=0.0025, opening=1,
+              precision=agrav.FP32 if prec == "fp32" else agrav.FP64, walk=agrav.WALK_GROUP)
+    out = []
+    for st in (None, np.zeros(n, np.uint8)):
+        with agrav.Gravity(agrav.make_params(**kw)) as g:
+            g.set_particles(pos, mass, softtype=st)
+            g.compute_grav_accelerations(first_step=True)
+            g.compute_grav_accelerations(first_step=False)
+            out.append(g.get_accelerations())
+    for k in ("acc", "nint", "old_acc"):
+        assert np.array_equal(out[0][k], out[1][k]), k
+
+
+def test_shorter_softening_table_must_cover_types(agrav):
+    g0 = np.random.Generator(np.random.Philox(53))
+    n = 3000
+    pos, mass = g0.random((n, 3)), np.full(n, 1.0 / n)
+    stype = (np.arange(n) % 2).astype(np.uint8)
+    with agrav.Gravity(agrav.make_params(soft=(0.002, 0.004))) as g:
+        g.set_particles(pos, mass, softtype=stype)
+        with pytest.raises(agrav.AgravError):
+            g.set_softenings([0.002])  # type 1 would have no softening
+        g.compute_grav_accelerations(first_step=True)  # the two-entry table still stands
+        a2 = g.get_accelerations()["acc"]
+        g.set_particles(pos, mass, softtype=np.zeros(n, np.uint8))
+        g.set_softenings([0.002])
+        g.compute_grav_accelerations(first_step=True)
+        a1 = g.get_accelerations()["acc"]
+    assert np.all(np.isfinite(a2)) and np.all(np.isfinite(a1))
+
+
 def test_periodic_positions_wrapped_once(agrav, oracle):
     g0 = np.random.Generator(np.random.Philox(43))
     n = 5000
