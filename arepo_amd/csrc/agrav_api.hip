@@ -483,6 +483,7 @@ void agrav_destroy(agrav_ctx *c)
   c->d_nmeta.release();
   c->d_nrecA.release();
   c->d_nrecB.release();
+  c->d_npos.release();
   c->d_mom_tiles.release();
   c->d_mom_tab.release();
   c->d_mom_tbeg.release();

@@ -739,14 +739,15 @@ __global__ void k_pnext(int n, const unsigned *__restrict__ lvlmask, const int *
 // ---------------------------------------------------------------------------
 
 // CSR child lists, node children first (the group walk pushes them) and then
-// particle children (the group walk streams them straight into its tile)
+// particle children (the group walk streams them straight into its tile), and the
+// position of every node in its parent's list (npos; -1 for a cube without parent)
 // Also the walk's per-node meta record and a copy of the FP32 records of the
 // particle children next to their list entries, so that the group walk reads a
 // node and its particle children in one memory round trip each.
 __global__ void k_fill_children(int m, int n, const int *__restrict__ nextnode, const int *__restrict__ sibling,
                                 const int *__restrict__ pnext, const int *__restrict__ cbeg, const int *__restrict__ ccnt,
                                 const int2 *__restrict__ nrange, const uint4 *__restrict__ prec, int *child, int *cnn,
-                                int4 *meta, uint4 *cprec)
+                                int4 *meta, uint4 *cprec, int *npos)
 {
   int j = blockIdx.x * B + threadIdx.x;
   if(j >= m)
@@ -756,7 +757,10 @@ __global__ void k_fill_children(int m, int n, const int *__restrict__ nextnode, 
   for(int c = nextnode[j]; c != end && c >= 0 && steps < ccnt[j]; steps++)
     {
       if(c >= n && o < stop)
-        child[o++] = c;
+        {
+          npos[c - n] = o;
+          child[o++]  = c;
+        }
       c = (c < n) ? pnext[c] : sibling[c - n];
     }
   const int nn = o - cbeg[j];
@@ -775,33 +779,14 @@ __global__ void k_fill_children(int m, int n, const int *__restrict__ nextnode, 
   meta[j]       = make_int4(cbeg[j] + nn, o - cbeg[j] - nn, rg.x, min(nn, 15) | (min(rg.y, 255) << 4));
 }
 
-// The walk records of every node that is some node's child, copied to its position in
-// the parent's child list: the FP32 group walk pushes child-list positions (consecutive
-// for the children of one node, so no index is read to push them) and reads a popped
-// entry's records there.
-__global__ void k_node_by_position(int m, int n, const int *__restrict__ cbeg, const int *__restrict__ cnn,
-                                   const int *__restrict__ child, const int4 *__restrict__ meta, const uint4 *__restrict__ recA,
-                                   const uint4 *__restrict__ recB, int4 *nmeta, uint4 *nrecA, uint4 *nrecB)
-{
-  int j = blockIdx.x * B + threadIdx.x;
-  if(j >= m)
-    return;
-  for(int o = cbeg[j], e = cbeg[j] + cnn[j]; o < e; o++)
-    {
-      const int k = child[o] - n;
-      nmeta[o]    = meta[k];
-      nrecA[o]    = recA[k];
-      nrecB[o]    = recB[k];
-    }
-}
-
 // children of each node, by walking its threading from nextnode to sibling
 __global__ void k_count_children(int m, int n, const int *__restrict__ nextnode, const int *__restrict__ sibling,
-                                 const int *__restrict__ pnext, int *ccnt, int limit, int *err)
+                                 const int *__restrict__ pnext, int *ccnt, int *npos, int limit, int *err)
 {
   int j = blockIdx.x * B + threadIdx.x;
   if(j >= m)
     return;
+  npos[j] = -1;  // set for every node with a parent by k_fill_children
   int k = 0;
   for(int c = nextnode[j], end = sibling[j]; c != end; k++)
     {
@@ -1120,7 +1105,9 @@ __global__ __launch_bounds__(B) void k_moments_prefix(int m, const int2 *__restr
                                                       const double *__restrict__ cx, const double *__restrict__ cy,
                                                       const double *__restrict__ cz, const double *__restrict__ len, double *sx,
                                                       double *sy, double *sz, double *nm, uint8_t *maxsoft, uint4 *recA,
-                                                      uint4 *recB, const Domain *__restrict__ doms, const int *__restrict__ pset)
+                                                      uint4 *recB, const Domain *__restrict__ doms, const int *__restrict__ pset,
+                                                      const int *__restrict__ npos, const int4 *__restrict__ meta, int4 *nmeta,
+                                                      uint4 *nrecA, uint4 *nrecB)
 {
   const int j = blockIdx.x * B + threadIdx.x;
   if(j >= m)
@@ -1190,8 +1177,19 @@ __global__ __launch_bounds__(B) void k_moments_prefix(int m, const int2 *__restr
   sz[j]      = s2;
   nm[j]      = mass;
   maxsoft[j] = (uint8_t)ms;
-  recA[j]    = make_uint4(to_fix(s0, dom, 0), to_fix(s1, dom, 1), to_fix(s2, dom, 2), __float_as_uint((float)mass));
-  recB[j]    = make_uint4(to_fix(cx[j], dom, 0), to_fix(cy[j], dom, 1), to_fix(cz[j], dom, 2), __float_as_uint((float)len[j]));
+  const uint4 ra = make_uint4(to_fix(s0, dom, 0), to_fix(s1, dom, 1), to_fix(s2, dom, 2), __float_as_uint((float)mass));
+  const uint4 rb = make_uint4(to_fix(cx[j], dom, 0), to_fix(cy[j], dom, 1), to_fix(cz[j], dom, 2), __float_as_uint((float)len[j]));
+  recA[j]        = ra;
+  recB[j]        = rb;
+  // the same records at the node's position in its parent's child list (the FP32 group
+  // walk pushes child-list positions and reads a popped entry's records there)
+  const int p = npos[j];
+  if(p >= 0)
+    {
+      nmeta[p] = meta[j];
+      nrecA[p] = ra;
+      nrecB[p] = rb;
+    }
 }
 
 // batch: the entry where each set's walk starts (its top cube, or its particle
@@ -1467,14 +1465,18 @@ agrav_status build_tree(agrav_ctx *c)
   if(scan_temp_bytes(M) + 4096 > c->d_temp.cap && (st = c->d_temp.ensure(scan_temp_bytes(M) + 4096)) != AGRAV_OK)
     return st;
   int *d_berr = c->d_counters.p + 50;  // cleared by the upload at the start
-  k_count_children<<<nblk(M), B, 0, s>>>(M, N, c->d_nextnode.p, c->d_sibling.p, c->d_pnext.p, c->d_ccnt.p, N + M, d_berr);
+  if((st = c->d_npos.ensure(M)))
+    return st;
+  k_count_children<<<nblk(M), B, 0, s>>>(M, N, c->d_nextnode.p, c->d_sibling.p, c->d_pnext.p, c->d_ccnt.p, c->d_npos.p, N + M,
+                                         d_berr);
   AG_HIP(hipGetLastError());
   if((st = exclusive_scan_i32(c->d_ccnt.p, c->d_cbeg.p, M, nullptr, c->d_temp.p, d_perr, s)) != AGRAV_OK)
     return st;
   if((st = c->d_meta.ensure(M)) || (st = c->d_cprec.ensure((size_t)M + n)))
     return st;
   k_fill_children<<<nblk(M), B, 0, s>>>(M, N, c->d_nextnode.p, c->d_sibling.p, c->d_pnext.p, c->d_cbeg.p, c->d_ccnt.p,
-                                         c->d_nrange.p, c->d_prec.p, c->d_child.p, c->d_cnn.p, c->d_meta.p, c->d_cprec.p);
+                                         c->d_nrange.p, c->d_prec.p, c->d_child.p, c->d_cnn.p, c->d_meta.p, c->d_cprec.p,
+                                         c->d_npos.p);
   int mtot = 0;
   for(int L = 0; L <= AG_MAX_LEVELS; L++)
     mtot += c->lvlcnt_h[L];
@@ -1537,15 +1539,13 @@ agrav_status build_tree(agrav_ctx *c)
             return st;
         }
     }
-  k_moments_prefix<<<nblk(M), B, 0, s>>>(M, c->d_nrange.p, c->d_mom_hi.p, c->d_mom_lo.p, stp, tcnt, N + 1, c->soft.nsoft, c->d_soft.p,
-                                         c->d_cx.p, c->d_cy.p, c->d_cz.p, c->d_len.p, c->d_sx.p, c->d_sy.p, c->d_sz.p, c->d_nm.p,
-                                         c->d_maxsoft.p, c->d_recA.p, c->d_recB.p, geo.dom, geo.pset);
-  AG_HIP(hipGetLastError());
   if((st = c->d_nmeta.ensure((size_t)M + n)) || (st = c->d_nrecA.ensure((size_t)M + n)) ||
      (st = c->d_nrecB.ensure((size_t)M + n)))
     return st;
-  k_node_by_position<<<nblk(M), B, 0, s>>>(M, N, c->d_cbeg.p, c->d_cnn.p, c->d_child.p, c->d_meta.p, c->d_recA.p, c->d_recB.p,
-                                            c->d_nmeta.p, c->d_nrecA.p, c->d_nrecB.p);
+  k_moments_prefix<<<nblk(M), B, 0, s>>>(M, c->d_nrange.p, c->d_mom_hi.p, c->d_mom_lo.p, stp, tcnt, N + 1, c->soft.nsoft, c->d_soft.p,
+                                         c->d_cx.p, c->d_cy.p, c->d_cz.p, c->d_len.p, c->d_sx.p, c->d_sy.p, c->d_sz.p, c->d_nm.p,
+                                         c->d_maxsoft.p, c->d_recA.p, c->d_recB.p, geo.dom, geo.pset, c->d_npos.p, c->d_meta.p,
+                                         c->d_nmeta.p, c->d_nrecA.p, c->d_nrecB.p);
   AG_HIP(hipGetLastError());
 
   // end of the build, recorded before the read-back so that timing it needs no second
