@@ -315,6 +315,14 @@ static constexpr int kStackCap = 1024;
 #define AG_SPLIT_MIN 64
 #endif
 static constexpr int kListCap  = 128;
+// FP32 TreePM walks of one softening type split the tile: a main list and, after it, a
+// list of the entries within the softening length of the group's box (only its loop has
+// the spline branch).  One staging round adds at most 64 entries to either.
+#ifndef AG_FAST_MAIN
+#define AG_FAST_MAIN 64
+#endif
+static constexpr int kSoftCap  = 64;
+static constexpr int kListAlloc = AG_FAST_MAIN + kSoftCap > kListCap ? AG_FAST_MAIN + kSoftCap : kListCap;
 
 // Inclusive prefix sum over the 64 lanes of a wavefront (all lanes active) with DPP
 // moves: shifts by 1, 2, 4, 8 within each row of 16 lanes, then the last lane of row
@@ -715,7 +723,7 @@ template <typename T, bool MSOFT> struct WalkLds
   alignas(16) float tab[512];  // short-range tables: FP64 walks [force 128 | potential 128], FP32 walks see tile_compute_f32
   int stack[kStackCap];
   int map[64];  // list and index offset of each slot of a particle-append chunk
-  Vec4<T> list[kListCap];
+  Vec4<T> list[kListAlloc];
   T h[MSOFT ? kListCap : 1];
 };
 
@@ -869,21 +877,67 @@ __device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, co
   T ax = 0, ay = 0, az = 0, pp = 0, cax = 0, cay = 0, caz = 0, cpp = 0;
   int nin        = 0;
   unsigned long long n_popped = 0, n_tiled = 0;  // diagnostics: stack entries evaluated, tile entries computed
-  bool tile_soft       = false;
-  auto flush_tile      = [&]() {
+  // the main list and (FP32 TreePM of one softening type) the list of entries within
+  // the softening length of the group's box, whose loop alone has the spline branch
+  constexpr int kMainCap = kFast ? AG_FAST_MAIN : kListCap;
+  Vec4<T> *s_soft        = s_list + kMainCap;
+  int ns                 = 0;
+  auto fast_tile         = [&](const Vec4<T> *lst, int cnt, bool soft) {
     if constexpr(kFast)
       {
         if(wc.eval_pot)
-          tile_compute_f32_pm<true>(s_list, nl, tx * cs, ty * cs, tz * cs, hi * cs, tile_soft, false, s_tab, ax, ay, az, pp, cax,
-                                    cay, caz, cpp, nin);
+          tile_compute_f32_pm<true>(lst, cnt, tx * cs, ty * cs, tz * cs, hi * cs, soft, false, s_tab, ax, ay, az, pp, cax, cay, caz,
+                                    cpp, nin);
         else
-          tile_compute_f32_pm<false>(s_list, nl, tx * cs, ty * cs, tz * cs, hi * cs, tile_soft, compact, s_tab, ax, ay, az, pp,
-                                     cax, cay, caz, cpp, nin);
+          tile_compute_f32_pm<false>(lst, cnt, tx * cs, ty * cs, tz * cs, hi * cs, soft, compact, s_tab, ax, ay, az, pp, cax, cay,
+                                     caz, cpp, nin);
       }
+  };
+  auto flush_tile = [&]() {
+    if constexpr(kFast)
+      fast_tile(s_list, nl, false);
     else
       tile_compute<T, PM, EWALD, MSOFT>(s_list, s_h, nl, tx, ty, tz, hi, s_tab, wc, asmthfac, asmthinv2, ewald_f, ewald_d, ax, ay,
                                         az, pp, cax, cay, caz, cpp, nin);
-    tile_soft = false;
+  };
+  // stage one entry per lane (acc: the lane has one; nr: within the softening length of
+  // the group's box), flushing a list first when the entries would not fit
+  auto stage = [&](bool acc, bool nr, const Vec4<T> &it, T hv) {
+    const bool to_soft = kFast && nr;
+    const uint64_t bm  = __ballot(acc && !to_soft);
+    const int cnt      = __popcll(bm);
+    if(nl + cnt > kMainCap)
+      {
+        __builtin_amdgcn_wave_barrier();
+        n_tiled += nl;
+        flush_tile();
+        nl = 0;
+        __builtin_amdgcn_wave_barrier();
+      }
+    if constexpr(kFast)
+      {
+        const uint64_t bs = __ballot(acc && to_soft);
+        const int sc      = __popcll(bs);
+        if(ns + sc > kSoftCap)
+          {
+            __builtin_amdgcn_wave_barrier();
+            n_tiled += ns;
+            fast_tile(s_soft, ns, true);
+            ns = 0;
+            __builtin_amdgcn_wave_barrier();
+          }
+        if(acc && to_soft)
+          s_soft[ns + __popcll(bs & lt)] = it;
+        ns += sc;
+      }
+    if(acc && !to_soft)
+      {
+        const int slot = nl + __popcll(bm & lt);
+        s_list[slot]   = it;
+        if(MSOFT)
+          s_h[slot] = hv;
+      }
+    nl += cnt;
   };
 
   bool overflow  = false;
@@ -1097,27 +1151,8 @@ __device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, co
             }
 
           // stage accepted entries in the LDS tile
-          const bool accp   = (act == 1 || act == 3) && item.w != (T)0 && keep_ia;
-          const uint64_t bm = __ballot(accp);
-          const int cnt     = __popcll(bm);
-          const bool soft_new = kFast && __ballot(accp && near) != 0;
-          if(nl + cnt > kListCap)
-            {
-              __builtin_amdgcn_wave_barrier();
-              n_tiled += nl;
-              flush_tile();
-              nl = 0;
-              __builtin_amdgcn_wave_barrier();
-            }
-          tile_soft = tile_soft || soft_new;
-          if(accp)
-            {
-              int slot     = nl + __popcll(bm & lt);
-              s_list[slot] = item;
-              if(MSOFT)
-                s_h[slot] = hj;
-            }
-          nl += cnt;
+          const bool accp = (act == 1 || act == 3) && item.w != (T)0 && keep_ia;
+          stage(accp, near, item, hj);
 
           // particles of small opened nodes (contiguous ranges) and the particle children of
           // opened nodes go straight into the tile: the ranges of all lanes are laid end to
@@ -1190,26 +1225,7 @@ __device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, co
                         pnear       = dd2 < hs2;
                       }
                   }
-                const uint64_t rb = __ballot(keep);
-                const int rcnt    = __popcll(rb);
-                const bool psoft  = kFast && __ballot(keep && pnear) != 0;
-                if(nl + rcnt > kListCap)
-                  {
-                    __builtin_amdgcn_wave_barrier();
-                    n_tiled += nl;
-                    flush_tile();
-                    nl = 0;
-                    __builtin_amdgcn_wave_barrier();
-                  }
-                tile_soft = tile_soft || psoft;
-                if(keep)
-                  {
-                    const int slot = nl + __popcll(rb & lt);
-                    s_list[slot]   = Vec4<T>{x * cs, y * cs, z * cs, m};
-                    if(MSOFT)
-                      s_h[slot] = hp;
-                  }
-                nl += rcnt;
+                stage(keep, pnear, Vec4<T>{x * cs, y * cs, z * cs, m}, hp);
               }
           }
           nchs[h] = (act == 2) ? (mt.w & 15) : 0;  // node children only (at most 8)
@@ -1261,6 +1277,11 @@ __device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, co
   __builtin_amdgcn_wave_barrier();
   n_tiled += nl;
   flush_tile();
+  if constexpr(kFast)
+    {
+      n_tiled += ns;
+      fast_tile(s_soft, ns, true);
+    }
 
   if constexpr(kFast)
     {
