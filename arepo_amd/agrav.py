@@ -104,7 +104,7 @@ EXPORTED = [
     "agrav_get_order", "agrav_get_tables", "agrav_hilbert_key21",
     "agrav_batch_set_particles", "agrav_batch_autotune", "agrav_batch_get_tuning", "agrav_batch_set_tuning",
     "agrav_subfind_unbind", "agrav_gravity_timesteps", "agrav_server_create", "agrav_server_submit", "agrav_server_wait", "agrav_server_poll", "agrav_server_stats",
-    "agrav_server_destroy", "agrav_solve_footprint",
+    "agrav_server_destroy", "agrav_solve_footprint", "agrav_host_alloc", "agrav_host_free",
 ]
 
 
@@ -218,6 +218,8 @@ def _declare(L):
     L.agrav_server_poll.argtypes = [vp, C.c_uint64, C.POINTER(C.c_int)]
     L.agrav_server_stats.argtypes = [vp, C.POINTER(ServerStats)]
     L.agrav_server_destroy.argtypes = [vp]
+    L.agrav_host_alloc.argtypes = [C.c_size_t, C.POINTER(vp)]
+    L.agrav_host_free.argtypes = [vp]
     L.agrav_hilbert_key21.restype = C.c_uint64
     L.agrav_hilbert_key21.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
     for name in EXPORTED:
@@ -259,6 +261,35 @@ def make_params(periodic=True, box_size=1.0, pmgrid=0, asmth=1.25, rcut=4.5, G=1
 
 def _ptr(a):
     return None if a is None else a.ctypes.data
+
+
+class _PinnedBlock:
+    """Page-locked host memory from agrav_host_alloc, freed when the last array viewing it goes."""
+
+    def __init__(self, nbytes):
+        self.L = lib()
+        p = C.c_void_p()
+        rc = self.L.agrav_host_alloc(max(int(nbytes), 1), C.byref(p))
+        if rc != 0:
+            raise AgravError(rc, (self.L.agrav_last_error(None) or b"").decode())
+        self.ptr = p.value
+        self.nbytes = max(int(nbytes), 1)
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            self.L.agrav_host_free(self.ptr)
+            self.ptr = None
+
+
+def host_empty(shape, dtype=np.float64):
+    """An uninitialised numpy array in page-locked host memory (agrav_host_alloc): copies between it
+    and the device run at the host link's DMA rate."""
+    dt = np.dtype(dtype)
+    count = int(np.prod(shape)) if np.ndim(shape) else int(shape)
+    blk = _PinnedBlock(count * dt.itemsize)
+    buf = (C.c_char * blk.nbytes).from_address(blk.ptr)
+    buf._agrav_block = blk  # the ctypes buffer keeps the block alive, the array keeps the buffer
+    return np.frombuffer(buf, dtype=dt, count=count).reshape(shape)
 
 
 class Gravity:

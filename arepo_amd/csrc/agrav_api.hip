@@ -201,6 +201,47 @@ __global__ void k_check_stype(const uint8_t *__restrict__ st, int64_t n, int nso
     atomicOr(bad, 1);
 }
 
+// Total mass in an order fixed by n alone: blocks of kMassChunk particles, each summed
+// by 256 threads over a fixed stride and then pairwise in LDS; the block sums likewise
+// by one block.  Host and device inputs give the same bits.
+static constexpr int kMassChunk = 4096;
+__global__ __launch_bounds__(B) void k_mass_partial(const double *__restrict__ m, int64_t n, double *part)
+{
+  __shared__ double sh[B];
+  const int64_t b0 = (int64_t)blockIdx.x * kMassChunk;
+  double v         = 0;
+  for(int k = threadIdx.x; k < kMassChunk; k += B)
+    if(b0 + k < n)
+      v += m[b0 + k];
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for(int w = B / 2; w > 0; w >>= 1)
+    {
+      if((int)threadIdx.x < w)
+        sh[threadIdx.x] += sh[threadIdx.x + w];
+      __syncthreads();
+    }
+  if(threadIdx.x == 0)
+    part[blockIdx.x] = sh[0];
+}
+__global__ __launch_bounds__(B) void k_mass_final(const double *__restrict__ part, int64_t nb, double *out)
+{
+  __shared__ double sh[B];
+  double v = 0;
+  for(int64_t k = threadIdx.x; k < nb; k += B)
+    v += part[k];
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for(int w = B / 2; w > 0; w >>= 1)
+    {
+      if((int)threadIdx.x < w)
+        sh[threadIdx.x] += sh[threadIdx.x + w];
+      __syncthreads();
+    }
+  if(threadIdx.x == 0)
+    out[0] = sh[0];
+}
+
 static void soft_from(agrav_ctx *c, int nsoft, const double *eps)
 {
   // set_softenings, grav_softening.c:90-95
@@ -484,6 +525,7 @@ void agrav_destroy(agrav_ctx *c)
   c->d_nrecA.release();
   c->d_nrecB.release();
   c->d_npos.release();
+  c->d_mass_part.release();
   c->d_mom_tiles.release();
   c->d_mom_tab.release();
   c->d_mom_tbeg.release();
@@ -616,20 +658,17 @@ static agrav_status set_particles_common(agrav_ctx *c, int64_t n, const double *
             return fail(c, AGRAV_ERR_BAD_PARAM, "softening type out of range");
         }
     }
-  // total mass (host sum for host input, device copy-back otherwise), fixed order
+  // total mass on the device (the PM mesh scale and the potential constant use it)
   double mt = 0;
-  if(kind == hipMemcpyHostToDevice)
+  if(n > 0)
     {
-      for(int64_t i = 0; i < n; i++)
-        mt += mass[i];
-    }
-  else if(n > 0)
-    {
-      std::vector<double> h(n);
-      AG_HIP(hipMemcpyAsync(h.data(), mass, n * sizeof(double), hipMemcpyDeviceToHost, s));
-      AG_HIP(hipStreamSynchronize(s));
-      for(int64_t i = 0; i < n; i++)
-        mt += h[i];
+      const int64_t nb = (n + kMassChunk - 1) / kMassChunk;
+      if((st = c->d_mass_part.ensure(nb + 1)))
+        return st;
+      k_mass_partial<<<(unsigned)nb, B, 0, s>>>(c->d_mass_in.p, n, c->d_mass_part.p);
+      k_mass_final<<<1, B, 0, s>>>(c->d_mass_part.p, nb, c->d_mass_part.p + nb);
+      AG_HIP(hipGetLastError());
+      AG_HIP(hipMemcpyAsync(&mt, c->d_mass_part.p + nb, sizeof(double), hipMemcpyDeviceToHost, s));
     }
   AG_HIP(hipMemsetAsync(c->d_oldacc.p, 0, nn * sizeof(float), s));
   AG_HIP(hipStreamSynchronize(s));
@@ -694,6 +733,29 @@ agrav_status agrav_highres_region(agrav_ctx *c, int reset, double *region)
       region[13] = c->hr.asmth;
       region[14] = c->hr.rcut;
     }
+  return AGRAV_OK;
+}
+
+agrav_status agrav_host_alloc(size_t bytes, void **ptr)
+{
+  if(!ptr)
+    return AGRAV_ERR_BAD_PARAM;
+  *ptr = nullptr;
+  if(bytes == 0)
+    return AGRAV_OK;
+  if(hipHostMalloc(ptr, bytes, hipHostMallocDefault) != hipSuccess)
+    {
+      *ptr = nullptr;
+      set_error_msg("cannot allocate page-locked host memory");
+      return AGRAV_ERR_OOM;
+    }
+  return AGRAV_OK;
+}
+
+agrav_status agrav_host_free(void *ptr)
+{
+  if(ptr && hipHostFree(ptr) != hipSuccess)
+    return AGRAV_ERR_BAD_PARAM;
   return AGRAV_OK;
 }
 

@@ -211,6 +211,7 @@ struct agrav_ctx
   agrav::DBuf<uint4> d_cprec;
   agrav::DBuf<int4> d_nmeta;           // node walk records at their child-list positions
   agrav::DBuf<uint4> d_nrecA, d_nrecB;
+  agrav::DBuf<double> d_mass_part;      // block sums of the total mass
   agrav::DBuf<int> d_npos;               // position of each node in its parent's child list
   // moments from prefix sums: per-tile sums, prefix high (FP64) and low (FP32)
   // parts of (m, m x, m y, m z), per-type particle counts (several softening types)

@@ -75,15 +75,25 @@ def run_path(A, pos, mass, eps, precision, walk, steps, warmup, old_acc=None, po
     return dict(total_s=total, step_ms=step_ms, timings=tim), res
 
 
-def host_io_run(A, pos, mass, eps, old_acc, steps):
+def host_io_run(A, pos, mass, eps, old_acc, steps, pinned=False):
     """The drop-in as AREPO's glue drives it every step: positions, masses and softening types from host
     arrays (agrav_set_particles), OldAcc from the host (agrav_set_old_acc), the step, and the
-    accelerations and OldAcc back to host arrays (agrav_get_accelerations); pageable host memory, as
-    the glue's myrealloc buffers.  Times each part per step."""
+    accelerations and OldAcc back to host arrays (agrav_get_accelerations).  Host memory: pageable, as
+    the glue's myrealloc buffers, or (pinned) staging arrays from agrav_host_alloc that the glue fills
+    instead.  Times each part per step."""
     g = A.Gravity(make_params(A, A.FP32, A.WALK_GROUP, eps))
-    st = np.zeros(len(pos), np.uint8)
-    oa = old_acc.copy()
-    out = {"acc": np.empty((len(pos), 3)), "old_acc": np.empty(len(pos), np.float32)}
+    n = len(pos)
+    if pinned:
+        hp, hm, st = A.host_empty((n, 3)), A.host_empty(n), A.host_empty(n, np.uint8)
+        hp[:], hm[:], st[:] = pos, mass, 0
+        pos, mass = hp, hm
+        oa = A.host_empty(n, np.float32)
+        oa[:] = old_acc
+        out = {"acc": A.host_empty((n, 3)), "old_acc": A.host_empty(n, np.float32)}
+    else:
+        st = np.zeros(n, np.uint8)
+        oa = old_acc.copy()
+        out = {"acc": np.empty((n, 3)), "old_acc": np.empty(n, np.float32)}
     t = {"h2d": [], "device": [], "d2h": [], "total": []}
     for k in range(steps + 1):
         t0 = time.perf_counter()
@@ -103,7 +113,7 @@ def host_io_run(A, pos, mass, eps, old_acc, steps):
     return {"steps": steps, "ms_per_step": med["total"], "particles_per_s": len(pos) / (med["total"] * 1e-3),
             "h2d_ms": med["h2d"], "device_ms": med["device"], "d2h_ms": med["d2h"],
             "bytes_in": int(len(pos) * (24 + 8 + 1 + 4)), "bytes_out": int(len(pos) * (24 + 4)),
-            "host_memory": "pageable numpy arrays"}
+            "host_memory": "page-locked (agrav_host_alloc)" if pinned else "pageable numpy arrays"}
 
 
 def substep_run(A, pos, mass, eps, old_acc, fractions, steps):
@@ -622,6 +632,8 @@ def main():
 
     log("drop-in with host arrays (set_particles + set_old_acc + step + get_accelerations)")
     out["fp32_group_host_io"] = host_io_run(A, pos, mass, eps, res32["old_acc"], max(3, min(args.steps, 10)))
+    out["fp32_group_host_io_pinned"] = host_io_run(A, pos, mass, eps, res32["old_acc"], max(3, min(args.steps, 10)),
+                                                   pinned=True)
     log("host io: %s" % json.dumps(out["fp32_group_host_io"]))
 
     cpu = None

@@ -17,6 +17,7 @@
 #ifndef AGRAV_H
 #define AGRAV_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -32,7 +33,7 @@ typedef enum
   AGRAV_ERR_CAPACITY  = -2, /* a capacity limit was exceeded (forcetree.c:930-933) */
   AGRAV_ERR_OUT_OF_BOX = -3, /* particle outside the periodic box (forcetree.c:432-438) */
   AGRAV_ERR_HIP       = -4, /* HIP runtime error */
-  AGRAV_ERR_OOM       = -5, /* device allocation failed */
+  AGRAV_ERR_OOM       = -5, /* device (or page-locked host) allocation failed */
   AGRAV_ERR_STATE     = -6, /* call order violated (e.g. walk before build) */
   AGRAV_ERR_FFT       = -7, /* rocFFT error */
   AGRAV_ERR_NO_DEVICE = -8  /* no usable GPU */
@@ -131,6 +132,13 @@ agrav_status agrav_set_softenings(agrav_ctx *ctx, int n_soft_types, const double
  * (forcetree.c:394-417).  Host pointers; data are copied to HBM.
  * softtype may be NULL (all type 0). */
 agrav_status agrav_set_particles(agrav_ctx *ctx, int64_t n, const double *pos, const double *mass, const uint8_t *softtype);
+/* Page-locked host memory for the arrays a caller stages particle data and results in
+ * (no counterpart in the reference, whose particle data never leave the host): copies
+ * from and to it run at the host link's DMA rate instead of through the runtime's
+ * pageable staging.  Any host pointer argument of this interface may point into it.
+ * agrav_host_free takes only pointers from agrav_host_alloc (NULL is ignored). */
+agrav_status agrav_host_alloc(size_t bytes, void **ptr);
+agrav_status agrav_host_free(void *ptr);
 /* Same with device pointers (already resident in HBM); copied device-to-device on the ctx stream. */
 agrav_status agrav_set_particles_device(agrav_ctx *ctx, int64_t n, const double *d_pos, const double *d_mass,
                                         const uint8_t *d_softtype);

@@ -120,3 +120,12 @@ def test_hilbert_key_fits_63_bits(agrav):
     m = (1 << 21) - 1
     for c in [(0, 0, 0), (m, m, m), (m, 0, 0), (0, m, 0), (0, 0, m)]:
         assert agrav.hilbert_key21(*c) < (1 << 63)
+
+
+def test_host_alloc_edge_cases_need_no_device(agrav):
+    import ctypes as C
+    L = agrav.lib()
+    p = C.c_void_p(1)
+    assert L.agrav_host_alloc(0, C.byref(p)) == 0 and p.value is None  # nothing to allocate
+    assert L.agrav_host_alloc(16, None) == -1                           # no place for the pointer
+    assert L.agrav_host_free(None) == 0

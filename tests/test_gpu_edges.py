@@ -335,3 +335,26 @@ def test_rejected_set_particles_state(agrav):
     finally:
         dev_buf.free()
     assert np.array_equal(again["acc"], first["acc"])
+
+
+def test_page_locked_host_arrays(agrav):
+    """Inputs staged in and results copied to agrav_host_alloc memory give the bits of pageable arrays."""
+    g0 = np.random.Generator(np.random.Philox(59))
+    n = 20000
+    pos, mass = g0.random((n, 3)), g0.random(n) / n
+    kw = dict(periodic=True, box_size=1.0, pmgrid=32, soft=(0.002,), precision=agrav.FP32, walk=agrav.WALK_GROUP)
+    res = []
+    for pinned in (False, True):
+        with agrav.Gravity(agrav.make_params(**kw)) as g:
+            if pinned:
+                hp, hm = agrav.host_empty((n, 3)), agrav.host_empty(n)
+                hp[:], hm[:] = pos, mass
+                out = {"acc": agrav.host_empty((n, 3)), "pot": agrav.host_empty(n)}
+                g.set_particles(hp, hm)
+            else:
+                out = None
+                g.set_particles(pos, mass)
+            g.compute_grav_accelerations(first_step=True)
+            r = g.get_accelerations(fields=("acc", "pot"), out=out)
+            res.append({k: np.array(r[k]) for k in ("acc", "pot")})
+    assert np.array_equal(res[0]["acc"], res[1]["acc"]) and np.array_equal(res[0]["pot"], res[1]["pot"])
