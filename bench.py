@@ -644,7 +644,8 @@ def main():
         log("CPU oracle baseline: %d threads, walk sample %d" % (threads, len(cpu_sel)))
         cpu = cpu_baseline(pos, mass, eps, res32["old_acc"], cpu_sel, threads)
         out["cpu_baseline"] = {"value": cpu["value"], "unit": "particles/s", "cores": threads, "kind": "port",
-                               "sample": "full tree build + 128^3 PM on the host, relative walk of %d random "
+                               "sample": "full tree build + 128^3 PM on the host (the oracle's own complex radix-2 "
+                                         "FFT, no FFTW: slower than the reference's PM leg), relative walk of %d random "
                                          "targets scaled to N (build %.2f s, pm %.2f s, walk sample %.2f s)"
                                          % (cpu["sample"], cpu["t_build"], cpu["t_pm"], cpu["t_walk_sample"])}
         out["speedup_vs_cpu"] = value / cpu["value"]
