@@ -20,6 +20,14 @@
   walk and FP64 walks against the FP64 direct sum on a seeded 16,384-target
   subset, next to the oracle walk on the same targets with the same OldAcc.
 
+* 4096 independent isolated groups, 100,311,619 particles (32 ... 199,900 per
+  group, NFW- and Hernquist-like; the batched unbinding workload), one batched
+  FP32 solve of potentials and accelerations with Barnes-Hut theta 0.7 after
+  the autotuner: on 512 targets in each of 32 groups, the potential error
+  against the FP64 direct sum over the group is no worse than that of the
+  oracle's restatement of the reference's unbinding walk
+  (subfind_loctree.c:564-730) on the same targets; no group falls back.
+
 Each test prints its error table (run pytest with -s to see it).
 """
 import numpy as np
@@ -167,3 +175,43 @@ def test_halo_disk_precision_study(agrav, oracle, halo_disk, halo_disk_reference
         assert row[k][1] <= np.percentile(e_or, 99) * 1.05, (k, row[k])
     # error follows the tolerance: well below 2 x ErrTolForceAcc at the median
     assert row["fp32_group"][0] < 2 * tol
+
+
+def test_batched_unbinding_4096_groups_full_size(agrav, oracle):
+    from arepo_amd import ics
+    off, rel, mass, eps, cen = ics.unbinding_batch(4096, seed=4)
+    n = int(off[-1])
+    print("batch of %d particles generated" % n, flush=True)
+    p = agrav.make_params(periodic=False, box_size=0.0, pmgrid=0, soft=(0.01,), precision=agrav.FP32,
+                          walk=agrav.WALK_GROUP, err_tol_theta=0.7, opening=0, leaf_bucket=4, eval_potential=True)
+    groups = np.arange(0, 4096, 128)
+    g0 = np.random.Generator(np.random.Philox(61))
+    sel = np.concatenate([off[s] + np.sort(g0.choice(off[s + 1] - off[s], min(512, off[s + 1] - off[s]), replace=False))
+                          for s in groups]).astype(np.int32)
+    with agrav.Gravity(p) as g:
+        g.batch_set_particles(off, rel, mass, eps, centres=cen)
+        g.batch_autotune()
+        g.compute_grav_accelerations(first_step=True)
+        out = g.get_accelerations(fields=("acc", "pot"))
+        tim = g.timings()
+        ft = g.gravity_forcetest(sel)  # FP64 direct sum over each target's own group
+    assert tim["walk_retries"] == 0
+    assert np.all(np.isfinite(out["acc"])) and np.all(np.isfinite(out["pot"]))
+    ep_gpu = np.abs(out["pot"][sel] - ft["pot"]) / np.abs(ft["pot"])
+    ea_gpu = rel_err(out["acc"][sel], ft["acc"])
+    ep_ref = np.empty(len(sel))
+    k = 0
+    for s in groups:
+        sl = slice(off[s], off[s + 1])
+        pot_o, _ = oracle.group_potential_bh(rel[sl].astype(np.float64) + cen[s], mass[sl], 2.8 * eps[s], 0.7)
+        m = min(512, off[s + 1] - off[s])
+        idx = sel[k:k + m] - off[s]
+        ep_ref[k:k + m] = np.abs(pot_o[idx] - ft["pot"][k:k + m]) / np.abs(ft["pot"][k:k + m])
+        k += m
+    print("potential error median %.3g p99 %.3g (reference walk %.3g / %.3g); acceleration median %.3g p99 %.3g"
+          % (np.median(ep_gpu), np.percentile(ep_gpu, 99), np.median(ep_ref), np.percentile(ep_ref, 99),
+             np.median(ea_gpu), np.percentile(ea_gpu, 99)))
+    assert np.median(ep_ref) < 2e-3  # the reference walk and the direct sum agree on the convention
+    assert np.median(ep_gpu) <= 1.05 * np.median(ep_ref) + 1e-6
+    assert np.percentile(ep_gpu, 99) <= 1.05 * np.percentile(ep_ref, 99) + 1e-6
+    assert np.median(ea_gpu) < 5e-3 and np.percentile(ea_gpu, 99) < 5e-2
