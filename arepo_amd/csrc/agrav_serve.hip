@@ -69,14 +69,24 @@ struct agrav_server
 namespace
 {
 
-// What a slot is charged: its resident buffers, or what its running job will
-// grow them to (buffers only grow while a context lives).
+// Bytes slot s holds at most once job j has run on it.  Buffers only grow while a
+// context lives: a fresh context ends within j's footprint; a context grown by
+// earlier jobs keeps what it holds, and the buffers sized by other quantities than the
+// particle count (groups, chunks, tie runs) may still grow past it, by far less than
+// 16 B per particle of j.
+size_t job_bound(const agrav_server *S, int s, const Job *j)
+{
+  const size_t res = S->resident[s];
+  size_t b         = res > j->footprint ? res : j->footprint;
+  if(res > S->base)
+    b += (size_t)j->req.n * 16 + ((size_t)1 << 20);
+  return b;
+}
+
+// What a slot is charged: its resident buffers, or the bound of its running job.
 size_t charge(const agrav_server *S, int s)
 {
-  size_t r = S->resident[s];
-  if(S->busy[s] && S->busy[s]->footprint > r)
-    r = S->busy[s]->footprint;
-  return r;
+  return S->busy[s] ? job_bound(S, s, S->busy[s]) : S->resident[s];
 }
 
 size_t in_use(const agrav_server *S)
@@ -108,7 +118,7 @@ bool evict(agrav_server *S, int s)
 bool fits(const agrav_server *S, int s, const Job *j)
 {
   const size_t mine = S->resident[s];  // slot s is idle: charged its resident bytes
-  return in_use(S) - mine + (mine > j->footprint ? mine : j->footprint) <= S->budget;
+  return in_use(S) - mine + job_bound(S, s, j) <= S->budget;
 }
 
 // First queued job (arrival order) that slot s can admit within the budget,
@@ -226,9 +236,9 @@ void worker(agrav_server *S, int slot)
       lk.lock();
       pick->t_end = clk::now();
       pick->done  = true;
-      // the model is an upper bound of what a solve leaves resident; count the
-      // solves that exceed it (their real bytes are charged from now on)
-      const size_t bound = pick->footprint > S->resident[slot] ? pick->footprint : S->resident[slot];
+      // the bound charged while the job ran; count the solves that exceed it (their
+      // real bytes are charged from now on)
+      const size_t bound = job_bound(S, slot, pick);
       if(now > bound)
         S->model_exceeded++;
       S->resident[slot] = now;
@@ -251,14 +261,15 @@ double ms_between(clk::time_point a, clk::time_point b) { return std::chrono::du
 
 namespace agrav
 {
-// Device bytes one solve of n particles adds to a fresh context: about 300 B per
-// particle of inputs, sorted copies, keys, prefix sums and results, about 175 B
-// per octree node (at most 1.5 nodes per particle assumed; clustered sets have
-// 0.5-0.7), the sort's scratch and a fixed reserve for the small buffers.
+// Device bytes one solve of n particles adds to a fresh context: about 350 B per
+// particle of inputs, sorted copies, keys, prefix sums, results and the walk's
+// records at child-list positions, about 225 B per octree node (at most 1.5 nodes
+// per particle assumed; clustered sets have 0.5-0.7), the sort's scratch and a fixed
+// reserve for the small buffers.
 size_t solve_footprint(const agrav_params *p, int64_t n)
 {
   const size_t nn = (size_t)(n > 0 ? n : 1);
-  size_t per      = 720;
+  size_t per      = 860;
   if(p && p->pmgrid > 0)
     per += 32;  // PM accelerations and potentials per particle
   return nn * per + ((size_t)16 << 20);

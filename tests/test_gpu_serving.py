@@ -162,3 +162,54 @@ def test_server_rejects_unknown_ticket(agrav):
     with agrav.Server(_params(agrav), slots=1) as s:
         with pytest.raises(agrav.AgravError):
             s.wait(123456)
+
+
+def test_request_stream_at_full_rate(agrav):
+    """The serving workload's request stream (Poisson arrivals at 200 per second, isolated Hernquist sets
+    with N log-normal around 5e4, sigma 1.2 in ln N, in [256, 4M]) for 8 s in real time on 4 slots: every
+    request completes, the server keeps up with the arrivals, stays within its HBM budget, and sampled
+    results equal solves of the same sets alone."""
+    import time
+    from arepo_amd import ics
+    g = ics.rng(71)
+    pool_n = 1 << 22
+    pool = ics._isotropic(ics.hernquist_radii(pool_n, 1.0, g), g)
+    rate, seconds = 200.0, 8.0
+    t_arr = np.cumsum(g.exponential(1.0 / rate, int(rate * seconds * 1.5) + 10))
+    t_arr = t_arr[t_arr < seconds]
+    sizes = np.clip(np.exp(np.log(5e4) + 1.2 * g.standard_normal(len(t_arr))), 256, 4e6).astype(np.int64)
+    starts = g.integers(0, pool_n - sizes + 1)
+    eps = (1.0 + np.sqrt(2.0)) / 50.0
+    p = agrav.make_params(periodic=False, box_size=0.0, pmgrid=0, soft=(eps,), precision=agrav.FP32,
+                          walk=agrav.WALK_GROUP, group_size=0, leaf_bucket=4, eval_potential=True)
+    budget = 64 << 30
+    sample = set(range(0, len(t_arr), 200))
+    kept = {}
+    with agrav.Server(p, slots=4, hbm_budget_bytes=budget) as srv:
+        tickets = []
+        t0 = time.perf_counter()
+        for k, (ta, n, s0) in enumerate(zip(t_arr, sizes, starts)):
+            now = time.perf_counter() - t0
+            if ta > now:
+                time.sleep(ta - now)
+            tickets.append(srv.submit(pool[s0:s0 + n], np.full(n, 1.0 / n)))
+        outs = []
+        for k, t in enumerate(tickets):
+            r = srv.wait(t)
+            if k in sample:
+                kept[k] = r["acc"].copy()
+            outs.append(r["total_ms"])
+        makespan = time.perf_counter() - t0
+        st = srv.stats()
+    lat = np.array(outs)
+    print("%d requests (mean N %.3g) in %.2f s: latency p50 %.2f ms, p99 %.2f ms; %s"
+          % (len(tickets), sizes.mean(), makespan, np.percentile(lat, 50), np.percentile(lat, 99), st))
+    assert st["completed"] == len(tickets) and st["failed"] == 0
+    assert makespan < seconds + 2.0  # kept up with the arrivals
+    assert st["hbm_peak"] <= budget and st["model_exceeded"] == 0
+    with agrav.Gravity(p) as g1:
+        for k, acc in kept.items():
+            n, s0 = int(sizes[k]), int(starts[k])
+            g1.set_particles(pool[s0:s0 + n], np.full(n, 1.0 / n))
+            g1.compute_grav_accelerations(first_step=True)
+            assert np.array_equal(g1.get_accelerations(fields=("acc",))["acc"], acc), k
