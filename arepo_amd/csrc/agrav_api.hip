@@ -377,7 +377,8 @@ agrav_status agrav_create(const agrav_params *p, agrav_ctx **out)
   dbuf_registry = nullptr;
   c->bufs       = reg;
   c->par        = *p;
-  if(hipGetDevice(&c->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+  if(hipGetDevice(&c->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+     hipHostMalloc(&c->hstage, agrav_ctx::kStageBytes, hipHostMallocDefault) != hipSuccess)
     {
       delete c;
       set_error_msg("cannot create a HIP stream");
@@ -464,6 +465,8 @@ void agrav_destroy(agrav_ctx *c)
     return;
   if(c->stream)
     (void)hipStreamSynchronize(c->stream);
+  if(c->hstage)
+    (void)hipHostFree(c->hstage);
   pm_free(c);
   hr_free(c);
   for(auto &e : c->ev)
@@ -668,10 +671,12 @@ static agrav_status set_particles_common(agrav_ctx *c, int64_t n, const double *
       k_mass_partial<<<(unsigned)nb, B, 0, s>>>(c->d_mass_in.p, n, c->d_mass_part.p);
       k_mass_final<<<1, B, 0, s>>>(c->d_mass_part.p, nb, c->d_mass_part.p + nb);
       AG_HIP(hipGetLastError());
-      AG_HIP(hipMemcpyAsync(&mt, c->d_mass_part.p + nb, sizeof(double), hipMemcpyDeviceToHost, s));
+      AG_HIP(hipMemcpyAsync(c->stage<double>(0), c->d_mass_part.p + nb, sizeof(double), hipMemcpyDeviceToHost, s));
     }
   AG_HIP(hipMemsetAsync(c->d_oldacc.p, 0, nn * sizeof(float), s));
   AG_HIP(hipStreamSynchronize(s));
+  if(n > 0)
+    mt = *c->stage<double>(0);
   c->mass_total     = mt;
   c->n              = n;
   c->have_stype     = stype != nullptr;

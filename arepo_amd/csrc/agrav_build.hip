@@ -1421,8 +1421,8 @@ agrav_status build_tree(agrav_ctx *c)
     return st;
   // the one host synchronisation of the octree: node count and cubes per level
   static_assert(14 + AG_MAX_LEVELS + 1 <= 48, "cubes per level fit below the walk counters");
-  int hc[2 + AG_MAX_LEVELS + 1];
-  AG_HIP(hipMemcpyAsync(hc, d_total, sizeof(hc), hipMemcpyDeviceToHost, s));
+  int *hc = c->stage<int>(0);
+  AG_HIP(hipMemcpyAsync(hc, d_total, (2 + AG_MAX_LEVELS + 1) * sizeof(int), hipMemcpyDeviceToHost, s));
   AG_HIP(hipStreamSynchronize(s));
   for(int L = 0; L <= AG_MAX_LEVELS; L++)
     c->lvlcnt_h[L] = hc[2 + L];
@@ -1552,10 +1552,12 @@ agrav_status build_tree(agrav_ctx *c)
   // round trip to the device
   AG_HIP(hipEventRecord(c->ev[1], s));
   // out-of-box check (forcetree.c:432-438 aborts; here it is an error code)
-  int cnow[64];
-  AG_HIP(hipMemcpyAsync(cnow, c->d_counters.p, sizeof(cnow), hipMemcpyDeviceToHost, s));
-  AG_HIP(hipMemcpyAsync(&c->h_dom, c->d_dom.p, sizeof(Domain), hipMemcpyDeviceToHost, s));
+  static_assert(64 * sizeof(int) + sizeof(Domain) <= agrav_ctx::kStageBytes, "read-backs fit the staging block");
+  const int *cnow = c->stage<int>(0);
+  AG_HIP(hipMemcpyAsync(c->stage<int>(0), c->d_counters.p, 64 * sizeof(int), hipMemcpyDeviceToHost, s));
+  AG_HIP(hipMemcpyAsync(c->stage<Domain>(256), c->d_dom.p, sizeof(Domain), hipMemcpyDeviceToHost, s));
   AG_HIP(hipStreamSynchronize(s));
+  c->h_dom = *c->stage<Domain>(256);
   const int perr = cnow[13], berr = cnow[50];
   if(berr)
     return fail(c, AGRAV_ERR_CAPACITY, "octree threading inconsistent (moment pass stopped)");

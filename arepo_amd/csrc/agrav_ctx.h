@@ -141,6 +141,11 @@ struct agrav_ctx
 {
   agrav_params par{};
   hipStream_t stream = nullptr;
+  // page-locked host block for the small read-backs of a step (counters, node count,
+  // domain): copies to it are asynchronous, so each read-back costs one stream sync
+  static constexpr size_t kStageBytes = 4096;
+  void *hstage = nullptr;
+  template <typename T> T *stage(size_t off) { return reinterpret_cast<T *>(static_cast<char *>(hstage) + off); }
   int device         = 0;
   std::string err;
 

@@ -16,6 +16,8 @@
 //    per tile and the tile sums are added with compensated (Kahan) sums.
 #include "agrav_ctx.h"
 
+#include <string.h>
+
 namespace agrav
 {
 
@@ -1871,8 +1873,9 @@ static agrav_status walk_pass(agrav_ctx *c, int use_bh, const WalkConst &wc, con
       AG_HIP(hipEventRecord(c->ev[4], s));  // the walk's end, unless groups are re-walked below
       // groups whose LDS stack overflowed (their count is isum[3]) are re-walked one
       // lane per target
-      AG_HIP(hipMemcpyAsync(cnt, c->d_isum.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
+      AG_HIP(hipMemcpyAsync(c->stage<unsigned long long>(0), c->d_isum.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
       AG_HIP(hipStreamSynchronize(s));
+      memcpy(cnt, c->stage<unsigned long long>(0), sizeof(cnt));
       const int nf = (int)(unsigned)cnt[3];
       have_cnt     = nf == 0;
       if(nf > 0)
@@ -1889,8 +1892,9 @@ static agrav_status walk_pass(agrav_ctx *c, int use_bh, const WalkConst &wc, con
     {
       // other walks, or a group walk with re-walked groups: one more read-back
       AG_HIP(hipEventRecord(c->ev[4], s));
-      AG_HIP(hipMemcpyAsync(cnt, c->d_isum.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
+      AG_HIP(hipMemcpyAsync(c->stage<unsigned long long>(0), c->d_isum.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
       AG_HIP(hipStreamSynchronize(s));
+      memcpy(cnt, c->stage<unsigned long long>(0), sizeof(cnt));
     }
   const unsigned long long tot = cnt[0];
   c->tim.walk_popped           = (long long)cnt[1];
