@@ -390,6 +390,16 @@ agrav_status agrav_create(const agrav_params *p, agrav_ctx **out)
         agrav_destroy(c);
         return AGRAV_ERR_HIP;
       }
+  // the PM stream of a periodic PMGRID (same default priority as the main stream)
+  if(p->pmgrid > 0 && p->periodic &&
+     (hipStreamCreateWithFlags(&c->pm_stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess))
+    {
+      agrav_destroy(c);
+      set_error_msg("cannot create the PM stream");
+      return AGRAV_ERR_HIP;
+    }
   soft_from(c, p->n_soft_types, p->softening);
   // force_short_range_init, forcetree_walk.c:72-85 (FP64 compute, FP32 store)
   for(int i = 0; i <= AG_NTAB; i++)
@@ -465,11 +475,16 @@ void agrav_destroy(agrav_ctx *c)
     return;
   if(c->stream)
     (void)hipStreamSynchronize(c->stream);
+  if(c->pm_stream)
+    (void)hipStreamSynchronize(c->pm_stream);
   if(c->hstage)
     (void)hipHostFree(c->hstage);
   pm_free(c);
   hr_free(c);
   for(auto &e : c->ev)
+    if(e)
+      (void)hipEventDestroy(e);
+  for(hipEvent_t e : {c->ev_fork, c->ev_join})
     if(e)
       (void)hipEventDestroy(e);
   // DBuf members release their memory explicitly
@@ -566,6 +581,8 @@ void agrav_destroy(agrav_ctx *c)
   c->d_apos.release();
   c->d_ainv.release();
   c->unb.release();
+  if(c->pm_stream)
+    (void)hipStreamDestroy(c->pm_stream);
   if(c->stream)
     (void)hipStreamDestroy(c->stream);
   delete c;
@@ -944,13 +961,24 @@ agrav_status agrav_compute_grav_accelerations(agrav_ctx *c, int first_step)
     return AGRAV_ERR_BAD_PARAM;
   if(!c->have_particles)
     return fail(c, AGRAV_ERR_STATE, "compute_grav_accelerations before set_particles");
-  // the PM does not depend on the tree; running it after the build lets it use the
-  // Hilbert-ordered particles (the reference calls long_range_force first)
-  agrav_status st = agrav_construct_forcetree(c, nullptr);
+  // the PM does not depend on the tree; running it after the build's gather lets it use
+  // the Hilbert-ordered particles (the reference calls long_range_force first).  A full
+  // tree's periodic PM runs on the PM stream beside the octree kernels; the main stream
+  // joins it before the walk, whose epilogue adds the PM force.
+  const bool partial_build = c->partial_tree && c->n_active >= 0 && c->batch_nsets == 0;
+  const bool need_pm       = c->par.pmgrid > 0 && !(partial_build && c->have_pm);
+  c->pm_fork               = need_pm && c->pm_stream && !partial_build && c->batch_nsets == 0 && !hr_on(c) && c->n > 0;
+  agrav_status st          = agrav_construct_forcetree(c, nullptr);
+  c->pm_fork               = false;
+  if(c->pm_forked)
+    {
+      AG_HIP(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+      c->pm_forked = false;
+    }
+  else if(st == AGRAV_OK && need_pm)
+    st = pm_force(c);
   // a partial-tree step keeps the long-range force of the last PM while the particles
   // are unchanged, as GravPM between PM steps (do_gravity_hydro.c:105-123)
-  if(st == AGRAV_OK && c->par.pmgrid > 0 && !(c->tree_partial && c->have_pm))
-    st = pm_force(c);
   else if(st == AGRAV_OK && c->par.pmgrid > 0 && (st = pm_time_settle(c)) == AGRAV_OK)
     c->tim.pm_grav = 0;  // no PM in this call
   // the walks of a full step write the final outputs from their epilogue

@@ -1232,6 +1232,13 @@ __global__ void k_check_active(const int *__restrict__ active, int64_t na, int64
     atomicOr(bad, 2);
 }
 
+// build counters at the start of a build: the three extent minima (u64, ints 0..5)
+// start at all ones, the maxima and every other counter (ints 6..63) at zero
+__global__ void k_counters_init(int *cnt)
+{
+  cnt[threadIdx.x] = threadIdx.x < 6 ? -1 : 0;
+}
+
 // PM results kept in the sorted order of the tree about to be replaced, back to caller order
 __global__ void k_unsort_pm(const int *__restrict__ perm, int64_t n, const double *__restrict__ acc_s,
                             const double *__restrict__ pot_s, double *acc, double *pot)
@@ -1322,12 +1329,9 @@ agrav_status build_tree(agrav_ctx *c)
   if((st = c->d_counters.ensure(128)) != AGRAV_OK)
     return st;
   int *d_total = c->d_counters.p + 12, *d_perr = c->d_counters.p + 13, *d_lvlcnt = c->d_counters.p + 14;
-  // one upload sets the extent's start values and clears every other build counter
-  int cinit[64] = {};
+  // one small kernel sets the extent's start values and clears every other build counter
   unsigned long long *d_mm = (unsigned long long *)c->d_counters.p;  // 6 x u64 = 12 ints
-  for(int k = 0; k < 6; k++)
-    cinit[k] = -1;  // the three minima start at all ones, the maxima at zero
-  AG_HIP(hipMemcpyAsync(c->d_counters.p, cinit, sizeof(cinit), hipMemcpyHostToDevice, s));
+  k_counters_init<<<1, 64, 0, s>>>(c->d_counters.p);
 
   // 1. domain (one cube, or one per set of a batch)
   const int nsets = c->batch_nsets;
@@ -1372,8 +1376,16 @@ agrav_status build_tree(agrav_ctx *c)
   // one set: the first words are sorted from bit 15 up (6 passes instead of 8); runs
   // sharing a level-16 cell are finished by the tie-run sorts
   const int tshift = nsets == 0 ? 15 : 0;
-  if((st = radix_sort_u64(c->d_key.p, c->d_idx.p, c->d_key2.p, c->d_idx2.p, n, 63, c->d_temp.p, d_perr, s, tshift)) != AGRAV_OK)
+  bool sorted_in_keys = false;
+  if((st = radix_sort_u64(c->d_key.p, c->d_idx.p, c->d_key2.p, c->d_idx2.p, n, 63, c->d_temp.p, d_perr, s, tshift,
+                          &sorted_in_keys)) != AGRAV_OK)
     return st;
+  if(sorted_in_keys)
+    {
+      // an even pass count ends in the input pair: the buffers trade places instead of a copy
+      c->d_key.swap_storage(c->d_key2);
+      c->d_idx.swap_storage(c->d_idx2);
+    }
   // sorted first key words and perm are now in d_key2/d_idx2; runs of equal first
   // words are put in the order of their second words
   int *d_nties = c->d_counters.p + 60;  // cleared by the upload at the start
@@ -1400,6 +1412,10 @@ agrav_status build_tree(agrav_ctx *c)
                                  nsets > 0 ? c->d_pset.p : nullptr, c->d_px.p, c->d_py.p, c->d_pz.p, c->d_pm.p, multisoft ? c->d_ps.p : nullptr, c->d_prec.p,
                                  c->d_oldacc_s.p);
   AG_HIP(hipGetLastError());
+  // a full step's periodic PM needs only the sorted particles: it runs on the PM stream
+  // beside the octree kernels below
+  if(c->pm_fork && !part && nsets == 0 && (st = pm_fork(c)) != AGRAV_OK)
+    return st;
 
   // 3./4. octree cubes straight from the sorted keys
   if((st = c->d_lvlmask.ensure(n)) || (st = c->d_pcnt.ensure(n)) || (st = c->d_pbase.ensure(n)))

@@ -250,6 +250,12 @@ struct agrav_ctx
   agrav::DBuf<double> d_acc_pm2, d_pot_pm2;  // caller-order copies made before a partial tree replaces the order
   bool pm_sorted = false;
   bool pm_time_pending = false;  // the last PM's events are recorded, its time not yet read
+  // a full step's periodic PM runs on its own stream beside the octree kernels: forked
+  // once the gather has written the Hilbert-ordered particles, joined before the walk
+  hipStream_t pm_stream = nullptr;  // created with a periodic PMGRID (default priority)
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  bool pm_fork   = false;  // the next build forks the PM (set by compute_grav_accelerations)
+  bool pm_forked = false;  // the build forked it; the main stream has not joined it yet
   // finalize fused into the group walk's epilogue (set by compute_grav_accelerations)
   bool fuse_final   = false;  // the next group walk may write the final outputs
   bool final_fused  = false;  // the last walk wrote them for every target
@@ -307,6 +313,7 @@ agrav_status active_targets(agrav_ctx *c);
 agrav_status pm_init(agrav_ctx *c);
 void pm_free(agrav_ctx *c);
 agrav_status pm_force(agrav_ctx *c);
+agrav_status pm_fork(agrav_ctx *c);
 agrav_status pm_time_settle(agrav_ctx *c);
 bool hr_on(const agrav_ctx *c);
 void hr_free(agrav_ctx *c);

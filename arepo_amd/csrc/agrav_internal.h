@@ -173,10 +173,12 @@ size_t scan_temp_bytes(int64_t n);
 agrav_status exclusive_scan_i32(const int *d_in, int *d_out, int64_t n, int *d_total, void *d_temp, int *d_err, hipStream_t s);
 size_t sort_temp_bytes(int64_t n);
 // LSD radix sort of 64-bit keys with 32-bit values, stable; bits [lo_bit, key_bits)
-// are sorted (keys equal in those bits keep their input order).  Results end up in
-// keys_out/vals_out.
+// are sorted (keys equal in those bits keep their input order).  The passes alternate
+// between the two buffer pairs.  Results end up in keys_out/vals_out; with in_input
+// given, an even pass count instead leaves them in keys_in/vals_in (no copy) and sets
+// *in_input.
 agrav_status radix_sort_u64(uint64_t *keys_in, int *vals_in, uint64_t *keys_out, int *vals_out, int64_t n, int key_bits,
-                            void *d_temp, int *d_err, hipStream_t s, int lo_bit = 0);
+                            void *d_temp, int *d_err, hipStream_t s, int lo_bit = 0, bool *in_input = nullptr);
 
 // AREPO's 3-D Peano-Hilbert key (peano_hilbert_key, peano.c:374-390) on b-bit
 // coordinates, evaluated as a state machine: each level's key digit depends on

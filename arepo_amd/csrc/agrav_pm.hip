@@ -434,7 +434,8 @@ agrav_status pm_init(agrav_ctx *c)
   return AGRAV_OK;
 }
 
-template <typename R> static agrav_status pm_run(agrav_ctx *c)
+// sorted: the particles are in the Hilbert order of a current full tree (px, py, pz, pm)
+template <typename R> static agrav_status pm_run(agrav_ctx *c, hipStream_t s, bool sorted)
 {
   PmState &p      = c->pm;
   const int G     = p.grid;
@@ -442,7 +443,6 @@ template <typename R> static agrav_status pm_run(agrav_ctx *c)
   const int64_t n = c->n;
   const int64_t g3 = (int64_t)G * G * G;
   const int64_t nc = (int64_t)G * G * (G / 2 + 1);
-  hipStream_t s   = c->stream;
   agrav_status st;
   if((st = c->d_acc_pm.ensure(3 * (n ? n : 1))) || (st = c->d_pot_pm.ensure(n ? n : 1)))
     return st;
@@ -454,7 +454,6 @@ template <typename R> static agrav_status pm_run(agrav_ctx *c)
   AG_HIP(hipMemsetAsync(p.rho_fix.p, 0, g3 * sizeof(unsigned long long), s));
   // with a current tree the particles are available in Hilbert order: tiled deposit and
   // coherent readout, results kept in sorted order (the finalize kernel maps them back)
-  const bool sorted = c->have_tree && !c->tree_partial;  // a partial tree lacks the inactive particles
   if(n > 0 && sorted)
     k_cic_deposit_sorted<<<nb(n), B, 0, s>>>(c->d_px.p, c->d_py.p, c->d_pz.p, c->d_pm.p, n, G, L, scale, p.rho_fix.p);
   int *d_bad = nullptr;
@@ -471,6 +470,8 @@ template <typename R> static agrav_status pm_run(agrav_ctx *c)
   AG_HIP(hipGetLastError());
   void *in[1]  = {p.real.p};
   void *out[1] = {p.cplx.p};
+  rocfft_execution_info_set_stream(p.info_f, s);
+  rocfft_execution_info_set_stream(p.info_b, s);
   if((st = fft_check(c, rocfft_execute(p.fwd, in, out, p.info_f), "execute forward")))
     return st;
   k_green_apply<R><<<nb(nc), B, 0, s>>>((R *)p.cplx.p, p.green.p, nc);
@@ -515,24 +516,45 @@ agrav_status pm_time_settle(agrav_ctx *c)
   return AGRAV_OK;
 }
 
-agrav_status pm_force(agrav_ctx *c)
+// the periodic grid on stream s; the high-resolution grid follows on the main stream
+static agrav_status pm_force_on(agrav_ctx *c, hipStream_t s, bool sorted)
 {
   if(c->par.pmgrid <= 0)
     return AGRAV_OK;
   agrav_status st = pm_init(c);
   if(st != AGRAV_OK || (st = pm_time_settle(c)) != AGRAV_OK)
     return st;
-  AG_HIP(hipEventRecord(c->ev[5], c->stream));
-  st = (c->par.precision == AGRAV_FP32) ? pm_run<float>(c) : pm_run<double>(c);
+  AG_HIP(hipEventRecord(c->ev[5], s));
+  st = (c->par.precision == AGRAV_FP32) ? pm_run<float>(c, s, sorted) : pm_run<double>(c, s, sorted);
   // the high-resolution grid after the periodic one (longrange.c:129-140)
   if(st == AGRAV_OK && hr_on(c))
     st = hr_pm(c);
   if(st != AGRAV_OK)
     return st;
-  AG_HIP(hipEventRecord(c->ev[6], c->stream));
+  AG_HIP(hipEventRecord(c->ev[6], s));
   c->pm_time_pending = true;
   c->have_pm         = true;
   return AGRAV_OK;
+}
+
+agrav_status pm_force(agrav_ctx *c)
+{
+  // a partial tree lacks the inactive particles: the PM then reads caller order
+  return pm_force_on(c, c->stream, c->have_tree && !c->tree_partial);
+}
+
+// a full step's PM on the PM stream, once the build's gather has written the sorted
+// particles (the build calls this right after the gather; the caller joins ev_join)
+agrav_status pm_fork(agrav_ctx *c)
+{
+  if(!c->pm_stream || hr_on(c))
+    return fail(c, AGRAV_ERR_STATE, "no PM stream for this context");
+  AG_HIP(hipEventRecord(c->ev_fork, c->stream));
+  AG_HIP(hipStreamWaitEvent(c->pm_stream, c->ev_fork, 0));
+  c->pm_forked    = true;  // from here on the main stream must join, whatever happens
+  agrav_status st = pm_force_on(c, c->pm_stream, true);
+  AG_HIP(hipEventRecord(c->ev_join, c->pm_stream));
+  return st;
 }
 
 }  // namespace agrav

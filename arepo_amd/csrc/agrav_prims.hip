@@ -458,8 +458,10 @@ size_t sort_temp_bytes(int64_t n)
 }
 
 agrav_status radix_sort_u64(uint64_t *keys_in, int *vals_in, uint64_t *keys_out, int *vals_out, int64_t n, int key_bits,
-                            void *d_temp, int *d_err, hipStream_t s, int lo_bit)
+                            void *d_temp, int *d_err, hipStream_t s, int lo_bit, bool *in_input)
 {
+  if(in_input)
+    *in_input = false;
   if(n <= 0)
     return AGRAV_OK;
   const int64_t nt  = (n + kSortTile - 1) / kSortTile;
@@ -488,7 +490,9 @@ agrav_status radix_sort_u64(uint64_t *keys_in, int *vals_in, uint64_t *keys_out,
       va           = vb;
       vb           = tv;
     }
-  if(ka != keys_out)
+  if(ka != keys_out && in_input)
+    *in_input = true;
+  else if(ka != keys_out)
     {
       AG_HIP(hipMemcpyAsync(keys_out, ka, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
       AG_HIP(hipMemcpyAsync(vals_out, va, n * sizeof(int), hipMemcpyDeviceToDevice, s));
