@@ -809,10 +809,15 @@ agrav_status agrav_construct_forcetree(agrav_ctx *c, int *numnodes)
       c->have_tree = false;
       return st;
     }
-  AG_HIP(hipEventSynchronize(c->ev[1]));  // recorded by the build (already passed when it read back)
-  float ms = 0;
-  AG_HIP(hipEventElapsedTime(&ms, c->ev[7], c->ev[1]));
-  c->tim.treebuild = ms;
+  if(c->build_check_pending)
+    c->build_time_pending = true;  // read with the deferred check
+  else
+    {
+      AG_HIP(hipEventSynchronize(c->ev[1]));  // recorded by the build (already passed when it read back)
+      float ms = 0;
+      AG_HIP(hipEventElapsedTime(&ms, c->ev[7], c->ev[1]));
+      c->tim.treebuild = ms;
+    }
   c->tim.numnodes  = c->m;
   c->tmap_dirty    = true;  // the sorted positions of the active targets changed
   if(numnodes)
@@ -967,9 +972,14 @@ agrav_status agrav_compute_grav_accelerations(agrav_ctx *c, int first_step)
   // joins it before the walk, whose epilogue adds the PM force.
   const bool partial_build = c->partial_tree && c->n_active >= 0 && c->batch_nsets == 0;
   const bool need_pm       = c->par.pmgrid > 0 && !(partial_build && c->have_pm);
-  c->pm_fork               = need_pm && c->pm_stream && !partial_build && c->batch_nsets == 0 && !hr_on(c) && c->n > 0;
+  const bool plain_step    = c->n_active < 0 && c->batch_nsets == 0 && !hr_on(c) && c->n > 0;
+  c->pm_fork               = need_pm && c->pm_stream && plain_step;
+  // a full step of every particle checks its build after the walk's read-back (the walk
+  // tests the build's flags on the device before it starts)
+  c->defer_build_check     = plain_step;
   agrav_status st          = agrav_construct_forcetree(c, nullptr);
   c->pm_fork               = false;
+  c->defer_build_check     = false;
   if(c->pm_forked)
     {
       AG_HIP(hipStreamWaitEvent(c->stream, c->ev_join, 0));
@@ -995,6 +1005,14 @@ agrav_status agrav_compute_grav_accelerations(agrav_ctx *c, int first_step)
     st = finalize(c);
   c->fuse_final  = false;
   c->final_fused = false;
+  if(c->build_check_pending)
+    {
+      // normally read with the walk's counters; otherwise here
+      AG_HIP(hipStreamSynchronize(c->stream));
+      const agrav_status sb = build_check(c);
+      if(st == AGRAV_OK)
+        st = sb;
+    }
   return st;
 }
 

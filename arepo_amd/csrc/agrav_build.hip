@@ -357,11 +357,16 @@ __global__ void k_tie_runs(const uint64_t *__restrict__ keys, int n, int tshift,
 
 __device__ inline bool tie_less(uint64_t ka, int va, uint64_t kb, int vb);
 
+// a sorted value read as a caller index: a sort whose look-back timed out (the build
+// reports it) may leave entries that are not indices; they read index 0, so that no
+// read leaves the arrays
+__device__ inline int caller_index(int i, int nall) { return (unsigned)i < (unsigned)nall ? i : 0; }
+
 // Short runs (the usual case: a few particles sharing a deep cell), one thread
 // each: the run's second key words, then an insertion sort by (second word,
 // caller index) in place.  Runs longer than kShortRun go to the workgroup sort.
 static constexpr int kShortRun = 32;
-__global__ void k_tie_sort_short(uint64_t *keys, const uint64_t *__restrict__ keys_lo, int n, int tshift,
+__global__ void k_tie_sort_short(uint64_t *keys, const uint64_t *__restrict__ keys_lo, int n, int nall, int tshift,
                                  const int *__restrict__ runs, const int *__restrict__ nruns, int *idx, uint64_t *lo_s,
                                  int *long_runs, int *nlong)
 {
@@ -379,7 +384,7 @@ __global__ void k_tie_sort_short(uint64_t *keys, const uint64_t *__restrict__ ke
           continue;
         }
       for(int t = k0; t < e; t++)
-        lo_s[t] = tie_sec(keys[t], keys_lo[idx[t]], tshift);
+        lo_s[t] = tie_sec(keys[t], keys_lo[caller_index(idx[t], nall)], tshift);
       for(int t = k0 + 1; t < e; t++)
         {
           const uint64_t kk = lo_s[t];
@@ -412,7 +417,7 @@ __device__ inline bool tie_less(uint64_t ka, int va, uint64_t kb, int vb) { retu
 // whose every compare-exchange puts the smaller pair first, so that the padding
 // to a power of two (virtual +inf entries) never moves; in LDS up to 1024
 // entries, in place in global memory beyond.  Also the sorted second words.
-__global__ __launch_bounds__(256) void k_tie_sort(uint64_t *keys, const uint64_t *__restrict__ keys_lo, int n, int tshift,
+__global__ __launch_bounds__(256) void k_tie_sort(uint64_t *keys, const uint64_t *__restrict__ keys_lo, int n, int nall, int tshift,
                                                   const int *__restrict__ runs, const int *__restrict__ nruns, int *idx,
                                                   uint64_t *lo_s)
 {
@@ -441,7 +446,7 @@ __global__ __launch_bounds__(256) void k_tie_sort(uint64_t *keys, const uint64_t
       int *gv        = idx + k0;
       for(int t = threadIdx.x; t < P; t += 256)
         {
-          const uint64_t kk = t < len ? tie_sec(keys[k0 + t], keys_lo[idx[k0 + t]], tshift) : ~0ull;
+          const uint64_t kk = t < len ? tie_sec(keys[k0 + t], keys_lo[caller_index(idx[k0 + t], nall)], tshift) : ~0ull;
           if(lds)
             {
               s_k[t] = kk;
@@ -506,14 +511,14 @@ __global__ __launch_bounds__(256) void k_tie_sort(uint64_t *keys, const uint64_t
 }
 
 __global__ void k_gather(const int *__restrict__ idx, const double *__restrict__ pos, const double *__restrict__ mass,
-                         const uint8_t *__restrict__ stype, const float *__restrict__ oldacc, int64_t n, int periodic, double box,
-                         BuildGeo geo, int *pset_sorted, double *px, double *py, double *pz, double *pm, uint8_t *ps, uint4 *prec,
-                         float *oldacc_s)
+                         const uint8_t *__restrict__ stype, const float *__restrict__ oldacc, int64_t n, int64_t nall, int periodic,
+                         double box, BuildGeo geo, int *pset_sorted, double *px, double *py, double *pz, double *pm, uint8_t *ps,
+                         uint4 *prec, float *oldacc_s)
 {
   int64_t k = (int64_t)blockIdx.x * B + threadIdx.x;
   if(k >= n)
     return;
-  int i            = idx[k];
+  const int i = caller_index(idx[k], (int)nall);
   const int set    = geo.pset_in ? geo.pset_in[i] : 0;
   const Domain dom = geo.dom[set];
   if(pset_sorted)
@@ -1233,10 +1238,19 @@ __global__ void k_check_active(const int *__restrict__ active, int64_t na, int64
 }
 
 // build counters at the start of a build: the three extent minima (u64, ints 0..5)
-// start at all ones, the maxima and every other counter (ints 6..63) at zero
+// start at all ones, the maxima and every other counter (ints 6..63) at zero, and the
+// key sort's digit counts (ints kSortHist .. + 2048) at zero
+static constexpr int kSortHist = 128;
+// offset of the build's read-back in the context's staging block (the walk reads back
+// its counters at offset 0)
+static constexpr size_t kBuildStage = 1024;
 __global__ void k_counters_init(int *cnt)
 {
-  cnt[threadIdx.x] = threadIdx.x < 6 ? -1 : 0;
+  const int t = threadIdx.x;
+  if(t < 64)
+    cnt[t] = t < 6 ? -1 : 0;
+  for(int q = t; q < 2048; q += blockDim.x)
+    cnt[kSortHist + q] = 0;
 }
 
 // PM results kept in the sorted order of the tree about to be replaced, back to caller order
@@ -1326,12 +1340,12 @@ agrav_status build_tree(agrav_ctx *c)
   // device counters: [0..11] domain extent (6 u64), [12] octree node count, [13] a scan or
   // sort look-back that timed out, [14..35] cubes per level (read back together),
   // [48..55] walk queue and failures, [50] threading error
-  if((st = c->d_counters.ensure(128)) != AGRAV_OK)
+  if((st = c->d_counters.ensure(kSortHist + 2048)) != AGRAV_OK)
     return st;
   int *d_total = c->d_counters.p + 12, *d_perr = c->d_counters.p + 13, *d_lvlcnt = c->d_counters.p + 14;
   // one small kernel sets the extent's start values and clears every other build counter
   unsigned long long *d_mm = (unsigned long long *)c->d_counters.p;  // 6 x u64 = 12 ints
-  k_counters_init<<<1, 64, 0, s>>>(c->d_counters.p);
+  k_counters_init<<<1, 256, 0, s>>>(c->d_counters.p);
 
   // 1. domain (one cube, or one per set of a batch)
   const int nsets = c->batch_nsets;
@@ -1378,7 +1392,7 @@ agrav_status build_tree(agrav_ctx *c)
   const int tshift = nsets == 0 ? 15 : 0;
   bool sorted_in_keys = false;
   if((st = radix_sort_u64(c->d_key.p, c->d_idx.p, c->d_key2.p, c->d_idx2.p, n, 63, c->d_temp.p, d_perr, s, tshift,
-                          &sorted_in_keys)) != AGRAV_OK)
+                          &sorted_in_keys, c->d_counters.p + kSortHist)) != AGRAV_OK)
     return st;
   if(sorted_in_keys)
     {
@@ -1388,14 +1402,15 @@ agrav_status build_tree(agrav_ctx *c)
     }
   // sorted first key words and perm are now in d_key2/d_idx2; runs of equal first
   // words are put in the order of their second words
-  int *d_nties = c->d_counters.p + 60;  // cleared by the upload at the start
+  int *d_nties = c->d_counters.p + 60;  // cleared at the start of the build
   if(n > 1)
     {
       int *d_nlong = c->d_counters.p + 61, *d_long = c->d_ties.p + (n / 2 + 1);
       k_tie_runs<<<nblk(n), B, 0, s>>>(c->d_key2.p, (int)n, tshift, c->d_ties.p, d_nties);
-      k_tie_sort_short<<<512, B, 0, s>>>(c->d_key2.p, c->d_key_lo.p, (int)n, tshift, c->d_ties.p, d_nties, c->d_idx2.p,
+      k_tie_sort_short<<<512, B, 0, s>>>(c->d_key2.p, c->d_key_lo.p, (int)n, (int)nall, tshift, c->d_ties.p, d_nties, c->d_idx2.p,
                                          c->d_key_lo_s.p, d_long, d_nlong);
-      k_tie_sort<<<256, 256, 0, s>>>(c->d_key2.p, c->d_key_lo.p, (int)n, tshift, d_long, d_nlong, c->d_idx2.p, c->d_key_lo_s.p);
+      k_tie_sort<<<256, 256, 0, s>>>(c->d_key2.p, c->d_key_lo.p, (int)n, (int)nall, tshift, d_long, d_nlong, c->d_idx2.p,
+                                     c->d_key_lo_s.p);
       AG_HIP(hipGetLastError());
     }
 
@@ -1408,7 +1423,7 @@ agrav_status build_tree(agrav_ctx *c)
   if(multisoft && (st = c->d_ps.ensure(n)))
     return st;
   k_gather<<<nblk(n), B, 0, s>>>(c->d_idx2.p, c->d_pos_in.p, c->d_mass_in.p, c->have_stype ? c->d_stype_in.p : nullptr,
-                                 c->have_oldacc ? c->d_oldacc.p : nullptr, n, c->par.periodic, c->par.box_size, geo,
+                                 c->have_oldacc ? c->d_oldacc.p : nullptr, n, nall, c->par.periodic, c->par.box_size, geo,
                                  nsets > 0 ? c->d_pset.p : nullptr, c->d_px.p, c->d_py.p, c->d_pz.p, c->d_pm.p, multisoft ? c->d_ps.p : nullptr, c->d_prec.p,
                                  c->d_oldacc_s.p);
   AG_HIP(hipGetLastError());
@@ -1480,7 +1495,7 @@ agrav_status build_tree(agrav_ctx *c)
   //    levels share one single-workgroup launch
   if(scan_temp_bytes(M) + 4096 > c->d_temp.cap && (st = c->d_temp.ensure(scan_temp_bytes(M) + 4096)) != AGRAV_OK)
     return st;
-  int *d_berr = c->d_counters.p + 50;  // cleared by the upload at the start
+  int *d_berr = c->d_counters.p + 50;  // cleared at the start of the build
   if((st = c->d_npos.ensure(M)))
     return st;
   k_count_children<<<nblk(M), B, 0, s>>>(M, N, c->d_nextnode.p, c->d_sibling.p, c->d_pnext.p, c->d_ccnt.p, c->d_npos.p, N + M,
@@ -1567,23 +1582,47 @@ agrav_status build_tree(agrav_ctx *c)
   // end of the build, recorded before the read-back so that timing it needs no second
   // round trip to the device
   AG_HIP(hipEventRecord(c->ev[1], s));
-  // out-of-box check (forcetree.c:432-438 aborts; here it is an error code)
-  static_assert(64 * sizeof(int) + sizeof(Domain) <= agrav_ctx::kStageBytes, "read-backs fit the staging block");
-  const int *cnow = c->stage<int>(0);
-  AG_HIP(hipMemcpyAsync(c->stage<int>(0), c->d_counters.p, 64 * sizeof(int), hipMemcpyDeviceToHost, s));
-  AG_HIP(hipMemcpyAsync(c->stage<Domain>(256), c->d_dom.p, sizeof(Domain), hipMemcpyDeviceToHost, s));
+  // error flags and domain (out-of-box: forcetree.c:432-438 aborts; here it is an error
+  // code), read back into their own part of the staging block
+  static_assert(kBuildStage + 64 * sizeof(int) + sizeof(Domain) <= agrav_ctx::kStageBytes, "read-backs fit the staging block");
+  AG_HIP(hipMemcpyAsync(c->stage<int>(kBuildStage), c->d_counters.p, 64 * sizeof(int), hipMemcpyDeviceToHost, s));
+  AG_HIP(hipMemcpyAsync(c->stage<Domain>(kBuildStage + 256), c->d_dom.p, sizeof(Domain), hipMemcpyDeviceToHost, s));
+  c->m                   = M;
+  c->have_tree           = true;
+  c->build_check_pending = true;
+  // a deferred check is read after the next synchronisation (the walk's read-back):
+  // the walks test the same flags on the device before they start
+  if(c->defer_build_check)
+    return AGRAV_OK;
   AG_HIP(hipStreamSynchronize(s));
-  c->h_dom = *c->stage<Domain>(256);
+  return build_check(c);
+}
+
+agrav_status build_check(agrav_ctx *c)
+{
+  if(!c->build_check_pending)
+    return AGRAV_OK;
+  c->build_check_pending = false;
+  if(c->build_time_pending)
+    {
+      float ms = 0;
+      AG_HIP(hipEventElapsedTime(&ms, c->ev[7], c->ev[1]));
+      c->tim.treebuild      = ms;
+      c->build_time_pending = false;
+    }
+  const int *cnow        = c->stage<int>(kBuildStage);
+  c->h_dom               = *c->stage<Domain>(kBuildStage + 256);
   const int perr = cnow[13], berr = cnow[50];
+  agrav_status st = AGRAV_OK;
   if(berr)
-    return fail(c, AGRAV_ERR_CAPACITY, "octree threading inconsistent (moment pass stopped)");
-  if(perr)
-    return fail(c, AGRAV_ERR_HIP, "scan look-back did not complete");
-  if(c->h_dom.out_of_box)
-    return fail(c, AGRAV_ERR_OUT_OF_BOX, "particle outside the periodic box after one wrap");
-  c->m         = M;
-  c->have_tree = true;
-  return AGRAV_OK;
+    st = fail(c, AGRAV_ERR_CAPACITY, "octree threading inconsistent (moment pass stopped)");
+  else if(perr)
+    st = fail(c, AGRAV_ERR_HIP, "scan look-back did not complete");
+  else if(c->h_dom.out_of_box)
+    st = fail(c, AGRAV_ERR_OUT_OF_BOX, "particle outside the periodic box after one wrap");
+  if(st != AGRAV_OK)
+    c->have_tree = c->have_tree_force = c->have_final = false;
+  return st;
 }
 
 TreeView tree_view(const agrav_ctx *c)
@@ -1621,6 +1660,8 @@ TreeView tree_view(const agrav_ctx *c)
   v.nmeta    = c->d_nmeta.p;
   v.nrecA    = c->d_nrecA.p;
   v.nrecB    = c->d_nrecB.p;
+  v.dom      = c->d_dom.p;
+  v.cnt      = c->d_counters.p;
   return v;
 }
 

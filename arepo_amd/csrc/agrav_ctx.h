@@ -256,6 +256,11 @@ struct agrav_ctx
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   bool pm_fork   = false;  // the next build forks the PM (set by compute_grav_accelerations)
   bool pm_forked = false;  // the build forked it; the main stream has not joined it yet
+  // a full step reads the build's error flags back with the walk's counters (one
+  // synchronisation); the walks test the flags on the device first
+  bool defer_build_check   = false;  // set by compute_grav_accelerations around its build
+  bool build_check_pending = false;  // the build's read-back is queued, not yet checked
+  bool build_time_pending  = false;  // its duration (events ev[7], ev[1]) is read with the check
   // finalize fused into the group walk's epilogue (set by compute_grav_accelerations)
   bool fuse_final   = false;  // the next group walk may write the final outputs
   bool final_fused  = false;  // the last walk wrote them for every target
@@ -308,6 +313,7 @@ namespace agrav
 agrav_status fail(agrav_ctx *c, agrav_status s, const std::string &m);
 size_t device_bytes(const agrav_ctx *c);
 agrav_status build_tree(agrav_ctx *c);
+agrav_status build_check(agrav_ctx *c);
 agrav_status walk_tree(agrav_ctx *c, int use_bh);
 agrav_status active_targets(agrav_ctx *c);
 agrav_status pm_init(agrav_ctx *c);

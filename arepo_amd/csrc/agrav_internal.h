@@ -81,7 +81,16 @@ struct TreeView
   const uint4 *cprec;      // FP32 records of particle children, at their child[] positions
   const int4 *nmeta;       // meta, recA, recB of node children, at their child[] positions
   const uint4 *nrecA, *nrecB;
+  const Domain *dom;       // the domain cube of one set (its fixed-point step; out_of_box)
+  const int *cnt;          // build counters: [13] a sort or scan look-back timed out, [50] threading error
 };
+
+// a tree whose build reported an error on the device (read by the walks before they
+// start, so that a step whose build check is read back after the walk never walks it)
+__device__ inline bool tree_invalid(const TreeView &tv)
+{
+  return tv.cnt[13] != 0 || tv.cnt[50] != 0 || tv.dom->out_of_box != 0;
+}
 
 struct WalkConst
 {
@@ -95,7 +104,6 @@ struct WalkConst
   int ewald;          // periodic tree-only: Ewald correction per interaction
   double ewald_fac;   // 2 EN / box
   int multisoft;
-  float fix_unit;     // fixed-point step as float
   int leaf_bucket;    // opened nodes with at most this many particles are expanded into particles
   int eval_pot;       // EVALPOTENTIAL: also sum the potential
 };
@@ -176,9 +184,11 @@ size_t sort_temp_bytes(int64_t n);
 // are sorted (keys equal in those bits keep their input order).  The passes alternate
 // between the two buffer pairs.  Results end up in keys_out/vals_out; with in_input
 // given, an even pass count instead leaves them in keys_in/vals_in (no copy) and sets
-// *in_input.
+// *in_input.  ghist_zero: a device array of 8 x 256 ints that is zero on entry (the
+// digit counts go there), or null (the sort clears its own in d_temp).
 agrav_status radix_sort_u64(uint64_t *keys_in, int *vals_in, uint64_t *keys_out, int *vals_out, int64_t n, int key_bits,
-                            void *d_temp, int *d_err, hipStream_t s, int lo_bit = 0, bool *in_input = nullptr);
+                            void *d_temp, int *d_err, hipStream_t s, int lo_bit = 0, bool *in_input = nullptr,
+                            int *ghist_zero = nullptr);
 
 // AREPO's 3-D Peano-Hilbert key (peano_hilbert_key, peano.c:374-390) on b-bit
 // coordinates, evaluated as a state machine: each level's key digit depends on

@@ -279,12 +279,19 @@ static_assert(kSortBlock == 256, "the scatter gives each thread one of the 256 d
 // digit histograms of all passes at once: per-block partials, one row per block
 static constexpr int kHistBlocks = 256;
 
+// Digit counts of every pass from one read of the keys: per-block counts in LDS, then
+// one global atomic per non-zero bin into ghist (passes x 256, zero on entry).  The
+// blocks also clear the passes' look-back states (nstates words) for the onesweep
+// kernels that follow.
 __global__ __launch_bounds__(kSortBlock) void k_radix_hist_all(const uint64_t *__restrict__ keys, int64_t n, int passes, int lo_bit,
-                                                               int *__restrict__ part)
+                                                               int *__restrict__ ghist, uint32_t *__restrict__ states,
+                                                               int64_t nstates)
 {
   __shared__ int s_h[8][256];
   for(int q = threadIdx.x; q < 8 * 256; q += kSortBlock)
     s_h[q >> 8][q & 255] = 0;
+  for(int64_t q = (int64_t)blockIdx.x * kSortBlock + threadIdx.x; q < nstates; q += (int64_t)gridDim.x * kSortBlock)
+    states[q] = 0;
   __syncthreads();
   for(int64_t k = (int64_t)blockIdx.x * kSortBlock + threadIdx.x; k < n; k += (int64_t)gridDim.x * kSortBlock)
     {
@@ -294,46 +301,11 @@ __global__ __launch_bounds__(kSortBlock) void k_radix_hist_all(const uint64_t *_
     }
   __syncthreads();
   for(int q = threadIdx.x; q < passes * 256; q += kSortBlock)
-    part[(int64_t)blockIdx.x * 2048 + q] = s_h[q >> 8][q & 255];
-}
-
-// start of every digit in the output of every pass: block p sums the partials of
-// pass p (each wave a quarter of the rows, 8 loads in flight per lane) and scans
-// its 256 digit counts
-__global__ __launch_bounds__(kSortBlock) void k_radix_digit_base(const int *__restrict__ part, int nblocks, int *__restrict__ base)
-{
-  __shared__ int s_part[kSortWaves][256];
-  __shared__ int s_wave[kSortWaves];
-  const int p = blockIdx.x, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  for(int d = lane; d < 256; d += 64)
     {
-      int c = 0;
-      for(int b0 = w * 8; b0 < nblocks; b0 += kSortWaves * 8)
-        {
-          int v[8];
-#pragma unroll
-          for(int u = 0; u < 8; u++)
-            v[u] = b0 + u < nblocks ? part[(int64_t)(b0 + u) * 2048 + p * 256 + d] : 0;
-#pragma unroll
-          for(int u = 0; u < 8; u++)
-            c += v[u];
-        }
-      s_part[w][d] = c;
+      const int v = s_h[q >> 8][q & 255];
+      if(v)
+        atomicAdd(&ghist[q], v);
     }
-  __syncthreads();
-  const int d = threadIdx.x;
-  int c       = 0;
-#pragma unroll
-  for(int ww = 0; ww < kSortWaves; ww++)
-    c += s_part[ww][d];
-  const int incl = wave_incl_scan(c);
-  if(lane == 63)
-    s_wave[w] = incl;
-  __syncthreads();
-  int wpre = 0;
-  for(int ww = 0; ww < w; ww++)
-    wpre += s_wave[ww];
-  base[p * 256 + d] = wpre + incl - c;
 }
 
 // One pass of the sort over one tile of 2048 keys: rank stably by the digit,
@@ -342,13 +314,13 @@ __global__ __launch_bounds__(kSortBlock) void k_radix_digit_base(const int *__re
 // its global position with consecutive lanes on consecutive addresses.
 __global__ __launch_bounds__(kSortBlock) void k_radix_onesweep(const uint64_t *__restrict__ kin, const int *__restrict__ vin,
                                                                uint64_t *__restrict__ kout, int *__restrict__ vout, int64_t n,
-                                                               int shift, const int *__restrict__ digit_base, uint32_t *state,
+                                                               int shift, const int *__restrict__ ghist, uint32_t *state,
                                                                int *ticket, int *err)
 {
   __shared__ uint64_t s_key[kSortTile];
   __shared__ int s_val[kSortTile];
   __shared__ int s_cnt[kSortWaves][256];
-  __shared__ int s_dbase[256], s_gbase[256], s_wsum[kSortWaves];
+  __shared__ int s_dbase[256], s_gbase[256], s_wsum[kSortWaves], s_gsum[kSortWaves];
   __shared__ int s_t;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   if(threadIdx.x == 0)
@@ -396,9 +368,14 @@ __global__ __launch_bounds__(kSortBlock) void k_radix_onesweep(const uint64_t *_
   __syncthreads();
 
   // 2. per digit (one per thread): wave offsets, the tile's count published, the
-  //    earlier tiles' counts by look-back, the tile-local start by a scan over digits
+  //    earlier tiles' counts by look-back, the tile-local start and the digit's start in
+  //    the output (exclusive scan of the pass's digit counts) by scans over digits
   {
-    const int d = threadIdx.x;
+    const int d    = threadIdx.x;
+    const int gh   = ghist[d];
+    const int ginc = wave_incl_scan(gh);
+    if(lane == 63)
+      s_gsum[w] = ginc;
     int run     = 0;
 #pragma unroll
     for(int ww = 0; ww < kSortWaves; ww++)
@@ -414,16 +391,19 @@ __global__ __launch_bounds__(kSortBlock) void k_radix_onesweep(const uint64_t *_
         excl = look_back<uint32_t, 8>(state + d, 256, t, err);
       }
     lb_store(&state[(int64_t)t * 256 + d], kFlagIncl32 | ((uint32_t)(excl + run) & kValMask32));
-    s_gbase[d] = digit_base[d] + (int)excl;
     const int incl = wave_incl_scan(run);
     if(lane == 63)
       s_wsum[w] = incl;
     __syncthreads();
-    int wpre = 0;
+    int wpre = 0, gpre = 0;
 #pragma unroll
     for(int ww = 0; ww < kSortWaves; ww++)
-      wpre += ww < w ? s_wsum[ww] : 0;
+      {
+        wpre += ww < w ? s_wsum[ww] : 0;
+        gpre += ww < w ? s_gsum[ww] : 0;
+      }
     s_dbase[d] = wpre + incl - run;
+    s_gbase[d] = gpre + ginc - gh + (int)excl;
   }
   __syncthreads();
 
@@ -453,12 +433,12 @@ __global__ __launch_bounds__(kSortBlock) void k_radix_onesweep(const uint64_t *_
 size_t sort_temp_bytes(int64_t n)
 {
   const int64_t nt = (n + kSortTile - 1) / kSortTile;
-  // per pass: look-back states (nt x 256 words) + ticket; digit partials and bases
-  return (size_t)8 * (nt * 256 + 1) * sizeof(uint32_t) + (size_t)kHistBlocks * 2048 * sizeof(int) + 8 * 256 * sizeof(int) + 1024;
+  // per pass: look-back states (nt x 256 words) + ticket; digit counts of every pass
+  return (size_t)8 * (nt * 256 + 1) * sizeof(uint32_t) + 8 * 256 * sizeof(int) + 1024;
 }
 
 agrav_status radix_sort_u64(uint64_t *keys_in, int *vals_in, uint64_t *keys_out, int *vals_out, int64_t n, int key_bits,
-                            void *d_temp, int *d_err, hipStream_t s, int lo_bit, bool *in_input)
+                            void *d_temp, int *d_err, hipStream_t s, int lo_bit, bool *in_input, int *ghist_zero)
 {
   if(in_input)
     *in_input = false;
@@ -468,12 +448,14 @@ agrav_status radix_sort_u64(uint64_t *keys_in, int *vals_in, uint64_t *keys_out,
   const int passes  = (key_bits - lo_bit + 7) / 8;
   uint32_t *states  = (uint32_t *)d_temp;                     // passes x (nt x 256 + 1)
   const int64_t per = nt * 256 + 1;
-  int *part         = (int *)(states + (size_t)passes * per);  // kHistBlocks x 2048
-  int *dbase        = part + (size_t)kHistBlocks * 2048;       // passes x 256
-  AG_HIP(hipMemsetAsync(states, 0, (size_t)passes * per * sizeof(uint32_t), s));
+  int *ghist        = ghist_zero;                              // passes x 256 digit counts
+  if(!ghist)
+    {
+      ghist = (int *)(states + (size_t)passes * per);
+      AG_HIP(hipMemsetAsync(ghist, 0, (size_t)passes * 256 * sizeof(int), s));
+    }
   const int hb = (int)(nt < kHistBlocks ? nt : kHistBlocks);
-  k_radix_hist_all<<<hb, kSortBlock, 0, s>>>(keys_in, n, passes, lo_bit, part);
-  k_radix_digit_base<<<passes, kSortBlock, 0, s>>>(part, hb, dbase);
+  k_radix_hist_all<<<hb, kSortBlock, 0, s>>>(keys_in, n, passes, lo_bit, ghist, states, (int64_t)passes * per);
   AG_HIP(hipGetLastError());
   uint64_t *ka = keys_in, *kb = keys_out;
   int *va = vals_in, *vb = vals_out;
@@ -481,7 +463,7 @@ agrav_status radix_sort_u64(uint64_t *keys_in, int *vals_in, uint64_t *keys_out,
     {
       uint32_t *st = states + (size_t)p * per;
       int *ticket  = (int *)(st + nt * 256);
-      k_radix_onesweep<<<(unsigned)nt, kSortBlock, 0, s>>>(ka, va, kb, vb, n, lo_bit + 8 * p, dbase + 256 * p, st, ticket, d_err);
+      k_radix_onesweep<<<(unsigned)nt, kSortBlock, 0, s>>>(ka, va, kb, vb, n, lo_bit + 8 * p, ghist + 256 * p, st, ticket, d_err);
       AG_HIP(hipGetLastError());
       uint64_t *tk = ka;
       ka           = kb;

@@ -150,6 +150,8 @@ __global__ __launch_bounds__(256) void k_walk_plain(TreeView tv, WalkConst wc, c
                                                     double *pt, int *nint, unsigned long long *isum)
 {
   __shared__ float s_tab[256];
+  if(tree_invalid(tv))
+    return;
   s_tab[threadIdx.x] = tabs[threadIdx.x];
   __syncthreads();
   int t = blockIdx.x * 256 + threadIdx.x;
@@ -1335,6 +1337,8 @@ __global__ __launch_bounds__(64 * W, (sizeof(T) == 4 && PM && !EWALD && !MSOFT) 
 {
   __shared__ WalkLds<T, MSOFT> lds[W];
   __shared__ int s_fail;
+  if(tree_invalid(tv))
+    return;
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   load_tables<T, PM, sizeof(T) == 4 && PM && !EWALD && !MSOFT>(lds[wv].tab, tabs, wc);
   if(threadIdx.x == 0)
@@ -1345,7 +1349,7 @@ __global__ __launch_bounds__(64 * W, (sizeof(T) == 4 && PM && !EWALD && !MSOFT) 
   unsigned long long cnt[3] = {0, 0, 0};
   GroupAcc<T> acc;
   const bool ok = walk_group<T, PM, EWALD, MSOFT>(lds[wv], tv, wc, hsoft, oldacc_s, ewald_f, ewald_d, tmap, p0, np, tv.n,
-                                                  (T)wc.fix_unit, wc.leaf_bucket, W, wv, acc, cnt);
+                                                  (T)(float)tv.dom->fix_unit, wc.leaf_bucket, W, wv, acc, cnt);
   flush_counts(cnt, isum);
   const int64_t pi = lane < np ? (tmap ? tmap[p0 + lane] : p0 + lane) : 0;
   if(W == 1)
@@ -1484,7 +1488,6 @@ WalkConst walk_const(const agrav_ctx *c, int use_bh)
   wc.ewald     = c->par.periodic && c->par.pmgrid == 0;
   wc.ewald_fac = wc.ewald ? 2.0 * 64 / c->par.box_size : 0;
   wc.multisoft = c->soft.nsoft > 1;
-  wc.fix_unit  = (float)c->h_dom.fix_unit;
   wc.leaf_bucket = c->par.leaf_bucket > 0 ? c->par.leaf_bucket : 0;
   wc.eval_pot    = c->par.eval_potential ? 1 : 0;
   return wc;
@@ -1896,6 +1899,10 @@ static agrav_status walk_pass(agrav_ctx *c, int use_bh, const WalkConst &wc, con
       AG_HIP(hipStreamSynchronize(s));
       memcpy(cnt, c->stage<unsigned long long>(0), sizeof(cnt));
     }
+  // a step's build is checked here, with the walk's read-back (its flags were tested by
+  // the walk kernels on the device)
+  if((st = build_check(c)) != AGRAV_OK)
+    return st;
   const unsigned long long tot = cnt[0];
   c->tim.walk_popped           = (long long)cnt[1];
   c->tim.tile_entries          = (long long)cnt[2];
