@@ -1389,7 +1389,10 @@ agrav_status build_tree(agrav_ctx *c)
   AG_HIP(hipGetLastError());
   // one set: the first words are sorted from bit 15 up (6 passes instead of 8); runs
   // sharing a level-16 cell are finished by the tie-run sorts
-  const int tshift = nsets == 0 ? 15 : 0;
+#ifndef AG_TSHIFT
+#define AG_TSHIFT 15
+#endif
+  const int tshift = nsets == 0 ? AG_TSHIFT : 0;
   bool sorted_in_keys = false;
   if((st = radix_sort_u64(c->d_key.p, c->d_idx.p, c->d_key2.p, c->d_idx2.p, n, 63, c->d_temp.p, d_perr, s, tshift,
                           &sorted_in_keys, c->d_counters.p + kSortHist)) != AGRAV_OK)
