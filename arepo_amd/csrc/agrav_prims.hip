@@ -293,11 +293,24 @@ __global__ __launch_bounds__(kSortBlock) void k_radix_hist_all(const uint64_t *_
   for(int64_t q = (int64_t)blockIdx.x * kSortBlock + threadIdx.x; q < nstates; q += (int64_t)gridDim.x * kSortBlock)
     states[q] = 0;
   __syncthreads();
-  for(int64_t k = (int64_t)blockIdx.x * kSortBlock + threadIdx.x; k < n; k += (int64_t)gridDim.x * kSortBlock)
+  // kHistBatch keys per thread in flight at once (loads from an index clamped to the
+  // last key, whose duplicates are not counted), then their digits counted
+  constexpr int kHistBatch = 8;
+  const int64_t stride     = (int64_t)gridDim.x * kSortBlock;
+  for(int64_t k0 = (int64_t)blockIdx.x * kSortBlock + threadIdx.x; k0 < n; k0 += kHistBatch * stride)
     {
-      const uint64_t key = keys[k];
-      for(int p = 0; p < passes; p++)
-        atomicAdd(&s_h[p][(key >> (lo_bit + 8 * p)) & 255], 1);
+      uint64_t kv[kHistBatch];
+#pragma unroll
+      for(int j = 0; j < kHistBatch; j++)
+        {
+          const int64_t k = k0 + j * stride;
+          kv[j]           = keys[k < n ? k : n - 1];
+        }
+#pragma unroll
+      for(int j = 0; j < kHistBatch; j++)
+        if(k0 + j * stride < n)
+          for(int p = 0; p < passes; p++)
+            atomicAdd(&s_h[p][(kv[j] >> (lo_bit + 8 * p)) & 255], 1);
     }
   __syncthreads();
   for(int q = threadIdx.x; q < passes * 256; q += kSortBlock)
@@ -339,14 +352,25 @@ __global__ __launch_bounds__(kSortBlock) void k_radix_onesweep(const uint64_t *_
   int pos[kSortRounds];
   int dig[kSortRounds];
 
-  // 1. rank inside the wave: 8 ballots give the lanes with the same digit
+  // 1. rank inside the wave: 8 ballots give the lanes with the same digit.  All the
+  //    wave's keys are loaded first, unconditionally from an index clamped to the last
+  //    key (n >= 1), so that every load is in flight before the first ballot (a load
+  //    under the bounds test was issued and waited for round by round)
+#pragma unroll
+  for(int r = 0; r < kSortRounds; r++)
+    {
+      const int64_t k  = wbase + r * 64 + lane;
+      const int64_t kc = k < n ? k : n - 1;
+      key[r]           = kin[kc];
+      val[r]           = vin[kc];
+    }
 #pragma unroll
   for(int r = 0; r < kSortRounds; r++)
     {
       int64_t k   = wbase + r * 64 + lane;
       bool valid  = k < n;
-      key[r]      = valid ? kin[k] : 0;
-      val[r]      = valid ? vin[k] : 0;
+      key[r]      = valid ? key[r] : 0;
+      val[r]      = valid ? val[r] : 0;
       int d       = (int)((key[r] >> shift) & 255);
       dig[r]      = valid ? d : -1;
       uint64_t m  = __ballot(valid);
