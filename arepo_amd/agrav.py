@@ -72,6 +72,7 @@ class Timings(C.Structure):
         ("unbind_solve_ms", C.c_double),
         ("unbind_pass_ms", C.c_double),
         ("device_bytes", C.c_longlong),
+        ("walk_ordered", C.c_int),
     ]
 
     def as_dict(self):

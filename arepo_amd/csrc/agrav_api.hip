@@ -104,6 +104,8 @@ agrav_status finalize(agrav_ctx *c)
       c->tim.finalize = 0;
       c->have_oldacc  = c->have_oldacc || !c->keep_oldacc;
       c->have_final   = true;
+      if(c->n_active < 0 && c->batch_nsets == 0)
+        c->cost_n = c->n;  // every particle's count is in d_nint_out
       return AGRAV_OK;
     }
   FinalOut f;
@@ -123,6 +125,8 @@ agrav_status finalize(agrav_ctx *c)
   c->tim.finalize = ms;
   c->have_oldacc  = c->have_oldacc || !c->keep_oldacc;
   c->have_final   = true;
+  if(c->n_active < 0 && c->batch_nsets == 0)
+    c->cost_n = c->n;  // every particle's count is in d_nint_out
   return AGRAV_OK;
 }
 
@@ -698,6 +702,10 @@ static agrav_status set_particles_common(agrav_ctx *c, int64_t n, const double *
   c->n              = n;
   c->have_stype     = stype != nullptr;
   c->have_particles = true;
+  // the last step's interaction counts stay the walk's cost estimate while the particle
+  // count is unchanged (particle i is the same particle, as in AREPO's P[i].GravCost)
+  if(c->cost_n != n)
+    c->cost_n = -1;
   return AGRAV_OK;
 }
 

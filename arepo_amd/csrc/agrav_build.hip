@@ -510,10 +510,12 @@ __global__ __launch_bounds__(256) void k_tie_sort(uint64_t *keys, const uint64_t
     }
 }
 
+// (cost: every particle's interaction count of the last full step, caller order, or
+// null; gathered into cost_s as the walk's estimate of each group's work)
 __global__ void k_gather(const int *__restrict__ idx, const double *__restrict__ pos, const double *__restrict__ mass,
-                         const uint8_t *__restrict__ stype, const float *__restrict__ oldacc, int64_t n, int64_t nall, int periodic,
-                         double box, BuildGeo geo, int *pset_sorted, double *px, double *py, double *pz, double *pm, uint8_t *ps,
-                         uint4 *prec, float *oldacc_s)
+                         const uint8_t *__restrict__ stype, const float *__restrict__ oldacc, const int *__restrict__ cost, int64_t n,
+                         int64_t nall, int periodic, double box, BuildGeo geo, int *pset_sorted, double *px, double *py, double *pz,
+                         double *pm, uint8_t *ps, uint4 *prec, float *oldacc_s, int *cost_s)
 {
   int64_t k = (int64_t)blockIdx.x * B + threadIdx.x;
   if(k >= n)
@@ -540,6 +542,8 @@ __global__ void k_gather(const int *__restrict__ idx, const double *__restrict__
   prec[k] = make_uint4(to_fix(x, dom, 0), to_fix(y, dom, 1), to_fix(z, dom, 2), __float_as_uint((float)m));
   if(oldacc_s)
     oldacc_s[k] = oldacc ? oldacc[i] : 0.f;
+  if(cost)
+    cost_s[k] = cost[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -1425,10 +1429,15 @@ agrav_status build_tree(agrav_ctx *c)
   const bool multisoft = c->soft.nsoft > 1;
   if(multisoft && (st = c->d_ps.ensure(n)))
     return st;
+  // the last full step's interaction counts order the walk groups of a full tree
+  c->nint_full         = false;
+  c->have_cost_s       = !part && nsets == 0 && c->cost_n == nall && c->d_nint_out.cap >= (size_t)nall;
+  if(c->have_cost_s && (st = c->d_cost_s.ensure(n)) != AGRAV_OK)
+    return st;
   k_gather<<<nblk(n), B, 0, s>>>(c->d_idx2.p, c->d_pos_in.p, c->d_mass_in.p, c->have_stype ? c->d_stype_in.p : nullptr,
-                                 c->have_oldacc ? c->d_oldacc.p : nullptr, n, nall, c->par.periodic, c->par.box_size, geo,
-                                 nsets > 0 ? c->d_pset.p : nullptr, c->d_px.p, c->d_py.p, c->d_pz.p, c->d_pm.p, multisoft ? c->d_ps.p : nullptr, c->d_prec.p,
-                                 c->d_oldacc_s.p);
+                                 c->have_oldacc ? c->d_oldacc.p : nullptr, c->have_cost_s ? c->d_nint_out.p : nullptr, n, nall,
+                                 c->par.periodic, c->par.box_size, geo, nsets > 0 ? c->d_pset.p : nullptr, c->d_px.p, c->d_py.p,
+                                 c->d_pz.p, c->d_pm.p, multisoft ? c->d_ps.p : nullptr, c->d_prec.p, c->d_oldacc_s.p, c->d_cost_s.p);
   AG_HIP(hipGetLastError());
   // a full step's periodic PM needs only the sorted particles: it runs on the PM stream
   // beside the octree kernels below

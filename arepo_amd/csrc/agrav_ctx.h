@@ -231,6 +231,18 @@ struct agrav_ctx
   // walk outputs in sorted order (G=1 units) and per-target counts
   agrav::DBuf<double> d_at, d_pt;
   agrav::DBuf<int> d_nint;
+  // Walk groups are launched in descending order of their estimated cost, the
+  // interaction counts of their targets in the last walk (as AREPO balances its domains
+  // by the last step's GravCost): the costliest groups start first, so that no long
+  // group is left running alone at the end of the launch.
+  agrav::DBuf<int> d_cost_s;              // per-particle cost in sorted order (gathered by the build)
+  agrav::DBuf<uint64_t> d_gkey, d_gkey2;  // group sort keys
+  agrav::DBuf<int> d_gord, d_gord2;       // group indices; sorted: the launch order
+  agrav::DBuf<char> d_gtemp;              // the group sort's look-back states
+  int64_t cost_n   = -1;     // d_nint_out holds every particle's count of a full step with this n
+  bool have_cost_s = false;  // the current tree's build gathered d_cost_s
+  bool nint_full   = false;  // d_nint holds every target's count of a walk on the current tree
+  const int *walk_gorder = nullptr;  // launch order of the current group walk, or null
   agrav::DBuf<int> d_stack;       // group-walk spill/overflow bookkeeping
   agrav::DBuf<int> d_failed;      // groups that overflowed their LDS stack
   agrav::DBuf<int> d_counters;    // [0] group work counter, [1] failed count, [2..] flags

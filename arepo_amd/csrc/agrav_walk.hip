@@ -1353,7 +1353,8 @@ __global__ __launch_bounds__(64 * W, (sizeof(T) == 4 && PM && !EWALD && !MSOFT) 
                                                        const float *__restrict__ oldacc_s, const float *__restrict__ tabs,
                                                        const float *__restrict__ ewald_f, const double *__restrict__ ewald_d, int gs,
                                                        const int *__restrict__ tmap, int ntarg, double *at, double *pt, int *nint,
-                                                       int *failed, int *nfailed, unsigned long long *isum, FinalOut fin)
+                                                       int *failed, int *nfailed, unsigned long long *isum, FinalOut fin,
+                                                       const int *__restrict__ gorder)
 {
   __shared__ WalkLds<T, MSOFT> lds[W];
   __shared__ int s_fail;
@@ -1363,7 +1364,18 @@ __global__ __launch_bounds__(64 * W, (sizeof(T) == 4 && PM && !EWALD && !MSOFT) 
   load_tables<T, PM, sizeof(T) == 4 && PM && !EWALD && !MSOFT>(lds[wv].tab, tabs, wc);
   if(threadIdx.x == 0)
     s_fail = 0;
-  const int g  = xcd_block(blockIdx.x, gridDim.x);
+  // the group: by the launch order (costliest first) when given, else in Hilbert order
+  // with consecutive groups on one XCD.  Blocks go to the 8 XCDs in turn, so within every
+  // full run of 64 blocks the order positions are transposed: each XCD then takes 8
+  // consecutive positions (groups of similar cost, in Hilbert order) and shares their
+  // nodes in its L2.  An order entry out of range (its sort did not complete, which the
+  // host reports) walks nothing.
+  int gpos = blockIdx.x;
+  if(gorder && (int)(gpos | 63) < (int)gridDim.x)
+    gpos = (gpos & ~63) | ((gpos & 7) << 3) | ((gpos >> 3) & 7);
+  const int g = gorder ? gorder[gpos] : xcd_block(blockIdx.x, gridDim.x);
+  if(g < 0 || g >= (int)gridDim.x)
+    return;
   const int p0 = g * gs;
   const int np = min(gs, ntarg - p0);
   unsigned long long cnt[3] = {0, 0, 0};
@@ -1535,7 +1547,8 @@ static void launch_group_w(int ngroups, hipStream_t s, const TreeView &tv, const
   k_walk_group<T, PM, EW, MS, W><<<ngroups, 64 * W, 0, s>>>(tv, wc, c->d_soft.p, c->d_oldacc_s.p, c->d_tab.p, c->d_ewald_f.p,
                                                             c->d_ewald.p, gs, target_map(c), target_count(c), c->d_at.p,
                                                             c->d_pt.p, c->d_nint.p, c->d_failed.p,
-                                                            reinterpret_cast<int *>(c->d_isum.p + 3), c->d_isum.p, c->fin);
+                                                            reinterpret_cast<int *>(c->d_isum.p + 3), c->d_isum.p, c->fin,
+                                                            c->walk_gorder);
 }
 
 template <typename T, bool PM, bool EW, bool MS>
@@ -1740,6 +1753,55 @@ agrav_status active_targets(agrav_ctx *c)
   return AGRAV_OK;
 }
 
+// Sort key of each walk group: its targets' summed cost (interaction counts of the last
+// walk, by sorted position), as an 8-bit code falling with log2 of the sum (1/8 octave
+// per step), so that one stable radix pass orders the groups costliest first and keeps
+// groups of similar cost in Hilbert order.
+__global__ void k_group_keys(int ngroups, int gs, int nt, const int *__restrict__ tmap, const int *__restrict__ cost,
+                             uint64_t *keys, int *ids)
+{
+  const int g = blockIdx.x * 256 + threadIdx.x;
+  if(g >= ngroups)
+    return;
+  const int p0 = g * gs, np = min(gs, nt - p0);
+  long long sum = 0;
+  for(int j = 0; j < np; j++)
+    {
+      const int v = cost[tmap ? tmap[p0 + j] : p0 + j];
+      sum += v > 0 ? v : 0;
+    }
+  const int code = min(255, (int)(8.0f * __log2f(1.0f + (float)sum)));
+  keys[g]        = (uint64_t)(255 - code);
+  ids[g]         = g;
+}
+
+// counter slot of the group sort's look-back error (cleared by every build)
+static constexpr int kGroupSortErr = 62;
+// staging offset of its read-back, beside the walk counters
+static constexpr size_t kWalkFlagStage = 512;
+// fewer groups than this fill the device in one round and gain nothing from an order
+static constexpr int kOrderMinGroups = 8192;
+
+// launch order of the group walk from the cost of each group's targets (sorted
+// positions), or null
+static agrav_status group_order(agrav_ctx *c, const int *cost, int ngroups, int gs, int nt, const int **order)
+{
+  *order = nullptr;
+  agrav_status st;
+  if((st = c->d_gkey.ensure(ngroups)) || (st = c->d_gkey2.ensure(ngroups)) || (st = c->d_gord.ensure(ngroups)) ||
+     (st = c->d_gord2.ensure(ngroups)) || (st = c->d_gtemp.ensure(sort_temp_bytes(ngroups))))
+    return st;
+  hipStream_t s = c->stream;
+  k_group_keys<<<(ngroups + 255) / 256, 256, 0, s>>>(ngroups, gs, nt, target_map(c), cost, c->d_gkey.p, c->d_gord.p);
+  AG_HIP(hipGetLastError());
+  bool in_input = false;
+  if((st = radix_sort_u64(c->d_gkey.p, c->d_gord.p, c->d_gkey2.p, c->d_gord2.p, ngroups, 8, c->d_gtemp.p,
+                          c->d_counters.p + kGroupSortErr, s, 0, &in_input, nullptr)) != AGRAV_OK)
+    return st;
+  *order = in_input ? c->d_gord.p : c->d_gord2.p;
+  return AGRAV_OK;
+}
+
 static agrav_status walk_pass(agrav_ctx *c, int use_bh, const WalkConst &wc, const TreeView &tv);
 
 agrav_status walk_tree(agrav_ctx *c, int use_bh)
@@ -1834,6 +1896,7 @@ static agrav_status walk_pass(agrav_ctx *c, int use_bh, const WalkConst &wc, con
   const bool pm = wc.pm, ew = wc.ewald, ms = wc.multisoft;
   const int64_t nt = target_count(c);
   c->tim.walk_retries = 0;
+  c->tim.walk_ordered = 0;
   unsigned long long cnt[4] = {0, 0, 0, 0};
   bool have_cnt       = false;  // the counters are read already (group walk, no re-walk)
   c->final_fused      = false;
@@ -1887,18 +1950,31 @@ static agrav_status walk_pass(agrav_ctx *c, int use_bh, const WalkConst &wc, con
       // has run, in sorted order); the finalize kernel then has nothing left to do
       if(c->fuse_final && (!pm || (c->have_pm && c->pm_sorted)) && (st = final_out(c, c->fin)) != AGRAV_OK)
         return st;
+      // costliest groups first, by the counts of the last walk on this tree or else of the
+      // last full step (gathered by the build)
+      const int *cost = c->nint_full ? c->d_nint.p : (c->have_cost_s ? c->d_cost_s.p : nullptr);
+      const int *gorder = nullptr;
+      if(cost && ngroups >= kOrderMinGroups && (st = group_order(c, cost, ngroups, gs, (int)nt, &gorder)) != AGRAV_OK)
+        return st;
+      c->walk_gorder      = gorder;
+      c->tim.walk_ordered = gorder != nullptr;
       if(c->par.precision == AGRAV_FP32)
         dispatch_group<float>(pm, ew, ms, ngroups, s, tv, wc, c, gs, waves);
       else
         dispatch_group<double>(pm, ew, ms, ngroups, s, tv, wc, c, gs, 1);
+      c->walk_gorder = nullptr;
       AG_HIP(hipGetLastError());
       AG_HIP(hipEventRecord(c->ev[3], s));
       AG_HIP(hipEventRecord(c->ev[4], s));  // the walk's end, unless groups are re-walked below
       // groups whose LDS stack overflowed (their count is isum[3]) are re-walked one
       // lane per target
       AG_HIP(hipMemcpyAsync(c->stage<unsigned long long>(0), c->d_isum.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
+      if(gorder)
+        AG_HIP(hipMemcpyAsync(c->stage<int>(kWalkFlagStage), c->d_counters.p + kGroupSortErr, sizeof(int), hipMemcpyDeviceToHost, s));
       AG_HIP(hipStreamSynchronize(s));
       memcpy(cnt, c->stage<unsigned long long>(0), sizeof(cnt));
+      if(gorder && *c->stage<int>(kWalkFlagStage))
+        return fail(c, AGRAV_ERR_HIP, "walk group order: sort look-back did not complete");
       const int nf = (int)(unsigned)cnt[3];
       have_cnt     = nf == 0;
       if(nf > 0)
@@ -1933,6 +2009,8 @@ static agrav_status walk_pass(agrav_ctx *c, int use_bh, const WalkConst &wc, con
   c->tim.walk_kernel  = ms_kernel;
   c->tim.interactions = (long long)tot;
   c->have_tree_force  = true;
+  // d_nint now holds every target's count when the walk covered the whole tree
+  c->nint_full = c->batch_nsets == 0 && target_map(c) == nullptr && target_count(c) == c->n_tree;
   return AGRAV_OK;
 }
 
