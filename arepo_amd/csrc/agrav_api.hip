@@ -104,8 +104,6 @@ agrav_status finalize(agrav_ctx *c)
       c->tim.finalize = 0;
       c->have_oldacc  = c->have_oldacc || !c->keep_oldacc;
       c->have_final   = true;
-      if(c->n_active < 0 && c->batch_nsets == 0)
-        c->cost_n = c->n;  // every particle's count is in d_nint_out
       return AGRAV_OK;
     }
   FinalOut f;
@@ -125,8 +123,6 @@ agrav_status finalize(agrav_ctx *c)
   c->tim.finalize = ms;
   c->have_oldacc  = c->have_oldacc || !c->keep_oldacc;
   c->have_final   = true;
-  if(c->n_active < 0 && c->batch_nsets == 0)
-    c->cost_n = c->n;  // every particle's count is in d_nint_out
   return AGRAV_OK;
 }
 
@@ -394,6 +390,11 @@ agrav_status agrav_create(const agrav_params *p, agrav_ctx **out)
         agrav_destroy(c);
         return AGRAV_ERR_HIP;
       }
+  if(hipEventCreateWithFlags(&c->ev_count, hipEventDisableTiming) != hipSuccess)
+    {
+      agrav_destroy(c);
+      return AGRAV_ERR_HIP;
+    }
   // the PM stream of a periodic PMGRID (same default priority as the main stream)
   if(p->pmgrid > 0 && p->periodic &&
      (hipStreamCreateWithFlags(&c->pm_stream, hipStreamNonBlocking) != hipSuccess ||
@@ -488,7 +489,7 @@ void agrav_destroy(agrav_ctx *c)
   for(auto &e : c->ev)
     if(e)
       (void)hipEventDestroy(e);
-  for(hipEvent_t e : {c->ev_fork, c->ev_join})
+  for(hipEvent_t e : {c->ev_fork, c->ev_join, c->ev_count})
     if(e)
       (void)hipEventDestroy(e);
   // DBuf members release their memory explicitly
@@ -702,10 +703,6 @@ static agrav_status set_particles_common(agrav_ctx *c, int64_t n, const double *
   c->n              = n;
   c->have_stype     = stype != nullptr;
   c->have_particles = true;
-  // the last step's interaction counts stay the walk's cost estimate while the particle
-  // count is unchanged (particle i is the same particle, as in AREPO's P[i].GravCost)
-  if(c->cost_n != n)
-    c->cost_n = -1;
   return AGRAV_OK;
 }
 

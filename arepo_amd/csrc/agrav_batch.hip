@@ -198,7 +198,7 @@ agrav_status agrav_batch_set_particles(agrav_ctx *c, int nsets, const int64_t *o
   c->have_particles = false;
   c->have_stype     = false;
   c->have_oldacc    = false;
-  c->cost_n         = -1;
+  c->gcost_groups   = -1;
   c->have_tree = c->have_tree_force = c->have_pm = c->have_final = false;
   agrav_status st;
   if((st = c->d_pos_in.ensure(3 * nn)) || (st = c->d_mass_in.ensure(nn)) || (st = c->d_oldacc.ensure(nn)) ||
@@ -244,7 +244,7 @@ agrav_status agrav_batch_set_particles(agrav_ctx *c, int nsets, const int64_t *o
   c->n                  = n;
   c->have_particles     = true;
   c->have_oldacc        = false;
-  c->cost_n            = -1;
+  c->gcost_groups      = -1;
   c->have_tree = c->have_tree_force = c->have_pm = c->have_final = false;
   return AGRAV_OK;
 }

@@ -510,12 +510,10 @@ __global__ __launch_bounds__(256) void k_tie_sort(uint64_t *keys, const uint64_t
     }
 }
 
-// (cost: every particle's interaction count of the last full step, caller order, or
-// null; gathered into cost_s as the walk's estimate of each group's work)
 __global__ void k_gather(const int *__restrict__ idx, const double *__restrict__ pos, const double *__restrict__ mass,
-                         const uint8_t *__restrict__ stype, const float *__restrict__ oldacc, const int *__restrict__ cost, int64_t n,
-                         int64_t nall, int periodic, double box, BuildGeo geo, int *pset_sorted, double *px, double *py, double *pz,
-                         double *pm, uint8_t *ps, uint4 *prec, float *oldacc_s, int *cost_s)
+                         const uint8_t *__restrict__ stype, const float *__restrict__ oldacc, int64_t n, int64_t nall, int periodic,
+                         double box, BuildGeo geo, int *pset_sorted, double *px, double *py, double *pz, double *pm, uint8_t *ps,
+                         uint4 *prec, float *oldacc_s)
 {
   int64_t k = (int64_t)blockIdx.x * B + threadIdx.x;
   if(k >= n)
@@ -542,8 +540,6 @@ __global__ void k_gather(const int *__restrict__ idx, const double *__restrict__
   prec[k] = make_uint4(to_fix(x, dom, 0), to_fix(y, dom, 1), to_fix(z, dom, 2), __float_as_uint((float)m));
   if(oldacc_s)
     oldacc_s[k] = oldacc ? oldacc[i] : 0.f;
-  if(cost)
-    cost_s[k] = cost[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -1429,15 +1425,10 @@ agrav_status build_tree(agrav_ctx *c)
   const bool multisoft = c->soft.nsoft > 1;
   if(multisoft && (st = c->d_ps.ensure(n)))
     return st;
-  // the last full step's interaction counts order the walk groups of a full tree
-  c->nint_full         = false;
-  c->have_cost_s       = !part && nsets == 0 && c->cost_n == nall && c->d_nint_out.cap >= (size_t)nall;
-  if(c->have_cost_s && (st = c->d_cost_s.ensure(n)) != AGRAV_OK)
-    return st;
   k_gather<<<nblk(n), B, 0, s>>>(c->d_idx2.p, c->d_pos_in.p, c->d_mass_in.p, c->have_stype ? c->d_stype_in.p : nullptr,
-                                 c->have_oldacc ? c->d_oldacc.p : nullptr, c->have_cost_s ? c->d_nint_out.p : nullptr, n, nall,
-                                 c->par.periodic, c->par.box_size, geo, nsets > 0 ? c->d_pset.p : nullptr, c->d_px.p, c->d_py.p,
-                                 c->d_pz.p, c->d_pm.p, multisoft ? c->d_ps.p : nullptr, c->d_prec.p, c->d_oldacc_s.p, c->d_cost_s.p);
+                                 c->have_oldacc ? c->d_oldacc.p : nullptr, n, nall, c->par.periodic, c->par.box_size, geo,
+                                 nsets > 0 ? c->d_pset.p : nullptr, c->d_px.p, c->d_py.p, c->d_pz.p, c->d_pm.p, multisoft ? c->d_ps.p : nullptr, c->d_prec.p,
+                                 c->d_oldacc_s.p);
   AG_HIP(hipGetLastError());
   // a full step's periodic PM needs only the sorted particles: it runs on the PM stream
   // beside the octree kernels below
@@ -1466,65 +1457,10 @@ agrav_status build_tree(agrav_ctx *c)
   static_assert(14 + AG_MAX_LEVELS + 1 <= 48, "cubes per level fit below the walk counters");
   int *hc = c->stage<int>(0);
   AG_HIP(hipMemcpyAsync(hc, d_total, (2 + AG_MAX_LEVELS + 1) * sizeof(int), hipMemcpyDeviceToHost, s));
-  AG_HIP(hipStreamSynchronize(s));
-  for(int L = 0; L <= AG_MAX_LEVELS; L++)
-    c->lvlcnt_h[L] = hc[2 + L];
-  if(hc[1])
-    return fail(c, AGRAV_ERR_HIP, "sort/scan look-back did not complete");
-  const int mt = hc[0];
-  // capacity check in the spirit of TreeAllocFactor (forcetree.c:186-228): the
-  // exact count is known here, so the node arrays are sized to it.  The cubes per
-  // level are counted without overflow (each at most N); their 64-bit sum must
-  // equal the 32-bit scan total, and nodes and particles share one int32 entry
-  // space (entries >= N are nodes).
-  int64_t mt64 = 0;
-  for(int L = 0; L <= AG_MAX_LEVELS; L++)
-    mt64 += c->lvlcnt_h[L];
-  if(mt <= 0 || (int64_t)mt != mt64 || mt64 > 24 * (int64_t)N + 1 || mt64 + (int64_t)N >= ((int64_t)1 << 31))
-    return fail(c, AGRAV_ERR_CAPACITY, "octree node count out of range");
-  const int M = mt;
-  if((st = c->d_nfirst.ensure(M)) || (st = c->d_nlast.ensure(M)) || (st = c->d_nlevel.ensure(M)) || (st = c->d_sx.ensure(M)) ||
-     (st = c->d_sy.ensure(M)) || (st = c->d_sz.ensure(M)) || (st = c->d_nm.ensure(M)) || (st = c->d_cx.ensure(M)) ||
-     (st = c->d_cy.ensure(M)) || (st = c->d_cz.ensure(M)) || (st = c->d_len.ensure(M)) || (st = c->d_nextnode.ensure(M)) ||
-     (st = c->d_sibling.ensure(M)) || (st = c->d_maxsoft.ensure(M)) || (st = c->d_recA.ensure(M)) ||
-     (st = c->d_recB.ensure(M)) || (st = c->d_nrange.ensure(M)) || (st = c->d_cbeg.ensure(M)) || (st = c->d_ccnt.ensure(M)) ||
-     (st = c->d_cnn.ensure(M)) || (st = c->d_child.ensure((size_t)M + n)))
-    return st;
-  k_place<<<nblk(N), B, 0, s>>>(N, c->d_lvlmask.p, c->d_pbase.p, c->d_px.p, c->d_py.p, c->d_pz.p, geo, c->d_nfirst.p,
-                                c->d_nlevel.p, c->d_cx.p, c->d_cy.p, c->d_cz.p, c->d_len.p, c->d_nextnode.p);
-  k_cube_last<<<nblk(M), B, 0, s>>>(M, N, c->d_key2.p, c->d_key_lo_s.p, c->d_nfirst.p, c->d_nlevel.p, c->d_lvlmask.p,
-                                    c->d_pbase.p, geo.sbits, c->d_nlast.p, c->d_nrange.p, c->d_sibling.p);
+  AG_HIP(hipEventRecord(c->ev_count, s));
+  // the kernels that need only the sorted particles run while the host waits for the
+  // count: the particle threading and the prefix sums of the moments
   k_pnext<<<nblk(N), B, 0, s>>>(N, c->d_lvlmask.p, c->d_pbase.p, c->d_pnext.p);
-  if(nsets > 0)
-    {
-      if((st = c->d_sroot.ensure(nsets)) || (st = c->d_send.ensure(nsets)))
-        return st;
-      k_set_roots<<<nblk(nsets), B, 0, s>>>(nsets, N, c->d_soff.p, c->d_lvlmask.p, c->d_pbase.p, c->d_sroot.p, c->d_send.p);
-    }
-  AG_HIP(hipGetLastError());
-
-  // 5. child lists, then moments level by level (deepest first); runs of small
-  //    levels share one single-workgroup launch
-  if(scan_temp_bytes(M) + 4096 > c->d_temp.cap && (st = c->d_temp.ensure(scan_temp_bytes(M) + 4096)) != AGRAV_OK)
-    return st;
-  int *d_berr = c->d_counters.p + 50;  // cleared at the start of the build
-  if((st = c->d_npos.ensure(M)))
-    return st;
-  k_count_children<<<nblk(M), B, 0, s>>>(M, N, c->d_nextnode.p, c->d_sibling.p, c->d_pnext.p, c->d_ccnt.p, c->d_npos.p, N + M,
-                                         d_berr);
-  AG_HIP(hipGetLastError());
-  if((st = exclusive_scan_i32(c->d_ccnt.p, c->d_cbeg.p, M, nullptr, c->d_temp.p, d_perr, s)) != AGRAV_OK)
-    return st;
-  if((st = c->d_meta.ensure(M)) || (st = c->d_cprec.ensure((size_t)M + n)))
-    return st;
-  k_fill_children<<<nblk(M), B, 0, s>>>(M, N, c->d_nextnode.p, c->d_sibling.p, c->d_pnext.p, c->d_cbeg.p, c->d_ccnt.p,
-                                         c->d_nrange.p, c->d_prec.p, c->d_child.p, c->d_cnn.p, c->d_meta.p, c->d_cprec.p,
-                                         c->d_npos.p);
-  int mtot = 0;
-  for(int L = 0; L <= AG_MAX_LEVELS; L++)
-    mtot += c->lvlcnt_h[L];
-  if(mtot != M)
-    return fail(c, AGRAV_ERR_CAPACITY, "inconsistent octree level counts");
   // moments of all cubes from prefix sums over the particles, restarted at every set
   MomTiles mtl{nullptr, nullptr, N};
   int ntiles = (N + kMomTile - 1) / kMomTile;
@@ -1566,6 +1502,64 @@ agrav_status build_tree(agrav_ctx *c)
   if(ntiles > 0)
     k_mom_prefix<<<ntiles, B, 0, s>>>(mtl, c->d_px.p, c->d_py.p, c->d_pz.p, c->d_pm.p, tsum, c->d_mom_hi.p, c->d_mom_lo.p);
   AG_HIP(hipGetLastError());
+  AG_HIP(hipEventSynchronize(c->ev_count));
+  for(int L = 0; L <= AG_MAX_LEVELS; L++)
+    c->lvlcnt_h[L] = hc[2 + L];
+  if(hc[1])
+    return fail(c, AGRAV_ERR_HIP, "sort/scan look-back did not complete");
+  const int mt = hc[0];
+  // capacity check in the spirit of TreeAllocFactor (forcetree.c:186-228): the
+  // exact count is known here, so the node arrays are sized to it.  The cubes per
+  // level are counted without overflow (each at most N); their 64-bit sum must
+  // equal the 32-bit scan total, and nodes and particles share one int32 entry
+  // space (entries >= N are nodes).
+  int64_t mt64 = 0;
+  for(int L = 0; L <= AG_MAX_LEVELS; L++)
+    mt64 += c->lvlcnt_h[L];
+  if(mt <= 0 || (int64_t)mt != mt64 || mt64 > 24 * (int64_t)N + 1 || mt64 + (int64_t)N >= ((int64_t)1 << 31))
+    return fail(c, AGRAV_ERR_CAPACITY, "octree node count out of range");
+  const int M = mt;
+  if((st = c->d_nfirst.ensure(M)) || (st = c->d_nlast.ensure(M)) || (st = c->d_nlevel.ensure(M)) || (st = c->d_sx.ensure(M)) ||
+     (st = c->d_sy.ensure(M)) || (st = c->d_sz.ensure(M)) || (st = c->d_nm.ensure(M)) || (st = c->d_cx.ensure(M)) ||
+     (st = c->d_cy.ensure(M)) || (st = c->d_cz.ensure(M)) || (st = c->d_len.ensure(M)) || (st = c->d_nextnode.ensure(M)) ||
+     (st = c->d_sibling.ensure(M)) || (st = c->d_maxsoft.ensure(M)) || (st = c->d_recA.ensure(M)) ||
+     (st = c->d_recB.ensure(M)) || (st = c->d_nrange.ensure(M)) || (st = c->d_cbeg.ensure(M)) || (st = c->d_ccnt.ensure(M)) ||
+     (st = c->d_cnn.ensure(M)) || (st = c->d_child.ensure((size_t)M + n)))
+    return st;
+  k_place<<<nblk(N), B, 0, s>>>(N, c->d_lvlmask.p, c->d_pbase.p, c->d_px.p, c->d_py.p, c->d_pz.p, geo, c->d_nfirst.p,
+                                c->d_nlevel.p, c->d_cx.p, c->d_cy.p, c->d_cz.p, c->d_len.p, c->d_nextnode.p);
+  k_cube_last<<<nblk(M), B, 0, s>>>(M, N, c->d_key2.p, c->d_key_lo_s.p, c->d_nfirst.p, c->d_nlevel.p, c->d_lvlmask.p,
+                                    c->d_pbase.p, geo.sbits, c->d_nlast.p, c->d_nrange.p, c->d_sibling.p);
+  if(nsets > 0)
+    {
+      if((st = c->d_sroot.ensure(nsets)) || (st = c->d_send.ensure(nsets)))
+        return st;
+      k_set_roots<<<nblk(nsets), B, 0, s>>>(nsets, N, c->d_soff.p, c->d_lvlmask.p, c->d_pbase.p, c->d_sroot.p, c->d_send.p);
+    }
+  AG_HIP(hipGetLastError());
+
+  // 5. child lists, then moments level by level (deepest first); runs of small
+  //    levels share one single-workgroup launch
+  if(scan_temp_bytes(M) + 4096 > c->d_temp.cap && (st = c->d_temp.ensure(scan_temp_bytes(M) + 4096)) != AGRAV_OK)
+    return st;
+  int *d_berr = c->d_counters.p + 50;  // cleared at the start of the build
+  if((st = c->d_npos.ensure(M)))
+    return st;
+  k_count_children<<<nblk(M), B, 0, s>>>(M, N, c->d_nextnode.p, c->d_sibling.p, c->d_pnext.p, c->d_ccnt.p, c->d_npos.p, N + M,
+                                         d_berr);
+  AG_HIP(hipGetLastError());
+  if((st = exclusive_scan_i32(c->d_ccnt.p, c->d_cbeg.p, M, nullptr, c->d_temp.p, d_perr, s)) != AGRAV_OK)
+    return st;
+  if((st = c->d_meta.ensure(M)) || (st = c->d_cprec.ensure((size_t)M + n)))
+    return st;
+  k_fill_children<<<nblk(M), B, 0, s>>>(M, N, c->d_nextnode.p, c->d_sibling.p, c->d_pnext.p, c->d_cbeg.p, c->d_ccnt.p,
+                                         c->d_nrange.p, c->d_prec.p, c->d_child.p, c->d_cnn.p, c->d_meta.p, c->d_cprec.p,
+                                         c->d_npos.p);
+  int mtot = 0;
+  for(int L = 0; L <= AG_MAX_LEVELS; L++)
+    mtot += c->lvlcnt_h[L];
+  if(mtot != M)
+    return fail(c, AGRAV_ERR_CAPACITY, "inconsistent octree level counts");
   int *tcnt = nullptr;
   if(multisoft)
     {

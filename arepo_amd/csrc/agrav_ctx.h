@@ -231,18 +231,20 @@ struct agrav_ctx
   // walk outputs in sorted order (G=1 units) and per-target counts
   agrav::DBuf<double> d_at, d_pt;
   agrav::DBuf<int> d_nint;
-  // Walk groups are launched in descending order of their estimated cost, the
-  // interaction counts of their targets in the last walk (as AREPO balances its domains
-  // by the last step's GravCost): the costliest groups start first, so that no long
-  // group is left running alone at the end of the launch.
-  agrav::DBuf<int> d_cost_s;              // per-particle cost in sorted order (gathered by the build)
+  // Walk groups are launched in descending order of their estimated cost, the tile
+  // entries each group streamed in the last walk of the same grouping (as AREPO balances
+  // its domains by the last step's GravCost): the costliest groups start first, so that
+  // no long group is left running alone at the end of the launch.  Group g of the next
+  // walk holds about the same particles while n and the group size are unchanged.
+  agrav::DBuf<int> d_gcost;               // per group: tile entries of the last walk
   agrav::DBuf<uint64_t> d_gkey, d_gkey2;  // group sort keys
   agrav::DBuf<int> d_gord, d_gord2;       // group indices; sorted: the launch order
-  agrav::DBuf<char> d_gtemp;              // the group sort's look-back states
-  int64_t cost_n   = -1;     // d_nint_out holds every particle's count of a full step with this n
-  bool have_cost_s = false;  // the current tree's build gathered d_cost_s
-  bool nint_full   = false;  // d_nint holds every target's count of a walk on the current tree
+  agrav::DBuf<char> d_gtemp;              // the group sort's look-back states and digit counts
+  int gcost_groups = -1;     // d_gcost holds the costs of a walk of this many groups (-1: none) ...
+  int gcost_gs     = 0;      // ... of this group size ...
+  int64_t gcost_n  = -1;     // ... over every one of this many particles
   const int *walk_gorder = nullptr;  // launch order of the current group walk, or null
+  int *walk_gcost        = nullptr;  // where the current group walk records its groups' costs, or null
   agrav::DBuf<int> d_stack;       // group-walk spill/overflow bookkeeping
   agrav::DBuf<int> d_failed;      // groups that overflowed their LDS stack
   agrav::DBuf<int> d_counters;    // [0] group work counter, [1] failed count, [2..] flags
@@ -266,6 +268,7 @@ struct agrav_ctx
   // once the gather has written the Hilbert-ordered particles, joined before the walk
   hipStream_t pm_stream = nullptr;  // created with a periodic PMGRID (default priority)
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipEvent_t ev_count = nullptr;  // the build's node-count read-back (main stream)
   bool pm_fork   = false;  // the next build forks the PM (set by compute_grav_accelerations)
   bool pm_forked = false;  // the build forked it; the main stream has not joined it yet
   // a full step reads the build's error flags back with the walk's counters (one

@@ -580,7 +580,7 @@ static agrav_status unbind_potential(agrav_ctx *c, int ng, int64_t n, double g_a
   c->n                  = m;
   c->have_particles     = true;
   c->have_oldacc        = false;
-  c->cost_n            = -1;
+  c->gcost_groups      = -1;
   c->have_tree = c->have_tree_force = c->have_pm = c->have_final = false;
   if(m == 0)
     return AGRAV_OK;

@@ -1354,7 +1354,7 @@ __global__ __launch_bounds__(64 * W, (sizeof(T) == 4 && PM && !EWALD && !MSOFT) 
                                                        const float *__restrict__ ewald_f, const double *__restrict__ ewald_d, int gs,
                                                        const int *__restrict__ tmap, int ntarg, double *at, double *pt, int *nint,
                                                        int *failed, int *nfailed, unsigned long long *isum, FinalOut fin,
-                                                       const int *__restrict__ gorder)
+                                                       const int *__restrict__ gorder, int *gcost)
 {
   __shared__ WalkLds<T, MSOFT> lds[W];
   __shared__ int s_fail;
@@ -1383,6 +1383,10 @@ __global__ __launch_bounds__(64 * W, (sizeof(T) == 4 && PM && !EWALD && !MSOFT) 
   const bool ok = walk_group<T, PM, EWALD, MSOFT>(lds[wv], tv, wc, hsoft, oldacc_s, ewald_f, ewald_d, tmap, p0, np, tv.n,
                                                   (T)(float)tv.dom->fix_unit, wc.leaf_bucket, W, wv, acc, cnt);
   flush_counts(cnt, isum);
+  // the group's cost for the launch order of the next walk: its tile entries (wave 0's
+  // share when the group is split over wavefronts)
+  if(gcost && threadIdx.x == 0)
+    gcost[g] = (int)min(cnt[2], (unsigned long long)INT_MAX);
   const int64_t pi = lane < np ? (tmap ? tmap[p0 + lane] : p0 + lane) : 0;
   if(W == 1)
     {
@@ -1548,7 +1552,7 @@ static void launch_group_w(int ngroups, hipStream_t s, const TreeView &tv, const
                                                             c->d_ewald.p, gs, target_map(c), target_count(c), c->d_at.p,
                                                             c->d_pt.p, c->d_nint.p, c->d_failed.p,
                                                             reinterpret_cast<int *>(c->d_isum.p + 3), c->d_isum.p, c->fin,
-                                                            c->walk_gorder);
+                                                            c->walk_gorder, c->walk_gcost);
 }
 
 template <typename T, bool PM, bool EW, bool MS>
@@ -1753,24 +1757,19 @@ agrav_status active_targets(agrav_ctx *c)
   return AGRAV_OK;
 }
 
-// Sort key of each walk group: its targets' summed cost (interaction counts of the last
-// walk, by sorted position), as an 8-bit code falling with log2 of the sum (1/8 octave
-// per step), so that one stable radix pass orders the groups costliest first and keeps
-// groups of similar cost in Hilbert order.
-__global__ void k_group_keys(int ngroups, int gs, int nt, const int *__restrict__ tmap, const int *__restrict__ cost,
-                             uint64_t *keys, int *ids)
+// Sort key of each walk group: its cost (tile entries of the last walk of the same
+// grouping) as an 8-bit code falling with log2 of the cost (1/8 octave per step), so
+// that one stable radix pass orders the groups costliest first and keeps groups of
+// similar cost in Hilbert order.  Block 0 also clears the pass's digit counts.
+__global__ void k_group_keys(int ngroups, const int *__restrict__ gcost, uint64_t *keys, int *ids, int *ghist)
 {
+  if(blockIdx.x == 0)
+    ghist[threadIdx.x] = 0;  // 256 threads, one digit each
   const int g = blockIdx.x * 256 + threadIdx.x;
   if(g >= ngroups)
     return;
-  const int p0 = g * gs, np = min(gs, nt - p0);
-  long long sum = 0;
-  for(int j = 0; j < np; j++)
-    {
-      const int v = cost[tmap ? tmap[p0 + j] : p0 + j];
-      sum += v > 0 ? v : 0;
-    }
-  const int code = min(255, (int)(8.0f * __log2f(1.0f + (float)sum)));
+  const int v    = gcost[g];
+  const int code = min(255, (int)(8.0f * __log2f(1.0f + (float)(v > 0 ? v : 0))));
   keys[g]        = (uint64_t)(255 - code);
   ids[g]         = g;
 }
@@ -1782,21 +1781,22 @@ static constexpr size_t kWalkFlagStage = 512;
 // fewer groups than this fill the device in one round and gain nothing from an order
 static constexpr int kOrderMinGroups = 8192;
 
-// launch order of the group walk from the cost of each group's targets (sorted
-// positions), or null
-static agrav_status group_order(agrav_ctx *c, const int *cost, int ngroups, int gs, int nt, const int **order)
+// launch order of the group walk from the costs in d_gcost, or null
+static agrav_status group_order(agrav_ctx *c, int ngroups, const int **order)
 {
   *order = nullptr;
   agrav_status st;
+  const size_t tb = sort_temp_bytes(ngroups);  // look-back states, then 256 digit counts
   if((st = c->d_gkey.ensure(ngroups)) || (st = c->d_gkey2.ensure(ngroups)) || (st = c->d_gord.ensure(ngroups)) ||
-     (st = c->d_gord2.ensure(ngroups)) || (st = c->d_gtemp.ensure(sort_temp_bytes(ngroups))))
+     (st = c->d_gord2.ensure(ngroups)) || (st = c->d_gtemp.ensure(tb + 256 * sizeof(int))))
     return st;
   hipStream_t s = c->stream;
-  k_group_keys<<<(ngroups + 255) / 256, 256, 0, s>>>(ngroups, gs, nt, target_map(c), cost, c->d_gkey.p, c->d_gord.p);
+  int *ghist    = reinterpret_cast<int *>(c->d_gtemp.p + tb);
+  k_group_keys<<<(ngroups + 255) / 256, 256, 0, s>>>(ngroups, c->d_gcost.p, c->d_gkey.p, c->d_gord.p, ghist);
   AG_HIP(hipGetLastError());
   bool in_input = false;
   if((st = radix_sort_u64(c->d_gkey.p, c->d_gord.p, c->d_gkey2.p, c->d_gord2.p, ngroups, 8, c->d_gtemp.p,
-                          c->d_counters.p + kGroupSortErr, s, 0, &in_input, nullptr)) != AGRAV_OK)
+                          c->d_counters.p + kGroupSortErr, s, 0, &in_input, ghist)) != AGRAV_OK)
     return st;
   *order = in_input ? c->d_gord.p : c->d_gord2.p;
   return AGRAV_OK;
@@ -1950,20 +1950,33 @@ static agrav_status walk_pass(agrav_ctx *c, int use_bh, const WalkConst &wc, con
       // has run, in sorted order); the finalize kernel then has nothing left to do
       if(c->fuse_final && (!pm || (c->have_pm && c->pm_sorted)) && (st = final_out(c, c->fin)) != AGRAV_OK)
         return st;
-      // costliest groups first, by the counts of the last walk on this tree or else of the
-      // last full step (gathered by the build)
-      const int *cost = c->nint_full ? c->d_nint.p : (c->have_cost_s ? c->d_cost_s.p : nullptr);
+      // costliest groups first, by the costs of the last walk of the same grouping: every
+      // target of as many particles, in groups of the same size; this walk's costs order
+      // the next one
+      const bool whole  = c->batch_nsets == 0 && target_map(c) == nullptr && nt == c->n_tree;
       const int *gorder = nullptr;
-      if(cost && ngroups >= kOrderMinGroups && (st = group_order(c, cost, ngroups, gs, (int)nt, &gorder)) != AGRAV_OK)
+      if(whole && ngroups >= kOrderMinGroups && c->gcost_groups == ngroups && c->gcost_gs == gs && c->gcost_n == c->n &&
+         (st = group_order(c, ngroups, &gorder)) != AGRAV_OK)
         return st;
+      if(whole && (st = c->d_gcost.ensure(ngroups)) != AGRAV_OK)
+        return st;
+      c->gcost_groups     = -1;  // valid again once this walk has run
       c->walk_gorder      = gorder;
+      c->walk_gcost       = whole ? c->d_gcost.p : nullptr;
       c->tim.walk_ordered = gorder != nullptr;
       if(c->par.precision == AGRAV_FP32)
         dispatch_group<float>(pm, ew, ms, ngroups, s, tv, wc, c, gs, waves);
       else
         dispatch_group<double>(pm, ew, ms, ngroups, s, tv, wc, c, gs, 1);
       c->walk_gorder = nullptr;
+      c->walk_gcost  = nullptr;
       AG_HIP(hipGetLastError());
+      if(whole)
+        {
+          c->gcost_groups = ngroups;
+          c->gcost_gs     = gs;
+          c->gcost_n      = c->n;
+        }
       AG_HIP(hipEventRecord(c->ev[3], s));
       AG_HIP(hipEventRecord(c->ev[4], s));  // the walk's end, unless groups are re-walked below
       // groups whose LDS stack overflowed (their count is isum[3]) are re-walked one
@@ -2009,8 +2022,6 @@ static agrav_status walk_pass(agrav_ctx *c, int use_bh, const WalkConst &wc, con
   c->tim.walk_kernel  = ms_kernel;
   c->tim.interactions = (long long)tot;
   c->have_tree_force  = true;
-  // d_nint now holds every target's count when the walk covered the whole tree
-  c->nint_full = c->batch_nsets == 0 && target_map(c) == nullptr && target_count(c) == c->n_tree;
   return AGRAV_OK;
 }
 

@@ -108,9 +108,9 @@ typedef struct
   double unbind_solve_ms; /* agrav_subfind_unbind: device time of those solves */
   double unbind_pass_ms;  /* agrav_subfind_unbind: device time of the per-group pass kernels */
   long long device_bytes; /* HBM held by the context's device buffers now (they are kept between calls) */
-  int walk_ordered;       /* group walk: 1 when its groups were launched costliest first, by the interaction counts of
-                             the last walk on the same tree or of the last step of as many particles (GravCost-like);
-                             the order changes no result */
+  int walk_ordered;       /* group walk: 1 when its groups were launched costliest first, by the tile entries each
+                             group streamed in the last walk of the same grouping (every target of as many particles,
+                             same group size; like GravCost); the order changes no result */
 } agrav_timings;
 
 typedef struct agrav_ctx agrav_ctx;

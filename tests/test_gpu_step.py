@@ -101,11 +101,11 @@ def test_step_timings_cover_build_pm_and_walk(agrav):
 
 @pytest.mark.parametrize("pmgrid", [0, 64])
 def test_costliest_first_walk_equals_hilbert_order_walk(agrav, pmgrid):
-    """A step whose walk groups start costliest first (by the interaction counts of the
-    last step of as many particles, AREPO's GravCost idea) gives bitwise the accelerations,
-    potentials and counts of the same step launched in Hilbert order: each group's sums
-    do not depend on when it runs.  600,000 particles make 9,375 groups of 64, enough for
-    the order to be used."""
+    """A step whose walk groups start costliest first (by the tile entries each group
+    streamed in the last walk of the same grouping, AREPO's GravCost idea) gives bitwise
+    the accelerations, potentials and counts of the same step launched in Hilbert order:
+    each group's sums do not depend on when it runs.  600,000 particles make 9,375 groups
+    of 64, enough for the order to be used."""
     n = 600000
     pos, mass = _box(n, 5)
     g0 = np.random.Generator(np.random.Philox(9))
