@@ -312,8 +312,8 @@ __global__ void k_keys(const double *__restrict__ pos, const int *__restrict__ s
 }
 
 // Tie runs.  The radix sort orders the first key words from bit tshift up (tshift = 0:
-// all of them; tshift = 15 for one set, whose second words have their low 36 bits
-// zero).  A run of keys equal in those bits is then ordered by a composite secondary
+// all of them; tshift = AG_TSHIFT for one set, whose second words have their low 36
+// bits zero).  A run of keys equal in those bits is then ordered by a composite secondary
 // key: the first word's low tshift bits on top of the second word shifted down by
 // tshift, which loses nothing; after the run is sorted both words are written back.
 __device__ inline uint64_t tie_sec(uint64_t k1, uint64_t klo, int tshift)
@@ -1387,11 +1387,15 @@ agrav_status build_tree(agrav_ctx *c)
   k_keys<<<nblk(n), B, 0, s>>>(c->d_pos_in.p, part ? c->d_active.p : nullptr, n, geo, c->par.periodic, c->par.box_size,
                                c->d_key.p, c->d_key_lo.p, c->d_idx.p, c->d_dom.p);
   AG_HIP(hipGetLastError());
-  // one set: the first words are sorted from bit 15 up (6 passes instead of 8); runs
-  // sharing a level-16 cell are finished by the tie-run sorts
+  // one set: the first words are sorted from bit 23 up (5 passes instead of 8); runs
+  // sharing a level-13 cell are finished by the tie-run sorts (from bit 15: 6 passes,
+  // 30 us slower at 2M clustered particles; from bit 31 the runs in the densest cores
+  // grow long enough to cost more than the pass saved)
 #ifndef AG_TSHIFT
-#define AG_TSHIFT 15
+#define AG_TSHIFT 23
 #endif
+  static_assert(AG_TSHIFT <= 36, "the composite tie key holds the first word's low tshift bits in the second "
+                                 "word's zero low bits");
   const int tshift = nsets == 0 ? AG_TSHIFT : 0;
   bool sorted_in_keys = false;
   if((st = radix_sort_u64(c->d_key.p, c->d_idx.p, c->d_key2.p, c->d_idx2.p, n, 63, c->d_temp.p, d_perr, s, tshift,
