@@ -1917,7 +1917,6 @@ static agrav_status walk_pass(agrav_ctx *c, int use_bh, const WalkConst &wc, con
     }
   else
     {
-      AG_HIP(hipEventRecord(c->ev[2], s));
       // by workload when not set (MI355X timings of FP32 walks): periodic sets and large
       // isolated sets fill the device with groups of 64; small isolated sets are bound
       // by the latency of each group's walk and gain from groups of 16 (200k Hernquist
@@ -1961,6 +1960,8 @@ static agrav_status walk_pass(agrav_ctx *c, int use_bh, const WalkConst &wc, con
       if(whole && (st = c->d_gcost.ensure(ngroups)) != AGRAV_OK)
         return st;
       c->gcost_groups     = -1;  // valid again once this walk has run
+      // the walk's time starts here: the order kernels above are not part of it
+      AG_HIP(hipEventRecord(c->ev[2], s));
       c->walk_gorder      = gorder;
       c->walk_gcost       = whole ? c->d_gcost.p : nullptr;
       c->tim.walk_ordered = gorder != nullptr;
