@@ -589,34 +589,44 @@ __global__ __launch_bounds__(B) void k_lvlmask(const uint64_t *__restrict__ keys
       const int a = kdelta(keys, klo, n, f, f - 1), b = kdelta(keys, klo, n, f, f + 1);  // -1 beyond the ends
       const int lo = a < sbits ? 0 : (a - sbits) / 3 + 1;
       const int hi = b < sbits ? -1 : min((b - sbits) / 3, levels);
-      for(int L = lo; L <= hi; L++)
-        m |= 1u << L;
+      // levels lo..hi (0 <= lo <= hi <= 30 when any)
+      m          = lo <= hi ? (((2u << hi) - 1u) & ~((1u << lo) - 1u)) : 0u;
       lvlmask[f] = m;
       pcnt[f]    = __popc(m);
     }
-  for(int L = 0; L <= levels; L++)
+  // per level, the wavefront's count: only the levels present in the wavefront
+  unsigned u = m;
+#pragma unroll
+  for(int o = 32; o > 0; o >>= 1)
+    u |= (unsigned)__shfl_xor((int)u, o, 64);
+  u = (unsigned)__builtin_amdgcn_readfirstlane((int)u);
+  while(u)
     {
+      const int L = __ffs(u) - 1;
+      u &= u - 1;
       const int c = __popcll(__ballot((m >> L) & 1u));
-      if(c && (threadIdx.x & 63) == 0)
+      if((threadIdx.x & 63) == 0)
         atomicAdd(&s_cnt[L], c);
     }
   __syncthreads();
+  // level-major rows: k_level_totals reads each level's counts contiguously
   if(threadIdx.x <= AG_MAX_LEVELS)
-    lvlpart[(int64_t)blockIdx.x * (AG_MAX_LEVELS + 1) + threadIdx.x] = s_cnt[threadIdx.x];
+    lvlpart[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] = s_cnt[threadIdx.x];
 }
 
-// cubes per level: block L sums column L of the partial counts
+// cubes per level: block L sums row L of the partial counts (one per block of k_lvlmask)
 __global__ __launch_bounds__(B) void k_level_totals(const int *__restrict__ lvlpart, int nrows, int *lvlcnt)
 {
   __shared__ int s_w[B / 64];
-  const int L = blockIdx.x;
-  int c       = 0;
-  for(int r0 = threadIdx.x * 8; r0 < nrows; r0 += B * 8)
+  const int L    = blockIdx.x;
+  const int *row = lvlpart + (int64_t)L * nrows;
+  int c          = 0;
+  for(int r0 = threadIdx.x; r0 < nrows; r0 += B * 8)
     {
       int v[8];
 #pragma unroll
       for(int u = 0; u < 8; u++)
-        v[u] = r0 + u < nrows ? lvlpart[(int64_t)(r0 + u) * (AG_MAX_LEVELS + 1) + L] : 0;
+        v[u] = r0 + u * B < nrows ? row[r0 + u * B] : 0;
 #pragma unroll
       for(int u = 0; u < 8; u++)
         c += v[u];
@@ -1444,7 +1454,7 @@ agrav_status build_tree(agrav_ctx *c)
     return st;
   if(N > 1)
     {
-      int *lvlpart = c->d_bcnt.p;  // nblk(N) rows of AG_MAX_LEVELS + 1 partial counts
+      int *lvlpart = c->d_bcnt.p;  // AG_MAX_LEVELS + 1 rows of nblk(N) partial counts
       if((st = c->d_bcnt.ensure((size_t)nblk(N) * (AG_MAX_LEVELS + 1))) != AGRAV_OK)
         return st;
       lvlpart = c->d_bcnt.p;
