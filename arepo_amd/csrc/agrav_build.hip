@@ -761,48 +761,46 @@ __global__ void k_pnext(int n, const unsigned *__restrict__ lvlmask, const int *
 // node and its particle children in one memory round trip each.
 __global__ void k_fill_children(int m, int n, const int *__restrict__ nextnode, const int *__restrict__ sibling,
                                 const int *__restrict__ pnext, const int *__restrict__ cbeg, const int *__restrict__ ccnt,
-                                const int2 *__restrict__ nrange, const uint4 *__restrict__ prec, int *child, int *cnn,
-                                int4 *meta, uint4 *cprec, int *npos)
+                                const int2 *__restrict__ nrange, const uint4 *__restrict__ prec, int *child,
+                                const int *__restrict__ cnn, int4 *meta, uint4 *cprec, int *npos)
 {
   int j = blockIdx.x * B + threadIdx.x;
   if(j >= m)
     return;
-  const int end = sibling[j], stop = cbeg[j] + ccnt[j];
-  int o = cbeg[j], steps = 0;
+  // one walk of the threading: node children from the list's start, particle children
+  // after the nn node children (counted by k_count_children), each in threading order
+  const int end = sibling[j], beg = cbeg[j], stop = beg + ccnt[j], nn = cnn[j];
+  int on = beg, op = beg + nn, steps = 0;
   for(int c = nextnode[j]; c != end && c >= 0 && steps < ccnt[j]; steps++)
     {
-      if(c >= n && o < stop)
+      if(c >= n)
         {
-          npos[c - n] = o;
-          child[o++]  = c;
+          if(on < beg + nn)
+            {
+              npos[c - n] = on;
+              child[on++] = c;
+            }
         }
-      c = (c < n) ? pnext[c] : sibling[c - n];
-    }
-  const int nn = o - cbeg[j];
-  cnn[j]       = nn;
-  steps        = 0;
-  for(int c = nextnode[j]; c != end && c >= 0 && steps < ccnt[j]; steps++)
-    {
-      if(c < n && o < stop)
+      else if(op < stop)
         {
-          cprec[o]   = prec[c];
-          child[o++] = c;
+          cprec[op]   = prec[c];
+          child[op++] = c;
         }
       c = (c < n) ? pnext[c] : sibling[c - n];
     }
   const int2 rg = nrange[j];
-  meta[j]       = make_int4(cbeg[j] + nn, o - cbeg[j] - nn, rg.x, min(nn, 15) | (min(rg.y, 255) << 4));
+  meta[j]       = make_int4(beg + nn, op - beg - nn, rg.x, min(nn, 15) | (min(rg.y, 255) << 4));
 }
 
 // children of each node, by walking its threading from nextnode to sibling
 __global__ void k_count_children(int m, int n, const int *__restrict__ nextnode, const int *__restrict__ sibling,
-                                 const int *__restrict__ pnext, int *ccnt, int *npos, int limit, int *err)
+                                 const int *__restrict__ pnext, int *ccnt, int *cnn, int *npos, int limit, int *err)
 {
   int j = blockIdx.x * B + threadIdx.x;
   if(j >= m)
     return;
   npos[j] = -1;  // set for every node with a parent by k_fill_children
-  int k = 0;
+  int k = 0, kn = 0;
   for(int c = nextnode[j], end = sibling[j]; c != end; k++)
     {
       if(k >= limit || c < 0)
@@ -810,9 +808,11 @@ __global__ void k_count_children(int m, int n, const int *__restrict__ nextnode,
           atomicExch(err, 1);  // broken threading: stop instead of looping
           break;
         }
+      kn += c >= n;
       c = (c < n) ? pnext[c] : sibling[c - n];
     }
   ccnt[j] = k;
+  cnn[j]  = kn;  // node children, listed first
 }
 
 // Moments (forcetree.c:1217-1429) from prefix sums.  Every cube holds a
@@ -1559,8 +1559,8 @@ agrav_status build_tree(agrav_ctx *c)
   int *d_berr = c->d_counters.p + 50;  // cleared at the start of the build
   if((st = c->d_npos.ensure(M)))
     return st;
-  k_count_children<<<nblk(M), B, 0, s>>>(M, N, c->d_nextnode.p, c->d_sibling.p, c->d_pnext.p, c->d_ccnt.p, c->d_npos.p, N + M,
-                                         d_berr);
+  k_count_children<<<nblk(M), B, 0, s>>>(M, N, c->d_nextnode.p, c->d_sibling.p, c->d_pnext.p, c->d_ccnt.p, c->d_cnn.p,
+                                         c->d_npos.p, N + M, d_berr);
   AG_HIP(hipGetLastError());
   if((st = exclusive_scan_i32(c->d_ccnt.p, c->d_cbeg.p, M, nullptr, c->d_temp.p, d_perr, s)) != AGRAV_OK)
     return st;

@@ -1774,10 +1774,10 @@ __global__ void k_group_keys(int ngroups, const int *__restrict__ gcost, uint64_
   ids[g]         = g;
 }
 
-// counter slot of the group sort's look-back error (cleared by every build)
-static constexpr int kGroupSortErr = 62;
-// staging offset of its read-back, beside the walk counters
-static constexpr size_t kWalkFlagStage = 512;
+// The group sort's look-back error flag is the upper half of the walk counters' last
+// word (the lower half counts the failed groups; the walk clears both), so the walk's
+// one read-back of its counters carries it.
+static int *group_sort_err(agrav_ctx *c) { return reinterpret_cast<int *>(c->d_isum.p + 3) + 1; }
 // fewer groups than this fill the device in one round and gain nothing from an order
 static constexpr int kOrderMinGroups = 8192;
 
@@ -1796,7 +1796,7 @@ static agrav_status group_order(agrav_ctx *c, int ngroups, const int **order)
   AG_HIP(hipGetLastError());
   bool in_input = false;
   if((st = radix_sort_u64(c->d_gkey.p, c->d_gord.p, c->d_gkey2.p, c->d_gord2.p, ngroups, 8, c->d_gtemp.p,
-                          c->d_counters.p + kGroupSortErr, s, 0, &in_input, ghist)) != AGRAV_OK)
+                          group_sort_err(c), s, 0, &in_input, ghist)) != AGRAV_OK)
     return st;
   *order = in_input ? c->d_gord.p : c->d_gord2.p;
   return AGRAV_OK;
@@ -1983,11 +1983,9 @@ static agrav_status walk_pass(agrav_ctx *c, int use_bh, const WalkConst &wc, con
       // groups whose LDS stack overflowed (their count is isum[3]) are re-walked one
       // lane per target
       AG_HIP(hipMemcpyAsync(c->stage<unsigned long long>(0), c->d_isum.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
-      if(gorder)
-        AG_HIP(hipMemcpyAsync(c->stage<int>(kWalkFlagStage), c->d_counters.p + kGroupSortErr, sizeof(int), hipMemcpyDeviceToHost, s));
       AG_HIP(hipStreamSynchronize(s));
       memcpy(cnt, c->stage<unsigned long long>(0), sizeof(cnt));
-      if(gorder && *c->stage<int>(kWalkFlagStage))
+      if(gorder && (cnt[3] >> 32) != 0)
         return fail(c, AGRAV_ERR_HIP, "walk group order: sort look-back did not complete");
       const int nf = (int)(unsigned)cnt[3];
       have_cnt     = nf == 0;
