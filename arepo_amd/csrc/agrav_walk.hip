@@ -1383,9 +1383,9 @@ __global__ __launch_bounds__(64 * W, (sizeof(T) == 4 && PM && !EWALD && !MSOFT) 
   const bool ok = walk_group<T, PM, EWALD, MSOFT>(lds[wv], tv, wc, hsoft, oldacc_s, ewald_f, ewald_d, tmap, p0, np, tv.n,
                                                   (T)(float)tv.dom->fix_unit, wc.leaf_bucket, W, wv, acc, cnt);
   flush_counts(cnt, isum);
-  // the group's cost for the launch order of the next walk: its tile entries (wave 0's
-  // share when the group is split over wavefronts)
-  if(gcost && threadIdx.x == 0)
+  // the group's cost for the launch order of the next walk: its tile entries (summed
+  // over the wavefronts of a split group below)
+  if(W == 1 && gcost && threadIdx.x == 0)
     gcost[g] = (int)min(cnt[2], (unsigned long long)INT_MAX);
   const int64_t pi = lane < np ? (tmap ? tmap[p0 + lane] : p0 + lane) : 0;
   if(W == 1)
@@ -1418,7 +1418,16 @@ __global__ __launch_bounds__(64 * W, (sizeof(T) == 4 && PM && !EWALD && !MSOFT) 
   red[128 + lane]  = (double)acc.az - (double)acc.caz;
   red[192 + lane]  = (double)acc.pp - (double)acc.cpp;
   reinterpret_cast<int *>(red + 256)[lane] = acc.nin;
+  if(lane == 0)
+    reinterpret_cast<unsigned long long *>(red + 288)[0] = cnt[2];  // after the 64 counts; < 4 KB
   __syncthreads();
+  if(wv == 0 && gcost && lane == 0)
+    {
+      unsigned long long te = 0;
+      for(int v = 0; v < W; v++)
+        te += reinterpret_cast<const unsigned long long *>(reinterpret_cast<const double *>(lds[v].stack) + 288)[0];
+      gcost[g] = (int)min(te, (unsigned long long)INT_MAX);
+    }
   if(wv == 0)
     {
       if(s_fail)
@@ -1778,8 +1787,9 @@ __global__ void k_group_keys(int ngroups, const int *__restrict__ gcost, uint64_
 // word (the lower half counts the failed groups; the walk clears both), so the walk's
 // one read-back of its counters carries it.
 static int *group_sort_err(agrav_ctx *c) { return reinterpret_cast<int *>(c->d_isum.p + 3) + 1; }
-// fewer groups than this fill the device in one round and gain nothing from an order
-static constexpr int kOrderMinGroups = 8192;
+// fewer wavefronts than this (groups x wavefronts per group) fill the device in about
+// one round and gain nothing from an order
+static constexpr int kOrderMinWaves = 8192;
 
 // launch order of the group walk from the costs in d_gcost, or null
 static agrav_status group_order(agrav_ctx *c, int ngroups, const int **order)
@@ -1954,7 +1964,9 @@ static agrav_status walk_pass(agrav_ctx *c, int use_bh, const WalkConst &wc, con
       // the next one
       const bool whole  = c->batch_nsets == 0 && target_map(c) == nullptr && nt == c->n_tree;
       const int *gorder = nullptr;
-      if(whole && ngroups >= kOrderMinGroups && c->gcost_groups == ngroups && c->gcost_gs == gs && c->gcost_n == c->n &&
+      const int order_waves = c->par.precision == AGRAV_FP32 ? waves : 1;
+      if(whole && (int64_t)ngroups * order_waves >= kOrderMinWaves && c->gcost_groups == ngroups && c->gcost_gs == gs &&
+         c->gcost_n == c->n &&
          (st = group_order(c, ngroups, &gorder)) != AGRAV_OK)
         return st;
       if(whole && (st = c->d_gcost.ensure(ngroups)) != AGRAV_OK)

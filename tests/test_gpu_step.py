@@ -99,20 +99,20 @@ def test_step_timings_cover_build_pm_and_walk(agrav):
     assert rel_err(a, b).max() == 0.0
 
 
-@pytest.mark.parametrize("pmgrid", [0, 64])
-def test_costliest_first_walk_equals_hilbert_order_walk(agrav, pmgrid):
+@pytest.mark.parametrize("pmgrid,n,gs,waves", [(0, 600000, 64, 1), (64, 600000, 64, 1), (0, 50000, 16, 4)])
+def test_costliest_first_walk_equals_hilbert_order_walk(agrav, pmgrid, n, gs, waves):
     """A step whose walk groups start costliest first (by the tile entries each group
     streamed in the last walk of the same grouping, AREPO's GravCost idea) gives bitwise
     the accelerations, potentials and counts of the same step launched in Hilbert order:
     each group's sums do not depend on when it runs.  600,000 particles make 9,375 groups
-    of 64, enough for the order to be used."""
-    n = 600000
+    of 64; 50,000 particles in groups of 16 shared by 4 wavefronts make 3,125 groups of
+    12,500 wavefronts; both are enough for the order to be used."""
     pos, mass = _box(n, 5)
     g0 = np.random.Generator(np.random.Philox(9))
     old = (0.5 + g0.random(n)).astype(np.float32)
     kw = dict(periodic=True, box_size=1.0, pmgrid=pmgrid) if pmgrid else dict(periodic=False, box_size=0.0, pmgrid=0)
-    p = agrav.make_params(**kw, soft=(0.002,), precision=agrav.FP32, walk=agrav.WALK_GROUP, group_size=64,
-                          eval_potential=True)
+    p = agrav.make_params(**kw, soft=(0.002,), precision=agrav.FP32, walk=agrav.WALK_GROUP, group_size=gs,
+                          group_waves=waves, eval_potential=True)
     runs = []
     with agrav.Gravity(p) as g:
         g.set_particles(pos, mass)
