@@ -996,19 +996,40 @@ __global__ __launch_bounds__(B) void k_mom_tile_sums(MomTiles mt, const double *
 __global__ __launch_bounds__(B) void k_mom_set_scan(MomTiles mt, int ntiles, const int64_t *__restrict__ soff, DD4 *tsum,
                                                     DD4 *settot, double4 *phi, float4 *plo)
 {
-  __shared__ DD4 s_buf[B];
+  __shared__ DD4 s_wave[B / 64];
   const int set = blockIdx.x;
   const int t0 = mt.tbeg ? mt.tbeg[set] : 0, t1 = mt.tbeg ? mt.tbeg[set + 1] : ntiles;
   const int nt = t1 - t0, per = (nt + B - 1) / B, b0 = t0 + threadIdx.x * per;
-  DD4 acc      = dd4_zero();
-  for(int k = b0; k < b0 + per && k < t1; k++)
-    acc = dd4_add(acc, tsum[k]);
-  DD4 run = block_excl_scan_dd4(acc, s_buf);
-  for(int k = b0; k < b0 + per && k < t1; k++)
+  const int e0 = b0 + per < t1 ? b0 + per : t1;  // this thread's tiles [b0, e0)
+  // runs of up to 8 tile sums, all read before the first is added (one memory round
+  // trip per run instead of one per tile)
+  constexpr int kRun = 8;
+  DD4 acc = dd4_zero();
+  for(int k0 = b0; k0 < e0; k0 += kRun)
     {
-      const DD4 v = tsum[k];
-      tsum[k]     = run;
-      run         = dd4_add(run, v);
+      DD4 v[kRun];
+#pragma unroll
+      for(int u = 0; u < kRun; u++)
+        v[u] = k0 + u < e0 ? tsum[k0 + u] : dd4_zero();
+#pragma unroll
+      for(int u = 0; u < kRun; u++)
+        if(k0 + u < e0)
+          acc = dd4_add(acc, v[u]);
+    }
+  DD4 run = block_excl_scan_dd4_waves(acc, s_wave);
+  for(int k0 = b0; k0 < e0; k0 += kRun)
+    {
+      DD4 v[kRun];
+#pragma unroll
+      for(int u = 0; u < kRun; u++)
+        v[u] = k0 + u < e0 ? tsum[k0 + u] : dd4_zero();
+#pragma unroll
+      for(int u = 0; u < kRun; u++)
+        if(k0 + u < e0)
+          {
+            tsum[k0 + u] = run;
+            run          = dd4_add(run, v[u]);
+          }
     }
   if(threadIdx.x == B - 1 && settot)
     settot[set] = run;  // the last thread's running value has passed every tile
