@@ -354,6 +354,9 @@ agrav_status final_out(agrav_ctx *c, FinalOut &f);
 WalkConst walk_const(const agrav_ctx *c, int use_bh);
 BatchView batch_view(const agrav_ctx *c);
 agrav_status batch_groups(agrav_ctx *c);
-agrav_status walk_batch_groups(agrav_ctx *c, int use_bh, const int4 *groups, int ngroups, float *kernel_ms);
+// fin: the final outputs the walk's epilogue writes (perm null: none); *fused is set
+// when every target's outputs were written so (no group re-walked)
+agrav_status walk_batch_groups(agrav_ctx *c, int use_bh, const int4 *groups, int ngroups, float *kernel_ms,
+                               const FinalOut &fin = FinalOut{}, bool *fused = nullptr);
 TreeView tree_view(const agrav_ctx *c);
 }  // namespace agrav

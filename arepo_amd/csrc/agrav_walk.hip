@@ -1465,10 +1465,11 @@ __global__ __launch_bounds__(64 * W, (sizeof(T) == 4 && PM && !EWALD && !MSOFT) 
 // until it is empty; every wave leaves when the counter passes the group count.
 // (Per-XCD queues over contiguous parts of the list measured 3 % slower: the
 // sets are small enough to stay in L2 either way, and the split adds a tail.)
+// Within a full solve the epilogue also writes the final outputs (fin.perm set).
 template <typename T>
 __global__ __launch_bounds__(64) void k_walk_batch(TreeView tv, WalkConst wc, BatchView bv, const float *__restrict__ oldacc_s,
                                                    const float *__restrict__ tabs, int ngroups, int *queue, double *at, double *pt,
-                                                   int *nint, int *failed, int *nfailed, unsigned long long *isum)
+                                                   int *nint, int *failed, int *nfailed, unsigned long long *isum, FinalOut fin)
 {
   __shared__ WalkLds<T, false> lds;
   load_tables<T, false, false>(lds.tab, tabs, wc);
@@ -1490,11 +1491,15 @@ __global__ __launch_bounds__(64) void k_walk_batch(TreeView tv, WalkConst wc, Ba
           if((int)threadIdx.x < d.y)
             {
               const int64_t pi = (int64_t)d.x + threadIdx.x;
-              at[3 * pi + 0]   = (double)acc.ax - (double)acc.cax;
-              at[3 * pi + 1]   = (double)acc.ay - (double)acc.cay;
-              at[3 * pi + 2]   = (double)acc.az - (double)acc.caz;
-              pt[pi]           = (double)acc.pp - (double)acc.cpp;
-              nint[pi]         = acc.nin;
+              const double a0 = (double)acc.ax - (double)acc.cax, a1 = (double)acc.ay - (double)acc.cay,
+                           a2 = (double)acc.az - (double)acc.caz, p = (double)acc.pp - (double)acc.cpp;
+              at[3 * pi + 0] = a0;
+              at[3 * pi + 1] = a1;
+              at[3 * pi + 2] = a2;
+              pt[pi]         = p;
+              nint[pi]       = acc.nin;
+              if(fin.perm)
+                finalize_one(fin, pi, a0, a1, a2, p, acc.nin);
             }
         }
       else if(threadIdx.x == 0)
@@ -1652,7 +1657,8 @@ template <typename T> static int batch_grid(agrav_ctx *c)
 // One batched group walk over the groups [0, ngroups) of `groups` (all sets of the
 // batch, or a sample of them while tuning); failed groups are re-walked one lane
 // per target.  Interaction totals accumulate in d_isum.
-agrav_status walk_batch_groups(agrav_ctx *c, int use_bh, const int4 *groups, int ngroups, float *kernel_ms)
+agrav_status walk_batch_groups(agrav_ctx *c, int use_bh, const int4 *groups, int ngroups, float *kernel_ms, const FinalOut &fin,
+                               bool *fused)
 {
   hipStream_t s      = c->stream;
   agrav_status st;
@@ -1675,11 +1681,11 @@ agrav_status walk_batch_groups(agrav_ctx *c, int use_bh, const int4 *groups, int
     {
       if(c->par.precision == AGRAV_FP32)
         k_walk_batch<float><<<batch_grid<float>(c), 64, 0, s>>>(tv, wc, bv, c->d_oldacc_s.p, c->d_tab.p, ngroups, queue, c->d_at.p,
-                                                                 c->d_pt.p, c->d_nint.p, c->d_failed.p, nfailed, c->d_isum.p);
+                                                                 c->d_pt.p, c->d_nint.p, c->d_failed.p, nfailed, c->d_isum.p, fin);
       else
         k_walk_batch<double><<<batch_grid<double>(c), 64, 0, s>>>(tv, wc, bv, c->d_oldacc_s.p, c->d_tab.p, ngroups, queue,
                                                                    c->d_at.p, c->d_pt.p, c->d_nint.p, c->d_failed.p, nfailed,
-                                                                   c->d_isum.p);
+                                                                   c->d_isum.p, fin);
       AG_HIP(hipGetLastError());
     }
   AG_HIP(hipEventRecord(c->ev[3], s));
@@ -1692,6 +1698,9 @@ agrav_status walk_batch_groups(agrav_ctx *c, int use_bh, const int4 *groups, int
       launch_plain(false, false, nf * 64, c->d_failed.p, 64, s, tv, wc, c, bv);
       AG_HIP(hipGetLastError());
     }
+  // re-walked targets still need the finalize kernel
+  if(fused)
+    *fused = fin.perm != nullptr && nf == 0;
   if(kernel_ms)
     AG_HIP(hipEventElapsedTime(kernel_ms, c->ev[2], c->ev[3]));
   return AGRAV_OK;
@@ -1913,10 +1922,14 @@ static agrav_status walk_pass(agrav_ctx *c, int use_bh, const WalkConst &wc, con
   c->fin.perm         = nullptr;
   if(c->batch_nsets > 0 && c->par.walk != AGRAV_WALK_PLAIN)
     {
-      // batch of sets: one persistent launch over the group queue
-      if((st = batch_groups(c)) != AGRAV_OK ||
-         (st = walk_batch_groups(c, use_bh, c->d_groups.p, c->batch_ngroups, nullptr)) != AGRAV_OK)
+      // batch of sets: one persistent launch over the group queue; within a full solve
+      // its epilogue writes the final outputs too
+      FinalOut fin{};
+      bool fused = false;
+      if((st = batch_groups(c)) != AGRAV_OK || (c->fuse_final && (st = final_out(c, fin)) != AGRAV_OK) ||
+         (st = walk_batch_groups(c, use_bh, c->d_groups.p, c->batch_ngroups, nullptr, fin, &fused)) != AGRAV_OK)
         return st;
+      c->final_fused = fused;
     }
   else if(c->par.walk == AGRAV_WALK_PLAIN)
     {
