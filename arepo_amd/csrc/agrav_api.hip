@@ -390,7 +390,10 @@ agrav_status agrav_create(const agrav_params *p, agrav_ctx **out)
         agrav_destroy(c);
         return AGRAV_ERR_HIP;
       }
-  if(hipEventCreateWithFlags(&c->ev_count, hipEventDisableTiming) != hipSuccess)
+  if(hipEventCreateWithFlags(&c->ev_count, hipEventDisableTiming) != hipSuccess ||
+     hipStreamCreateWithFlags(&c->mom_stream, hipStreamNonBlocking) != hipSuccess ||
+     hipEventCreateWithFlags(&c->ev_mfork, hipEventDisableTiming) != hipSuccess ||
+     hipEventCreateWithFlags(&c->ev_mjoin, hipEventDisableTiming) != hipSuccess)
     {
       agrav_destroy(c);
       return AGRAV_ERR_HIP;
@@ -482,6 +485,8 @@ void agrav_destroy(agrav_ctx *c)
     (void)hipStreamSynchronize(c->stream);
   if(c->pm_stream)
     (void)hipStreamSynchronize(c->pm_stream);
+  if(c->mom_stream)
+    (void)hipStreamSynchronize(c->mom_stream);
   if(c->hstage)
     (void)hipHostFree(c->hstage);
   pm_free(c);
@@ -489,7 +494,7 @@ void agrav_destroy(agrav_ctx *c)
   for(auto &e : c->ev)
     if(e)
       (void)hipEventDestroy(e);
-  for(hipEvent_t e : {c->ev_fork, c->ev_join, c->ev_count})
+  for(hipEvent_t e : {c->ev_fork, c->ev_join, c->ev_count, c->ev_mfork, c->ev_mjoin})
     if(e)
       (void)hipEventDestroy(e);
   // DBuf members release their memory explicitly
@@ -588,6 +593,8 @@ void agrav_destroy(agrav_ctx *c)
   c->unb.release();
   if(c->pm_stream)
     (void)hipStreamDestroy(c->pm_stream);
+  if(c->mom_stream)
+    (void)hipStreamDestroy(c->mom_stream);
   if(c->stream)
     (void)hipStreamDestroy(c->stream);
   delete c;

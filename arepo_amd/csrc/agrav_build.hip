@@ -1374,6 +1374,12 @@ agrav_status build_tree(agrav_ctx *c)
   if((st = c->d_counters.ensure(kSortHist + 2048)) != AGRAV_OK)
     return st;
   int *d_total = c->d_counters.p + 12, *d_perr = c->d_counters.p + 13, *d_lvlcnt = c->d_counters.p + 14;
+  // prefix sums of a build that failed after forking them still read its particles
+  if(c->mom_forked)
+    {
+      AG_HIP(hipStreamWaitEvent(s, c->ev_mjoin, 0));
+      c->mom_forked = false;
+    }
   // one small kernel sets the extent's start values and clears every other build counter
   unsigned long long *d_mm = (unsigned long long *)c->d_counters.p;  // 6 x u64 = 12 ints
   k_counters_init<<<1, 256, 0, s>>>(c->d_counters.p);
@@ -1470,6 +1476,62 @@ agrav_status build_tree(agrav_ctx *c)
   if(c->pm_fork && !part && nsets == 0 && (st = pm_fork(c)) != AGRAV_OK)
     return st;
 
+  // moments of all cubes from prefix sums over the particles, restarted at every set
+  MomTiles mtl{nullptr, nullptr, N};
+  int ntiles = (N + kMomTile - 1) / kMomTile;
+  const int nseg = nsets > 0 ? nsets : 1;
+  DD4 *stp        = nullptr;  // prefix sums at the set starts (batches)
+  auto moment_prefix = [&](hipStream_t ms) -> agrav_status {
+    if(nsets > 0)
+      {
+        std::vector<int4> tab;
+        std::vector<int> tbeg(nsets + 1);
+        for(int q = 0; q < nsets; q++)
+          {
+            tbeg[q]         = (int)tab.size();
+            const int64_t f = c->h_soff[q], e = c->h_soff[q + 1];
+            for(int64_t st0 = f; st0 < e; st0 += kMomTile)
+              tab.push_back(make_int4((int)st0, (int)std::min<int64_t>(kMomTile, e - st0), q, st0 + kMomTile >= e));
+          }
+        tbeg[nsets] = (int)tab.size();
+        ntiles      = (int)tab.size();
+        if((st = c->d_mom_tab.ensure(tab.size() ? tab.size() : 1)) || (st = c->d_mom_tbeg.ensure(nsets + 1)))
+          return st;
+        if(!tab.empty())
+          AG_HIP(hipMemcpyAsync(c->d_mom_tab.p, tab.data(), tab.size() * sizeof(int4), hipMemcpyHostToDevice, ms));
+        AG_HIP(hipMemcpyAsync(c->d_mom_tbeg.p, tbeg.data(), (nsets + 1) * sizeof(int), hipMemcpyHostToDevice, ms));
+        AG_HIP(hipStreamSynchronize(ms));  // the host vectors go out of scope
+        mtl.tab  = c->d_mom_tab.p;
+        mtl.tbeg = c->d_mom_tbeg.p;
+      }
+    const size_t ntile_bytes = (size_t)(ntiles ? ntiles : 1) * sizeof(DD4), nset_bytes = (size_t)(nseg + 1) * sizeof(DD4);
+    if((st = c->d_mom_tiles.ensure(ntile_bytes + 2 * nset_bytes)) || (st = c->d_mom_hi.ensure((size_t)N + nseg)) ||
+       (st = c->d_mom_lo.ensure((size_t)N + nseg)))
+      return st;
+    DD4 *tsum   = reinterpret_cast<DD4 *>(c->d_mom_tiles.p);
+    DD4 *settot = reinterpret_cast<DD4 *>(c->d_mom_tiles.p + ntile_bytes);
+    stp         = reinterpret_cast<DD4 *>(c->d_mom_tiles.p + ntile_bytes + nset_bytes);
+    if(ntiles > 0)
+      k_mom_tile_sums<<<ntiles, B, 0, ms>>>(mtl, c->d_px.p, c->d_py.p, c->d_pz.p, c->d_pm.p, tsum);
+    k_mom_set_scan<<<nseg, B, 0, ms>>>(mtl, ntiles, nsets > 0 ? c->d_soff.p : nullptr, tsum, settot, c->d_mom_hi.p, c->d_mom_lo.p);
+    if(nsets > 0)
+      k_mom_settot_scan<<<1, B, 0, ms>>>(nsets, settot, stp);
+    if(ntiles > 0)
+      k_mom_prefix<<<ntiles, B, 0, ms>>>(mtl, c->d_px.p, c->d_py.p, c->d_pz.p, c->d_pm.p, tsum, c->d_mom_hi.p, c->d_mom_lo.p);
+    AG_HIP(hipGetLastError());
+    return AGRAV_OK;
+  };
+  const bool fork_moments = nsets == 0 && c->mom_stream != nullptr;
+  if(fork_moments)
+    {
+      AG_HIP(hipEventRecord(c->ev_mfork, s));
+      AG_HIP(hipStreamWaitEvent(c->mom_stream, c->ev_mfork, 0));
+      if((st = moment_prefix(c->mom_stream)) != AGRAV_OK)
+        return st;
+      AG_HIP(hipEventRecord(c->ev_mjoin, c->mom_stream));
+      c->mom_forked = true;
+    }
+
   // 3./4. octree cubes straight from the sorted keys
   if((st = c->d_lvlmask.ensure(n)) || (st = c->d_pcnt.ensure(n)) || (st = c->d_pbase.ensure(n)))
     return st;
@@ -1494,49 +1556,10 @@ agrav_status build_tree(agrav_ctx *c)
   AG_HIP(hipMemcpyAsync(hc, d_total, (2 + AG_MAX_LEVELS + 1) * sizeof(int), hipMemcpyDeviceToHost, s));
   AG_HIP(hipEventRecord(c->ev_count, s));
   // the kernels that need only the sorted particles run while the host waits for the
-  // count: the particle threading and the prefix sums of the moments
+  // count: the particle threading (and the prefix sums of the moments, unless forked)
   k_pnext<<<nblk(N), B, 0, s>>>(N, c->d_lvlmask.p, c->d_pbase.p, c->d_pnext.p);
-  // moments of all cubes from prefix sums over the particles, restarted at every set
-  MomTiles mtl{nullptr, nullptr, N};
-  int ntiles = (N + kMomTile - 1) / kMomTile;
-  const int nseg = nsets > 0 ? nsets : 1;
-  if(nsets > 0)
-    {
-      std::vector<int4> tab;
-      std::vector<int> tbeg(nsets + 1);
-      for(int q = 0; q < nsets; q++)
-        {
-          tbeg[q]         = (int)tab.size();
-          const int64_t f = c->h_soff[q], e = c->h_soff[q + 1];
-          for(int64_t st0 = f; st0 < e; st0 += kMomTile)
-            tab.push_back(make_int4((int)st0, (int)std::min<int64_t>(kMomTile, e - st0), q, st0 + kMomTile >= e));
-        }
-      tbeg[nsets] = (int)tab.size();
-      ntiles      = (int)tab.size();
-      if((st = c->d_mom_tab.ensure(tab.size() ? tab.size() : 1)) || (st = c->d_mom_tbeg.ensure(nsets + 1)))
-        return st;
-      if(!tab.empty())
-        AG_HIP(hipMemcpyAsync(c->d_mom_tab.p, tab.data(), tab.size() * sizeof(int4), hipMemcpyHostToDevice, s));
-      AG_HIP(hipMemcpyAsync(c->d_mom_tbeg.p, tbeg.data(), (nsets + 1) * sizeof(int), hipMemcpyHostToDevice, s));
-      AG_HIP(hipStreamSynchronize(s));  // the host vectors go out of scope
-      mtl.tab  = c->d_mom_tab.p;
-      mtl.tbeg = c->d_mom_tbeg.p;
-    }
-  const size_t ntile_bytes = (size_t)(ntiles ? ntiles : 1) * sizeof(DD4), nset_bytes = (size_t)(nseg + 1) * sizeof(DD4);
-  if((st = c->d_mom_tiles.ensure(ntile_bytes + 2 * nset_bytes)) || (st = c->d_mom_hi.ensure((size_t)N + nseg)) ||
-     (st = c->d_mom_lo.ensure((size_t)N + nseg)))
+  if(!fork_moments && (st = moment_prefix(s)) != AGRAV_OK)
     return st;
-  DD4 *tsum   = reinterpret_cast<DD4 *>(c->d_mom_tiles.p);
-  DD4 *settot = reinterpret_cast<DD4 *>(c->d_mom_tiles.p + ntile_bytes);
-  DD4 *stp    = reinterpret_cast<DD4 *>(c->d_mom_tiles.p + ntile_bytes + nset_bytes);
-  if(ntiles > 0)
-    k_mom_tile_sums<<<ntiles, B, 0, s>>>(mtl, c->d_px.p, c->d_py.p, c->d_pz.p, c->d_pm.p, tsum);
-  k_mom_set_scan<<<nseg, B, 0, s>>>(mtl, ntiles, nsets > 0 ? c->d_soff.p : nullptr, tsum, settot, c->d_mom_hi.p, c->d_mom_lo.p);
-  if(nsets > 0)
-    k_mom_settot_scan<<<1, B, 0, s>>>(nsets, settot, stp);
-  if(ntiles > 0)
-    k_mom_prefix<<<ntiles, B, 0, s>>>(mtl, c->d_px.p, c->d_py.p, c->d_pz.p, c->d_pm.p, tsum, c->d_mom_hi.p, c->d_mom_lo.p);
-  AG_HIP(hipGetLastError());
   AG_HIP(hipEventSynchronize(c->ev_count));
   for(int L = 0; L <= AG_MAX_LEVELS; L++)
     c->lvlcnt_h[L] = hc[2 + L];
@@ -1614,6 +1637,11 @@ agrav_status build_tree(agrav_ctx *c)
   if((st = c->d_nmeta.ensure((size_t)M + n)) || (st = c->d_nrecA.ensure((size_t)M + n)) ||
      (st = c->d_nrecB.ensure((size_t)M + n)))
     return st;
+  if(c->mom_forked)
+    {
+      AG_HIP(hipStreamWaitEvent(s, c->ev_mjoin, 0));
+      c->mom_forked = false;
+    }
   k_moments_prefix<<<nblk(M), B, 0, s>>>(M, c->d_nrange.p, c->d_mom_hi.p, c->d_mom_lo.p, stp, tcnt, N + 1, c->soft.nsoft, c->d_soft.p,
                                          c->d_cx.p, c->d_cy.p, c->d_cz.p, c->d_len.p, c->d_sx.p, c->d_sy.p, c->d_sz.p, c->d_nm.p,
                                          c->d_maxsoft.p, c->d_recA.p, c->d_recB.p, geo.dom, geo.pset, c->d_npos.p, c->d_meta.p,

@@ -269,6 +269,11 @@ struct agrav_ctx
   hipStream_t pm_stream = nullptr;  // created with a periodic PMGRID (default priority)
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   hipEvent_t ev_count = nullptr;  // the build's node-count read-back (main stream)
+  // a one-set build runs the prefix sums of the moments on a stream of their own beside
+  // the octree kernels (they need only the sorted particles); joined before the moments
+  hipStream_t mom_stream = nullptr;  // default priority
+  hipEvent_t ev_mfork = nullptr, ev_mjoin = nullptr;
+  bool mom_forked = false;  // ev_mjoin marks the end of the last forked prefix sums
   bool pm_fork   = false;  // the next build forks the PM (set by compute_grav_accelerations)
   bool pm_forked = false;  // the build forked it; the main stream has not joined it yet
   // a full step reads the build's error flags back with the walk's counters (one
