@@ -393,7 +393,8 @@ agrav_status agrav_create(const agrav_params *p, agrav_ctx **out)
   if(hipEventCreateWithFlags(&c->ev_count, hipEventDisableTiming) != hipSuccess ||
      hipStreamCreateWithFlags(&c->mom_stream, hipStreamNonBlocking) != hipSuccess ||
      hipEventCreateWithFlags(&c->ev_mfork, hipEventDisableTiming) != hipSuccess ||
-     hipEventCreateWithFlags(&c->ev_mjoin, hipEventDisableTiming) != hipSuccess)
+     hipEventCreateWithFlags(&c->ev_mjoin, hipEventDisableTiming) != hipSuccess ||
+     hipEventCreateWithFlags(&c->ev_order, hipEventDisableTiming) != hipSuccess)
     {
       agrav_destroy(c);
       return AGRAV_ERR_HIP;
@@ -494,7 +495,7 @@ void agrav_destroy(agrav_ctx *c)
   for(auto &e : c->ev)
     if(e)
       (void)hipEventDestroy(e);
-  for(hipEvent_t e : {c->ev_fork, c->ev_join, c->ev_count, c->ev_mfork, c->ev_mjoin})
+  for(hipEvent_t e : {c->ev_fork, c->ev_join, c->ev_count, c->ev_mfork, c->ev_mjoin, c->ev_order})
     if(e)
       (void)hipEventDestroy(e);
   // DBuf members release their memory explicitly
@@ -1017,6 +1018,14 @@ agrav_status agrav_compute_grav_accelerations(agrav_ctx *c, int first_step)
     st = finalize(c);
   c->fuse_final  = false;
   c->final_fused = false;
+  if(c->pre_ngroups >= 0)
+    {
+      // an order sorted for a walk that did not run (a failed build): later work on the
+      // main stream waits for its sort
+      (void)hipStreamWaitEvent(c->stream, c->ev_order, 0);
+      c->pre_ngroups = -1;
+    }
+  c->isum_clear = false;
   if(c->build_check_pending)
     {
       // normally read with the walk's counters; otherwise here

@@ -1444,6 +1444,10 @@ agrav_status build_tree(agrav_ctx *c)
       c->d_key.swap_storage(c->d_key2);
       c->d_idx.swap_storage(c->d_idx2);
     }
+  // a full step's walk order is sorted beside the rest of the build (queued here, when the
+  // main stream holds enough work that the launches cost the device no time)
+  if(c->defer_build_check && !part && nsets == 0 && (st = walk_preorder(c)) != AGRAV_OK)
+    return st;
   // sorted first key words and perm are now in d_key2/d_idx2; runs of equal first
   // words are put in the order of their second words
   int *d_nties = c->d_counters.p + 60;  // cleared at the start of the build
@@ -1652,10 +1656,21 @@ agrav_status build_tree(agrav_ctx *c)
   // round trip to the device
   AG_HIP(hipEventRecord(c->ev[1], s));
   // error flags and domain (out-of-box: forcetree.c:432-438 aborts; here it is an error
-  // code), read back into their own part of the staging block
+  // code), read back into their own part of the staging block; a deferred check's copies
+  // go to the side stream, off the walk's path (the next build and the check wait for them)
   static_assert(kBuildStage + 64 * sizeof(int) + sizeof(Domain) <= agrav_ctx::kStageBytes, "read-backs fit the staging block");
-  AG_HIP(hipMemcpyAsync(c->stage<int>(kBuildStage), c->d_counters.p, 64 * sizeof(int), hipMemcpyDeviceToHost, s));
-  AG_HIP(hipMemcpyAsync(c->stage<Domain>(kBuildStage + 256), c->d_dom.p, sizeof(Domain), hipMemcpyDeviceToHost, s));
+  const bool side_copy = c->defer_build_check && c->mom_stream != nullptr;
+  hipStream_t rs       = side_copy ? c->mom_stream : s;
+  if(side_copy)
+    AG_HIP(hipStreamWaitEvent(rs, c->ev[1], 0));
+  AG_HIP(hipMemcpyAsync(c->stage<int>(kBuildStage), c->d_counters.p, 64 * sizeof(int), hipMemcpyDeviceToHost, rs));
+  AG_HIP(hipMemcpyAsync(c->stage<Domain>(kBuildStage + 256), c->d_dom.p, sizeof(Domain), hipMemcpyDeviceToHost, rs));
+  if(side_copy)
+    {
+      AG_HIP(hipEventRecord(c->ev_mjoin, rs));
+      c->mom_forked = true;
+    }
+  c->bcopy_side = side_copy;
   c->m                   = M;
   c->have_tree           = true;
   c->build_check_pending = true;
@@ -1672,6 +1687,11 @@ agrav_status build_check(agrav_ctx *c)
   if(!c->build_check_pending)
     return AGRAV_OK;
   c->build_check_pending = false;
+  if(c->bcopy_side)
+    {
+      AG_HIP(hipEventSynchronize(c->ev_mjoin));  // the read-back copies on the side stream
+      c->bcopy_side = false;
+    }
   if(c->build_time_pending)
     {
       float ms = 0;

@@ -273,7 +273,13 @@ struct agrav_ctx
   // the octree kernels (they need only the sorted particles); joined before the moments
   hipStream_t mom_stream = nullptr;  // default priority
   hipEvent_t ev_mfork = nullptr, ev_mjoin = nullptr;
-  bool mom_forked = false;  // ev_mjoin marks the end of the last forked prefix sums
+  bool mom_forked = false;  // ev_mjoin marks the end of the last work forked to mom_stream
+  bool bcopy_side = false;  // the build's read-back copies went to mom_stream (ev_mjoin)
+  // a full step's walk order, sorted on mom_stream beside the build (event ev_order)
+  hipEvent_t ev_order = nullptr;
+  const int *pre_gorder = nullptr;
+  int pre_ngroups = -1, pre_gs = 0;  // its grouping (-1: none pending)
+  bool isum_clear = false;           // the walk counters were cleared before that sort
   bool pm_fork   = false;  // the next build forks the PM (set by compute_grav_accelerations)
   bool pm_forked = false;  // the build forked it; the main stream has not joined it yet
   // a full step reads the build's error flags back with the walk's counters (one
@@ -361,6 +367,7 @@ BatchView batch_view(const agrav_ctx *c);
 agrav_status batch_groups(agrav_ctx *c);
 // fin: the final outputs the walk's epilogue writes (perm null: none); *fused is set
 // when every target's outputs were written so (no group re-walked)
+agrav_status walk_preorder(agrav_ctx *c);
 agrav_status walk_batch_groups(agrav_ctx *c, int use_bh, const int4 *groups, int ngroups, float *kernel_ms,
                                const FinalOut &fin = FinalOut{}, bool *fused = nullptr);
 TreeView tree_view(const agrav_ctx *c);

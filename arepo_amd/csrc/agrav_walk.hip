@@ -1778,11 +1778,17 @@ agrav_status active_targets(agrav_ctx *c)
 // Sort key of each walk group: its cost (tile entries of the last walk of the same
 // grouping) as an 8-bit code falling with log2 of the cost (1/8 octave per step), so
 // that one stable radix pass orders the groups costliest first and keeps groups of
-// similar cost in Hilbert order.  Block 0 also clears the pass's digit counts.
-__global__ void k_group_keys(int ngroups, const int *__restrict__ gcost, uint64_t *keys, int *ids, int *ghist)
+// similar cost in Hilbert order.  Block 0 also clears the pass's digit counts and the
+// walk's counters (where the sort reports a look-back that did not complete).
+__global__ void k_group_keys(int ngroups, const int *__restrict__ gcost, uint64_t *keys, int *ids, int *ghist,
+                             unsigned long long *isum)
 {
   if(blockIdx.x == 0)
-    ghist[threadIdx.x] = 0;  // 256 threads, one digit each
+    {
+      ghist[threadIdx.x] = 0;  // 256 threads, one digit each
+      if(threadIdx.x < 4)
+        isum[threadIdx.x] = 0;
+    }
   const int g = blockIdx.x * 256 + threadIdx.x;
   if(g >= ngroups)
     return;
@@ -1801,7 +1807,7 @@ static int *group_sort_err(agrav_ctx *c) { return reinterpret_cast<int *>(c->d_i
 static constexpr int kOrderMinWaves = 8192;
 
 // launch order of the group walk from the costs in d_gcost, or null
-static agrav_status group_order(agrav_ctx *c, int ngroups, const int **order)
+static agrav_status group_order(agrav_ctx *c, int ngroups, const int **order, hipStream_t s)
 {
   *order = nullptr;
   agrav_status st;
@@ -1809,9 +1815,8 @@ static agrav_status group_order(agrav_ctx *c, int ngroups, const int **order)
   if((st = c->d_gkey.ensure(ngroups)) || (st = c->d_gkey2.ensure(ngroups)) || (st = c->d_gord.ensure(ngroups)) ||
      (st = c->d_gord2.ensure(ngroups)) || (st = c->d_gtemp.ensure(tb + 256 * sizeof(int))))
     return st;
-  hipStream_t s = c->stream;
-  int *ghist    = reinterpret_cast<int *>(c->d_gtemp.p + tb);
-  k_group_keys<<<(ngroups + 255) / 256, 256, 0, s>>>(ngroups, c->d_gcost.p, c->d_gkey.p, c->d_gord.p, ghist);
+  int *ghist = reinterpret_cast<int *>(c->d_gtemp.p + tb);
+  k_group_keys<<<(ngroups + 255) / 256, 256, 0, s>>>(ngroups, c->d_gcost.p, c->d_gkey.p, c->d_gord.p, ghist, c->d_isum.p);
   AG_HIP(hipGetLastError());
   bool in_input = false;
   if((st = radix_sort_u64(c->d_gkey.p, c->d_gord.p, c->d_gkey2.p, c->d_gord2.p, ngroups, 8, c->d_gtemp.p,
@@ -1823,6 +1828,70 @@ static agrav_status group_order(agrav_ctx *c, int ngroups, const int **order)
 
 static agrav_status walk_pass(agrav_ctx *c, int use_bh, const WalkConst &wc, const TreeView &tv);
 
+// Group size and wavefronts per group of a group walk over nt of a tree's ntree
+// particles, by workload when not set (MI355X timings of FP32 walks): periodic sets
+// and large isolated sets fill the device with groups of 64; small isolated sets are
+// bound by the latency of each group's walk and gain from groups of 16 (200k Hernquist
+// particles: 1.3 ms against 8.3 ms) and from sharing a group over 4 wavefronts
+static void group_shape(const agrav_ctx *c, int64_t nt, int64_t ntree, bool isolated, int *gs_out, int *waves_out)
+{
+  int gs = c->par.group_size;
+  if(gs <= 0)
+    {
+      gs = (isolated && nt < (1 << 19)) ? 16 : 64;
+      // a sparse active list spreads a group over a larger volume (more nodes opened)
+      // and gives fewer groups: about 256 N_active / N targets per group, 8 at least
+      // (2M-particle clustered TreePM box, 3 % active: walk 1.7 ms against 5.0 ms)
+      if(nt < ntree / 2)
+        {
+          int sg = 64;
+          while(sg > 8 && (int64_t)sg * ntree > 256 * nt)
+            sg >>= 1;
+          gs = std::min(gs, sg);
+        }
+    }
+  // wavefronts per group: small sets have too few groups to fill the device
+  int waves = c->par.group_waves;
+  if(waves <= 0)
+    waves = isolated ? (nt < (1 << 17) ? 4 : 1) : (nt < (1 << 18) ? 4 : (nt < (1 << 20) ? 2 : 1));
+  *gs_out    = gs;
+  *waves_out = waves;
+}
+
+// A full step's launch order, sorted on the side stream while the tree is built (the
+// build calls this once its sort is queued): it needs only the costs of the last walk of
+// the same grouping (every particle, groups of the size this step's walk will take).  The
+// key kernel clears the walk's counters, where the sort reports a look-back that did not
+// complete; the walk waits for the sort's event and leaves the counters as they are.
+agrav_status walk_preorder(agrav_ctx *c)
+{
+  c->pre_ngroups = -1;
+  const int64_t n = c->n;
+  if(c->par.walk == AGRAV_WALK_PLAIN || !c->mom_stream || n <= 0 || c->batch_nsets > 0)
+    return AGRAV_OK;
+  const bool pm = c->par.pmgrid > 0, ew = c->par.periodic && !pm;
+  int gs = 0, waves = 0;
+  group_shape(c, n, n, !pm && !ew, &gs, &waves);
+  const int ngroups     = (int)((n + gs - 1) / gs);
+  const int order_waves = c->par.precision == AGRAV_FP32 ? waves : 1;
+  if((int64_t)ngroups * order_waves < kOrderMinWaves || c->gcost_groups != ngroups || c->gcost_gs != gs || c->gcost_n != n)
+    return AGRAV_OK;
+  agrav_status st;
+  if((st = c->d_isum.ensure(4)) != AGRAV_OK)
+    return st;
+  c->isum_clear = true;
+  AG_HIP(hipEventRecord(c->ev_mfork, c->stream));
+  AG_HIP(hipStreamWaitEvent(c->mom_stream, c->ev_mfork, 0));
+  const int *order = nullptr;
+  if((st = group_order(c, ngroups, &order, c->mom_stream)) != AGRAV_OK)
+    return st;
+  AG_HIP(hipEventRecord(c->ev_order, c->mom_stream));
+  c->pre_gorder  = order;
+  c->pre_ngroups = ngroups;
+  c->pre_gs      = gs;
+  return AGRAV_OK;
+}
+
 agrav_status walk_tree(agrav_ctx *c, int use_bh)
 {
   if(!c->have_tree)
@@ -1833,7 +1902,9 @@ agrav_status walk_tree(agrav_ctx *c, int use_bh)
   if((st = c->d_at.ensure(3 * (n ? n : 1))) || (st = c->d_pt.ensure(n ? n : 1)) || (st = c->d_nint.ensure(n ? n : 1)) ||
      (st = c->d_isum.ensure(4)))
     return st;
-  AG_HIP(hipMemsetAsync(c->d_isum.p, 0, 4 * sizeof(unsigned long long), s));
+  if(!c->isum_clear)
+    AG_HIP(hipMemsetAsync(c->d_isum.p, 0, 4 * sizeof(unsigned long long), s));
+  c->isum_clear = false;  // cleared by the pre-order of this step's first walk
   if(n == 0)
     {
       c->have_tree_force = true;
@@ -1920,6 +1991,16 @@ static agrav_status walk_pass(agrav_ctx *c, int use_bh, const WalkConst &wc, con
   bool have_cnt       = false;  // the counters are read already (group walk, no re-walk)
   c->final_fused      = false;
   c->fin.perm         = nullptr;
+  // a launch order sorted beside the build (walk_preorder) is taken by the group walk of
+  // its grouping below; any other walk clears the counters its sort may have flagged
+  const bool pre = c->pre_ngroups >= 0;
+  if(pre)
+    AG_HIP(hipStreamWaitEvent(s, c->ev_order, 0));
+  if(pre && (c->batch_nsets > 0 || c->par.walk == AGRAV_WALK_PLAIN))
+    {
+      AG_HIP(hipMemsetAsync(c->d_isum.p, 0, 4 * sizeof(unsigned long long), s));
+      c->pre_ngroups = -1;
+    }
   if(c->batch_nsets > 0 && c->par.walk != AGRAV_WALK_PLAIN)
     {
       // batch of sets: one persistent launch over the group queue; within a full solve
@@ -1944,29 +2025,11 @@ static agrav_status walk_pass(agrav_ctx *c, int use_bh, const WalkConst &wc, con
       // isolated sets fill the device with groups of 64; small isolated sets are bound
       // by the latency of each group's walk and gain from groups of 16 (200k Hernquist
       // particles: 1.3 ms against 8.3 ms) and from sharing a group over 4 wavefronts
-      const bool isolated = !pm && !ew;
-      int gs = c->par.group_size;
-      if(gs <= 0)
-        {
-          gs = (isolated && nt < (1 << 19)) ? 16 : 64;
-          // a sparse active list spreads a group over a larger volume (more nodes opened)
-          // and gives fewer groups: about 256 N_active / N targets per group, 8 at least
-          // (2M-particle clustered TreePM box, 3 % active: walk 1.7 ms against 5.0 ms)
-          if(nt < c->n_tree / 2)
-            {
-              int sg = 64;
-              while(sg > 8 && (int64_t)sg * c->n_tree > 256 * nt)
-                sg >>= 1;
-              gs = std::min(gs, sg);
-            }
-        }
+      int gs = 0, waves = 0;
+      group_shape(c, nt, c->n_tree, !pm && !ew, &gs, &waves);
       int ngroups = (int)((nt + gs - 1) / gs);
       if((st = c->d_failed.ensure(ngroups)) != AGRAV_OK)
         return st;
-      // wavefronts per group: small sets have too few groups to fill the device
-      int waves = c->par.group_waves;
-      if(waves <= 0)
-        waves = isolated ? (nt < (1 << 17) ? 4 : 1) : (nt < (1 << 18) ? 4 : (nt < (1 << 20) ? 2 : 1));
       c->tim_group_waves = c->par.precision == AGRAV_FP32 ? waves : 1;
       // within a full step the walk's epilogue also writes the final outputs (the PM
       // has run, in sorted order); the finalize kernel then has nothing left to do
@@ -1978,9 +2041,13 @@ static agrav_status walk_pass(agrav_ctx *c, int use_bh, const WalkConst &wc, con
       const bool whole  = c->batch_nsets == 0 && target_map(c) == nullptr && nt == c->n_tree;
       const int *gorder = nullptr;
       const int order_waves = c->par.precision == AGRAV_FP32 ? waves : 1;
-      if(whole && (int64_t)ngroups * order_waves >= kOrderMinWaves && c->gcost_groups == ngroups && c->gcost_gs == gs &&
-         c->gcost_n == c->n &&
-         (st = group_order(c, ngroups, &gorder)) != AGRAV_OK)
+      if(pre && whole && c->pre_ngroups == ngroups && c->pre_gs == gs)
+        gorder = c->pre_gorder;  // sorted beside the build
+      else if(pre)
+        AG_HIP(hipMemsetAsync(c->d_isum.p, 0, 4 * sizeof(unsigned long long), s));
+      c->pre_ngroups = -1;
+      if(!gorder && whole && (int64_t)ngroups * order_waves >= kOrderMinWaves && c->gcost_groups == ngroups &&
+         c->gcost_gs == gs && c->gcost_n == c->n && (st = group_order(c, ngroups, &gorder, s)) != AGRAV_OK)
         return st;
       if(whole && (st = c->d_gcost.ensure(ngroups)) != AGRAV_OK)
         return st;
