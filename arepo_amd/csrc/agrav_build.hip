@@ -337,20 +337,21 @@ __device__ inline void tie_unpack(uint64_t sec, uint64_t k1_run, int tshift, uin
 }
 
 // runs of first key words equal from bit tshift up in the sorted order (particles
-// sharing a cell at that depth): the start of each run.  Each wavefront takes
-// kTieSpan consecutive keys and reserves the slots of all its run starts with one
-// atomic (one per 64 keys serialised on the counter: 5.7 ms at 1e8 keys).
-static constexpr int kTieSpan = 1024;
-__global__ void k_tie_runs(const uint64_t *__restrict__ keys, int n, int tshift, int *runs, int *nruns)
+// sharing a cell at that depth): the start of each run.  Each wavefront takes `span`
+// consecutive keys (a multiple of 64) and reserves the slots of all its run starts with
+// one atomic: 64 keys fill the device at a few million keys, while at 1e8 keys one
+// atomic per 64 serialised on the counter for 5.7 ms (1024: 0.2 ms).
+static inline int tie_span(int64_t n) { return n < (1 << 23) ? 64 : 1024; }
+__global__ void k_tie_runs(const uint64_t *__restrict__ keys, int n, int tshift, int span, int *runs, int *nruns)
 {
   const int lane   = threadIdx.x & 63;
-  const int64_t w0 = ((int64_t)blockIdx.x * (B / 64) + (threadIdx.x >> 6)) * kTieSpan;
+  const int64_t w0 = ((int64_t)blockIdx.x * (B / 64) + (threadIdx.x >> 6)) * span;
   const uint64_t lt = (1ull << lane) - 1;
   auto head = [&](int64_t k) {
     return k + 1 < n && (keys[k] >> tshift) == (keys[k + 1] >> tshift) && (k == 0 || (keys[k - 1] >> tshift) != (keys[k] >> tshift));
   };
   int cnt = 0;
-  for(int j = 0; j < kTieSpan; j += 64)
+  for(int j = 0; j < span; j += 64)
     cnt += __popcll(__ballot(head(w0 + j + lane)));
   if(cnt == 0)
     return;
@@ -358,7 +359,7 @@ __global__ void k_tie_runs(const uint64_t *__restrict__ keys, int n, int tshift,
   if(lane == 0)
     base = atomicAdd(nruns, cnt);
   base = __shfl(base, 0, 64);
-  for(int j = 0; j < kTieSpan; j += 64)
+  for(int j = 0; j < span; j += 64)
     {
       const bool h     = head(w0 + j + lane);
       const uint64_t m = __ballot(h);
@@ -1467,8 +1468,9 @@ agrav_status build_tree(agrav_ctx *c)
   if(n > 1)
     {
       int *d_nlong = c->d_counters.p + 61, *d_long = c->d_ties.p + (n / 2 + 1);
-      k_tie_runs<<<(unsigned)((n + (int64_t)(B / 64) * kTieSpan - 1) / ((int64_t)(B / 64) * kTieSpan)), B, 0, s>>>(
-          c->d_key2.p, (int)n, tshift, c->d_ties.p, d_nties);
+      const int64_t per_block = (int64_t)(B / 64) * tie_span(n);
+      k_tie_runs<<<(unsigned)((n + per_block - 1) / per_block), B, 0, s>>>(c->d_key2.p, (int)n, tshift, tie_span(n), c->d_ties.p,
+                                                                          d_nties);
       k_tie_sort_short<<<512, B, 0, s>>>(c->d_key2.p, c->d_key_lo.p, (int)n, (int)nall, tshift, c->d_ties.p, d_nties, c->d_idx2.p,
                                          c->d_key_lo_s.p, d_long, d_nlong);
       k_tie_sort<<<256, 256, 0, s>>>(c->d_key2.p, c->d_key_lo.p, (int)n, (int)nall, tshift, d_long, d_nlong, c->d_idx2.p,
