@@ -74,6 +74,35 @@ def test_step_reports_build_error_after_walk(agrav, pmgrid):
     assert np.array_equal(r["nint"], r2["nint"])
 
 
+def test_build_error_with_ordered_walk_then_valid_step(agrav):
+    """The same with enough groups for the costliest-first order (sorted beside the
+    build from the last walk's costs): repeated failing steps with the order queued,
+    then a valid set equals a fresh context's result."""
+    pos, mass = _box(600000, 21)
+    bad = pos.copy()
+    bad[123] = (2.6, 0.3, 0.3)
+    p = agrav.make_params(periodic=True, box_size=1.0, pmgrid=64, soft=(0.002,), precision=agrav.FP32,
+                          walk=agrav.WALK_GROUP, group_size=64, group_waves=1)
+    with agrav.Gravity(p) as g:
+        g.set_particles(bad, mass)
+        for first in (True, False, False):
+            with pytest.raises(agrav.AgravError) as ei:
+                g.compute_grav_accelerations(first_step=first)
+            assert ei.value.code == agrav.ERR_OUT_OF_BOX
+        g.set_particles(pos, mass)
+        g.compute_grav_accelerations(first_step=True)
+        g.compute_grav_accelerations(first_step=False)
+        assert g.timings()["walk_ordered"] == 1
+        r = g.get_accelerations()
+    with agrav.Gravity(p) as g2:
+        g2.set_particles(pos, mass)
+        g2.compute_grav_accelerations(first_step=True)
+        g2.compute_grav_accelerations(first_step=False)
+        r2 = g2.get_accelerations()
+    for f in ("acc", "pot", "nint", "old_acc"):
+        assert np.array_equal(r[f], r2[f]), f
+
+
 def test_step_timings_cover_build_pm_and_walk(agrav):
     """The deferred build check still reports the build's duration, and the PM's."""
     pos, mass = _box(20000, 13)
