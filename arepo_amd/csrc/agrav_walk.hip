@@ -490,7 +490,9 @@ __device__ inline void tile_compute_f64(const Vec4<double> *__restrict__ s_list,
 // validity is only the table test.
 static constexpr float kTinyR2 = 1e-30f;
 
-template <bool PM, bool EWALD, bool MSOFT, bool POT>
+// SPLINE = false: the caller has shown that no entry lies within the softening length
+// of any target (the entries near the group go to a list of their own).
+template <bool PM, bool EWALD, bool MSOFT, bool POT, bool SPLINE = true>
 __device__ inline void tile_compute_f32(const Vec4<float> *__restrict__ s_list, const float *__restrict__ s_h, int nl_in, float tx,
                                         float ty, float tz, float hi, const float *__restrict__ s_tab, const WalkConst &wc,
                                         float asmthfac, float asmthinv2, const float *__restrict__ ewald_f, float &ax, float &ay,
@@ -553,7 +555,7 @@ __device__ inline void tile_compute_f32(const Vec4<float> *__restrict__ s_list, 
           fac = rinv2 * rinv;
           wp  = -rinv;
         }
-      if(r < hmax)
+      if(SPLINE && r < hmax)
         {
           // spline softening (allvars.h:557-573); u < 1 here
           const float h3 = hinv * hinv * hinv;
@@ -725,16 +727,22 @@ template <typename T, bool PM, bool EWALD, bool MSOFT>
 __device__ inline void tile_compute(const Vec4<T> *__restrict__ s_list, const T *__restrict__ s_h, int nl, T tx, T ty, T tz, T hi,
                                     const float *__restrict__ s_tab, const WalkConst &wc, T asmthfac, T asmthinv2,
                                     const float *__restrict__ ewald_f, const double *__restrict__ ewald_d, T &ax, T &ay, T &az, T &pp,
-                                    T &cax, T &cay, T &caz, T &cpp, int &nin)
+                                    T &cax, T &cay, T &caz, T &cpp, int &nin, bool spline = true)
 {
   if constexpr(sizeof(T) == 4)
     {
-      if(wc.eval_pot)
+      if(wc.eval_pot && spline)
         tile_compute_f32<PM, EWALD, MSOFT, true>(s_list, s_h, nl, tx, ty, tz, hi, s_tab, wc, asmthfac, asmthinv2, ewald_f, ax, ay,
                                                  az, pp, cax, cay, caz, cpp, nin);
-      else
+      else if(wc.eval_pot)
+        tile_compute_f32<PM, EWALD, MSOFT, true, false>(s_list, s_h, nl, tx, ty, tz, hi, s_tab, wc, asmthfac, asmthinv2, ewald_f, ax,
+                                                        ay, az, pp, cax, cay, caz, cpp, nin);
+      else if(spline)
         tile_compute_f32<PM, EWALD, MSOFT, false>(s_list, s_h, nl, tx, ty, tz, hi, s_tab, wc, asmthfac, asmthinv2, ewald_f, ax,
                                                   ay, az, pp, cax, cay, caz, cpp, nin);
+      else
+        tile_compute_f32<PM, EWALD, MSOFT, false, false>(s_list, s_h, nl, tx, ty, tz, hi, s_tab, wc, asmthfac, asmthinv2, ewald_f,
+                                                         ax, ay, az, pp, cax, cay, caz, cpp, nin);
     }
   else
     tile_compute_f64<PM, EWALD, MSOFT>(s_list, s_h, nl, tx, ty, tz, hi, s_tab, wc, asmthfac, asmthinv2, ewald_d, ax, ay, az, pp, cax,
@@ -904,7 +912,13 @@ __device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, co
   // the main list and (FP32 TreePM of one softening type) the list of entries within
   // the softening length of the group's box, whose loop alone has the spline branch
   constexpr int kMainCap = kFast ? AG_FAST_MAIN : kListCap;
-  Vec4<T> *s_soft        = s_list + kMainCap;
+  // isolated FP32 walks of one softening type also keep the entries near the group in a
+  // list of their own (in the LDS of the short-range table, which they do not read), so
+  // that the main list's loop has no softening test
+  constexpr bool kNearList = sizeof(T) == 4 && !PM && !EWALD && !MSOFT;
+  constexpr bool kSplit    = kFast || kNearList;
+  static_assert(!kNearList || sizeof(lds.tab) >= kSoftCap * sizeof(Vec4<float>), "near list fits the table's LDS");
+  Vec4<T> *s_soft        = kNearList ? reinterpret_cast<Vec4<T> *>(lds.tab) : s_list + kMainCap;
   int ns                 = 0;
   auto fast_tile         = [&](const Vec4<T> *lst, int cnt, bool soft) {
     if constexpr(kFast)
@@ -922,12 +936,19 @@ __device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, co
       fast_tile(s_list, nl, false);
     else
       tile_compute<T, PM, EWALD, MSOFT>(s_list, s_h, nl, tx, ty, tz, hi, s_tab, wc, asmthfac, asmthinv2, ewald_f, ewald_d, ax, ay,
-                                        az, pp, cax, cay, caz, cpp, nin);
+                                        az, pp, cax, cay, caz, cpp, nin, !kNearList);
+  };
+  auto soft_tile = [&]() {
+    if constexpr(kFast)
+      fast_tile(s_soft, ns, true);
+    else if constexpr(kNearList)
+      tile_compute<T, PM, EWALD, MSOFT>(s_soft, s_h, ns, tx, ty, tz, hi, s_tab, wc, asmthfac, asmthinv2, ewald_f, ewald_d, ax, ay,
+                                        az, pp, cax, cay, caz, cpp, nin, true);
   };
   // stage one entry per lane (acc: the lane has one; nr: within the softening length of
   // the group's box), flushing a list first when the entries would not fit
   auto stage = [&](bool acc, bool nr, const Vec4<T> &it, T hv) {
-    const bool to_soft = kFast && nr;
+    const bool to_soft = kSplit && nr;
     const uint64_t bm  = __ballot(acc && !to_soft);
     const int cnt      = __popcll(bm);
     if(nl + cnt > kMainCap)
@@ -938,7 +959,7 @@ __device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, co
         nl = 0;
         __builtin_amdgcn_wave_barrier();
       }
-    if constexpr(kFast)
+    if constexpr(kSplit)
       {
         const uint64_t bs = __ballot(acc && to_soft);
         const int sc      = __popcll(bs);
@@ -946,7 +967,7 @@ __device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, co
           {
             __builtin_amdgcn_wave_barrier();
             n_tiled += ns;
-            fast_tile(s_soft, ns, true);
+            soft_tile();
             ns = 0;
             __builtin_amdgcn_wave_barrier();
           }
@@ -1160,7 +1181,7 @@ __device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, co
               item = Vec4<T>{x * cs, y * cs, z * cs, m};
               if(MSOFT)
                 hj = (T)hsoft[tv.ps[e]];
-              if(PM)
+              if(PM || kNearList)
                 {
                   // a particle farther than Rcut from the whole group fails the table test for every target
                   T ddx = (T)fabs(x) - hx, ddy = (T)fabs(y) - hy, ddz = (T)fabs(z) - hz;
@@ -1168,7 +1189,7 @@ __device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, co
                   ddy   = ddy > 0 ? ddy : 0;
                   ddz   = ddz > 0 ? ddz : 0;
                   const T dd2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                  if(dd2 > rcut2 * (T)1.0001)
+                  if(PM && dd2 > rcut2 * (T)1.0001)
                     act = 0;
                   near = dd2 < hs2;
                 }
@@ -1238,15 +1259,16 @@ __device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, co
                     if(MSOFT)
                       hp = (T)hsoft[tv.ps[fl ? tv.child[pi2] : pi2]];
                     keep = m != (T)0;
-                    if(PM && keep)
+                    if((PM || kNearList) && keep)
                       {
                         T ddx = (T)fabs(x) - hx, ddy = (T)fabs(y) - hy, ddz = (T)fabs(z) - hz;
                         ddx   = ddx > 0 ? ddx : 0;
                         ddy   = ddy > 0 ? ddy : 0;
                         ddz   = ddz > 0 ? ddz : 0;
                         const T dd2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                        keep        = dd2 <= rcut2 * (T)1.0001;
-                        pnear       = dd2 < hs2;
+                        if(PM)
+                          keep = dd2 <= rcut2 * (T)1.0001;
+                        pnear = dd2 < hs2;
                       }
                   }
                 stage(keep, pnear, Vec4<T>{x * cs, y * cs, z * cs, m}, hp);
@@ -1301,10 +1323,10 @@ __device__ inline bool walk_group(WalkLds<T, MSOFT> &lds, const TreeView &tv, co
   __builtin_amdgcn_wave_barrier();
   n_tiled += nl;
   flush_tile();
-  if constexpr(kFast)
+  if constexpr(kSplit)
     {
       n_tiled += ns;
-      fast_tile(s_soft, ns, true);
+      soft_tile();
     }
 
   if constexpr(kFast)
