@@ -1488,8 +1488,13 @@ __global__ __launch_bounds__(64 * W, (sizeof(T) == 4 && PM && !EWALD && !MSOFT) 
 // (Per-XCD queues over contiguous parts of the list measured 3 % slower: the
 // sets are small enough to stay in L2 either way, and the split adds a tail.)
 // Within a full solve the epilogue also writes the final outputs (fin.perm set).
+// The FP32 kernel is held to 96 VGPRs (5 wavefronts per SIMD; its LDS allows 19 per
+// CU): 4096-set batch walk 63.7 against 66.1 ms at the compiler's 98.
+#ifndef AG_BATCH_MINW
+#define AG_BATCH_MINW 5
+#endif
 template <typename T>
-__global__ __launch_bounds__(64) void k_walk_batch(TreeView tv, WalkConst wc, BatchView bv, const float *__restrict__ oldacc_s,
+__global__ __launch_bounds__(64, sizeof(T) == 4 ? AG_BATCH_MINW : 1) void k_walk_batch(TreeView tv, WalkConst wc, BatchView bv, const float *__restrict__ oldacc_s,
                                                    const float *__restrict__ tabs, int ngroups, int *queue, double *at, double *pt,
                                                    int *nint, int *failed, int *nfailed, unsigned long long *isum, FinalOut fin)
 {
