@@ -1707,6 +1707,7 @@ agrav_status build_check(agrav_ctx *c)
     {
       AG_HIP(hipEventSynchronize(c->ev_mjoin));  // the read-back copies on the side stream
       c->bcopy_side = false;
+      c->mom_forked = false;  // the side stream's work so far is done: the next build need not wait
     }
   if(c->build_time_pending)
     {
